@@ -1,0 +1,460 @@
+"""ctypes binding of the C ABI in include/beep_gpu.h (libbeep_gpu.so).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for any entry point."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import spec as specmod
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbeep_gpu.so")
+
+BEEP_OK = 0
+ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ENAN", -5: "EBOUNDS", -6: "ELARGE"}
+KEEP_TRANSNUM, KEEP_TRANSMEAN, KEEP_POP, KEEP_ALL = 1, 2, 4, 7
+SIM_PLAIN = 2
+
+# Exported symbols declared by include/beep_gpu.h.
+SYMBOLS = [
+    "beep_spec_tables", "beep_create", "beep_destroy", "beep_last_error", "beep_get_dims", "beep_device_bytes", "beep_set_param",
+    "beep_get_tables", "beep_simulate", "beep_obs_loglik", "beep_obs_values", "beep_pois_loglik", "beep_mbp",
+    "beep_accept", "beep_get_transnum", "beep_get_transmean", "beep_get_pop", "beep_get_state", "beep_put_state",
+    "beep_put_transnum", "beep_copy_instances", "beep_synchronize", "beep_debug_philox", "beep_debug_poisson",
+    "beep_debug_binomial", "beep_timing",
+]
+
+
+class BeepError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Prior(C.Structure):
+    _fields_ = [("type", C.c_int32), ("a", C.c_double), ("b", C.c_double)]
+
+
+class Comp(C.Structure):
+    _fields_ = [("inf_param", C.c_int32), ("mean_param", C.POINTER(C.c_int32)), ("shape", C.c_int32)]
+
+
+class Trans(C.Structure):
+    _fields_ = [("from_", C.c_int32), ("to", C.c_int32), ("infection", C.c_int32),
+                ("prob_param", C.POINTER(C.c_int32))]
+
+
+class Spline(C.Structure):
+    _fields_ = [("nknot", C.c_int32), ("t", C.POINTER(C.c_double)), ("param", C.POINTER(C.c_int32)),
+                ("multfac", C.POINTER(C.c_double)), ("factor_param", C.c_int32)]
+
+
+class Dirichlet(C.Structure):
+    _fields_ = [("n", C.c_int32), ("param", C.POINTER(C.c_int32)), ("frac", C.POINTER(C.c_double))]
+
+
+class Covar(C.Structure):
+    _fields_ = [("param", C.c_int32), ("value", C.POINTER(C.c_double))]
+
+
+class Obs(C.Structure):
+    _fields_ = [("y", C.c_double), ("sd", C.c_double), ("shape", C.c_double), ("factor", C.c_double),
+                ("thresh", C.c_double), ("percent", C.c_double), ("type", C.c_int32), ("datatable", C.c_int32),
+                ("graph", C.c_int32), ("sett_i", C.c_int32), ("sett_f", C.c_int32),
+                ("narea", C.c_int32), ("area", C.POINTER(C.c_int32)), ("ndp", C.c_int32), ("dp", C.POINTER(C.c_int32)),
+                ("ntr", C.c_int32), ("tr", C.POINTER(C.c_int32)), ("nco", C.c_int32), ("co", C.POINTER(C.c_int32))]
+
+
+class Spec(C.Structure):
+    _fields_ = [("A", C.c_int32), ("G", C.c_int32), ("C", C.c_int32), ("T", C.c_int32),
+                ("period", C.c_int32), ("steps_per_day", C.c_int32), ("nparam", C.c_int32),
+                ("prior", C.POINTER(Prior)), ("comp", C.POINTER(Comp)), ("trans", C.POINTER(Trans)),
+                ("nspline", C.c_int32), ("spline", C.POINTER(Spline)), ("R_spline", C.c_int32),
+                ("Rfactor_param", C.c_int32), ("efoi_spline", C.c_int32), ("efoi_agedist", C.POINTER(C.c_double)),
+                ("geo_spline", C.c_int32), ("N_raw", C.POINTER(C.c_double)), ("M_diag", C.POINTER(C.c_double)),
+                ("M_rowptr", C.POINTER(C.c_int32)), ("M_to", C.POINTER(C.c_int32)), ("M_val", C.POINTER(C.c_double)),
+                ("pop_init", C.POINTER(C.c_int32)), ("ncovar", C.c_int32), ("covar", C.POINTER(Covar)),
+                ("sus_param", C.POINTER(C.c_int32)), ("ndirichlet", C.c_int32), ("dirichlet", C.POINTER(Dirichlet)),
+                ("nobs", C.c_int32), ("obs", C.POINTER(Obs)), ("nsection", C.c_int32),
+                ("section_ti", C.POINTER(C.c_int32)), ("section_tf", C.POINTER(C.c_int32))]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() (there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        i32p, u32p, f64p, f32p, u8p = (C.POINTER(C.c_int32), C.POINTER(C.c_uint32), C.POINTER(C.c_double),
+                                      C.POINTER(C.c_float), C.POINTER(C.c_uint8))
+        sig = {
+            "beep_spec_tables": (C.c_int, [C.POINTER(Spec), f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
+                                           i32p, C.c_char_p, C.c_int]),
+            "beep_create": (C.c_int, [C.POINTER(Spec), C.c_int, C.c_size_t, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
+            "beep_destroy": (C.c_int, [P]),
+            "beep_last_error": (C.c_char_p, [P]),
+            "beep_get_dims": (C.c_int, [P, i32p]),
+            "beep_device_bytes": (C.c_size_t, [P]),
+            "beep_set_param": (C.c_int, [P, C.c_int, i32p, f64p, f64p, i32p]),
+            "beep_get_tables": (C.c_int, [P, C.c_int, f64p, f64p, f64p, f64p, f64p, f64p, f64p]),
+            "beep_simulate": (C.c_int, [P, C.c_int, i32p, i32p, u32p, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_int]),
+            "beep_obs_loglik": (C.c_int, [P, C.c_int, i32p, C.c_int, f64p, f64p]),
+            "beep_obs_values": (C.c_int, [P, C.c_int, f64p]),
+            "beep_pois_loglik": (C.c_int, [P, C.c_int, i32p, f64p]),
+            "beep_mbp": (C.c_int, [P, C.c_int, i32p, i32p, i32p, u32p, u8p, C.c_uint64, C.c_uint32, C.c_int]),
+            "beep_accept": (C.c_int, [P, C.c_int, i32p, i32p, C.c_int, C.c_double, C.c_double, f64p, u32p, C.c_uint64,
+                                      C.c_uint32, f64p, u8p]),
+            "beep_get_transnum": (C.c_int, [P, C.c_int, i32p]),
+            "beep_get_transmean": (C.c_int, [P, C.c_int, f32p]),
+            "beep_get_pop": (C.c_int, [P, C.c_int, i32p]),
+            "beep_get_state": (C.c_int, [P, C.c_int, i32p]),
+            "beep_put_state": (C.c_int, [P, C.c_int, i32p]),
+            "beep_put_transnum": (C.c_int, [P, C.c_int, C.c_int, i32p]),
+            "beep_copy_instances": (C.c_int, [P, C.c_int, i32p, i32p]),
+            "beep_synchronize": (C.c_int, [P]),
+            "beep_timing": (C.c_int, [P, f64p]),
+            "beep_debug_philox": (C.c_int, [C.c_int, C.c_int, u32p, C.c_uint32, C.c_uint32, u32p]),
+            "beep_debug_poisson": (C.c_int, [C.c_int, C.c_int, f32p, C.c_uint64, C.c_uint32, i32p]),
+            "beep_debug_binomial": (C.c_int, [C.c_int, C.c_int, i32p, f64p, C.c_uint64, C.c_uint32, i32p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _i32(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+def _f64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+class _SpecBuffers:
+    """Holds the numpy arrays a ctypes Spec points into."""
+
+    def __init__(self, spec):
+        keep = self.keep = []
+
+        def i32(x):
+            a = _i32(x)
+            keep.append(a)
+            return _p(a, C.c_int32)
+
+        def f64(x):
+            a = _f64(x)
+            keep.append(a)
+            return _p(a, C.c_double)
+
+        null_i32 = C.POINTER(C.c_int32)()
+        null_f64 = C.POINTER(C.c_double)()
+        A, G = spec["A"], spec["G"]
+        comps, trans, params = spec["comps"], spec["trans"], spec["params"]
+        s = Spec()
+        s.A, s.G, s.C, s.T = A, G, len(comps), len(trans)
+        s.period, s.steps_per_day = spec["period"], spec["spd"]
+        s.nparam = len(params)
+        pri = (Prior * max(1, len(params)))()
+        for i, p in enumerate(params):
+            pr = p["prior"]
+            pri[i].type = specmod.PRIOR_CODES[pr[0]]
+            pri[i].a = float(pr[1]) if len(pr) > 1 else 0.0
+            pri[i].b = float(pr[2]) if len(pr) > 2 else 0.0
+        keep.append(pri)
+        s.prior = pri
+        cs = (Comp * len(comps))()
+        for i, c in enumerate(comps):
+            cs[i].inf_param = c.get("inf", -1)
+            cs[i].mean_param = i32(c["mean"]) if c.get("mean") is not None else null_i32
+            cs[i].shape = c.get("shape", 1)
+        keep.append(cs)
+        s.comp = cs
+        ts = (Trans * len(trans))()
+        for i, t in enumerate(trans):
+            ts[i].from_, ts[i].to = t["from"], t["to"]
+            ts[i].infection = 1 if t.get("infection") else 0
+            ts[i].prob_param = i32(t["prob"]) if t.get("prob") is not None else null_i32
+        keep.append(ts)
+        s.trans = ts
+        spl = spec["splines"]
+        ss = (Spline * max(1, len(spl)))()
+        for i, sp in enumerate(spl):
+            ss[i].nknot = len(sp["t"])
+            ss[i].t = f64(sp["t"])
+            ss[i].param = i32(sp["param"])
+            ss[i].multfac = f64(sp.get("multfac", [1.0] * len(sp["t"])))
+            ss[i].factor_param = sp.get("factor", -1)
+        keep.append(ss)
+        s.nspline, s.spline = len(spl), ss
+        s.R_spline = spec["R_spline"]
+        s.Rfactor_param = spec.get("Rfactor", -1)
+        s.efoi_spline = spec.get("efoi_spline", -1)
+        s.efoi_agedist = f64(spec.get("efoi_agedist", [1.0 / G] * G))
+        s.geo_spline = spec.get("geo_spline", -1)
+        s.N_raw = f64(np.asarray(spec["N"]).reshape(-1)) if spec.get("N") is not None else null_f64
+        M = spec.get("M")
+        if M is not None:
+            s.M_diag, s.M_rowptr, s.M_to, s.M_val = f64(M["diag"]), i32(M["rowptr"]), i32(M["to"]), f64(M["val"])
+        else:
+            s.M_diag, s.M_rowptr, s.M_to, s.M_val = null_f64, null_i32, null_i32, null_f64
+        s.pop_init = i32(np.asarray(spec["pop"]).reshape(-1))
+        cov = spec.get("covars", [])
+        cv = (Covar * max(1, len(cov)))()
+        for i, c in enumerate(cov):
+            cv[i].param = c["param"]
+            cv[i].value = f64(np.asarray(c["value"]).reshape(-1))
+        keep.append(cv)
+        s.ncovar, s.covar = len(cov), cv
+        s.sus_param = i32(spec["sus"]) if spec.get("sus") is not None else null_i32
+        dirs = spec.get("dirichlet", [])
+        ds = (Dirichlet * max(1, len(dirs)))()
+        for i, d in enumerate(dirs):
+            ds[i].n = len(d["param"])
+            ds[i].param = i32(d["param"])
+            ds[i].frac = f64(d.get("frac", [1.0] * len(d["param"])))
+        keep.append(ds)
+        s.ndirichlet, s.dirichlet = len(dirs), ds
+        obs = spec.get("obs", [])
+        os_ = (Obs * max(1, len(obs)))()
+        for i, o in enumerate(obs):
+            x = os_[i]
+            x.y = o["y"]
+            x.sd = o.get("sd", 0.0)
+            x.shape = o.get("shape", 1.0)
+            x.factor = o.get("factor", 1.0)
+            x.thresh = o.get("thresh", 0.0)
+            x.percent = o.get("percent", 0.0)
+            x.type = specmod.OBS_CODES[o["type"]] if isinstance(o["type"], str) else o["type"]
+            x.datatable = o.get("datatable", 0)
+            x.graph = o.get("graph", 0)
+            x.sett_i, x.sett_f = o["sett_i"], o["sett_f"]
+            x.narea, x.area = len(o["area"]), i32(o["area"])
+            x.ndp, x.dp = len(o["dp"]), i32(o["dp"])
+            tr, co = o.get("tr", []), o.get("co", [])
+            x.ntr, x.tr = len(tr), i32(tr) if tr else null_i32
+            x.nco, x.co = len(co), i32(co) if co else null_i32
+        keep.append(os_)
+        s.nobs, s.obs = len(obs), os_
+        secs = spec.get("sections", [])
+        s.nsection = len(secs)
+        s.section_ti = i32([a for a, b in secs]) if secs else null_i32
+        s.section_tf = i32([b for a, b in secs]) if secs else null_i32
+        self.spec = s
+
+
+def spec_tables(spec, paramval):
+    """Host-side FP64 tables of State::set_param (no device needed)."""
+    L = lib()
+    buf = _SpecBuffers(spec)
+    A, G = spec["A"], spec["G"]
+    C_, T = len(spec["comps"]), len(spec["trans"])
+    S = spec["period"] * spec["spd"]
+    pv = _f64(paramval)
+    out = dict(rate=np.zeros((T, G)), sus=np.zeros(G), inf=np.zeros(C_), N=np.zeros((G, G)),
+               beta_af=np.zeros((S, A)), efoi=np.zeros((S, G)), geo=np.zeros(S))
+    Pr = np.zeros(1)
+    inb = np.zeros(1, dtype=np.int32)
+    err = C.create_string_buffer(256)
+    rc = L.beep_spec_tables(C.byref(buf.spec), _p(pv, C.c_double),
+                            *[_p(out[k], C.c_double) for k in ("rate", "sus", "inf", "N", "beta_af", "efoi", "geo")],
+                            _p(Pr, C.c_double), _p(inb, C.c_int32), err, 256)
+    if rc != BEEP_OK:
+        raise BeepError(rc, err.value.decode())
+    out["Pr"] = float(Pr[0])
+    out["inbounds"] = bool(inb[0])
+    return out
+
+
+class Context:
+    """One device context: a model, `capacity` instances and `nslot` parameter
+    slots (the state of many State objects at once)."""
+
+    def __init__(self, spec, capacity, nslot=None, keep=KEEP_ALL, device=0, hbm_budget=0):
+        L = lib()
+        self.spec = spec
+        self.capacity = capacity
+        self.nslot = nslot if nslot is not None else capacity
+        buf = _SpecBuffers(spec)
+        h = C.c_void_p()
+        rc = L.beep_create(C.byref(buf.spec), device, hbm_budget, capacity, self.nslot, keep, C.byref(h))
+        self._h = h
+        if rc != BEEP_OK:
+            msg = L.beep_last_error(h).decode() if h.value else "beep_create failed"
+            if h.value:
+                L.beep_destroy(h)
+            self._h = C.c_void_p()
+            raise BeepError(rc, msg)
+        d = np.zeros(10, dtype=np.int32)
+        L.beep_get_dims(h, _p(d, C.c_int32))
+        self.A, self.G, self.C, self.T, self.S, self.nparam, self.nobs, self.ndt = [int(v) for v in d[:8]]
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().beep_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != BEEP_OK:
+            raise BeepError(rc, lib().beep_last_error(self._h).decode())
+
+    @property
+    def device_bytes(self):
+        return lib().beep_device_bytes(self._h)
+
+    def set_param(self, slots, paramvals):
+        slots = _i32(np.atleast_1d(slots))
+        pv = _f64(np.atleast_2d(paramvals))
+        n = len(slots)
+        assert pv.shape == (n, self.nparam)
+        Pr = np.zeros(n)
+        inb = np.zeros(n, dtype=np.int32)
+        self._chk(lib().beep_set_param(self._h, n, _p(slots, C.c_int32), _p(pv, C.c_double), _p(Pr, C.c_double),
+                                       _p(inb, C.c_int32)))
+        return Pr, inb.astype(bool)
+
+    def get_tables(self, slot):
+        A, G, C_, T, S = self.A, self.G, self.C, self.T, self.S
+        out = dict(rate=np.zeros((T, G)), sus=np.zeros(G), inf=np.zeros(C_), N=np.zeros((G, G)),
+                   beta_af=np.zeros((S, A)), efoi=np.zeros((S, G)), geo=np.zeros(S))
+        self._chk(lib().beep_get_tables(self._h, slot, *[_p(out[k], C.c_double) for k in
+                                                         ("rate", "sus", "inf", "N", "beta_af", "efoi", "geo")]))
+        return out
+
+    def simulate(self, inst, slot, key, t0=0, t1=None, seed=1, stream=0, plain=False):
+        inst, slot, key = _i32(inst), _i32(slot), np.ascontiguousarray(np.asarray(key, dtype=np.uint32))
+        t1 = self.S if t1 is None else t1
+        self._chk(lib().beep_simulate(self._h, len(inst), _p(inst, C.c_int32), _p(slot, C.c_int32),
+                                      _p(key, C.c_uint32), t0, t1, seed, stream, SIM_PLAIN if plain else 0))
+
+    def obs_loglik(self, inst, section=-1, with_dt=False):
+        inst = _i32(inst)
+        L = np.zeros(len(inst))
+        EFdt = np.zeros((len(inst), self.ndt))
+        self._chk(lib().beep_obs_loglik(self._h, len(inst), _p(inst, C.c_int32), section, _p(L, C.c_double),
+                                        _p(EFdt, C.c_double)))
+        return (L, EFdt) if with_dt else L
+
+    def obs_values(self, inst):
+        out = np.zeros(self.nobs)
+        self._chk(lib().beep_obs_values(self._h, inst, _p(out, C.c_double)))
+        return out
+
+    def pois_loglik(self, inst):
+        inst = _i32(inst)
+        L = np.zeros(len(inst))
+        self._chk(lib().beep_pois_loglik(self._h, len(inst), _p(inst, C.c_int32), _p(L, C.c_double)))
+        return L
+
+    def mbp(self, init, prop, slot, key, simu_mask=None, seed=1, stream=0, inf_update=1):
+        init, prop, slot = _i32(init), _i32(prop), _i32(slot)
+        key = np.ascontiguousarray(np.asarray(key, dtype=np.uint32))
+        mp = C.POINTER(C.c_uint8)()
+        if simu_mask is not None:
+            simu_mask = np.ascontiguousarray(np.asarray(simu_mask, dtype=np.uint8))
+            assert simu_mask.shape == (len(init), self.S, self.A)
+            mp = _p(simu_mask, C.c_uint8)
+        self._chk(lib().beep_mbp(self._h, len(init), _p(init, C.c_int32), _p(prop, C.c_int32), _p(slot, C.c_int32),
+                                 _p(key, C.c_uint32), mp, seed, stream, inf_update))
+
+    def accept(self, init, prop, mode, dlogq, key, invT=1.0, EFcut=0.0, seed=1, stream=0):
+        init, prop = _i32(init), _i32(prop)
+        dlogq = _f64(dlogq)
+        key = np.ascontiguousarray(np.asarray(key, dtype=np.uint32))
+        al = np.zeros(len(init))
+        acc = np.zeros(len(init), dtype=np.uint8)
+        self._chk(lib().beep_accept(self._h, len(init), _p(init, C.c_int32), _p(prop, C.c_int32), mode, invT, EFcut,
+                                    _p(dlogq, C.c_double), _p(key, C.c_uint32), seed, stream, _p(al, C.c_double),
+                                    _p(acc, C.c_uint8)))
+        return al, acc.astype(bool)
+
+    def get_transnum(self, inst):
+        out = np.zeros((self.S, self.A, self.T, self.G), dtype=np.int32)
+        self._chk(lib().beep_get_transnum(self._h, inst, _p(out, C.c_int32)))
+        return out
+
+    def get_transmean(self, inst):
+        out = np.zeros((self.S, self.A, self.T, self.G), dtype=np.float32)
+        self._chk(lib().beep_get_transmean(self._h, inst, _p(out, C.c_float)))
+        return out
+
+    def get_pop(self, inst):
+        out = np.zeros((self.S, self.A, self.C, self.G), dtype=np.int32)
+        self._chk(lib().beep_get_pop(self._h, inst, _p(out, C.c_int32)))
+        return out
+
+    def get_state(self, inst):
+        out = np.zeros((self.A, self.C, self.G), dtype=np.int32)
+        self._chk(lib().beep_get_state(self._h, inst, _p(out, C.c_int32)))
+        return out
+
+    def put_state(self, inst, pop):
+        pop = _i32(pop)
+        assert pop.size == self.A * self.C * self.G
+        self._chk(lib().beep_put_state(self._h, inst, _p(pop, C.c_int32)))
+
+    def put_transnum(self, inst, slot, transnum):
+        tn = _i32(transnum)
+        assert tn.size == self.S * self.A * self.T * self.G
+        self._chk(lib().beep_put_transnum(self._h, inst, slot, _p(tn, C.c_int32)))
+
+    def copy_instances(self, dst, src):
+        dst, src = _i32(dst), _i32(src)
+        self._chk(lib().beep_copy_instances(self._h, len(dst), _p(dst, C.c_int32), _p(src, C.c_int32)))
+
+    def synchronize(self):
+        self._chk(lib().beep_synchronize(self._h))
+
+    def timing(self):
+        """(simulate/MBP kernel ms, observation kernel ms, Poisson log-lik kernel ms, launches)."""
+        out = np.zeros(4)
+        self._chk(lib().beep_timing(self._h, _p(out, C.c_double)))
+        return out
+
+
+def debug_philox(ctr, k0, k1, device=0):
+    ctr = np.ascontiguousarray(np.asarray(ctr, dtype=np.uint32).reshape(-1, 4))
+    out = np.zeros_like(ctr)
+    rc = lib().beep_debug_philox(device, len(ctr), _p(ctr, C.c_uint32), k0, k1, _p(out, C.c_uint32))
+    if rc:
+        raise BeepError(rc, "beep_debug_philox")
+    return out
+
+
+def debug_poisson(lam, seed=1, key=0, device=0):
+    lam = np.ascontiguousarray(np.asarray(lam, dtype=np.float32))
+    out = np.zeros(len(lam), dtype=np.int32)
+    rc = lib().beep_debug_poisson(device, len(lam), _p(lam, C.c_float), seed, key, _p(out, C.c_int32))
+    if rc:
+        raise BeepError(rc, "beep_debug_poisson")
+    return out
+
+
+def debug_binomial(nn, p, seed=1, key=0, device=0):
+    nn = _i32(nn)
+    p = _f64(p)
+    out = np.zeros(len(nn), dtype=np.int32)
+    rc = lib().beep_debug_binomial(device, len(nn), _p(nn, C.c_int32), _p(p, C.c_double), seed, key,
+                                   _p(out, C.c_int32))
+    if rc:
+        raise BeepError(rc, "beep_debug_binomial")
+    return out

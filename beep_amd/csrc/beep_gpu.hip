@@ -1,0 +1,818 @@
+// C ABI of the MI355X hot path: contexts, parameter slots, instances, and the
+// launches of the simulation / observation / likelihood / MBP / accept kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/beep_gpu.h"
+#include "host_model.hh"
+#include "kernels.hh"
+#include "obs_kernels.hh"
+
+using beepdev::Job;
+using beepdev::TopoDev;
+
+struct beep_ctx {
+  beephost::HostModel m;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int keep = 0, capacity = 0, nslot = 0;
+  size_t budget = 0, allocated = 0;
+  std::string err;
+  std::vector<void*> allocs;
+
+  // topology
+  TopoDev* d_topo = nullptr;
+  TopoDev h_topo;
+  // parameter slots
+  float* d_tab = nullptr;
+  size_t slot_stride = 0;
+  std::vector<double> slot_param;   // [nslot][nparam] last parameters of each slot
+  std::vector<char> slot_set;
+  // instances
+  int* d_state = nullptr;
+  int* d_tn = nullptr;
+  float* d_tm = nullptr;
+  int* d_pt = nullptr;
+  double* d_EF = nullptr;           // last EF of each instance
+  int* d_err = nullptr;
+  size_t state_sz = 0, traj_sz = 0, pop_traj_sz = 0;
+  // observations
+  beepdev::ObsDev obs;
+  beepdev::ObsDev* d_obs = nullptr;
+  std::vector<int> sec_ptr;
+  int* d_sec_ptr = nullptr;
+  int* d_sec_obs = nullptr;
+  // staging
+  Job* d_jobs = nullptr;
+  size_t jobs_cap = 0;
+  void* d_scratch = nullptr;
+  size_t scratch_cap = 0;
+  // timing of the last launch of each kernel kind
+  hipEvent_t ev[6] = {};
+  double timing[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(beep_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(c, x)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (x);                                                          \
+    if (e_ != hipSuccess) return fail(c, BEEP_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+int dalloc(beep_ctx* c, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return BEEP_OK;
+  if (c->budget && c->allocated + bytes > c->budget)
+    return fail(c, BEEP_ENOMEM, "HBM budget exceeded: %zu + %zu > %zu bytes", c->allocated, bytes, c->budget);
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(c, BEEP_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  c->allocated += bytes;
+  c->allocs.push_back(*p);
+  return BEEP_OK;
+}
+
+template <class T>
+int upload(beep_ctx* c, T** d, const std::vector<T>& h) {
+  size_t bytes = std::max<size_t>(1, h.size()) * sizeof(T);
+  int rc = dalloc(c, (void**)d, bytes);
+  if (rc) return rc;
+  if (!h.empty()) HIPCHK(c, hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return BEEP_OK;
+}
+
+std::vector<float> to_f32(const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); }
+
+int ensure_jobs(beep_ctx* c, size_t n) {
+  if (n <= c->jobs_cap) return BEEP_OK;
+  if (c->d_jobs) {
+    HIPCHK(c, hipFree(c->d_jobs));
+    c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->d_jobs));
+    c->allocated -= c->jobs_cap * sizeof(Job);
+    c->d_jobs = nullptr;
+    c->jobs_cap = 0;
+  }
+  size_t cap = std::max<size_t>(n, 1024);
+  int rc = dalloc(c, (void**)&c->d_jobs, cap * sizeof(Job));
+  if (rc) return rc;
+  c->jobs_cap = cap;
+  return BEEP_OK;
+}
+
+int ensure_scratch(beep_ctx* c, size_t bytes) {
+  if (bytes <= c->scratch_cap) return BEEP_OK;
+  if (c->d_scratch) {
+    HIPCHK(c, hipFree(c->d_scratch));
+    c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), c->d_scratch));
+    c->allocated -= c->scratch_cap;
+    c->d_scratch = nullptr;
+    c->scratch_cap = 0;
+  }
+  size_t cap = std::max<size_t>(bytes, 1 << 16);
+  int rc = dalloc(c, &c->d_scratch, cap);
+  if (rc) return rc;
+  c->scratch_cap = cap;
+  return BEEP_OK;
+}
+
+void record(beep_ctx* c, int k) { (void)hipEventRecord(c->ev[k], c->stream); }
+double elapsed(beep_ctx* c, int k0, int k1) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, c->ev[k0], c->ev[k1]) != hipSuccess) return -1;
+  return ms;
+}
+
+bool valid_inst(const beep_ctx* c, int i) { return i >= 0 && i < c->capacity; }
+bool valid_slot(const beep_ctx* c, int s) { return s >= 0 && s < c->nslot; }
+
+int check_device_error(beep_ctx* c) {
+  int e = 0;
+  HIPCHK(c, hipMemcpyAsync(&e, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (e) {
+    HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    if (e & 1) return fail(c, BEEP_ENAN, "NaN transition mean");
+    return fail(c, BEEP_ELARGE, "transition mean above 1e9");
+  }
+  return BEEP_OK;
+}
+
+// Threads per workgroup for the fused kernel: enough lanes for the cells of
+// one instance, at most 256.
+int fused_block(int AG) {
+  if (AG <= 64) return 64;
+  if (AG <= 128) return 128;
+  if (AG <= 192) return 192;
+  return 256;
+}
+
+template <int MODE>
+int launch_fused_t(beep_ctx* c, const Job* djobs, int njobs, int T, size_t lds, int block, uint64_t seed,
+                   uint32_t stream) {
+  if (njobs == 0) return BEEP_OK;
+  dim3 grid(njobs), blk(block);
+#define BEEP_LAUNCH(TM)                                                                                   \
+  do {                                                                                                    \
+    auto kfn = beepdev::k_fused<MODE, TM>;                                                                \
+    if (lds > 65536) HIPCHK(c, hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+    hipLaunchKernelGGL(kfn, grid, blk, lds, c->stream, djobs, seed, stream);                              \
+  } while (0)
+  if (T <= 4) BEEP_LAUNCH(4);
+  else if (T <= 8) BEEP_LAUNCH(8);
+  else if (T <= 12) BEEP_LAUNCH(12);
+  else if (T <= 16) BEEP_LAUNCH(16);
+  else return fail(c, BEEP_EINVAL, "at most 16 transitions are supported");
+#undef BEEP_LAUNCH
+  HIPCHK(c, hipGetLastError());
+  return BEEP_OK;
+}
+
+int launch_fused(beep_ctx* c, int mode, const Job* djobs, int njobs, uint64_t seed, uint32_t stream) {
+  const auto& m = c->m;
+  size_t lds = beepdev::fused_lds_bytes(m.AG, m.C, m.T, m.G);
+  if (lds > 160 * 1024) return fail(c, BEEP_EINVAL, "instance needs %zu bytes of LDS (max 163840)", lds);
+  int block = fused_block(m.AG);
+  if (mode == beepdev::MODE_SIM) return launch_fused_t<beepdev::MODE_SIM>(c, djobs, njobs, m.T, lds, block, seed, stream);
+  if (mode == beepdev::MODE_MBP) return launch_fused_t<beepdev::MODE_MBP>(c, djobs, njobs, m.T, lds, block, seed, stream);
+  return launch_fused_t<beepdev::MODE_REPLAY>(c, djobs, njobs, m.T, lds, block, seed, stream);
+}
+
+Job make_job(beep_ctx* c, int inst, int slot, uint32_t key, int t0, int t1) {
+  Job j;
+  std::memset(&j, 0, sizeof j);
+  j.topo = c->d_topo;
+  j.tab = c->d_tab + (size_t)slot * c->slot_stride;
+  j.state = c->d_state + (size_t)inst * c->state_sz;
+  j.tn = c->d_tn ? c->d_tn + (size_t)inst * c->traj_sz : nullptr;
+  j.tm = c->d_tm ? c->d_tm + (size_t)inst * c->traj_sz : nullptr;
+  j.pt = c->d_pt ? c->d_pt + (size_t)inst * c->pop_traj_sz : nullptr;
+  j.err = c->d_err;
+  j.key = key;
+  j.t0 = t0;
+  j.t1 = t1;
+  j.from_init = t0 == 0 ? 1 : 0;
+  return j;
+}
+
+// Copies host-side job descriptors to the device (the copy is ordered before
+// the kernels queued after it on the context stream).
+int put_jobs(beep_ctx* c, const std::vector<Job>& jobs) {
+  int rc = ensure_jobs(c, jobs.size());
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_jobs, jobs.data(), jobs.size() * sizeof(Job), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));   // the host vector is released after return
+  return BEEP_OK;
+}
+
+template <class F>
+int debug_run(int device, size_t in_bytes, const void* in, size_t in2_bytes, const void* in2, size_t out_bytes, void* out,
+              F launch) {
+  if (hipSetDevice(device) != hipSuccess) return BEEP_EHIP;
+  void *d_in = nullptr, *d_in2 = nullptr, *d_out = nullptr;
+  int rc = BEEP_OK;
+  if (hipMalloc(&d_in, in_bytes) != hipSuccess || hipMalloc(&d_out, out_bytes) != hipSuccess ||
+      (in2_bytes && hipMalloc(&d_in2, in2_bytes) != hipSuccess))
+    rc = BEEP_ENOMEM;
+  if (!rc && hipMemcpy(d_in, in, in_bytes, hipMemcpyHostToDevice) != hipSuccess) rc = BEEP_EHIP;
+  if (!rc && in2_bytes && hipMemcpy(d_in2, in2, in2_bytes, hipMemcpyHostToDevice) != hipSuccess) rc = BEEP_EHIP;
+  if (!rc) {
+    launch(d_in, d_in2, d_out);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = BEEP_EHIP;
+  }
+  if (!rc && hipMemcpy(out, d_out, out_bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = BEEP_EHIP;
+  if (d_in) (void)hipFree(d_in);
+  if (d_in2) (void)hipFree(d_in2);
+  if (d_out) (void)hipFree(d_out);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* beep_last_error(const beep_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int beep_spec_tables(const beep_spec* spec, const double* paramval, double* rate, double* sus, double* inf,
+                     double* Ntime, double* beta_af, double* efoi, double* geo, double* Pr, int32_t* inb, char* err,
+                     int errlen) {
+  auto put_err = [&](const std::string& e) {
+    if (err && errlen > 0) { std::strncpy(err, e.c_str(), errlen - 1); err[errlen - 1] = 0; }
+  };
+  if (!spec || !paramval) return BEEP_EINVAL;
+  beephost::HostModel m;
+  std::string e = m.build(*spec);
+  if (!e.empty()) { put_err(e); return BEEP_EINVAL; }
+  beephost::Tables tb;
+  int rc = m.derive(paramval, tb, e);
+  if (rc) { put_err(e); return rc; }
+  if (rate) std::copy(tb.rate.begin(), tb.rate.end(), rate);
+  if (sus) std::copy(tb.sus.begin(), tb.sus.end(), sus);
+  if (inf) std::copy(tb.inf.begin(), tb.inf.end(), inf);
+  if (Ntime) std::copy(tb.Ntime.begin(), tb.Ntime.end(), Ntime);
+  if (beta_af) std::copy(tb.beta_af.begin(), tb.beta_af.end(), beta_af);
+  if (efoi) std::copy(tb.efoi.begin(), tb.efoi.end(), efoi);
+  if (geo) std::copy(tb.geo.begin(), tb.geo.end(), geo);
+  if (Pr) *Pr = m.prior_logpdf(paramval);
+  if (inb) *inb = m.inbounds(paramval) ? 1 : 0;
+  return BEEP_OK;
+}
+
+int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capacity, int nslot, int keep,
+                beep_ctx** out) {
+  if (!out) return BEEP_EINVAL;
+  *out = nullptr;
+  if (!spec || capacity <= 0 || nslot <= 0) return BEEP_EINVAL;
+  beep_ctx* c = new beep_ctx();
+  std::string e = c->m.build(*spec);
+  if (!e.empty()) {
+    c->err = e;
+    *out = c;   // the caller reads the message, then destroys the context
+    return BEEP_EINVAL;
+  }
+  c->device = device;
+  c->budget = hbm_budget;
+  c->capacity = capacity;
+  c->nslot = nslot;
+  c->keep = keep;
+  *out = c;
+  HIPCHK(c, hipSetDevice(device));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
+  const auto& m = c->m;
+
+  // Topology.
+  int *d_from, *d_to, *d_in_ptr, *d_in_src, *d_pop_init;
+  float *d_diag, *d_fac, *d_in_w;
+  std::vector<int> from(m.T), to(m.T);
+  for (int t = 0; t < m.T; t++) { from[t] = m.trans[t].from; to[t] = m.trans[t].to; }
+  std::vector<int> pop_dev((size_t)m.C * m.AG);     // [C][AG]
+  for (int a = 0; a < m.A; a++)
+    for (int co = 0; co < m.C; co++)
+      for (int g = 0; g < m.G; g++)
+        pop_dev[(size_t)co * m.AG + a * m.G + g] = m.pop_init[((size_t)a * m.C + co) * m.G + g];
+  int rc;
+  if ((rc = upload(c, &d_from, from)) || (rc = upload(c, &d_to, to)) || (rc = upload(c, &d_in_ptr, m.in_ptr)) ||
+      (rc = upload(c, &d_in_src, m.in_src)) || (rc = upload(c, &d_pop_init, pop_dev)))
+    return rc;
+  std::vector<float> diag = to_f32(m.M_diag), fac = to_f32(m.M_factor), inw = to_f32(m.in_w);
+  if ((rc = upload(c, &d_diag, diag)) || (rc = upload(c, &d_fac, fac)) || (rc = upload(c, &d_in_w, inw))) return rc;
+
+  TopoDev& tp = c->h_topo;
+  tp.A = m.A; tp.G = m.G; tp.C = m.C; tp.T = m.T; tp.S = m.S; tp.AG = m.AG; tp.inf_tr = m.inf_tr; tp.spd = m.spd;
+  tp.dt = (float)m.dt;
+  int off = 0;
+  tp.off_rate = off; off += m.T * m.G;
+  tp.off_sus = off; off += m.G;
+  tp.off_inf = off; off += m.C;
+  tp.off_N = off; off += m.G * m.G;
+  tp.off_efoi = off; off += m.S * m.G;
+  tp.off_geo = off; off += m.S;
+  tp.off_beta = off; off += m.S * m.A;
+  c->slot_stride = ((size_t)off + 63) / 64 * 64;
+  tp.from = d_from; tp.to = d_to; tp.diag = d_diag; tp.fac = d_fac;
+  tp.in_ptr = d_in_ptr; tp.in_src = d_in_src; tp.in_w = d_in_w; tp.pop_init = d_pop_init;
+  if ((rc = dalloc(c, (void**)&c->d_topo, sizeof(TopoDev)))) return rc;
+  HIPCHK(c, hipMemcpy(c->d_topo, &tp, sizeof tp, hipMemcpyHostToDevice));
+
+  // Slots and instances.
+  if ((rc = dalloc(c, (void**)&c->d_tab, c->slot_stride * nslot * sizeof(float)))) return rc;
+  HIPCHK(c, hipMemset(c->d_tab, 0, c->slot_stride * nslot * sizeof(float)));
+  c->slot_param.assign((size_t)nslot * m.nparam, 0.0);
+  c->slot_set.assign(nslot, 0);
+  c->state_sz = (size_t)m.C * m.AG;
+  c->traj_sz = (size_t)m.S * m.T * m.AG;
+  c->pop_traj_sz = (size_t)m.S * m.C * m.AG;
+  if ((rc = dalloc(c, (void**)&c->d_state, c->state_sz * capacity * sizeof(int)))) return rc;
+  if (keep & BEEP_KEEP_TRANSNUM)
+    if ((rc = dalloc(c, (void**)&c->d_tn, c->traj_sz * capacity * sizeof(int)))) return rc;
+  if (keep & BEEP_KEEP_TRANSMEAN)
+    if ((rc = dalloc(c, (void**)&c->d_tm, c->traj_sz * capacity * sizeof(float)))) return rc;
+  if (keep & BEEP_KEEP_POP)
+    if ((rc = dalloc(c, (void**)&c->d_pt, c->pop_traj_sz * capacity * sizeof(int)))) return rc;
+  if ((rc = dalloc(c, (void**)&c->d_EF, capacity * sizeof(double)))) return rc;
+  HIPCHK(c, hipMemset(c->d_EF, 0, capacity * sizeof(double)));
+  if ((rc = dalloc(c, (void**)&c->d_err, sizeof(int)))) return rc;
+  HIPCHK(c, hipMemset(c->d_err, 0, sizeof(int)));
+  for (int i = 0; i < capacity; i++)
+    HIPCHK(c, hipMemcpy(c->d_state + (size_t)i * c->state_sz, d_pop_init, c->state_sz * sizeof(int),
+                        hipMemcpyDeviceToDevice));
+
+  // Observations.
+  {
+    std::vector<int> optr(1, 0), type, dtab;
+    std::vector<uint32_t> elem;
+    std::vector<double> y, sd, shape, factor, thresh;
+    for (const auto& o : m.obs) {
+      elem.insert(elem.end(), o.elem.begin(), o.elem.end());
+      optr.push_back((int)elem.size());
+      type.push_back(o.type); dtab.push_back(o.datatable);
+      y.push_back(o.y); sd.push_back(o.sd); shape.push_back(o.shape); factor.push_back(o.factor); thresh.push_back(o.thresh);
+    }
+    beepdev::ObsDev& od = c->obs;
+    od.nobs = (int)m.obs.size();
+    od.ndt = m.ndatatable;
+    od.TAG = (uint32_t)(m.T * m.AG);
+    od.CAG = (uint32_t)(m.C * m.AG);
+    if ((rc = upload(c, (int**)&od.ptr, optr)) || (rc = upload(c, (uint32_t**)&od.elem, elem)) ||
+        (rc = upload(c, (int**)&od.type, type)) || (rc = upload(c, (int**)&od.dt, dtab)) ||
+        (rc = upload(c, (double**)&od.y, y)) || (rc = upload(c, (double**)&od.sd, sd)) ||
+        (rc = upload(c, (double**)&od.shape, shape)) || (rc = upload(c, (double**)&od.factor, factor)) ||
+        (rc = upload(c, (double**)&od.thresh, thresh)))
+      return rc;
+    if ((rc = dalloc(c, (void**)&c->d_obs, sizeof(beepdev::ObsDev)))) return rc;
+    HIPCHK(c, hipMemcpy(c->d_obs, &od, sizeof od, hipMemcpyHostToDevice));
+    std::vector<int> sptr(1, 0), sobs;
+    for (const auto& l : m.section_obs) {
+      sobs.insert(sobs.end(), l.begin(), l.end());
+      sptr.push_back((int)sobs.size());
+    }
+    c->sec_ptr = sptr;
+    if ((rc = upload(c, &c->d_sec_ptr, sptr)) || (rc = upload(c, &c->d_sec_obs, sobs))) return rc;
+  }
+  return BEEP_OK;
+}
+
+int beep_destroy(beep_ctx* c) {
+  if (!c) return BEEP_EINVAL;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BEEP_OK;
+}
+
+int beep_get_dims(const beep_ctx* c, int32_t* d) {
+  if (!c || !d) return BEEP_EINVAL;
+  const auto& m = c->m;
+  d[0] = m.A; d[1] = m.G; d[2] = m.C; d[3] = m.T; d[4] = m.S; d[5] = m.nparam;
+  d[6] = (int)m.obs.size(); d[7] = m.ndatatable; d[8] = c->capacity; d[9] = c->nslot;
+  return BEEP_OK;
+}
+
+size_t beep_device_bytes(const beep_ctx* c) { return c ? c->allocated : 0; }
+
+int beep_set_param(beep_ctx* c, int n, const int32_t* slot, const double* paramval, double* Pr_out,
+                   int32_t* inb_out) {
+  if (!c || n < 0 || (n > 0 && (!slot || !paramval))) return fail(c, BEEP_EINVAL, "beep_set_param: bad arguments");
+  const auto& m = c->m;
+  std::vector<float> block(c->slot_stride);
+  beephost::Tables tb;
+  for (int i = 0; i < n; i++) {
+    if (!valid_slot(c, slot[i])) return fail(c, BEEP_EINVAL, "slot %d out of range", slot[i]);
+    const double* pv = paramval + (size_t)i * m.nparam;
+    if (Pr_out) Pr_out[i] = m.prior_logpdf(pv);
+    if (inb_out) inb_out[i] = m.inbounds(pv) ? 1 : 0;
+    std::string e;
+    int rc = m.derive(pv, tb, e);
+    if (rc) return fail(c, rc, "set_param: %s", e.c_str());
+    const TopoDev& tp = c->h_topo;
+    std::fill(block.begin(), block.end(), 0.f);
+    for (size_t k = 0; k < tb.rate.size(); k++) block[tp.off_rate + k] = (float)tb.rate[k];
+    for (size_t k = 0; k < tb.sus.size(); k++) block[tp.off_sus + k] = (float)tb.sus[k];
+    for (size_t k = 0; k < tb.inf.size(); k++) block[tp.off_inf + k] = (float)tb.inf[k];
+    for (size_t k = 0; k < tb.Ntime.size(); k++) block[tp.off_N + k] = (float)tb.Ntime[k];
+    for (size_t k = 0; k < tb.efoi.size(); k++) block[tp.off_efoi + k] = (float)tb.efoi[k];
+    for (size_t k = 0; k < tb.geo.size(); k++) block[tp.off_geo + k] = (float)tb.geo[k];
+    for (size_t k = 0; k < tb.beta_af.size(); k++) block[tp.off_beta + k] = (float)tb.beta_af[k];
+    HIPCHK(c, hipMemcpyAsync(c->d_tab + (size_t)slot[i] * c->slot_stride, block.data(), block.size() * sizeof(float),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::copy(pv, pv + m.nparam, c->slot_param.begin() + (size_t)slot[i] * m.nparam);
+    c->slot_set[slot[i]] = 1;
+  }
+  return BEEP_OK;
+}
+
+int beep_get_tables(beep_ctx* c, int slot, double* rate, double* sus, double* inf, double* Ntime, double* beta_af,
+                    double* efoi, double* geo) {
+  if (!c || !valid_slot(c, slot) || !c->slot_set[slot]) return fail(c, BEEP_EINVAL, "slot not set");
+  beephost::Tables tb;
+  std::string e;
+  int rc = c->m.derive(&c->slot_param[(size_t)slot * c->m.nparam], tb, e);
+  if (rc) return fail(c, rc, "%s", e.c_str());
+  if (rate) std::copy(tb.rate.begin(), tb.rate.end(), rate);
+  if (sus) std::copy(tb.sus.begin(), tb.sus.end(), sus);
+  if (inf) std::copy(tb.inf.begin(), tb.inf.end(), inf);
+  if (Ntime) std::copy(tb.Ntime.begin(), tb.Ntime.end(), Ntime);
+  if (beta_af) std::copy(tb.beta_af.begin(), tb.beta_af.end(), beta_af);
+  if (efoi) std::copy(tb.efoi.begin(), tb.efoi.end(), efoi);
+  if (geo) std::copy(tb.geo.begin(), tb.geo.end(), geo);
+  return BEEP_OK;
+}
+
+int beep_simulate(beep_ctx* c, int n, const int32_t* inst, const int32_t* slot, const uint32_t* key, int t0, int t1,
+                  uint64_t seed, uint32_t stream, int flags) {
+  if (!c) return BEEP_EINVAL;
+  if (n < 0 || (n > 0 && (!inst || !slot || !key))) return fail(c, BEEP_EINVAL, "beep_simulate: bad arguments");
+  if (t0 < 0 || t1 > c->m.S || t0 > t1) return fail(c, BEEP_EINVAL, "beep_simulate: bad step range [%d,%d)", t0, t1);
+  if (stream > 0xFFFF) return fail(c, BEEP_EINVAL, "stream must be below 65536");
+  for (int i = 0; i < n; i++) {
+    if (!valid_inst(c, inst[i])) return fail(c, BEEP_EINVAL, "instance %d out of range", inst[i]);
+    if (!valid_slot(c, slot[i]) || !c->slot_set[slot[i]]) return fail(c, BEEP_EINVAL, "slot %d not set", slot[i]);
+  }
+  if (n == 0 || t0 == t1) return BEEP_OK;
+  int rc;
+  if (flags & BEEP_SIM_PLAIN) {
+    // The unbatched path: one launch per instance per step.
+    std::vector<Job> jobs;
+    jobs.reserve((size_t)n * (t1 - t0));
+    for (int i = 0; i < n; i++)
+      for (int s = t0; s < t1; s++) jobs.push_back(make_job(c, inst[i], slot[i], key[i], s, s + 1));
+    if ((rc = put_jobs(c, jobs))) return rc;
+    record(c, 0);
+    for (size_t k = 0; k < jobs.size(); k++)
+      if ((rc = launch_fused(c, beepdev::MODE_SIM, c->d_jobs + k, 1, seed, stream))) return rc;
+    record(c, 1);
+    c->timing[3] = (double)jobs.size();
+  } else {
+    // Longest jobs first: all instances of a context share one horizon, so
+    // order is immaterial here; mixed batches sort by cost.
+    std::vector<Job> jobs;
+    jobs.reserve(n);
+    for (int i = 0; i < n; i++) jobs.push_back(make_job(c, inst[i], slot[i], key[i], t0, t1));
+    if ((rc = put_jobs(c, jobs))) return rc;
+    record(c, 0);
+    if ((rc = launch_fused(c, beepdev::MODE_SIM, c->d_jobs, n, seed, stream))) return rc;
+    record(c, 1);
+    c->timing[3] = 1;
+  }
+  rc = check_device_error(c);
+  c->timing[0] = elapsed(c, 0, 1);
+  return rc;
+}
+
+int beep_mbp(beep_ctx* c, int n, const int32_t* init, const int32_t* prop, const int32_t* slot, const uint32_t* key,
+             const uint8_t* simu_mask, uint64_t seed, uint32_t stream, int inf_update) {
+  (void)inf_update;
+  if (!c) return BEEP_EINVAL;
+  if (!c->d_tn || !c->d_tm) return fail(c, BEEP_EINVAL, "MBPs need the context to keep transnum and transmean");
+  if (n < 0 || (n > 0 && (!init || !prop || !slot || !key))) return fail(c, BEEP_EINVAL, "beep_mbp: bad arguments");
+  if (stream > 0xFFFF) return fail(c, BEEP_EINVAL, "stream must be below 65536");
+  for (int i = 0; i < n; i++) {
+    if (!valid_inst(c, init[i]) || !valid_inst(c, prop[i]) || init[i] == prop[i])
+      return fail(c, BEEP_EINVAL, "bad instance pair (%d,%d)", init[i], prop[i]);
+    if (!valid_slot(c, slot[i]) || !c->slot_set[slot[i]]) return fail(c, BEEP_EINVAL, "slot %d not set", slot[i]);
+  }
+  if (n == 0) return BEEP_OK;
+  const auto& m = c->m;
+  int rc;
+  uint8_t* d_mask = nullptr;
+  size_t mask_sz = (size_t)m.S * m.A;
+  if (simu_mask) {
+    if ((rc = ensure_scratch(c, mask_sz * n))) return rc;
+    d_mask = (uint8_t*)c->d_scratch;
+    HIPCHK(c, hipMemcpyAsync(d_mask, simu_mask, mask_sz * n, hipMemcpyHostToDevice, c->stream));
+  }
+  std::vector<Job> jobs;
+  jobs.reserve(n);
+  for (int i = 0; i < n; i++) {
+    Job j = make_job(c, prop[i], slot[i], key[i], 0, m.S);
+    j.init_tn = c->d_tn + (size_t)init[i] * c->traj_sz;
+    j.init_tm = c->d_tm + (size_t)init[i] * c->traj_sz;
+    j.simu = d_mask ? d_mask + mask_sz * i : nullptr;
+    jobs.push_back(j);
+  }
+  if ((rc = put_jobs(c, jobs))) return rc;
+  record(c, 0);
+  if ((rc = launch_fused(c, beepdev::MODE_MBP, c->d_jobs, n, seed, stream))) return rc;
+  record(c, 1);
+  rc = check_device_error(c);
+  c->timing[0] = elapsed(c, 0, 1);
+  c->timing[3] = 1;
+  return rc;
+}
+
+int beep_obs_loglik(beep_ctx* c, int n, const int32_t* inst, int section, double* L_out, double* EF_dt_out) {
+  if (!c) return BEEP_EINVAL;
+  if (n < 0 || (n > 0 && (!inst || !L_out))) return fail(c, BEEP_EINVAL, "beep_obs_loglik: bad arguments");
+  if (!c->d_tn) return fail(c, BEEP_EINVAL, "the observation model needs transnum to be kept");
+  if (section >= (int)c->m.section_obs.size()) return fail(c, BEEP_EINVAL, "section %d out of range", section);
+  for (int i = 0; i < n; i++)
+    if (!valid_inst(c, inst[i])) return fail(c, BEEP_EINVAL, "instance %d out of range", inst[i]);
+  if (n == 0) return BEEP_OK;
+  const int ndt = c->m.ndatatable;
+  size_t out_doubles = (size_t)n * (1 + ndt);
+  int rc;
+  if ((rc = ensure_scratch(c, out_doubles * sizeof(double) + (size_t)n * sizeof(int)))) return rc;
+  double* d_out = (double*)c->d_scratch;
+  int* d_inst = (int*)(d_out + out_doubles);
+  HIPCHK(c, hipMemcpyAsync(d_inst, inst, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  const int* sobs = nullptr;
+  int nsel = c->m.obs.size();
+  if (section >= 0) {
+    sobs = c->d_sec_obs + c->sec_ptr[section];
+    nsel = c->sec_ptr[section + 1] - c->sec_ptr[section];
+  }
+  record(c, 2);
+  hipLaunchKernelGGL(beepdev::k_obs_loglik, dim3(n), dim3(256), 0, c->stream, c->d_obs, d_inst, c->d_tn, c->d_pt,
+                     c->traj_sz, c->pop_traj_sz, sobs, nsel, d_out, c->d_EF, section < 0 ? 1 : 0);
+  HIPCHK(c, hipGetLastError());
+  record(c, 3);
+  std::vector<double> h(out_doubles);
+  HIPCHK(c, hipMemcpyAsync(h.data(), d_out, out_doubles * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->timing[1] = elapsed(c, 2, 3);
+  for (int i = 0; i < n; i++) {
+    L_out[i] = h[(size_t)i * (1 + ndt)];
+    if (std::isnan(L_out[i])) return fail(c, BEEP_ENAN, "NaN observation log-likelihood");   // obsmodel.cc:32
+    if (EF_dt_out)
+      for (int d = 0; d < ndt; d++) EF_dt_out[(size_t)i * ndt + d] = h[(size_t)i * (1 + ndt) + 1 + d];
+  }
+  return BEEP_OK;
+}
+
+int beep_obs_values(beep_ctx* c, int inst, double* value_out) {
+  if (!c || !valid_inst(c, inst) || !value_out) return fail(c, BEEP_EINVAL, "beep_obs_values: bad arguments");
+  if (!c->d_tn) return fail(c, BEEP_EINVAL, "the observation model needs transnum to be kept");
+  int nobs = c->m.obs.size();
+  if (nobs == 0) return BEEP_OK;
+  int rc;
+  if ((rc = ensure_scratch(c, (size_t)nobs * sizeof(double)))) return rc;
+  double* d_out = (double*)c->d_scratch;
+  hipLaunchKernelGGL(beepdev::k_obs_values, dim3((nobs + 255) / 256), dim3(256), 0, c->stream, c->d_obs,
+                     c->d_tn + (size_t)inst * c->traj_sz, c->d_pt ? c->d_pt + (size_t)inst * c->pop_traj_sz : nullptr,
+                     d_out);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(value_out, d_out, nobs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BEEP_OK;
+}
+
+int beep_pois_loglik(beep_ctx* c, int n, const int32_t* inst, double* L_out) {
+  if (!c) return BEEP_EINVAL;
+  if (n < 0 || (n > 0 && (!inst || !L_out))) return fail(c, BEEP_EINVAL, "beep_pois_loglik: bad arguments");
+  if (!c->d_tn || !c->d_tm) return fail(c, BEEP_EINVAL, "State::likelihood needs transnum and transmean");
+  for (int i = 0; i < n; i++)
+    if (!valid_inst(c, inst[i])) return fail(c, BEEP_EINVAL, "instance %d out of range", inst[i]);
+  if (n == 0) return BEEP_OK;
+  int rc;
+  if ((rc = ensure_scratch(c, (size_t)n * (sizeof(double) + sizeof(int))))) return rc;
+  double* d_out = (double*)c->d_scratch;
+  int* d_inst = (int*)(d_out + n);
+  HIPCHK(c, hipMemcpyAsync(d_inst, inst, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  record(c, 4);
+  hipLaunchKernelGGL(beepdev::k_pois_loglik, dim3(n), dim3(256), 0, c->stream, d_inst, c->d_tn, c->d_tm, c->traj_sz,
+                     d_out);
+  HIPCHK(c, hipGetLastError());
+  record(c, 5);
+  HIPCHK(c, hipMemcpyAsync(L_out, d_out, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->timing[2] = elapsed(c, 4, 5);
+  return BEEP_OK;
+}
+
+int beep_accept(beep_ctx* c, int n, const int32_t* init, const int32_t* prop, int mode, double invT, double EFcut,
+                const double* dlogq, const uint32_t* key, uint64_t seed, uint32_t stream, double* al_out,
+                uint8_t* accepted) {
+  if (!c) return BEEP_EINVAL;
+  if (n < 0 || (n > 0 && (!init || !prop || !dlogq || !key || !al_out || !accepted)) || (mode != 0 && mode != 1))
+    return fail(c, BEEP_EINVAL, "beep_accept: bad arguments");
+  for (int i = 0; i < n; i++)
+    if (!valid_inst(c, init[i]) || !valid_inst(c, prop[i])) return fail(c, BEEP_EINVAL, "instance out of range");
+  if (n == 0) return BEEP_OK;
+  int rc;
+  size_t bytes = (size_t)n * (2 * sizeof(int) + 2 * sizeof(double) + sizeof(uint32_t) + sizeof(double) + 1);
+  if ((rc = ensure_scratch(c, bytes + 64))) return rc;
+  char* p = (char*)c->d_scratch;
+  double* d_dlogq = (double*)p; p += n * sizeof(double);
+  double* d_al = (double*)p; p += n * sizeof(double);
+  int* d_init = (int*)p; p += n * sizeof(int);
+  int* d_prop = (int*)p; p += n * sizeof(int);
+  uint32_t* d_key = (uint32_t*)p; p += n * sizeof(uint32_t);
+  uint8_t* d_acc = (uint8_t*)p;
+  HIPCHK(c, hipMemcpyAsync(d_dlogq, dlogq, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_init, init, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_prop, prop, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_key, key, n * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(beepdev::k_accept, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, d_init, d_prop, c->d_EF,
+                     d_dlogq, d_key, mode, invT, EFcut, seed, stream, d_al, d_acc);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(al_out, d_al, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(accepted, d_acc, n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++)
+    if (std::isnan(al_out[i])) return fail(c, BEEP_ENAN, "NaN acceptance probability");   // src/mbp.cc:353
+  return BEEP_OK;
+}
+
+// ---- state transfer ----
+
+int beep_get_transnum(beep_ctx* c, int inst, int32_t* out) {
+  if (!c || !valid_inst(c, inst) || !out || !c->d_tn) return fail(c, BEEP_EINVAL, "beep_get_transnum: bad arguments");
+  const auto& m = c->m;
+  std::vector<int> h(c->traj_sz);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_tn + (size_t)inst * c->traj_sz, c->traj_sz * sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < m.S; s++)
+    for (int a = 0; a < m.A; a++)
+      for (int t = 0; t < m.T; t++)
+        for (int g = 0; g < m.G; g++)
+          out[(((size_t)s * m.A + a) * m.T + t) * m.G + g] = h[((size_t)s * m.T + t) * m.AG + a * m.G + g];
+  return BEEP_OK;
+}
+
+int beep_get_transmean(beep_ctx* c, int inst, float* out) {
+  if (!c || !valid_inst(c, inst) || !out || !c->d_tm) return fail(c, BEEP_EINVAL, "beep_get_transmean: bad arguments");
+  const auto& m = c->m;
+  std::vector<float> h(c->traj_sz);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_tm + (size_t)inst * c->traj_sz, c->traj_sz * sizeof(float),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < m.S; s++)
+    for (int a = 0; a < m.A; a++)
+      for (int t = 0; t < m.T; t++)
+        for (int g = 0; g < m.G; g++)
+          out[(((size_t)s * m.A + a) * m.T + t) * m.G + g] = h[((size_t)s * m.T + t) * m.AG + a * m.G + g];
+  return BEEP_OK;
+}
+
+int beep_get_pop(beep_ctx* c, int inst, int32_t* out) {
+  if (!c || !valid_inst(c, inst) || !out || !c->d_pt) return fail(c, BEEP_EINVAL, "beep_get_pop: bad arguments");
+  const auto& m = c->m;
+  std::vector<int> h(c->pop_traj_sz);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_pt + (size_t)inst * c->pop_traj_sz, c->pop_traj_sz * sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int s = 0; s < m.S; s++)
+    for (int a = 0; a < m.A; a++)
+      for (int co = 0; co < m.C; co++)
+        for (int g = 0; g < m.G; g++)
+          out[(((size_t)s * m.A + a) * m.C + co) * m.G + g] = h[((size_t)s * m.C + co) * m.AG + a * m.G + g];
+  return BEEP_OK;
+}
+
+int beep_get_state(beep_ctx* c, int inst, int32_t* out) {
+  if (!c || !valid_inst(c, inst) || !out) return fail(c, BEEP_EINVAL, "beep_get_state: bad arguments");
+  const auto& m = c->m;
+  std::vector<int> h(c->state_sz);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_state + (size_t)inst * c->state_sz, c->state_sz * sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int a = 0; a < m.A; a++)
+    for (int co = 0; co < m.C; co++)
+      for (int g = 0; g < m.G; g++) out[((size_t)a * m.C + co) * m.G + g] = h[(size_t)co * m.AG + a * m.G + g];
+  return BEEP_OK;
+}
+
+int beep_put_state(beep_ctx* c, int inst, const int32_t* pop) {
+  if (!c || !valid_inst(c, inst) || !pop) return fail(c, BEEP_EINVAL, "beep_put_state: bad arguments");
+  const auto& m = c->m;
+  std::vector<int> h(c->state_sz);
+  for (int a = 0; a < m.A; a++)
+    for (int co = 0; co < m.C; co++)
+      for (int g = 0; g < m.G; g++) h[(size_t)co * m.AG + a * m.G + g] = pop[((size_t)a * m.C + co) * m.G + g];
+  HIPCHK(c, hipMemcpyAsync(c->d_state + (size_t)inst * c->state_sz, h.data(), c->state_sz * sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BEEP_OK;
+}
+
+int beep_put_transnum(beep_ctx* c, int inst, int slot, const int32_t* tn) {
+  if (!c || !valid_inst(c, inst) || !tn || !c->d_tn) return fail(c, BEEP_EINVAL, "beep_put_transnum: bad arguments");
+  if (!valid_slot(c, slot) || !c->slot_set[slot]) return fail(c, BEEP_EINVAL, "slot %d not set", slot);
+  const auto& m = c->m;
+  std::vector<int> h(c->traj_sz);
+  for (int s = 0; s < m.S; s++)
+    for (int a = 0; a < m.A; a++)
+      for (int t = 0; t < m.T; t++)
+        for (int g = 0; g < m.G; g++)
+          h[((size_t)s * m.T + t) * m.AG + a * m.G + g] = tn[(((size_t)s * m.A + a) * m.T + t) * m.G + g];
+  HIPCHK(c, hipMemcpyAsync(c->d_tn + (size_t)inst * c->traj_sz, h.data(), c->traj_sz * sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  std::vector<Job> jobs(1, make_job(c, inst, slot, 0, 0, m.S));
+  int rc;
+  if ((rc = put_jobs(c, jobs))) return rc;
+  if ((rc = launch_fused(c, beepdev::MODE_REPLAY, c->d_jobs, 1, 0, 0))) return rc;
+  return check_device_error(c);
+}
+
+int beep_copy_instances(beep_ctx* c, int n, const int32_t* dst, const int32_t* src) {
+  if (!c || n < 0 || (n > 0 && (!dst || !src))) return fail(c, BEEP_EINVAL, "beep_copy_instances: bad arguments");
+  for (int i = 0; i < n; i++) {
+    if (!valid_inst(c, dst[i]) || !valid_inst(c, src[i])) return fail(c, BEEP_EINVAL, "instance out of range");
+    if (dst[i] == src[i]) continue;
+    HIPCHK(c, hipMemcpyAsync(c->d_state + (size_t)dst[i] * c->state_sz, c->d_state + (size_t)src[i] * c->state_sz,
+                             c->state_sz * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    if (c->d_tn)
+      HIPCHK(c, hipMemcpyAsync(c->d_tn + (size_t)dst[i] * c->traj_sz, c->d_tn + (size_t)src[i] * c->traj_sz,
+                               c->traj_sz * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    if (c->d_tm)
+      HIPCHK(c, hipMemcpyAsync(c->d_tm + (size_t)dst[i] * c->traj_sz, c->d_tm + (size_t)src[i] * c->traj_sz,
+                               c->traj_sz * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    if (c->d_pt)
+      HIPCHK(c, hipMemcpyAsync(c->d_pt + (size_t)dst[i] * c->pop_traj_sz, c->d_pt + (size_t)src[i] * c->pop_traj_sz,
+                               c->pop_traj_sz * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_EF + dst[i], c->d_EF + src[i], sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BEEP_OK;
+}
+
+int beep_timing(beep_ctx* c, double* out) {
+  if (!c || !out) return BEEP_EINVAL;
+  for (int k = 0; k < 4; k++) out[k] = c->timing[k];
+  return BEEP_OK;
+}
+
+int beep_synchronize(beep_ctx* c) {
+  if (!c) return BEEP_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BEEP_OK;
+}
+
+// ---- diagnostics ----
+
+
+int beep_debug_philox(int device, int n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out) {
+  if (n <= 0 || !ctr || !out) return BEEP_EINVAL;
+  return debug_run(device, (size_t)n * 16, ctr, 0, nullptr, (size_t)n * 16, out, [&](void* a, void*, void* o) {
+    hipLaunchKernelGGL(beepdev::k_dbg_philox, dim3((n + 255) / 256), dim3(256), 0, 0, n, (const beepdev::u4*)a, k0, k1,
+                       (beepdev::u4*)o);
+  });
+}
+
+int beep_debug_poisson(int device, int n, const float* lam, uint64_t seed, uint32_t key, int32_t* out) {
+  if (n <= 0 || !lam || !out) return BEEP_EINVAL;
+  for (int i = 0; i < n; i++)
+    if (!(lam[i] >= 0.f) || lam[i] > 1e9f) return BEEP_EINVAL;
+  return debug_run(device, (size_t)n * 4, lam, 0, nullptr, (size_t)n * 4, out, [&](void* a, void*, void* o) {
+    hipLaunchKernelGGL(beepdev::k_dbg_poisson, dim3((n + 255) / 256), dim3(256), 0, 0, n, (const float*)a, seed, key,
+                       (int*)o);
+  });
+}
+
+int beep_debug_binomial(int device, int n, const int32_t* nn, const double* p, uint64_t seed, uint32_t key,
+                        int32_t* out) {
+  if (n <= 0 || !nn || !p || !out) return BEEP_EINVAL;
+  for (int i = 0; i < n; i++)
+    if (nn[i] < 0 || !(p[i] > 0.0) || !(p[i] < 1.0)) return BEEP_EINVAL;
+  return debug_run(device, (size_t)n * 4, nn, (size_t)n * 8, p, (size_t)n * 4, out, [&](void* a, void* b, void* o) {
+    hipLaunchKernelGGL(beepdev::k_dbg_binomial, dim3((n + 255) / 256), dim3(256), 0, 0, n, (const int*)a,
+                       (const double*)b, seed, key, (int*)o);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,483 @@
+// Host-side model set-up and State::set_param derivations (FP64).
+#include "host_model.hh"
+
+#include <cmath>
+#include <cstring>
+
+namespace beephost {
+
+namespace {
+const double LARGE = 1e9;
+const double EPSILON_FACTOR = 0.01;   // src/consts.hh:178
+const double VTINY = 1e-15;           // src/consts.hh:171
+
+// Gauss-Jordan inverse with partial pivoting.  Returns false when singular.
+bool invert(std::vector<double> M, int n, std::vector<double>& inv) {
+  inv.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
+  for (int col = 0; col < n; col++) {
+    int piv = col;
+    for (int r = col + 1; r < n; r++)
+      if (std::fabs(M[(size_t)r * n + col]) > std::fabs(M[(size_t)piv * n + col])) piv = r;
+    if (M[(size_t)piv * n + col] == 0) return false;
+    if (piv != col)
+      for (int k = 0; k < n; k++) {
+        std::swap(M[(size_t)piv * n + k], M[(size_t)col * n + k]);
+        std::swap(inv[(size_t)piv * n + k], inv[(size_t)col * n + k]);
+      }
+    double d = M[(size_t)col * n + col];
+    for (int k = 0; k < n; k++) { M[(size_t)col * n + k] /= d; inv[(size_t)col * n + k] /= d; }
+    for (int r = 0; r < n; r++) {
+      if (r == col) continue;
+      double f = M[(size_t)r * n + col];
+      if (f == 0) continue;
+      for (int k = 0; k < n; k++) {
+        M[(size_t)r * n + k] -= f * M[(size_t)col * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)col * n + k];
+      }
+    }
+  }
+  return true;
+}
+
+// Dominant eigenvalue by power iteration with sum normalisation and the
+// convergence test of the reference's largest_eigenvalue (src/matrix.cc:1139-1219):
+// eigenvalue and normalised vector both within 1e-13 (relaxed x10 after 100 and
+// again after 1000 iterations).  Returns NaN when it does not converge.
+double dominant_eigenvalue(const std::vector<double>& M, int n) {
+  std::vector<double> v(n, 1.0 / n), v2(n);
+  double ev = 0;
+  for (int loop = 0; loop <= 10000; loop++) {
+    for (int i = 0; i < n; i++) {
+      double s = 0;
+      for (int k = 0; k < n; k++) s += M[(size_t)i * n + k] * v[k];
+      v2[i] = s;
+    }
+    double sum = 0;
+    for (int i = 0; i < n; i++) sum += v2[i];
+    double ev_new = sum;
+    for (int i = 0; i < n; i++) v2[i] /= sum;
+    if (loop % 10 == 0) {
+      double lim = 100 * VTINY;
+      if (loop > 100) lim *= 10;
+      if (loop > 1000) lim *= 10;
+      if (ev_new - ev > -lim && ev_new - ev < lim) {
+        int k = 0;
+        for (; k < n; k++) {
+          double d = v2[k] - v[k];
+          if (d > lim || d < -lim) break;
+        }
+        if (k == n) return ev_new;
+      }
+    }
+    v = v2;
+    ev = ev_new;
+  }
+  return NAN;
+}
+}  // namespace
+
+std::string HostModel::build(const beep_spec& s) {
+  A = s.A; G = s.G; C = s.C; T = s.T;
+  period = s.period; spd = s.steps_per_day;
+  if (A <= 0 || G <= 0 || C <= 0 || T <= 0 || period <= 0 || spd <= 0) return "bad dimensions";
+  if (T > 255) return "too many transitions";
+  S = period * spd;
+  if (S > 65535) return "too many steps";
+  AG = A * G;
+  dt = (double)period / S;                                   // src/state.cc:384
+  nparam = s.nparam;
+  prior.assign(s.prior, s.prior + nparam);
+
+  comp.resize(C);
+  for (int c = 0; c < C; c++) {
+    comp[c].inf_param = s.comp[c].inf_param;
+    comp[c].shape = s.comp[c].shape > 0 ? s.comp[c].shape : 1;
+    if (s.comp[c].mean_param) comp[c].mean_param.assign(s.comp[c].mean_param, s.comp[c].mean_param + G);
+    if (comp[c].inf_param >= nparam) return "infectivity parameter out of range";
+  }
+  trans.resize(T);
+  inf_tr = -1;
+  for (int t = 0; t < T; t++) {
+    const beep_trans& x = s.trans[t];
+    if (x.from < 0 || x.from >= C || x.to < 0 || x.to >= C) return "transition compartment out of range";
+    trans[t].from = x.from; trans[t].to = x.to; trans[t].infection = x.infection;
+    if (x.prob_param) trans[t].prob_param.assign(x.prob_param, x.prob_param + G);
+    if (x.infection) {
+      if (inf_tr >= 0) return "more than one infection transition";
+      inf_tr = t;
+    }
+    comp[x.from].exits.push_back(t);
+  }
+  if (inf_tr < 0) return "no infection transition";           // src/model.cc:186-198
+  for (int t = 0; t < T; t++) {
+    if (trans[t].infection) continue;
+    const Compartment& c = comp[trans[t].from];
+    if (c.mean_param.empty()) return "compartment with an exit has no mean parameter";
+    if (c.exits.size() > 1 && trans[t].prob_param.empty()) return "branching transition has no probability parameter";
+  }
+
+  spline.resize(s.nspline);
+  for (int k = 0; k < s.nspline; k++) {
+    const beep_spline& x = s.spline[k];
+    if (x.nknot < 2) return "spline needs two knots";
+    spline[k].t.assign(x.t, x.t + x.nknot);
+    spline[k].param.assign(x.param, x.param + x.nknot);
+    spline[k].multfac.assign(x.multfac, x.multfac + x.nknot);
+    spline[k].factor_param = x.factor_param;
+  }
+  R_spline = s.R_spline; Rfactor_param = s.Rfactor_param;
+  efoi_spline = s.efoi_spline; geo_spline = s.geo_spline;
+  if (R_spline < 0 || R_spline >= s.nspline) return "R spline not set";
+  efoi_agedist.assign(G, 0.0);
+  if (s.efoi_agedist) efoi_agedist.assign(s.efoi_agedist, s.efoi_agedist + G);
+  if (s.sus_param) sus_param.assign(s.sus_param, s.sus_param + G);
+  dirichlet.resize(s.ndirichlet);
+  for (int k = 0; k < s.ndirichlet; k++) {
+    dirichlet[k].param.assign(s.dirichlet[k].param, s.dirichlet[k].param + s.dirichlet[k].n);
+    if (s.dirichlet[k].frac) dirichlet[k].frac.assign(s.dirichlet[k].frac, s.dirichlet[k].frac + s.dirichlet[k].n);
+    else dirichlet[k].frac.assign(s.dirichlet[k].n, 1.0);
+  }
+
+  // Populations (src/data.cc:288-332,421-457).
+  pop_init.assign(s.pop_init, s.pop_init + (size_t)A * C * G);
+  area_pop.assign(A, 0.0);
+  popsize = 0;
+  std::vector<double> pop_age(G, 0.0);
+  for (int c = 0; c < A; c++)
+    for (int co = 0; co < C; co++)
+      for (int g = 0; g < G; g++) {
+        double p = pop_init[((size_t)c * C + co) * G + g];
+        area_pop[c] += p;
+        pop_age[g] += p;
+      }
+  for (int c = 0; c < A; c++) {
+    if (area_pop[c] <= 0) return "area with zero population";
+    popsize += area_pop[c];
+  }
+  Rdist.assign(G, 0.0);                                       // SplineInfo::democatpos_dist (src/model.cc:409-424)
+  for (int g = 0; g < G; g++) Rdist[g] = pop_age[g] / popsize;
+
+  // Age mixing: Data::age_mixing_matrix + agematrix_normalise (src/data_matrix.cc:100-202).
+  N.assign((size_t)G * G, 0.0);
+  if (s.N_raw) {
+    std::vector<double> m((size_t)G * G);
+    for (int j = 0; j < G; j++)
+      for (int i = 0; i < G; i++) m[(size_t)j * G + i] = s.N_raw[(size_t)j * G + i] / (pop_age[j] / popsize);
+    for (int j = 0; j < G; j++)
+      for (int i = 0; i < G; i++) N[(size_t)j * G + i] = 0.5 * (m[(size_t)j * G + i] + m[(size_t)i * G + j]);
+  } else {
+    if (G != 1) return "an age mixing matrix is needed when there is more than one age group";
+    N[0] = 1;
+  }
+  {
+    double sum = 0, sum2 = 0;
+    for (int a = 0; a < G; a++)
+      for (int aa = 0; aa < G; aa++) { sum += pop_age[aa] * N[(size_t)aa * G + a]; sum2 += pop_age[aa]; }
+    sum /= sum2;
+    for (auto& v : N) v /= sum;
+  }
+
+  // Geographic mixing: Data::geo_normalise (src/data_matrix.cc:55-96).
+  M_rowptr.assign(A + 1, 0);
+  if (s.M_diag) {
+    M_diag.assign(s.M_diag, s.M_diag + A);
+    M_rowptr.assign(s.M_rowptr, s.M_rowptr + A + 1);
+    int nnz = M_rowptr[A];
+    M_to.assign(s.M_to, s.M_to + nnz);
+    M_val.assign(s.M_val, s.M_val + nnz);
+    for (int k = 0; k < nnz; k++)
+      if (M_to[k] < 0 || M_to[k] >= A) return "geo mixing destination out of range";
+  } else {
+    if (A != 1) return "a geographic mixing matrix is needed when there is more than one area";
+    M_diag.assign(1, 1.0);
+  }
+  M_factor.assign(A, 0.0);
+  for (int c = 0; c < A; c++) {
+    if (M_diag[c] == 0) return "the geographic mixing matrix cannot have zero on the diagonal";
+    double sum = M_diag[c] * area_pop[c];
+    for (int j = M_rowptr[c]; j < M_rowptr[c + 1]; j++) sum += M_val[j] * area_pop[M_to[j]];
+    M_diag[c] /= sum;
+    for (int j = M_rowptr[c]; j < M_rowptr[c + 1]; j++) M_val[j] /= sum;
+  }
+  for (int c = 0; c < A; c++) M_factor[c] = (1.0 / area_pop[c]) / M_diag[c];
+  in_ptr.assign(A + 1, 0);
+  for (int j = 0; j < M_rowptr[A]; j++) in_ptr[M_to[j] + 1]++;
+  for (int c = 0; c < A; c++) in_ptr[c + 1] += in_ptr[c];
+  in_src.assign(M_rowptr[A], 0);
+  in_w.assign(M_rowptr[A], 0.0);
+  {
+    std::vector<int> fill(in_ptr.begin(), in_ptr.end() - 1);
+    for (int c = 0; c < A; c++)
+      for (int j = M_rowptr[c]; j < M_rowptr[c + 1]; j++) {
+        int d = M_to[j];
+        in_src[fill[d]] = c;
+        in_w[fill[d]] = M_val[j];
+        fill[d]++;
+      }
+  }
+
+  // Covariates centred as Data::read_covars does (src/data.cc:557-565).
+  covar.resize(s.ncovar);
+  for (int k = 0; k < s.ncovar; k++) {
+    covar[k].param = s.covar[k].param;
+    covar[k].value.assign(s.covar[k].value, s.covar[k].value + (size_t)A * period);
+    double av = 0;
+    for (double v : covar[k].value) av += v;
+    av /= (double)A * period;
+    for (double& v : covar[k].value) v -= av;
+  }
+
+  // Infected states (Model::set_infected_uninfected, src/model.cc:110-166).
+  {
+    std::vector<char> fwd(C, 0), back(C, 0);
+    std::vector<int> list;
+    int c0 = trans[inf_tr].to;
+    list.push_back(c0);
+    fwd[c0] = 1;
+    for (size_t i = 0; i < list.size(); i++) {
+      int c = list[i];
+      for (int tr : comp[c].exits) {
+        if (trans[tr].infection) continue;
+        int cc = trans[tr].to;
+        if (!fwd[cc] && cc != trans[inf_tr].from) { list.push_back(cc); fwd[cc] = 1; }
+      }
+    }
+    for (int co = 0; co < C; co++) {
+      if (comp[co].inf_param < 0) continue;
+      std::vector<int> l2(1, co);
+      back[co] = 1;
+      for (size_t i = 0; i < l2.size(); i++)
+        for (int tr = 0; tr < T; tr++)
+          if (!trans[tr].infection && trans[tr].to == l2[i] && !back[trans[tr].from]) {
+            back[trans[tr].from] = 1;
+            l2.push_back(trans[tr].from);
+          }
+    }
+    inf_state.clear();
+    for (int co = 0; co < C; co++)
+      if (fwd[co] && back[co]) inf_state.push_back(co);
+  }
+
+  // Observations: ObservationModel::initialise_obs_change (src/obsmodel.cc:174-217)
+  // flattened per observation, and Data::set_obs_sd (src/data.cc:1589-1623).
+  obs.resize(s.nobs);
+  ndatatable = 1;
+  std::vector<double> graph_max;
+  for (int i = 0; i < s.nobs; i++) {
+    const beep_obs& o = s.obs[i];
+    if (o.graph >= (int)graph_max.size()) graph_max.resize(o.graph + 1, 0.0);
+    if (o.y != BEEP_OBS_UNKNOWN && o.y != BEEP_OBS_THRESH && o.y > graph_max[o.graph]) graph_max[o.graph] = o.y;
+  }
+  for (int i = 0; i < s.nobs; i++) {
+    const beep_obs& o = s.obs[i];
+    Obs& ob = obs[i];
+    ob.y = o.y; ob.shape = o.shape; ob.factor = o.factor; ob.thresh = o.thresh;
+    ob.type = o.type; ob.datatable = o.datatable; ob.sett_i = o.sett_i; ob.sett_f = o.sett_f;
+    if (o.type < 0 || o.type > 3) return "unknown observation model";
+    if (o.sett_i < 0 || o.sett_f > S || o.sett_i >= o.sett_f) return "observation window out of range";
+    if (o.datatable < 0 || o.datatable >= 8) return "at most 8 data tables are supported";
+    if (o.datatable + 1 > ndatatable) ndatatable = o.datatable + 1;
+    ob.sd = o.sd;
+    if (o.type == BEEP_OBS_NORMAL_PERCENT && !(o.sd > 0))
+      ob.sd = (o.y * (1 - EPSILON_FACTOR) + graph_max[o.graph] * EPSILON_FACTOR) * 0.01 * o.percent;
+    if ((o.type == BEEP_OBS_NORMAL || o.type == BEEP_OBS_NORMAL_PERCENT) && !(ob.sd > 0) &&
+        o.y != BEEP_OBS_UNKNOWN)
+      return "normal observation with non-positive sd";
+    for (int st = o.sett_i; st < o.sett_f; st++)
+      for (int a = 0; a < o.narea; a++) {
+        int c = o.area[a];
+        if (c < 0 || c >= A) return "observation area out of range";
+        for (int d = 0; d < o.ndp; d++) {
+          int dp = o.dp[d];
+          if (dp < 0 || dp >= G) return "observation dp out of range";
+          uint32_t cell = (uint32_t)(c * G + dp);
+          for (int k = 0; k < o.ntr; k++) {
+            int tr = o.tr[k];
+            if (tr < 0 || tr >= T) return "observation transition out of range";
+            ob.elem.push_back((uint32_t)(((size_t)st * T + tr) * AG + cell));
+          }
+          for (int k = 0; k < o.nco; k++) {
+            int co = o.co[k];
+            if (co < 0 || co >= C) return "observation compartment out of range";
+            ob.elem.push_back(0x80000000u | (uint32_t)(((size_t)st * C + co) * AG + cell));
+          }
+        }
+      }
+  }
+  // ObservationModel::split_observations (src/obsmodel.cc:326-348).
+  section_ti.assign(s.section_ti, s.section_ti + s.nsection);
+  section_tf.assign(s.section_tf, s.section_tf + s.nsection);
+  section_obs.assign(s.nsection, std::vector<int>());
+  for (int k = 0; k < s.nsection; k++) {
+    int ti = section_ti[k], tf = section_tf[k];
+    if (ti < 0 || tf > S || ti >= tf) return "bad section";
+    for (int i = 0; i < s.nobs; i++)
+      if (obs[i].sett_i < tf && obs[i].sett_f >= ti && obs[i].sett_i >= ti && obs[i].sett_f <= tf)
+        section_obs[k].push_back(i);
+  }
+  return "";
+}
+
+std::vector<double> HostModel::dirichlet_correct(const double* paramv) const {
+  std::vector<double> p(paramv, paramv + nparam);
+  for (const auto& d : dirichlet) {                          // src/model.cc:2685-2695
+    double sum = 0;
+    for (int th : d.param) sum += paramv[th];
+    for (size_t k = 0; k < d.param.size(); k++) p[d.param[k]] = paramv[d.param[k]] / (d.frac[k] * sum);
+  }
+  return p;
+}
+
+// Model::create_disc_spline (src/model.cc:1020-1053), SIM-mode hold past the last knot.
+std::vector<double> HostModel::disc_spline(int sp, const std::vector<double>& pd) const {
+  std::vector<double> out(S, 0.0);
+  const Spline& spl = spline[sp];
+  double factor = spl.factor_param >= 0 ? pd[spl.factor_param] : 1.0;
+  size_t p = 0;
+  for (int s = 0; s < S; s++) {
+    double t = (s + 0.5) * period / S;
+    while (p + 1 < spl.t.size() && t > spl.t[p + 1]) p++;
+    if (p + 1 == spl.t.size()) {
+      out[s] = s > 0 ? out[s - 1] : 0.0;
+    } else {
+      double fac = (t - spl.t[p]) / (spl.t[p + 1] - spl.t[p]);
+      out[s] = factor * (pd[spl.param[p]] * spl.multfac[p] * (1 - fac) + pd[spl.param[p + 1]] * spl.multfac[p + 1] * fac);
+    }
+  }
+  return out;
+}
+
+int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const {
+  std::vector<double> pd = dirichlet_correct(paramv);
+
+  // Model::create_transrate (src/model.cc:1116-1140).
+  tb.rate.assign((size_t)T * G, 0.0);
+  for (int t = 0; t < T; t++) {
+    if (trans[t].infection) continue;
+    const Compartment& c = comp[trans[t].from];
+    for (int g = 0; g < G; g++) {
+      double mean = pd[c.mean_param[g]] / c.shape;
+      if (mean == 0) { err = "zero mean residence time"; return BEEP_EINVAL; }
+      double r = 1.0 / mean;
+      if (c.exits.size() > 1) r *= pd[trans[t].prob_param[g]];
+      tb.rate[(size_t)t * G + g] = r;
+    }
+  }
+  // Model::create_susceptibility (src/model.cc:1144-1178), age the only category.
+  tb.sus.assign(G, 1.0);
+  if (!sus_param.empty())
+    for (int g = 0; g < G; g++) tb.sus[g] = sus_param[g] >= 0 ? pd[sus_param[g]] : 1.0;
+  tb.inf.assign(C, 0.0);
+  for (int c = 0; c < C; c++) tb.inf[c] = comp[c].inf_param >= 0 ? pd[comp[c].inf_param] : 0.0;
+  tb.Ntime = N;                                              // Model::create_Ntime without modifications
+
+  // Model::create_areafactor (src/model.cc:1182-1250).
+  std::vector<double> af((size_t)period * A, 1.0);
+  for (const auto& cv : covar) {
+    double b = pd[cv.param];
+    for (int t = 0; t < period; t++)
+      for (int c = 0; c < A; c++) af[(size_t)t * A + c] *= std::exp(b * cv.value[(size_t)c * period + t]);
+  }
+  {
+    double area_av = 0;
+    for (int c = 0; c < A; c++) {
+      double av = 0;
+      for (int t = 0; t < period; t++) av += af[(size_t)t * A + c];
+      av /= period;
+      area_av += av * area_pop[c];
+    }
+    area_av /= popsize;
+    for (auto& v : af) v /= area_av;
+  }
+
+  // Model::calculate_beta_from_R (src/model.cc:1318-1362) with calculate_Vinv,
+  // calculate_F, calculate_NGM and the dominant eigenvalue (2485-2621).
+  const int nis = (int)inf_state.size();
+  if (nis == 0) { err = "no infected states"; return BEEP_EINVAL; }
+  std::vector<int> ref(C, -1);
+  for (int i = 0; i < nis; i++) ref[inf_state[i]] = i;
+  const int NV = nis * G;
+  std::vector<double> V((size_t)NV * NV, 0.0), Vinv;
+  for (int t = 0; t < T; t++) {
+    int fi = ref[trans[t].from], ti = ref[trans[t].to];
+    if (fi < 0) continue;
+    for (int g = 0; g < G; g++) {
+      double r = tb.rate[(size_t)t * G + g];
+      int j = fi * G + g;
+      V[(size_t)j * NV + j] += r;
+      if (ti >= 0) V[(size_t)(ti * G + g) * NV + j] -= r;
+    }
+  }
+  if (!invert(V, NV, Vinv)) { err = "singular transition matrix V"; return BEEP_EINVAL; }
+  std::vector<double> F((size_t)G * NV, 0.0);
+  for (int si = 0; si < nis; si++) {
+    double inf = tb.inf[inf_state[si]];
+    if (inf == 0) continue;
+    for (int dp = 0; dp < G; dp++) {
+      int i = si * G + dp;
+      for (int q = 0; q < G; q++) F[(size_t)q * NV + i] += Rdist[q] * tb.sus[q] * inf * N[(size_t)q * G + dp];
+    }
+  }
+  std::vector<double> NGM((size_t)G * G, 0.0);
+  for (int q = 0; q < G; q++)
+    for (int i = 0; i < G; i++) {
+      double sum = 0;
+      for (int j = 0; j < NV; j++) sum += F[(size_t)q * NV + j] * Vinv[(size_t)j * NV + i];
+      NGM[(size_t)q * G + i] = sum;
+    }
+  double ratio = dominant_eigenvalue(NGM, G);
+  if (!(ratio > 0) || std::isnan(ratio)) { err = "next-generation matrix has no positive dominant eigenvalue"; return BEEP_EINVAL; }
+  double Rfac = Rfactor_param >= 0 ? pd[Rfactor_param] : 1.0;
+  std::vector<double> Rt = disc_spline(R_spline, pd);
+
+  tb.beta_af.assign((size_t)S * A, 0.0);
+  for (int s = 0; s < S; s++) {
+    double beta = Rfac * Rt[s] / ratio;
+    int day = s / spd;
+    for (int c = 0; c < A; c++) tb.beta_af[(size_t)s * A + c] = beta * af[(size_t)day * A + c];
+  }
+  tb.efoi.assign((size_t)S * G, 0.0);
+  if (efoi_spline >= 0) {
+    std::vector<double> e = disc_spline(efoi_spline, pd);
+    for (int s = 0; s < S; s++)
+      for (int g = 0; g < G; g++) tb.efoi[(size_t)s * G + g] = e[s] / popsize * efoi_agedist[g];
+  }
+  tb.geo.assign(S, 1.0);
+  if (geo_spline >= 0) tb.geo = disc_spline(geo_spline, pd);
+  for (double v : tb.beta_af) if (std::isnan(v)) { err = "NaN transmission rate"; return BEEP_ENAN; }
+  return BEEP_OK;
+}
+
+double HostModel::prior_logpdf(const double* pv) const {   // Model::prior (src/model.cc:1590-1659)
+  double Pr = 0;
+  for (int th = 0; th < nparam; th++) {
+    const beep_prior& p = prior[th];
+    switch (p.type) {
+      case BEEP_PRIOR_UNIFORM: Pr += std::log(1.0 / (p.b - p.a)); break;
+      case BEEP_PRIOR_EXP: Pr += std::log(p.a) - p.a * pv[th]; break;
+      case BEEP_PRIOR_NORMAL:
+        Pr += -0.5 * std::log(2 * M_PI * p.b * p.b) - (pv[th] - p.a) * (pv[th] - p.a) / (2 * p.b * p.b);
+        break;
+      case BEEP_PRIOR_DIRFLAT: Pr += -pv[th]; break;
+      default: break;
+    }
+  }
+  return Pr;
+}
+
+bool HostModel::inbounds(const double* pv) const {         // Model::inbounds (src/model.cc:2625-2669)
+  for (int th = 0; th < nparam; th++) {
+    const beep_prior& p = prior[th];
+    double v = pv[th];
+    switch (p.type) {
+      case BEEP_PRIOR_FIXED: if (v != p.a) return false; break;
+      case BEEP_PRIOR_UNIFORM: if (v < p.a || v > p.b) return false; break;
+      case BEEP_PRIOR_EXP: case BEEP_PRIOR_DIRFLAT: if (v < 0) return false; break;
+      default: break;
+    }
+  }
+  return true;
+}
+
+}  // namespace beephost

@@ -1,0 +1,87 @@
+// Host-side model: the normalised topology and the per-parameter derived
+// tables that feed the device kernels.  This is the C++ host's counterpart of
+// the reference's Data / Model set-up (src/data_matrix.cc, src/data.cc) and
+// of State::set_param (src/state.cc:66-91), written for flat arrays.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/beep_gpu.h"
+
+namespace beephost {
+
+struct Compartment { int inf_param; std::vector<int> mean_param; int shape; std::vector<int> exits; };
+struct Transition { int from, to, infection; std::vector<int> prob_param; };
+struct Spline { std::vector<double> t, multfac; std::vector<int> param; int factor_param; };
+struct Dirichlet { std::vector<int> param; std::vector<double> frac; };
+struct Covar { int param; std::vector<double> value; /* [A][period], centred */ };
+
+// One observation flattened to element indices into an instance's device
+// trajectories: bit 31 clear -> transnum[s][tr][cell] (index s*T*AG + tr*AG + cell),
+// bit 31 set -> pop[s][co][cell] (index s*C*AG + co*AG + cell).
+struct Obs {
+  double y, sd, shape, factor, thresh;
+  int type, datatable, sett_i, sett_f;
+  std::vector<uint32_t> elem;
+};
+
+// Tables State::set_param derives from one parameter vector (FP64).
+struct Tables {
+  std::vector<double> rate;     // [T][G]
+  std::vector<double> sus;      // [G]
+  std::vector<double> inf;      // [C]
+  std::vector<double> Ntime;    // [G][G] (no age_matrix_perturb: constant in time)
+  std::vector<double> beta_af;  // [S][A] beta * areafactor(day of s)
+  std::vector<double> efoi;     // [S][G] efoi(t_s) / popsize * agedist
+  std::vector<double> geo;      // [S]    m(t_s)
+};
+
+class HostModel {
+ public:
+  // Returns an empty string on success, else the error message.
+  std::string build(const beep_spec& spec);
+
+  int A = 0, G = 0, C = 0, T = 0, S = 0, AG = 0, period = 0, spd = 0, inf_tr = -1, nparam = 0;
+  double dt = 1.0;
+  double popsize = 0;
+
+  std::vector<Compartment> comp;
+  std::vector<Transition> trans;
+  std::vector<beep_prior> prior;
+  std::vector<Spline> spline;
+  std::vector<Dirichlet> dirichlet;
+  std::vector<Covar> covar;
+  std::vector<int> sus_param;
+  int R_spline = -1, Rfactor_param = -1, efoi_spline = -1, geo_spline = -1;
+  std::vector<double> efoi_agedist;
+  std::vector<double> Rdist;          // [G] population fraction of the R-spline area group
+
+  // Data::geo_normalise (src/data_matrix.cc:55-96), reference orientation ...
+  std::vector<double> M_diag, M_val, M_factor;
+  std::vector<int> M_rowptr, M_to;
+  // ... and its transpose, the gather form the kernels use: Imap[j] = sum over
+  // incoming (src, w) of w * dinf[src].
+  std::vector<int> in_ptr, in_src;
+  std::vector<double> in_w;
+  std::vector<double> N;              // [G][G] normalised age mixing
+  std::vector<int> pop_init;          // [A][C][G]
+  std::vector<double> area_pop;       // [A]
+  std::vector<int> inf_state;         // infected compartments (Model::set_infected_uninfected)
+
+  std::vector<Obs> obs;
+  int ndatatable = 1;
+  std::vector<int> section_ti, section_tf;
+  std::vector<std::vector<int>> section_obs;
+
+  std::vector<double> dirichlet_correct(const double* paramv) const;
+  // Returns 0 or a BEEP_* error code.
+  int derive(const double* paramv, Tables& out, std::string& err) const;
+  double prior_logpdf(const double* paramv) const;
+  bool inbounds(const double* paramv) const;
+
+ private:
+  std::vector<double> disc_spline(int sp, const std::vector<double>& pdir) const;
+};
+
+}  // namespace beephost

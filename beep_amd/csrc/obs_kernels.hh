@@ -1,0 +1,187 @@
+// Observation log-likelihood, fixed-state Poisson log-likelihood and the
+// Metropolis-Hastings acceptance kernels.  Values are accumulated in FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_rng.hh"
+
+namespace beepdev {
+
+struct ObsDev {
+  int nobs, ndt;
+  uint32_t TAG, CAG;         // T*AG and C*AG of one step
+  const int* ptr;            // [nobs+1] element ranges
+  const uint32_t* elem;      // bit 31 clear: transnum index s*TAG + tr*AG + cell; set: pop index
+  const int* type;           // BEEP_OBS_*
+  const int* dt;             // data table
+  const double* y;
+  const double* sd;
+  const double* shape;
+  const double* factor;
+  const double* thresh;
+};
+
+__device__ inline double poisson_lpmf(double k, double lam) {          // src/utils.cc:157-170
+  if (lam == 0) return k == 0 ? 0.0 : -1e9;
+  if (k < 0) return -1e9;
+  if (k == 0) return -lam;
+  return k * log(lam) - lam - lgamma(k + 1.0);
+}
+
+__device__ inline double negbin_lpmf(double k, double m, double shape) {   // src/utils.cc:124-127
+  return lgamma(shape + k) - lgamma(k + 1.0) - lgamma(shape) + shape * log(shape / (shape + m)) +
+         k * log(m / (shape + m));
+}
+
+__device__ inline double normal_lpdf(double x, double mean, double var) {  // src/utils.cc:107-111
+  return -0.5 * log(6.283185307179586476925 * var) - (x - mean) * (x - mean) / (2 * var);
+}
+
+// ObservationModel::obs_prob (src/obsmodel.cc:245-305).
+__device__ inline double obs_prob(double value, double y, double sd, int type, double shape, double thresh) {
+  if (value < 0) value = 0;
+  if (y == 999999997.0) return 0.0;
+  if (y == 999999998.0) {
+    if (type <= 1) return value < thresh ? normal_lpdf(thresh, thresh, sd * sd) : normal_lpdf(thresh, value, sd * sd);
+    double lam = value < 1 ? 1 : value, sum = 0;
+    for (double k = 0; k <= thresh; k += 1.0)
+      sum += exp(type == 2 ? poisson_lpmf(k, lam) : negbin_lpmf(k, lam, shape));
+    return log(sum);
+  }
+  if (type <= 1) return normal_lpdf(y, value, sd * sd);
+  double k = floor(y + 0.5), lam = value < 1 ? 1 : value;
+  if (type == 2) return poisson_lpmf(k, lam);
+  return negbin_lpmf(k, lam, shape);
+}
+
+__device__ inline double obs_value(const ObsDev& od, int i, const int* tn, const int* pt) {
+  double v = 0, f = od.factor[i];
+  for (int e = od.ptr[i]; e < od.ptr[i + 1]; e++) {
+    uint32_t x = od.elem[e];
+    int num = (x & 0x80000000u) ? (pt ? pt[x & 0x7FFFFFFFu] : 0) : tn[x];
+    if (num != 0) v += f * (double)num;
+  }
+  return v;
+}
+
+__device__ inline double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double t = 0;
+  int nw = (blockDim.x + 63) >> 6;
+  for (int k = 0; k < nw; k++) t += red[k];
+  return t;
+}
+
+// One workgroup per instance.  out[b*(1+ndt)] = log-likelihood,
+// out[b*(1+ndt)+1+d] = -2 * (sum over data table d)  (get_EF_datatable).
+__global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ odp, const int* __restrict__ inst,
+                                                    const int* __restrict__ tn, const int* __restrict__ pt,
+                                                    size_t traj_sz, size_t pop_traj_sz, const int* __restrict__ sel,
+                                                    int nsel, double* __restrict__ out, double* __restrict__ EF,
+                                                    int store_EF) {
+  __shared__ double red[8];
+  const ObsDev od = *odp;
+  const int b = blockIdx.x;
+  const int in = inst[b];
+  const int* tnb = tn + (size_t)in * traj_sz;
+  const int* ptb = pt ? pt + (size_t)in * pop_traj_sz : nullptr;
+  double L = 0, acc[8];
+#pragma unroll
+  for (int d = 0; d < 8; d++) acc[d] = 0;
+  for (int k = threadIdx.x; k < nsel; k += blockDim.x) {
+    int i = sel ? sel[k] : k;
+    double v = obs_value(od, i, tnb, ptb);
+    double lp = obs_prob(v, od.y[i], od.sd[i], od.type[i], od.shape[i], od.thresh[i]);
+    L += lp;
+    int dt = od.dt[i];
+#pragma unroll
+    for (int d = 0; d < 8; d++)
+      if (d == dt) acc[d] += lp;
+  }
+  double Lt = block_sum(L, red);
+  double* o = out + (size_t)b * (1 + od.ndt);
+  if (threadIdx.x == 0) {
+    o[0] = Lt;
+    if (store_EF) EF[in] = -2.0 * Lt;
+  }
+  for (int d = 0; d < od.ndt; d++) {
+    double t = block_sum(acc[d < 8 ? d : 0], red);
+    if (threadIdx.x == 0) o[1 + d] = -2.0 * t;
+  }
+}
+
+__global__ void k_obs_values(const ObsDev* __restrict__ odp, const int* __restrict__ tn, const int* __restrict__ pt,
+                             double* __restrict__ out) {
+  const ObsDev od = *odp;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < od.nobs) out[i] = obs_value(od, i, tn, pt);
+}
+
+// State::likelihood (src/state.cc:812-826): one workgroup per instance.
+__global__ void __launch_bounds__(256) k_pois_loglik(const int* __restrict__ inst, const int* __restrict__ tn,
+                                                     const float* __restrict__ tm, size_t traj_sz,
+                                                     double* __restrict__ out) {
+  __shared__ double red[8];
+  const int in = inst[blockIdx.x];
+  const int* n = tn + (size_t)in * traj_sz;
+  const float* l = tm + (size_t)in * traj_sz;
+  double L = 0;
+  for (size_t e = threadIdx.x; e < traj_sz; e += blockDim.x) L += poisson_lpmf((double)n[e], (double)l[e]);
+  double t = block_sum(L, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// Mbp::get_al + MVN::MH (src/mbp.cc:334-361, src/mvn.cc:233-249), FP64.
+__global__ void k_accept(int n, const int* __restrict__ init, const int* __restrict__ prop, const double* __restrict__ EF,
+                         const double* __restrict__ dlogq, const uint32_t* __restrict__ key, int mode, double invT,
+                         double EFcut, uint64_t seed, uint32_t stream, double* __restrict__ al_out,
+                         uint8_t* __restrict__ acc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double EFp = EF[prop[i]], EFi = EF[init[i]], al = 0;
+  if (mode == 0) {
+    if (EFp < EFcut) al = exp(dlogq[i]);
+  } else {
+    al = exp(-invT * 0.5 * (EFp - EFi) + dlogq[i]);
+  }
+  if (al > 1) al = 1;
+  u4 blk = philox4x32_10(u4{0xFFFFu | (stream << 16), 0xFFFFFFFFu, 0x7FFFFFFFu, key[i]}, (uint32_t)seed,
+                         (uint32_t)(seed >> 32));
+  double u = u01d(blk.x);
+  al_out[i] = al;
+  acc[i] = u < al ? 1 : 0;
+}
+
+// ---- diagnostics: raw generator and sampler outputs for the tests ----
+
+__global__ void k_dbg_philox(int n, const u4* __restrict__ ctr, uint32_t k0, uint32_t k1, u4* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = philox4x32_10(ctr[i], k0, k1);
+}
+
+// Draw i uses the counter of transition 0 of cell i at step 0 of instance key.
+__global__ void k_dbg_poisson(int n, const float* __restrict__ lam, uint64_t seed, uint32_t key, int* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  u4 blk = philox4x32_10(u4{0u, (uint32_t)i, 0u, key}, k0, k1);
+  ExtraUniforms xu{k0, k1, 0u, (uint32_t)i, key, 0u, 0u, 4, u4{0, 0, 0, 0}};
+  out[i] = lam[i] > 0.f ? poisson_draw(lam[i], u01f(blk.x), u01d(blk.x), xu) : 0;
+}
+
+__global__ void k_dbg_binomial(int n, const int* __restrict__ nn, const double* __restrict__ p, uint64_t seed,
+                               uint32_t key, int* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  u4 blk = philox4x32_10(u4{0u, (uint32_t)i, 0u, key}, k0, k1);
+  ExtraUniforms xu{k0, k1, 0u, (uint32_t)i, key, 0u, 0u, 4, u4{0, 0, 0, 0}};
+  out[i] = binomial_draw(nn[i], p[i], u01d(blk.x), xu);
+}
+
+}  // namespace beepdev

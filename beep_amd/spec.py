@@ -1,0 +1,107 @@
+"""Model specification: the content of Data / Model once BEEP's TOML is read.
+
+A spec is a plain dict (JSON-serialisable once arrays are lists):
+
+  A, G          areas, age groups (dp == age group, one strain)
+  period, spd   days, steps per day (details.division_per_time)
+  comps         [{"name", "inf": param or -1, "mean": [G] params or None, "shape": int}]
+                after Erlang expansion (src/inputs_comps.cc:33-46)
+  trans         [{"from", "to", "infection": bool, "prob": [G] params or None}]
+                internal Erlang transitions first (src/inputs_comps.cc:77-88)
+  params        [{"name", "prior": ("fixed", v) | ("uniform", lo, hi) | ("exp", rate)
+                  | ("normal", mu, sd) | ("dirflat",)}]
+  splines       [{"t": [...], "param": [...], "multfac": [...], "factor": param or -1}]
+  R_spline, Rfactor, efoi_spline, efoi_agedist, geo_spline
+  N             [G][G] age contact table or None (unit matrix for G == 1)
+  M             {"diag": [A], "rowptr": [A+1], "to": [nnz], "val": [nnz]} raw, or None for A == 1
+  pop           [A][C][G] initial populations
+  covars        [{"param", "value": [A][period]}] (centred on load)
+  sus           [G] params or None
+  dirichlet     [{"param": [...], "frac": [...]}]
+  obs           [{"y", "sd", "type", "shape", "factor", "thresh", "percent", "datatable",
+                  "graph", "sett_i", "sett_f", "area": [...], "dp": [...], "tr": [...], "co": [...]}]
+  sections      [(ti, tf), ...] PMCMC observation sections
+
+`param_index(spec, name)` finds parameters by name.
+"""
+import numpy as np
+
+PRIOR_CODES = {"fixed": 0, "uniform": 1, "exp": 2, "normal": 3, "dirflat": 4}
+OBS_CODES = {"normal": 0, "normal_percent": 1, "poisson": 2, "negbin": 3}
+OBS_UNKNOWN = 999999997.0
+OBS_THRESH = 999999998.0
+
+
+def param_index(spec, name):
+    for i, p in enumerate(spec["params"]):
+        if p["name"] == name:
+            return i
+    raise KeyError(name)
+
+
+def nsteps(spec):
+    return spec["period"] * spec["spd"]
+
+
+def dims(spec):
+    return dict(A=spec["A"], G=spec["G"], C=len(spec["comps"]), T=len(spec["trans"]), S=nsteps(spec))
+
+
+def sample_prior(spec, rng, n):
+    """Draws n parameter vectors from the priors (uniform / fixed / exp / normal;
+    flat-Dirichlet weights as Gamma(1,1)), like Model::sample_from_prior."""
+    P = len(spec["params"])
+    out = np.empty((n, P))
+    for i, p in enumerate(spec["params"]):
+        pr = p["prior"]
+        kind = pr[0]
+        if kind == "fixed":
+            out[:, i] = pr[1]
+        elif kind == "uniform":
+            out[:, i] = rng.uniform(pr[1], pr[2], n)
+        elif kind == "exp":
+            out[:, i] = rng.exponential(1.0 / pr[1], n)
+        elif kind == "normal":
+            out[:, i] = rng.normal(pr[1], pr[2], n)
+        elif kind == "dirflat":
+            out[:, i] = rng.gamma(1.0, 1.0, n)
+        else:
+            raise ValueError(kind)
+    return out
+
+
+def prior_mid(spec):
+    """A parameter vector at the middle of each prior (fixed values kept)."""
+    v = []
+    for p in spec["params"]:
+        pr = p["prior"]
+        if pr[0] == "fixed":
+            v.append(pr[1])
+        elif pr[0] == "uniform":
+            v.append(0.5 * (pr[1] + pr[2]))
+        elif pr[0] == "exp":
+            v.append(1.0 / pr[1])
+        elif pr[0] == "normal":
+            v.append(pr[1])
+        else:
+            v.append(1.0)
+    return np.array(v, dtype=np.float64)
+
+
+def obs_elements(spec):
+    """Per observation, the (kind, s, area, x, dp) elements it sums in the
+    reference's accumulation order (src/obsmodel.cc:74-109): kind 0 = transnum
+    (x = transition), kind 1 = pop (x = compartment)."""
+    out = []
+    for o in spec.get("obs", []):
+        el = []
+        for s in range(o["sett_i"], o["sett_f"]):
+            for c in o["area"]:
+                for tr in o.get("tr", []):
+                    for dp in o["dp"]:
+                        el.append((0, s, c, tr, dp))
+                for co in o.get("co", []):
+                    for dp in o["dp"]:
+                        el.append((1, s, c, co, dp))
+        out.append(el)
+    return out

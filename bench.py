@@ -1,0 +1,144 @@
+"""Throughput of the batched simulation + observation log-likelihood path.
+
+Workload (one "step"): 4096 ABC-SMC particles of the 100-area x 4-age-group
+spatial model (8 compartments, 10-nearest-neighbour mixing, 10-knot R(t)
+spline), each simulated over 200 daily tau-leap steps and scored by the
+observation model; parameters drawn from the uniform priors.
+Metric: particle-timesteps/s = particles x daily steps / wall time, inputs
+resident in HBM.  Also reported: the plain path (one instance per launch per
+step) on a sample, the fused kernel's roofline, and the FP64 CPU oracle on
+this host's cores as a baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PARTICLES = 4096
+
+
+def algorithmic_bytes_per_particle_step(A, G, C, T):
+    """HBM bytes the fused kernel must move per particle per step with the
+    State trajectory kept: transnum (int32) + transmean (fp32) per (transition,
+    cell), pop (int32) per (compartment, cell), and the per-step tables it
+    reads (beta*areafactor per area, efoi per age group, m(t))."""
+    AG = A * G
+    return 4 * AG * (2 * T + C) + 4 * A + 4 * G + 4
+
+
+def cpu_baseline(spec, params, seconds=12.0):
+    """FP64 CPU oracle (libstdc++ samplers), one core, over a bounded sample
+    of the same particles."""
+    from oracle import oracle as O
+    orc = O.Oracle(spec)
+    S = orc.S
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds and n < len(params):
+        tb = orc.tables(params[n])
+        st = orc.simulate(tb, seed=1000 + n)
+        orc.obs_loglik(st)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n * S / dt, "unit": "particle-timesteps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} of the {PARTICLES} particles (same priors), {S} steps each, single thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=PARTICLES)
+    ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path shards by particle, "
+              f"see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from beep_amd import _lib, configs, spec as specmod
+
+    spec, _truth = configs.spatial_abcsmc()
+    P = args.particles
+    rng = np.random.default_rng(20250102)
+    params = specmod.sample_prior(spec, rng, P)
+    ctx = _lib.Context(spec, capacity=P, nslot=P, keep=_lib.KEEP_ALL)
+    slots = np.arange(P, dtype=np.int32)
+    ctx.set_param(slots, params)
+    inst = np.arange(P, dtype=np.int32)
+    keys = np.arange(P, dtype=np.uint32)
+    S, A, G, C, T = ctx.S, ctx.A, ctx.G, ctx.C, ctx.T
+
+    def step(k):
+        ctx.simulate(inst, slots, keys, 0, S, seed=20250102, stream=k & 0xFFFF)
+        L = ctx.obs_loglik(inst)
+        return ctx.timing(), L
+
+    for w in range(args.warmup):
+        step(10000 + w)
+    ctx.synchronize()
+    sim_ms, obs_ms = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        tm, L = step(k)
+        sim_ms.append(tm[0])
+        obs_ms.append(tm[1])
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    assert np.isfinite(L).all()
+    pts = args.steps * P * S
+    value = pts / wall
+
+    # Plain path: one instance per launch per step, on a sample of particles.
+    ns = args.plain_sample
+    t1 = time.perf_counter()
+    ctx.simulate(inst[:ns], slots[:ns], keys[:ns], 0, S, seed=20250102, stream=777, plain=True)
+    for i in range(ns):
+        ctx.obs_loglik(inst[i:i + 1])
+    ctx.synchronize()
+    plain_wall = time.perf_counter() - t1
+    plain_value = ns * S / plain_wall
+
+    sim_avg = float(np.mean(sim_ms))
+    bytes_per = algorithmic_bytes_per_particle_step(A, G, C, T)
+    achieved = bytes_per * P * S / (sim_avg * 1e-3) / 1e9
+    out = {
+        "metric": "particle-timesteps/sec (batched, simulation + observation log-likelihood)",
+        "value": value,
+        "unit": "particle-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 counts, fp32 rates, fp64 log-likelihood",
+        "data": "synthetic (uniform-prior parameters, synthetic observations)",
+        "config": {"workload": "spatial ABC-SMC: 100 areas x 4 age groups, 8 compartments, 10-NN mixing, "
+                               "10-knot Rt spline, 200 days, 4096 particles",
+                   "particles": P, "days": S, "areas": A, "age_groups": G},
+        "plain": {"value": plain_value, "sample_instances": ns, "batched_over_plain": value / plain_value},
+        "kernels_ms": {"simulate_fused": sim_avg, "obs_loglik": float(np.mean(obs_ms))},
+        "roofline": {"bound": "hbm", "kernel": "k_fused<MODE_SIM,8>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_particle_step": bytes_per},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(spec, params)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

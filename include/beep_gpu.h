@@ -1,0 +1,273 @@
+/*
+ * beep_gpu.h -- C ABI of the MI355X (gfx950) implementation of BEEP's
+ * stochastic simulation and likelihood hot path.
+ *
+ * The reference (ScottishCovidResponse/BEEP) has no FFI: its boundary is the
+ * C++ class API of State / ObservationModel / Mbp (src/state.hh:11-133,
+ * src/obsmodel.hh:8-39, src/mbp.hh:15-82) fed by the immutable Details /
+ * Data / Model objects built from the TOML spec.  This header is the thin
+ * C layer those classes' replacements call (beep_amd/csrc/host/gpu_state.hh
+ * mirrors the C++ signatures on top of it).  Every entry point below names
+ * the reference method whose arithmetic it carries.
+ *
+ * Conventions
+ *  - All functions return BEEP_OK (0) or a negative error code; the message
+ *    of the last error on a context is beep_last_error().
+ *  - Host arrays are copied; the context owns every device buffer.
+ *  - One context per host thread; calls on a context are queued on its own
+ *    HIP stream and synchronise only where a result comes back to the host.
+ *  - Index order of host-visible arrays is the reference's own:
+ *      pop[s][area][comp][dp], transnum/transmean[s][area][trans][dp]
+ *    (src/state.hh:24-28), one strain, dp == age group.
+ *  - Counts are int32, rates FP32 on the device; log-likelihood sums and
+ *    Metropolis-Hastings ratios are FP64.
+ */
+#ifndef BEEP_GPU_H
+#define BEEP_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (the reference exits through emsg/emsgEC, src/utils.cc:296-316) ---- */
+#define BEEP_OK        0
+#define BEEP_EINVAL   (-1)  /* bad argument or spec (emsg)                                 */
+#define BEEP_ENOMEM   (-2)  /* allocation failed or over the HBM budget                    */
+#define BEEP_EHIP     (-3)  /* HIP runtime error                                            */
+#define BEEP_ENAN     (-4)  /* NaN transition mean (emsgEC("State",4), src/state.cc:429)    */
+#define BEEP_EBOUNDS  (-5)  /* parameters outside the prior (Mbp::mbp FAIL, src/mbp.cc:254) */
+#define BEEP_ELARGE   (-6)  /* transition mean > 1e9 (emsgEC("Utils",6), src/utils.cc:150)  */
+
+/* ---- model specification: what Data / Model hold once the TOML is read ---- */
+
+/* Prior types of Model::prior (src/model.cc:1590-1659) and Model::inbounds (2625-2669). */
+#define BEEP_PRIOR_FIXED    0   /* a = value                         */
+#define BEEP_PRIOR_UNIFORM  1   /* a = min, b = max                  */
+#define BEEP_PRIOR_EXP      2   /* a = rate                          */
+#define BEEP_PRIOR_NORMAL   3   /* a = mean, b = sd                  */
+#define BEEP_PRIOR_DIRFLAT  4   /* flat Dirichlet weight, Pr = -value */
+
+typedef struct { int32_t type; double a, b; } beep_prior;
+
+/* Compartment after Erlang expansion (src/inputs_comps.cc:33-46). */
+typedef struct {
+  int32_t inf_param;            /* parameter giving the infectivity; -1 = zero       */
+  const int32_t* mean_param;    /* [G] parameter of the mean residence time; NULL if
+                                   the compartment has no non-infection exit          */
+  int32_t shape;                /* Erlang shape (transrate = shape / mean)           */
+} beep_comp;
+
+/* Transition (src/struct.hh:111-121); internal Erlang transitions first. */
+typedef struct {
+  int32_t from, to;
+  int32_t infection;            /* 1 for the single infection transition             */
+  const int32_t* prob_param;    /* [G] branching weight parameters, NULL = single exit */
+} beep_trans;
+
+/* Piecewise-linear spline (Model::create_disc_spline, src/model.cc:1020-1053). */
+typedef struct {
+  int32_t nknot;
+  const double* t;              /* knot times (days from start)                      */
+  const int32_t* param;         /* [nknot] parameter at each knot                    */
+  const double* multfac;        /* [nknot] multiplying factor at each knot           */
+  int32_t factor_param;         /* overall factor parameter; -1 = 1                  */
+} beep_spline;
+
+/* Dirichlet group normalised by Model::dirichlet_correct (src/model.cc:2685-2695). */
+typedef struct { int32_t n; const int32_t* param; const double* frac; } beep_dirichlet;
+
+/* Linear covariate on the area factor (Model::create_areafactor, src/model.cc:1182-1235);
+ * values are centred on load as Data::read_covars does (src/data.cc:557-565). */
+typedef struct { int32_t param; const double* value; /* [A][period], after any log transform */ } beep_covar;
+
+/* Observation model functions (src/consts.hh ObsModelFunc). */
+#define BEEP_OBS_NORMAL          0  /* sd given                                            */
+#define BEEP_OBS_NORMAL_PERCENT  1  /* sd = (y(1-eps) + max_graph*eps)*percent/100          */
+#define BEEP_OBS_POISSON         2
+#define BEEP_OBS_NEGBIN          3  /* shape                                               */
+#define BEEP_OBS_UNKNOWN   999999997.0  /* y value meaning "unknown" (contributes 0)       */
+#define BEEP_OBS_THRESH    999999998.0  /* y value meaning "below threshold"               */
+
+/* One observation (src/struct.hh:340-354): sums factor * transnum over steps
+ * [sett_i, sett_f), the listed areas, dps and transitions, plus factor * pop
+ * over the listed compartments (src/obsmodel.cc:74-109). */
+typedef struct {
+  double y, sd, shape, factor, thresh;
+  double percent;               /* BEEP_OBS_NORMAL_PERCENT only                      */
+  int32_t type, datatable, graph;
+  int32_t sett_i, sett_f;
+  int32_t narea; const int32_t* area;
+  int32_t ndp;   const int32_t* dp;
+  int32_t ntr;   const int32_t* tr;
+  int32_t nco;   const int32_t* co;
+} beep_obs;
+
+typedef struct {
+  int32_t A, G, C, T;           /* areas, age groups, compartments, transitions      */
+  int32_t period;               /* days                                               */
+  int32_t steps_per_day;        /* details.division_per_time (steps_per_unit_time)   */
+  int32_t nparam;
+  const beep_prior* prior;      /* [nparam]                                          */
+  const beep_comp* comp;        /* [C]                                               */
+  const beep_trans* trans;      /* [T]                                               */
+  int32_t nspline;
+  const beep_spline* spline;
+  int32_t R_spline;             /* spline giving R(t)                                */
+  int32_t Rfactor_param;        /* strain Rfactor, -1 = 1                             */
+  int32_t efoi_spline;          /* external force of infection, -1 = none             */
+  const double* efoi_agedist;   /* [G]                                               */
+  int32_t geo_spline;           /* m(t) of geo_mixing_modify, -1 = unity              */
+  const double* N_raw;          /* [G][G] age contact table (Data::age_mixing_matrix
+                                   input), NULL = unit matrix for G == 1              */
+  const double* M_diag;         /* [A]   raw geographic mixing (before geo_normalise) */
+  const int32_t* M_rowptr;      /* [A+1] row c lists off-diagonal destinations        */
+  const int32_t* M_to;          /* [nnz]                                             */
+  const double* M_val;          /* [nnz]                                             */
+  const int32_t* pop_init;      /* [A][C][G] initial populations                     */
+  int32_t ncovar;
+  const beep_covar* covar;
+  const int32_t* sus_param;     /* [G] susceptibility parameters, NULL = 1            */
+  int32_t ndirichlet;
+  const beep_dirichlet* dirichlet;
+  int32_t nobs;
+  const beep_obs* obs;
+  int32_t nsection;             /* PMCMC observation sections (0 = none); section k */
+  const int32_t* section_ti;    /* covers steps [section_ti[k], section_tf[k])        */
+  const int32_t* section_tf;    /* (ObservationModel::split_observations, 326-348)    */
+} beep_spec;
+
+/* Host only (no device is touched): State::set_param's FP64 tables for one
+ * parameter vector of `spec`, in the layout of beep_get_tables, plus
+ * Model::prior and Model::inbounds.  Any output pointer may be NULL. */
+int beep_spec_tables(const beep_spec* spec, const double* paramval, double* rate, double* sus, double* inf,
+                     double* Ntime, double* beta_af, double* efoi, double* geo, double* Pr, int32_t* inbounds,
+                     char* err, int errlen);
+
+/* ---- context ---- */
+typedef struct beep_ctx beep_ctx;
+
+/* Which trajectories an instance keeps in HBM (State holds all three,
+ * src/state.hh:24-28; MBPs need transnum and transmean). */
+#define BEEP_KEEP_TRANSNUM   1
+#define BEEP_KEEP_TRANSMEAN  2
+#define BEEP_KEEP_POP        4
+#define BEEP_KEEP_ALL        7
+
+/* Builds Data::generate_matrices' normalised mixing (src/data_matrix.cc:14-202),
+ * the observation index (ObservationModel::initialise_obs_change, obsmodel.cc:174-217)
+ * and device buffers for `capacity` instances and `nslot` parameter slots.
+ * hbm_budget (bytes, 0 = unlimited) bounds the allocation (BEEP_ENOMEM above it). */
+int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capacity, int nslot,
+                int keep, beep_ctx** out);
+int beep_destroy(beep_ctx* ctx);
+const char* beep_last_error(const beep_ctx* ctx);
+/* Sizes: A, G, C, T, S (steps), nparam, nobs, ndatatable, capacity, nslot. */
+int beep_get_dims(const beep_ctx* ctx, int32_t* dims10);
+size_t beep_device_bytes(const beep_ctx* ctx);
+
+/* State::set_param (src/state.cc:66-91): for each i, derives the tables of
+ * parameter vector paramval[i*nparam..] (Model::dirichlet_correct,
+ * create_transrate, create_disc_spline, create_susceptibility,
+ * create_areafactor, create_Ntime, calculate_beta_from_R) into table slot
+ * slot[i].  Pr_out[i] (may be NULL) receives Model::prior; inb_out[i] (may be
+ * NULL) receives Model::inbounds (1/0).  Slots whose parameters are outside
+ * the prior are still derived when the derivation is defined. */
+int beep_set_param(beep_ctx* ctx, int n, const int32_t* slot, const double* paramval,
+                   double* Pr_out, int32_t* inb_out);
+
+/* Copies the FP64 tables derived for a slot (tests): rate[T][G], sus[G], inf[C],
+ * Ntime[G][G], beta_af[S][A] (beta * areafactor), efoi[S][G], geo[S]. */
+int beep_get_tables(beep_ctx* ctx, int slot, double* rate, double* sus, double* inf, double* Ntime,
+                    double* beta_af, double* efoi, double* geo);
+
+/* ---- hot path ---- */
+
+/* Simulation flags.  (The reference's deterministic dynamics keep fractional
+ * means as transition counts, src/state.cc:627; the int32 device state does not
+ * represent them, so only stochastic dynamics run here.) */
+#define BEEP_SIM_PLAIN          2   /* one launch per instance per step (the unbatched
+                                       path); default is one persistent launch per batch */
+
+/* State::simulate(ti, tf) (src/state.cc:596-638) for n instances at once:
+ * instance inst[i] uses parameter slot slot[i] and Philox key key[i]; when
+ * t0 == 0 the state starts from the spec's initial populations
+ * (State::pop_init), otherwise from the instance's current state (pop at t0).
+ * Draws are keyed by (seed, stream, key, step, area*G+dp, transition), never by
+ * the instance index, so a result does not depend on what else is batched. */
+int beep_simulate(beep_ctx* ctx, int n, const int32_t* inst, const int32_t* slot, const uint32_t* key,
+                  int t0, int t1, uint64_t seed, uint32_t stream, int flags);
+
+/* ObservationModel::calculate (src/obsmodel.cc:22-35) when section < 0, else
+ * calculate_section(section) (39-58); L_out[i] = log-likelihood (FP64).
+ * EF_dt_out (may be NULL) receives get_EF_datatable (309-322) as
+ * [n][ndatatable] (-2 * per-table sums). */
+int beep_obs_loglik(beep_ctx* ctx, int n, const int32_t* inst, int section, double* L_out,
+                    double* EF_dt_out);
+
+/* ObservationModel::get_obs_value (src/obsmodel.cc:62-70) for one instance: [nobs]. */
+int beep_obs_values(beep_ctx* ctx, int inst, double* value_out);
+
+/* State::likelihood (src/state.cc:812-826): sum over all cells of
+ * log Pois(transnum | transmean), FP64. */
+int beep_pois_loglik(beep_ctx* ctx, int n, const int32_t* inst, double* L_out);
+
+/* Mbp::mbp (src/mbp.cc:252-330) for n (initial, proposed) instance pairs:
+ * proposed instance prop[i] is re-simulated with parameter slot slot[i],
+ * coupled to initial instance init[i]: where the proposed mean rises,
+ * n_p = n_i + Pois(lp - li); where it falls, n_p = Bin(n_i, lp/li); equal
+ * means keep n_i; cells with simu_mask[i][s][area] != 0 (slicetime / fixed
+ * tree, src/mbp.cc:656-716) draw afresh.  simu_mask may be NULL.
+ * inf_update is accepted for interface parity: INF_UPDATE (0) and
+ * INF_DIF_UPDATE (1) give the same infectivity map, which the kernel
+ * recomputes from the proposed populations every step. */
+int beep_mbp(beep_ctx* ctx, int n, const int32_t* init, const int32_t* prop, const int32_t* slot,
+             const uint32_t* key, const uint8_t* simu_mask, uint64_t seed, uint32_t stream, int inf_update);
+
+/* Mbp::get_al + MVN::MH (src/mbp.cc:334-361, src/mvn.cc:233-249), FP64 on the
+ * device: mode 0 = CUTOFF (al = exp(dlogq) if EF_p < EFcut), 1 = INVT
+ * (al = exp(-invT/2 (EF_p - EF_i) + dlogq)); dlogq[i] = Pr_p - Pr_i plus any
+ * proposal-density term; EF values are those of the last beep_obs_loglik on
+ * each instance.  al_out[i] = min(al,1); accepted[i] = (u < al) with u a
+ * Philox uniform keyed by (seed, stream, key[i]). */
+int beep_accept(beep_ctx* ctx, int n, const int32_t* init, const int32_t* prop, int mode, double invT,
+                double EFcut, const double* dlogq, const uint32_t* key, uint64_t seed, uint32_t stream,
+                double* al_out, uint8_t* accepted);
+
+/* ---- state transfer (State::save / load, src/state.cc:709-808; PMCMC moves) ---- */
+int beep_get_transnum(beep_ctx* ctx, int inst, int32_t* out);   /* [S][A][T][G] */
+int beep_get_transmean(beep_ctx* ctx, int inst, float* out);    /* [S][A][T][G] */
+int beep_get_pop(beep_ctx* ctx, int inst, int32_t* out);        /* [S][A][C][G] */
+int beep_get_state(beep_ctx* ctx, int inst, int32_t* pop_out);  /* current [A][C][G] */
+int beep_put_state(beep_ctx* ctx, int inst, const int32_t* pop); /* current [A][C][G] */
+/* Loads a transition trajectory [S][A][T][G] and rebuilds pop and transmean
+ * with slot `slot` (State::initialise_from_particle, src/state.cc:294-322). */
+int beep_put_transnum(beep_ctx* ctx, int inst, int slot, const int32_t* transnum);
+/* Copies instance state src[i] -> dst[i] (current state and kept trajectories);
+ * the device form of the particle copies of src/mpi.cc:81-165,215-327. */
+int beep_copy_instances(beep_ctx* ctx, int n, const int32_t* dst, const int32_t* src);
+
+/* Waits for the context's stream. */
+int beep_synchronize(beep_ctx* ctx);
+
+/* Device time (ms, from HIP events on the context stream) of the kernels of
+ * the last call of each kind: out[0] simulate / MBP / replay kernel,
+ * out[1] observation log-likelihood kernel, out[2] Poisson log-likelihood
+ * kernel; out[3] the number of kernel launches those times cover. */
+int beep_timing(beep_ctx* ctx, double* out4);
+
+/* ---- diagnostics (tests of the device generator and samplers) ---- */
+/* Philox-4x32-10 of counters ctr[n][4] under key (k0,k1) -> out[n][4]. */
+int beep_debug_philox(int device, int n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out);
+/* out[i] ~ Poisson(lam[i]) / Binomial(nn[i], p[i]) drawn with the kernels' samplers
+ * from the stream of transition 0 of cell i, step 0, instance `key`. */
+int beep_debug_poisson(int device, int n, const float* lam, uint64_t seed, uint32_t key, int32_t* out);
+int beep_debug_binomial(int device, int n, const int32_t* nn, const double* p, uint64_t seed, uint32_t key,
+                        int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BEEP_GPU_H */

@@ -1,0 +1,262 @@
+"""TEST INFRASTRUCTURE ONLY: FP64 numpy restatement of the reference's
+set-up and State::set_param derivations, used to check the product's C++ host
+model and to feed the C++ oracle (beep_oracle.cc).
+
+Restates, for one strain with age as the only demographic category:
+  Data::geo_normalise               src/data_matrix.cc:55-96
+  Data::age_mixing_matrix           src/data_matrix.cc:138-202
+  Data::agematrix_normalise         src/data_matrix.cc:100-134
+  Data::read_covars (centring)      src/data.cc:557-565
+  Data::set_obs_sd                  src/data.cc:1589-1623
+  Model::set_infected_uninfected    src/model.cc:110-166
+  Model::dirichlet_correct          src/model.cc:2685-2695
+  Model::create_transrate           src/model.cc:1116-1140
+  Model::create_disc_spline         src/model.cc:1020-1053
+  Model::create_susceptibility      src/model.cc:1144-1178
+  Model::create_areafactor          src/model.cc:1182-1250
+  Model::create_Ntime               src/model.cc:1254-1301 (no matrix modifications)
+  Model::calculate_beta_from_R      src/model.cc:1318-1362 with calculate_Vinv,
+                                    calculate_F, calculate_NGM (2485-2576) and
+                                    largest_eigenvalue (src/matrix.cc:1139-1219)
+  Model::prior / inbounds           src/model.cc:1590-1659, 2625-2669
+
+Pins: beta-from-R against the SEIR closed form rho = mean infectious period
+and against numpy.linalg.eig of the same next-generation matrix; the
+normalisations against their defining identities (row sums of the per-capita
+mixing equal one; population-weighted age mixing averages one).
+"""
+import numpy as np
+
+EPSILON_FACTOR = 0.01
+VTINY = 1e-15
+
+
+def normalise(spec):
+    """Data-level quantities: normalised mixing, populations, infected states."""
+    A, G = spec["A"], spec["G"]
+    comps, trans = spec["comps"], spec["trans"]
+    C = len(comps)
+    pop = np.asarray(spec["pop"], dtype=np.float64).reshape(A, C, G)
+    area_pop = pop.sum(axis=(1, 2))
+    popsize = area_pop.sum()
+    pop_age = pop.sum(axis=(0, 1))
+    # Age mixing.
+    if spec.get("N") is not None:
+        raw = np.asarray(spec["N"], dtype=np.float64)
+        m = raw / (pop_age[:, None] / popsize)
+        N = 0.5 * (m + m.T)
+    else:
+        N = np.ones((1, 1))
+    s = sum(pop_age[aa] * N[aa, a] for a in range(G) for aa in range(G)) / (G * pop_age.sum())
+    N = N / s
+    # Geographic mixing.
+    M = spec.get("M")
+    if M is None:
+        diag, rowptr, to, val = np.ones(1), np.array([0, 0]), np.zeros(0, int), np.zeros(0)
+    else:
+        diag = np.array(M["diag"], dtype=np.float64)
+        rowptr, to, val = np.array(M["rowptr"]), np.array(M["to"]), np.array(M["val"], dtype=np.float64)
+    diag = diag.copy()
+    val = val.copy()
+    for c in range(A):
+        tot = diag[c] * area_pop[c] + sum(val[j] * area_pop[to[j]] for j in range(rowptr[c], rowptr[c + 1]))
+        diag[c] /= tot
+        val[rowptr[c]:rowptr[c + 1]] /= tot
+    factor = (1.0 / area_pop) / diag
+    # Infected states.
+    itr = [k for k, t in enumerate(trans) if t.get("infection")][0]
+    exits = [[k for k, t in enumerate(trans) if t["from"] == c] for c in range(C)]
+    fwd = [False] * C
+    lst = [trans[itr]["to"]]
+    fwd[lst[0]] = True
+    i = 0
+    while i < len(lst):
+        for k in exits[lst[i]]:
+            if trans[k].get("infection"):
+                continue
+            cc = trans[k]["to"]
+            if not fwd[cc] and cc != trans[itr]["from"]:
+                lst.append(cc)
+                fwd[cc] = True
+        i += 1
+    back = [False] * C
+    for co in range(C):
+        if comps[co].get("inf", -1) < 0:
+            continue
+        l2 = [co]
+        back[co] = True
+        i = 0
+        while i < len(l2):
+            for k, t in enumerate(trans):
+                if not t.get("infection") and t["to"] == l2[i] and not back[t["from"]]:
+                    back[t["from"]] = True
+                    l2.append(t["from"])
+            i += 1
+    inf_state = [c for c in range(C) if fwd[c] and back[c]]
+    covars = []
+    for cv in spec.get("covars", []):
+        v = np.asarray(cv["value"], dtype=np.float64).reshape(A, spec["period"])
+        covars.append((cv["param"], v - v.mean()))
+    return dict(N=N, diag=diag, rowptr=rowptr, to=to, val=val, factor=factor, area_pop=area_pop,
+                popsize=popsize, Rdist=pop_age / popsize, inf_state=inf_state, itr=itr, exits=exits, covars=covars)
+
+
+def dirichlet_correct(spec, pv):
+    pd = np.array(pv, dtype=np.float64)
+    for d in spec.get("dirichlet", []):
+        par = d["param"]
+        frac = d.get("frac", [1.0] * len(par))
+        tot = sum(pv[t] for t in par)
+        for t, f in zip(par, frac):
+            pd[t] = pv[t] / (f * tot)
+    return pd
+
+
+def disc_spline(spec, sp, pd):
+    S = spec["period"] * spec["spd"]
+    spl = spec["splines"][sp]
+    t_k, par, mf = spl["t"], spl["param"], spl.get("multfac", [1.0] * len(spl["t"]))
+    factor = pd[spl["factor"]] if spl.get("factor", -1) >= 0 else 1.0
+    out = np.zeros(S)
+    p = 0
+    for s in range(S):
+        t = (s + 0.5) * spec["period"] / S
+        while p + 1 < len(t_k) and t > t_k[p + 1]:
+            p += 1
+        if p + 1 == len(t_k):
+            out[s] = out[s - 1] if s > 0 else 0.0
+        else:
+            f = (t - t_k[p]) / (t_k[p + 1] - t_k[p])
+            out[s] = factor * (pd[par[p]] * mf[p] * (1 - f) + pd[par[p + 1]] * mf[p + 1] * f)
+    return out
+
+
+def largest_eigenvalue(M, rng=None):
+    """Power iteration with sum normalisation and the reference's convergence
+    test; the start vector is random like the reference's ran() warm start."""
+    n = M.shape[0]
+    rng = rng or np.random.default_rng(0)
+    vec = rng.uniform(1e-10, 1.0, n)
+    ev = 0.0
+    for loop in range(10001):
+        v2 = M @ vec
+        ev_new = v2.sum()
+        v2 = v2 / ev_new
+        if loop % 10 == 0:
+            lim = 100 * VTINY * (10 if loop > 100 else 1) * (10 if loop > 1000 else 1)
+            if abs(ev_new - ev) < lim and np.all(np.abs(v2 - vec) < lim):
+                return ev_new
+        vec, ev = v2, ev_new
+    raise RuntimeError("eigen-vector convergence problem")
+
+
+def ngm(spec, nz, tr_rate, sus, inf):
+    """Next-generation matrix of calculate_R_beta_ratio_using_NGM."""
+    G = spec["G"]
+    trans = spec["trans"]
+    ist = nz["inf_state"]
+    ref = {c: i for i, c in enumerate(ist)}
+    NV = len(ist) * G
+    V = np.zeros((NV, NV))
+    for k, t in enumerate(trans):
+        fi, ti = ref.get(t["from"]), ref.get(t["to"])
+        if fi is None:
+            continue
+        for g in range(G):
+            j = fi * G + g
+            V[j, j] += tr_rate[k, g]
+            if ti is not None:
+                V[ti * G + g, j] -= tr_rate[k, g]
+    Vinv = np.linalg.inv(V)
+    F = np.zeros((G, NV))
+    for si, co in enumerate(ist):
+        if inf[co] == 0:
+            continue
+        for dp in range(G):
+            for q in range(G):
+                F[q, si * G + dp] += nz["Rdist"][q] * sus[q] * inf[co] * nz["N"][q, dp]
+    return (F @ Vinv)[:, :G]
+
+
+def derive(spec, nz, pv):
+    """FP64 tables of State::set_param for parameter vector pv."""
+    A, G, period, spd = spec["A"], spec["G"], spec["period"], spec["spd"]
+    S = period * spd
+    comps, trans = spec["comps"], spec["trans"]
+    C, T = len(comps), len(trans)
+    pd = dirichlet_correct(spec, pv)
+    rate = np.zeros((T, G))
+    for k, t in enumerate(trans):
+        if t.get("infection"):
+            continue
+        c = comps[t["from"]]
+        shape = c.get("shape", 1)
+        for g in range(G):
+            r = 1.0 / (pd[c["mean"][g]] / shape)
+            if len(nz["exits"][t["from"]]) > 1:
+                r *= pd[t["prob"][g]]
+            rate[k, g] = r
+    sus = np.array([pd[p] if p >= 0 else 1.0 for p in spec["sus"]]) if spec.get("sus") is not None else np.ones(G)
+    inf = np.array([pd[c["inf"]] if c.get("inf", -1) >= 0 else 0.0 for c in comps])
+    af = np.ones((period, A))
+    for par, v in nz["covars"]:
+        af *= np.exp(pd[par] * v.T)
+    area_av = (af.mean(axis=0) * nz["area_pop"]).sum() / nz["popsize"]
+    af /= area_av
+    ratio = largest_eigenvalue(ngm(spec, nz, rate, sus, inf))
+    Rfac = pd[spec["Rfactor"]] if spec.get("Rfactor", -1) >= 0 else 1.0
+    beta = Rfac * disc_spline(spec, spec["R_spline"], pd) / ratio          # [S]
+    beta_SA = np.repeat(beta[:, None], A, axis=1)
+    if spec.get("efoi_spline", -1) >= 0:
+        efoi = disc_spline(spec, spec["efoi_spline"], pd) / nz["popsize"]
+    else:
+        efoi = np.zeros(S)
+    agedist = np.asarray(spec.get("efoi_agedist", [1.0 / G] * G), dtype=np.float64)
+    geo = disc_spline(spec, spec["geo_spline"], pd) if spec.get("geo_spline", -1) >= 0 else np.ones(S)
+    Ntime = np.repeat(nz["N"][None, :, :], S, axis=0)
+    return dict(rate=rate, sus=sus, inf=inf, af=af, beta=beta_SA, efoi=efoi, agedist=agedist, geo=geo,
+                Ntime=Ntime, ratio=ratio)
+
+
+def prior(spec, pv):
+    Pr = 0.0
+    for p, v in zip(spec["params"], pv):
+        pr = p["prior"]
+        if pr[0] == "uniform":
+            Pr += np.log(1.0 / (pr[2] - pr[1]))
+        elif pr[0] == "exp":
+            Pr += np.log(pr[1]) - pr[1] * v
+        elif pr[0] == "normal":
+            Pr += -0.5 * np.log(2 * np.pi * pr[2] ** 2) - (v - pr[1]) ** 2 / (2 * pr[2] ** 2)
+        elif pr[0] == "dirflat":
+            Pr += -v
+    return Pr
+
+
+def inbounds(spec, pv):
+    for p, v in zip(spec["params"], pv):
+        pr = p["prior"]
+        if pr[0] == "fixed" and v != pr[1]:
+            return False
+        if pr[0] == "uniform" and (v < pr[1] or v > pr[2]):
+            return False
+        if pr[0] in ("exp", "dirflat") and v < 0:
+            return False
+    return True
+
+
+def obs_sd(spec):
+    obs = spec.get("obs", [])
+    gmax = {}
+    for o in obs:
+        y = o["y"]
+        if y not in (999999997.0, 999999998.0):
+            gmax[o.get("graph", 0)] = max(gmax.get(o.get("graph", 0), 0.0), y)
+    sd = []
+    for o in obs:
+        if o["type"] == "normal_percent" and not (o.get("sd", 0) > 0):
+            sd.append((o["y"] * (1 - EPSILON_FACTOR) + gmax.get(o.get("graph", 0), 0.0) * EPSILON_FACTOR)
+                      * 0.01 * o["percent"])
+        else:
+            sd.append(o.get("sd", 0.0))
+    return np.array(sd)

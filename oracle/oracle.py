@@ -1,0 +1,270 @@
+"""TEST INFRASTRUCTURE ONLY: Python side of the FP64 CPU oracle.
+
+Wraps liboracle.so (beep_oracle.cc, the restatement of State / ObservationModel
+/ Mbp) and the numpy set-up restatement (derive.py).  Imported only by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg, as the checker.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from . import derive
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libbeepref_utils.so")
+
+
+def build(ref=False):
+    """Compiles liboracle.so (and, if asked and the reference tree is present,
+    the reference's utils.cc into _ref/)."""
+    targets = ["all"] + (["ref"] if ref and os.path.isdir("/root/reference/src") else [])
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.or_seed.argtypes = [C.c_uint]
+        for f in ("or_simulate",):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_void_p, dp, C.c_void_p, C.c_int, C.c_int]
+        L.or_replay.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_void_p]
+        L.or_likelihood.argtypes = [C.c_void_p, C.c_void_p]
+        L.or_likelihood.restype = C.c_double
+        L.or_obs_loglik.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, ip, C.c_int, dp]
+        L.or_obs_loglik.restype = C.c_double
+        L.or_obs_prob.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double]
+        L.or_obs_prob.restype = C.c_double
+        L.or_mbp.argtypes = [C.c_void_p, C.c_void_p, dp, C.c_void_p, C.c_void_p, C.POINTER(C.c_ubyte), C.c_int]
+        L.or_get_al.argtypes = [C.c_int] + [C.c_double] * 6
+        L.or_get_al.restype = C.c_double
+        L.or_poisson_stream.argtypes = [C.c_uint, C.c_double, C.c_int, ip]
+        L.or_binomial_stream.argtypes = [C.c_uint, C.c_double, C.c_uint, C.c_int, ip]
+        for f in ("or_normal_probability",):
+            getattr(L, f).argtypes = [C.c_double] * 3
+            getattr(L, f).restype = C.c_double
+        L.or_poisson_probability.argtypes = [C.c_int, C.c_double]
+        L.or_poisson_probability.restype = C.c_double
+        L.or_negative_binomial_probability.argtypes = [C.c_uint, C.c_double, C.c_double]
+        L.or_negative_binomial_probability.restype = C.c_double
+        L.or_update_pop.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        _lib = L
+    return _lib
+
+
+class _OrModel(C.Structure):
+    _fields_ = [("A", C.c_int), ("G", C.c_int), ("C", C.c_int), ("T", C.c_int), ("S", C.c_int), ("spd", C.c_int),
+                ("inf_tr", C.c_int), ("stochastic", C.c_int), ("dt", C.c_double),
+                ("from_", C.POINTER(C.c_int)), ("to", C.POINTER(C.c_int)), ("Mdiag", C.POINTER(C.c_double)),
+                ("Mptr", C.POINTER(C.c_int)), ("Mto", C.POINTER(C.c_int)), ("Mval", C.POINTER(C.c_double)),
+                ("Mfac", C.POINTER(C.c_double))]
+
+
+class _OrTables(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("inf", "rate", "sus", "beta", "af", "efoi", "agedist", "geo",
+                                                     "Ntime")]
+
+
+class _OrState(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("pop", "transnum", "transmean", "Imap", "Idiag")]
+
+
+class _OrObs(C.Structure):
+    _fields_ = [("nobs", C.c_int)] + [(n, C.POINTER(C.c_double)) for n in ("y", "sd")] + \
+               [("type", C.POINTER(C.c_int))] + [(n, C.POINTER(C.c_double)) for n in ("shape", "factor", "thresh")] + \
+               [(n, C.POINTER(C.c_int)) for n in ("el_ptr", "el_kind", "el_s", "el_c", "el_x", "el_dp")]
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+OBS_TYPES = {"normal": 0, "normal_percent": 1, "poisson": 2, "negbin": 3}
+
+
+class State:
+    """Trajectory arrays in the reference's order."""
+
+    def __init__(self, A, G, C_, T, S):
+        self.pop = np.zeros((S, A, C_, G))
+        self.transnum = np.zeros((S, A, T, G))
+        self.transmean = np.zeros((S, A, T, G))
+        self.Imap = np.zeros((S, A * G))
+        self.Idiag = np.zeros((S, A * G))
+        self._c = _OrState(_d(self.pop), _d(self.transnum), _d(self.transmean), _d(self.Imap), _d(self.Idiag))
+
+
+class Oracle:
+    def __init__(self, spec, stochastic=True):
+        self.spec = spec
+        self.nz = derive.normalise(spec)
+        A, G = spec["A"], spec["G"]
+        C_, T = len(spec["comps"]), len(spec["trans"])
+        S = spec["period"] * spec["spd"]
+        self.A, self.G, self.C, self.T, self.S = A, G, C_, T, S
+        nz = self.nz
+        self._from = np.array([t["from"] for t in spec["trans"]], dtype=np.int32)
+        self._to = np.array([t["to"] for t in spec["trans"]], dtype=np.int32)
+        self._diag = np.ascontiguousarray(nz["diag"], dtype=np.float64)
+        self._ptr = np.ascontiguousarray(nz["rowptr"], dtype=np.int32)
+        self._mto = np.ascontiguousarray(nz["to"], dtype=np.int32) if len(nz["to"]) else np.zeros(1, np.int32)
+        self._val = np.ascontiguousarray(nz["val"], dtype=np.float64) if len(nz["val"]) else np.zeros(1)
+        self._fac = np.ascontiguousarray(nz["factor"], dtype=np.float64)
+        self.model = _OrModel(A, G, C_, T, S, spec["spd"], nz["itr"], 1 if stochastic else 0,
+                              spec["period"] / S, _i(self._from), _i(self._to), _d(self._diag), _i(self._ptr),
+                              _i(self._mto), _d(self._val), _d(self._fac))
+        self.pop_init = np.ascontiguousarray(np.asarray(spec["pop"], dtype=np.float64).reshape(A, C_, G))
+        self._build_obs()
+
+    def _build_obs(self):
+        spec = self.spec
+        obs = spec.get("obs", [])
+        sd = derive.obs_sd(spec)
+        ptr, kind, s_, c_, x_, dp_ = [0], [], [], [], [], []
+        for o in obs:
+            for s in range(o["sett_i"], o["sett_f"]):
+                for c in o["area"]:
+                    for tr in o.get("tr", []):
+                        for dp in o["dp"]:
+                            kind.append(0); s_.append(s); c_.append(c); x_.append(tr); dp_.append(dp)
+                    for co in o.get("co", []):
+                        for dp in o["dp"]:
+                            kind.append(1); s_.append(s); c_.append(c); x_.append(co); dp_.append(dp)
+            ptr.append(len(kind))
+        self._obs_arrays = [
+            np.array([o["y"] for o in obs] or [0.0]), np.ascontiguousarray(sd if len(sd) else np.zeros(1)),
+            np.array([OBS_TYPES[o["type"]] if isinstance(o["type"], str) else o["type"] for o in obs] or [0],
+                     dtype=np.int32),
+            np.array([o.get("shape", 1.0) for o in obs] or [1.0]), np.array([o.get("factor", 1.0) for o in obs] or [1.0]),
+            np.array([o.get("thresh", 0.0) for o in obs] or [0.0]),
+            np.array(ptr, dtype=np.int32), np.array(kind or [0], dtype=np.int32), np.array(s_ or [0], dtype=np.int32),
+            np.array(c_ or [0], dtype=np.int32), np.array(x_ or [0], dtype=np.int32), np.array(dp_ or [0], dtype=np.int32),
+        ]
+        a = self._obs_arrays
+        self.obs = _OrObs(len(obs), _d(a[0]), _d(a[1]), _i(a[2]), _d(a[3]), _d(a[4]), _d(a[5]),
+                          _i(a[6]), _i(a[7]), _i(a[8]), _i(a[9]), _i(a[10]), _i(a[11]))
+        self.obs_sd = sd
+
+    def tables(self, pv):
+        """State::set_param (FP64) as the oracle's table struct (arrays kept alive on it)."""
+        d = derive.derive(self.spec, self.nz, np.asarray(pv, dtype=np.float64))
+        arrs = [np.ascontiguousarray(d[k], dtype=np.float64) for k in
+                ("inf", "rate", "sus", "beta", "af", "efoi", "agedist", "geo", "Ntime")]
+        t = _OrTables(*[_d(a) for a in arrs])
+        t._keep = arrs
+        t.derived = d
+        return t
+
+    def new_state(self):
+        return State(self.A, self.G, self.C, self.T, self.S)
+
+    def simulate(self, tb, seed, ti=0, tf=None, state=None):
+        """State::simulate(ti, tf) with the libstdc++ engine seeded by `seed`."""
+        L = lib()
+        st = state or self.new_state()
+        L.or_seed(seed)
+        rc = L.or_simulate(C.byref(self.model), C.byref(tb), _d(self.pop_init), C.byref(st._c), ti,
+                           self.S if tf is None else tf)
+        if rc:
+            raise RuntimeError(f"oracle simulate failed: {rc}")
+        return st
+
+    def replay(self, tb, transnum):
+        st = self.new_state()
+        st.transnum[...] = transnum
+        rc = lib().or_replay(C.byref(self.model), C.byref(tb), _d(self.pop_init), C.byref(st._c))
+        if rc:
+            raise RuntimeError(f"oracle replay failed: {rc}")
+        return st
+
+    def likelihood(self, st):
+        return lib().or_likelihood(C.byref(self.model), C.byref(st._c))
+
+    def obs_loglik(self, st, ti=0, tf=None, sel=None):
+        values = np.zeros(max(1, len(self.spec.get("obs", []))))
+        if sel is not None:
+            sel = np.ascontiguousarray(sel, dtype=np.int32)
+        L = lib().or_obs_loglik(C.byref(self.model), C.byref(self.obs), C.byref(st._c), ti,
+                                self.S if tf is None else tf, _i(sel) if sel is not None else None,
+                                len(sel) if sel is not None else 0, _d(values))
+        return L, values[:len(self.spec.get("obs", []))]
+
+    def mbp(self, tbp, initial, seed, simu=None, inf_update=1):
+        L = lib()
+        pro = self.new_state()
+        L.or_seed(seed)
+        mp = None
+        if simu is not None:
+            simu = np.ascontiguousarray(simu, dtype=np.uint8)
+            mp = simu.ctypes.data_as(C.POINTER(C.c_ubyte))
+        rc = L.or_mbp(C.byref(self.model), C.byref(tbp), _d(self.pop_init), C.byref(initial._c), C.byref(pro._c), mp,
+                      inf_update)
+        if rc:
+            raise RuntimeError(f"oracle mbp failed: {rc}")
+        return pro
+
+
+def get_al(mode, EFp, EFi, Prp, Pri, invT=1.0, EFcut=0.0):
+    return lib().or_get_al(mode, EFp, EFi, Prp, Pri, invT, EFcut)
+
+
+def poisson_stream(seed, lam, n):
+    out = np.zeros(n, dtype=np.int32)
+    lib().or_poisson_stream(seed, lam, n, _i(out))
+    return out
+
+
+def binomial_stream(seed, p, nn, n):
+    out = np.zeros(n, dtype=np.int32)
+    lib().or_binomial_stream(seed, p, nn, n, _i(out))
+    return out
+
+
+def normal_probability(x, mean, var):
+    return lib().or_normal_probability(x, mean, var)
+
+
+def poisson_probability(i, lam):
+    return lib().or_poisson_probability(i, lam)
+
+
+def negative_binomial_probability(v, m, shape):
+    return lib().or_negative_binomial_probability(v, m, shape)
+
+
+def obs_prob(value, y, sd, type_, shape=1.0, thresh=0.0):
+    return lib().or_obs_prob(value, y, sd, type_, shape, thresh)
+
+
+_ref = None
+
+
+def ref_lib():
+    """The reference's own src/utils.cc (oracle/_ref), or None when not built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB):
+        L = C.CDLL(REF_LIB)
+        ip = C.POINTER(C.c_int)
+        L.ref_poisson_stream.argtypes = [C.c_int, C.c_double, C.c_int, ip]
+        L.ref_binomial_stream.argtypes = [C.c_int, C.c_double, C.c_uint, C.c_int, ip]
+        L.ref_normal_probability.argtypes = [C.c_double] * 3
+        L.ref_normal_probability.restype = C.c_double
+        L.ref_poisson_probability.argtypes = [C.c_int, C.c_double]
+        L.ref_poisson_probability.restype = C.c_double
+        L.ref_negative_binomial_probability.argtypes = [C.c_uint, C.c_double, C.c_double]
+        L.ref_negative_binomial_probability.restype = C.c_double
+        _ref = L
+    return _ref

@@ -1,0 +1,252 @@
+"""Parity of the HIP path with the FP64 oracle.
+
+Fixed trajectories (the oracle's own stochastic draws loaded into the device):
+populations bit-exact, transition means within FP32 rounding, observation
+log-likelihood within 1e-6 relative.  Stochastic simulation: per-compartment
+means, variances and two-sample KS statistics over 10^4 replicates."""
+import numpy as np
+import pytest
+from scipy import stats
+
+from beep_amd import _lib, configs
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(spec, cap, nslot=None, keep=_lib.KEEP_ALL):
+    return _lib.Context(spec, capacity=cap, nslot=nslot or 1, keep=keep)
+
+
+def _transmean_magnitude(orc, tb, st):
+    """The means recomputed from the absolute values of every summand
+    (|inf * pop|, |weights|): the scale of the FP32 rounding error."""
+    d, nz = tb.derived, orc.nz
+    A, G, T, S = orc.A, orc.G, orc.T, orc.S
+    itr = nz["itr"]
+    out = np.abs(st.transmean).copy()
+    for s in range(S):
+        pop = st.pop[s]
+        dinf = np.einsum("c,acg->ag", np.abs(d["inf"]), np.abs(pop))
+        idg = nz["diag"][:, None] * dinf
+        imp = np.zeros((A, G))
+        for c in range(A):
+            for j in range(nz["rowptr"][c], nz["rowptr"][c + 1]):
+                imp[nz["to"][j]] += abs(nz["val"][j]) * dinf[c]
+        m = d["geo"][s]
+        I = (idg + imp) * abs(m) + np.abs(nz["factor"])[:, None] * idg * abs(1 - m)
+        nmi = I @ np.abs(d["Ntime"][s]).T
+        S_ = np.maximum(pop[:, orc.spec["trans"][itr]["from"], :], 0)
+        lam = (orc.spec["period"] / S) * S_ * d["sus"][None, :] * (
+            np.abs(d["beta"][s])[:, None] * d["af"][s // orc.spec["spd"]][:, None] * nmi + abs(d["efoi"][s]))
+        out[s][:, itr, :] = np.maximum(out[s][:, itr, :], lam)
+    return out
+
+
+@pytest.mark.parametrize("mk", [configs.seir_single, configs.spatial_abcsmc])
+def test_fixed_trajectory_replay(gpu_lib, mk):
+    spec, truth = mk()
+    orc = O.Oracle(spec)
+    tb = orc.tables(truth)
+    st = orc.simulate(tb, seed=20250101)
+    ctx = _ctx(spec, 1)
+    ctx.set_param([0], truth[None, :])
+    ctx.put_transnum(0, 0, st.transnum.astype(np.int32))
+    # Integer state update: bit-exact.
+    pop = ctx.get_pop(0)
+    assert np.array_equal(pop.astype(np.float64), st.pop)
+    # Transition means: FP32 rates against the FP64 oracle.
+    # The reference updates Imap incrementally in FP64 and leaves residues of
+    # ~1e-22 where the exact infectivity is zero (means ~1e-19 instead of 0);
+    # the device recomputes it from the populations.  Means above 1e-9 are
+    # compared relatively, the rest absolutely.
+    # Negative populations (uncapped Poisson draws, src/state.cc:620-630) make
+    # the force of infection a sum of terms of both signs, so the FP32 error is
+    # bounded relative to the sum of the terms' magnitudes.
+    tm = ctx.get_transmean(0).astype(np.float64)
+    ref = st.transmean
+    mag = _transmean_magnitude(orc, tb, st)
+    err = np.abs(tm - ref)
+    assert (err <= 2e-5 * ref + 1e-5 * mag + 1e-12).all(), (err - 2e-5 * ref - 1e-5 * mag).max()
+    big = ref > 1e-3 * np.maximum(mag, 1e-300)
+    assert np.median(np.abs(tm[big] / ref[big] - 1)) < 1e-6
+    # Observation log-likelihood: FP64 on device vs FP64 oracle.
+    L_gpu = ctx.obs_loglik([0])[0]
+    L_ref, values = orc.obs_loglik(st)
+    assert np.array_equal(ctx.obs_values(0), values)
+    assert abs(L_gpu / L_ref - 1) < 1e-6, (L_gpu, L_ref)
+    # State::likelihood with the device's FP32 means.
+    Lp = ctx.pois_loglik([0])[0]
+    Lp_ref = orc.likelihood(st)
+    assert abs(Lp / Lp_ref - 1) < 1e-5, (Lp, Lp_ref)
+    ctx.close()
+
+
+def test_obs_sections_and_datatables(gpu_lib):
+    spec, truth = configs.spatial_abcsmc()
+    spec["sections"] = [(0, 50), (50, 120), (120, 200)]
+    orc = O.Oracle(spec)
+    st = orc.simulate(orc.tables(truth), seed=5)
+    ctx = _ctx(spec, 1)
+    ctx.set_param([0], truth[None, :])
+    ctx.put_transnum(0, 0, st.transnum.astype(np.int32))
+    L, EFdt = ctx.obs_loglik([0], with_dt=True)
+    Lr, _ = orc.obs_loglik(st)
+    assert abs(L[0] / Lr - 1) < 1e-12
+    # per data table (get_EF_datatable)
+    obs = spec["obs"]
+    for d in range(2):
+        sel = [i for i, o in enumerate(obs) if o["datatable"] == d]
+        Ld, _ = orc.obs_loglik(st, sel=sel)
+        assert abs(EFdt[0, d] / (-2 * Ld) - 1) < 1e-12
+    for k, (ti, tf) in enumerate(spec["sections"]):
+        sel = [i for i, o in enumerate(obs) if o["sett_i"] >= ti and o["sett_f"] <= tf]
+        Ls, _ = orc.obs_loglik(st, ti, tf, sel=sel)
+        Lg = ctx.obs_loglik([0], section=k)[0]
+        assert abs(Lg / Ls - 1) < 1e-12
+    ctx.close()
+
+
+def _ks_compare(a, b, label):
+    d, p = stats.ks_2samp(a, b)
+    assert p > 1e-4, (label, d, p)
+    ma, mb = a.mean(), b.mean()
+    se = np.sqrt(a.var() / len(a) + b.var() / len(b)) + 1e-12
+    assert abs(ma - mb) < 5 * se, (label, ma, mb, se)
+    if a.var() > 0 and b.var() > 0:
+        assert 0.85 < a.var() / b.var() < 1.18, (label, a.var(), b.var())
+
+
+def test_simulate_distribution_seir(gpu_lib):
+    """10^4 device replicates vs 10^4 oracle replicates of the single-area SEIR."""
+    spec, truth = configs.seir_single()
+    n = 10_000
+    orc = O.Oracle(spec)
+    tb = orc.tables(truth)
+    ref = np.zeros((n, 4, spec["period"]))
+    for r in range(n):
+        st = orc.simulate(tb, seed=1000 + r)
+        ref[r] = st.pop[:, 0, :, 0].T
+    ctx = _ctx(spec, n, keep=_lib.KEEP_ALL)
+    ctx.set_param([0], truth[None, :])
+    ctx.simulate(np.arange(n), np.zeros(n), np.arange(n) + 77, seed=2024)
+    gpu = np.zeros_like(ref)
+    for r in range(n):
+        gpu[r] = ctx.get_pop(r)[:, 0, :, 0].T
+    for t in (10, 30, 60, 99):
+        for c, name in enumerate("SEIR"):
+            _ks_compare(gpu[:, c, t], ref[:, c, t], f"{name}@{t}")
+    ctx.close()
+
+
+def test_simulate_distribution_spatial(gpu_lib):
+    """Per-compartment totals of the spatial model over 2000 replicates."""
+    spec, truth = configs.spatial_abcsmc()
+    truth = truth.copy()
+    truth[:10] = 2.0
+    n = 2000
+    orc = O.Oracle(spec)
+    tb = orc.tables(truth)
+    S = spec["period"]
+    ts = (40, 80, 120, 199)
+    ref = np.zeros((n, len(ts), 8))
+    for r in range(n):
+        st = orc.simulate(tb, seed=5000 + r)
+        ref[r] = st.pop[list(ts)].sum(axis=(1, 3))
+    ctx = _ctx(spec, n)
+    ctx.set_param([0], truth[None, :])
+    ctx.simulate(np.arange(n), np.zeros(n), np.arange(n) + 1, seed=99)
+    gpu = np.zeros_like(ref)
+    for r in range(n):
+        gpu[r] = ctx.get_pop(r)[list(ts)].sum(axis=(1, 3))
+    for i, t in enumerate(ts):
+        for c in (0, 3, 4, 6):
+            _ks_compare(gpu[:, i, c], ref[:, i, c], f"c{c}@{t}")
+    ctx.close()
+
+
+def test_batch_independence(gpu_lib):
+    """An instance's trajectory does not depend on its slot or its batch."""
+    spec, truth = configs.spatial_abcsmc()
+    ctx = _ctx(spec, 8, nslot=2)
+    ctx.set_param([0, 1], np.stack([truth, truth]))
+    ctx.simulate([3], [1], [555], seed=7)
+    alone = ctx.get_transnum(3)
+    ctx.simulate([0, 1, 2, 5, 6], [0, 0, 1, 0, 1], [1, 2, 555, 4, 5], seed=7)
+    mixed = ctx.get_transnum(2)
+    assert np.array_equal(alone, mixed)
+    ctx.simulate([0], [0], [1], seed=7, plain=True)
+    plain = ctx.get_transnum(0)
+    ctx.simulate([7], [0], [1], seed=7)
+    assert np.array_equal(plain, ctx.get_transnum(7))
+    ctx.close()
+
+
+def test_mbp_identity_and_law(gpu_lib):
+    """Unchanged parameters give n_p == n_i exactly; changed parameters give
+    proposals distributed as fresh simulations (the MBP property)."""
+    spec, truth = configs.seir_single()
+    n = 10_000
+    ctx = _ctx(spec, 3 * n, nslot=2)
+    th2 = truth.copy()
+    th2[0] = 2.5
+    ctx.set_param([0, 1], np.stack([truth, th2]))
+    init = np.arange(n)
+    ctx.simulate(init, np.zeros(n), init + 1, seed=11)
+    prop = init + n
+    ctx.mbp(init, prop, np.zeros(n), init + 1, seed=12)
+    for r in (0, 17, 999):
+        assert np.array_equal(ctx.get_transnum(r), ctx.get_transnum(r + n))
+    ctx.mbp(init, prop, np.ones(n), init + 1, seed=13)
+    fresh = init + 2 * n
+    ctx.simulate(fresh, np.ones(n), fresh + 1, seed=14)
+    a = np.array([ctx.get_pop(r)[-1, 0, 3, 0] for r in prop])
+    b = np.array([ctx.get_pop(r)[-1, 0, 3, 0] for r in fresh])
+    _ks_compare(a, b, "R final after MBP")
+    ctx.close()
+
+
+def test_mbp_against_oracle_law(gpu_lib):
+    """Device MBP proposals and oracle MBP proposals (libstdc++ samplers) from
+    their own initial states have the same law."""
+    spec, truth = configs.seir_single()
+    n = 4000
+    th2 = truth.copy()
+    th2[0] = 1.7
+    orc = O.Oracle(spec)
+    tb1, tb2 = orc.tables(truth), orc.tables(th2)
+    ref = []
+    for r in range(n):
+        st = orc.simulate(tb1, seed=300 + r)
+        pr = orc.mbp(tb2, st, seed=900000 + r)
+        ref.append(pr.pop[-1, 0, :, 0])
+    ref = np.array(ref)
+    ctx = _ctx(spec, 2 * n, nslot=2)
+    ctx.set_param([0, 1], np.stack([truth, th2]))
+    init = np.arange(n)
+    ctx.simulate(init, np.zeros(n), init + 1, seed=21)
+    ctx.mbp(init, init + n, np.ones(n), init + 1, seed=22)
+    gpu = np.array([ctx.get_pop(r)[-1, 0, :, 0] for r in init + n])
+    for c in range(4):
+        _ks_compare(gpu[:, c], ref[:, c], f"c{c}")
+    ctx.close()
+
+
+def test_accept_matches_closed_form(gpu_lib):
+    spec, truth = configs.seir_single()
+    n = 64
+    ctx = _ctx(spec, 2 * n, nslot=1)
+    ctx.set_param([0], truth[None, :])
+    ctx.simulate(np.arange(2 * n), np.zeros(2 * n), np.arange(2 * n) + 5, seed=3)
+    L = ctx.obs_loglik(np.arange(2 * n))
+    EF = -2 * L
+    rng = np.random.default_rng(0)
+    dlogq = rng.normal(0, 1, n)
+    init, prop = np.arange(n), np.arange(n) + n
+    for mode, invT, EFcut in ((1, 0.3, 0.0), (0, 1.0, float(np.median(EF)))):
+        al, acc = ctx.accept(init, prop, mode, dlogq, np.arange(n), invT=invT, EFcut=EFcut, seed=8)
+        for i in range(n):
+            want = O.get_al(mode, EF[prop[i]], EF[init[i]], dlogq[i], 0.0, invT, EFcut)
+            assert abs(al[i] - want) <= 1e-12 * max(1.0, want)
+        assert acc[al >= 1.0].all() and not acc[al <= 0.0].any()
+    ctx.close()

@@ -224,7 +224,7 @@ def dense_pmcmc(A=317, period=300, seed=20250103, with_obs=True):
     ]
     pop = np.zeros((A, 10, G), dtype=np.int64)
     pop[:, S_, :] = ages
-    for c in rng.choice(A, 10, replace=False):
+    for c in rng.choice(A, min(A, 10), replace=False):
         pop[c, I_, 3] += 3
         pop[c, S_, 3] -= 3
     Nraw = 0.3 * np.ones((G, G)) + np.diag(np.linspace(1.5, 0.6, G))

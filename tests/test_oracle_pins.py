@@ -1,0 +1,183 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each check is chosen so a plausible slip in the restatement (a dropped term, a
+wrong sign or index, a transposed operand) fails it: the reference's own
+utils.cc outputs (golden vectors made by tests/golden/make_ref_vectors.py, and
+the live oracle/_ref build where present), closed forms, textbook routines
+(numpy.linalg, scipy.stats, forward Euler) and the reference's own invariants."""
+import json
+import os
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from beep_amd import configs
+from oracle import derive, oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "ref_utils_vectors.json")))
+
+
+def test_sampler_streams_match_reference_vectors():
+    for g in GOLD["poisson_streams"]:
+        assert O.poisson_stream(g["seed"], g["lam"], len(g["samples"])).tolist() == g["samples"]
+    for g in GOLD["binomial_streams"]:
+        assert O.binomial_stream(g["seed"], g["p"], g["n"], len(g["samples"])).tolist() == g["samples"]
+
+
+def test_logpmfs_match_reference_vectors_and_scipy():
+    for g in GOLD["normal_probability"]:
+        v = O.normal_probability(g["x"], g["mean"], g["var"])
+        assert v == pytest.approx(g["value"], rel=1e-14, abs=1e-14)
+        assert v == pytest.approx(stats.norm.logpdf(g["x"], g["mean"], np.sqrt(g["var"])), rel=1e-12)
+    for g in GOLD["poisson_probability"]:
+        v = O.poisson_probability(g["i"], g["lam"])
+        assert v == pytest.approx(g["value"], rel=1e-14, abs=1e-14)
+        if g["lam"] > 0 and g["i"] >= 0:
+            assert v == pytest.approx(stats.poisson.logpmf(g["i"], g["lam"]), rel=1e-12)
+    for g in GOLD["negative_binomial_probability"]:
+        v = O.negative_binomial_probability(g["val"], g["m"], g["shape"])
+        assert v == pytest.approx(g["value"], rel=1e-14, abs=1e-14)
+        r, m = g["shape"], g["m"]
+        assert v == pytest.approx(stats.nbinom.logpmf(g["val"], r, r / (r + m)), rel=1e-10)
+
+
+def test_live_reference_build_when_present():
+    L = O.ref_lib()
+    if L is None:
+        pytest.skip("oracle/_ref not built here")
+    import ctypes as C
+    for seed, lam in ((21, 0.33), (22, 9.99), (23, 10.0), (24, 150.0), (25, 3e6)):
+        a = np.zeros(200, dtype=np.int32)
+        L.ref_poisson_stream(seed, lam, 200, a.ctypes.data_as(C.POINTER(C.c_int)))
+        assert np.array_equal(a, O.poisson_stream(seed, lam, 200))
+    for seed, p, n in ((31, 0.2, 50), (32, 0.999, 7), (33, 0.04, 100000)):
+        a = np.zeros(200, dtype=np.int32)
+        L.ref_binomial_stream(seed, p, n, 200, a.ctypes.data_as(C.POINTER(C.c_int)))
+        assert np.array_equal(a, O.binomial_stream(seed, p, n, 200))
+
+
+@pytest.mark.parametrize("mI,inf", [(7.0, 1.0), (4.0, 1.0), (9.5, 2.0)])
+def test_beta_from_R_seir_closed_form(mI, inf):
+    """SEIR, one group: rho(NGM) = infectivity x mean infectious period."""
+    spec, th = configs.seir_single()
+    th = th.copy()
+    th[2], th[3] = mI, inf
+    nz = derive.normalise(spec)
+    d = derive.derive(spec, nz, th)
+    assert d["ratio"] == pytest.approx(mI * inf, rel=1e-12)
+    assert d["beta"][0, 0] == pytest.approx(th[0] / (mI * inf), rel=1e-12)
+
+
+def test_ngm_eigenvalue_vs_numpy():
+    for mk in (configs.spatial_abcsmc, configs.dense_pmcmc):
+        spec, th = mk()
+        nz = derive.normalise(spec)
+        d = derive.derive(spec, nz, th)
+        K = derive.ngm(spec, nz, d["rate"], d["sus"], d["inf"])
+        ev = np.linalg.eigvals(K)
+        assert d["ratio"] == pytest.approx(ev.real.max(), rel=1e-10)
+
+
+def test_normalisation_identities():
+    spec, _ = configs.spatial_abcsmc()
+    nz = derive.normalise(spec)
+    A = spec["A"]
+    for c in range(A):     # per-capita rows: diag*pop_c + sum val*pop_to == 1
+        tot = nz["diag"][c] * nz["area_pop"][c] + sum(
+            nz["val"][j] * nz["area_pop"][nz["to"][j]] for j in range(nz["rowptr"][c], nz["rowptr"][c + 1]))
+        assert tot == pytest.approx(1.0, rel=1e-13)
+    assert np.allclose(nz["factor"], (1 / nz["area_pop"]) / nz["diag"])
+    N = nz["N"]
+    G = N.shape[0]
+    pop = np.asarray(spec["pop"]).sum(axis=(0, 1))
+    assert np.allclose(N, N.T)
+    assert sum(pop[aa] * N[aa, a] for a in range(G) for aa in range(G)) / (G * pop.sum()) == pytest.approx(1.0)
+
+
+def test_deterministic_seir_is_forward_euler():
+    """Deterministic dynamics with one area: the forward-Euler SEIR recursion
+    with FOI = beta I / N (Idiag = I / N when A = 1)."""
+    spec, th = configs.seir_single()
+    orc = O.Oracle(spec, stochastic=False)
+    tb = orc.tables(th)
+    st = orc.simulate(tb, seed=0)
+    S, E, I, R = 99990.0, 0.0, 10.0, 0.0
+    N = 1e5
+    for s in range(spec["period"]):
+        assert st.pop[s, 0, :, 0] == pytest.approx([S, E, I, R], rel=1e-12, abs=1e-9)
+        inf, ei, ir = 0.3 * S * I / N, E / 5.0, I / 7.0
+        assert st.transnum[s, 0, :, 0] == pytest.approx([inf, ei, ir], rel=1e-12, abs=1e-12)
+        S, E, I, R = S - inf, E + inf - ei, I + ei - ir, R + ir
+
+
+def _recompute_imap(orc, tb, pop_s):
+    nz, d = orc.nz, tb.derived
+    A, G = orc.A, orc.G
+    dinf = np.einsum("c,acg->ag", d["inf"], pop_s)
+    M = np.zeros((A, A))
+    for c in range(A):
+        for j in range(nz["rowptr"][c], nz["rowptr"][c + 1]):
+            M[c, nz["to"][j]] += nz["val"][j]
+    return (M.T @ dinf).ravel(), (nz["diag"][:, None] * dinf).ravel()
+
+
+def test_conservation_replay_and_incremental_imap():
+    spec, th = configs.spatial_abcsmc()
+    th = th.copy()
+    th[:10] = 2.2
+    orc = O.Oracle(spec)
+    tb = orc.tables(th)
+    st = orc.simulate(tb, seed=4)
+    tot = st.pop.sum(axis=2)                       # [S][A][G]: constant in time
+    assert (tot == tot[0][None]).all()
+    rp = orc.replay(tb, st.transnum)               # initialise_from_particle
+    assert np.array_equal(rp.pop, st.pop)
+    assert np.array_equal(rp.transmean, st.transmean)
+    for s in (0, 50, 120, 199):                    # State::check: incremental == recomputed
+        imap, idiag = _recompute_imap(orc, tb, st.pop[s])
+        assert np.allclose(st.Imap[s], imap, rtol=1e-9, atol=1e-12)
+        assert np.allclose(st.Idiag[s], idiag, rtol=1e-9, atol=1e-12)
+
+
+def test_dense_contraction_brute_force():
+    spec, th = configs.dense_pmcmc(A=6, period=20)
+    orc = O.Oracle(spec)
+    tb = orc.tables(th)
+    st = orc.simulate(tb, seed=9)
+    for s in (0, 7, 19):
+        imap, idiag = _recompute_imap(orc, tb, st.pop[s])
+        assert np.allclose(st.Imap[s], imap, rtol=1e-10, atol=1e-13)
+
+
+def test_mbp_identity_and_thinning():
+    spec, th = configs.seir_single()
+    orc = O.Oracle(spec)
+    tb = orc.tables(th)
+    st = orc.simulate(tb, seed=5)
+    pr = orc.mbp(tb, st, seed=6)
+    assert np.array_equal(pr.transnum, st.transnum)
+    th2 = th.copy()
+    th2[0] = 1.5                                   # lower R: infections thinned, never added
+    pr2 = orc.mbp(orc.tables(th2), st, seed=7)
+    assert (pr2.transnum[:3, 0, 0, 0] <= st.transnum[:3, 0, 0, 0]).all()
+
+
+def test_get_al_closed_form():
+    assert O.get_al(1, 10.0, 12.0, -1.0, -1.5, 0.5, 0.0) == pytest.approx(min(1, np.exp(-0.25 * (10 - 12) + 0.5)))
+    assert O.get_al(1, 30.0, 12.0, -1.0, -1.0, 0.5, 0.0) == pytest.approx(np.exp(-0.25 * 18))
+    assert O.get_al(0, 10.0, 0.0, -2.0, -1.0, 1.0, 11.0) == pytest.approx(np.exp(-1.0))
+    assert O.get_al(0, 12.0, 0.0, -2.0, -1.0, 1.0, 11.0) == 0.0
+
+
+def test_obs_prob_edge_cases():
+    assert O.obs_prob(5.0, 999999997.0, 1.0, 0) == 0.0                       # UNKNOWN
+    assert O.obs_prob(-3.0, 4.0, 2.0, 0) == pytest.approx(stats.norm.logpdf(4.0, 0.0, 2.0))
+    assert O.obs_prob(0.3, 2.0, 0.0, 2) == pytest.approx(stats.poisson.logpmf(2, 1.0))   # lambda floored at 1
+    assert O.obs_prob(5.0, 2.6, 0.0, 2) == pytest.approx(stats.poisson.logpmf(3, 5.0))   # int(y + 0.5)
+    assert O.obs_prob(8.0, 3.0, 0.0, 3, 2.5) == pytest.approx(stats.nbinom.logpmf(3, 2.5, 2.5 / 10.5))
+    th = 999999998.0
+    assert O.obs_prob(2.0, th, 0.0, 2, 1.0, 4.0) == pytest.approx(np.log(stats.poisson.cdf(4, 2.0)))
+    assert O.obs_prob(1.0, th, 3.0, 0, 1.0, 5.0) == pytest.approx(stats.norm.logpdf(5.0, 5.0, 3.0))
+    assert O.obs_prob(9.0, th, 3.0, 0, 1.0, 5.0) == pytest.approx(stats.norm.logpdf(5.0, 9.0, 3.0))

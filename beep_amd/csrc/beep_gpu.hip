@@ -10,7 +10,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/beep_gpu.h"
+#include "beep_gpu.h"
 #include "host_model.hh"
 #include "kernels.hh"
 #include "obs_kernels.hh"
@@ -157,44 +157,31 @@ int check_device_error(beep_ctx* c) {
   return BEEP_OK;
 }
 
-// Threads per workgroup for the fused kernel: enough lanes for the cells of
-// one instance, at most 256.
+// Threads per workgroup for the fused kernel: whole wavefronts covering the
+// cells of one instance in one pass, at most BEEP_FUSED_MAX_BLOCK.
 int fused_block(int AG) {
-  if (AG <= 64) return 64;
-  if (AG <= 128) return 128;
-  if (AG <= 192) return 192;
-  return 256;
+  int b = (AG + 63) / 64 * 64;
+  return b > BEEP_FUSED_MAX_BLOCK ? BEEP_FUSED_MAX_BLOCK : b;
 }
 
 template <int MODE>
-int launch_fused_t(beep_ctx* c, const Job* djobs, int njobs, int T, size_t lds, int block, uint64_t seed,
-                   uint32_t stream) {
+int launch_fused_t(beep_ctx* c, const Job* djobs, int njobs, size_t lds, int block, uint64_t seed, uint32_t stream) {
   if (njobs == 0) return BEEP_OK;
-  dim3 grid(njobs), blk(block);
-#define BEEP_LAUNCH(TM)                                                                                   \
-  do {                                                                                                    \
-    auto kfn = beepdev::k_fused<MODE, TM>;                                                                \
-    if (lds > 65536) HIPCHK(c, hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
-    hipLaunchKernelGGL(kfn, grid, blk, lds, c->stream, djobs, seed, stream);                              \
-  } while (0)
-  if (T <= 4) BEEP_LAUNCH(4);
-  else if (T <= 8) BEEP_LAUNCH(8);
-  else if (T <= 12) BEEP_LAUNCH(12);
-  else if (T <= 16) BEEP_LAUNCH(16);
-  else return fail(c, BEEP_EINVAL, "at most 16 transitions are supported");
-#undef BEEP_LAUNCH
+  auto kfn = beepdev::k_fused<MODE>;
+  if (lds > 65536) HIPCHK(c, hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kfn, dim3(njobs), dim3(block), lds, c->stream, djobs, seed, stream);
   HIPCHK(c, hipGetLastError());
   return BEEP_OK;
 }
 
 int launch_fused(beep_ctx* c, int mode, const Job* djobs, int njobs, uint64_t seed, uint32_t stream) {
   const auto& m = c->m;
-  size_t lds = beepdev::fused_lds_bytes(m.AG, m.C, m.T, m.G);
+  size_t lds = beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, c->h_topo.nnz, c->h_topo.stage_mix);
   if (lds > 160 * 1024) return fail(c, BEEP_EINVAL, "instance needs %zu bytes of LDS (max 163840)", lds);
   int block = fused_block(m.AG);
-  if (mode == beepdev::MODE_SIM) return launch_fused_t<beepdev::MODE_SIM>(c, djobs, njobs, m.T, lds, block, seed, stream);
-  if (mode == beepdev::MODE_MBP) return launch_fused_t<beepdev::MODE_MBP>(c, djobs, njobs, m.T, lds, block, seed, stream);
-  return launch_fused_t<beepdev::MODE_REPLAY>(c, djobs, njobs, m.T, lds, block, seed, stream);
+  if (mode == beepdev::MODE_SIM) return launch_fused_t<beepdev::MODE_SIM>(c, djobs, njobs, lds, block, seed, stream);
+  if (mode == beepdev::MODE_MBP) return launch_fused_t<beepdev::MODE_MBP>(c, djobs, njobs, lds, block, seed, stream);
+  return launch_fused_t<beepdev::MODE_REPLAY>(c, djobs, njobs, lds, block, seed, stream);
 }
 
 Job make_job(beep_ctx* c, int inst, int slot, uint32_t key, int t0, int t1) {
@@ -319,6 +306,8 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
 
   TopoDev& tp = c->h_topo;
   tp.A = m.A; tp.G = m.G; tp.C = m.C; tp.T = m.T; tp.S = m.S; tp.AG = m.AG; tp.inf_tr = m.inf_tr; tp.spd = m.spd;
+  tp.nnz = (int)m.in_src.size();
+  tp.stage_mix = beepdev::fused_mix_bytes(m.A, tp.nnz) <= 32768 ? 1 : 0;
   tp.dt = (float)m.dt;
   int off = 0;
   tp.off_rate = off; off += m.T * m.G;

@@ -2,16 +2,16 @@
 // the tau-leap kernels.
 //
 // Streams.  Every draw is a pure function of
-//   (seed, stream, instance key, step, cell = area*G + dp, transition, draw#)
+//   (seed, stream, instance key, step, cell = area*G + dp, transition order)
 // so a trajectory is the same whichever batch, slot or launch shape computes
 // it.  Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
 // as easy as 1, 2, 3", SC'11) maps a 128-bit counter and 64-bit key to 128
 // random bits:
 //   key     = seed
-//   counter = (step | stream << 16, cell, word2, instance key)
-// word2 = transition group q (transitions 4q..4q+3 take their first uniform
-// from lanes 0..3 of that block) or, for further uniforms of transition tr,
-// 0x80000000 | tr << 22 | block index.
+//   counter = (step | stream << 16, cell, block, instance key)
+// The uniforms of one (step, cell) form a pool read in order by the draws of
+// its transitions, taken in transition order; block counts the 4-word Philox
+// blocks the pool has used.
 //
 // Samplers (the reference draws with libstdc++ std::poisson_distribution and
 // std::binomial_distribution, src/utils.cc:148-180; the device draws the same
@@ -25,6 +25,11 @@
 //                        Schmeiser, CACM 31 (1988) 216-222);
 //   binomial otherwise:  BTRD (W. Hormann, "The generation of binomial random
 //                        variates", J. Statist. Comput. Simul. 46 (1993) 101-110).
+// All arithmetic is FP32: the rejection tests use C. Loader's saddle-point
+// log-pmf, candidates are formed as floor(mean) + floor(frac(mean) + x) so they
+// are exact integers, means of 2^23 and above are drawn as sums of Poisson
+// variates (superposition) and binomials of 2^23 trials and above as sums of
+// binomials over chunks of the trials, both exact in distribution.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -40,13 +45,14 @@ __device__ __host__ inline uint32_t mulhi32(uint32_t a, uint32_t b) {
 __device__ __host__ inline u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int i = 0; i < 10; i++) {
-    uint32_t hi0 = mulhi32(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-    uint32_t hi1 = mulhi32(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    // One 32x32->64 product gives both halves (v_mad_u64_u32).
+    uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
     u4 o;
-    o.x = hi1 ^ c.y ^ k0;
-    o.y = lo1;
-    o.z = hi0 ^ c.w ^ k1;
-    o.w = lo0;
+    o.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    o.y = (uint32_t)p1;
+    o.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    o.w = (uint32_t)p0;
     c = o;
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
@@ -55,7 +61,11 @@ __device__ __host__ inline u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
 }
 
 __device__ inline uint32_t pick(const u4& b, int i) {
-  return i == 0 ? b.x : (i == 1 ? b.y : (i == 2 ? b.z : b.w));
+  uint32_t x = b.x;
+  x = i == 1 ? b.y : x;
+  x = i == 2 ? b.z : x;
+  x = i == 3 ? b.w : x;
+  return x;
 }
 
 // Uniform on the open interval (0,1) from 32 random bits, FP64.
@@ -63,30 +73,82 @@ __device__ inline double u01d(uint32_t x) { return ((double)x + 0.5) * 2.3283064
 // Uniform on (0,1) from the top 24 bits, FP32.
 __device__ inline float u01f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 5.9604644775390625e-08f; }
 
-// Further uniforms of one (step, cell, transition) draw.
-struct ExtraUniforms {
-  uint32_t k0, k1, c0, c1, c3, tr;
+// The uniform pool of one (step, cell).
+struct CellRng {
+  uint32_t k0, k1, c0, c1, c3;
   uint32_t block;
   int idx;
   u4 buf;
-  __device__ inline double next() {
+  __device__ inline void init(uint32_t kk0, uint32_t kk1, uint32_t cc0, uint32_t cc1, uint32_t cc3) {
+    k0 = kk0; k1 = kk1; c0 = cc0; c1 = cc1; c3 = cc3; block = 0; idx = 4;
+  }
+  __device__ inline uint32_t bits() {
     if (idx >= 4) {
-      u4 c = {c0, c1, 0x80000000u | (tr << 22) | block, c3};
-      buf = philox4x32_10(c, k0, k1);
+      buf = philox4x32_10(u4{c0, c1, block, c3}, k0, k1);
       block++;
       idx = 0;
     }
     uint32_t x = pick(buf, idx);
     idx++;
-    return u01d(x);
+    return x;
   }
+  __device__ inline float next() { return u01f(bits()); }
 };
+typedef CellRng ExtraUniforms;
 
-#define BEEP_MAX_EXTRA_BLOCKS 64u
+#define BEEP_MAX_EXTRA_BLOCKS 4096u
+
+// ---- log-pmf pieces in FP32 (C. Loader, "Fast and accurate computation of
+// binomial probabilities", 2000): log n! = (n+1/2) log n - n + log(2 pi)/2 +
+// stirlerr(n), and bd0(x, m) = x log(x/m) + m - x evaluated without
+// cancellation, so log Pois(k; m) = -log(2 pi k)/2 - stirlerr(k) - bd0(k, m).
+
+__device__ inline float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// stirlerr(0..16) from lgamma (entry 0 unused).
+__constant__ float kStirlerr[17] = {0.0f, 8.106146680e-02f, 4.134069596e-02f, 2.767792568e-02f, 2.079067210e-02f,
+                                    1.664469119e-02f, 1.387612882e-02f, 1.189670995e-02f, 1.041126526e-02f,
+                                    9.255462183e-03f, 8.330563433e-03f, 7.573675488e-03f, 6.942840107e-03f,
+                                    6.408994188e-03f, 5.951370113e-03f, 5.554733552e-03f, 5.207655920e-03f};
+
+__device__ inline float stirlerr_f(float n) {
+  // The table below 17, the asymptotic series above.
+  if (n <= 16.f) return kStirlerr[(int)n];
+  const float S0 = 1.f / 12.f, S1 = 1.f / 360.f, S2 = 1.f / 1260.f, S3 = 1.f / 1680.f;
+  float in = __builtin_amdgcn_rcpf(n), inn = in * in;
+  if (n > 80.f) return (S0 - (S1 - S2 * inn) * inn) * in;
+  return (S0 - (S1 - (S2 - S3 * inn) * inn) * inn) * in;
+}
+
+__device__ inline float bd0_f(float x, float m) {
+  float d = x - m;
+  if (fabsf(d) < 0.1f * (x + m)) {
+    // 2 x sum_{j>=1} v^(2j+1)/(2j+1) with v = d/(x+m): |v| < 0.1, so 8 terms
+    // reach FP32 precision.
+    const float inv_odd[9] = {0.f, 1.f / 3.f, 1.f / 5.f, 1.f / 7.f, 1.f / 9.f, 1.f / 11.f, 1.f / 13.f, 1.f / 15.f,
+                              1.f / 17.f};
+    float v = fdiv(d, x + m);
+    float s = d * v;
+    float ej = 2.f * x * v;
+    v = v * v;
+#pragma unroll
+    for (int j = 1; j < 9; j++) {
+      ej *= v;
+      s += ej * inv_odd[j];
+    }
+    return s;
+  }
+  return x * logf(x / m) + m - x;
+}
+
+// log Pois(k; lam), k >= 1.
+__device__ inline float log_pois_f(float k, float lam) {
+  return -0.5f * __logf(6.2831853071795865f * k) - stirlerr_f(k) - bd0_f(k, lam);
+}
 
 // Poisson(lam) with first uniform u (FP32 path for small means).
 __device__ inline int poisson_inversion(float lam, float u) {
-  float p = expf(-lam);
+  float p = __expf(-lam);
   if (u <= p) return 0;
   float F = p;
   int k = 0;
@@ -95,7 +157,7 @@ __device__ inline int poisson_inversion(float lam, float u) {
   // no longer change the sum (a tail event of probability ~1e-7).
   while (k < 64) {
     k++;
-    p *= lam / (float)k;
+    p *= lam * __builtin_amdgcn_rcpf((float)k);
     float Fn = F + p;
     if (u <= Fn || Fn == F) return k;
     F = Fn;
@@ -103,125 +165,148 @@ __device__ inline int poisson_inversion(float lam, float u) {
   return k;
 }
 
-__device__ inline int poisson_ptrs(double lam, double ufirst, ExtraUniforms& r) {
-  const double slam = sqrt(lam), loglam = log(lam);
-  const double b = 0.931 + 2.53 * slam;
-  const double a = -0.059 + 0.02483 * b;
-  const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
-  const double vr = 0.9277 - 3.6224 / (b - 2.0);
-  double Uraw = ufirst;
+// PTRS (Hormann 1993) for 10 <= lam < 2^23: the squeeze accepts most
+// candidates; the exact test compares with the saddle-point log-pmf.
+__device__ inline int poisson_ptrs(float lam, float ufirst, ExtraUniforms& r) {
+  const float slam = __builtin_amdgcn_sqrtf(lam);
+  const float b = 0.931f + 2.53f * slam;
+  const float a = -0.059f + 0.02483f * b;
+  const float vr = 0.9277f - fdiv(3.6224f, b - 2.f);
+  const float lfl = floorf(lam), lfr = lam - lfl;
+  float Uraw = ufirst;
   for (;;) {
-    double U = Uraw - 0.5;
-    double V = r.next();
-    double us = 0.5 - fabs(U);
-    double k = floor((2.0 * a / us + b) * U + lam + 0.43);
-    if (us >= 0.07 && V <= vr) return (int)k;
-    if (!(k < 0.0 || (us < 0.013 && V > us))) {
-      if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lam + k * loglam - lgamma(k + 1.0)) return (int)k;
+    float U = Uraw - 0.5f;
+    float V = r.next();
+    float us = 0.5f - fabsf(U);
+    float k = lfl + floorf((fdiv(2.f * a, us) + b) * U + (lfr + 0.43f));
+    if (us >= 0.07f && V <= vr) return (int)k;
+    if (!(k < 0.f || (us < 0.013f && V > us))) {
+      float lhs = __logf(V * (1.1239f + fdiv(1.1328f, b - 3.4f))) - __logf(fdiv(a, us * us) + b);
+      float rhs = k == 0.f ? -lam : log_pois_f(k, lam);
+      if (lhs <= rhs) return (int)k;
     }
-    if (r.block >= BEEP_MAX_EXTRA_BLOCKS) return (int)floor(lam);   // unreachable in practice
+    if (r.block >= BEEP_MAX_EXTRA_BLOCKS) return (int)lfl;   // unreachable in practice
     Uraw = r.next();
   }
 }
 
-__device__ inline int poisson_draw(float lam, float ufirst_f, double ufirst_d, ExtraUniforms& r) {
-  if (lam < 10.0f) return poisson_inversion(lam, ufirst_f);
-  return poisson_ptrs((double)lam, ufirst_d, r);
+#define BEEP_SPLIT (8388608.0f)   // 2^23
+
+__device__ inline int poisson_draw(float lam, float ufirst, ExtraUniforms& r) {
+  if (lam < 10.0f) return poisson_inversion(lam, ufirst);
+  if (lam < BEEP_SPLIT) return poisson_ptrs(lam, ufirst, r);
+  // Superposition: Pois(lam) = sum of m independent Pois(lam / m).
+  int m = (int)ceilf(lam / BEEP_SPLIT);
+  float part = lam / (float)m;
+  int n = 0;
+  for (int i = 0; i < m; i++) n += poisson_ptrs(part, i == 0 ? ufirst : r.next(), r);
+  return n;
 }
 
-// log(k!) - [(k+1/2) log(k+1) - (k+1) + log(2 pi)/2]: the Stirling-series
-// remainder used by BTRD's final acceptance test.
-__device__ inline double stirling_fc(double k) {
-  return lgamma(k + 1.0) - ((k + 0.5) * log(k + 1.0) - (k + 1.0) + 0.91893853320467274178);
-}
-
-__device__ inline int binomial_inversion(int n, double p, double ufirst, ExtraUniforms& r) {
-  const double q = 1.0 - p, s = p / q, a = (n + 1) * s;
-  const double r0 = pow(q, (double)n);
-  double u = ufirst;
+// Inversion from k = 0 (BINV), FP32, for n * p < 10 with p <= 1/2.
+__device__ inline int binomial_inversion(int n, float p, float ufirst, ExtraUniforms& r) {
+  const float q = 1.f - p, s = fdiv(p, q), a = (float)(n + 1) * s;
+  const float r0 = __expf((float)n * log1pf(-p));
+  float u = ufirst;
   for (;;) {
-    double rr = r0;
+    float rr = r0;
     int k = 0;
     while (u > rr) {
       u -= rr;
       k++;
-      if (k > n) break;
-      rr *= (a / k - s);
-      if (rr <= 0.0) { k = n + 1; break; }
+      if (k > n || k > 110) break;
+      rr *= (a * __builtin_amdgcn_rcpf((float)k) - s);
+      if (rr <= 0.f) { k = n + 1; break; }
     }
-    if (k <= n) return k;
-    if (r.block >= BEEP_MAX_EXTRA_BLOCKS) return (int)floor(n * p);   // unreachable in practice
-    u = r.next();
+    if (k <= n && k <= 110) return k;
+    if (r.block >= BEEP_MAX_EXTRA_BLOCKS) return (int)floorf(n * p);   // unreachable in practice
+    u = (float)r.next();
   }
 }
 
-__device__ inline int binomial_btrd(int n, double p, double ufirst, ExtraUniforms& r) {
-  const double q = 1.0 - p;
-  const double m = floor((n + 1) * p);
-  const double rr = p / q, nr = (n + 1) * rr, npq = n * p * q;
-  const double snpq = sqrt(npq);
-  const double b = 1.15 + 2.53 * snpq;
-  const double a = -0.0873 + 0.0248 * b + 0.01 * p;
-  const double c = n * p + 0.5;
-  const double alpha = (2.83 + 5.1 / b) * snpq;
-  const double vr = 0.92 - 4.2 / b;
-  const double urvr = 0.86 * vr;
-  double v = ufirst;
+// BTRD (Hormann 1993) in FP32 for n * p >= 10, p <= 1/2, n < 2^24;
+// fc(k) = stirlerr(k + 1).
+__device__ inline int binomial_btrd(int n, float p, float ufirst, ExtraUniforms& r) {
+  const float q = 1.f - p;
+  const float nf = (float)n;
+  const float m = floorf((nf + 1.f) * p);
+  const float rr = fdiv(p, q), nr = (nf + 1.f) * rr, npq = nf * p * q;
+  const float snpq = __builtin_amdgcn_sqrtf(npq);
+  const float b = 1.15f + 2.53f * snpq;
+  const float a = -0.0873f + 0.0248f * b + 0.01f * p;
+  const float c = nf * p + 0.5f;
+  const float alpha = (2.83f + fdiv(5.1f, b)) * snpq;
+  const float vr = 0.92f - fdiv(4.2f, b);
+  const float urvr = 0.86f * vr;
+  float v = ufirst;
   for (;;) {
-    double u;
+    float u;
     if (v <= urvr) {
-      u = v / vr - 0.43;
-      return (int)floor((2.0 * a / (0.5 - fabs(u)) + b) * u + c);
+      u = fdiv(v, vr) - 0.43f;
+      return (int)floorf((fdiv(2.f * a, 0.5f - fabsf(u)) + b) * u + c);
     }
     if (v >= vr) {
-      u = r.next() - 0.5;
+      u = (float)r.next() - 0.5f;
     } else {
-      u = v / vr - 0.93;
-      u = (u < 0 ? -0.5 : 0.5) - u;
-      v = r.next() * vr;
+      u = fdiv(v, vr) - 0.93f;
+      u = (u < 0.f ? -0.5f : 0.5f) - u;
+      v = (float)r.next() * vr;
     }
-    double us = 0.5 - fabs(u);
-    double k = floor((2.0 * a / us + b) * u + c);
+    float us = 0.5f - fabsf(u);
+    float k = floorf((fdiv(2.f * a, us) + b) * u + c);
     bool accept = false;
-    if (k >= 0.0 && k <= (double)n) {
-      v = v * alpha / (a / (us * us) + b);
-      double km = fabs(k - m);
-      if (km <= 15.0) {
-        double f = 1.0;
+    if (k >= 0.f && k <= nf) {
+      v = v * fdiv(alpha, fdiv(a, us * us) + b);
+      float km = fabsf(k - m);
+      if (km <= 15.f) {
+        float f = 1.f;
         if (m < k) {
-          for (double i = m + 1; i <= k; i += 1.0) f *= (nr / i - rr);
+          for (float i = m + 1.f; i <= k; i += 1.f) f *= (fdiv(nr, i) - rr);
         } else if (m > k) {
-          for (double i = k + 1; i <= m; i += 1.0) v *= (nr / i - rr);
+          for (float i = k + 1.f; i <= m; i += 1.f) v *= (fdiv(nr, i) - rr);
         }
         accept = (v <= f);
       } else {
-        double lv = log(v);
-        double rho = (km / npq) * (((km / 3.0 + 0.625) * km + 1.0 / 6.0) / npq + 0.5);
-        double t = -km * km / (2.0 * npq);
+        float lv = __logf(v);
+        float inpq = __builtin_amdgcn_rcpf(npq);
+        float rho = (km * inpq) * (((km * (1.f / 3.f) + 0.625f) * km + 1.f / 6.f) * inpq + 0.5f);
+        float t = -km * km * 0.5f * inpq;
         if (lv < t - rho) accept = true;
         else if (lv <= t + rho) {
-          double nm = n - m + 1;
-          double h = (m + 0.5) * log((m + 1) / (rr * nm)) + stirling_fc(m) + stirling_fc(n - m);
-          double nk = n - k + 1;
-          accept = lv <= h + (n + 1) * log(nm / nk) + (k + 0.5) * log(nk * rr / (k + 1)) - stirling_fc(k) -
-                             stirling_fc(n - k);
+          float nm = nf - m + 1.f;
+          float h = (m + 0.5f) * logf((m + 1.f) / (rr * nm)) + stirlerr_f(m + 1.f) + stirlerr_f(nf - m + 1.f);
+          float nk = nf - k + 1.f;
+          accept = lv <= h + (nf + 1.f) * logf(nm / nk) + (k + 0.5f) * logf(nk * rr / (k + 1.f)) -
+                             stirlerr_f(k + 1.f) - stirlerr_f(nf - k + 1.f);
         }
       }
     }
     if (accept) return (int)k;
-    if (r.block >= BEEP_MAX_EXTRA_BLOCKS) return (int)floor(n * p);   // unreachable in practice
-    v = r.next();
+    if (r.block >= BEEP_MAX_EXTRA_BLOCKS) return (int)floorf(n * p);   // unreachable in practice
+    v = (float)r.next();
   }
 }
 
+__device__ inline int binomial_chunk(int n, float p, float ufirst, ExtraUniforms& r) {
+  if ((float)n * p < 10.f) return binomial_inversion(n, p, ufirst, r);
+  return binomial_btrd(n, p, ufirst, r);
+}
+
 // Binomial(n, p) for 0 < p < 1.
-__device__ inline int binomial_draw(int n, double p, double ufirst, ExtraUniforms& r) {
+__device__ inline int binomial_draw(int n, float p, float ufirst, ExtraUniforms& r) {
   if (n <= 0) return 0;
-  bool flip = p > 0.5;
-  double pp = flip ? 1.0 - p : p;
-  int k;
-  if (n * pp < 10.0) k = binomial_inversion(n, pp, ufirst, r);
-  else k = binomial_btrd(n, pp, ufirst, r);
+  bool flip = p > 0.5f;
+  float pp = flip ? 1.f - p : p;
+  int k = 0;
+  const int chunk = 1 << 23;
+  int left = n;
+  bool first = true;
+  while (left > 0) {          // Bin(n, p) = sum of Bin(chunk, p) over chunks of the trials
+    int c = left < chunk ? left : chunk;
+    k += binomial_chunk(c, pp, first ? ufirst : r.next(), r);
+    first = false;
+    left -= c;
+  }
   return flip ? n - k : k;
 }
 

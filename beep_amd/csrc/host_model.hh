@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/beep_gpu.h"
+#include "beep_gpu.h"
 
 namespace beephost {
 

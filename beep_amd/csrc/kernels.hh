@@ -18,6 +18,8 @@ namespace beepdev {
 
 struct TopoDev {
   int A, G, C, T, S, AG, inf_tr, spd;
+  int nnz;               // off-diagonal mixing entries
+  int stage_mix;         // 1: the mixing CSR is copied into LDS
   float dt;
   int off_rate, off_sus, off_inf, off_N, off_efoi, off_geo, off_beta;
   const int* from;       // [T]
@@ -49,42 +51,111 @@ struct Job {
 
 enum { MODE_SIM = 0, MODE_MBP = 1, MODE_REPLAY = 2 };
 
-// LDS bytes a job needs.
-__host__ __device__ inline size_t fused_lds_bytes(int AG, int C, int T, int G) {
-  return (size_t)4 * ((size_t)C * AG + 2 * (size_t)AG + (size_t)T * G + (size_t)G * G + G + C + 2 * T);
+// Trajectory outputs are written once and read by later kernels: global
+// address space, non-temporal (streaming) stores.
+typedef __attribute__((address_space(1))) int gint;
+typedef __attribute__((address_space(1))) float gfloat;
+__device__ inline void st_nt(int* p, size_t i, int v) { __builtin_nontemporal_store(v, (gint*)p + i); }
+__device__ inline void st_nt(float* p, size_t i, float v) { __builtin_nontemporal_store(v, (gfloat*)p + i); }
+
+// Queue capacity of large draws per step (items beyond it are drawn in place).
+__host__ __device__ inline int fused_qcap(int AG) { return AG < 32 ? 64 : 2 * AG; }
+
+// Bytes of the mixing CSR when staged in LDS.
+__host__ __device__ inline size_t fused_mix_bytes(int A, int nnz) { return (size_t)4 * (3 * (size_t)A + 1 + 2 * (size_t)nnz); }
+
+// LDS bytes a job needs: populations double-buffered, infectivity, the mixed
+// infectivity I, the queue of large draws, the small per-slot tables and,
+// when staged, the mixing CSR.
+__host__ __device__ inline size_t fused_lds_bytes(int A, int G, int C, int T, int nnz, int stage_mix) {
+  int AG = A * G;
+  size_t b = (size_t)4 * (2 * (size_t)C * AG + 2 * (size_t)AG + 2 * (size_t)fused_qcap(AG) + 4 + (size_t)T * G +
+                          (size_t)G * G + G + C + 2 * T);
+  if (stage_mix) b += fused_mix_bytes(A, nnz);
+  return b;
 }
 
-template <int MODE, int TMAX>
-__global__ void __launch_bounds__(256) k_fused(const Job* __restrict__ jobs, uint64_t seed, uint32_t stream) {
+#define BEEP_FUSED_MAX_BLOCK 512
+#define BEEP_POIS_INLINE 10.0f     // means below this are drawn in place by inversion
+
+enum { ITEM_POIS = 0, ITEM_POIS_ADD = 1, ITEM_BINOM = 2 };
+
+// A large draw: its own Philox stream (word2 = 0x80000000 | tr << 22 | block)
+// so the value does not depend on which lane draws it or when.
+__device__ inline int draw_item(int kind, float val, int ni, uint32_t k0, uint32_t k1, uint32_t c0, uint32_t cell,
+                                uint32_t tr, uint32_t key) {
+  ExtraUniforms xr;
+  xr.init(k0, k1, c0, cell, key);
+  xr.block = 0x80000000u | (tr << 22);
+  float u = xr.next();
+  if (kind == ITEM_POIS) return poisson_draw(val, u, xr);
+  if (kind == ITEM_POIS_ADD) return ni + poisson_draw(val, u, xr);
+  return binomial_draw(ni, val, u, xr);
+}
+
+// One workgroup advances one instance over steps [t0, t1).  Each thread owns
+// the cells tid, tid + blockDim, ...; populations live in LDS for the whole
+// horizon and only the trajectory (transnum, transmean, pop) goes to HBM.
+// Per step: (1) infectivity per cell, (2) area mixing, (3a) means, small
+// draws in place, large draws queued, (3b) queued draws by all lanes.
+template <int MODE>
+__global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
+    k_fused(const Job* __restrict__ jobs, uint64_t seed, uint32_t stream) {
   const Job jb = jobs[blockIdx.x];
-  const TopoDev tp = *jb.topo;
-  const int A = tp.A, G = tp.G, C = tp.C, T = tp.T, S = tp.S, AG = tp.AG, inf_tr = tp.inf_tr;
+  const TopoDev* __restrict__ tpp = jb.topo;
+  const int A = tpp->A, G = tpp->G, C = tpp->C, T = tpp->T, S = tpp->S, AG = tpp->AG, inf_tr = tpp->inf_tr;
   const int nt = blockDim.x, tid = threadIdx.x;
-  const float dt = tp.dt;
+  const float dt = tpp->dt;
+  const int qcap = fused_qcap(AG);
 
   extern __shared__ int smem[];
-  int* spop = smem;                                    // [C][AG]
-  float* sdinf = (float*)(spop + (size_t)C * AG);      // [AG]
+  int* popA = smem;                                    // [C][AG]
+  int* popB = popA + (size_t)C * AG;                   // [C][AG]
+  float* sdinf = (float*)(popB + (size_t)C * AG);      // [AG]
   float* sI = sdinf + AG;                              // [AG]
-  float* srate = sI + AG;                              // [T][G]
+  uint32_t* qcode = (uint32_t*)(sI + AG);              // [qcap]
+  float* qval = (float*)(qcode + qcap);                // [qcap]
+  int* qcount = (int*)(qval + qcap);                   // [4]
+  float* srate = (float*)(qcount + 4);                 // [T][G]
   float* sN = srate + T * G;                           // [G][G]
   float* ssus = sN + G * G;                            // [G]
   float* sinf = ssus + G;                              // [C]
   int* sfrom = (int*)(sinf + C);                       // [T]
   int* sto = sfrom + T;                                // [T]
+  const float* diag = tpp->diag;
+  const float* fac = tpp->fac;
+  const int* in_ptr = tpp->in_ptr;
+  const int* in_src = tpp->in_src;
+  const float* in_w = tpp->in_w;
+  if (tpp->stage_mix) {
+    float* sdiag = (float*)(sto + T);
+    float* sfac = sdiag + A;
+    int* sptr = (int*)(sfac + A);
+    int* ssrc = sptr + A + 1;
+    float* sw = (float*)(ssrc + tpp->nnz);
+    for (int i = tid; i < A; i += nt) { sdiag[i] = diag[i]; sfac[i] = fac[i]; }
+    for (int i = tid; i <= A; i += nt) sptr[i] = in_ptr[i];
+    for (int i = tid; i < tpp->nnz; i += nt) { ssrc[i] = in_src[i]; sw[i] = in_w[i]; }
+    diag = sdiag; fac = sfac; in_ptr = sptr; in_src = ssrc; in_w = sw;
+  }
 
-  const float* tab = jb.tab;
-  const int* src = jb.from_init ? tp.pop_init : jb.state;
-  for (int i = tid; i < C * AG; i += nt) spop[i] = src[i];
-  for (int i = tid; i < T * G; i += nt) srate[i] = tab[tp.off_rate + i];
-  for (int i = tid; i < G * G; i += nt) sN[i] = tab[tp.off_N + i];
-  for (int i = tid; i < G; i += nt) ssus[i] = tab[tp.off_sus + i];
-  for (int i = tid; i < C; i += nt) sinf[i] = tab[tp.off_inf + i];
-  for (int i = tid; i < T; i += nt) { sfrom[i] = tp.from[i]; sto[i] = tp.to[i]; }
+  const float* __restrict__ tab = jb.tab;
+  {
+    const int* src = jb.from_init ? tpp->pop_init : jb.state;
+    for (int i = tid; i < C * AG; i += nt) popA[i] = src[i];
+    for (int i = tid; i < T * G; i += nt) srate[i] = tab[tpp->off_rate + i];
+    for (int i = tid; i < G * G; i += nt) sN[i] = tab[tpp->off_N + i];
+    for (int i = tid; i < G; i += nt) ssus[i] = tab[tpp->off_sus + i];
+    for (int i = tid; i < C; i += nt) sinf[i] = tab[tpp->off_inf + i];
+    for (int i = tid; i < T; i += nt) { sfrom[i] = tpp->from[i]; sto[i] = tpp->to[i]; }
+    if (tid == 0) qcount[0] = 0;
+  }
   __syncthreads();
 
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   const size_t TAG = (size_t)T * AG, CAG = (size_t)C * AG;
+  int* cur = popA;
+  int* nxt = popB;
 
   for (int s = jb.t0; s < jb.t1; s++) {
     // Phase 1: infectivity of each cell, dinf = sum_c inf_c * pop_c (State::set_I_from_pop).
@@ -92,132 +163,128 @@ __global__ void __launch_bounds__(256) k_fused(const Job* __restrict__ jobs, uin
       float d = 0.f;
       for (int c = 0; c < C; c++) {
         float w = sinf[c];
-        if (w != 0.f) d += w * (float)spop[c * AG + cell];
+        if (w != 0.f) d += w * (float)cur[c * AG + cell];
       }
       sdinf[cell] = d;
     }
     __syncthreads();
     // Phase 2: area-mixing contraction Imap = M_off^T dinf, Idiag = diag * dinf,
-    // and I = (Idiag + Imap) m + fac Idiag (1 - m)  (State::get_NMI first half).
-    const float m = tab[tp.off_geo + s];
+    // and I = (Idiag + Imap) m + fac Idiag (1 - m)  (State::get_NMI, first half).
+    const float m = tab[tpp->off_geo + s];
     for (int cell = tid; cell < AG; cell += nt) {
       int j = cell / G, g = cell - j * G;
-      float idg = tp.diag[j] * sdinf[cell];
+      float idg = diag[j] * sdinf[cell];
       float imp = 0.f;
-      int e1 = tp.in_ptr[j + 1];
-      for (int e = tp.in_ptr[j]; e < e1; e++) imp += tp.in_w[e] * sdinf[tp.in_src[e] * G + g];
-      sI[cell] = (idg + imp) * m + tp.fac[j] * idg * (1.f - m);
+      int e1 = in_ptr[j + 1];
+      for (int e = in_ptr[j]; e < e1; e++) imp += in_w[e] * sdinf[in_src[e] * G + g];
+      sI[cell] = (idg + imp) * m + fac[j] * idg * (1.f - m);
     }
     __syncthreads();
-    // Phase 3: age mixing, transition means, draws and the population update.
-    const float* beta_s = tab + tp.off_beta + (size_t)s * A;
-    const float* efoi_s = tab + tp.off_efoi + (size_t)s * G;
+    // Phase 3a: age mixing, transition means (State::set_transmean), small
+    // draws and the population update (State::update_pop) into the other buffer.
+    const float* beta_s = tab + tpp->off_beta + (size_t)s * A;
+    const float* efoi_s = tab + tpp->off_efoi + (size_t)s * G;
     int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;
     float* tm_s = jb.tm ? jb.tm + (size_t)s * TAG : nullptr;
+    int* pt_s = jb.pt ? jb.pt + (size_t)s * CAG : nullptr;
+    const int* itn = MODE == MODE_MBP ? jb.init_tn + (size_t)s * TAG : nullptr;
+    const float* itm = MODE == MODE_MBP ? jb.init_tm + (size_t)s * TAG : nullptr;
+    const bool update = s < S - 1;           // the last step is not applied (src/state.cc:634)
+    const uint32_t c0 = (uint32_t)s | (stream << 16);
     for (int cell = tid; cell < AG; cell += nt) {
       int j = cell / G, g = cell - j * G;
+      for (int c = 0; c < C; c++) {
+        int v = cur[c * AG + cell];
+        nxt[c * AG + cell] = v;
+        if (pt_s) st_nt(pt_s, c * AG + cell, v);
+      }
       float nmi = 0.f;
       for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * sI[j * G + gg];
       if (nmi < 0.f) nmi = 0.f;
-      const float be = beta_s[j];
-      const float et = efoi_s[g];
-      float lam[TMAX];
-      int nn[TMAX];
-#pragma unroll
-      for (int tr = 0; tr < TMAX; tr++) {
-        lam[tr] = 0.f;
-        nn[tr] = 0;
-        if (tr < T) {
-          float popu = (float)spop[sfrom[tr] * AG + cell];
-          if (popu > 0.f)
-            lam[tr] = tr == inf_tr ? dt * popu * ssus[g] * (be * nmi + et) : dt * popu * srate[tr * G + g];
-          // src/state.cc:429 (NaN mean) and src/utils.cc:150 (mean > 1e9) abort the reference.
-          if (lam[tr] != lam[tr]) { atomicOr(jb.err, 1); lam[tr] = 0.f; }
-          else if (lam[tr] > 1e9f) { atomicOr(jb.err, 2); lam[tr] = 0.f; }
-        }
-      }
-      if (MODE == MODE_REPLAY) {
-#pragma unroll
-        for (int tr = 0; tr < TMAX; tr++)
-          if (tr < T) nn[tr] = tn_s[tr * AG + cell];
-      } else {
-        const int* itn = MODE == MODE_MBP ? jb.init_tn + (size_t)s * TAG : nullptr;
-        const float* itm = MODE == MODE_MBP ? jb.init_tm + (size_t)s * TAG : nullptr;
-        const bool simu = MODE == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
-        const uint32_t c0 = (uint32_t)s | (stream << 16);
-#pragma unroll
-        for (int q = 0; q < (TMAX + 3) / 4; q++) {
-          bool need = false;
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            int tr = 4 * q + r;
-            if (tr < TMAX && tr < T && lam[tr] > 0.f) need = true;
-          }
-          if (!need) continue;
-          u4 blk = philox4x32_10(u4{c0, (uint32_t)cell, (uint32_t)q, jb.key}, k0, k1);
-#pragma unroll
-          for (int r = 0; r < 4; r++) {
-            int tr = 4 * q + r;
-            if (tr >= TMAX || tr >= T) continue;
-            float lp = lam[tr];
-            uint32_t bits = pick(blk, r);
-            ExtraUniforms xu{k0, k1, c0, (uint32_t)cell, jb.key, (uint32_t)tr, 0u, 4, u4{0, 0, 0, 0}};
-            int n = 0;
-            if (MODE == MODE_SIM) {
-              if (lp > 0.f) n = poisson_draw(lp, u01f(bits), u01d(bits), xu);
+      const float foi = beta_s[j] * nmi + efoi_s[g];
+      const bool simu = MODE == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
+      CellRng rng;
+      rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
+#pragma unroll 1
+      for (int tr = 0; tr < T; tr++) {
+        const int fr = sfrom[tr];
+        float popu = (float)cur[fr * AG + cell];
+        float lp = 0.f;
+        if (popu > 0.f) lp = tr == inf_tr ? dt * popu * ssus[g] * foi : dt * popu * srate[tr * G + g];
+        // src/state.cc:429 (NaN mean) and src/utils.cc:150 (mean > 1e9) stop the reference.
+        if (lp != lp) { atomicOr(jb.err, 1); lp = 0.f; }
+        else if (lp > 1e9f) { atomicOr(jb.err, 2); lp = 0.f; }
+        if (tm_s) st_nt(tm_s, tr * AG + cell, lp);
+        int n = 0;
+        int kind = -1;
+        float val = 0.f;
+        if (MODE == MODE_REPLAY) {
+          n = tn_s[tr * AG + cell];
+        } else if (lp > 0.f) {
+          if (MODE == MODE_SIM || simu) {
+            if (lp < BEEP_POIS_INLINE) n = poisson_inversion(lp, rng.next());
+            else { kind = ITEM_POIS; val = lp; }
+          } else {
+            // Coupled proposal (src/mbp.cc:286-311).
+            int ni = itn[tr * AG + cell];
+            float li = itm[tr * AG + cell];
+            if (lp > li) {
+              float dl = lp - li;
+              if (dl < BEEP_POIS_INLINE) n = ni + poisson_inversion(dl, rng.next());
+              else { kind = ITEM_POIS_ADD; val = dl; }
+            } else if (lp == li) {
+              n = ni;
             } else {
-              int ni = itn[tr * AG + cell];
-              if (lp > 0.f) {
-                if (simu) n = poisson_draw(lp, u01f(bits), u01d(bits), xu);
-                else {
-                  float li = itm[tr * AG + cell];
-                  if (lp > li) {
-                    float dl = lp - li;
-                    n = ni + poisson_draw(dl, u01f(bits), u01d(bits), xu);
-                  } else if (lp == li) n = ni;
-                  else n = binomial_draw(ni, (double)lp / (double)li, u01d(bits), xu);
-                }
-              }
+              float p = lp / li;
+              float pp = p > 0.5f ? 1.f - p : p;
+              if ((float)ni * pp < 10.f) n = binomial_draw(ni, p, rng.next(), rng);
+              else { kind = ITEM_BINOM; val = p; }
             }
-            nn[tr] = n;
           }
         }
-        if (MODE == MODE_MBP) {
-          // Cells whose proposed mean is zero draw nothing (src/mbp.cc:289).
-#pragma unroll
-          for (int tr = 0; tr < TMAX; tr++)
-            if (tr < T && !(lam[tr] > 0.f)) nn[tr] = 0;
-        }
-      }
-      // Outputs of this step.
-      if (jb.pt) {
-        int* pt_s = jb.pt + (size_t)s * CAG;
-        for (int c = 0; c < C; c++) pt_s[c * AG + cell] = spop[c * AG + cell];
-      }
-#pragma unroll
-      for (int tr = 0; tr < TMAX; tr++) {
-        if (tr < T) {
-          if (tn_s && MODE != MODE_REPLAY) tn_s[tr * AG + cell] = nn[tr];
-          if (tm_s) tm_s[tr * AG + cell] = lam[tr];
-        }
-      }
-      // State::update_pop: the last step of the horizon is not applied.
-      if (s < S - 1) {
-#pragma unroll
-        for (int tr = 0; tr < TMAX; tr++) {
-          if (tr < T && nn[tr] != 0) {
-            spop[sfrom[tr] * AG + cell] -= nn[tr];
-            spop[sto[tr] * AG + cell] += nn[tr];
+        if (kind >= 0) {
+          int slot = atomicAdd(qcount, 1);
+          if (slot < qcap) {
+            qcode[slot] = (uint32_t)cell | ((uint32_t)tr << 20) | ((uint32_t)kind << 28);
+            qval[slot] = val;
+            continue;                        // drawn in phase 3b
           }
+          int ni = MODE == MODE_MBP ? itn[tr * AG + cell] : 0;
+          n = draw_item(kind, val, ni, k0, k1, c0, (uint32_t)cell, (uint32_t)tr, jb.key);
+        }
+        if (tn_s && MODE != MODE_REPLAY) st_nt(tn_s, tr * AG + cell, n);
+        if (update && n != 0) {
+          nxt[fr * AG + cell] -= n;
+          nxt[sto[tr] * AG + cell] += n;
         }
       }
     }
-    // Phase 1 of the next step writes sdinf, which other threads read in phase
-    // 2 of this step; the barrier after phase 2 already orders those accesses,
-    // and each thread only touches the populations of its own cells.
+    __syncthreads();
+    // Phase 3b: the queued large draws (PTRS / BTRD), every lane busy.
+    if (MODE != MODE_REPLAY) {
+      int nq = qcount[0];
+      if (nq > qcap) nq = qcap;
+      for (int i = tid; i < nq; i += nt) {
+        uint32_t code = qcode[i];
+        int cell = (int)(code & 0xFFFFFu), tr = (int)((code >> 20) & 0xFFu), kind = (int)(code >> 28);
+        int ni = MODE == MODE_MBP ? itn[tr * AG + cell] : 0;
+        int n = draw_item(kind, qval[i], ni, k0, k1, c0, (uint32_t)cell, (uint32_t)tr, jb.key);
+        if (tn_s) st_nt(tn_s, tr * AG + cell, n);
+        if (update && n != 0) {
+          atomicSub(&nxt[sfrom[tr] * AG + cell], n);
+          atomicAdd(&nxt[sto[tr] * AG + cell], n);
+        }
+      }
+      __syncthreads();
+      if (tid == 0) qcount[0] = 0;
+    }
+    int* t = cur;
+    cur = nxt;
+    nxt = t;
   }
+  __syncthreads();
   for (int cell = tid; cell < AG; cell += nt)
-    for (int c = 0; c < C; c++) jb.state[c * AG + cell] = spop[c * AG + cell];
+    for (int c = 0; c < C; c++) jb.state[c * AG + cell] = cur[c * AG + cell];
 }
 
 }  // namespace beepdev

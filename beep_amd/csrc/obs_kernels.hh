@@ -164,24 +164,22 @@ __global__ void k_dbg_philox(int n, const u4* __restrict__ ctr, uint32_t k0, uin
   if (i < n) out[i] = philox4x32_10(ctr[i], k0, k1);
 }
 
-// Draw i uses the counter of transition 0 of cell i at step 0 of instance key.
+// Draw i uses the uniform pool of cell i at step 0 of instance key.
 __global__ void k_dbg_poisson(int n, const float* __restrict__ lam, uint64_t seed, uint32_t key, int* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-  u4 blk = philox4x32_10(u4{0u, (uint32_t)i, 0u, key}, k0, k1);
-  ExtraUniforms xu{k0, k1, 0u, (uint32_t)i, key, 0u, 0u, 4, u4{0, 0, 0, 0}};
-  out[i] = lam[i] > 0.f ? poisson_draw(lam[i], u01f(blk.x), u01d(blk.x), xu) : 0;
+  CellRng rng;
+  rng.init((uint32_t)seed, (uint32_t)(seed >> 32), 0u, (uint32_t)i, key);
+  out[i] = lam[i] > 0.f ? poisson_draw(lam[i], rng.next(), rng) : 0;
 }
 
 __global__ void k_dbg_binomial(int n, const int* __restrict__ nn, const double* __restrict__ p, uint64_t seed,
                                uint32_t key, int* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-  u4 blk = philox4x32_10(u4{0u, (uint32_t)i, 0u, key}, k0, k1);
-  ExtraUniforms xu{k0, k1, 0u, (uint32_t)i, key, 0u, 0u, 4, u4{0, 0, 0, 0}};
-  out[i] = binomial_draw(nn[i], p[i], u01d(blk.x), xu);
+  CellRng rng;
+  rng.init((uint32_t)seed, (uint32_t)(seed >> 32), 0u, (uint32_t)i, key);
+  out[i] = binomial_draw(nn[i], (float)p[i], rng.next(), rng);
 }
 
 }  // namespace beepdev

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libbeep_gpu.so")
 BEEP_OK = 0
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ENAN", -5: "EBOUNDS", -6: "ELARGE"}
 KEEP_TRANSNUM, KEEP_TRANSMEAN, KEEP_POP, KEEP_ALL = 1, 2, 4, 7
+STEPWISE = 16
 SIM_PLAIN = 2
 
 # Exported symbols declared by include/beep_gpu.h.
@@ -23,7 +24,7 @@ SYMBOLS = [
     "beep_get_tables", "beep_simulate", "beep_obs_loglik", "beep_obs_values", "beep_pois_loglik", "beep_mbp",
     "beep_accept", "beep_get_transnum", "beep_get_transmean", "beep_get_pop", "beep_get_state", "beep_put_state",
     "beep_put_transnum", "beep_copy_instances", "beep_synchronize", "beep_debug_philox", "beep_debug_poisson",
-    "beep_debug_binomial", "beep_timing",
+    "beep_debug_binomial", "beep_timing", "beep_debug_gemm",
 ]
 
 
@@ -123,6 +124,7 @@ def lib():
             "beep_debug_philox": (C.c_int, [C.c_int, C.c_int, u32p, C.c_uint32, C.c_uint32, u32p]),
             "beep_debug_poisson": (C.c_int, [C.c_int, C.c_int, f32p, C.c_uint64, C.c_uint32, i32p]),
             "beep_debug_binomial": (C.c_int, [C.c_int, C.c_int, i32p, f64p, C.c_uint64, C.c_uint32, i32p]),
+            "beep_debug_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, f32p, f32p, f32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -446,6 +448,17 @@ def debug_poisson(lam, seed=1, key=0, device=0):
     rc = lib().beep_debug_poisson(device, len(lam), _p(lam, C.c_float), seed, key, _p(out, C.c_int32))
     if rc:
         raise BeepError(rc, "beep_debug_poisson")
+    return out
+
+
+def debug_gemm(A, B, device=0):
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    rows, K = A.shape
+    out = np.zeros((rows, K), dtype=np.float32)
+    rc = lib().beep_debug_gemm(device, rows, K, _p(A, C.c_float), _p(B, C.c_float), _p(out, C.c_float))
+    if rc:
+        raise BeepError(rc, "beep_debug_gemm")
     return out
 
 

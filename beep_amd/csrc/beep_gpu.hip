@@ -14,6 +14,7 @@
 #include "host_model.hh"
 #include "kernels.hh"
 #include "obs_kernels.hh"
+#include "step_kernels.hh"
 
 using beepdev::Job;
 using beepdev::TopoDev;
@@ -57,6 +58,14 @@ struct beep_ctx {
   // timing of the last launch of each kernel kind
   hipEvent_t ev[6] = {};
   double timing[4] = {0, 0, 0, 0};
+  // step-wise path (populations in HBM, one launch per kernel per step)
+  int stepwise = 0, dense = 0, Apad = 0;
+  float* d_M = nullptr;            // [Apad][Apad] dense off-diagonal mixing
+  float* d_D = nullptr;            // [capacity*G][Apad]
+  float* d_I = nullptr;            // [capacity*G][Apad]
+  uint64_t* d_rngp = nullptr;      // seed, stream read by the captured step kernels
+  struct Graph { int mode = -1, n = 0, t0 = 0, t1 = 0; const Job* jobs = nullptr; hipGraphExec_t exec = nullptr; };
+  Graph graph;
 };
 
 namespace {
@@ -106,6 +115,10 @@ std::vector<float> to_f32(const std::vector<double>& v) { return std::vector<flo
 int ensure_jobs(beep_ctx* c, size_t n) {
   if (n <= c->jobs_cap) return BEEP_OK;
   if (c->d_jobs) {
+    if (c->graph.exec) {          // the captured graph refers to the old job buffer
+      HIPCHK(c, hipGraphExecDestroy(c->graph.exec));
+      c->graph.exec = nullptr;
+    }
     HIPCHK(c, hipFree(c->d_jobs));
     c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->d_jobs));
     c->allocated -= c->jobs_cap * sizeof(Job);
@@ -171,6 +184,60 @@ int launch_fused_t(beep_ctx* c, const Job* djobs, int njobs, size_t lds, int blo
   if (lds > 65536) HIPCHK(c, hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kfn, dim3(njobs), dim3(block), lds, c->stream, djobs, seed, stream);
   HIPCHK(c, hipGetLastError());
+  return BEEP_OK;
+}
+
+// The step-wise path over [t0, t1) for the n jobs at djobs: initial
+// infectivity, then per step the contraction (MFMA GEMM when dense) and the
+// transition kernel.  The launch sequence is captured once into a hipGraph and
+// replayed while (mode, n, t0, t1, jobs) stay the same; seed and stream are
+// read from device memory so one graph serves every call.
+int launch_stepwise(beep_ctx* c, int mode, const Job* djobs, int n, int t0, int t1, uint64_t seed, uint32_t stream) {
+  const auto& m = c->m;
+  uint64_t rp[2] = {seed, (uint64_t)stream};
+  HIPCHK(c, hipMemcpyAsync(c->d_rngp, rp, sizeof rp, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  auto& G = c->graph;
+  if (G.exec && G.mode == mode && G.n == n && G.t0 == t0 && G.t1 == t1 && G.jobs == djobs) {
+    HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
+    return BEEP_OK;
+  }
+  if (G.exec) {
+    HIPCHK(c, hipGraphExecDestroy(G.exec));
+    G.exec = nullptr;
+  }
+  const int rows = n * m.G;
+  dim3 gcell((m.AG + BEEP_STEP_BLOCK - 1) / BEEP_STEP_BLOCK, n);
+  hipGraph_t graph;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipLaunchKernelGGL(beepdev::k_step_init, gcell, dim3(BEEP_STEP_BLOCK), 0, c->stream, djobs, c->d_D, c->Apad);
+  for (int s = t0; s < t1; s++) {
+    if (c->dense) {
+      beepdev::DenseMix p{c->d_D, c->d_M, c->d_I, c->h_topo.diag, c->h_topo.fac, djobs, rows, m.A, c->Apad, m.G, s,
+                          c->h_topo.off_geo};
+      hipLaunchKernelGGL(beepdev::k_dense_contract, dim3(c->Apad / 64, (rows + 63) / 64), dim3(256), 0, c->stream, p);
+    } else {
+      beepdev::SparseMix p{c->d_D, c->d_I, c->h_topo.diag, c->h_topo.fac, c->h_topo.in_ptr, c->h_topo.in_src,
+                           c->h_topo.in_w, djobs, rows, m.A, c->Apad, m.G, s, c->h_topo.off_geo};
+      hipLaunchKernelGGL(beepdev::k_sparse_contract, dim3((m.A + 255) / 256, rows), dim3(256), 0, c->stream, p);
+    }
+    if (mode == beepdev::MODE_SIM)
+      hipLaunchKernelGGL(beepdev::k_step_trans<beepdev::MODE_SIM>, gcell, dim3(BEEP_STEP_BLOCK), 0, c->stream, djobs,
+                         c->d_I, c->d_D, c->Apad, s, c->d_rngp);
+    else if (mode == beepdev::MODE_MBP)
+      hipLaunchKernelGGL(beepdev::k_step_trans<beepdev::MODE_MBP>, gcell, dim3(BEEP_STEP_BLOCK), 0, c->stream, djobs,
+                         c->d_I, c->d_D, c->Apad, s, c->d_rngp);
+    else
+      hipLaunchKernelGGL(beepdev::k_step_trans<beepdev::MODE_REPLAY>, gcell, dim3(BEEP_STEP_BLOCK), 0, c->stream, djobs,
+                         c->d_I, c->d_D, c->Apad, s, c->d_rngp);
+  }
+  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (e != hipSuccess) return fail(c, BEEP_EHIP, "graph capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(c, BEEP_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+  G.mode = mode; G.n = n; G.t0 = t0; G.t1 = t1; G.jobs = djobs;
+  HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
   return BEEP_OK;
 }
 
@@ -323,6 +390,28 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   if ((rc = dalloc(c, (void**)&c->d_topo, sizeof(TopoDev)))) return rc;
   HIPCHK(c, hipMemcpy(c->d_topo, &tp, sizeof tp, hipMemcpyHostToDevice));
 
+  // Path: the fused per-instance kernel when an instance fits in LDS and the
+  // mixing is sparse; otherwise the step-wise path, with the contraction on
+  // the matrix cores when the mixing is dense.
+  if (m.C > 16) return fail(c, BEEP_EINVAL, "at most 16 compartments are supported");
+  c->dense = (size_t)tp.nnz * 4 >= (size_t)m.A * m.A && m.A >= 32;
+  size_t lds = beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, tp.stage_mix);
+  c->stepwise = (keep & BEEP_STEPWISE) || c->dense || lds > 160 * 1024;
+  c->Apad = (m.A + 63) / 64 * 64;
+  if (c->dense) {
+    std::vector<float> M((size_t)c->Apad * c->Apad, 0.f);
+    for (int a = 0; a < m.A; a++)
+      for (int k = m.M_rowptr[a]; k < m.M_rowptr[a + 1]; k++) M[(size_t)a * c->Apad + m.M_to[k]] += (float)m.M_val[k];
+    if ((rc = upload(c, &c->d_M, M))) return rc;
+  }
+  if (c->stepwise) {
+    size_t sz = (size_t)capacity * m.G * c->Apad * sizeof(float);
+    if ((rc = dalloc(c, (void**)&c->d_D, sz)) || (rc = dalloc(c, (void**)&c->d_I, sz))) return rc;
+    HIPCHK(c, hipMemset(c->d_D, 0, sz));
+    HIPCHK(c, hipMemset(c->d_I, 0, sz));
+    if ((rc = dalloc(c, (void**)&c->d_rngp, 2 * sizeof(uint64_t)))) return rc;
+  }
+
   // Slots and instances.
   if ((rc = dalloc(c, (void**)&c->d_tab, c->slot_stride * nslot * sizeof(float)))) return rc;
   HIPCHK(c, hipMemset(c->d_tab, 0, c->slot_stride * nslot * sizeof(float)));
@@ -387,6 +476,7 @@ int beep_destroy(beep_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return BEEP_OK;
@@ -471,8 +561,11 @@ int beep_simulate(beep_ctx* c, int n, const int32_t* inst, const int32_t* slot, 
       for (int s = t0; s < t1; s++) jobs.push_back(make_job(c, inst[i], slot[i], key[i], s, s + 1));
     if ((rc = put_jobs(c, jobs))) return rc;
     record(c, 0);
-    for (size_t k = 0; k < jobs.size(); k++)
-      if ((rc = launch_fused(c, beepdev::MODE_SIM, c->d_jobs + k, 1, seed, stream))) return rc;
+    for (size_t k = 0; k < jobs.size(); k++) {
+      if (c->stepwise) rc = launch_stepwise(c, beepdev::MODE_SIM, c->d_jobs + k, 1, jobs[k].t0, jobs[k].t1, seed, stream);
+      else rc = launch_fused(c, beepdev::MODE_SIM, c->d_jobs + k, 1, seed, stream);
+      if (rc) return rc;
+    }
     record(c, 1);
     c->timing[3] = (double)jobs.size();
   } else {
@@ -483,9 +576,11 @@ int beep_simulate(beep_ctx* c, int n, const int32_t* inst, const int32_t* slot, 
     for (int i = 0; i < n; i++) jobs.push_back(make_job(c, inst[i], slot[i], key[i], t0, t1));
     if ((rc = put_jobs(c, jobs))) return rc;
     record(c, 0);
-    if ((rc = launch_fused(c, beepdev::MODE_SIM, c->d_jobs, n, seed, stream))) return rc;
+    if (c->stepwise) rc = launch_stepwise(c, beepdev::MODE_SIM, c->d_jobs, n, t0, t1, seed, stream);
+    else rc = launch_fused(c, beepdev::MODE_SIM, c->d_jobs, n, seed, stream);
+    if (rc) return rc;
     record(c, 1);
-    c->timing[3] = 1;
+    c->timing[3] = c->stepwise ? 2 * (t1 - t0) + 1 : 1;
   }
   rc = check_device_error(c);
   c->timing[0] = elapsed(c, 0, 1);
@@ -525,7 +620,9 @@ int beep_mbp(beep_ctx* c, int n, const int32_t* init, const int32_t* prop, const
   }
   if ((rc = put_jobs(c, jobs))) return rc;
   record(c, 0);
-  if ((rc = launch_fused(c, beepdev::MODE_MBP, c->d_jobs, n, seed, stream))) return rc;
+  if (c->stepwise) rc = launch_stepwise(c, beepdev::MODE_MBP, c->d_jobs, n, 0, m.S, seed, stream);
+  else rc = launch_fused(c, beepdev::MODE_MBP, c->d_jobs, n, seed, stream);
+  if (rc) return rc;
   record(c, 1);
   rc = check_device_error(c);
   c->timing[0] = elapsed(c, 0, 1);
@@ -734,7 +831,9 @@ int beep_put_transnum(beep_ctx* c, int inst, int slot, const int32_t* tn) {
   std::vector<Job> jobs(1, make_job(c, inst, slot, 0, 0, m.S));
   int rc;
   if ((rc = put_jobs(c, jobs))) return rc;
-  if ((rc = launch_fused(c, beepdev::MODE_REPLAY, c->d_jobs, 1, 0, 0))) return rc;
+  if (c->stepwise) rc = launch_stepwise(c, beepdev::MODE_REPLAY, c->d_jobs, 1, 0, m.S, 0, 0);
+  else rc = launch_fused(c, beepdev::MODE_REPLAY, c->d_jobs, 1, 0, 0);
+  if (rc) return rc;
   return check_device_error(c);
 }
 
@@ -791,6 +890,15 @@ int beep_debug_poisson(int device, int n, const float* lam, uint64_t seed, uint3
     hipLaunchKernelGGL(beepdev::k_dbg_poisson, dim3((n + 255) / 256), dim3(256), 0, 0, n, (const float*)a, seed, key,
                        (int*)o);
   });
+}
+
+int beep_debug_gemm(int device, int rows, int K, const float* A, const float* B, float* C) {
+  if (rows <= 0 || K <= 0 || K % 64 || !A || !B || !C) return BEEP_EINVAL;
+  return debug_run(device, (size_t)rows * K * 4, A, (size_t)K * K * 4, B, (size_t)rows * K * 4, C,
+                   [&](void* a, void* b, void* o) {
+                     hipLaunchKernelGGL(beepdev::k_dbg_gemm, dim3(K / 64, (rows + 63) / 64), dim3(256), 0, 0,
+                                        (const float*)a, (const float*)b, (float*)o, rows, K);
+                   });
 }
 
 int beep_debug_binomial(int device, int n, const int32_t* nn, const double* p, uint64_t seed, uint32_t key,

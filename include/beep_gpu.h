@@ -155,6 +155,11 @@ typedef struct beep_ctx beep_ctx;
 #define BEEP_KEEP_TRANSMEAN  2
 #define BEEP_KEEP_POP        4
 #define BEEP_KEEP_ALL        7
+/* Option bit of `keep`: always use the step-wise path (populations in HBM,
+ * one launch per kernel per step, captured in a hipGraph) even when an
+ * instance fits the fused per-instance kernel.  The step-wise path is chosen
+ * automatically for dense mixing and for instances too large for LDS. */
+#define BEEP_STEPWISE       16
 
 /* Builds Data::generate_matrices' normalised mixing (src/data_matrix.cc:14-202),
  * the observation index (ObservationModel::initialise_obs_change, obsmodel.cc:174-217)
@@ -266,6 +271,9 @@ int beep_debug_philox(int device, int n, const uint32_t* ctr, uint32_t k0, uint3
 int beep_debug_poisson(int device, int n, const float* lam, uint64_t seed, uint32_t key, int32_t* out);
 int beep_debug_binomial(int device, int n, const int32_t* nn, const double* p, uint64_t seed, uint32_t key,
                         int32_t* out);
+/* C[rows][K] = A[rows][K] x B[K][K] with the dense contraction's MFMA tiling
+ * (K a multiple of 64), FP32. */
+int beep_debug_gemm(int device, int rows, int K, const float* A, const float* B, float* C);
 
 #ifdef __cplusplus
 }

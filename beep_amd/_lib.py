@@ -24,7 +24,7 @@ SYMBOLS = [
     "beep_get_tables", "beep_simulate", "beep_obs_loglik", "beep_obs_values", "beep_pois_loglik", "beep_mbp",
     "beep_accept", "beep_get_transnum", "beep_get_transmean", "beep_get_pop", "beep_get_state", "beep_put_state",
     "beep_put_transnum", "beep_copy_instances", "beep_synchronize", "beep_debug_philox", "beep_debug_poisson",
-    "beep_debug_binomial", "beep_timing", "beep_debug_gemm",
+    "beep_debug_binomial", "beep_timing", "beep_debug_gemm", "beep_run_batch",
 ]
 
 
@@ -125,6 +125,7 @@ def lib():
             "beep_debug_poisson": (C.c_int, [C.c_int, C.c_int, f32p, C.c_uint64, C.c_uint32, i32p]),
             "beep_debug_binomial": (C.c_int, [C.c_int, C.c_int, i32p, f64p, C.c_uint64, C.c_uint32, i32p]),
             "beep_debug_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, f32p, f32p, f32p]),
+            "beep_run_batch": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, f64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -279,6 +280,30 @@ def spec_tables(spec, paramval):
     out["Pr"] = float(Pr[0])
     out["inbounds"] = bool(inb[0])
     return out
+
+
+class Work(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("kind", C.c_int32), ("inst", C.c_int32), ("init", C.c_int32),
+                ("slot", C.c_int32), ("key", C.c_uint32), ("t0", C.c_int32), ("t1", C.c_int32)]
+
+
+WORK_SIMULATE, WORK_MBP = 0, 1
+BATCH_AUTOTUNE = 1
+
+
+def run_batch(items, seed=1, stream=0, autotune=False):
+    """Runs (ctx, kind, inst, init, slot, key, t0, t1) work items across
+    contexts in one batch; returns (wall ms, fused launches, step-wise
+    contexts, cell-steps)."""
+    n = len(items)
+    arr = (Work * n)()
+    for i, (ctx, kind, inst, init, slot, key, t0, t1) in enumerate(items):
+        arr[i] = Work(ctx._h.value, kind, inst, init, slot, key, t0, t1)
+    stats = np.zeros(4)
+    rc = lib().beep_run_batch(n, arr, seed, stream, BATCH_AUTOTUNE if autotune else 0, _p(stats, C.c_double))
+    if rc != BEEP_OK:
+        raise BeepError(rc, lib().beep_last_error(C.c_void_p(items[0][0]._h.value)).decode())
+    return stats
 
 
 class Context:

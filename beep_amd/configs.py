@@ -292,3 +292,26 @@ def seir_areas(A, G, period, seed, obs_every=0):
 
 def steps(spec):
     return nsteps(spec)
+
+
+def uneven_batch(n=16384, seed=4, frac_mc3=0.5):
+    """Instances of the SEIR-in-every-area model with area counts uniform in
+    10..400, horizons uniform in 50..400 days and age groups in {1, 4, 9}.
+    Instances with the same (areas, age groups) share one topology.  Returns
+    a list of groups {A, G, S, sim_horizons, n_mc3}: the ABC-SMC particles
+    (fresh simulations over their own horizons) and the MC3 chains (MBP
+    proposals over the group's horizon S, the longest in the group)."""
+    rng = np.random.default_rng(seed)
+    A = rng.integers(10, 401, n)
+    S = rng.integers(50, 401, n)
+    G = rng.choice([1, 4, 9], n)
+    mc3 = rng.random(n) < frac_mc3
+    groups = {}
+    for a, s, g, m in zip(A, S, G, mc3):
+        d = groups.setdefault((int(a), int(g)), {"A": int(a), "G": int(g), "S": 0, "sim_horizons": [], "n_mc3": 0})
+        d["S"] = max(d["S"], int(s))
+        if m:
+            d["n_mc3"] += 1
+        else:
+            d["sim_horizons"].append(int(s))
+    return [groups[k] for k in sorted(groups)]

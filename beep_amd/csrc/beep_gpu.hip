@@ -3,6 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -53,6 +59,8 @@ struct beep_ctx {
   // staging
   Job* d_jobs = nullptr;
   size_t jobs_cap = 0;
+  Job* d_bjobs = nullptr;          // fused jobs of mixed batches launched from this context
+  size_t bjobs_cap = 0;
   void* d_scratch = nullptr;
   size_t scratch_cap = 0;
   // timing of the last launch of each kernel kind
@@ -246,6 +254,7 @@ int launch_fused(beep_ctx* c, int mode, const Job* djobs, int njobs, uint64_t se
   size_t lds = beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, c->h_topo.nnz, c->h_topo.stage_mix);
   if (lds > 160 * 1024) return fail(c, BEEP_EINVAL, "instance needs %zu bytes of LDS (max 163840)", lds);
   int block = fused_block(m.AG);
+  if (mode == beepdev::MODE_ANY) return launch_fused_t<beepdev::MODE_ANY>(c, djobs, njobs, lds, block, seed, stream);
   if (mode == beepdev::MODE_SIM) return launch_fused_t<beepdev::MODE_SIM>(c, djobs, njobs, lds, block, seed, stream);
   if (mode == beepdev::MODE_MBP) return launch_fused_t<beepdev::MODE_MBP>(c, djobs, njobs, lds, block, seed, stream);
   return launch_fused_t<beepdev::MODE_REPLAY>(c, djobs, njobs, lds, block, seed, stream);
@@ -265,6 +274,7 @@ Job make_job(beep_ctx* c, int inst, int slot, uint32_t key, int t0, int t1) {
   j.t0 = t0;
   j.t1 = t1;
   j.from_init = t0 == 0 ? 1 : 0;
+  j.mode = beepdev::MODE_SIM;
   return j;
 }
 
@@ -613,6 +623,7 @@ int beep_mbp(beep_ctx* c, int n, const int32_t* init, const int32_t* prop, const
   jobs.reserve(n);
   for (int i = 0; i < n; i++) {
     Job j = make_job(c, prop[i], slot[i], key[i], 0, m.S);
+    j.mode = beepdev::MODE_MBP;
     j.init_tn = c->d_tn + (size_t)init[i] * c->traj_sz;
     j.init_tm = c->d_tm + (size_t)init[i] * c->traj_sz;
     j.simu = d_mask ? d_mask + mask_sz * i : nullptr;
@@ -829,6 +840,7 @@ int beep_put_transnum(beep_ctx* c, int inst, int slot, const int32_t* tn) {
   HIPCHK(c, hipMemcpyAsync(c->d_tn + (size_t)inst * c->traj_sz, h.data(), c->traj_sz * sizeof(int),
                            hipMemcpyHostToDevice, c->stream));
   std::vector<Job> jobs(1, make_job(c, inst, slot, 0, 0, m.S));
+  jobs[0].mode = beepdev::MODE_REPLAY;
   int rc;
   if ((rc = put_jobs(c, jobs))) return rc;
   if (c->stepwise) rc = launch_stepwise(c, beepdev::MODE_REPLAY, c->d_jobs, 1, 0, m.S, 0, 0);
@@ -862,6 +874,202 @@ int beep_copy_instances(beep_ctx* c, int n, const int32_t* dst, const int32_t* s
 int beep_timing(beep_ctx* c, double* out) {
   if (!c || !out) return BEEP_EINVAL;
   for (int k = 0; k < 4; k++) out[k] = c->timing[k];
+  return BEEP_OK;
+}
+
+// ---- mixed batches ----
+
+namespace {
+struct TuneKey {
+  int ag2, C, T;
+  bool operator<(const TuneKey& o) const { return std::tie(ag2, C, T) < std::tie(o.ag2, o.C, o.T); }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+bool g_tune_loaded = false;
+
+const char* tune_path() {
+  const char* p = std::getenv("BEEP_TUNE_CACHE");
+  return (p && *p) ? p : nullptr;
+}
+
+void tune_load() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const char* p = tune_path();
+  if (!p) return;
+  std::ifstream f(p);
+  TuneKey k;
+  int b;
+  while (f >> k.ag2 >> k.C >> k.T >> b) g_tune[k] = b;
+}
+
+void tune_save() {
+  const char* p = tune_path();
+  if (!p) return;
+  std::ofstream f(p);
+  for (auto& kv : g_tune) f << kv.first.ag2 << " " << kv.first.C << " " << kv.first.T << " " << kv.second << "\n";
+}
+
+int pow2_ceil(int x) {
+  int p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+}  // namespace
+
+int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, int flags, double* stats) {
+  if (n <= 0 || !w || !w[0].ctx) return BEEP_EINVAL;
+  if (stream > 0xFFFF) return BEEP_EINVAL;
+  beep_ctx* c0 = w[0].ctx;
+  auto t_start = std::chrono::steady_clock::now();
+  // Validate and build jobs.
+  struct Item { Job job; beep_ctx* ctx; size_t lds; double cost; int AG; };
+  std::vector<Item> fused;
+  std::map<beep_ctx*, std::vector<Job>> step_sim, step_mbp;
+  double cellsteps = 0;
+  for (int i = 0; i < n; i++) {
+    beep_ctx* c = w[i].ctx;
+    if (!c) return fail(c0, BEEP_EINVAL, "work %d has no context", i);
+    if (c->device != c0->device) return fail(c0, BEEP_EINVAL, "batch contexts must share one device");
+    const auto& m = c->m;
+    if (!valid_inst(c, w[i].inst) || !valid_slot(c, w[i].slot) || !c->slot_set[w[i].slot])
+      return fail(c0, BEEP_EINVAL, "work %d: bad instance or slot", i);
+    Job j;
+    if (w[i].kind == BEEP_WORK_SIMULATE) {
+      if (w[i].t0 < 0 || w[i].t1 > m.S || w[i].t0 > w[i].t1) return fail(c0, BEEP_EINVAL, "work %d: bad steps", i);
+      j = make_job(c, w[i].inst, w[i].slot, w[i].key, w[i].t0, w[i].t1);
+    } else if (w[i].kind == BEEP_WORK_MBP) {
+      if (!c->d_tn || !c->d_tm || !valid_inst(c, w[i].init) || w[i].init == w[i].inst)
+        return fail(c0, BEEP_EINVAL, "work %d: bad MBP pair", i);
+      j = make_job(c, w[i].inst, w[i].slot, w[i].key, 0, m.S);
+      j.mode = beepdev::MODE_MBP;
+      j.init_tn = c->d_tn + (size_t)w[i].init * c->traj_sz;
+      j.init_tm = c->d_tm + (size_t)w[i].init * c->traj_sz;
+    } else {
+      return fail(c0, BEEP_EINVAL, "work %d: unknown kind", i);
+    }
+    cellsteps += (double)m.AG * (j.t1 - j.t0);
+    if (j.t0 == j.t1) continue;
+    if (c->stepwise) {
+      (j.mode == beepdev::MODE_MBP ? step_mbp : step_sim)[c].push_back(j);
+    } else {
+      size_t lds = beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, c->h_topo.nnz, c->h_topo.stage_mix);
+      fused.push_back({j, c, lds, (double)m.AG * m.T * (j.t1 - j.t0), m.AG});
+    }
+  }
+  int rc;
+  int nlaunch = 0;
+  // Step-wise contexts first, each on its own stream so they overlap with one
+  // another and with the fused launches below.
+  int nstep = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    auto& mp = pass == 0 ? step_sim : step_mbp;
+    for (auto& kv : mp) {
+      beep_ctx* c = kv.first;
+      auto& jobs = kv.second;
+      int t0 = jobs[0].t0, t1 = jobs[0].t1;
+      for (auto& j : jobs) { t0 = std::min(t0, j.t0); t1 = std::max(t1, j.t1); }
+      if (pass == 1 && step_sim.count(c)) HIPCHK(c, hipStreamSynchronize(c->stream));   // job buffer reuse
+      if ((rc = put_jobs(c, jobs))) return rc;
+      if ((rc = launch_stepwise(c, pass == 0 ? beepdev::MODE_SIM : beepdev::MODE_MBP, c->d_jobs, (int)jobs.size(), t0,
+                                t1, seed, stream)))
+        return rc;
+      nstep++;
+    }
+  }
+  if (!fused.empty()) {
+    // Size classes, each sorted longest first.
+    std::map<TuneKey, std::vector<Item>> groups;
+    for (auto& it : fused) groups[{pow2_ceil(it.AG), it.ctx->m.C, it.ctx->m.T}].push_back(it);
+    std::vector<Job> all;
+    std::vector<std::tuple<TuneKey, size_t, size_t, size_t, int>> launches;   // key, offset, count, lds, maxAG
+    for (auto& kv : groups) {
+      auto& v = kv.second;
+      std::stable_sort(v.begin(), v.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
+      size_t lds = 0;
+      int maxAG = 0;
+      for (auto& it : v) { lds = std::max(lds, it.lds); maxAG = std::max(maxAG, it.AG); }
+      launches.emplace_back(kv.first, all.size(), v.size(), lds, maxAG);
+      for (auto& it : v) all.push_back(it.job);
+    }
+    if (all.size() > c0->bjobs_cap) {
+      if (c0->d_bjobs) {
+        HIPCHK(c0, hipStreamSynchronize(c0->stream));
+        HIPCHK(c0, hipFree(c0->d_bjobs));
+        c0->allocs.erase(std::find(c0->allocs.begin(), c0->allocs.end(), (void*)c0->d_bjobs));
+        c0->allocated -= c0->bjobs_cap * sizeof(Job);
+        c0->d_bjobs = nullptr;
+        c0->bjobs_cap = 0;
+      }
+      size_t cap = std::max<size_t>(all.size(), 1024);
+      if ((rc = dalloc(c0, (void**)&c0->d_bjobs, cap * sizeof(Job)))) return rc;
+      c0->bjobs_cap = cap;
+    }
+    HIPCHK(c0, hipMemcpyAsync(c0->d_bjobs, all.data(), all.size() * sizeof(Job), hipMemcpyHostToDevice, c0->stream));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    tune_load();
+    bool tuned_any = false;
+    for (auto& L : launches) {
+      const TuneKey key = std::get<0>(L);
+      size_t off = std::get<1>(L), cnt = std::get<2>(L), lds = std::get<3>(L);
+      int maxAG = std::get<4>(L);
+      int block = fused_block(maxAG);
+      auto f = g_tune.find(key);
+      if (f != g_tune.end()) block = f->second;
+      else if (flags & BEEP_BATCH_AUTOTUNE) {
+        // Same keys give the same trajectories whatever the workgroup size, so
+        // every candidate run leaves valid results; the fastest is kept.
+        float best = 1e30f;
+        for (int cand = 64; cand <= BEEP_FUSED_MAX_BLOCK; cand *= 2) {
+          if (cand > fused_block(maxAG) && cand > 64) break;
+          record(c0, 0);
+          if ((rc = launch_fused_t<beepdev::MODE_ANY>(c0, c0->d_bjobs + off, (int)cnt, lds, cand, seed, stream))) return rc;
+          record(c0, 1);
+          HIPCHK(c0, hipStreamSynchronize(c0->stream));
+          float ms = (float)elapsed(c0, 0, 1);
+          nlaunch++;
+          if (ms < best) { best = ms; block = cand; }
+        }
+        int fb = fused_block(maxAG);
+        if (fb > block && (fb & (fb - 1))) {   // the one-pass size when it is not a power of two
+          record(c0, 0);
+          if ((rc = launch_fused_t<beepdev::MODE_ANY>(c0, c0->d_bjobs + off, (int)cnt, lds, fb, seed, stream))) return rc;
+          record(c0, 1);
+          HIPCHK(c0, hipStreamSynchronize(c0->stream));
+          float ms = (float)elapsed(c0, 0, 1);
+          nlaunch++;
+          if (ms < best) { best = ms; block = fb; }
+        }
+        g_tune[key] = block;
+        tuned_any = true;
+        continue;
+      }
+      if ((rc = launch_fused_t<beepdev::MODE_ANY>(c0, c0->d_bjobs + off, (int)cnt, lds, block, seed, stream))) return rc;
+      nlaunch++;
+    }
+    if (tuned_any) tune_save();
+    HIPCHK(c0, hipStreamSynchronize(c0->stream));   // `all` is the source of an async copy
+  }
+  for (auto& kv : step_sim) HIPCHK(kv.first, hipStreamSynchronize(kv.first->stream));
+  for (auto& kv : step_mbp) HIPCHK(kv.first, hipStreamSynchronize(kv.first->stream));
+  HIPCHK(c0, hipStreamSynchronize(c0->stream));   // also keeps `all` alive until its copy is done
+  // Errors raised by any context's kernels.
+  std::map<beep_ctx*, int> seen;
+  for (int i = 0; i < n; i++) {
+    if (seen.count(w[i].ctx)) continue;
+    seen[w[i].ctx] = 1;
+    if ((rc = check_device_error(w[i].ctx))) {
+      if (w[i].ctx != c0) c0->err = w[i].ctx->err;
+      return rc;
+    }
+  }
+  if (stats) {
+    stats[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    stats[1] = nlaunch;
+    stats[2] = nstep;
+    stats[3] = cellsteps;
+  }
   return BEEP_OK;
 }
 

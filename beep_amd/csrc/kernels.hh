@@ -47,9 +47,10 @@ struct Job {
   uint32_t key;
   int t0, t1;
   int from_init;              // 1: start from topo->pop_init
+  int mode;                   // MODE_SIM / MODE_MBP / MODE_REPLAY (read by MODE_ANY launches)
 };
 
-enum { MODE_SIM = 0, MODE_MBP = 1, MODE_REPLAY = 2 };
+enum { MODE_SIM = 0, MODE_MBP = 1, MODE_REPLAY = 2, MODE_ANY = 3 };
 
 // Trajectory outputs are written once and read by later kernels: global
 // address space, non-temporal (streaming) stores.
@@ -59,7 +60,7 @@ __device__ inline void st_nt(int* p, size_t i, int v) { __builtin_nontemporal_st
 __device__ inline void st_nt(float* p, size_t i, float v) { __builtin_nontemporal_store(v, (gfloat*)p + i); }
 
 // Queue capacity of large draws per step (items beyond it are drawn in place).
-__host__ __device__ inline int fused_qcap(int AG) { return AG < 32 ? 64 : 2 * AG; }
+__host__ __device__ inline int fused_qcap(int AG) { return AG < 32 ? 64 : (2 * AG < 1024 ? 2 * AG : 1024); }
 
 // Bytes of the mixing CSR when staged in LDS.
 __host__ __device__ inline size_t fused_mix_bytes(int A, int nnz) { return (size_t)4 * (3 * (size_t)A + 1 + 2 * (size_t)nnz); }
@@ -102,6 +103,8 @@ template <int MODE>
 __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
     k_fused(const Job* __restrict__ jobs, uint64_t seed, uint32_t stream) {
   const Job jb = jobs[blockIdx.x];
+  // MODE_ANY launches mix simulations and proposals: the job says which.
+  const int mode = MODE == MODE_ANY ? jb.mode : MODE;
   const TopoDev* __restrict__ tpp = jb.topo;
   const int A = tpp->A, G = tpp->G, C = tpp->C, T = tpp->T, S = tpp->S, AG = tpp->AG, inf_tr = tpp->inf_tr;
   const int nt = blockDim.x, tid = threadIdx.x;
@@ -187,8 +190,8 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
     int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;
     float* tm_s = jb.tm ? jb.tm + (size_t)s * TAG : nullptr;
     int* pt_s = jb.pt ? jb.pt + (size_t)s * CAG : nullptr;
-    const int* itn = MODE == MODE_MBP ? jb.init_tn + (size_t)s * TAG : nullptr;
-    const float* itm = MODE == MODE_MBP ? jb.init_tm + (size_t)s * TAG : nullptr;
+    const int* itn = mode == MODE_MBP ? jb.init_tn + (size_t)s * TAG : nullptr;
+    const float* itm = mode == MODE_MBP ? jb.init_tm + (size_t)s * TAG : nullptr;
     const bool update = s < S - 1;           // the last step is not applied (src/state.cc:634)
     const uint32_t c0 = (uint32_t)s | (stream << 16);
     for (int cell = tid; cell < AG; cell += nt) {
@@ -202,7 +205,7 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
       for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * sI[j * G + gg];
       if (nmi < 0.f) nmi = 0.f;
       const float foi = beta_s[j] * nmi + efoi_s[g];
-      const bool simu = MODE == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
+      const bool simu = mode == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
       CellRng rng;
       rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
 #pragma unroll 1
@@ -218,10 +221,10 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
         int n = 0;
         int kind = -1;
         float val = 0.f;
-        if (MODE == MODE_REPLAY) {
+        if (mode == MODE_REPLAY) {
           n = tn_s[tr * AG + cell];
         } else if (lp > 0.f) {
-          if (MODE == MODE_SIM || simu) {
+          if (mode == MODE_SIM || simu) {
             if (lp < BEEP_POIS_INLINE) n = poisson_inversion(lp, rng.next());
             else { kind = ITEM_POIS; val = lp; }
           } else {
@@ -249,10 +252,10 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
             qval[slot] = val;
             continue;                        // drawn in phase 3b
           }
-          int ni = MODE == MODE_MBP ? itn[tr * AG + cell] : 0;
+          int ni = mode == MODE_MBP ? itn[tr * AG + cell] : 0;
           n = draw_item(kind, val, ni, k0, k1, c0, (uint32_t)cell, (uint32_t)tr, jb.key);
         }
-        if (tn_s && MODE != MODE_REPLAY) st_nt(tn_s, tr * AG + cell, n);
+        if (tn_s && mode != MODE_REPLAY) st_nt(tn_s, tr * AG + cell, n);
         if (update && n != 0) {
           nxt[fr * AG + cell] -= n;
           nxt[sto[tr] * AG + cell] += n;
@@ -261,13 +264,13 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
     }
     __syncthreads();
     // Phase 3b: the queued large draws (PTRS / BTRD), every lane busy.
-    if (MODE != MODE_REPLAY) {
+    if (mode != MODE_REPLAY) {
       int nq = qcount[0];
       if (nq > qcap) nq = qcap;
       for (int i = tid; i < nq; i += nt) {
         uint32_t code = qcode[i];
         int cell = (int)(code & 0xFFFFFu), tr = (int)((code >> 20) & 0xFFu), kind = (int)(code >> 28);
-        int ni = MODE == MODE_MBP ? itn[tr * AG + cell] : 0;
+        int ni = mode == MODE_MBP ? itn[tr * AG + cell] : 0;
         int n = draw_item(kind, qval[i], ni, k0, k1, c0, (uint32_t)cell, (uint32_t)tr, jb.key);
         if (tn_s) st_nt(tn_s, tr * AG + cell, n);
         if (update && n != 0) {

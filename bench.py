@@ -60,6 +60,8 @@ def main():
     ap.add_argument("--particles", type=int, default=PARTICLES)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="spatial", choices=["spatial", "uneven", "dense"])
+    ap.add_argument("--instances", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path shards by particle, "
@@ -68,6 +70,10 @@ def main():
 
     from beep_amd import _lib, configs, spec as specmod
 
+    if args.workload == "uneven":
+        return bench_uneven(args)
+    if args.workload == "dense":
+        return bench_dense(args)
     spec, _truth = configs.spatial_abcsmc()
     P = args.particles
     rng = np.random.default_rng(20250102)
@@ -136,6 +142,74 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, params)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_uneven(args):
+    """Mixed batch: ABC-SMC particles (fresh simulations over their own
+    horizons) and MC3 chains (MBP proposals coupled to their current
+    trajectories) of SEIR-in-every-area models with 10..400 areas, 1/4/9 age
+    groups and 50..400-day horizons, one batch call per sweep."""
+    import time
+    from beep_amd import _lib, configs
+    from beep_amd.batch import UnevenBatch
+    groups = configs.uneven_batch(args.instances, seed=4)
+    t0 = time.perf_counter()
+    ub = UnevenBatch(groups, seed=4)
+    setup = time.perf_counter() - t0
+    _lib.run_batch(ub.init_items, seed=4, stream=1)             # the chains' current trajectories
+    items = ub.sim_items + ub.mbp_items
+    _lib.run_batch(items, seed=4, stream=2, autotune=True)      # tunes the size classes not yet cached
+    pts = ub.particle_steps(items)
+    walls, cells = [], 0
+    for k in range(args.steps):
+        st = _lib.run_batch(items, seed=4, stream=10 + k)
+        walls.append(st[0])
+        cells = st[3]
+    wall = float(np.mean(walls)) * 1e-3
+    out = {"metric": "particle-timesteps/sec (uneven batch: ABC-SMC simulations + MC3 MBP proposals)",
+           "value": pts / wall, "unit": "particle-timesteps/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+           "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "int32 counts, fp32 rates", "data": "synthetic",
+           "config": {"workload": "uneven batch", "instances": args.instances, "topologies": len(groups),
+                      "contexts": len(ub.ctx), "abc_particles": len(ub.sim_items), "mc3_chains": len(ub.mbp_items),
+                      "particle_steps_per_sweep": pts, "cell_steps_per_sweep": cells,
+                      "device_gb": ub.device_bytes / 1e9, "setup_s": setup},
+           "cell_steps_per_s": cells / wall}
+    ub.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_dense(args):
+    """317 areas x 9 age groups with dense gravity mixing, 1024 PMCMC
+    particles sharing one parameter vector, 300 days: step-wise path with the
+    contraction as an MFMA GEMM, replayed from a hipGraph."""
+    import time
+    from beep_amd import _lib, configs
+    spec, truth = configs.dense_pmcmc()
+    P = 1024
+    ctx = _lib.Context(spec, capacity=P, nslot=1)
+    ctx.set_param([0], truth[None, :])
+    inst = np.arange(P, dtype=np.int32)
+    slot = np.zeros(P, dtype=np.int32)
+    ctx.simulate(inst, slot, inst.astype(np.uint32), seed=3, stream=99)
+    ms = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.simulate(inst, slot, inst.astype(np.uint32), seed=3, stream=k)
+        ms.append(ctx.timing()[0])
+        ctx.obs_loglik(inst)
+    wall = time.perf_counter() - t0
+    S = ctx.S
+    flop = 2.0 * ctx.A * ctx.A * ctx.G * P * S
+    out = {"metric": "particle-timesteps/sec (dense PMCMC batch, simulation + observation log-likelihood)",
+           "value": args.steps * P * S / wall, "unit": "particle-timesteps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": 1, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int32 counts, fp32 rates, fp64 log-likelihood", "data": "synthetic",
+           "config": {"workload": "dense PMCMC: 317 areas x 9 age groups, 10 compartments, dense gravity mixing, "
+                                  "300 days, 1024 particles"},
+           "simulate_ms": float(np.mean(ms)), "contraction_tflops_algorithmic_whole_sim": flop / (np.mean(ms) * 1e-3) / 1e12}
     ctx.close()
     print(json.dumps(out), flush=True)
 

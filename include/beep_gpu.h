@@ -241,6 +241,35 @@ int beep_accept(beep_ctx* ctx, int n, const int32_t* init, const int32_t* prop, 
                 double EFcut, const double* dlogq, const uint32_t* key, uint64_t seed, uint32_t stream,
                 double* al_out, uint8_t* accepted);
 
+/* ---- mixed batches across contexts ----
+ * One call runs simulations and MBP proposals of many instances that may
+ * belong to different contexts (different area counts, age groups, horizons):
+ * the batch-submitting counterpart of the reference drivers' one-particle-at-
+ * a-time loops (ABC-SMC particles, MC3 chains, src/abcsmc.cc:42-81,
+ * src/mc3.cc:60-66).  Fused-kernel work is grouped by size class, each group
+ * sorted longest first and launched as one kernel whose workgroups each take
+ * one instance; step-wise work runs per context.  Results are those of the
+ * single-context calls with the same keys. */
+typedef struct {
+  beep_ctx* ctx;
+  int32_t kind;      /* BEEP_WORK_SIMULATE or BEEP_WORK_MBP                       */
+  int32_t inst;      /* simulated instance, or the proposed instance of an MBP    */
+  int32_t init;      /* MBP: the initial instance                                */
+  int32_t slot;      /* parameter slot                                           */
+  uint32_t key;      /* Philox key                                               */
+  int32_t t0, t1;    /* simulate: steps [t0, t1); MBP: whole horizon             */
+} beep_work;
+#define BEEP_WORK_SIMULATE   0
+#define BEEP_WORK_MBP        1
+/* flags of beep_run_batch */
+#define BEEP_BATCH_AUTOTUNE  1   /* time the candidate workgroup sizes of each size
+                                    class not yet tuned and keep the fastest (cached
+                                    in memory and in the file named by the
+                                    BEEP_TUNE_CACHE environment variable)           */
+/* stats_out (may be NULL): [0] wall ms, [1] fused launches, [2] step-wise
+ * contexts, [3] cell-steps (instances x areas x age groups x steps). */
+int beep_run_batch(int n, const beep_work* work, uint64_t seed, uint32_t stream, int flags, double* stats_out4);
+
 /* ---- state transfer (State::save / load, src/state.cc:709-808; PMCMC moves) ---- */
 int beep_get_transnum(beep_ctx* ctx, int inst, int32_t* out);   /* [S][A][T][G] */
 int beep_get_transmean(beep_ctx* ctx, int inst, float* out);    /* [S][A][T][G] */

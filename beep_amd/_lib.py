@@ -24,7 +24,8 @@ SYMBOLS = [
     "beep_get_tables", "beep_simulate", "beep_obs_loglik", "beep_obs_values", "beep_pois_loglik", "beep_mbp",
     "beep_accept", "beep_get_transnum", "beep_get_transmean", "beep_get_pop", "beep_get_state", "beep_put_state",
     "beep_put_transnum", "beep_copy_instances", "beep_synchronize", "beep_debug_philox", "beep_debug_poisson",
-    "beep_debug_binomial", "beep_timing", "beep_debug_gemm", "beep_run_batch",
+    "beep_debug_binomial", "beep_timing", "beep_debug_gemm", "beep_run_batch", "beep_copy_states",
+    "beep_derive_slots", "beep_sum_transnum",
 ]
 
 
@@ -126,6 +127,9 @@ def lib():
             "beep_debug_binomial": (C.c_int, [C.c_int, C.c_int, i32p, f64p, C.c_uint64, C.c_uint32, i32p]),
             "beep_debug_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, f32p, f32p, f32p]),
             "beep_run_batch": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, f64p]),
+            "beep_copy_states": (C.c_int, [P, C.c_int, i32p, i32p]),
+            "beep_derive_slots": (C.c_int, [P, C.c_int, i32p, i32p, f64p, i32p]),
+            "beep_sum_transnum": (C.c_int, [P, C.c_int, i32p, C.c_int, C.c_int, C.c_int, f64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -450,6 +454,23 @@ class Context:
 
     def synchronize(self):
         self._chk(lib().beep_synchronize(self._h))
+
+    def copy_states(self, dst, src):
+        dst, src = _i32(dst), _i32(src)
+        self._chk(lib().beep_copy_states(self._h, len(dst), _p(dst, C.c_int32), _p(src, C.c_int32)))
+
+    def derive_slots(self, dst, src, factor, from_step):
+        dst, src, fs = _i32(dst), _i32(src), _i32(from_step)
+        f = _f64(factor)
+        self._chk(lib().beep_derive_slots(self._h, len(dst), _p(dst, C.c_int32), _p(src, C.c_int32),
+                                          _p(f, C.c_double), _p(fs, C.c_int32)))
+
+    def sum_transnum(self, inst, tr, t0, t1):
+        inst = _i32(inst)
+        out = np.zeros((len(inst), t1 - t0))
+        self._chk(lib().beep_sum_transnum(self._h, len(inst), _p(inst, C.c_int32), tr, t0, t1,
+                                          _p(out, C.c_double)))
+        return out
 
     def timing(self):
         """(simulate/MBP kernel ms, observation kernel ms, Poisson log-lik kernel ms, launches)."""

@@ -871,6 +871,76 @@ int beep_copy_instances(beep_ctx* c, int n, const int32_t* dst, const int32_t* s
   return BEEP_OK;
 }
 
+int beep_copy_states(beep_ctx* c, int n, const int32_t* dst, const int32_t* src) {
+  if (!c || n < 0 || (n > 0 && (!dst || !src))) return fail(c, BEEP_EINVAL, "beep_copy_states: bad arguments");
+  for (int i = 0; i < n; i++) {
+    if (!valid_inst(c, dst[i]) || !valid_inst(c, src[i])) return fail(c, BEEP_EINVAL, "instance out of range");
+    if (dst[i] == src[i]) continue;
+    HIPCHK(c, hipMemcpyAsync(c->d_state + (size_t)dst[i] * c->state_sz, c->d_state + (size_t)src[i] * c->state_sz,
+                             c->state_sz * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BEEP_OK;
+}
+
+int beep_derive_slots(beep_ctx* c, int n, const int32_t* dst, const int32_t* src, const double* factor,
+                      const int32_t* from_step) {
+  if (!c || n < 0 || (n > 0 && (!dst || !src || !factor || !from_step)))
+    return fail(c, BEEP_EINVAL, "beep_derive_slots: bad arguments");
+  if (n == 0) return BEEP_OK;
+  std::vector<float> f(n);
+  for (int i = 0; i < n; i++) {
+    if (!valid_slot(c, dst[i]) || !valid_slot(c, src[i]) || !c->slot_set[src[i]])
+      return fail(c, BEEP_EINVAL, "slot out of range or not set");
+    if (from_step[i] < 0 || from_step[i] > c->m.S) return fail(c, BEEP_EINVAL, "from_step out of range");
+    if (!(factor[i] >= 0) || !std::isfinite(factor[i])) return fail(c, BEEP_EINVAL, "bad factor");
+    f[i] = (float)factor[i];
+  }
+  int rc;
+  size_t bytes = (size_t)n * (3 * sizeof(int) + sizeof(float));
+  if ((rc = ensure_scratch(c, bytes))) return rc;
+  int* d_dst = (int*)c->d_scratch;
+  int* d_src = d_dst + n;
+  int* d_from = d_src + n;
+  float* d_f = (float*)(d_from + n);
+  HIPCHK(c, hipMemcpyAsync(d_dst, dst, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_src, src, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_from, from_step, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_f, f.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  int bx = (int)std::min<size_t>(64, (c->slot_stride + 255) / 256);
+  hipLaunchKernelGGL(beepdev::k_derive_slot, dim3(bx, n), dim3(256), 0, c->stream, c->d_tab, c->slot_stride, d_dst,
+                     d_src, d_f, d_from, c->h_topo.off_beta, c->m.A);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < n; i++) {
+    c->slot_set[dst[i]] = 1;
+    std::copy(c->slot_param.begin() + (size_t)src[i] * c->m.nparam,
+              c->slot_param.begin() + (size_t)(src[i] + 1) * c->m.nparam,
+              c->slot_param.begin() + (size_t)dst[i] * c->m.nparam);
+  }
+  return BEEP_OK;
+}
+
+int beep_sum_transnum(beep_ctx* c, int n, const int32_t* inst, int tr, int t0, int t1, double* out) {
+  if (!c || n < 0 || (n > 0 && (!inst || !out)) || !c->d_tn) return fail(c, BEEP_EINVAL, "beep_sum_transnum: bad arguments");
+  if (tr < 0 || tr >= c->m.T || t0 < 0 || t1 > c->m.S || t0 >= t1) return fail(c, BEEP_EINVAL, "bad transition or steps");
+  for (int i = 0; i < n; i++)
+    if (!valid_inst(c, inst[i])) return fail(c, BEEP_EINVAL, "instance out of range");
+  if (n == 0) return BEEP_OK;
+  int rc;
+  size_t nst = (size_t)(t1 - t0);
+  if ((rc = ensure_scratch(c, (size_t)n * nst * sizeof(double) + (size_t)n * sizeof(int)))) return rc;
+  double* d_out = (double*)c->d_scratch;
+  int* d_inst = (int*)(d_out + (size_t)n * nst);
+  HIPCHK(c, hipMemcpyAsync(d_inst, inst, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(beepdev::k_sum_transnum, dim3((unsigned)nst, n), dim3(256), 0, c->stream, d_inst, c->d_tn,
+                     c->traj_sz, c->m.AG, c->m.T, tr, t0, (int)nst, d_out);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out, d_out, (size_t)n * nst * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BEEP_OK;
+}
+
 int beep_timing(beep_ctx* c, double* out) {
   if (!c || !out) return BEEP_EINVAL;
   for (int k = 0; k < 4; k++) out[k] = c->timing[k];

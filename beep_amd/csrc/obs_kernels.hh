@@ -157,6 +157,37 @@ __global__ void k_accept(int n, const int* __restrict__ init, const int* __restr
   acc[i] = u < al ? 1 : 0;
 }
 
+// ---- prediction helpers ----
+
+// tab[dst] = tab[src] with beta_af rows s >= from scaled by factor.
+__global__ void k_derive_slot(float* __restrict__ tab, size_t stride, const int* __restrict__ dst,
+                              const int* __restrict__ src, const float* __restrict__ factor,
+                              const int* __restrict__ from, int off_beta, int A) {
+  const int i = blockIdx.y;
+  const float* s_ = tab + (size_t)src[i] * stride;
+  float* d_ = tab + (size_t)dst[i] * stride;
+  const float f = factor[i];
+  const size_t first = (size_t)off_beta + (size_t)from[i] * A;
+  for (size_t k = blockIdx.x * blockDim.x + threadIdx.x; k < stride; k += (size_t)gridDim.x * blockDim.x) {
+    float v = s_[k];
+    if (k >= first && k >= (size_t)off_beta) v *= f;
+    d_[k] = v;
+  }
+}
+
+// out[b][s - t0] = sum over cells of transnum[s][tr][cell]; one workgroup per (b, s).
+__global__ void __launch_bounds__(256) k_sum_transnum(const int* __restrict__ inst, const int* __restrict__ tn,
+                                                      size_t traj_sz, int AG, int T, int tr, int t0, int nsteps,
+                                                      double* __restrict__ out) {
+  __shared__ double red[8];
+  const int b = blockIdx.y, s = t0 + blockIdx.x;
+  const int* p = tn + (size_t)inst[b] * traj_sz + ((size_t)s * T + tr) * AG;
+  double v = 0;
+  for (int c = threadIdx.x; c < AG; c += blockDim.x) v += p[c];
+  double t = block_sum(v, red);
+  if (threadIdx.x == 0) out[(size_t)b * nsteps + blockIdx.x] = t;
+}
+
 // ---- diagnostics: raw generator and sampler outputs for the tests ----
 
 __global__ void k_dbg_philox(int n, const u4* __restrict__ ctr, uint32_t k0, uint32_t k1, u4* __restrict__ out) {

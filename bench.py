@@ -60,7 +60,9 @@ def main():
     ap.add_argument("--particles", type=int, default=PARTICLES)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="spatial", choices=["spatial", "uneven", "dense"])
+    ap.add_argument("--workload", default="spatial", choices=["spatial", "uneven", "dense", "serving"])
+    ap.add_argument("--rates", default="5,20,80")
+    ap.add_argument("--requests", type=int, default=120)
     ap.add_argument("--instances", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -74,6 +76,8 @@ def main():
         return bench_uneven(args)
     if args.workload == "dense":
         return bench_dense(args)
+    if args.workload == "serving":
+        return bench_serving(args)
     spec, _truth = configs.spatial_abcsmc()
     P = args.particles
     rng = np.random.default_rng(20250102)
@@ -211,6 +215,41 @@ def bench_dense(args):
                                   "300 days, 1024 particles"},
            "simulate_ms": float(np.mean(ms)), "contraction_tflops_algorithmic_whole_sim": flop / (np.mean(ms) * 1e-3) / 1e12}
     ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_serving(args):
+    """Prediction requests with Poisson arrivals at each offered rate: model
+    sizes from the single-area SEIR to 100 areas x 4 age groups, 100-1000
+    posterior samples, an R(t) factor in [0.5, 1.5] after a random change
+    point, 28-180 day horizons; p50/p99 arrival-to-result latency and
+    particle-timesteps/s."""
+    import time
+    from beep_amd import serving
+    srv = serving.PredictionServer(serving.CATALOGUE, t_post=100, max_horizon=180, nposterior=1000,
+                                   hbm_budget=96 << 30, chunk=14)
+    print("serving: models loaded", file=sys.stderr, flush=True)
+    warm = serving.make_requests(8, rate=1000.0, catalogue_size=len(serving.CATALOGUE), seed=1)
+    srv.run(warm, realtime=False)
+    rows = []
+    for rate in [float(x) for x in args.rates.split(",")]:
+        reqs = serving.make_requests(args.requests, rate=rate, catalogue_size=len(serving.CATALOGUE), seed=int(rate))
+        t0 = time.perf_counter()
+        srv.run(reqs, realtime=True)
+        wall = time.perf_counter() - t0
+        pts = sum(r.nsamp * r.horizon for r in reqs)
+        row = {"offered_rate": rate, "requests": len(reqs), "wall_s": wall, "particle_timesteps_per_s": pts / wall}
+        row.update(serving.latency_stats(reqs))
+        rows.append(row)
+        print(f"serving: rate {rate}/s done: {row}", file=sys.stderr, flush=True)
+    srv.close()
+    best = max(rows, key=lambda r: r["particle_timesteps_per_s"])
+    out = {"metric": "serving: particle-timesteps/sec with p50/p99 request latency", "value": best["particle_timesteps_per_s"],
+           "unit": "particle-timesteps/s", "n_gpus": 1, "steps": len(rows), "warmup": 1, "ms_per_step": None,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32 counts, fp32 rates",
+           "data": "synthetic", "config": {"workload": "prediction serving", "catalogue": serving.CATALOGUE,
+                                           "posterior_samples_per_model": 1000, "chunk_days": 14},
+           "rates": rows}
     print(json.dumps(out), flush=True)
 
 

@@ -283,6 +283,22 @@ int beep_put_transnum(beep_ctx* ctx, int inst, int slot, const int32_t* transnum
  * the device form of the particle copies of src/mpi.cc:81-165,215-327. */
 int beep_copy_instances(beep_ctx* ctx, int n, const int32_t* dst, const int32_t* src);
 
+/* Copies only the current state [C][AG] of src[i] to dst[i]. */
+int beep_copy_states(beep_ctx* ctx, int n, const int32_t* dst, const int32_t* src);
+
+/* Counterfactual slots for prediction (Simulate::model_modification,
+ * src/simulate.cc:142-165): table slot dst[i] becomes slot src[i] with the
+ * reproduction number multiplied by factor[i] from step from_step[i] on --
+ * the `spline_fac` modification of the R spline (src/model.cc:1557-1575,
+ * applied in create_disc_spline 1044-1050), which scales beta * areafactor
+ * linearly.  Runs on the device; src slots must be set. */
+int beep_derive_slots(beep_ctx* ctx, int n, const int32_t* dst, const int32_t* src, const double* factor,
+                      const int32_t* from_step);
+
+/* Prediction summary: out[i][s - t0] = sum over areas and dps of
+ * transnum[s][area][tr][dp] of instance inst[i], s in [t0, t1). */
+int beep_sum_transnum(beep_ctx* ctx, int n, const int32_t* inst, int tr, int t0, int t1, double* out);
+
 /* Waits for the context's stream. */
 int beep_synchronize(beep_ctx* ctx);
 

@@ -10,8 +10,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbeep_gpu.so")
-SOURCES = ["beep_gpu.hip", "host_model.cc"]
-HEADERS = ["device_rng.hh", "kernels.hh", "obs_kernels.hh", "host_model.hh"]
+SOURCES = ["beep_gpu.hip", "host_model.cc", "host/gpu_state.cc"]
+HEADERS = ["device_rng.hh", "kernels.hh", "obs_kernels.hh", "step_kernels.hh", "host_model.hh", "host/gpu_state.hh"]
+CHECK_SRC = os.path.join(CSRC, "host", "gpu_state_check.cc")
+CHECK_BIN = os.path.join(HERE, "bin", "gpu_state_check")
 
 
 def hipcc():
@@ -36,6 +38,17 @@ def build(verbose=True):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libbeep_gpu.so")
     os.replace(LIB + ".tmp", LIB)
+    # The C++ host mirror's self-check program, linked against the library.
+    os.makedirs(os.path.dirname(CHECK_BIN), exist_ok=True)
+    cmd = [hipcc(), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(CSRC, "host"),
+           CHECK_SRC, "-o", CHECK_BIN + ".tmp", "-L", HERE, "-lbeep_gpu", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("building gpu_state_check failed")
+    os.replace(CHECK_BIN + ".tmp", CHECK_BIN)
     return LIB
 
 

@@ -25,7 +25,7 @@ SYMBOLS = [
     "beep_accept", "beep_get_transnum", "beep_get_transmean", "beep_get_pop", "beep_get_state", "beep_put_state",
     "beep_put_transnum", "beep_copy_instances", "beep_synchronize", "beep_debug_philox", "beep_debug_poisson",
     "beep_debug_binomial", "beep_timing", "beep_debug_gemm", "beep_run_batch", "beep_copy_states",
-    "beep_derive_slots", "beep_sum_transnum",
+    "beep_derive_slots", "beep_sum_transnum", "beep_pmcmc_obs_prob", "beep_pmcmc_particle_sample", "beep_bootstrap",
 ]
 
 
@@ -130,6 +130,11 @@ def lib():
             "beep_copy_states": (C.c_int, [P, C.c_int, i32p, i32p]),
             "beep_derive_slots": (C.c_int, [P, C.c_int, i32p, i32p, f64p, i32p]),
             "beep_sum_transnum": (C.c_int, [P, C.c_int, i32p, C.c_int, C.c_int, C.c_int, f64p]),
+            "beep_pmcmc_obs_prob": (C.c_int, [P, C.c_int, i32p, C.c_int, u32p, C.c_double, C.c_uint64, C.c_uint32,
+                                              f64p, i32p]),
+            "beep_pmcmc_particle_sample": (C.c_int, [P, C.c_int]),
+            "beep_bootstrap": (C.c_int, [C.c_int, C.c_int, f64p, C.c_double, C.c_uint64, C.c_uint32, C.c_int, i32p,
+                                         f64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -149,6 +154,10 @@ def _i32(x):
 
 def _f64(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+def _u32(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.uint32))
 
 
 class _SpecBuffers:
@@ -472,11 +481,39 @@ class Context:
                                           _p(out, C.c_double)))
         return out
 
+    def pmcmc_obs_prob(self, inst, slot, keys, invT=1.0, seed=1, stream=0):
+        """PMCMC::obs_prob over the particles `inst` (one parameter slot):
+        returns (log-likelihood estimate, ancestry [(nsection+1), n])."""
+        inst, keys = _i32(inst), _u32(keys)
+        nsec = len(self.spec.get("sections", []))
+        ob = np.zeros(1)
+        bp = np.zeros((nsec + 1, len(inst)), dtype=np.int32)
+        self._chk(lib().beep_pmcmc_obs_prob(self._h, len(inst), _p(inst, C.c_int32), int(slot), _p(keys, C.c_uint32),
+                                            float(invT), int(seed), int(stream), _p(ob, C.c_double),
+                                            _p(bp, C.c_int32)))
+        return float(ob[0]), bp
+
+    def pmcmc_particle_sample(self, dst):
+        """Mpi::particle_sample of the last particle-filter run into instance dst."""
+        self._chk(lib().beep_pmcmc_particle_sample(self._h, int(dst)))
+
     def timing(self):
         """(simulate/MBP kernel ms, observation kernel ms, Poisson log-lik kernel ms, launches)."""
         out = np.zeros(4)
         self._chk(lib().beep_timing(self._h, _p(out, C.c_double)))
         return out
+
+
+def bootstrap(L, invT=1.0, seed=1, stream=0, sec=0, device=0):
+    """PMCMC::bootstrap on the device: (ancestors, the section's term)."""
+    L = np.ascontiguousarray(np.asarray(L, dtype=np.float64))
+    bp = np.zeros(len(L), dtype=np.int32)
+    ob = np.zeros(1)
+    rc = lib().beep_bootstrap(device, len(L), _p(L, C.c_double), float(invT), int(seed), int(stream), int(sec),
+                              _p(bp, C.c_int32), _p(ob, C.c_double))
+    if rc:
+        raise BeepError(rc, "beep_bootstrap")
+    return bp, float(ob[0])
 
 
 def debug_philox(ctr, k0, k1, device=0):

@@ -20,6 +20,7 @@
 #include "host_model.hh"
 #include "kernels.hh"
 #include "obs_kernels.hh"
+#include "pmcmc_kernels.hh"
 #include "step_kernels.hh"
 
 using beepdev::Job;
@@ -73,7 +74,13 @@ struct beep_ctx {
   float* d_I = nullptr;            // [capacity*G][Apad]
   uint64_t* d_rngp = nullptr;      // seed, stream read by the captured step kernels
   struct Graph { int mode = -1, n = 0, t0 = 0, t1 = 0; const Job* jobs = nullptr; hipGraphExec_t exec = nullptr; };
-  Graph graph;
+  std::vector<Graph> graphs;       // captured step loops, most recently made last
+  // particle filter: ancestry rows [(nsection+1)][pf_n] then the instances [pf_n]
+  int* d_sec_ti = nullptr;
+  int* d_sec_tf = nullptr;
+  int* d_pf = nullptr;
+  size_t pf_cap = 0;
+  int pf_n = 0;
 };
 
 namespace {
@@ -123,10 +130,13 @@ std::vector<float> to_f32(const std::vector<double>& v) { return std::vector<flo
 int ensure_jobs(beep_ctx* c, size_t n) {
   if (n <= c->jobs_cap) return BEEP_OK;
   if (c->d_jobs) {
-    if (c->graph.exec) {          // the captured graph refers to the old job buffer
-      HIPCHK(c, hipGraphExecDestroy(c->graph.exec));
-      c->graph.exec = nullptr;
-    }
+    for (auto& g : c->graphs)     // captured graphs refer to the old job buffer
+      if (g.jobs >= c->d_jobs && g.jobs < c->d_jobs + c->jobs_cap) {
+        HIPCHK(c, hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+      }
+    c->graphs.erase(std::remove_if(c->graphs.begin(), c->graphs.end(), [](const beep_ctx::Graph& g) { return !g.exec; }),
+                    c->graphs.end());
     HIPCHK(c, hipFree(c->d_jobs));
     c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->d_jobs));
     c->allocated -= c->jobs_cap * sizeof(Job);
@@ -172,6 +182,8 @@ int check_device_error(beep_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (e) {
     HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (e & 4) return fail(c, BEEP_ENAN, "PMCMC bootstrap: non-finite likelihood estimate or weights");
     if (e & 1) return fail(c, BEEP_ENAN, "NaN transition mean");
     return fail(c, BEEP_ELARGE, "transition mean above 1e9");
   }
@@ -205,15 +217,16 @@ int launch_stepwise(beep_ctx* c, int mode, const Job* djobs, int n, int t0, int 
   uint64_t rp[2] = {seed, (uint64_t)stream};
   HIPCHK(c, hipMemcpyAsync(c->d_rngp, rp, sizeof rp, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  auto& G = c->graph;
-  if (G.exec && G.mode == mode && G.n == n && G.t0 == t0 && G.t1 == t1 && G.jobs == djobs) {
-    HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
-    return BEEP_OK;
+  for (const auto& g : c->graphs)
+    if (g.mode == mode && g.n == n && g.t0 == t0 && g.t1 == t1 && g.jobs == djobs) {
+      HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+      return BEEP_OK;
+    }
+  if (c->graphs.size() >= 32) {
+    HIPCHK(c, hipGraphExecDestroy(c->graphs.front().exec));
+    c->graphs.erase(c->graphs.begin());
   }
-  if (G.exec) {
-    HIPCHK(c, hipGraphExecDestroy(G.exec));
-    G.exec = nullptr;
-  }
+  beep_ctx::Graph G;
   const int rows = n * m.G;
   dim3 gcell((m.AG + BEEP_STEP_BLOCK - 1) / BEEP_STEP_BLOCK, n);
   hipGraph_t graph;
@@ -245,6 +258,7 @@ int launch_stepwise(beep_ctx* c, int mode, const Job* djobs, int n, int t0, int 
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(c, BEEP_EHIP, "graph instantiate: %s", hipGetErrorString(e));
   G.mode = mode; G.n = n; G.t0 = t0; G.t1 = t1; G.jobs = djobs;
+  c->graphs.push_back(G);
   HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
   return BEEP_OK;
 }
@@ -475,7 +489,9 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
       sptr.push_back((int)sobs.size());
     }
     c->sec_ptr = sptr;
-    if ((rc = upload(c, &c->d_sec_ptr, sptr)) || (rc = upload(c, &c->d_sec_obs, sobs))) return rc;
+    if ((rc = upload(c, &c->d_sec_ptr, sptr)) || (rc = upload(c, &c->d_sec_obs, sobs)) ||
+        (rc = upload(c, &c->d_sec_ti, m.section_ti)) || (rc = upload(c, &c->d_sec_tf, m.section_tf)))
+      return rc;
   }
   return BEEP_OK;
 }
@@ -486,7 +502,7 @@ int beep_destroy(beep_ctx* c) {
   for (void* p : c->allocs) (void)hipFree(p);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
-  if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return BEEP_OK;
@@ -945,6 +961,152 @@ int beep_timing(beep_ctx* c, double* out) {
   if (!c || !out) return BEEP_EINVAL;
   for (int k = 0; k < 4; k++) out[k] = c->timing[k];
   return BEEP_OK;
+}
+
+// ---- PMCMC particle filter ----
+
+int beep_pmcmc_obs_prob(beep_ctx* c, int n, const int32_t* inst, int slot, const uint32_t* key, double invT,
+                        uint64_t seed, uint32_t stream, double* obprob_out, int32_t* backpart_out) {
+  if (!c) return BEEP_EINVAL;
+  if (n <= 0 || !inst || !key || !obprob_out) return fail(c, BEEP_EINVAL, "beep_pmcmc_obs_prob: bad arguments");
+  if (!c->d_tn) return fail(c, BEEP_EINVAL, "the observation model needs transnum to be kept");
+  if (stream > 0xFFFF) return fail(c, BEEP_EINVAL, "stream must be below 65536");
+  if (!valid_slot(c, slot) || !c->slot_set[slot]) return fail(c, BEEP_EINVAL, "slot %d not set", slot);
+  if (!(invT >= 0) || !std::isfinite(invT)) return fail(c, BEEP_EINVAL, "bad invT");
+  const auto& m = c->m;
+  const int nsec = (int)m.section_ti.size();
+  if (nsec == 0) return fail(c, BEEP_EINVAL, "the model has no observation sections");
+  for (int k = 0; k < nsec; k++)
+    if (m.section_ti[k] != (k ? m.section_tf[k - 1] : 0) || (k == nsec - 1 && m.section_tf[k] != m.S))
+      return fail(c, BEEP_EINVAL, "the observation sections must tile [0, S) in order");
+  std::vector<char> seen(c->capacity, 0);
+  for (int i = 0; i < n; i++) {
+    if (!valid_inst(c, inst[i])) return fail(c, BEEP_EINVAL, "instance %d out of range", inst[i]);
+    if (seen[inst[i]]++) return fail(c, BEEP_EINVAL, "instance %d appears twice", inst[i]);
+  }
+  int rc;
+  // Persistent ancestry + instance list (read later by particle_sample).
+  size_t pf_ints = (size_t)(nsec + 1) * n + n;
+  if (pf_ints > c->pf_cap) {
+    if (c->d_pf) {
+      HIPCHK(c, hipFree(c->d_pf));
+      c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->d_pf));
+      c->allocated -= c->pf_cap * sizeof(int);
+      c->d_pf = nullptr;
+      c->pf_cap = 0;
+    }
+    if ((rc = dalloc(c, (void**)&c->d_pf, pf_ints * sizeof(int)))) return rc;
+    c->pf_cap = pf_ints;
+  }
+  c->pf_n = 0;
+  int* d_bp = c->d_pf;
+  int* d_inst = c->d_pf + (size_t)(nsec + 1) * n;
+  std::vector<int> head((size_t)n);
+  for (int i = 0; i < n; i++) head[i] = i;
+  HIPCHK(c, hipMemcpyAsync(d_bp, head.data(), n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_inst, inst, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  // Scratch: L rows, bootstrap work arrays, the estimate.
+  const int Lstride = 1 + m.ndatatable;
+  size_t bytes = (size_t)n * Lstride * sizeof(double) + (size_t)n * sizeof(double) + 3 * (size_t)n * sizeof(int) + 64;
+  if ((rc = ensure_scratch(c, bytes))) return rc;
+  double* d_ob = (double*)c->d_scratch;
+  double* d_L = d_ob + 1;
+  double* d_w = d_L + (size_t)n * Lstride;
+  int* d_first = (int*)(d_w + n);
+  int* d_draw = d_first + n;
+  int* d_extra = d_draw + n;
+  HIPCHK(c, hipMemsetAsync(d_ob, 0, sizeof(double), c->stream));
+  // Jobs of every section.
+  std::vector<Job> jobs;
+  jobs.reserve((size_t)nsec * n);
+  for (int k = 0; k < nsec; k++)
+    for (int i = 0; i < n; i++) jobs.push_back(make_job(c, inst[i], slot, key[i], m.section_ti[k], m.section_tf[k]));
+  if ((rc = put_jobs(c, jobs))) return rc;
+  const unsigned mv_x = (unsigned)std::min<size_t>(16, (c->state_sz + 255) / 256);
+  record(c, 0);
+  for (int k = 0; k < nsec; k++) {
+    if (k > 0) {
+      hipLaunchKernelGGL(beepdev::k_move_states, dim3(n, mv_x), dim3(256), 0, c->stream, d_inst, d_bp + (size_t)k * n,
+                         c->d_state, c->state_sz);
+      HIPCHK(c, hipGetLastError());
+    }
+    const Job* dj = c->d_jobs + (size_t)k * n;
+    if (c->stepwise) rc = launch_stepwise(c, beepdev::MODE_SIM, dj, n, m.section_ti[k], m.section_tf[k], seed, stream);
+    else rc = launch_fused(c, beepdev::MODE_SIM, dj, n, seed, stream);
+    if (rc) return rc;
+    const int nsel = c->sec_ptr[k + 1] - c->sec_ptr[k];
+    hipLaunchKernelGGL(beepdev::k_obs_loglik, dim3(n), dim3(256), 0, c->stream, c->d_obs, d_inst, c->d_tn, c->d_pt,
+                       c->traj_sz, c->pop_traj_sz, c->d_sec_obs + c->sec_ptr[k], nsel, d_L, c->d_EF, 0);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(beepdev::k_bootstrap, dim3(1), dim3(BEEP_BOOT_BLOCK), 0, c->stream, n, d_L, Lstride, invT, seed,
+                       stream, k, d_bp + (size_t)(k + 1) * n, d_w, d_first, d_draw, d_extra, d_ob, c->d_err);
+    HIPCHK(c, hipGetLastError());
+  }
+  record(c, 1);
+  double ob = 0;
+  HIPCHK(c, hipMemcpyAsync(&ob, d_ob, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (backpart_out)
+    HIPCHK(c, hipMemcpyAsync(backpart_out, d_bp, (size_t)(nsec + 1) * n * sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+  rc = check_device_error(c);      // synchronises the stream
+  c->timing[0] = elapsed(c, 0, 1);
+  c->timing[3] = 3.0 * nsec - 1 + (c->stepwise ? 2.0 * m.S + nsec : nsec);   // kernel launches
+  if (rc) return rc;
+  *obprob_out = ob;
+  c->pf_n = n;
+  return BEEP_OK;
+}
+
+int beep_pmcmc_particle_sample(beep_ctx* c, int dst) {
+  if (!c) return BEEP_EINVAL;
+  if (!valid_inst(c, dst)) return fail(c, BEEP_EINVAL, "instance %d out of range", dst);
+  if (c->pf_n <= 0) return fail(c, BEEP_EINVAL, "no particle filter run to sample from");
+  const auto& m = c->m;
+  const int nsec = (int)m.section_ti.size(), n = c->pf_n;
+  const int* d_inst = c->d_pf + (size_t)(nsec + 1) * n;
+  size_t maxlen = 0;
+  for (int k = 0; k < nsec; k++) maxlen = std::max<size_t>(maxlen, (size_t)(m.section_tf[k] - m.section_ti[k]));
+  const unsigned gx = (unsigned)std::min<size_t>(64, (maxlen * m.T * m.AG + 255) / 256);
+  hipLaunchKernelGGL(beepdev::k_particle_sample, dim3(gx, nsec), dim3(256), 0, c->stream, n, nsec, c->d_sec_ti,
+                     c->d_sec_tf, c->d_pf, d_inst, dst, c->d_tn, c->d_tm, c->d_pt, c->traj_sz, c->pop_traj_sz,
+                     m.T * m.AG, m.C * m.AG);
+  HIPCHK(c, hipGetLastError());
+  // EF of the sampled particle is not defined by the section run.
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BEEP_OK;
+}
+
+int beep_bootstrap(int device, int n, const double* L, double invT, uint64_t seed, uint32_t stream, int sec,
+                   int32_t* bp_out, double* obprob_out) {
+  if (n <= 0 || !L || !bp_out || !obprob_out || stream > 0xFFFF || sec < 0 || !(invT >= 0) || !std::isfinite(invT))
+    return BEEP_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return BEEP_EHIP;
+  const size_t bytes = 16 + (size_t)n * (8 + 8 + 4 * 4);     // estimate, error flag, L, wsum, 4 int arrays
+  char* d = nullptr;
+  if (hipMalloc((void**)&d, bytes) != hipSuccess) return BEEP_ENOMEM;
+  double* d_ob = (double*)d;
+  int* d_err = (int*)(d + 8);
+  double* d_L = (double*)(d + 16);
+  double* d_w = d_L + n;
+  int* d_bp = (int*)(d_w + n);
+  int* d_first = d_bp + n;
+  int* d_draw = d_first + n;
+  int* d_extra = d_draw + n;
+  int rc = BEEP_OK, err = 0;
+  if (hipMemset(d, 0, 16) != hipSuccess || hipMemcpy(d_L, L, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = BEEP_EHIP;
+  if (!rc) {
+    hipLaunchKernelGGL(beepdev::k_bootstrap, dim3(1), dim3(BEEP_BOOT_BLOCK), 0, 0, n, d_L, 1, invT, seed, stream, sec,
+                       d_bp, d_w, d_first, d_draw, d_extra, d_ob, d_err);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = BEEP_EHIP;
+  }
+  if (!rc && (hipMemcpy(bp_out, d_bp, n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(obprob_out, d_ob, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+              hipMemcpy(&err, d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess))
+    rc = BEEP_EHIP;
+  (void)hipFree(d);
+  if (!rc && err) rc = BEEP_ENAN;
+  return rc;
 }
 
 // ---- mixed batches ----

@@ -37,6 +37,7 @@ GpuModel::GpuModel(const beep_spec& spec, int cap, int keep, int device, size_t 
   A = d[0]; G = d[1]; C = d[2]; T = d[3]; S = d[4]; nparam = d[5]; nobs = d[6]; ndatatable = d[7];
   capacity = d[8];
   steps_per_day = spec.steps_per_day;
+  nsection = spec.nsection;
   if ((int)param_names.size() != nparam) {
     param_names.clear();
     for (int i = 0; i < nparam; i++) param_names.push_back("param" + std::to_string(i));
@@ -256,6 +257,37 @@ bool GpuMbp::accept(double dlogq_extra) {
   stream_++;
   if (acc) std::swap(initial_, propose_);
   return acc != 0;
+}
+
+// ---- GpuParticleFilter ----
+
+GpuParticleFilter::GpuParticleFilter(GpuModel& m, std::vector<int32_t> inst, int slot, uint32_t key0)
+    : m_(m), inst_(std::move(inst)), slot_(slot) {
+  if (inst_.empty()) throw Error(BEEP_EINVAL, "no particles");
+  for (size_t i = 0; i < inst_.size(); i++) key_.push_back(key0 + (uint32_t)i);
+}
+
+double GpuParticleFilter::obs_prob(const std::vector<double>& paramv) {
+  if ((int)paramv.size() != m_.nparam) throw Error(BEEP_EINVAL, "wrong number of parameters");
+  int32_t slot = slot_, inb = 1;
+  double pr = 0;
+  m_.check(beep_set_param(m_.ctx(), 1, &slot, paramv.data(), &pr, &inb));   // every particle's set_param
+  paramval_ = paramv;
+  const int nsec = m_.nsection;
+  backpart_.assign((size_t)(nsec + 1) * inst_.size(), 0);
+  m_.check(beep_pmcmc_obs_prob(m_.ctx(), (int)inst_.size(), inst_.data(), slot_, key_.data(), invT, m_.seed,
+                               stream_++ & 0xFFFF, &obprob_, backpart_.data()));
+  return obprob_;
+}
+
+Particle GpuParticleFilter::particle_sample(unsigned run, int dst) {
+  m_.check(beep_pmcmc_particle_sample(m_.ctx(), dst));
+  Particle p;
+  p.run = run;
+  p.paramval = paramval_;
+  p.transnum.resize((size_t)m_.S * m_.A * m_.T * m_.G);
+  m_.check(beep_get_transnum(m_.ctx(), dst, p.transnum.data()));
+  return p;
 }
 
 }  // namespace beep

@@ -49,7 +49,7 @@ class GpuModel {
   GpuModel& operator=(const GpuModel&) = delete;
 
   beep_ctx* ctx() const { return ctx_; }
-  int A, G, C, T, S, nparam, nobs, ndatatable, capacity, steps_per_day;
+  int A, G, C, T, S, nparam, nobs, ndatatable, capacity, steps_per_day, nsection;
   std::vector<std::string> param_names;
   uint64_t seed = 1;
   void check(int rc) const;
@@ -133,6 +133,31 @@ class GpuMbp {
   GpuState* initial_;
   GpuState* propose_;
   uint32_t stream_ = 0x8000;
+};
+
+// PMCMC::obs_prob and Mpi::particle_sample (src/pmcmc.cc:180-268,
+// src/mpi.cc:215-385) for N particles held in instances inst[0..N): the
+// whole filter (sections, bootstrap resampling, state moves) runs on the
+// device in one call.
+class GpuParticleFilter {
+ public:
+  GpuParticleFilter(GpuModel& m, std::vector<int32_t> inst, int slot, uint32_t key0 = 0);
+  double invT = 1.0;                                   // PMCMC::invT
+  double obs_prob(const std::vector<double>& paramv);  // src/pmcmc.cc:180-207
+  // The trajectory traced back from the first particle of the last
+  // bootstrap, gathered into instance dst and returned as a Particle.
+  Particle particle_sample(unsigned run, int dst);     // src/mpi.cc:331-385
+  const std::vector<int32_t>& backpart() const { return backpart_; }   // [(nsection+1)][N]
+
+ private:
+  GpuModel& m_;
+  std::vector<int32_t> inst_;
+  std::vector<uint32_t> key_;
+  int slot_;
+  uint32_t stream_ = 0;
+  std::vector<double> paramval_;
+  double obprob_ = 0;
+  std::vector<int32_t> backpart_;
 };
 
 }  // namespace beep

@@ -63,7 +63,7 @@ int main(int argc, char** argv) {
   spec.nsection = 2; spec.section_ti = sec_ti; spec.section_tf = sec_tf;
 
   try {
-    GpuModel m(spec, 4, BEEP_KEEP_ALL, 0, 0, {"R", "mE", "mI", "infI"});
+    GpuModel m(spec, 4 + 32, BEEP_KEEP_ALL, 0, 0, {"R", "mE", "mI", "infI"});
     GpuObservationModel om(m);
     GpuState a(m, 0, 11), b(m, 1, 12), c(m, 2, 13);
     std::vector<double> theta = {2.1, 5.0, 7.0, 1.0};
@@ -109,9 +109,26 @@ int main(int argc, char** argv) {
     CHECK(mbp.mbp(theta2, INF_DIF_UPDATE) == SUCCESS, "mbp to a new R");
     double al2 = mbp.get_al();
     bool acc = mbp.accept();
+    // Particle filter over the two sections with 32 particles (instances 4..35).
+    std::vector<int32_t> pinst;
+    for (int i = 0; i < 32; i++) pinst.push_back(4 + i);
+    GpuParticleFilter pf(m, pinst, 3, 500);
+    pf.invT = 1.0;
+    double ob = pf.obs_prob(theta);
+    CHECK(std::isfinite(ob), "particle filter estimate is finite");
+    const auto& bpk = pf.backpart();
+    CHECK(bpk.size() == 3 * 32, "backpart rows");
+    bool kept = true;
+    for (int k = 1; k <= 2; k++)
+      for (int p = 0; p < 32; p++) kept &= bpk[k * 32 + bpk[k * 32 + p]] == bpk[k * 32 + p];
+    CHECK(kept, "every ancestor keeps its own place");
+    Particle ps = pf.particle_sample(0, 3);
+    long tot_inf = 0;
+    for (int s = 0; s < m.S; s++) tot_inf += ps.transnum[(size_t)s * 3 + 0];
+    CHECK(tot_inf > 0 && ps.paramval == theta, "sampled particle");
     std::printf("{\"EF\": %.17g, \"Pr\": %.17g, \"likelihood\": %.17g, \"al_identity\": %.17g, \"al2\": %.17g, "
-                "\"accepted\": %d, \"failures\": %d}\n",
-                a.EF, a.Pr, a.likelihood(), al, al2, acc ? 1 : 0, failures);
+                "\"accepted\": %d, \"pf_obprob\": %.17g, \"failures\": %d}\n",
+                a.EF, a.Pr, a.likelihood(), al, al2, acc ? 1 : 0, ob, failures);
   } catch (const Error& e) {
     std::fprintf(stderr, "error %d: %s\n", e.code, e.what());
     return 2;

@@ -83,7 +83,7 @@ std::string HostModel::build(const beep_spec& s) {
   if (A <= 0 || G <= 0 || C <= 0 || T <= 0 || period <= 0 || spd <= 0) return "bad dimensions";
   if (T > 255) return "too many transitions";
   S = period * spd;
-  if (S > 65535) return "too many steps";
+  if (S > 65533) return "too many steps (at most 65533)";
   AG = A * G;
   dt = (double)period / S;                                   // src/state.cc:384
   nparam = s.nparam;

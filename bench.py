@@ -24,6 +24,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PARTICLES = 4096
+METRIC = "particle-timesteps/sec batched vs plain GPU path; p50/p99 request latency"
+# HBM bytes per launch of the fused kernel on this workload (4096 particles),
+# from rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the
+# ratio of known bytes to counter reading on a plain copy kernel
+# (profiles/k_fused_spatial_abcsmc_traffic.md).  None until measured.
+FUSED_TRAFFIC_BYTES = None
 
 
 def algorithmic_bytes_per_particle_step(A, G, C, T):
@@ -35,21 +41,50 @@ def algorithmic_bytes_per_particle_step(A, G, C, T):
     return 4 * AG * (2 * T + C) + 4 * A + 4 * G + 4
 
 
-def cpu_baseline(spec, params, seconds=12.0):
-    """FP64 CPU oracle (libstdc++ samplers), one core, over a bounded sample
-    of the same particles."""
+def _cpu_worker(args):
+    """One host core: simulate + score particles idx, idx+stride, ... for
+    `seconds`; returns (particles done, particle-timesteps, seconds)."""
+    spec, params, idx, stride, seconds = args
     from oracle import oracle as O
     orc = O.Oracle(spec)
-    S = orc.S
     n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds and n < len(params):
-        tb = orc.tables(params[n])
-        st = orc.simulate(tb, seed=1000 + n)
+    i = idx
+    while time.perf_counter() - t0 < seconds and i < len(params):
+        tb = orc.tables(params[i])
+        st = orc.simulate(tb, seed=1000 + i)
         orc.obs_loglik(st)
         n += 1
-    dt = time.perf_counter() - t0
-    return {"value": n * S / dt, "unit": "particle-timesteps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} of the {PARTICLES} particles (same priors), {S} steps each, single thread, {dt:.1f} s"}
+        i += stride
+    return n, n * orc.S, time.perf_counter() - t0
+
+
+def cpu_cores():
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail))
+
+
+def cpu_baseline(spec, params, seconds=10.0, cores=None):
+    """FP64 CPU oracle (the reference's libstdc++ samplers), one process per
+    host core with a static split of the particles (as the reference's MPI
+    ranks split them), over a bounded sample of the same particles.  Runs in
+    forked processes before this process touches the GPU."""
+    import multiprocessing as mp
+    cores = cores or cpu_cores()
+    jobs = [(spec, params, k, cores, seconds) for k in range(cores)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, jobs)
+    wall = time.perf_counter() - t0
+    n = sum(r[0] for r in res)
+    pts = sum(r[1] for r in res)
+    per_core = [r[1] / r[2] for r in res]
+    return {"value": sum(per_core), "unit": "particle-timesteps/s", "cores": cores, "kind": "port",
+            "sample": f"{n} of the {len(params)} particles (same priors), 200 steps each, simulate + observation "
+                      f"log-likelihood, {cores} processes x {seconds:.0f} s (wall {wall:.1f} s); single core "
+                      f"{float(np.mean(per_core)):.0f} particle-timesteps/s"}
 
 
 def main():
@@ -60,7 +95,7 @@ def main():
     ap.add_argument("--particles", type=int, default=PARTICLES)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="spatial", choices=["spatial", "uneven", "dense", "serving"])
+    ap.add_argument("--workload", default="spatial", choices=["spatial", "uneven", "dense", "serving", "copy"])
     ap.add_argument("--rates", default="5,20,80")
     ap.add_argument("--requests", type=int, default=120)
     ap.add_argument("--instances", type=int, default=16384)
@@ -78,10 +113,15 @@ def main():
         return bench_dense(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "copy":
+        return bench_copy(args)
     spec, _truth = configs.spatial_abcsmc()
     P = args.particles
     rng = np.random.default_rng(20250102)
     params = specmod.sample_prior(spec, rng, P)
+    # The CPU baseline runs first, in forked processes, before anything here
+    # initialises the GPU.
+    cpu = None if args.no_cpu else cpu_baseline(spec, params)
     ctx = _lib.Context(spec, capacity=P, nslot=P, keep=_lib.KEEP_ALL)
     slots = np.arange(P, dtype=np.int32)
     ctx.set_param(slots, params)
@@ -123,7 +163,7 @@ def main():
     bytes_per = algorithmic_bytes_per_particle_step(A, G, C, T)
     achieved = bytes_per * P * S / (sim_avg * 1e-3) / 1e9
     out = {
-        "metric": "particle-timesteps/sec (batched, simulation + observation log-likelihood)",
+        "metric": METRIC,
         "value": value,
         "unit": "particle-timesteps/s",
         "n_gpus": 1,
@@ -141,12 +181,37 @@ def main():
         "plain": {"value": plain_value, "sample_instances": ns, "batched_over_plain": value / plain_value},
         "kernels_ms": {"simulate_fused": sim_avg, "obs_loglik": float(np.mean(obs_ms))},
         "roofline": {"bound": "hbm", "kernel": "k_fused<MODE_SIM,8>", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_particle_step": bytes_per},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": FUSED_TRAFFIC_BYTES if P == PARTICLES else None,
+                     "bytes_per_particle_step": bytes_per, "algorithmic_bytes_per_launch": bytes_per * P * S},
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(spec, params)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_copy(args):
+    """A plain device-to-device copy of known size (the calibration kernel for
+    the FETCH_SIZE / WRITE_SIZE counters): 256 MiB read + 256 MiB written per
+    launch."""
+    import torch
+    n = 64 << 20
+    x = torch.arange(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert bool((y[-4:] == x[-4:]).all())
+    out = {"metric": "copy GB/s (counter calibration)", "value": 2 * 4 * n * args.steps / wall / 1e9, "unit": "GB/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "device copy", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n}}
     print(json.dumps(out), flush=True)
 
 

@@ -299,6 +299,39 @@ int beep_derive_slots(beep_ctx* ctx, int n, const int32_t* dst, const int32_t* s
  * transnum[s][area][tr][dp] of instance inst[i], s in [t0, t1). */
 int beep_sum_transnum(beep_ctx* ctx, int n, const int32_t* inst, int tr, int t0, int t1, double* out);
 
+/* ---- PMCMC particle filter ---- */
+/* PMCMC::obs_prob (src/pmcmc.cc:180-207) on the device for the n particles
+ * inst[] (distinct), all with parameter slot `slot` (set beforehand, as
+ * obs_prob's set_param): for each observation section k in order,
+ *   k > 0: particle p continues from the state of its ancestor bp_k[p]
+ *          (Mpi::end_sec_swap, src/mpi.cc:215-327);
+ *   simulate(section_ti[k], section_tf[k]) with draws keyed by key[p];
+ *   L_p = ObservationModel::calculate_section (src/obsmodel.cc:39-58);
+ *   PMCMC::bootstrap (src/pmcmc.cc:211-268): obprob += invT*Lmax +
+ *     log(mean exp(invT (L - Lmax))) in FP64 and ancestors bp_{k+1} by the
+ *     reference's multinomial draw and survivor / extra assignment.
+ * The sections must tile [0, S) in order.  *obprob_out is the estimate;
+ * backpart_out, if not null, receives [(nsection + 1)][n] ancestor rows
+ * (row 0 is the identity).  The ancestry stays on the device for
+ * beep_pmcmc_particle_sample.  A non-finite estimate is BEEP_ENAN
+ * (src/pmcmc.cc:201).  One host synchronisation per call. */
+int beep_pmcmc_obs_prob(beep_ctx* ctx, int n, const int32_t* inst, int slot, const uint32_t* key, double invT,
+                        uint64_t seed, uint32_t stream, double* obprob_out, int32_t* backpart_out);
+
+/* Mpi::particle_sample (src/mpi.cc:331-385): the trajectory (kept transnum,
+ * transmean and pop, section by section) of the particle the last bootstrap
+ * of the last beep_pmcmc_obs_prob call put first, traced back through the
+ * ancestry, is gathered into instance dst (which may be one of the
+ * particles, as particle 0 is in the reference). */
+int beep_pmcmc_particle_sample(beep_ctx* ctx, int dst);
+
+/* PMCMC::bootstrap alone (src/pmcmc.cc:211-268) on host arrays: section
+ * log-likelihoods L[n] -> ancestors bp_out[n] and the section's term of the
+ * estimate.  Draw j uses the same device uniform as section `sec` of
+ * beep_pmcmc_obs_prob under (seed, stream). */
+int beep_bootstrap(int device, int n, const double* L, double invT, uint64_t seed, uint32_t stream, int sec,
+                   int32_t* bp_out, double* obprob_out);
+
 /* Waits for the context's stream. */
 int beep_synchronize(beep_ctx* ctx);
 

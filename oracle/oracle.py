@@ -217,6 +217,85 @@ class Oracle:
         return pro
 
 
+def bootstrap(L, invT, u):
+    """PMCMC::bootstrap (src/pmcmc.cc:211-268), sequentially in FP64: the
+    weights exp(invT (L - Lmax)) and their running sum (224-233), the
+    estimate invT Lmax + log(sum / N) (235), draw p at z = u[p] * sum found by
+    the linear search 'first pp with z <= wsum[pp]' (245-248), the first draw
+    of a particle keeps it in place and later ones become extras (250-251),
+    and the particles no draw chose take extras from the back, in increasing
+    order (254-259).  Returns (ancestors, estimate term)."""
+    import bisect
+    import math
+    N = len(L)
+    Lmax = max(L)
+    wsum, tot = [], 0.0
+    for p in range(N):
+        tot += math.exp(invT * (L[p] - Lmax))
+        wsum.append(tot)
+    obprob = invT * Lmax + math.log(tot / N)
+    bp = [-1] * N
+    extra = []
+    for p in range(N):
+        z = u[p] * tot
+        pp = bisect.bisect_left(wsum, z)          # == the linear search on nondecreasing wsum
+        if pp == N:
+            raise RuntimeError("bootstrap: draw beyond the weights")
+        if bp[pp] == -1:
+            bp[pp] = pp
+        else:
+            extra.append(pp)
+    for p in range(N):
+        if bp[p] == -1:
+            bp[p] = extra.pop()
+    if extra:
+        raise RuntimeError("bootstrap: extras left over")
+    return np.array(bp, dtype=np.int32), obprob
+
+
+def particle_sample(backpart, sections, traj):
+    """Mpi::particle_sample (src/mpi.cc:331-385) on host arrays: backpart
+    [(nsec+1)][N]; traj[p] an array [S, ...] of particle p's per-step rows.
+    Returns the gathered [S, ...] trajectory."""
+    nsec = len(sections)
+    out = np.array(traj[0], copy=True)
+    p = backpart[nsec][0]
+    for sec in range(nsec - 1, -1, -1):
+        ti, tf = sections[sec]
+        out[ti:tf] = traj[p][ti:tf]
+        p = backpart[sec][p]
+    return out
+
+
+def pmcmc_obs_prob(orc, tb, N, invT, seeds, uniforms):
+    """PMCMC::obs_prob (src/pmcmc.cc:180-207) with the oracle's states: per
+    section, each particle continues from its ancestor's populations at the
+    section start (Mpi::end_sec_swap, src/mpi.cc:215-327), simulates the
+    section with libstdc++ draws seeded by seeds[sec][p], is scored by the
+    section's observations, and the bootstrap draws with uniforms[sec].
+    Returns (estimate, backpart, states)."""
+    sections = orc.spec["sections"]
+    obs = orc.spec.get("obs", [])
+    states = [orc.new_state() for _ in range(N)]
+    backpart = [list(range(N))]
+    ob = 0.0
+    for sec, (ti, tf) in enumerate(sections):
+        if sec > 0:
+            bp = backpart[sec]
+            start = [st.pop[ti].copy() for st in states]
+            for p in range(N):
+                states[p].pop[ti] = start[bp[p]]
+        sel = [i for i, o in enumerate(obs) if o["sett_i"] >= ti and o["sett_f"] <= tf]
+        L = []
+        for p in range(N):
+            orc.simulate(tb, seed=int(seeds[sec][p]), ti=ti, tf=tf, state=states[p])
+            L.append(orc.obs_loglik(states[p], ti, tf, sel=sel)[0])
+        bp, term = bootstrap(L, invT, uniforms[sec])
+        backpart.append(list(bp))
+        ob += term
+    return ob, np.array(backpart, dtype=np.int32), states
+
+
 def get_al(mode, EFp, EFi, Prp, Pri, invT=1.0, EFcut=0.0):
     return lib().or_get_al(mode, EFp, EFi, Prp, Pri, invT, EFcut)
 

@@ -181,3 +181,57 @@ def test_obs_prob_edge_cases():
     assert O.obs_prob(2.0, th, 0.0, 2, 1.0, 4.0) == pytest.approx(np.log(stats.poisson.cdf(4, 2.0)))
     assert O.obs_prob(1.0, th, 3.0, 0, 1.0, 5.0) == pytest.approx(stats.norm.logpdf(5.0, 5.0, 3.0))
     assert O.obs_prob(9.0, th, 3.0, 0, 1.0, 5.0) == pytest.approx(stats.norm.logpdf(5.0, 9.0, 3.0))
+
+
+def test_bootstrap_hand_worked_cases():
+    """PMCMC::bootstrap restatement against cases worked by hand from
+    src/pmcmc.cc:224-260."""
+    L = np.log([1.0, 2.0, 1.0])                  # w = 0.5, 1, 0.5; wsum = 0.5, 1.5, 2
+    bp, ob = O.bootstrap(L, 1.0, [0.1, 0.5, 0.9])
+    assert bp.tolist() == [0, 1, 2]
+    assert ob == pytest.approx(np.log(2.0) + np.log(2.0 / 3.0))
+    assert O.bootstrap(L, 1.0, [0.6, 0.65, 0.1])[0].tolist() == [0, 1, 1]
+    assert O.bootstrap(L, 1.0, [0.9, 0.95, 0.99])[0].tolist() == [2, 2, 2]
+    # Extras are handed out from the back: draws 1,1,3,3 leave extras [1, 3];
+    # particle 0 takes 3, particle 2 takes 1.
+    assert O.bootstrap(np.zeros(4), 1.0, [0.3, 0.3, 0.8, 0.8])[0].tolist() == [3, 1, 1, 3]
+    # A boundary draw z == wsum[p] selects p (linear search stops at z <= wsum).
+    assert O.bootstrap(np.zeros(4), 1.0, [0.25, 0.5, 0.75, 1.0])[0].tolist() == [0, 1, 2, 3]
+    # invT scales the log-likelihoods inside the weights and the estimate.
+    bp, ob = O.bootstrap(np.array([-10.0, -12.0]), 0.5, [0.2, 0.9])
+    w = np.exp(0.5 * np.array([0.0, -2.0]))
+    assert ob == pytest.approx(0.5 * -10.0 + np.log(w.sum() / 2))
+    assert 0.9 * w.sum() > w[0] and bp.tolist() == [0, 1]
+
+
+def test_bootstrap_closed_forms_and_law():
+    N = 50
+    bp, ob = O.bootstrap(np.full(N, -7.5), 0.8, (np.arange(N) + 0.5) / N)   # equal weights, one draw each
+    assert bp.tolist() == list(range(N)) and ob == pytest.approx(0.8 * -7.5)
+    L = np.full(N, -np.inf)
+    L[17] = -3.0                                 # one particle carries all the weight
+    bp, ob = O.bootstrap(L, 1.0, np.random.default_rng(1).uniform(size=N))
+    assert (bp == 17).all() and ob == pytest.approx(-3.0 + np.log(1.0 / N))
+    # Multinomial law: each ancestor's offspring count has mean N w_p / sum w.
+    rng = np.random.default_rng(2)
+    L = rng.normal(0, 1, N)
+    w = np.exp(L - L.max())
+    counts = np.zeros(N)
+    R = 2000
+    for _ in range(R):
+        bp, _ob = O.bootstrap(L, 1.0, rng.uniform(size=N))
+        counts += np.bincount(bp, minlength=N)
+    expect = R * N * w / w.sum()
+    chi2 = ((counts - expect) ** 2 / expect).sum()
+    assert stats.chi2.sf(chi2, N - 1) > 1e-4
+
+
+def test_particle_sample_traces_the_ancestry():
+    """Mpi::particle_sample restatement: each section's rows come from the
+    ancestor named by the ancestry rows, traced from particle 0 backwards."""
+    sections = [(0, 2), (2, 4), (4, 6)]
+    traj = [np.full((6, 1), 10 * p) + np.arange(6)[:, None] for p in range(3)]
+    backpart = [[0, 1, 2], [2, 2, 0], [1, 0, 0], [1, 2, 2]]
+    out = O.particle_sample(backpart, sections, traj)
+    # section 2 from bp[3][0] = 1; section 1 from bp[2][1] = 0; section 0 from bp[1][0] = 2
+    assert out[:, 0].tolist() == [20, 21, 2, 3, 14, 15]

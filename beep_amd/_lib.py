@@ -16,7 +16,9 @@ BEEP_OK = 0
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ENAN", -5: "EBOUNDS", -6: "ELARGE"}
 KEEP_TRANSNUM, KEEP_TRANSMEAN, KEEP_POP, KEEP_ALL = 1, 2, 4, 7
 STEPWISE = 16
+GENERIC_KERNEL = 32
 SIM_PLAIN = 2
+PATH_STEPWISE, PATH_FUSED, PATH_FUSED_SPEC = 0, 1, 2
 
 # Exported symbols declared by include/beep_gpu.h.
 SYMBOLS = [
@@ -25,7 +27,7 @@ SYMBOLS = [
     "beep_accept", "beep_get_transnum", "beep_get_transmean", "beep_get_pop", "beep_get_state", "beep_put_state",
     "beep_put_transnum", "beep_copy_instances", "beep_synchronize", "beep_debug_philox", "beep_debug_poisson",
     "beep_debug_binomial", "beep_timing", "beep_debug_gemm", "beep_run_batch", "beep_copy_states",
-    "beep_derive_slots", "beep_sum_transnum", "beep_pmcmc_obs_prob", "beep_pmcmc_particle_sample", "beep_bootstrap",
+    "beep_derive_slots", "beep_sum_transnum", "beep_pmcmc_obs_prob", "beep_pmcmc_particle_sample", "beep_bootstrap", "beep_path",
 ]
 
 
@@ -133,6 +135,7 @@ def lib():
             "beep_pmcmc_obs_prob": (C.c_int, [P, C.c_int, i32p, C.c_int, u32p, C.c_double, C.c_uint64, C.c_uint32,
                                               f64p, i32p]),
             "beep_pmcmc_particle_sample": (C.c_int, [P, C.c_int]),
+            "beep_path": (C.c_int, [P]),
             "beep_bootstrap": (C.c_int, [C.c_int, C.c_int, f64p, C.c_double, C.c_uint64, C.c_uint32, C.c_int, i32p,
                                          f64p]),
         }
@@ -496,6 +499,11 @@ class Context:
     def pmcmc_particle_sample(self, dst):
         """Mpi::particle_sample of the last particle-filter run into instance dst."""
         self._chk(lib().beep_pmcmc_particle_sample(self._h, int(dst)))
+
+    @property
+    def path(self):
+        """PATH_STEPWISE, PATH_FUSED or PATH_FUSED_SPEC."""
+        return lib().beep_path(self._h)
 
     def timing(self):
         """(simulate/MBP kernel ms, observation kernel ms, Poisson log-lik kernel ms, launches)."""

@@ -255,6 +255,20 @@ def dense_pmcmc(A=317, period=300, seed=20250103, with_obs=True):
     return spec, truth
 
 
+def daily_observations(transnum, tr, percent=10.0, datatable=0):
+    """Daily observations of transition tr per area (all age groups summed),
+    with normal errors of `percent` percent, from a transnum trajectory
+    [S][A][T][G] -- e.g. one simulation at the true parameters, so that a
+    likelihood of the model at those parameters is not degenerate."""
+    tn = np.asarray(transnum)
+    S, A = tn.shape[0], tn.shape[1]
+    y = tn[:, :, tr, :].sum(-1)
+    G = tn.shape[3]
+    return [{"y": float(y[d, c]), "type": "normal_percent", "percent": percent, "datatable": datatable,
+             "graph": c, "sett_i": d, "sett_f": d + 1, "area": [c], "dp": list(range(G)), "tr": [tr]}
+            for c in range(A) for d in range(S)]
+
+
 def seir_areas(A, G, period, seed, obs_every=0):
     """SEIR in every area with 10-NN mixing, G in {1, 4, 9} age groups."""
     rng = np.random.default_rng(seed)

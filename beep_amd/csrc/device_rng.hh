@@ -31,8 +31,15 @@
 // variates (superposition) and binomials of 2^23 trials and above as sums of
 // binomials over chunks of the trials, both exact in distribution.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#else
+// Run-time compilation provides the fixed-width types in its own namespace.
+typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+#endif
 
 namespace beepdev {
 

@@ -1,0 +1,278 @@
+// Structure-specialised fused simulation kernel, compiled at run time for one
+// compartment structure (hiprtc).  The text compiled is a prelude defining
+// namespace beepspec, then device_rng.hh, fused_common.hh and this file.  The
+// prelude fixes:
+//   constexpr int C, T, G, CP;        compartments, transitions, age groups,
+//                                     C rounded up to a multiple of 4
+//   constexpr int INF_TR;             the infection transition
+//   constexpr int FROM[T], TO[T];     transition source / destination
+//   constexpr bool INFC[C];           compartments with an infectivity weight
+//
+// It computes exactly what k_fused (kernels.hh) computes -- the same FP32
+// expressions in the same order and the same uniforms consumed in the same
+// order -- so trajectories are bit-identical to the generic kernel and to the
+// step-wise path for the same keys.  What the specialisation buys: a cell's
+// populations sit in registers through the step with compile-time source and
+// destination indices, the transition loop is unrolled, divisions by G are by
+// a constant, and populations are single-buffered in LDS as [AG][CP] so one
+// cell's compartments load and store as 16-byte vectors.
+//
+// Per step: (B) area mixing, (C) means, small draws, updates of the cell's own
+// populations, large draws queued, (D) the queued draws by all lanes, applied
+// with LDS atomics, (E) the infectivity of each cell for the next step.
+#pragma once
+
+namespace beepdev {
+
+static_assert(beepspec::CP % 4 == 0 && beepspec::CP >= beepspec::C, "CP");
+
+template <int MODE>
+__device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t seed, uint32_t stream) {
+  using namespace beepspec;
+  const Job jb = jobs[blockIdx.x];
+  const int mode = MODE == MODE_ANY ? jb.mode : MODE;
+  const TopoDev* __restrict__ tpp = jb.topo;
+  const int A = tpp->A, S = tpp->S, AG = tpp->AG;
+  const int nt = blockDim.x, tid = threadIdx.x;
+  const float dt = tpp->dt;
+  const int qcap = T * AG;                             // every (cell, transition): never full
+
+  extern __shared__ int smem[];
+  int* pop = smem;                                     // [AG][CP]
+  float* sdinf = (float*)(pop + (size_t)AG * CP);      // [AG]
+  float* sI = sdinf + AG;                              // [AG]
+  uint32_t* qcode = (uint32_t*)(sI + AG);              // [qcap]
+  float* qval = (float*)(qcode + qcap);                // [qcap]
+  int* qcount = (int*)(qval + qcap);                   // [4]
+  float* srate = (float*)(qcount + 4);                 // [T][G]
+  float* sN = srate + T * G;                           // [G][G]
+  float* ssus = sN + G * G;                            // [G]
+  float* sinf = ssus + G;                              // [C]
+  const float* diag = tpp->diag;
+  const float* fac = tpp->fac;
+  const int* in_ptr = tpp->in_ptr;
+  const int* in_src = tpp->in_src;
+  const float* in_w = tpp->in_w;
+  if (tpp->stage_mix) {
+    float* sdiag = sinf + C;
+    float* sfac = sdiag + A;
+    int* sptr = (int*)(sfac + A);
+    int* ssrc = sptr + A + 1;
+    float* sw = (float*)(ssrc + tpp->nnz);
+    for (int i = tid; i < A; i += nt) { sdiag[i] = diag[i]; sfac[i] = fac[i]; }
+    for (int i = tid; i <= A; i += nt) sptr[i] = in_ptr[i];
+    for (int i = tid; i < tpp->nnz; i += nt) { ssrc[i] = in_src[i]; sw[i] = in_w[i]; }
+    diag = sdiag; fac = sfac; in_ptr = sptr; in_src = ssrc; in_w = sw;
+  }
+
+  const float* __restrict__ tab = jb.tab;
+  {
+    const int* src = jb.from_init ? tpp->pop_init : jb.state;
+    for (int cell = tid; cell < AG; cell += nt) {
+#pragma unroll
+      for (int c = 0; c < CP; c++) pop[cell * CP + c] = c < C ? src[c * AG + cell] : 0;
+    }
+    for (int i = tid; i < T * G; i += nt) srate[i] = tab[tpp->off_rate + i];
+    for (int i = tid; i < G * G; i += nt) sN[i] = tab[tpp->off_N + i];
+    for (int i = tid; i < G; i += nt) ssus[i] = tab[tpp->off_sus + i];
+    for (int i = tid; i < C; i += nt) sinf[i] = tab[tpp->off_inf + i];
+    if (tid == 0) qcount[0] = 0;
+  }
+  __syncthreads();
+
+  // (E) for the first step: dinf = sum_c inf_c * pop_c (State::set_I_from_pop).
+  for (int cell = tid; cell < AG; cell += nt) {
+    const int4* pv = (const int4*)(pop + cell * CP);
+    float d = 0.f;
+#pragma unroll
+    for (int c4 = 0; c4 < CP / 4; c4++) {
+      int4 v = pv[c4];
+      int pc[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int c = 4 * c4 + k;
+        if (c < C && INFC[c]) {
+          float w = sinf[c];
+          if (w != 0.f) d += w * (float)pc[k];
+        }
+      }
+    }
+    sdinf[cell] = d;
+  }
+  __syncthreads();
+
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  const size_t TAG = (size_t)T * AG, CAG = (size_t)C * AG;
+
+  for (int s = jb.t0; s < jb.t1; s++) {
+    // (B) Imap = M_off^T dinf, Idiag = diag * dinf, I = (Idiag + Imap) m + fac Idiag (1 - m).
+    const float m = tab[tpp->off_geo + s];
+    for (int cell = tid; cell < AG; cell += nt) {
+      const int j = cell / G, g = cell - j * G;
+      float idg = diag[j] * sdinf[cell];
+      float imp = 0.f;
+      int e1 = in_ptr[j + 1];
+      for (int e = in_ptr[j]; e < e1; e++) imp += in_w[e] * sdinf[in_src[e] * G + g];
+      sI[cell] = (idg + imp) * m + fac[j] * idg * (1.f - m);
+    }
+    __syncthreads();
+    // (C) means, small draws, own-cell updates; large draws queued.
+    const float* beta_s = tab + tpp->off_beta + (size_t)s * A;
+    const float* efoi_s = tab + tpp->off_efoi + (size_t)s * G;
+    int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;
+    float* tm_s = jb.tm ? jb.tm + (size_t)s * TAG : nullptr;
+    int* pt_s = jb.pt ? jb.pt + (size_t)s * CAG : nullptr;
+    const int* itn = mode == MODE_MBP ? jb.init_tn + (size_t)s * TAG : nullptr;
+    const float* itm = mode == MODE_MBP ? jb.init_tm + (size_t)s * TAG : nullptr;
+    const bool update = s < S - 1;           // the last step is not applied (src/state.cc:634)
+    const uint32_t c0 = (uint32_t)s | (stream << 16);
+    for (int cell = tid; cell < AG; cell += nt) {
+      const int j = cell / G, g = cell - j * G;
+      int p[CP];
+      {
+        const int4* pv = (const int4*)(pop + cell * CP);
+#pragma unroll
+        for (int c4 = 0; c4 < CP / 4; c4++) {
+          int4 v = pv[c4];
+          p[4 * c4] = v.x; p[4 * c4 + 1] = v.y; p[4 * c4 + 2] = v.z; p[4 * c4 + 3] = v.w;
+        }
+      }
+      if (pt_s) {
+#pragma unroll
+        for (int c = 0; c < C; c++) st_nt(pt_s, c * AG + cell, p[c]);
+      }
+      int q[CP];
+#pragma unroll
+      for (int c = 0; c < CP; c++) q[c] = p[c];
+      float nmi = 0.f;
+#pragma unroll
+      for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * sI[j * G + gg];
+      if (nmi < 0.f) nmi = 0.f;
+      const float foi = beta_s[j] * nmi + efoi_s[g];
+      const bool simu = mode == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
+      CellRng rng;
+      rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
+#pragma unroll
+      for (int tr = 0; tr < T; tr++) {
+        const int fr = FROM[tr], to = TO[tr];
+        float popu = (float)p[fr];
+        float lp = 0.f;
+        if (popu > 0.f) lp = tr == INF_TR ? dt * popu * ssus[g] * foi : dt * popu * srate[tr * G + g];
+        // src/state.cc:429 (NaN mean) and src/utils.cc:150 (mean > 1e9) stop the reference.
+        if (lp != lp) { atomicOr(jb.err, 1); lp = 0.f; }
+        else if (lp > 1e9f) { atomicOr(jb.err, 2); lp = 0.f; }
+        if (tm_s) st_nt(tm_s, tr * AG + cell, lp);
+        int n = 0;
+        int kind = -1;
+        float val = 0.f;
+        if (mode == MODE_REPLAY) {
+          n = tn_s[tr * AG + cell];
+        } else if (lp > 0.f) {
+          if (mode == MODE_SIM || simu) {
+            if (lp < BEEP_POIS_INLINE) n = poisson_inversion(lp, rng.next());
+            else { kind = ITEM_POIS; val = lp; }
+          } else {
+            // Coupled proposal (src/mbp.cc:286-311).
+            int ni = itn[tr * AG + cell];
+            float li = itm[tr * AG + cell];
+            if (lp > li) {
+              float dl = lp - li;
+              if (dl < BEEP_POIS_INLINE) n = ni + poisson_inversion(dl, rng.next());
+              else { kind = ITEM_POIS_ADD; val = dl; }
+            } else if (lp == li) {
+              n = ni;
+            } else {
+              float pr = lp / li;
+              float pp = pr > 0.5f ? 1.f - pr : pr;
+              if ((float)ni * pp < 10.f) n = binomial_draw(ni, pr, rng.next(), rng);
+              else { kind = ITEM_BINOM; val = pr; }
+            }
+          }
+        }
+        if (kind >= 0) {                     // drawn in (D)
+          int slot = atomicAdd(qcount, 1);
+          qcode[slot] = (uint32_t)cell | ((uint32_t)tr << 20) | ((uint32_t)kind << 28);
+          qval[slot] = val;
+        } else {
+          if (tn_s && mode != MODE_REPLAY) st_nt(tn_s, tr * AG + cell, n);
+          if (update) {
+            q[fr] -= n;
+            q[to] += n;
+          }
+        }
+      }
+      {
+        int4* pv = (int4*)(pop + cell * CP);
+#pragma unroll
+        for (int c4 = 0; c4 < CP / 4; c4++) pv[c4] = make_int4(q[4 * c4], q[4 * c4 + 1], q[4 * c4 + 2], q[4 * c4 + 3]);
+      }
+    }
+    __syncthreads();
+    // (D) the queued large draws (PTRS / BTRD), every lane busy.
+    if (mode != MODE_REPLAY) {
+      const int nq = qcount[0];
+      if (nq > 0) {
+        for (int i = tid; i < nq; i += nt) {
+          uint32_t code = qcode[i];
+          int cell = (int)(code & 0xFFFFFu), tr = (int)((code >> 20) & 0xFFu), kind = (int)(code >> 28);
+          int ni = mode == MODE_MBP ? itn[tr * AG + cell] : 0;
+          int n = draw_item(kind, qval[i], ni, k0, k1, c0, (uint32_t)cell, (uint32_t)tr, jb.key);
+          if (tn_s) st_nt(tn_s, tr * AG + cell, n);
+          if (update && n != 0) {
+            int fr = 0, to = 0;
+#pragma unroll
+            for (int k = 0; k < T; k++)
+              if (k == tr) { fr = FROM[k]; to = TO[k]; }
+            atomicSub(&pop[cell * CP + fr], n);
+            atomicAdd(&pop[cell * CP + to], n);
+          }
+        }
+        __syncthreads();
+        if (tid == 0) qcount[0] = 0;
+      }
+    }
+    // (E) infectivity for the next step.
+    for (int cell = tid; cell < AG; cell += nt) {
+      const int4* pv = (const int4*)(pop + cell * CP);
+      float d = 0.f;
+#pragma unroll
+      for (int c4 = 0; c4 < CP / 4; c4++) {
+        int4 v = pv[c4];
+        int pc[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int c = 4 * c4 + k;
+          if (c < C && INFC[c]) {
+            float w = sinf[c];
+            if (w != 0.f) d += w * (float)pc[k];
+          }
+        }
+      }
+      sdinf[cell] = d;
+    }
+    __syncthreads();
+  }
+  for (int cell = tid; cell < AG; cell += nt) {
+#pragma unroll
+    for (int c = 0; c < C; c++) jb.state[c * AG + cell] = pop[cell * CP + c];
+  }
+}
+
+}  // namespace beepdev
+
+extern "C" __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, beepspec::MIN_WAVES)
+    beep_fused_spec_sim(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
+  beepdev::fused_spec_body<beepdev::MODE_SIM>(jobs, seed, stream);
+}
+extern "C" __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, beepspec::MIN_WAVES)
+    beep_fused_spec_mbp(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
+  beepdev::fused_spec_body<beepdev::MODE_MBP>(jobs, seed, stream);
+}
+extern "C" __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, beepspec::MIN_WAVES)
+    beep_fused_spec_replay(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
+  beepdev::fused_spec_body<beepdev::MODE_REPLAY>(jobs, seed, stream);
+}
+extern "C" __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, beepspec::MIN_WAVES)
+    beep_fused_spec_any(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
+  beepdev::fused_spec_body<beepdev::MODE_ANY>(jobs, seed, stream);
+}

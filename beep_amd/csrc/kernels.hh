@@ -12,87 +12,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "device_rng.hh"
+#include "fused_common.hh"
 
 namespace beepdev {
-
-struct TopoDev {
-  int A, G, C, T, S, AG, inf_tr, spd;
-  int nnz;               // off-diagonal mixing entries
-  int stage_mix;         // 1: the mixing CSR is copied into LDS
-  float dt;
-  int off_rate, off_sus, off_inf, off_N, off_efoi, off_geo, off_beta;
-  const int* from;       // [T]
-  const int* to;         // [T]
-  const float* diag;     // [A]  normalised diagonal of the area mixing
-  const float* fac;      // [A]  genQ.factor
-  const int* in_ptr;     // [A+1] incoming off-diagonal mixing (gather form)
-  const int* in_src;     // [nnz]
-  const float* in_w;     // [nnz]
-  const int* pop_init;   // [C][AG]
-};
-
-// One instance's work in a launch.
-struct Job {
-  const TopoDev* topo;
-  const float* tab;           // parameter slot tables
-  int* state;                 // [C][AG]
-  int* tn;                    // [S][T][AG] or null
-  float* tm;                  // [S][T][AG] or null
-  int* pt;                    // [S][C][AG] or null
-  const int* init_tn;         // MBP: initial instance's transnum
-  const float* init_tm;       // MBP: initial instance's transmean
-  const uint8_t* simu;        // MBP: [S][A] 1 = simulate afresh, or null
-  int* err;                   // bit 0: NaN mean, bit 1: mean > 1e9
-  uint32_t key;
-  int t0, t1;
-  int from_init;              // 1: start from topo->pop_init
-  int mode;                   // MODE_SIM / MODE_MBP / MODE_REPLAY (read by MODE_ANY launches)
-};
-
-enum { MODE_SIM = 0, MODE_MBP = 1, MODE_REPLAY = 2, MODE_ANY = 3 };
-
-// Trajectory outputs are written once and read by later kernels: global
-// address space, non-temporal (streaming) stores.
-typedef __attribute__((address_space(1))) int gint;
-typedef __attribute__((address_space(1))) float gfloat;
-__device__ inline void st_nt(int* p, size_t i, int v) { __builtin_nontemporal_store(v, (gint*)p + i); }
-__device__ inline void st_nt(float* p, size_t i, float v) { __builtin_nontemporal_store(v, (gfloat*)p + i); }
-
-// Queue capacity of large draws per step (items beyond it are drawn in place).
-__host__ __device__ inline int fused_qcap(int AG) { return AG < 32 ? 64 : (2 * AG < 1024 ? 2 * AG : 1024); }
-
-// Bytes of the mixing CSR when staged in LDS.
-__host__ __device__ inline size_t fused_mix_bytes(int A, int nnz) { return (size_t)4 * (3 * (size_t)A + 1 + 2 * (size_t)nnz); }
-
-// LDS bytes a job needs: populations double-buffered, infectivity, the mixed
-// infectivity I, the queue of large draws, the small per-slot tables and,
-// when staged, the mixing CSR.
-__host__ __device__ inline size_t fused_lds_bytes(int A, int G, int C, int T, int nnz, int stage_mix) {
-  int AG = A * G;
-  size_t b = (size_t)4 * (2 * (size_t)C * AG + 2 * (size_t)AG + 2 * (size_t)fused_qcap(AG) + 4 + (size_t)T * G +
-                          (size_t)G * G + G + C + 2 * T);
-  if (stage_mix) b += fused_mix_bytes(A, nnz);
-  return b;
-}
-
-#define BEEP_FUSED_MAX_BLOCK 512
-#define BEEP_POIS_INLINE 10.0f     // means below this are drawn in place by inversion
-
-enum { ITEM_POIS = 0, ITEM_POIS_ADD = 1, ITEM_BINOM = 2 };
-
-// A large draw: its own Philox stream (word2 = 0x80000000 | tr << 22 | block)
-// so the value does not depend on which lane draws it or when.
-__device__ inline int draw_item(int kind, float val, int ni, uint32_t k0, uint32_t k1, uint32_t c0, uint32_t cell,
-                                uint32_t tr, uint32_t key) {
-  ExtraUniforms xr;
-  xr.init(k0, k1, c0, cell, key);
-  xr.block = 0x80000000u | (tr << 22);
-  float u = xr.next();
-  if (kind == ITEM_POIS) return poisson_draw(val, u, xr);
-  if (kind == ITEM_POIS_ADD) return ni + poisson_draw(val, u, xr);
-  return binomial_draw(ni, val, u, xr);
-}
 
 // One workgroup advances one instance over steps [t0, t1).  Each thread owns
 // the cells tid, tid + blockDim, ...; populations live in LDS for the whole

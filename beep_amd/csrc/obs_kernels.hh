@@ -20,6 +20,7 @@ struct ObsDev {
   const double* shape;
   const double* factor;
   const double* thresh;
+  const int* orig;           // the model's index of each stored observation
 };
 
 __device__ inline double poisson_lpmf(double k, double lam) {          // src/utils.cc:157-170
@@ -119,7 +120,7 @@ __global__ void k_obs_values(const ObsDev* __restrict__ odp, const int* __restri
                              double* __restrict__ out) {
   const ObsDev od = *odp;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < od.nobs) out[i] = obs_value(od, i, tn, pt);
+  if (i < od.nobs) out[od.orig[i]] = obs_value(od, i, tn, pt);
 }
 
 // State::likelihood (src/state.cc:812-826): one workgroup per instance.

@@ -28,8 +28,8 @@ METRIC = "particle-timesteps/sec batched vs plain GPU path; p50/p99 request late
 # HBM bytes per launch of the fused kernel on this workload (4096 particles),
 # from rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the
 # ratio of known bytes to counter reading on a plain copy kernel
-# (profiles/k_fused_spatial_abcsmc_traffic.md).  None until measured.
-FUSED_TRAFFIC_BYTES = None
+# (profiles/k_fused_spatial_abcsmc_traffic.md).
+FUSED_TRAFFIC_BYTES = 41_895_375_565
 
 
 def algorithmic_bytes_per_particle_step(A, G, C, T):
@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--particles", type=int, default=PARTICLES)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--generic", action="store_true", help="use the generic fused kernel, not the specialised one")
     ap.add_argument("--workload", default="spatial", choices=["spatial", "uneven", "dense", "serving", "copy"])
     ap.add_argument("--rates", default="5,20,80")
     ap.add_argument("--requests", type=int, default=120)
@@ -122,7 +123,9 @@ def main():
     # The CPU baseline runs first, in forked processes, before anything here
     # initialises the GPU.
     cpu = None if args.no_cpu else cpu_baseline(spec, params)
-    ctx = _lib.Context(spec, capacity=P, nslot=P, keep=_lib.KEEP_ALL)
+    ctx = _lib.Context(spec, capacity=P, nslot=P, keep=_lib.KEEP_ALL | (_lib.GENERIC_KERNEL if args.generic else 0))
+    kernel = {_lib.PATH_FUSED: "k_fused<MODE_SIM> (generic)", _lib.PATH_FUSED_SPEC: "beep_fused_spec_sim (specialised)",
+              _lib.PATH_STEPWISE: "step-wise"}[ctx.path]
     slots = np.arange(P, dtype=np.int32)
     ctx.set_param(slots, params)
     inst = np.arange(P, dtype=np.int32)
@@ -180,7 +183,7 @@ def main():
                    "particles": P, "days": S, "areas": A, "age_groups": G},
         "plain": {"value": plain_value, "sample_instances": ns, "batched_over_plain": value / plain_value},
         "kernels_ms": {"simulate_fused": sim_avg, "obs_loglik": float(np.mean(obs_ms))},
-        "roofline": {"bound": "hbm", "kernel": "k_fused<MODE_SIM,8>", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": FUSED_TRAFFIC_BYTES if P == PARTICLES else None,
                      "bytes_per_particle_step": bytes_per, "algorithmic_bytes_per_launch": bytes_per * P * S},
@@ -251,34 +254,56 @@ def bench_uneven(args):
 
 
 def bench_dense(args):
-    """317 areas x 9 age groups with dense gravity mixing, 1024 PMCMC
-    particles sharing one parameter vector, 300 days: step-wise path with the
-    contraction as an MFMA GEMM, replayed from a hipGraph."""
+    """317 areas x 9 age groups with dense gravity mixing, 300 days in ten
+    30-day observation sections: one step is one PMCMC likelihood estimate
+    (PMCMC::obs_prob) over 1024 particles sharing one parameter vector --
+    per section the step-wise simulation (contraction as an MFMA GEMM,
+    replayed from a hipGraph), the section log-likelihood, the FP64 bootstrap
+    and the state moves, all on the device."""
     import time
     from beep_amd import _lib, configs
-    spec, truth = configs.dense_pmcmc()
+    # Observations: daily I->H counts per area from one simulation at the true
+    # parameters (synthetic data the model can explain).
+    spec, truth = configs.dense_pmcmc(with_obs=False)
+    c0 = _lib.Context(spec, capacity=1, nslot=1, keep=_lib.KEEP_TRANSNUM)
+    c0.set_param([0], truth[None, :])
+    c0.simulate([0], [0], [123456], seed=17, stream=1)
+    spec["obs"] = configs.daily_observations(c0.get_transnum(0), tr=6)
+    c0.close()
     P = 1024
     ctx = _lib.Context(spec, capacity=P, nslot=1)
     ctx.set_param([0], truth[None, :])
     inst = np.arange(P, dtype=np.int32)
-    slot = np.zeros(P, dtype=np.int32)
-    ctx.simulate(inst, slot, inst.astype(np.uint32), seed=3, stream=99)
-    ms = []
+    keys = inst.astype(np.uint32)
+    for w in range(max(1, args.warmup)):
+        ctx.pmcmc_obs_prob(inst, 0, keys, invT=1.0, seed=3, stream=900 + w)
+    ctx.synchronize()
+    ms, est = [], []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        ctx.simulate(inst, slot, inst.astype(np.uint32), seed=3, stream=k)
+        ob, _bp = ctx.pmcmc_obs_prob(inst, 0, keys, invT=1.0, seed=3, stream=k)
         ms.append(ctx.timing()[0])
-        ctx.obs_loglik(inst)
+        est.append(ob)
+    ctx.synchronize()
     wall = time.perf_counter() - t0
     S = ctx.S
+    # Contraction alone: one full-horizon simulation, its GEMM flops over the
+    # simulate kernels' device time.
+    ctx.simulate(inst, np.zeros(P, dtype=np.int32), keys, seed=3, stream=999)
+    sim_ms = ctx.timing()[0]
     flop = 2.0 * ctx.A * ctx.A * ctx.G * P * S
-    out = {"metric": "particle-timesteps/sec (dense PMCMC batch, simulation + observation log-likelihood)",
+    out = {"metric": METRIC,
            "value": args.steps * P * S / wall, "unit": "particle-timesteps/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": 1, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "int32 counts, fp32 rates, fp64 log-likelihood", "data": "synthetic",
+           "warmup": max(1, args.warmup), "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int32 counts, fp32 rates, fp64 log-likelihood",
+           "data": "synthetic",
            "config": {"workload": "dense PMCMC: 317 areas x 9 age groups, 10 compartments, dense gravity mixing, "
-                                  "300 days, 1024 particles"},
-           "simulate_ms": float(np.mean(ms)), "contraction_tflops_algorithmic_whole_sim": flop / (np.mean(ms) * 1e-3) / 1e12}
+                                  "300 days, 10 sections, 1024 particles, particle-filter likelihood per step",
+                      "observations": "daily I->H per area from one simulation at the true parameters"},
+           "pf_device_ms": float(np.mean(ms)), "log_likelihood_estimates": est,
+           "simulate_ms_full_horizon": sim_ms,
+           "contraction_flop_per_particle_step": 2.0 * ctx.A * ctx.A * ctx.G,
+           "contraction_tflops_algorithmic_over_whole_simulate": flop / (sim_ms * 1e-3) / 1e12}
     ctx.close()
     print(json.dumps(out), flush=True)
 

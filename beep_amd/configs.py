@@ -259,14 +259,16 @@ def daily_observations(transnum, tr, percent=10.0, datatable=0):
     """Daily observations of transition tr per area (all age groups summed),
     with normal errors of `percent` percent, from a transnum trajectory
     [S][A][T][G] -- e.g. one simulation at the true parameters, so that a
-    likelihood of the model at those parameters is not degenerate."""
+    likelihood of the model at those parameters is not degenerate.  Areas
+    whose series is zero throughout are left unobserved (their percentage
+    error would be zero)."""
     tn = np.asarray(transnum)
     S, A = tn.shape[0], tn.shape[1]
     y = tn[:, :, tr, :].sum(-1)
     G = tn.shape[3]
     return [{"y": float(y[d, c]), "type": "normal_percent", "percent": percent, "datatable": datatable,
              "graph": c, "sett_i": d, "sett_f": d + 1, "area": [c], "dp": list(range(G)), "tr": [tr]}
-            for c in range(A) for d in range(S)]
+            for c in range(A) if y[:, c].max() > 0 for d in range(S)]
 
 
 def seir_areas(A, G, period, seed, obs_every=0):

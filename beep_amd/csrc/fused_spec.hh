@@ -105,6 +105,8 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   const size_t TAG = (size_t)T * AG, CAG = (size_t)C * AG;
 
   for (int s = jb.t0; s < jb.t1; s++) {
+    const float* beta_s = tab + tpp->off_beta + (size_t)s * A;
+    const float* efoi_s = tab + tpp->off_efoi + (size_t)s * G;
     // (B) Imap = M_off^T dinf, Idiag = diag * dinf, I = (Idiag + Imap) m + fac Idiag (1 - m).
     const float m = tab[tpp->off_geo + s];
     for (int cell = tid; cell < AG; cell += nt) {
@@ -117,8 +119,6 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     }
     __syncthreads();
     // (C) means, small draws, own-cell updates; large draws queued.
-    const float* beta_s = tab + tpp->off_beta + (size_t)s * A;
-    const float* efoi_s = tab + tpp->off_efoi + (size_t)s * G;
     int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;
     float* tm_s = jb.tm ? jb.tm + (size_t)s * TAG : nullptr;
     int* pt_s = jb.pt ? jb.pt + (size_t)s * CAG : nullptr;

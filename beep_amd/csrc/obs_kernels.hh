@@ -21,6 +21,8 @@ struct ObsDev {
   const double* factor;
   const double* thresh;
   const int* orig;           // the model's index of each stored observation
+  const double* lc;          // normal types: -log(2 pi sd^2) / 2
+  const double* iv;          // normal types: 1 / (2 sd^2)
 };
 
 __device__ inline double poisson_lpmf(double k, double lam) {          // src/utils.cc:157-170
@@ -79,7 +81,14 @@ __device__ inline double block_sum(double v, double* red) {
 }
 
 // One workgroup per instance.  out[b*(1+ndt)] = log-likelihood,
-// out[b*(1+ndt)+1+d] = -2 * (sum over data table d)  (get_EF_datatable).
+// out[b*(1+ndt)+1+d] = -2 * (sum over data table d)  (get_EF_datatable),
+// written when WANT_DT.  The counts of an observation are summed as
+// integers and scaled by its factor once; normal terms use the per-
+// observation constants lc and iv.  GENERAL: the model has observations
+// other than plain normal ones (Poisson, negative binomial, thresholds,
+// unknowns); without it the kernel holds only the normal path, which keeps
+// its register use, and so its occupancy, low.
+template <bool WANT_DT, bool GENERAL>
 __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ odp, const int* __restrict__ inst,
                                                     const int* __restrict__ tn, const int* __restrict__ pt,
                                                     size_t traj_sz, size_t pop_traj_sz, const int* __restrict__ sel,
@@ -95,14 +104,30 @@ __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ o
 #pragma unroll
   for (int d = 0; d < 8; d++) acc[d] = 0;
   for (int k = threadIdx.x; k < nsel; k += blockDim.x) {
-    int i = sel ? sel[k] : k;
-    double v = obs_value(od, i, tnb, ptb);
-    double lp = obs_prob(v, od.y[i], od.sd[i], od.type[i], od.shape[i], od.thresh[i]);
+    const int i = sel ? sel[k] : k;
+    long long cnt = 0;
+    const int e1 = od.ptr[i + 1];
+    for (int e = od.ptr[i]; e < e1; e++) {
+      uint32_t x = od.elem[e];
+      cnt += (x & 0x80000000u) ? (ptb ? ptb[x & 0x7FFFFFFFu] : 0) : tnb[x];
+    }
+    const double v = od.factor[i] * (double)cnt;
+    const double y = od.y[i];
+    const int type = od.type[i];
+    double lp;
+    if (!GENERAL || (type <= 1 && y != 999999997.0 && y != 999999998.0)) {
+      const double d = y - (v < 0 ? 0.0 : v);
+      lp = od.lc[i] - d * d * od.iv[i];
+    } else {
+      lp = obs_prob(v, y, od.sd[i], type, od.shape[i], od.thresh[i]);
+    }
     L += lp;
-    int dt = od.dt[i];
+    if (WANT_DT) {
+      const int dt = od.dt[i];
 #pragma unroll
-    for (int d = 0; d < 8; d++)
-      if (d == dt) acc[d] += lp;
+      for (int d = 0; d < 8; d++)
+        if (d == dt) acc[d] += lp;
+    }
   }
   double Lt = block_sum(L, red);
   double* o = out + (size_t)b * (1 + od.ndt);
@@ -110,9 +135,11 @@ __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ o
     o[0] = Lt;
     if (store_EF) EF[in] = -2.0 * Lt;
   }
-  for (int d = 0; d < od.ndt; d++) {
-    double t = block_sum(acc[d < 8 ? d : 0], red);
-    if (threadIdx.x == 0) o[1 + d] = -2.0 * t;
+  if (WANT_DT) {
+    for (int d = 0; d < od.ndt; d++) {
+      double t = block_sum(acc[d < 8 ? d : 0], red);
+      if (threadIdx.x == 0) o[1 + d] = -2.0 * t;
+    }
   }
 }
 

@@ -18,7 +18,7 @@ KEEP_TRANSNUM, KEEP_TRANSMEAN, KEEP_POP, KEEP_ALL = 1, 2, 4, 7
 STEPWISE = 16
 GENERIC_KERNEL = 32
 SIM_PLAIN = 2
-PATH_STEPWISE, PATH_FUSED, PATH_FUSED_SPEC = 0, 1, 2
+PATH_STEPWISE, PATH_FUSED, PATH_FUSED_SPEC, PATH_STEPWISE_SPEC = 0, 1, 2, 3
 
 # Exported symbols declared by include/beep_gpu.h.
 SYMBOLS = [

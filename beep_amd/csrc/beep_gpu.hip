@@ -24,18 +24,21 @@
 #include "obs_kernels.hh"
 #include "pmcmc_kernels.hh"
 #include "step_kernels.hh"
-// kJitDeviceRng, kJitFusedCommon, kJitFusedSpec: the texts of device_rng.hh,
-// fused_common.hh and fused_spec.hh (generated by build.py).
+// kJitDeviceRng, kJitFusedCommon, kJitFusedSpec, kJitStepSpec: the texts of
+// device_rng.hh, fused_common.hh, fused_spec.hh and step_spec.hh (generated
+// by build.py).
 #include "jit_sources.inc"
 
 using beepdev::Job;
 using beepdev::TopoDev;
 
-// The fused kernel specialised for one compartment structure (fused_spec.hh),
-// compiled once per (device, structure) and kept for the life of the process.
+// The kernels specialised for one compartment structure (fused_spec.hh,
+// step_spec.hh), compiled once per (device, structure) and kept for the life
+// of the process.
 struct SpecKernels {
   hipModule_t mod = nullptr;
-  hipFunction_t fn[4] = {};        // indexed by MODE_SIM / MBP / REPLAY / ANY
+  hipFunction_t fn[4] = {};        // fused kernel, indexed by MODE_SIM / MBP / REPLAY / ANY
+  hipFunction_t step[3] = {};      // step-wise transition kernel, MODE_SIM / MBP / REPLAY
 };
 
 struct beep_ctx {
@@ -253,11 +256,12 @@ int spec_kernels(beep_ctx* c, SpecKernels** out) {
     *out = it->second;
     return BEEP_OK;
   }
-  const std::string src = prelude + "#include \"fused_common.hh\"\n#include \"fused_spec.hh\"\n";
-  const char* hdr[3] = {kJitDeviceRng, kJitFusedCommon, kJitFusedSpec};
-  const char* names[3] = {"device_rng.hh", "fused_common.hh", "fused_spec.hh"};
+  const std::string src =
+      prelude + "#include \"fused_common.hh\"\n#include \"fused_spec.hh\"\n#include \"step_spec.hh\"\n";
+  const char* hdr[4] = {kJitDeviceRng, kJitFusedCommon, kJitFusedSpec, kJitStepSpec};
+  const char* names[4] = {"device_rng.hh", "fused_common.hh", "fused_spec.hh", "step_spec.hh"};
   hiprtcProgram prog;
-  if (hiprtcCreateProgram(&prog, src.c_str(), "beep_fused_spec.hip", 3, hdr, names) != HIPRTC_SUCCESS)
+  if (hiprtcCreateProgram(&prog, src.c_str(), "beep_spec.hip", 4, hdr, names) != HIPRTC_SUCCESS)
     return fail(c, BEEP_EHIP, "hiprtcCreateProgram failed");
   const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast",
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
@@ -279,6 +283,8 @@ int spec_kernels(beep_ctx* c, SpecKernels** out) {
   hipError_t e = hipModuleLoadData(&k->mod, code.data());
   const char* fn[4] = {"beep_fused_spec_sim", "beep_fused_spec_mbp", "beep_fused_spec_replay", "beep_fused_spec_any"};
   for (int i = 0; e == hipSuccess && i < 4; i++) e = hipModuleGetFunction(&k->fn[i], k->mod, fn[i]);
+  const char* sn[3] = {"beep_step_spec_sim", "beep_step_spec_mbp", "beep_step_spec_replay"};
+  for (int i = 0; e == hipSuccess && i < 3; i++) e = hipModuleGetFunction(&k->step[i], k->mod, sn[i]);
   if (e != hipSuccess) {
     delete k;
     return fail(c, BEEP_EHIP, "loading the specialised kernel: %s", hipGetErrorString(e));
@@ -332,7 +338,18 @@ int launch_stepwise(beep_ctx* c, int mode, const Job* djobs, int n, int t0, int 
                            c->h_topo.in_w, djobs, rows, m.A, c->Apad, m.G, s, c->h_topo.off_geo};
       hipLaunchKernelGGL(beepdev::k_sparse_contract, dim3((m.A + 255) / 256, rows), dim3(256), 0, c->stream, p);
     }
-    if (mode == beepdev::MODE_SIM)
+    if (c->spec) {
+      int Apad = c->Apad, ss = s;
+      void* args[6] = {(void*)&djobs, (void*)&c->d_I, (void*)&c->d_D, (void*)&Apad, (void*)&ss, (void*)&c->d_rngp};
+      hipError_t le = hipModuleLaunchKernel(c->spec->step[mode], gcell.x, gcell.y, 1, BEEP_STEP_BLOCK, 1, 1, 0,
+                                            c->stream, args, nullptr);
+      if (le != hipSuccess) {
+        hipGraph_t g2 = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g2);
+        if (g2) (void)hipGraphDestroy(g2);
+        return fail(c, BEEP_EHIP, "step kernel launch under capture: %s", hipGetErrorString(le));
+      }
+    } else if (mode == beepdev::MODE_SIM)
       hipLaunchKernelGGL(beepdev::k_step_trans<beepdev::MODE_SIM>, gcell, dim3(BEEP_STEP_BLOCK), 0, c->stream, djobs,
                          c->d_I, c->d_D, c->Apad, s, c->d_rngp);
     else if (mode == beepdev::MODE_MBP)
@@ -355,7 +372,8 @@ int launch_stepwise(beep_ctx* c, int mode, const Job* djobs, int n, int t0, int 
 
 int launch_fused(beep_ctx* c, int mode, const Job* djobs, int njobs, uint64_t seed, uint32_t stream) {
   const auto& m = c->m;
-  if (c->spec) return launch_spec(c, c->spec, mode, djobs, njobs, c->spec_lds, fused_block(m.AG), seed, stream);
+  if (c->spec && !c->stepwise)
+    return launch_spec(c, c->spec, mode, djobs, njobs, c->spec_lds, fused_block(m.AG), seed, stream);
   size_t lds = beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, c->h_topo.nnz, c->h_topo.stage_mix);
   if (lds > 160 * 1024) return fail(c, BEEP_EINVAL, "instance needs %zu bytes of LDS (max 163840)", lds);
   int block = fused_block(m.AG);
@@ -522,11 +540,14 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   c->dense = (size_t)tp.nnz * 4 >= (size_t)m.A * m.A && m.A >= 32;
   size_t lds = beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, tp.stage_mix);
   c->stepwise = (keep & BEEP_STEPWISE) || c->dense || lds > 160 * 1024;
-  if (!c->stepwise && !(keep & BEEP_GENERIC_KERNEL) && m.C <= 16 && m.T <= 64) {
+  if (!(keep & BEEP_GENERIC_KERNEL) && m.C <= 16 && m.T <= 64) {
+    // The step-wise transition kernel's LDS: populations, queue, tables.
+    size_t step_lds = 4 * ((size_t)16 * BEEP_STEP_BLOCK + 2 * (size_t)m.T * BEEP_STEP_BLOCK + 1 + (size_t)m.T * m.G +
+                           (size_t)m.G * m.G + m.G + m.C);
     size_t sl = beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, tp.stage_mix);
-    if (sl <= 65536) {
+    if (c->stepwise ? step_lds <= 65536 : sl <= 65536) {
       if ((rc = spec_kernels(c, &c->spec))) return rc;
-      c->spec_lds = sl;
+      c->spec_lds = c->stepwise ? 0 : sl;
     }
   }
   c->Apad = (m.A + 63) / 64 * 64;
@@ -642,7 +663,8 @@ int beep_destroy(beep_ctx* c) {
 
 int beep_path(const beep_ctx* c) {
   if (!c) return BEEP_EINVAL;
-  return c->stepwise ? BEEP_PATH_STEPWISE : c->spec ? BEEP_PATH_FUSED_SPEC : BEEP_PATH_FUSED;
+  if (c->stepwise) return c->spec ? BEEP_PATH_STEPWISE_SPEC : BEEP_PATH_STEPWISE;
+  return c->spec ? BEEP_PATH_FUSED_SPEC : BEEP_PATH_FUSED;
 }
 
 int beep_get_dims(const beep_ctx* c, int32_t* d) {
@@ -1326,8 +1348,9 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
     if (c->stepwise) {
       (j.mode == beepdev::MODE_MBP ? step_mbp : step_sim)[c].push_back(j);
     } else {
-      size_t lds = c->spec ? c->spec_lds : beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, c->h_topo.nnz, c->h_topo.stage_mix);
-      fused.push_back({j, c, lds, (double)m.AG * m.T * (j.t1 - j.t0), m.AG, c->spec});
+      SpecKernels* sk = c->stepwise ? nullptr : c->spec;
+      size_t lds = sk ? c->spec_lds : beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, c->h_topo.nnz, c->h_topo.stage_mix);
+      fused.push_back({j, c, lds, (double)m.AG * m.T * (j.t1 - j.t0), m.AG, sk});
     }
   }
   int rc;

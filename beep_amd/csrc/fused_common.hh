@@ -73,6 +73,7 @@ __host__ __device__ inline size_t fused_lds_bytes(int A, int G, int C, int T, in
 }
 
 #define BEEP_FUSED_MAX_BLOCK 512
+#define BEEP_STEP_BLOCK 256         // cells per workgroup of the step-wise transition kernels
 #define BEEP_POIS_INLINE 10.0f     // means below this are drawn in place by inversion
 
 enum { ITEM_POIS = 0, ITEM_POIS_ADD = 1, ITEM_BINOM = 2 };

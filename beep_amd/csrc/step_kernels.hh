@@ -156,7 +156,6 @@ __global__ void __launch_bounds__(256) k_step_init(const Job* __restrict__ jobs,
   D[((size_t)blockIdx.y * G + g) * Apad + j] = d;
 }
 
-#define BEEP_STEP_BLOCK 256
 #define BEEP_STEP_QCAP 512
 
 // One step s of the tau-leap for a batch, one thread per cell, grid

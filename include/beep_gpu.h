@@ -160,8 +160,8 @@ typedef struct beep_ctx beep_ctx;
  * instance fits the fused per-instance kernel.  The step-wise path is chosen
  * automatically for dense mixing and for instances too large for LDS. */
 #define BEEP_STEPWISE       16
-/* Option bit of `keep`: on the fused path, use the generic kernel instead of
- * the one specialised (compiled at context creation) for the model's
+/* Option bit of `keep`: use the generic simulation kernels instead of the
+ * ones specialised (compiled at context creation) for the model's
  * compartment structure.  Both give identical trajectories. */
 #define BEEP_GENERIC_KERNEL 32
 
@@ -176,12 +176,14 @@ const char* beep_last_error(const beep_ctx* ctx);
 /* Sizes: A, G, C, T, S (steps), nparam, nobs, ndatatable, capacity, nslot. */
 int beep_get_dims(const beep_ctx* ctx, int32_t* dims10);
 size_t beep_device_bytes(const beep_ctx* ctx);
-/* The simulation path of the context: BEEP_PATH_STEPWISE, BEEP_PATH_FUSED
- * (generic fused kernel) or BEEP_PATH_FUSED_SPEC (fused kernel specialised
- * for the compartment structure); negative on error. */
+/* The simulation path of the context: BEEP_PATH_STEPWISE (generic step-wise
+ * kernels), BEEP_PATH_FUSED (generic fused kernel), BEEP_PATH_FUSED_SPEC or
+ * BEEP_PATH_STEPWISE_SPEC (the same with the kernels specialised for the
+ * compartment structure); negative on error. */
 #define BEEP_PATH_STEPWISE   0
 #define BEEP_PATH_FUSED      1
 #define BEEP_PATH_FUSED_SPEC 2
+#define BEEP_PATH_STEPWISE_SPEC 3
 int beep_path(const beep_ctx* ctx);
 
 /* State::set_param (src/state.cc:66-91): for each i, derives the tables of

@@ -98,3 +98,42 @@ def test_batch_identical(gpu_lib):
         _same(a, b, range(8))
         a.close()
         b.close()
+
+
+STEP_MODELS = [
+    ("spatial_100x4_sparse", lambda: configs.spatial_abcsmc(A=100, period=60)),
+    ("seir_areas_30x9", lambda: configs.seir_areas(30, 9, 60, seed=3)),
+    ("dense_40x9", lambda: configs.dense_pmcmc(A=40, period=40, with_obs=False)),
+]
+
+
+@pytest.mark.parametrize("name,make", STEP_MODELS)
+def test_stepwise_spec_identical(gpu_lib, name, make):
+    """The step-wise path's specialised transition kernel (inside the captured
+    hipGraph) matches the generic one: simulation, proposals and replay."""
+    spec, truth = make()
+    n = 6
+    keep = _lib.KEEP_ALL | _lib.STEPWISE
+    a = _lib.Context(spec, capacity=2 * n, nslot=2 * n, keep=keep)
+    b = _lib.Context(spec, capacity=2 * n, nslot=2 * n, keep=keep | _lib.GENERIC_KERNEL)
+    assert a.path == _lib.PATH_STEPWISE_SPEC and b.path == _lib.PATH_STEPWISE
+    rng = np.random.default_rng(9)
+    params = specmod.sample_prior(spec, rng, n)
+    params[0] = truth
+    inst = np.arange(n)
+    for ctx in (a, b):
+        ctx.set_param(inst, params)
+        ctx.simulate(inst, inst, 40 + inst, seed=3, stream=6)
+        ctx.simulate(inst, inst, 40 + inst, seed=3, stream=7)    # a second replay of the captured graph
+    _same(a, b, inst)
+    params2 = specmod.sample_prior(spec, rng, n)
+    for ctx in (a, b):
+        ctx.set_param(n + inst, params2)
+        ctx.mbp(inst, n + inst, n + inst, 90 + inst, seed=4, stream=8)
+    _same(a, b, n + inst)
+    tn = a.get_transnum(0)
+    for ctx in (a, b):
+        ctx.put_transnum(1, 0, tn)
+    _same(a, b, [1])
+    a.close()
+    b.close()

@@ -25,11 +25,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 PARTICLES = 4096
 METRIC = "particle-timesteps/sec batched vs plain GPU path; p50/p99 request latency"
-# HBM bytes per launch of the fused kernel on this workload (4096 particles),
-# from rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the
-# ratio of known bytes to counter reading on a plain copy kernel
-# (profiles/k_fused_spatial_abcsmc_traffic.md).
-FUSED_TRAFFIC_BYTES = 41_895_375_565
+# HBM bytes per launch of the simulation kernel on this workload (4096
+# particles), from rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, each scaled
+# by the ratio of known bytes to counter reading on a plain copy kernel:
+# the specialised kernel (profiles/beep_fused_spec_sim_spatial_abcsmc.md) and
+# the generic one (profiles/k_fused_spatial_abcsmc_traffic.md).
+SPEC_TRAFFIC_BYTES = 44_150_094_023
+GENERIC_TRAFFIC_BYTES = 41_895_375_565
 
 
 def algorithmic_bytes_per_particle_step(A, G, C, T):
@@ -185,7 +187,8 @@ def main():
         "kernels_ms": {"simulate_fused": sim_avg, "obs_loglik": float(np.mean(obs_ms))},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": FUSED_TRAFFIC_BYTES if P == PARTICLES else None,
+                     "traffic": (None if P != PARTICLES else GENERIC_TRAFFIC_BYTES if args.generic
+                                 else SPEC_TRAFFIC_BYTES),
                      "bytes_per_particle_step": bytes_per, "algorithmic_bytes_per_launch": bytes_per * P * S},
     }
     if cpu is not None:

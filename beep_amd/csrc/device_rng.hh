@@ -153,15 +153,40 @@ __device__ inline float log_pois_f(float k, float lam) {
   return -0.5f * __logf(6.2831853071795865f * k) - stirlerr_f(k) - bd0_f(k, lam);
 }
 
-// Poisson(lam) with first uniform u (FP32 path for small means).
+// Four terms k0+1 .. k0+4 of the inversion search with 1/k as constants:
+// p_k = p_{k-1} * lam / k, F_k = F_{k-1} + p_k; r = the first k whose F_k
+// reaches u (or stops growing), 0 if none.
+#define BEEP_INV_BLOCK(K0)                                                              \
+  {                                                                                     \
+    float p1 = p * (lam * (1.0f / (K0 + 1))), F1 = F + p1;                             \
+    float p2 = p1 * (lam * (1.0f / (K0 + 2))), F2 = F1 + p2;                           \
+    float p3 = p2 * (lam * (1.0f / (K0 + 3))), F3 = F2 + p3;                           \
+    float p4 = p3 * (lam * (1.0f / (K0 + 4))), F4 = F3 + p4;                           \
+    int r = (u <= F4 || F4 == F3) ? K0 + 4 : 0;                                         \
+    r = (u <= F3 || F3 == F2) ? K0 + 3 : r;                                             \
+    r = (u <= F2 || F2 == F1) ? K0 + 2 : r;                                             \
+    r = (u <= F1 || F1 == F) ? K0 + 1 : r;                                              \
+    if (r) return r;                                                                    \
+    p = p4;                                                                             \
+    F = F4;                                                                             \
+  }
+
+// Poisson(lam) with first uniform u (FP32 path for small means): sequential
+// inversion, the CDF summed until it passes u.  When FP32 rounding leaves u
+// above the largest representable CDF value, the search stops where further
+// terms no longer change the sum (a tail event of probability ~1e-7).  The
+// first 16 terms are evaluated four at a time with constant reciprocals (one
+// branch per four terms, which matters where the lanes of a wavefront need
+// different numbers of terms); later terms, rare below mean 10, one at a time.
 __device__ inline int poisson_inversion(float lam, float u) {
   float p = __expf(-lam);
   if (u <= p) return 0;
   float F = p;
-  int k = 0;
-  // The CDF is summed until it passes u.  When FP32 rounding leaves u above
-  // the largest representable CDF value, the search stops where further terms
-  // no longer change the sum (a tail event of probability ~1e-7).
+  BEEP_INV_BLOCK(0)
+  BEEP_INV_BLOCK(4)
+  BEEP_INV_BLOCK(8)
+  BEEP_INV_BLOCK(12)
+  int k = 16;
   while (k < 64) {
     k++;
     p *= lam * __builtin_amdgcn_rcpf((float)k);
@@ -171,6 +196,7 @@ __device__ inline int poisson_inversion(float lam, float u) {
   }
   return k;
 }
+#undef BEEP_INV_BLOCK
 
 // PTRS (Hormann 1993) for 10 <= lam < 2^23: the squeeze accepts most
 // candidates; the exact test compares with the saddle-point log-pmf.

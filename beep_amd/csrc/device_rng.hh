@@ -100,6 +100,14 @@ struct CellRng {
     return x;
   }
   __device__ inline float next() { return u01f(bits()); }
+  // Fills the first block now (the same block the first bits() would make):
+  // where the lanes of a wavefront make their first draw at different places
+  // of an unrolled loop, this runs the generator once, converged.
+  __device__ inline void prime() {
+    buf = philox4x32_10(u4{c0, c1, block, c3}, k0, k1);
+    block++;
+    idx = 0;
+  }
 };
 typedef CellRng ExtraUniforms;
 

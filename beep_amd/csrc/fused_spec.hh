@@ -152,6 +152,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
       const bool simu = mode == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
       CellRng rng;
       rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
+      if (mode != MODE_REPLAY) rng.prime();
 #pragma unroll
       for (int tr = 0; tr < T; tr++) {
         const int fr = FROM[tr], to = TO[tr];

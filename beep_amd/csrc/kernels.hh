@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
       const bool simu = mode == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
       CellRng rng;
       rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
+      if (mode != MODE_REPLAY) rng.prime();
 #pragma unroll 1
       for (int tr = 0; tr < T; tr++) {
         const int fr = sfrom[tr];

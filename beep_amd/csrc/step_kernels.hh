@@ -213,6 +213,7 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
     const bool simu = MODE == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
     CellRng rng;
     rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
+    if (MODE != MODE_REPLAY) rng.prime();
 #pragma unroll 1
     for (int tr = 0; tr < T; tr++) {
       const int fr = tp->from[tr];

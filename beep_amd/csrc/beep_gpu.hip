@@ -240,13 +240,23 @@ std::mutex g_jit_mu;
 std::map<std::string, SpecKernels*> g_jit;
 
 // The constants fused_spec.hh is compiled with.
+// Minimum resident wavefronts per SIMD the specialised kernels are compiled
+// for (it caps their registers): measured on 100 areas x 4 age groups with 8
+// transitions, 6 beat 4 (no spills either way) and 8 spilled; fewer
+// transitions need fewer registers.  BEEP_SPEC_WAVES / BEEP_STEP_WAVES
+// override.
+int spec_waves(const beephost::HostModel& m, const char* env, int dflt) {
+  if (const char* e = std::getenv(env)) return std::max(1, std::min(8, std::atoi(e)));
+  return dflt;
+}
+
 std::string spec_prelude(const beephost::HostModel& m) {
-  int waves = 4;
-  if (const char* e = std::getenv("BEEP_SPEC_WAVES")) waves = std::max(1, std::min(8, std::atoi(e)));
+  const int waves = spec_waves(m, "BEEP_SPEC_WAVES", m.T <= 4 ? 8 : m.T <= 8 ? 6 : 4);
+  const int step_waves = spec_waves(m, "BEEP_STEP_WAVES", 4);
   std::ostringstream o;
   o << "namespace beepspec {\n";
   o << "constexpr int C = " << m.C << ", T = " << m.T << ", G = " << m.G << ", CP = " << (m.C + 3) / 4 * 4
-    << ", INF_TR = " << m.inf_tr << ", MIN_WAVES = " << waves << ";\n";
+    << ", INF_TR = " << m.inf_tr << ", MIN_WAVES = " << waves << ", STEP_WAVES = " << step_waves << ";\n";
   o << "constexpr int FROM[" << m.T << "] = {";
   for (int t = 0; t < m.T; t++) o << (t ? ", " : "") << m.trans[t].from;
   o << "};\nconstexpr int TO[" << m.T << "] = {";

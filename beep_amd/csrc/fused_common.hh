@@ -52,8 +52,11 @@ enum { MODE_SIM = 0, MODE_MBP = 1, MODE_REPLAY = 2, MODE_ANY = 3 };
 // address space, non-temporal (streaming) stores.
 typedef __attribute__((address_space(1))) int gint;
 typedef __attribute__((address_space(1))) float gfloat;
-__device__ inline void st_nt(int* p, size_t i, int v) { __builtin_nontemporal_store(v, (gint*)p + i); }
-__device__ inline void st_nt(float* p, size_t i, float v) { __builtin_nontemporal_store(v, (gfloat*)p + i); }
+// The index is 32-bit (one step's slab of an instance): with the slab base
+// uniform across the wavefront the store takes a scalar 64-bit base and a
+// 32-bit per-lane offset, with no 64-bit address arithmetic per lane.
+__device__ inline void st_nt(int* p, uint32_t i, int v) { __builtin_nontemporal_store(v, (gint*)p + i); }
+__device__ inline void st_nt(float* p, uint32_t i, float v) { __builtin_nontemporal_store(v, (gfloat*)p + i); }
 
 // Queue capacity of large draws per step (items beyond it are drawn in place).
 __host__ __device__ inline int fused_qcap(int AG) { return AG < 32 ? 64 : (2 * AG < 1024 ? 2 * AG : 1024); }
@@ -79,9 +82,12 @@ __host__ __device__ inline size_t fused_lds_bytes(int A, int G, int C, int T, in
 enum { ITEM_POIS = 0, ITEM_POIS_ADD = 1, ITEM_BINOM = 2 };
 
 // A large draw: its own Philox stream (word2 = 0x80000000 | tr << 22 | block)
-// so the value does not depend on which lane draws it or when.
-__device__ inline int draw_item(int kind, float val, int ni, uint32_t k0, uint32_t k1, uint32_t c0, uint32_t cell,
-                                uint32_t tr, uint32_t key) {
+// so the value does not depend on which lane draws it or when.  The
+// specialised kernels call it from one place only and have it inlined.
+template <bool INLINE = false>
+__device__ inline __attribute__((always_inline)) int draw_item_body(int kind, float val, int ni, uint32_t k0,
+                                                                    uint32_t k1, uint32_t c0, uint32_t cell,
+                                                                    uint32_t tr, uint32_t key) {
   ExtraUniforms xr;
   xr.init(k0, k1, c0, cell, key);
   xr.block = 0x80000000u | (tr << 22);
@@ -89,6 +95,15 @@ __device__ inline int draw_item(int kind, float val, int ni, uint32_t k0, uint32
   if (kind == ITEM_POIS) return poisson_draw(val, u, xr);
   if (kind == ITEM_POIS_ADD) return ni + poisson_draw(val, u, xr);
   return binomial_draw(ni, val, u, xr);
+}
+__device__ inline int draw_item(int kind, float val, int ni, uint32_t k0, uint32_t k1, uint32_t c0, uint32_t cell,
+                                uint32_t tr, uint32_t key) {
+  return draw_item_body(kind, val, ni, k0, k1, c0, cell, tr, key);
+}
+__device__ inline __attribute__((always_inline)) int draw_item_inline(int kind, float val, int ni, uint32_t k0,
+                                                                      uint32_t k1, uint32_t c0, uint32_t cell,
+                                                                      uint32_t tr, uint32_t key) {
+  return draw_item_body<true>(kind, val, ni, k0, k1, c0, cell, tr, key);
 }
 
 // LDS bytes of the structure-specialised kernel: populations [AG][CP] (one

@@ -14,8 +14,8 @@
 // step-wise path for the same keys.  What the specialisation buys: a cell's
 // populations sit in registers through the step with compile-time source and
 // destination indices, the transition loop is unrolled, divisions by G are by
-// a constant, and populations are single-buffered in LDS as [AG][CP] so one
-// cell's compartments load and store as 16-byte vectors.
+// a constant, and populations are single-buffered in LDS in 16-byte chunks of
+// four compartments so one cell's compartments load and store as vectors.
 //
 // Per step: (B) area mixing, (C) means, small draws, updates of the cell's own
 // populations, large draws queued, (D) the queued draws by all lanes, applied
@@ -25,6 +25,11 @@
 namespace beepdev {
 
 static_assert(beepspec::CP % 4 == 0 && beepspec::CP >= beepspec::C, "CP");
+
+// Populations in LDS: compartment c of a cell in 16-byte chunk c / 4 of the
+// cell; chunk k of all cells is contiguous, so one wavefront's 16-byte
+// accesses to its cells' chunk are consecutive in LDS.
+__device__ inline int pidx(int c, int cell, int AG) { return ((c >> 2) * AG + cell) * 4 + (c & 3); }
 
 template <int MODE>
 __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t seed, uint32_t stream) {
@@ -38,7 +43,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   const int qcap = T * AG;                             // every (cell, transition): never full
 
   extern __shared__ int smem[];
-  int* pop = smem;                                     // [AG][CP]
+  int* pop = smem;                                     // [CP/4][AG] chunks of 4 compartments
   float* sdinf = (float*)(pop + (size_t)AG * CP);      // [AG]
   float* sI = sdinf + AG;                              // [AG]
   uint32_t* qcode = (uint32_t*)(sI + AG);              // [qcap]
@@ -70,7 +75,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     const int* src = jb.from_init ? tpp->pop_init : jb.state;
     for (int cell = tid; cell < AG; cell += nt) {
 #pragma unroll
-      for (int c = 0; c < CP; c++) pop[cell * CP + c] = c < C ? src[c * AG + cell] : 0;
+      for (int c = 0; c < CP; c++) pop[pidx(c, cell, AG)] = c < C ? src[c * AG + cell] : 0;
     }
     for (int i = tid; i < T * G; i += nt) srate[i] = tab[tpp->off_rate + i];
     for (int i = tid; i < G * G; i += nt) sN[i] = tab[tpp->off_N + i];
@@ -82,11 +87,11 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
 
   // (E) for the first step: dinf = sum_c inf_c * pop_c (State::set_I_from_pop).
   for (int cell = tid; cell < AG; cell += nt) {
-    const int4* pv = (const int4*)(pop + cell * CP);
+    const int4* pv = (const int4*)pop;
     float d = 0.f;
 #pragma unroll
     for (int c4 = 0; c4 < CP / 4; c4++) {
-      int4 v = pv[c4];
+      int4 v = pv[c4 * AG + cell];
       int pc[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -130,10 +135,10 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
       const int j = cell / G, g = cell - j * G;
       int p[CP];
       {
-        const int4* pv = (const int4*)(pop + cell * CP);
+        const int4* pv = (const int4*)pop;
 #pragma unroll
         for (int c4 = 0; c4 < CP / 4; c4++) {
-          int4 v = pv[c4];
+          int4 v = pv[c4 * AG + cell];
           p[4 * c4] = v.x; p[4 * c4 + 1] = v.y; p[4 * c4 + 2] = v.z; p[4 * c4 + 3] = v.w;
         }
       }
@@ -203,9 +208,10 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
         }
       }
       {
-        int4* pv = (int4*)(pop + cell * CP);
+        int4* pv = (int4*)pop;
 #pragma unroll
-        for (int c4 = 0; c4 < CP / 4; c4++) pv[c4] = make_int4(q[4 * c4], q[4 * c4 + 1], q[4 * c4 + 2], q[4 * c4 + 3]);
+        for (int c4 = 0; c4 < CP / 4; c4++)
+          pv[c4 * AG + cell] = make_int4(q[4 * c4], q[4 * c4 + 1], q[4 * c4 + 2], q[4 * c4 + 3]);
       }
     }
     __syncthreads();
@@ -217,15 +223,15 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
           uint32_t code = qcode[i];
           int cell = (int)(code & 0xFFFFFu), tr = (int)((code >> 20) & 0xFFu), kind = (int)(code >> 28);
           int ni = mode == MODE_MBP ? itn[tr * AG + cell] : 0;
-          int n = draw_item(kind, qval[i], ni, k0, k1, c0, (uint32_t)cell, (uint32_t)tr, jb.key);
+          int n = draw_item_inline(kind, qval[i], ni, k0, k1, c0, (uint32_t)cell, (uint32_t)tr, jb.key);
           if (tn_s) st_nt(tn_s, tr * AG + cell, n);
           if (update && n != 0) {
             int fr = 0, to = 0;
 #pragma unroll
             for (int k = 0; k < T; k++)
               if (k == tr) { fr = FROM[k]; to = TO[k]; }
-            atomicSub(&pop[cell * CP + fr], n);
-            atomicAdd(&pop[cell * CP + to], n);
+            atomicSub(&pop[pidx(fr, cell, AG)], n);
+            atomicAdd(&pop[pidx(to, cell, AG)], n);
           }
         }
         __syncthreads();
@@ -234,11 +240,11 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     }
     // (E) infectivity for the next step.
     for (int cell = tid; cell < AG; cell += nt) {
-      const int4* pv = (const int4*)(pop + cell * CP);
+      const int4* pv = (const int4*)pop;
       float d = 0.f;
 #pragma unroll
       for (int c4 = 0; c4 < CP / 4; c4++) {
-        int4 v = pv[c4];
+        int4 v = pv[c4 * AG + cell];
         int pc[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -255,7 +261,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   }
   for (int cell = tid; cell < AG; cell += nt) {
 #pragma unroll
-    for (int c = 0; c < C; c++) jb.state[c * AG + cell] = pop[cell * CP + c];
+    for (int c = 0; c < C; c++) jb.state[c * AG + cell] = pop[pidx(c, cell, AG)];
   }
 }
 

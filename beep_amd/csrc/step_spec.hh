@@ -126,7 +126,7 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
     int t2 = (int)(code & 0xFFFFFu), tr = (int)((code >> 20) & 0xFFu), kind = (int)(code >> 28);
     int c2 = blockIdx.x * BEEP_STEP_BLOCK + t2;
     int ni = MODE == MODE_MBP ? itn[tr * AG + c2] : 0;
-    int n = draw_item(kind, qval[i], ni, k0, k1, c0, (uint32_t)c2, (uint32_t)tr, jb.key);
+    int n = draw_item_inline(kind, qval[i], ni, k0, k1, c0, (uint32_t)c2, (uint32_t)tr, jb.key);
     if (tn_s) st_nt(tn_s, tr * AG + c2, n);
     if (update && n != 0) {
       int fr = 0, to = 0;
@@ -155,15 +155,15 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
 
 }  // namespace beepdev
 
-extern "C" __global__ void __launch_bounds__(BEEP_STEP_BLOCK)
+extern "C" __global__ void __launch_bounds__(BEEP_STEP_BLOCK, beepspec::STEP_WAVES)
     beep_step_spec_sim(const beepdev::Job* jobs, const float* I, float* D, int Apad, int s, const uint64_t* rngp) {
   beepdev::step_spec_body<beepdev::MODE_SIM>(jobs, I, D, Apad, s, rngp);
 }
-extern "C" __global__ void __launch_bounds__(BEEP_STEP_BLOCK)
+extern "C" __global__ void __launch_bounds__(BEEP_STEP_BLOCK, beepspec::STEP_WAVES)
     beep_step_spec_mbp(const beepdev::Job* jobs, const float* I, float* D, int Apad, int s, const uint64_t* rngp) {
   beepdev::step_spec_body<beepdev::MODE_MBP>(jobs, I, D, Apad, s, rngp);
 }
-extern "C" __global__ void __launch_bounds__(BEEP_STEP_BLOCK)
+extern "C" __global__ void __launch_bounds__(BEEP_STEP_BLOCK, beepspec::STEP_WAVES)
     beep_step_spec_replay(const beepdev::Job* jobs, const float* I, float* D, int Apad, int s, const uint64_t* rngp) {
   beepdev::step_spec_body<beepdev::MODE_REPLAY>(jobs, I, D, Apad, s, rngp);
 }

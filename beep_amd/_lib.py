@@ -394,7 +394,7 @@ class Context:
         L = np.zeros(len(inst))
         EFdt = np.zeros((len(inst), self.ndt))
         self._chk(lib().beep_obs_loglik(self._h, len(inst), _p(inst, C.c_int32), section, _p(L, C.c_double),
-                                        _p(EFdt, C.c_double)))
+                                        _p(EFdt, C.c_double) if with_dt else None))
         return (L, EFdt) if with_dt else L
 
     def obs_values(self, inst):

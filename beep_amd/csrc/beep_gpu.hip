@@ -39,6 +39,7 @@ struct SpecKernels {
   hipModule_t mod = nullptr;
   hipFunction_t fn[4] = {};        // fused kernel, indexed by MODE_SIM / MBP / REPLAY / ANY
   hipFunction_t step[3] = {};      // step-wise transition kernel, MODE_SIM / MBP / REPLAY
+  int max_dyn_lds = 65536;         // dynamic LDS the fused kernels accept (the device's, less static use)
 };
 
 struct beep_ctx {
@@ -309,6 +310,13 @@ int spec_kernels(beep_ctx* c, SpecKernels** out) {
     delete k;
     return fail(c, BEEP_EHIP, "loading the specialised kernel: %s", hipGetErrorString(e));
   }
+  int lim = 0;
+  for (int i = 0; i < 4; i++) {
+    int v = 0;
+    if (hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_DYNAMIC_SHARED_SIZE_BYTES, k->fn[i]) == hipSuccess && v > 0)
+      lim = i == 0 ? v : std::min(lim, v);
+  }
+  if (lim > 0) k->max_dyn_lds = lim;
   g_jit[key] = k;
   *out = k;
   return BEEP_OK;
@@ -565,9 +573,13 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
     size_t step_lds = 4 * ((size_t)16 * BEEP_STEP_BLOCK + 2 * (size_t)m.T * BEEP_STEP_BLOCK + 1 + (size_t)m.T * m.G +
                            (size_t)m.G * m.G + m.G + m.C);
     size_t sl = beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, tp.stage_mix);
-    if (c->stepwise ? step_lds <= 65536 : sl <= 65536) {
-      if ((rc = spec_kernels(c, &c->spec))) return rc;
-      c->spec_lds = c->stepwise ? 0 : sl;
+    if (c->stepwise ? step_lds <= 65536 : sl <= 160 * 1024) {
+      SpecKernels* k = nullptr;
+      if ((rc = spec_kernels(c, &k))) return rc;
+      if (c->stepwise || sl <= (size_t)k->max_dyn_lds) {
+        c->spec = k;
+        c->spec_lds = c->stepwise ? 0 : sl;
+      }
     }
   }
   c->Apad = (m.A + 63) / 64 * 64;

@@ -25,39 +25,71 @@ struct DenseMix {
   int rows, A, Apad, G, s, off_geo;
 };
 
-// 64x64 output tile per workgroup of 4 wavefronts; each wavefront owns a 32x32
-// quadrant computed with v_mfma_f32_32x32x2f32 over K in chunks of 32 staged
-// through LDS (A tile row-padded to 33 floats against bank conflicts).
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+// One 64x64 output tile of C = A x B (A [rows][lda], B [K][ldb], K a multiple
+// of 32, tile origin (tm, tn)) per workgroup of 4 wavefronts; each wavefront
+// owns a 32x32 quadrant computed with v_mfma_f32_32x32x2f32 over K in chunks
+// of 32 staged through LDS, double-buffered: the global loads of chunk k+1 are
+// issued before the matrix instructions of chunk k, and one barrier separates
+// chunks.  The A tile is row-padded to 33 floats against bank conflicts.
 // Accumulator layout of the 32x32 MFMA: lane l holds column l % 32 and, in
 // register v, row 8*(v/4) + 4*(l/32) + v%4.
-__global__ void __launch_bounds__(256) k_dense_contract(DenseMix p) {
-  __shared__ float As[64][33];
-  __shared__ float Bs[32][64];
+__device__ inline f16v mfma_tile_64x64(const float* __restrict__ A, int lda, int rows, const float* __restrict__ B,
+                                       int ldb, int K, int tm, int tn) {
+  __shared__ float As[2][64][33];
+  __shared__ float Bs[2][32][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tm = blockIdx.y * 64, tn = blockIdx.x * 64;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  typedef float f16v __attribute__((ext_vector_type(16)));
   f16v acc;
 #pragma unroll
   for (int v = 0; v < 16; v++) acc[v] = 0.f;
-  for (int k0 = 0; k0 < p.Apad; k0 += 32) {
-    // Stage A[tm..tm+64)[k0..k0+32) and B[k0..k0+32)[tn..tn+64).
-    for (int i = tid; i < 64 * 32; i += 256) {
+  // Each thread stages 8 elements of A and 8 of B per chunk.
+  float ra[8], rb[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int i = tid + q * 256;
       int r = i >> 5, k = i & 31;
       int gr = tm + r;
-      As[r][k] = gr < p.rows ? p.D[(size_t)gr * p.Apad + k0 + k] : 0.f;
+      ra[q] = gr < rows ? A[(size_t)gr * lda + k0 + k] : 0.f;
       int kb = i >> 6, c = i & 63;
-      Bs[kb][c] = p.M[(size_t)(k0 + kb) * p.Apad + tn + c];
+      rb[q] = B[(size_t)(k0 + kb) * ldb + tn + c];
     }
-    __syncthreads();
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int i = tid + q * 256;
+      As[buf][i >> 5][i & 31] = ra[q];
+      Bs[buf][i >> 6][i & 63] = rb[q];
+    }
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  const int nk = K / 32;
+  for (int kc = 0; kc < nk; kc++) {
+    const int buf = kc & 1;
+    if (kc + 1 < nk) load((kc + 1) * 32);
 #pragma unroll
     for (int kk = 0; kk < 32; kk += 2) {
-      float a = As[wm + (lane & 31)][kk + (lane >> 5)];
-      float b = Bs[kk + (lane >> 5)][wn + (lane & 31)];
+      float a = As[buf][wm + (lane & 31)][kk + (lane >> 5)];
+      float b = Bs[buf][kk + (lane >> 5)][wn + (lane & 31)];
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
     }
+    if (kc + 1 < nk) store(buf ^ 1);
     __syncthreads();
   }
+  return acc;
+}
+
+// The dense contraction with its FOI epilogue, grid (Apad / 64, ceil(rows / 64)).
+__global__ void __launch_bounds__(256) k_dense_contract(DenseMix p) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tm = blockIdx.y * 64, tn = blockIdx.x * 64;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f16v acc = mfma_tile_64x64(p.D, p.Apad, p.rows, p.M, p.Apad, p.Apad, tm, tn);
   const int col = tn + wn + (lane & 31);
   if (col >= p.A) return;
   const float dg = p.diag[col], fc = p.fac[col];
@@ -76,32 +108,10 @@ __global__ void __launch_bounds__(256) k_dense_contract(DenseMix p) {
 // Plain GEMM with the same tiling (no epilogue) for the tests of the
 // accumulator layout: C[rows][Apad] = A[rows][Apad] x B[Apad][Apad].
 __global__ void __launch_bounds__(256) k_dbg_gemm(const float* A_, const float* B_, float* C_, int rows, int Apad) {
-  __shared__ float As[64][33];
-  __shared__ float Bs[32][64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tm = blockIdx.y * 64, tn = blockIdx.x * 64;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  typedef float f16v __attribute__((ext_vector_type(16)));
-  f16v acc;
-#pragma unroll
-  for (int v = 0; v < 16; v++) acc[v] = 0.f;
-  for (int k0 = 0; k0 < Apad; k0 += 32) {
-    for (int i = tid; i < 64 * 32; i += 256) {
-      int r = i >> 5, k = i & 31;
-      int gr = tm + r;
-      As[r][k] = gr < rows ? A_[(size_t)gr * Apad + k0 + k] : 0.f;
-      int kb = i >> 6, c = i & 63;
-      Bs[kb][c] = B_[(size_t)(k0 + kb) * Apad + tn + c];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 32; kk += 2) {
-      float a = As[wm + (lane & 31)][kk + (lane >> 5)];
-      float b = Bs[kk + (lane >> 5)][wn + (lane & 31)];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
-    }
-    __syncthreads();
-  }
+  f16v acc = mfma_tile_64x64(A_, Apad, rows, B_, Apad, Apad, tm, tn);
   const int col = tn + wn + (lane & 31);
 #pragma unroll
   for (int v = 0; v < 16; v++) {

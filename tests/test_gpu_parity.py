@@ -250,3 +250,112 @@ def test_accept_matches_closed_form(gpu_lib):
             assert abs(al[i] - want) <= 1e-12 * max(1.0, want)
         assert acc[al >= 1.0].all() and not acc[al <= 0.0].any()
     ctx.close()
+
+
+def test_all_observation_kinds(gpu_lib):
+    """ObservationModel::obs_prob's branches (src/obsmodel.cc:245-305) on one
+    fixed trajectory: normal with given sd and with a percentage, Poisson,
+    negative binomial, below-threshold for normal and count models, unknown
+    values, factors, multi-day windows over several areas, subsets of age
+    groups and transitions, population terms, three data tables -- device
+    FP64 log-likelihood and per-table EF against the FP64 oracle."""
+    spec, truth = configs.seir_areas(6, 4, 60, seed=5)
+    spec = dict(spec)
+    kinds = []
+    for k in range(60):
+        d0 = (7 * k) % 53
+        base = {"sett_i": d0, "sett_f": d0 + 1 + (k % 7), "area": [k % 6, (k + 2) % 6][: 1 + k % 2],
+                "dp": [[0], [1, 2], [0, 1, 2, 3]][k % 3], "tr": [[0], [1], [0, 1], [2]][k % 4], "datatable": k % 3,
+                "graph": k % 3, "factor": [1.0, 0.5, 2.0][k % 3]}
+        t = k % 8
+        if t == 0:
+            base.update(type="normal", sd=3.0 + k % 5)
+        elif t == 1:
+            base.update(type="normal_percent", percent=10.0)
+        elif t == 2:
+            base.update(type="poisson")
+        elif t == 3:
+            base.update(type="negbin", shape=2.5)
+        elif t == 4:
+            base.update(type="poisson", thresh=4.0)
+        elif t == 5:
+            base.update(type="normal", sd=2.0, thresh=3.0)
+        elif t == 6:
+            base.update(type="poisson")
+        else:
+            base.update(type="negbin", shape=0.7, tr=[], co=[2])      # population of compartment 2
+        kinds.append(base)
+    spec["obs"] = [dict(o, y=0.0) for o in kinds]
+    orc0 = O.Oracle(spec)
+    st = orc0.simulate(orc0.tables(truth), seed=31)
+    _L, values = orc0.obs_loglik(st)
+    obs = []
+    for k, (o, v) in enumerate(zip(kinds, values)):
+        o = dict(o)
+        if k % 8 in (4, 5):
+            o["y"] = 999999998.0                                      # below threshold
+        elif k % 16 == 6:
+            o["y"] = 999999997.0                                      # unknown
+        else:
+            o["y"] = float(np.floor(max(v, 0.0) * (0.8 + 0.05 * (k % 9)) + 0.5))
+        if o["type"] == "normal_percent" and o["y"] == 0:
+            o["y"] = 1.0
+        obs.append(o)
+    spec["obs"] = obs
+    orc = O.Oracle(spec)
+    tb = orc.tables(truth)
+    st = orc.simulate(tb, seed=31)
+    L_ref, values = orc.obs_loglik(st)
+    ctx = _ctx(spec, 1)
+    ctx.set_param([0], truth[None, :])
+    ctx.put_transnum(0, 0, st.transnum.astype(np.int32))
+    assert np.array_equal(ctx.obs_values(0), values)
+    L = ctx.obs_loglik([0])[0]
+    assert np.isfinite(L_ref) and abs(L / L_ref - 1) < 1e-12, (L, L_ref)
+    # per data table: -2 x the sum over its observations
+    _Lr, ef = ctx.obs_loglik([0], with_dt=True)
+    per = np.zeros(3)
+    for k in range(3):
+        sel = [i for i, o in enumerate(obs) if o["datatable"] == k]
+        per[k] = -2 * orc.obs_loglik(st, sel=sel)[0]
+    assert np.allclose(ef[0], per, rtol=1e-12, atol=1e-9), (ef, per)
+    ctx.close()
+
+
+def test_mbp_mask_against_oracle_law(gpu_lib):
+    """Mbp::mbp with a simulate-afresh mask [S][A] (src/mbp.cc:252-330 with the
+    masks of mbp_slicetime / mbp_fixedtree, 656-716): the joint law of
+    (initial, proposal) -- through the change in each area's final recovered
+    count -- matches the oracle's, in masked and unmasked areas.  A mask read
+    in the wrong layout or ignored changes these laws."""
+    spec, truth = configs.seir_areas(5, 1, 50, seed=8)
+    S, A = 50, 5
+    mask = np.zeros((S, A), dtype=np.uint8)
+    mask[10:30, 1] = 1
+    mask[:, 3] = 1
+    th2 = truth.copy()
+    th2[0] *= 1.05
+    n = 3000
+    orc = O.Oracle(spec)
+    tb1, tb2 = orc.tables(truth), orc.tables(th2)
+    C_ = len(spec["comps"])
+    ref = []
+    for r in range(n):
+        st = orc.simulate(tb1, seed=500 + r)
+        pr = orc.mbp(tb2, st, seed=700000 + r, simu=mask)
+        ref.append(pr.pop[-1, :, C_ - 1, 0] - st.pop[-1, :, C_ - 1, 0])
+    ref = np.array(ref)
+    ctx = _ctx(spec, 2 * n, nslot=2)
+    ctx.set_param([0, 1], np.stack([truth, th2]))
+    init = np.arange(n)
+    ctx.simulate(init, np.zeros(n), init + 1, seed=31)
+    ctx.mbp(init, init + n, np.ones(n), init + 1, simu_mask=np.repeat(mask[None], n, 0), seed=32)
+    gpu = np.array([ctx.get_pop(r + n)[-1, :, C_ - 1, 0] - ctx.get_pop(r)[-1, :, C_ - 1, 0] for r in init])
+    for a in range(A):
+        d, pv = stats.ks_2samp(gpu[:, a], ref[:, a])
+        assert pv > 1e-4, (a, d, pv)
+        se = np.sqrt(gpu[:, a].var() / n + ref[:, a].var() / n) + 1e-12
+        assert abs(gpu[:, a].mean() - ref[:, a].mean()) < 5 * se, a
+    # masked areas move far more than coupled ones
+    assert gpu[:, 3].std() > 2 * gpu[:, 0].std()
+    ctx.close()

@@ -137,3 +137,20 @@ def test_stepwise_spec_identical(gpu_lib, name, make):
     _same(a, b, [1])
     a.close()
     b.close()
+
+
+def test_large_instance_identical(gpu_lib):
+    """An instance whose specialised kernel needs more than 64 KB of LDS
+    (400 areas x 4 age groups) runs specialised and matches the generic
+    kernel."""
+    spec, truth = configs.seir_areas(400, 4, 40, seed=12)
+    a = _lib.Context(spec, capacity=3, nslot=3, keep=_lib.KEEP_ALL)
+    b = _lib.Context(spec, capacity=3, nslot=3, keep=_lib.KEEP_ALL | _lib.GENERIC_KERNEL)
+    assert b.path == _lib.PATH_FUSED
+    for ctx in (a, b):
+        ctx.set_param(range(3), np.repeat(truth[None, :], 3, 0))
+        ctx.simulate(range(3), range(3), [7, 8, 9], seed=2, stream=1)
+    assert a.path == _lib.PATH_FUSED_SPEC
+    _same(a, b, range(3))
+    a.close()
+    b.close()

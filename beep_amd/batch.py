@@ -12,7 +12,7 @@ class UnevenBatch:
     model reads) and one for its MC3 chains (keeping transnum and transmean,
     what MBP proposals couple to), parameters drawn from the priors."""
 
-    def __init__(self, groups, seed=4):
+    def __init__(self, groups, seed=4, keep_extra=0):
         self.groups = groups
         self.ctx = []
         rng = np.random.default_rng(seed)
@@ -22,7 +22,7 @@ class UnevenBatch:
             spec, _truth = configs.seir_areas(g["A"], g["G"], g["S"], seed=seed * 100003 + gi)
             nsim, nmc3 = len(g["sim_horizons"]), g["n_mc3"]
             if nsim:
-                ctx = _lib.Context(spec, capacity=nsim, nslot=nsim, keep=_lib.KEEP_TRANSNUM)
+                ctx = _lib.Context(spec, capacity=nsim, nslot=nsim, keep=_lib.KEEP_TRANSNUM | keep_extra)
                 ctx.set_param(np.arange(nsim), specmod.sample_prior(spec, rng, nsim))
                 self.ctx.append(ctx)
                 for i, h in enumerate(g["sim_horizons"]):
@@ -30,7 +30,7 @@ class UnevenBatch:
                     key += 1
             if nmc3:
                 ctx = _lib.Context(spec, capacity=2 * nmc3, nslot=2 * nmc3,
-                                   keep=_lib.KEEP_TRANSNUM | _lib.KEEP_TRANSMEAN)
+                                   keep=_lib.KEEP_TRANSNUM | _lib.KEEP_TRANSMEAN | keep_extra)
                 ctx.set_param(np.arange(2 * nmc3), specmod.sample_prior(spec, rng, 2 * nmc3))
                 self.ctx.append(ctx)
                 for k in range(nmc3):

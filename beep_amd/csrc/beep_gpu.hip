@@ -100,6 +100,10 @@ struct beep_ctx {
   // structure-specialised fused kernel (null: generic kernel or step-wise path)
   SpecKernels* spec = nullptr;
   size_t spec_lds = 0;
+  // Streams over which a mixed batch launched from this context spreads its
+  // size classes (created on first use), and the event they start from.
+  hipStream_t bstream[4] = {};
+  hipEvent_t bstart = nullptr;
   bool obs_general = false;        // observations other than plain normal ones
 };
 
@@ -228,11 +232,12 @@ int spec_block(int AG) {
 }
 
 template <int MODE>
-int launch_fused_t(beep_ctx* c, const Job* djobs, int njobs, size_t lds, int block, uint64_t seed, uint32_t stream) {
+int launch_fused_t(beep_ctx* c, const Job* djobs, int njobs, size_t lds, int block, uint64_t seed, uint32_t stream,
+                   hipStream_t st = nullptr) {
   if (njobs == 0) return BEEP_OK;
   auto kfn = beepdev::k_fused<MODE>;
   if (lds > 65536) HIPCHK(c, hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(kfn, dim3(njobs), dim3(block), lds, c->stream, djobs, seed, stream);
+  hipLaunchKernelGGL(kfn, dim3(njobs), dim3(block), lds, st ? st : c->stream, djobs, seed, stream);
   HIPCHK(c, hipGetLastError());
   return BEEP_OK;
 }
@@ -323,11 +328,11 @@ int spec_kernels(beep_ctx* c, SpecKernels** out) {
 }
 
 int launch_spec(beep_ctx* c, const SpecKernels* k, int mode, const Job* djobs, int njobs, size_t lds, int block,
-                uint64_t seed, uint32_t stream) {
+                uint64_t seed, uint32_t stream, hipStream_t st = nullptr) {
   if (njobs == 0) return BEEP_OK;
   void* args[3] = {(void*)&djobs, (void*)&seed, (void*)&stream};
-  HIPCHK(c, hipModuleLaunchKernel(k->fn[mode], (unsigned)njobs, 1, 1, (unsigned)block, 1, 1, (unsigned)lds, c->stream,
-                                  args, nullptr));
+  HIPCHK(c, hipModuleLaunchKernel(k->fn[mode], (unsigned)njobs, 1, 1, (unsigned)block, 1, 1, (unsigned)lds,
+                                  st ? st : c->stream, args, nullptr));
   return BEEP_OK;
 }
 
@@ -545,7 +550,10 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   TopoDev& tp = c->h_topo;
   tp.A = m.A; tp.G = m.G; tp.C = m.C; tp.T = m.T; tp.S = m.S; tp.AG = m.AG; tp.inf_tr = m.inf_tr; tp.spd = m.spd;
   tp.nnz = (int)m.in_src.size();
-  tp.stage_mix = beepdev::fused_mix_bytes(m.A, tp.nnz) <= 32768 ? 1 : 0;
+  // The mixing CSR is staged in LDS when small, and only if the instance still
+  // fits the fused kernel's 160 KB with it.
+  tp.stage_mix = beepdev::fused_mix_bytes(m.A, tp.nnz) <= 32768 &&
+                 beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, 1) <= 160 * 1024;
   tp.dt = (float)m.dt;
   int off = 0;
   tp.off_rate = off; off += m.T * m.G;
@@ -565,7 +573,10 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   // mixing is sparse; otherwise the step-wise path, with the contraction on
   // the matrix cores when the mixing is dense.
   if (m.C > 16) return fail(c, BEEP_EINVAL, "at most 16 compartments are supported");
-  c->dense = (size_t)tp.nnz * 4 >= (size_t)m.A * m.A && m.A >= 32;
+  // Dense: a quarter of the matrix filled and rows of 32 entries or more on
+  // average -- where the per-cell gather would cost more than a batched GEMM.
+  // k-nearest-neighbour mixing of a few dozen areas stays on the fused path.
+  c->dense = (size_t)tp.nnz * 4 >= (size_t)m.A * m.A && (size_t)tp.nnz >= (size_t)32 * m.A && m.A >= 32;
   size_t lds = beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, tp.stage_mix);
   c->stepwise = (keep & BEEP_STEPWISE) || c->dense || lds > 160 * 1024;
   if (!(keep & BEEP_GENERIC_KERNEL) && m.C <= 16 && m.T <= 64) {
@@ -696,6 +707,12 @@ int beep_destroy(beep_ctx* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  for (auto& st : c->bstream)
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
+  if (c->bstart) (void)hipEventDestroy(c->bstart);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return BEEP_OK;
@@ -1405,11 +1422,13 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
       auto& jobs = kv.second;
       int t0 = jobs[0].t0, t1 = jobs[0].t1;
       for (auto& j : jobs) { t0 = std::min(t0, j.t0); t1 = std::max(t1, j.t1); }
-      if (pass == 1 && step_sim.count(c)) HIPCHK(c, hipStreamSynchronize(c->stream));   // job buffer reuse
-      if ((rc = put_jobs(c, jobs))) return rc;
-      if ((rc = launch_stepwise(c, pass == 0 ? beepdev::MODE_SIM : beepdev::MODE_MBP, c->d_jobs, (int)jobs.size(), t0,
-                                t1, seed, stream)))
+      if (pass == 1 && step_sim.count(c)) HIPCHK(c0, hipStreamSynchronize(c->stream));   // job buffer reuse
+      if ((rc = put_jobs(c, jobs)) ||
+          (rc = launch_stepwise(c, pass == 0 ? beepdev::MODE_SIM : beepdev::MODE_MBP, c->d_jobs, (int)jobs.size(), t0,
+                                t1, seed, stream))) {
+        if (c != c0) c0->err = "step-wise context: " + c->err;   // the message goes back through the first context
         return rc;
+      }
       nstep++;
     }
   }
@@ -1445,6 +1464,18 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
       c0->bjobs_cap = cap;
     }
     HIPCHK(c0, hipMemcpyAsync(c0->d_bjobs, all.data(), all.size() * sizeof(Job), hipMemcpyHostToDevice, c0->stream));
+    // The size classes run on a few streams, so that the last, longest
+    // instances of one launch overlap the next launch instead of leaving the
+    // device idle; each stream starts after the job upload.  (Autotuning
+    // launches run alone on the context stream to be timed.)
+    const int nbs = (int)(sizeof c0->bstream / sizeof c0->bstream[0]);
+    if (!c0->bstart) {
+      HIPCHK(c0, hipEventCreateWithFlags(&c0->bstart, hipEventDisableTiming));
+      for (auto& st : c0->bstream) HIPCHK(c0, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    }
+    HIPCHK(c0, hipEventRecord(c0->bstart, c0->stream));
+    for (auto& st : c0->bstream) HIPCHK(c0, hipStreamWaitEvent(st, c0->bstart, 0));
+    int next_stream = 0;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     tune_load();
     bool tuned_any = false;
@@ -1453,9 +1484,9 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
       size_t off = std::get<1>(L), cnt = std::get<2>(L), lds = std::get<3>(L);
       int maxAG = std::get<4>(L);
       SpecKernels* sk = std::get<5>(L);
-      auto launch_any = [&](int blk) {
-        return sk ? launch_spec(c0, sk, beepdev::MODE_ANY, c0->d_bjobs + off, (int)cnt, lds, blk, seed, stream)
-                  : launch_fused_t<beepdev::MODE_ANY>(c0, c0->d_bjobs + off, (int)cnt, lds, blk, seed, stream);
+      auto launch_any = [&](int blk, hipStream_t st) {
+        return sk ? launch_spec(c0, sk, beepdev::MODE_ANY, c0->d_bjobs + off, (int)cnt, lds, blk, seed, stream, st)
+                  : launch_fused_t<beepdev::MODE_ANY>(c0, c0->d_bjobs + off, (int)cnt, lds, blk, seed, stream, st);
       };
       int block = sk ? spec_block(maxAG) : fused_block(maxAG);
       auto f = g_tune.find(key);
@@ -1467,7 +1498,7 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
         for (int cand = 64; cand <= BEEP_FUSED_MAX_BLOCK; cand *= 2) {
           if (cand > pow2_ceil(fused_block(maxAG))) break;
           record(c0, 0);
-          if ((rc = launch_any(cand))) return rc;
+          if ((rc = launch_any(cand, nullptr))) return rc;
           record(c0, 1);
           HIPCHK(c0, hipStreamSynchronize(c0->stream));
           float ms = (float)elapsed(c0, 0, 1);
@@ -1477,7 +1508,7 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
         int fb = fused_block(maxAG);
         if (fb > block && (fb & (fb - 1))) {   // the one-pass size when it is not a power of two
           record(c0, 0);
-          if ((rc = launch_any(fb))) return rc;
+          if ((rc = launch_any(fb, nullptr))) return rc;
           record(c0, 1);
           HIPCHK(c0, hipStreamSynchronize(c0->stream));
           float ms = (float)elapsed(c0, 0, 1);
@@ -1488,14 +1519,18 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
         tuned_any = true;
         continue;
       }
-      if ((rc = launch_any(block))) return rc;
+      // An autotuning call keeps every launch on the context stream, so that
+      // the timed candidates run alone.
+      hipStream_t st = (flags & BEEP_BATCH_AUTOTUNE) ? nullptr : c0->bstream[next_stream++ % nbs];
+      if ((rc = launch_any(block, st))) return rc;
       nlaunch++;
     }
     if (tuned_any) tune_save();
+    for (auto& st : c0->bstream) HIPCHK(c0, hipStreamSynchronize(st));
     HIPCHK(c0, hipStreamSynchronize(c0->stream));   // `all` is the source of an async copy
   }
-  for (auto& kv : step_sim) HIPCHK(kv.first, hipStreamSynchronize(kv.first->stream));
-  for (auto& kv : step_mbp) HIPCHK(kv.first, hipStreamSynchronize(kv.first->stream));
+  for (auto& kv : step_sim) HIPCHK(c0, hipStreamSynchronize(kv.first->stream));
+  for (auto& kv : step_mbp) HIPCHK(c0, hipStreamSynchronize(kv.first->stream));
   HIPCHK(c0, hipStreamSynchronize(c0->stream));   // also keeps `all` alive until its copy is done
   // Errors raised by any context's kernels.
   std::map<beep_ctx*, int> seen;

@@ -231,8 +231,9 @@ def bench_uneven(args):
     from beep_amd.batch import UnevenBatch
     groups = configs.uneven_batch(args.instances, seed=4)
     t0 = time.perf_counter()
-    ub = UnevenBatch(groups, seed=4)
+    ub = UnevenBatch(groups, seed=4, keep_extra=_lib.GENERIC_KERNEL if args.generic else 0)
     setup = time.perf_counter() - t0
+    print(f"uneven: {len(ub.ctx)} contexts set up in {setup:.1f} s", file=sys.stderr, flush=True)
     _lib.run_batch(ub.init_items, seed=4, stream=1)             # the chains' current trajectories
     items = ub.sim_items + ub.mbp_items
     _lib.run_batch(items, seed=4, stream=2, autotune=True)      # tunes the size classes not yet cached
@@ -242,6 +243,8 @@ def bench_uneven(args):
         st = _lib.run_batch(items, seed=4, stream=10 + k)
         walls.append(st[0])
         cells = st[3]
+        print(f"uneven: sweep {k} {st[0]:.1f} ms, {int(st[1])} fused launches, {int(st[2])} step-wise contexts",
+              file=sys.stderr, flush=True)
     wall = float(np.mean(walls)) * 1e-3
     out = {"metric": "particle-timesteps/sec (uneven batch: ABC-SMC simulations + MC3 MBP proposals)",
            "value": pts / wall, "unit": "particle-timesteps/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,

@@ -28,6 +28,7 @@ SYMBOLS = [
     "beep_put_transnum", "beep_copy_instances", "beep_synchronize", "beep_debug_philox", "beep_debug_poisson",
     "beep_debug_binomial", "beep_timing", "beep_debug_gemm", "beep_run_batch", "beep_copy_states",
     "beep_derive_slots", "beep_sum_transnum", "beep_pmcmc_obs_prob", "beep_pmcmc_particle_sample", "beep_bootstrap", "beep_path",
+    "beep_abcsmc_weights", "beep_mc3_swap",
 ]
 
 
@@ -136,6 +137,9 @@ def lib():
                                               f64p, i32p]),
             "beep_pmcmc_particle_sample": (C.c_int, [P, C.c_int]),
             "beep_path": (C.c_int, [P]),
+            "beep_abcsmc_weights": (C.c_int, [C.c_int, C.c_int, C.c_int, f64p, f64p, C.c_int, f64p, f64p,
+                                              C.c_double, f64p, f64p]),
+            "beep_mc3_swap": (C.c_int, [P, C.c_int, i32p, i32p, f64p, C.c_uint64, C.c_uint32, i32p, i32p, i32p]),
             "beep_bootstrap": (C.c_int, [C.c_int, C.c_int, f64p, C.c_double, C.c_uint64, C.c_uint32, C.c_int, i32p,
                                          f64p]),
         }
@@ -500,6 +504,17 @@ class Context:
         """Mpi::particle_sample of the last particle-filter run into instance dst."""
         self._chk(lib().beep_pmcmc_particle_sample(self._h, int(dst)))
 
+    def mc3_swap(self, inst, run, invT, seed=1, stream=0):
+        """MC3::swap_states over chains in instances `inst`: returns (order,
+        ntr, nac); chain i takes the state of chain order[i]."""
+        inst, run, invT = _i32(inst), _i32(run), _f64(invT)
+        n = len(inst)
+        order, ntr, nac = (np.zeros(n, dtype=np.int32) for _ in range(3))
+        self._chk(lib().beep_mc3_swap(self._h, n, _p(inst, C.c_int32), _p(run, C.c_int32), _p(invT, C.c_double),
+                                      int(seed), int(stream), _p(order, C.c_int32), _p(ntr, C.c_int32),
+                                      _p(nac, C.c_int32)))
+        return order, ntr, nac
+
     @property
     def path(self):
         """PATH_STEPWISE, PATH_FUSED or PATH_FUSED_SPEC."""
@@ -510,6 +525,19 @@ class Context:
         out = np.zeros(4)
         self._chk(lib().beep_timing(self._h, _p(out, C.c_double)))
         return out
+
+
+def abcsmc_weights(prev, w_prev, theta, inv_M, size, logprior, device=0):
+    """ABCSMC::calculate_particle_weight on the device for every row of theta."""
+    prev, w_prev, theta, inv_M, logprior = _f64(prev), _f64(w_prev), _f64(theta), _f64(inv_M), _f64(logprior)
+    n_prev, nvar = prev.shape
+    out = np.zeros(len(theta))
+    rc = lib().beep_abcsmc_weights(device, nvar, n_prev, _p(prev, C.c_double), _p(w_prev, C.c_double), len(theta),
+                                   _p(theta, C.c_double), _p(inv_M, C.c_double), float(size),
+                                   _p(logprior, C.c_double), _p(out, C.c_double))
+    if rc:
+        raise BeepError(rc, "beep_abcsmc_weights")
+    return out
 
 
 def bootstrap(L, invT=1.0, seed=1, stream=0, sec=0, device=0):

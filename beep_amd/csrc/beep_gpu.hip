@@ -1290,6 +1290,79 @@ int beep_pmcmc_particle_sample(beep_ctx* c, int dst) {
   return BEEP_OK;
 }
 
+int beep_abcsmc_weights(int device, int nvar, int n_prev, const double* prev, const double* w_prev, int n_new,
+                        const double* theta, const double* inv_M, double size, const double* logprior,
+                        double* w_out) {
+  if (nvar <= 0 || nvar > 32 || n_prev <= 0 || n_new < 0 || !prev || !w_prev || !inv_M || !(size > 0) ||
+      (n_new > 0 && (!theta || !logprior || !w_out)))
+    return BEEP_EINVAL;
+  if (n_new == 0) return BEEP_OK;
+  if (hipSetDevice(device) != hipSuccess) return BEEP_EHIP;
+  const size_t b_prev = (size_t)n_prev * nvar * 8, b_w = (size_t)n_prev * 8, b_th = (size_t)n_new * nvar * 8,
+               b_M = (size_t)nvar * nvar * 8, b_lp = (size_t)n_new * 8, b_out = (size_t)n_new * 8;
+  char* d = nullptr;
+  if (hipMalloc((void**)&d, b_prev + b_w + b_th + b_M + b_lp + b_out) != hipSuccess) return BEEP_ENOMEM;
+  double* d_prev = (double*)d;
+  double* d_w = (double*)(d + b_prev);
+  double* d_th = (double*)(d + b_prev + b_w);
+  double* d_M = (double*)(d + b_prev + b_w + b_th);
+  double* d_lp = (double*)(d + b_prev + b_w + b_th + b_M);
+  double* d_out = (double*)(d + b_prev + b_w + b_th + b_M + b_lp);
+  int rc = BEEP_OK;
+  if (hipMemcpy(d_prev, prev, b_prev, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_w, w_prev, b_w, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_th, theta, b_th, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_M, inv_M, b_M, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_lp, logprior, b_lp, hipMemcpyHostToDevice) != hipSuccess)
+    rc = BEEP_EHIP;
+  if (!rc) {
+    const size_t lds = 8 * ((size_t)nvar * nvar + (size_t)nvar * BEEP_ABC_BLOCK + (size_t)BEEP_ABC_TILE * nvar +
+                            BEEP_ABC_TILE);
+    if (lds > 65536 &&
+        hipFuncSetAttribute((const void*)beepdev::k_abcsmc_weights, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+      rc = BEEP_EHIP;
+    if (!rc) {
+      hipLaunchKernelGGL(beepdev::k_abcsmc_weights, dim3((n_new + BEEP_ABC_BLOCK - 1) / BEEP_ABC_BLOCK),
+                         dim3(BEEP_ABC_BLOCK), lds, 0, nvar, n_prev, d_prev, d_w, n_new, d_th, d_M, size, d_lp, d_out);
+      if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = BEEP_EHIP;
+    }
+  }
+  if (!rc && hipMemcpy(w_out, d_out, b_out, hipMemcpyDeviceToHost) != hipSuccess) rc = BEEP_EHIP;
+  (void)hipFree(d);
+  return rc;
+}
+
+int beep_mc3_swap(beep_ctx* c, int n, const int32_t* inst, const int32_t* run, const double* invT, uint64_t seed,
+                  uint32_t stream, int32_t* order_out, int32_t* ntr_out, int32_t* nac_out) {
+  if (!c) return BEEP_EINVAL;
+  if (n <= 0 || !inst || !run || !invT || !order_out || !ntr_out || !nac_out || stream > 0xFFFF)
+    return fail(c, BEEP_EINVAL, "beep_mc3_swap: bad arguments");
+  for (int i = 0; i < n; i++)
+    if (!valid_inst(c, inst[i])) return fail(c, BEEP_EINVAL, "instance %d out of range", inst[i]);
+  int rc;
+  // scratch: invT[n], EF[n] (double); inst[n], run[n], order[n], ntr[n], nac[n] (int)
+  if ((rc = ensure_scratch(c, (size_t)n * (2 * 8 + 5 * 4) + 64))) return rc;
+  double* d_invT = (double*)c->d_scratch;
+  double* d_EF = d_invT + n;
+  int* d_inst = (int*)(d_EF + n);
+  int* d_run = d_inst + n;
+  int* d_order = d_run + n;
+  int* d_ntr = d_order + n;
+  int* d_nac = d_ntr + n;
+  HIPCHK(c, hipMemcpyAsync(d_invT, invT, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_inst, inst, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_run, run, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(beepdev::k_mc3_swap, dim3(1), dim3(64), 0, c->stream, n, d_run, d_invT, c->d_EF, d_inst, d_EF,
+                     d_order, d_ntr, d_nac, seed, stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(order_out, d_order, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ntr_out, d_ntr, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(nac_out, d_nac, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return BEEP_OK;
+}
+
 int beep_bootstrap(int device, int n, const double* L, double invT, uint64_t seed, uint32_t stream, int sec,
                    int32_t* bp_out, double* obprob_out) {
   if (n <= 0 || !L || !bp_out || !obprob_out || stream > 0xFFFF || sec < 0 || !(invT >= 0) || !std::isfinite(invT))

@@ -209,3 +209,94 @@ __global__ void k_particle_sample(int n, int nsec, const int* __restrict__ sec_t
 }
 
 }  // namespace beepdev
+
+namespace beepdev {
+
+#define BEEP_ABC_BLOCK 128
+#define BEEP_ABC_TILE 64
+
+// ABCSMC::calculate_particle_weight (src/abcsmc.cc:260-271) for n_new
+// proposals at once, FP64:
+//   den[i] = sum_j w[j] exp(MVN::probability(theta_i, prev_j)),
+//   MVN::probability(a, b) = -0.5 sum_{v1} sum_{v2} ((a-b)_v1 / size) invM[v1][v2] ((a-b)_v2 / size)
+//                                                         (src/mvn.cc:185-195)
+//   out[i] = exp(logprior[i]) / den[i].
+// One thread per proposal; the previous generation passes through LDS in
+// tiles.  LDS: invM [nvar][nvar], this block's scaled differences e[v][t]
+// = (theta_i - prev_j)_v / size, and a tile of prev and w.
+__global__ void __launch_bounds__(BEEP_ABC_BLOCK) k_abcsmc_weights(int nvar, int n_prev, const double* __restrict__ prev,
+                                                                 const double* __restrict__ w, int n_new,
+                                                                 const double* __restrict__ theta,
+                                                                 const double* __restrict__ invM, double size,
+                                                                 const double* __restrict__ logprior,
+                                                                 double* __restrict__ out) {
+  extern __shared__ double sh[];
+  double* sM = sh;                                   // [nvar][nvar]
+  double* se = sM + nvar * nvar;                     // [nvar][BEEP_ABC_BLOCK]
+  double* sp = se + nvar * BEEP_ABC_BLOCK;           // [BEEP_ABC_TILE][nvar]
+  double* sw = sp + BEEP_ABC_TILE * nvar;            // [BEEP_ABC_TILE]
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * BEEP_ABC_BLOCK + t;
+  for (int k = t; k < nvar * nvar; k += BEEP_ABC_BLOCK) sM[k] = invM[k];
+  double den = 0.0;
+  for (int j0 = 0; j0 < n_prev; j0 += BEEP_ABC_TILE) {
+    const int nj = min(BEEP_ABC_TILE, n_prev - j0);
+    __syncthreads();
+    for (int k = t; k < nj * nvar; k += BEEP_ABC_BLOCK) sp[k] = prev[(size_t)j0 * nvar + k];
+    for (int k = t; k < nj; k += BEEP_ABC_BLOCK) sw[k] = w[j0 + k];
+    __syncthreads();
+    if (i < n_new) {
+      for (int jj = 0; jj < nj; jj++) {
+        for (int v = 0; v < nvar; v++) se[v * BEEP_ABC_BLOCK + t] = (theta[(size_t)i * nvar + v] - sp[jj * nvar + v]) / size;
+        double sum = 0.0;
+        for (int v1 = 0; v1 < nvar; v1++) {
+          const double e1 = se[v1 * BEEP_ABC_BLOCK + t];
+          for (int v2 = 0; v2 < nvar; v2++) sum += e1 * sM[v1 * nvar + v2] * se[v2 * BEEP_ABC_BLOCK + t];
+        }
+        den += sw[jj] * exp(-0.5 * sum);
+      }
+    }
+  }
+  if (i < n_new) out[i] = exp(logprior[i]) / den;
+}
+
+// MC3::swap_states (src/mc3.cc:153-215) for the chains of one launch, each
+// group of chains sharing a run swapped among themselves: five sweeps over
+// neighbouring pairs (i, i+1) with the same run, accepting with probability
+//   al = exp(0.5 (invT_i - invT_{i+1}) (EF_i - EF_{i+1}))
+// and swapping the chain labels (order) and EF.  One thread per launch, as
+// the sweeps are sequential; draw k uses the Philox counter
+// {0xFFFD | stream << 16, k, 0, 0x7FFFFFFD}.
+__global__ void k_mc3_swap(int n, const int* __restrict__ run, const double* __restrict__ invT,
+                           const double* __restrict__ EFall, const int* __restrict__ inst, double* __restrict__ EF,
+                           int* __restrict__ order, int* __restrict__ ntr, int* __restrict__ nac, uint64_t seed,
+                           uint32_t stream) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int i = 0; i < n; i++) {
+    EF[i] = EFall[inst[i]];
+    order[i] = i;
+    ntr[i] = 0;
+    nac[i] = 0;
+  }
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  uint32_t draw = 0;
+  for (int loop = 0; loop < 5; loop++) {
+    for (int i = 0; i + 1 < n; i++) {
+      if (run[i] != run[i + 1]) continue;
+      const double al = exp(0.5 * (invT[i] - invT[i + 1]) * (EF[i] - EF[i + 1]));
+      ntr[i]++;
+      u4 r = philox4x32_10(u4{0xFFFDu | (stream << 16), draw++, 0u, 0x7FFFFFFDu}, k0, k1);
+      if (u01d(r.x) < al) {
+        nac[i]++;
+        int to = order[i];
+        order[i] = order[i + 1];
+        order[i + 1] = to;
+        double te = EF[i];
+        EF[i] = EF[i + 1];
+        EF[i + 1] = te;
+      }
+    }
+  }
+}
+
+}  // namespace beepdev

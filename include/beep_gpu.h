@@ -345,6 +345,31 @@ int beep_pmcmc_particle_sample(beep_ctx* ctx, int dst);
 int beep_bootstrap(int device, int n, const double* L, double invT, uint64_t seed, uint32_t stream, int sec,
                    int32_t* bp_out, double* obprob_out);
 
+/* ---- ABC-SMC and MC3 bookkeeping on the device ---- */
+/* ABCSMC::calculate_particle_weight (src/abcsmc.cc:260-271) with
+ * MVN::probability (src/mvn.cc:185-195) for a whole generation of proposals,
+ * FP64:
+ *   w_out[i] = exp(logprior[i]) / sum_j w_prev[j] exp(-1/2 sum_v1 sum_v2
+ *              (d_v1 / size) inv_M[v1][v2] (d_v2 / size)),   d = theta[i] - prev[j],
+ * over the nvar variables of the MVN kernel (the parameters that are not
+ * fixed, in its order); prev [n_prev][nvar], theta [n_new][nvar],
+ * inv_M [nvar][nvar]; nvar <= 32. */
+int beep_abcsmc_weights(int device, int nvar, int n_prev, const double* prev, const double* w_prev, int n_new,
+                        const double* theta, const double* inv_M, double size, const double* logprior,
+                        double* w_out);
+
+/* MC3::swap_states (src/mc3.cc:153-215): the n chains, in temperature order,
+ * are in instances inst[] of the context with runs run[] and inverse
+ * temperatures invT[]; their EF values are the instances' last ones on the
+ * device (set by beep_obs_loglik over the whole horizon).  Five sweeps of
+ * neighbouring swaps between chains of the same run, accepted with
+ * probability exp(0.5 (invT_i - invT_i+1)(EF_i - EF_i+1)).  order_out[i] is
+ * the chain whose state chain i takes (the caller relabels: chain i now uses
+ * instance inst[order_out[i]]; nothing is copied); ntr_out / nac_out [n] count
+ * the proposed and accepted swaps at each position. */
+int beep_mc3_swap(beep_ctx* ctx, int n, const int32_t* inst, const int32_t* run, const double* invT, uint64_t seed,
+                  uint32_t stream, int32_t* order_out, int32_t* ntr_out, int32_t* nac_out);
+
 /* Waits for the context's stream. */
 int beep_synchronize(beep_ctx* ctx);
 

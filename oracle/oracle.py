@@ -296,6 +296,50 @@ def pmcmc_obs_prob(orc, tb, N, invT, seeds, uniforms):
     return ob, np.array(backpart, dtype=np.int32), states
 
 
+def mvn_probability(pend, pstart, inv_M, size):
+    """MVN::probability (src/mvn.cc:185-195) over the MVN's variables, in the
+    reference's summation order."""
+    s = 0.0
+    nvar = len(pend)
+    for v1 in range(nvar):
+        for v2 in range(nvar):
+            val1 = pend[v1] - pstart[v1]
+            val2 = pend[v2] - pstart[v2]
+            s += (val1 / size) * inv_M[v1][v2] * (val2 / size)
+    return -0.5 * s
+
+
+def abcsmc_weight(theta, prev, w_prev, inv_M, size, logprior):
+    """ABCSMC::calculate_particle_weight (src/abcsmc.cc:260-271)."""
+    import math
+    s = 0.0
+    for j in range(len(prev)):
+        s += w_prev[j] * math.exp(mvn_probability(theta, prev[j], inv_M, size))
+    return math.exp(logprior) / s
+
+
+def mc3_swap(run, invT, EF, u):
+    """MC3::swap_states (src/mc3.cc:153-215) on one set of chains: five sweeps
+    of neighbouring swaps within a run, draw k compared with u[k].  Returns
+    (order, ntr, nac, EF after the swaps)."""
+    import math
+    n = len(run)
+    order, ntr, nac, EF = list(range(n)), [0] * n, [0] * n, list(EF)
+    k = 0
+    for _loop in range(5):
+        for i in range(n - 1):
+            if run[i] == run[i + 1]:
+                x = 0.5 * (invT[i] - invT[i + 1]) * (EF[i] - EF[i + 1])
+                al = math.exp(x) if x < 709.0 else math.inf      # C++ exp overflows to +inf: accepted
+                ntr[i] += 1
+                if u[k] < al:
+                    nac[i] += 1
+                    order[i], order[i + 1] = order[i + 1], order[i]
+                    EF[i], EF[i + 1] = EF[i + 1], EF[i]
+                k += 1
+    return order, ntr, nac, EF
+
+
 def get_al(mode, EFp, EFi, Prp, Pri, invT=1.0, EFcut=0.0):
     return lib().or_get_al(mode, EFp, EFi, Prp, Pri, invT, EFcut)
 

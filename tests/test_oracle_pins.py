@@ -235,3 +235,48 @@ def test_particle_sample_traces_the_ancestry():
     out = O.particle_sample(backpart, sections, traj)
     # section 2 from bp[3][0] = 1; section 1 from bp[2][1] = 0; section 0 from bp[1][0] = 2
     assert out[:, 0].tolist() == [20, 21, 2, 3, 14, 15]
+
+
+def test_abcsmc_weight_closed_forms():
+    """ABCSMC::calculate_particle_weight restatement: with a diagonal inverse
+    covariance the kernel is a product of 1-D Gaussian densities; with one
+    previous particle the weight is prior / kernel."""
+    rng = np.random.default_rng(3)
+    prev = rng.normal(0, 1, (40, 2))
+    w = rng.uniform(0.5, 2, 40)
+    theta = np.array([0.3, -0.4])
+    sd = np.array([0.7, 1.3])
+    size = 0.9
+    inv_M = np.diag(1 / sd ** 2)
+    got = O.abcsmc_weight(theta, prev, w, inv_M, size, np.log(0.25))
+    dens = np.prod(stats.norm.pdf(theta[None, :], prev, sd * size) * np.sqrt(2 * np.pi) * sd * size, axis=1)
+    assert got == pytest.approx(0.25 / (w * dens).sum(), rel=1e-12)
+    assert O.abcsmc_weight(theta, prev[:1], [2.0], inv_M, size, 0.0) == pytest.approx(
+        1 / (2 * np.exp(O.mvn_probability(theta, prev[0], inv_M, size))), rel=1e-14)
+    # a full inverse covariance: the quadratic form d^T A d / size^2
+    A = np.array([[2.0, 0.3], [0.3, 1.0]])
+    d = theta - prev[5]
+    assert O.mvn_probability(theta, prev[5], A, size) == pytest.approx(-0.5 * d @ A @ d / size ** 2, rel=1e-14)
+
+
+def test_mc3_swap_hand_worked():
+    """MC3::swap_states restatement: chains of one run swap when the draw is
+    below exp(0.5 (invT_i - invT_i+1)(EF_i - EF_i+1)); chains of different
+    runs never swap; five sweeps."""
+    # two chains, al = exp(0.5 * (1 - 0.5) * (10 - 2)) = e^2 > 1: the first
+    # sweep swaps; the EF values move with the states, so the later sweeps see
+    # al = e^-2 = 0.135 and reject draws of 0.9
+    order, ntr, nac, EF = O.mc3_swap([0, 0], [1.0, 0.5], [10.0, 2.0], [0.9] * 5)
+    assert order == [1, 0] and ntr == [5, 0] and nac == [1, 0] and EF == [2.0, 10.0]
+    # al = e^-2 = 0.135: draws of 0.2 reject throughout
+    order, _, nac, _ = O.mc3_swap([0, 0], [1.0, 0.5], [2.0, 10.0], [0.2, 0.2, 0.2, 0.2, 0.2])
+    assert order == [0, 1] and nac == [0, 0]
+    # a first draw of 0.1 accepts (EF now [10, 2]), the second sees al = e^2
+    # and accepts (back to [2, 10]), the rest see e^-2 and reject
+    order, _, nac, _ = O.mc3_swap([0, 0], [1.0, 0.5], [2.0, 10.0], [0.1, 0.2, 0.2, 0.2, 0.2])
+    assert nac == [2, 0] and order == [0, 1]
+    # different runs never swap; three chains of one run use two draws per sweep
+    order, ntr, _, _ = O.mc3_swap([0, 1], [1.0, 0.5], [10.0, 2.0], [])
+    assert order == [0, 1] and ntr == [0, 0]
+    _, ntr, _, _ = O.mc3_swap([0, 0, 0], [1.0, 0.7, 0.4], [5.0, 5.0, 5.0], [0.5] * 10)
+    assert ntr == [5, 5, 0]

@@ -67,14 +67,6 @@ __device__ __host__ inline u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1) {
   return c;
 }
 
-__device__ inline uint32_t pick(const u4& b, int i) {
-  uint32_t x = b.x;
-  x = i == 1 ? b.y : x;
-  x = i == 2 ? b.z : x;
-  x = i == 3 ? b.w : x;
-  return x;
-}
-
 // Uniform on the open interval (0,1) from 32 random bits, FP64.
 __device__ inline double u01d(uint32_t x) { return ((double)x + 0.5) * 2.3283064365386963e-10; }
 // Uniform on (0,1) from the top 24 bits, FP32.
@@ -95,7 +87,10 @@ struct CellRng {
       block++;
       idx = 0;
     }
-    uint32_t x = pick(buf, idx);
+    // The block is consumed from .x and shifted down: no per-lane index into
+    // it, which the compiler would otherwise lower to a stack array.
+    uint32_t x = buf.x;
+    buf.x = buf.y; buf.y = buf.z; buf.z = buf.w;
     idx++;
     return x;
   }

@@ -58,6 +58,22 @@ typedef __attribute__((address_space(1))) float gfloat;
 __device__ inline void st_nt(int* p, uint32_t i, int v) { __builtin_nontemporal_store(v, (gint*)p + i); }
 __device__ inline void st_nt(float* p, uint32_t i, float v) { __builtin_nontemporal_store(v, (gfloat*)p + i); }
 
+// Pointers typed by address space.  A plain pointer read out of a job
+// descriptor is generic, and a generic (flat) load counts against both the
+// vector-memory and the LDS counters, so every later wait on LDS also waits
+// for it; global and LDS reads typed as such avoid that.
+typedef const __attribute__((address_space(1))) float gcfloat;
+typedef const __attribute__((address_space(1))) int gcint;
+typedef const __attribute__((address_space(3))) float lcfloat;
+typedef const __attribute__((address_space(3))) int lcint;
+__device__ inline gcfloat* gptr(const float* p) { return (gcfloat*)p; }
+__device__ inline gcint* gptr(const int* p) { return (gcint*)p; }
+typedef const __attribute__((address_space(1))) uint8_t gcbyte;
+__device__ inline gcbyte* gptr(const uint8_t* p) { return (gcbyte*)p; }
+// A structure's descriptor is read-only for every kernel: read through a
+// constant-address-space pointer, its fields are uniform scalar loads.
+typedef const __attribute__((address_space(4))) TopoDev ctopo;
+
 // Queue capacity of large draws per step (items beyond it are drawn in place).
 __host__ __device__ inline int fused_qcap(int AG) { return AG < 32 ? 64 : (2 * AG < 1024 ? 2 * AG : 1024); }
 

@@ -31,12 +31,29 @@ static_assert(beepspec::CP % 4 == 0 && beepspec::CP >= beepspec::C, "CP");
 // accesses to its cells' chunk are consecutive in LDS.
 __device__ inline int pidx(int c, int cell, int AG) { return ((c >> 2) * AG + cell) * 4 + (c & 3); }
 
+// (B) for this thread's cells: I = (Idiag + Imap) m + fac Idiag (1 - m), with
+// Imap = M_off^T dinf and Idiag = diag * dinf.  F and I are the mixing arrays'
+// element types, qualified with the address space they are read from.
+template <typename F, typename I>
+__device__ inline void spec_mix(F* diag, F* fac, I* in_ptr, I* in_src, F* in_w, const float* sdinf, float* sI,
+                                float m, int AG, int tid, int nt) {
+  using namespace beepspec;
+  for (int cell = tid; cell < AG; cell += nt) {
+    const int j = cell / G, g = cell - j * G;
+    float idg = diag[j] * sdinf[cell];
+    float imp = 0.f;
+    int e1 = in_ptr[j + 1];
+    for (int e = in_ptr[j]; e < e1; e++) imp += in_w[e] * sdinf[in_src[e] * G + g];
+    sI[cell] = (idg + imp) * m + fac[j] * idg * (1.f - m);
+  }
+}
+
 template <int MODE>
 __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t seed, uint32_t stream) {
   using namespace beepspec;
   const Job jb = jobs[blockIdx.x];
   const int mode = MODE == MODE_ANY ? jb.mode : MODE;
-  const TopoDev* __restrict__ tpp = jb.topo;
+  ctopo* __restrict__ tpp = (ctopo*)jb.topo;
   const int A = tpp->A, S = tpp->S, AG = tpp->AG;
   const int nt = blockDim.x, tid = threadIdx.x;
   const float dt = tpp->dt;
@@ -53,26 +70,26 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   float* sN = srate + T * G;                           // [G][G]
   float* ssus = sN + G * G;                            // [G]
   float* sinf = ssus + G;                              // [C]
-  const float* diag = tpp->diag;
-  const float* fac = tpp->fac;
-  const int* in_ptr = tpp->in_ptr;
-  const int* in_src = tpp->in_src;
-  const float* in_w = tpp->in_w;
-  if (tpp->stage_mix) {
-    float* sdiag = sinf + C;
-    float* sfac = sdiag + A;
-    int* sptr = (int*)(sfac + A);
-    int* ssrc = sptr + A + 1;
-    float* sw = (float*)(ssrc + tpp->nnz);
+  gcfloat* diag = gptr(tpp->diag);
+  gcfloat* fac = gptr(tpp->fac);
+  gcint* in_ptr = gptr(tpp->in_ptr);
+  gcint* in_src = gptr(tpp->in_src);
+  gcfloat* in_w = gptr(tpp->in_w);
+  const bool staged = tpp->stage_mix != 0;
+  float* sdiag = sinf + C;
+  float* sfac = sdiag + A;
+  int* sptr = (int*)(sfac + A);
+  int* ssrc = sptr + A + 1;
+  float* sw = (float*)(ssrc + tpp->nnz);
+  if (staged) {
     for (int i = tid; i < A; i += nt) { sdiag[i] = diag[i]; sfac[i] = fac[i]; }
     for (int i = tid; i <= A; i += nt) sptr[i] = in_ptr[i];
     for (int i = tid; i < tpp->nnz; i += nt) { ssrc[i] = in_src[i]; sw[i] = in_w[i]; }
-    diag = sdiag; fac = sfac; in_ptr = sptr; in_src = ssrc; in_w = sw;
   }
 
-  const float* __restrict__ tab = jb.tab;
+  gcfloat* __restrict__ tab = gptr(jb.tab);
   {
-    const int* src = jb.from_init ? tpp->pop_init : jb.state;
+    gcint* src = gptr(jb.from_init ? tpp->pop_init : jb.state);
     for (int cell = tid; cell < AG; cell += nt) {
 #pragma unroll
       for (int c = 0; c < CP; c++) pop[pidx(c, cell, AG)] = c < C ? src[c * AG + cell] : 0;
@@ -110,25 +127,23 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   const size_t TAG = (size_t)T * AG, CAG = (size_t)C * AG;
 
   for (int s = jb.t0; s < jb.t1; s++) {
-    const float* beta_s = tab + tpp->off_beta + (size_t)s * A;
-    const float* efoi_s = tab + tpp->off_efoi + (size_t)s * G;
-    // (B) Imap = M_off^T dinf, Idiag = diag * dinf, I = (Idiag + Imap) m + fac Idiag (1 - m).
+    gcfloat* beta_s = tab + tpp->off_beta + (size_t)s * A;
+    gcfloat* efoi_s = tab + tpp->off_efoi + (size_t)s * G;
+    // (B) area mixing, from LDS when the CSR is staged there.
     const float m = tab[tpp->off_geo + s];
-    for (int cell = tid; cell < AG; cell += nt) {
-      const int j = cell / G, g = cell - j * G;
-      float idg = diag[j] * sdinf[cell];
-      float imp = 0.f;
-      int e1 = in_ptr[j + 1];
-      for (int e = in_ptr[j]; e < e1; e++) imp += in_w[e] * sdinf[in_src[e] * G + g];
-      sI[cell] = (idg + imp) * m + fac[j] * idg * (1.f - m);
-    }
+    if (staged)
+      spec_mix((lcfloat*)sdiag, (lcfloat*)sfac, (lcint*)sptr, (lcint*)ssrc, (lcfloat*)sw, sdinf, sI, m, AG, tid, nt);
+    else
+      spec_mix(diag, fac, in_ptr, in_src, in_w, sdinf, sI, m, AG, tid, nt);
     __syncthreads();
     // (C) means, small draws, own-cell updates; large draws queued.
     int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;
     float* tm_s = jb.tm ? jb.tm + (size_t)s * TAG : nullptr;
     int* pt_s = jb.pt ? jb.pt + (size_t)s * CAG : nullptr;
-    const int* itn = mode == MODE_MBP ? jb.init_tn + (size_t)s * TAG : nullptr;
-    const float* itm = mode == MODE_MBP ? jb.init_tm + (size_t)s * TAG : nullptr;
+    gcint* itn = mode == MODE_MBP ? gptr(jb.init_tn) + (size_t)s * TAG : nullptr;
+    gcfloat* itm = mode == MODE_MBP ? gptr(jb.init_tm) + (size_t)s * TAG : nullptr;
+    gcint* tn_r = mode == MODE_REPLAY ? gptr(jb.tn) + (size_t)s * TAG : nullptr;
+    gcbyte* simu_s = mode == MODE_MBP && jb.simu ? gptr(jb.simu) + (size_t)s * A : nullptr;
     const bool update = s < S - 1;           // the last step is not applied (src/state.cc:634)
     const uint32_t c0 = (uint32_t)s | (stream << 16);
     for (int cell = tid; cell < AG; cell += nt) {
@@ -154,7 +169,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
       for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * sI[j * G + gg];
       if (nmi < 0.f) nmi = 0.f;
       const float foi = beta_s[j] * nmi + efoi_s[g];
-      const bool simu = mode == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
+      const bool simu = mode == MODE_MBP ? (simu_s ? simu_s[j] != 0 : false) : true;
       CellRng rng;
       rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
       if (mode != MODE_REPLAY) rng.prime();
@@ -172,7 +187,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
         int kind = -1;
         float val = 0.f;
         if (mode == MODE_REPLAY) {
-          n = tn_s[tr * AG + cell];
+          n = tn_r[tr * AG + cell];
         } else if (lp > 0.f) {
           if (mode == MODE_SIM || simu) {
             if (lp < BEEP_POIS_INLINE) n = poisson_inversion(lp, rng.next());
@@ -261,7 +276,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   }
   for (int cell = tid; cell < AG; cell += nt) {
 #pragma unroll
-    for (int c = 0; c < C; c++) jb.state[c * AG + cell] = pop[pidx(c, cell, AG)];
+    for (int c = 0; c < C; c++) ((gint*)jb.state)[c * AG + cell] = pop[pidx(c, cell, AG)];
   }
 }
 

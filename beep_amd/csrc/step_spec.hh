@@ -17,7 +17,7 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
   const uint32_t stream = (uint32_t)rngp[1];
   const int b = blockIdx.y;
   const Job jb = jobs[b];
-  const TopoDev* __restrict__ tp = jb.topo;
+  ctopo* __restrict__ tp = (ctopo*)jb.topo;
   const int A = tp->A, S = tp->S, AG = tp->AG;
   const float dt = tp->dt;
   const int tid = threadIdx.x;
@@ -29,7 +29,7 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
   __shared__ float qval[T * BEEP_STEP_BLOCK];
   __shared__ int qcount;
   __shared__ float srate[T * G], sN[G * G], ssus[G], sinf[C];
-  const float* tab = jb.tab;
+  gcfloat* tab = gptr(jb.tab);
   for (int i = tid; i < T * G; i += BEEP_STEP_BLOCK) srate[i] = tab[tp->off_rate + i];
   for (int i = tid; i < G * G; i += BEEP_STEP_BLOCK) sN[i] = tab[tp->off_N + i];
   for (int i = tid; i < G; i += BEEP_STEP_BLOCK) ssus[i] = tab[tp->off_sus + i];
@@ -41,8 +41,10 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
   int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;
   float* tm_s = jb.tm ? jb.tm + (size_t)s * TAG : nullptr;
   int* pt_s = jb.pt ? jb.pt + (size_t)s * CAG : nullptr;
-  const int* itn = MODE == MODE_MBP ? jb.init_tn + (size_t)s * TAG : nullptr;
-  const float* itm = MODE == MODE_MBP ? jb.init_tm + (size_t)s * TAG : nullptr;
+  gcint* itn = MODE == MODE_MBP ? gptr(jb.init_tn) + (size_t)s * TAG : nullptr;
+  gcfloat* itm = MODE == MODE_MBP ? gptr(jb.init_tm) + (size_t)s * TAG : nullptr;
+  gcint* tn_r = MODE == MODE_REPLAY ? gptr(jb.tn) + (size_t)s * TAG : nullptr;
+  gint* state = (gint*)jb.state;
   const bool update = s < S - 1;
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   const uint32_t c0 = (uint32_t)s | (stream << 16);
@@ -52,7 +54,7 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
     g = cell - j * G;
     int p[CP];
 #pragma unroll
-    for (int c = 0; c < CP; c++) p[c] = c < C ? jb.state[c * AG + cell] : 0;
+    for (int c = 0; c < CP; c++) p[c] = c < C ? state[c * AG + cell] : 0;
     if (pt_s) {
 #pragma unroll
       for (int c = 0; c < C; c++) st_nt(pt_s, c * AG + cell, p[c]);
@@ -60,13 +62,13 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
     int q[CP];
 #pragma unroll
     for (int c = 0; c < CP; c++) q[c] = p[c];
-    const float* Ib = Ibuf + (size_t)b * G * Apad;
+    gcfloat* Ib = gptr(Ibuf) + (size_t)b * G * Apad;
     float nmi = 0.f;
 #pragma unroll
     for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * Ib[(size_t)gg * Apad + j];
     if (nmi < 0.f) nmi = 0.f;
     const float foi = tab[tp->off_beta + (size_t)s * A + j] * nmi + tab[tp->off_efoi + (size_t)s * G + g];
-    const bool simu = MODE == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
+    const bool simu = MODE == MODE_MBP ? (jb.simu ? gptr(jb.simu)[(size_t)s * A + j] != 0 : false) : true;
     CellRng rng;
     rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
     if (MODE != MODE_REPLAY) rng.prime();
@@ -82,7 +84,7 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
       int n = 0, kind = -1;
       float val = 0.f;
       if (MODE == MODE_REPLAY) {
-        n = tn_s[tr * AG + cell];
+        n = tn_r[tr * AG + cell];
       } else if (lp > 0.f) {
         if (MODE == MODE_SIM || simu) {
           if (lp < BEEP_POIS_INLINE) n = poisson_inversion(lp, rng.next());
@@ -143,13 +145,13 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
 #pragma unroll
     for (int c = 0; c < C; c++) {
       int v = spop[c][tid];
-      jb.state[c * AG + cell] = v;
+      state[c * AG + cell] = v;
       if (INFC[c]) {
         float w = sinf[c];
         if (w != 0.f) d += w * (float)v;
       }
     }
-    D[((size_t)b * G + g) * Apad + j] = d;
+    ((gfloat*)D)[((size_t)b * G + g) * Apad + j] = d;
   }
 }
 

@@ -74,6 +74,23 @@ __device__ inline gcbyte* gptr(const uint8_t* p) { return (gcbyte*)p; }
 // constant-address-space pointer, its fields are uniform scalar loads.
 typedef const __attribute__((address_space(4))) TopoDev ctopo;
 
+// Area mixing for this thread's cells (State::get_NMI, first half):
+// I = (Idiag + Imap) m + fac Idiag (1 - m), with Imap = M_off^T dinf and
+// Idiag = diag * dinf.  F and I are the mixing arrays' element types,
+// qualified with the address space they are read from (LDS when staged).
+template <typename F, typename I>
+__device__ inline void mix_cells(F* diag, F* fac, I* in_ptr, I* in_src, F* in_w, const float* sdinf, float* sI,
+                                 float m, int G, int AG, int tid, int nt) {
+  for (int cell = tid; cell < AG; cell += nt) {
+    const int j = cell / G, g = cell - j * G;
+    float idg = diag[j] * sdinf[cell];
+    float imp = 0.f;
+    int e1 = in_ptr[j + 1];
+    for (int e = in_ptr[j]; e < e1; e++) imp += in_w[e] * sdinf[in_src[e] * G + g];
+    sI[cell] = (idg + imp) * m + fac[j] * idg * (1.f - m);
+  }
+}
+
 // Queue capacity of large draws per step (items beyond it are drawn in place).
 __host__ __device__ inline int fused_qcap(int AG) { return AG < 32 ? 64 : (2 * AG < 1024 ? 2 * AG : 1024); }
 

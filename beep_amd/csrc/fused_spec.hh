@@ -31,23 +31,6 @@ static_assert(beepspec::CP % 4 == 0 && beepspec::CP >= beepspec::C, "CP");
 // accesses to its cells' chunk are consecutive in LDS.
 __device__ inline int pidx(int c, int cell, int AG) { return ((c >> 2) * AG + cell) * 4 + (c & 3); }
 
-// (B) for this thread's cells: I = (Idiag + Imap) m + fac Idiag (1 - m), with
-// Imap = M_off^T dinf and Idiag = diag * dinf.  F and I are the mixing arrays'
-// element types, qualified with the address space they are read from.
-template <typename F, typename I>
-__device__ inline void spec_mix(F* diag, F* fac, I* in_ptr, I* in_src, F* in_w, const float* sdinf, float* sI,
-                                float m, int AG, int tid, int nt) {
-  using namespace beepspec;
-  for (int cell = tid; cell < AG; cell += nt) {
-    const int j = cell / G, g = cell - j * G;
-    float idg = diag[j] * sdinf[cell];
-    float imp = 0.f;
-    int e1 = in_ptr[j + 1];
-    for (int e = in_ptr[j]; e < e1; e++) imp += in_w[e] * sdinf[in_src[e] * G + g];
-    sI[cell] = (idg + imp) * m + fac[j] * idg * (1.f - m);
-  }
-}
-
 template <int MODE>
 __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t seed, uint32_t stream) {
   using namespace beepspec;
@@ -132,9 +115,9 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     // (B) area mixing, from LDS when the CSR is staged there.
     const float m = tab[tpp->off_geo + s];
     if (staged)
-      spec_mix((lcfloat*)sdiag, (lcfloat*)sfac, (lcint*)sptr, (lcint*)ssrc, (lcfloat*)sw, sdinf, sI, m, AG, tid, nt);
+      mix_cells((lcfloat*)sdiag, (lcfloat*)sfac, (lcint*)sptr, (lcint*)ssrc, (lcfloat*)sw, sdinf, sI, m, G, AG, tid, nt);
     else
-      spec_mix(diag, fac, in_ptr, in_src, in_w, sdinf, sI, m, AG, tid, nt);
+      mix_cells(diag, fac, in_ptr, in_src, in_w, sdinf, sI, m, G, AG, tid, nt);
     __syncthreads();
     // (C) means, small draws, own-cell updates; large draws queued.
     int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;

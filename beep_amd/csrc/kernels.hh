@@ -27,7 +27,7 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
   const Job jb = jobs[blockIdx.x];
   // MODE_ANY launches mix simulations and proposals: the job says which.
   const int mode = MODE == MODE_ANY ? jb.mode : MODE;
-  const TopoDev* __restrict__ tpp = jb.topo;
+  ctopo* __restrict__ tpp = (ctopo*)jb.topo;
   const int A = tpp->A, G = tpp->G, C = tpp->C, T = tpp->T, S = tpp->S, AG = tpp->AG, inf_tr = tpp->inf_tr;
   const int nt = blockDim.x, tid = threadIdx.x;
   const float dt = tpp->dt;
@@ -47,32 +47,32 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
   float* sinf = ssus + G;                              // [C]
   int* sfrom = (int*)(sinf + C);                       // [T]
   int* sto = sfrom + T;                                // [T]
-  const float* diag = tpp->diag;
-  const float* fac = tpp->fac;
-  const int* in_ptr = tpp->in_ptr;
-  const int* in_src = tpp->in_src;
-  const float* in_w = tpp->in_w;
-  if (tpp->stage_mix) {
-    float* sdiag = (float*)(sto + T);
-    float* sfac = sdiag + A;
-    int* sptr = (int*)(sfac + A);
-    int* ssrc = sptr + A + 1;
-    float* sw = (float*)(ssrc + tpp->nnz);
+  gcfloat* diag = gptr(tpp->diag);
+  gcfloat* fac = gptr(tpp->fac);
+  gcint* in_ptr = gptr(tpp->in_ptr);
+  gcint* in_src = gptr(tpp->in_src);
+  gcfloat* in_w = gptr(tpp->in_w);
+  const bool staged = tpp->stage_mix != 0;
+  float* sdiag = (float*)(sto + T);
+  float* sfac = sdiag + A;
+  int* sptr = (int*)(sfac + A);
+  int* ssrc = sptr + A + 1;
+  float* sw = (float*)(ssrc + tpp->nnz);
+  if (staged) {
     for (int i = tid; i < A; i += nt) { sdiag[i] = diag[i]; sfac[i] = fac[i]; }
     for (int i = tid; i <= A; i += nt) sptr[i] = in_ptr[i];
     for (int i = tid; i < tpp->nnz; i += nt) { ssrc[i] = in_src[i]; sw[i] = in_w[i]; }
-    diag = sdiag; fac = sfac; in_ptr = sptr; in_src = ssrc; in_w = sw;
   }
 
-  const float* __restrict__ tab = jb.tab;
+  gcfloat* __restrict__ tab = gptr(jb.tab);
   {
-    const int* src = jb.from_init ? tpp->pop_init : jb.state;
+    gcint* src = gptr(jb.from_init ? tpp->pop_init : jb.state);
     for (int i = tid; i < C * AG; i += nt) popA[i] = src[i];
     for (int i = tid; i < T * G; i += nt) srate[i] = tab[tpp->off_rate + i];
     for (int i = tid; i < G * G; i += nt) sN[i] = tab[tpp->off_N + i];
     for (int i = tid; i < G; i += nt) ssus[i] = tab[tpp->off_sus + i];
     for (int i = tid; i < C; i += nt) sinf[i] = tab[tpp->off_inf + i];
-    for (int i = tid; i < T; i += nt) { sfrom[i] = tpp->from[i]; sto[i] = tpp->to[i]; }
+    for (int i = tid; i < T; i += nt) { sfrom[i] = gptr(tpp->from)[i]; sto[i] = gptr(tpp->to)[i]; }
     if (tid == 0) qcount[0] = 0;
   }
   __syncthreads();
@@ -96,24 +96,22 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
     // Phase 2: area-mixing contraction Imap = M_off^T dinf, Idiag = diag * dinf,
     // and I = (Idiag + Imap) m + fac Idiag (1 - m)  (State::get_NMI, first half).
     const float m = tab[tpp->off_geo + s];
-    for (int cell = tid; cell < AG; cell += nt) {
-      int j = cell / G, g = cell - j * G;
-      float idg = diag[j] * sdinf[cell];
-      float imp = 0.f;
-      int e1 = in_ptr[j + 1];
-      for (int e = in_ptr[j]; e < e1; e++) imp += in_w[e] * sdinf[in_src[e] * G + g];
-      sI[cell] = (idg + imp) * m + fac[j] * idg * (1.f - m);
-    }
+    if (staged)
+      mix_cells((lcfloat*)sdiag, (lcfloat*)sfac, (lcint*)sptr, (lcint*)ssrc, (lcfloat*)sw, sdinf, sI, m, G, AG, tid, nt);
+    else
+      mix_cells(diag, fac, in_ptr, in_src, in_w, sdinf, sI, m, G, AG, tid, nt);
     __syncthreads();
     // Phase 3a: age mixing, transition means (State::set_transmean), small
     // draws and the population update (State::update_pop) into the other buffer.
-    const float* beta_s = tab + tpp->off_beta + (size_t)s * A;
-    const float* efoi_s = tab + tpp->off_efoi + (size_t)s * G;
+    gcfloat* beta_s = tab + tpp->off_beta + (size_t)s * A;
+    gcfloat* efoi_s = tab + tpp->off_efoi + (size_t)s * G;
     int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;
     float* tm_s = jb.tm ? jb.tm + (size_t)s * TAG : nullptr;
     int* pt_s = jb.pt ? jb.pt + (size_t)s * CAG : nullptr;
-    const int* itn = mode == MODE_MBP ? jb.init_tn + (size_t)s * TAG : nullptr;
-    const float* itm = mode == MODE_MBP ? jb.init_tm + (size_t)s * TAG : nullptr;
+    gcint* itn = mode == MODE_MBP ? gptr(jb.init_tn) + (size_t)s * TAG : nullptr;
+    gcfloat* itm = mode == MODE_MBP ? gptr(jb.init_tm) + (size_t)s * TAG : nullptr;
+    gcint* tn_r = mode == MODE_REPLAY ? gptr(jb.tn) + (size_t)s * TAG : nullptr;
+    gcbyte* simu_s = mode == MODE_MBP && jb.simu ? gptr(jb.simu) + (size_t)s * A : nullptr;
     const bool update = s < S - 1;           // the last step is not applied (src/state.cc:634)
     const uint32_t c0 = (uint32_t)s | (stream << 16);
     for (int cell = tid; cell < AG; cell += nt) {
@@ -127,7 +125,7 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
       for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * sI[j * G + gg];
       if (nmi < 0.f) nmi = 0.f;
       const float foi = beta_s[j] * nmi + efoi_s[g];
-      const bool simu = mode == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
+      const bool simu = mode == MODE_MBP ? (simu_s ? simu_s[j] != 0 : false) : true;
       CellRng rng;
       rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
       if (mode != MODE_REPLAY) rng.prime();
@@ -145,7 +143,7 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
         int kind = -1;
         float val = 0.f;
         if (mode == MODE_REPLAY) {
-          n = tn_s[tr * AG + cell];
+          n = tn_r[tr * AG + cell];
         } else if (lp > 0.f) {
           if (mode == MODE_SIM || simu) {
             if (lp < BEEP_POIS_INLINE) n = poisson_inversion(lp, rng.next());
@@ -210,7 +208,7 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
   }
   __syncthreads();
   for (int cell = tid; cell < AG; cell += nt)
-    for (int c = 0; c < C; c++) jb.state[c * AG + cell] = cur[c * AG + cell];
+    for (int c = 0; c < C; c++) ((gint*)jb.state)[c * AG + cell] = cur[c * AG + cell];
 }
 
 }  // namespace beepdev

@@ -87,6 +87,13 @@ __device__ inline double obs_value(const ObsDev& od, int i, const int* tn, const
   return v;
 }
 
+// Global-typed view of a pointer held in a descriptor: a plain pointer read
+// from memory is generic, and loads through it would be flat loads.
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T* gview(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+
 __device__ inline double block_sum(double v, double* red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -119,30 +126,35 @@ __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ o
   const int in = inst[b];
   const int* tnb = tn + (size_t)in * traj_sz;
   const int* ptb = pt ? pt + (size_t)in * pop_traj_sz : nullptr;
+  const auto optr = gview(od.ptr);
+  const auto elem = gview(od.elem);
+  const auto factor = gview(od.factor);
+  const auto oy = gview(od.y);
+  const auto otype = gview(od.type);
   double L = 0, acc[8];
 #pragma unroll
   for (int d = 0; d < 8; d++) acc[d] = 0;
   for (int k = threadIdx.x; k < nsel; k += blockDim.x) {
     const int i = sel ? sel[k] : k;
     long long cnt = 0;
-    const int e1 = od.ptr[i + 1];
-    for (int e = od.ptr[i]; e < e1; e++) {
-      uint32_t x = od.elem[e];
+    const int e1 = optr[i + 1];
+    for (int e = optr[i]; e < e1; e++) {
+      uint32_t x = elem[e];
       cnt += (x & 0x80000000u) ? (ptb ? ptb[x & 0x7FFFFFFFu] : 0) : tnb[x];
     }
-    const double v = od.factor[i] * (double)cnt;
-    const double y = od.y[i];
-    const int type = od.type[i];
+    const double v = factor[i] * (double)cnt;
+    const double y = oy[i];
+    const int type = otype[i];
     double lp;
     if (!GENERAL || (type <= 1 && y != 999999997.0 && y != 999999998.0)) {
       const double d = y - (v < 0 ? 0.0 : v);
-      lp = od.lc[i] - d * d * od.iv[i];
+      lp = gview(od.lc)[i] - d * d * gview(od.iv)[i];
     } else {
-      lp = obs_prob(v, y, od.sd[i], type, od.shape[i], od.thresh[i]);
+      lp = obs_prob(v, y, gview(od.sd)[i], type, gview(od.shape)[i], gview(od.thresh)[i]);
     }
     L += lp;
     if (WANT_DT) {
-      const int dt = od.dt[i];
+      const int dt = gview(od.dt)[i];
 #pragma unroll
       for (int d = 0; d < 8; d++)
         if (d == dt) acc[d] += lp;

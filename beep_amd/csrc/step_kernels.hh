@@ -151,19 +151,20 @@ __global__ void __launch_bounds__(256) k_sparse_contract(SparseMix p) {
 // Grid (ceil(AG/256), nb).
 __global__ void __launch_bounds__(256) k_step_init(const Job* __restrict__ jobs, float* __restrict__ D, int Apad) {
   const Job jb = jobs[blockIdx.y];
-  const TopoDev* tp = jb.topo;
+  ctopo* tp = (ctopo*)jb.topo;
   const int AG = tp->AG, C = tp->C, G = tp->G;
   int cell = blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= AG) return;
+  gint* state = (gint*)jb.state;
   float d = 0.f;
   for (int c = 0; c < C; c++) {
-    int v = jb.from_init ? tp->pop_init[c * AG + cell] : jb.state[c * AG + cell];
-    if (jb.from_init) jb.state[c * AG + cell] = v;
-    float w = jb.tab[tp->off_inf + c];
+    int v = jb.from_init ? gptr(tp->pop_init)[c * AG + cell] : state[c * AG + cell];
+    if (jb.from_init) state[c * AG + cell] = v;
+    float w = gptr(jb.tab)[tp->off_inf + c];
     if (w != 0.f) d += w * (float)v;
   }
   int j = cell / G, g = cell - j * G;
-  D[((size_t)blockIdx.y * G + g) * Apad + j] = d;
+  ((gfloat*)D)[((size_t)blockIdx.y * G + g) * Apad + j] = d;
 }
 
 #define BEEP_STEP_QCAP 512
@@ -180,8 +181,10 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
   const uint32_t stream = (uint32_t)rngp[1];
   const int b = blockIdx.y;
   const Job jb = jobs[b];
-  const TopoDev* __restrict__ tp = jb.topo;
+  ctopo* __restrict__ tp = (ctopo*)jb.topo;
   const int A = tp->A, G = tp->G, C = tp->C, T = tp->T, S = tp->S, AG = tp->AG, inf_tr = tp->inf_tr;
+  gcint* tfrom = gptr(tp->from);
+  gcint* tto = gptr(tp->to);
   const float dt = tp->dt;
   const int tid = threadIdx.x;
   const int cell = blockIdx.x * BEEP_STEP_BLOCK + tid;
@@ -195,13 +198,15 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
   if (tid == 0) qcount = 0;
   __syncthreads();
 
-  const float* tab = jb.tab;
+  gcfloat* tab = gptr(jb.tab);
   const size_t TAG = (size_t)T * AG, CAG = (size_t)C * AG;
   int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;
   float* tm_s = jb.tm ? jb.tm + (size_t)s * TAG : nullptr;
   int* pt_s = jb.pt ? jb.pt + (size_t)s * CAG : nullptr;
-  const int* itn = MODE == MODE_MBP ? jb.init_tn + (size_t)s * TAG : nullptr;
-  const float* itm = MODE == MODE_MBP ? jb.init_tm + (size_t)s * TAG : nullptr;
+  gcint* itn = MODE == MODE_MBP ? gptr(jb.init_tn) + (size_t)s * TAG : nullptr;
+  gcfloat* itm = MODE == MODE_MBP ? gptr(jb.init_tm) + (size_t)s * TAG : nullptr;
+  gcint* tn_r = MODE == MODE_REPLAY ? gptr(jb.tn) + (size_t)s * TAG : nullptr;
+  gint* state = (gint*)jb.state;
   const bool update = s < S - 1;
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   const uint32_t c0 = (uint32_t)s | (stream << 16);
@@ -210,23 +215,23 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
     j = cell / G;
     g = cell - j * G;
     for (int c = 0; c < C; c++) {
-      int v = jb.state[c * AG + cell];
+      int v = state[c * AG + cell];
       spop0[c][tid] = v;
       spop[c][tid] = v;
       if (pt_s) st_nt(pt_s, c * AG + cell, v);
     }
-    const float* Ib = Ibuf + (size_t)b * G * Apad;
+    gcfloat* Ib = gptr(Ibuf) + (size_t)b * G * Apad;
     float nmi = 0.f;
     for (int gg = 0; gg < G; gg++) nmi += tab[tp->off_N + g * G + gg] * Ib[(size_t)gg * Apad + j];
     if (nmi < 0.f) nmi = 0.f;
     const float foi = tab[tp->off_beta + (size_t)s * A + j] * nmi + tab[tp->off_efoi + (size_t)s * G + g];
-    const bool simu = MODE == MODE_MBP ? (jb.simu ? jb.simu[(size_t)s * A + j] != 0 : false) : true;
+    const bool simu = MODE == MODE_MBP ? (jb.simu ? gptr(jb.simu)[(size_t)s * A + j] != 0 : false) : true;
     CellRng rng;
     rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
     if (MODE != MODE_REPLAY) rng.prime();
 #pragma unroll 1
     for (int tr = 0; tr < T; tr++) {
-      const int fr = tp->from[tr];
+      const int fr = tfrom[tr];
       float popu = (float)spop0[fr][tid];
       float lp = 0.f;
       if (popu > 0.f)
@@ -237,7 +242,7 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
       int n = 0, kind = -1;
       float val = 0.f;
       if (MODE == MODE_REPLAY) {
-        n = tn_s[tr * AG + cell];
+        n = tn_r[tr * AG + cell];
       } else if (lp > 0.f) {
         if (MODE == MODE_SIM || simu) {
           if (lp < BEEP_POIS_INLINE) n = poisson_inversion(lp, rng.next());
@@ -272,7 +277,7 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
       if (tn_s && MODE != MODE_REPLAY) st_nt(tn_s, tr * AG + cell, n);
       if (update && n != 0) {
         atomicSub(&spop[fr][tid], n);
-        atomicAdd(&spop[tp->to[tr]][tid], n);
+        atomicAdd(&spop[tto[tr]][tid], n);
       }
     }
   }
@@ -286,8 +291,8 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
     int n = draw_item(kind, qval[i], ni, k0, k1, c0, (uint32_t)c2, (uint32_t)tr, jb.key);
     if (tn_s) st_nt(tn_s, tr * AG + c2, n);
     if (update && n != 0) {
-      atomicSub(&spop[tp->from[tr]][t2], n);
-      atomicAdd(&spop[tp->to[tr]][t2], n);
+      atomicSub(&spop[tfrom[tr]][t2], n);
+      atomicAdd(&spop[tto[tr]][t2], n);
     }
   }
   __syncthreads();
@@ -295,11 +300,11 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
     float d = 0.f;
     for (int c = 0; c < C; c++) {
       int v = spop[c][tid];
-      jb.state[c * AG + cell] = v;
+      state[c * AG + cell] = v;
       float w = tab[tp->off_inf + c];
       if (w != 0.f) d += w * (float)v;
     }
-    D[((size_t)b * G + g) * Apad + j] = d;
+    ((gfloat*)D)[((size_t)b * G + g) * Apad + j] = d;
   }
 }
 

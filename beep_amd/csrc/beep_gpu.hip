@@ -105,6 +105,7 @@ struct beep_ctx {
   hipStream_t bstream[4] = {};
   hipEvent_t bstart = nullptr;
   bool obs_general = false;        // observations other than plain normal ones
+  bool obs_pop = false;            // observations counting population elements
 };
 
 namespace {
@@ -420,10 +421,15 @@ int launch_fused(beep_ctx* c, int mode, const Job* djobs, int njobs, uint64_t se
 // (all when null), into out[b * (1 + ndatatable)].
 void launch_obs_loglik(beep_ctx* c, int n, const int* d_inst, const int* sel, int nsel, double* out, int store_EF,
                        bool want_dt) {
-  auto* k = want_dt ? (c->obs_general ? beepdev::k_obs_loglik<true, true> : beepdev::k_obs_loglik<true, false>)
-                    : (c->obs_general ? beepdev::k_obs_loglik<false, true> : beepdev::k_obs_loglik<false, false>);
-  hipLaunchKernelGGL(k, dim3(n), dim3(256), 0, c->stream, c->d_obs, d_inst, c->d_tn, c->d_pt, c->traj_sz,
-                     c->pop_traj_sz, sel, nsel, out, c->d_EF, store_EF);
+  using K = decltype(&beepdev::k_obs_loglik<true, true, true>);
+  static const K ks[8] = {beepdev::k_obs_loglik<false, false, false>, beepdev::k_obs_loglik<false, false, true>,
+                          beepdev::k_obs_loglik<false, true, false>,  beepdev::k_obs_loglik<false, true, true>,
+                          beepdev::k_obs_loglik<true, false, false>,  beepdev::k_obs_loglik<true, false, true>,
+                          beepdev::k_obs_loglik<true, true, false>,   beepdev::k_obs_loglik<true, true, true>};
+  K k = ks[(want_dt ? 4 : 0) + (c->obs_general ? 2 : 0) + (c->obs_pop ? 1 : 0)];
+  const int ni = want_dt ? beepdev::OBS_INSTANCES<true> : beepdev::OBS_INSTANCES<false>;
+  hipLaunchKernelGGL(k, dim3((n + ni - 1) / ni), dim3(256), 0, c->stream, c->d_obs, n, d_inst, c->d_tn, c->d_pt,
+                     c->traj_sz, c->pop_traj_sz, sel, nsel, out, c->d_EF, store_EF);
 }
 
 Job make_job(beep_ctx* c, int inst, int slot, uint32_t key, int t0, int t1) {
@@ -659,6 +665,8 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
     for (int k = 0; k < nobs; k++) {
       const auto& o = m.obs[ord[k]];
       elem.insert(elem.end(), o.elem.begin(), o.elem.end());
+      for (uint32_t x : o.elem)
+        if (x & 0x80000000u) c->obs_pop = true;
       optr.push_back((int)elem.size());
       type.push_back(o.type); dtab.push_back(o.datatable);
       y.push_back(o.y); sd.push_back(o.sd); shape.push_back(o.shape); factor.push_back(o.factor); thresh.push_back(o.thresh);

@@ -106,67 +106,117 @@ __device__ inline double block_sum(double v, double* red) {
   return t;
 }
 
-// One workgroup per instance.  out[b*(1+ndt)] = log-likelihood,
+// Count of element x: a transnum index, or with bit 31 set a population
+// index (zero when the population trajectory is not kept).  POP false: the
+// model has no population elements.
+template <bool POP>
+__device__ inline int obs_elem(uint32_t x, const int* tnb, const int* ptb) {
+  if (!POP) return tnb[x];
+  return (x & 0x80000000u) ? (ptb ? ptb[x & 0x7FFFFFFFu] : 0) : tnb[x];
+}
+
+// Instances per workgroup of the observation kernel: four, or one when the
+// per-data-table sums are wanted (eight more accumulators per instance).
+template <bool WANT_DT>
+constexpr int OBS_INSTANCES = WANT_DT ? 1 : 4;
+
+// One workgroup per OBS_INSTANCES instances.  out[b*(1+ndt)] = log-likelihood,
 // out[b*(1+ndt)+1+d] = -2 * (sum over data table d)  (get_EF_datatable),
 // written when WANT_DT.  The counts of an observation are summed as
 // integers and scaled by its factor once; normal terms use the per-
 // observation constants lc and iv.  GENERAL: the model has observations
 // other than plain normal ones (Poisson, negative binomial, thresholds,
 // unknowns); without it the kernel holds only the normal path, which keeps
-// its register use, and so its occupancy, low.
-template <bool WANT_DT, bool GENERAL>
-__global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ odp, const int* __restrict__ inst,
+// its register use, and so its occupancy, low.  POP: some observation counts
+// population elements.
+template <bool WANT_DT, bool GENERAL, bool POP>
+__global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ odp, int n, const int* __restrict__ inst,
                                                     const int* __restrict__ tn, const int* __restrict__ pt,
                                                     size_t traj_sz, size_t pop_traj_sz, const int* __restrict__ sel,
                                                     int nsel, double* __restrict__ out, double* __restrict__ EF,
                                                     int store_EF) {
+  // Instances per workgroup: they share the observation structure, so one
+  // thread reads an observation's elements once and gathers the counts of
+  // all NI instances with independent loads.
+  constexpr int NI = OBS_INSTANCES<WANT_DT>;
   __shared__ double red[8];
   const ObsDev od = *odp;
-  const int b = blockIdx.x;
-  const int in = inst[b];
-  const int* tnb = tn + (size_t)in * traj_sz;
-  const int* ptb = pt ? pt + (size_t)in * pop_traj_sz : nullptr;
+  const int b0 = blockIdx.x * NI;
+  int ins[NI];
+  const int* tnb[NI];
+  const int* ptb[NI];
+#pragma unroll
+  for (int q = 0; q < NI; q++) {
+    ins[q] = inst[b0 + q < n ? b0 + q : b0];        // past the end: recomputes instance b0, not written
+    tnb[q] = tn + (size_t)ins[q] * traj_sz;
+    ptb[q] = pt ? pt + (size_t)ins[q] * pop_traj_sz : nullptr;
+  }
   const auto optr = gview(od.ptr);
   const auto elem = gview(od.elem);
   const auto factor = gview(od.factor);
   const auto oy = gview(od.y);
   const auto otype = gview(od.type);
-  double L = 0, acc[8];
+  double L[NI], acc[8];
+#pragma unroll
+  for (int q = 0; q < NI; q++) L[q] = 0;
 #pragma unroll
   for (int d = 0; d < 8; d++) acc[d] = 0;
   for (int k = threadIdx.x; k < nsel; k += blockDim.x) {
     const int i = sel ? sel[k] : k;
-    long long cnt = 0;
+    long long cnt[NI];
+#pragma unroll
+    for (int q = 0; q < NI; q++) cnt[q] = 0;
+    int e = optr[i];
     const int e1 = optr[i + 1];
-    for (int e = optr[i]; e < e1; e++) {
-      uint32_t x = elem[e];
-      cnt += (x & 0x80000000u) ? (ptb ? ptb[x & 0x7FFFFFFFu] : 0) : tnb[x];
+    // Four elements at a time: without population elements the trajectory
+    // loads are unconditional and in flight together.
+    for (; e + 4 <= e1; e += 4) {
+      const uint32_t x0 = elem[e], x1 = elem[e + 1], x2 = elem[e + 2], x3 = elem[e + 3];
+#pragma unroll
+      for (int q = 0; q < NI; q++) {
+        const int n0 = obs_elem<POP>(x0, tnb[q], ptb[q]), n1 = obs_elem<POP>(x1, tnb[q], ptb[q]);
+        const int n2 = obs_elem<POP>(x2, tnb[q], ptb[q]), n3 = obs_elem<POP>(x3, tnb[q], ptb[q]);
+        cnt[q] += (long long)n0 + n1 + n2 + n3;
+      }
     }
-    const double v = factor[i] * (double)cnt;
+    for (; e < e1; e++) {
+      const uint32_t x = elem[e];
+#pragma unroll
+      for (int q = 0; q < NI; q++) cnt[q] += obs_elem<POP>(x, tnb[q], ptb[q]);
+    }
+    const double f = factor[i];
     const double y = oy[i];
     const int type = otype[i];
-    double lp;
-    if (!GENERAL || (type <= 1 && y != 999999997.0 && y != 999999998.0)) {
-      const double d = y - (v < 0 ? 0.0 : v);
-      lp = gview(od.lc)[i] - d * d * gview(od.iv)[i];
-    } else {
-      lp = obs_prob(v, y, gview(od.sd)[i], type, gview(od.shape)[i], gview(od.thresh)[i]);
-    }
-    L += lp;
-    if (WANT_DT) {
-      const int dt = gview(od.dt)[i];
+    const bool plain = !GENERAL || (type <= 1 && y != 999999997.0 && y != 999999998.0);
 #pragma unroll
-      for (int d = 0; d < 8; d++)
-        if (d == dt) acc[d] += lp;
+    for (int q = 0; q < NI; q++) {
+      const double v = f * (double)cnt[q];
+      double lp;
+      if (plain) {
+        const double d = y - (v < 0 ? 0.0 : v);
+        lp = gview(od.lc)[i] - d * d * gview(od.iv)[i];
+      } else {
+        lp = obs_prob(v, y, gview(od.sd)[i], type, gview(od.shape)[i], gview(od.thresh)[i]);
+      }
+      L[q] += lp;
+      if (WANT_DT) {
+        const int dt = gview(od.dt)[i];
+#pragma unroll
+        for (int d = 0; d < 8; d++)
+          if (d == dt) acc[d] += lp;
+      }
     }
   }
-  double Lt = block_sum(L, red);
-  double* o = out + (size_t)b * (1 + od.ndt);
-  if (threadIdx.x == 0) {
-    o[0] = Lt;
-    if (store_EF) EF[in] = -2.0 * Lt;
+#pragma unroll
+  for (int q = 0; q < NI; q++) {
+    const double Lt = block_sum(L[q], red);
+    if (threadIdx.x == 0 && b0 + q < n) {
+      out[(size_t)(b0 + q) * (1 + od.ndt)] = Lt;
+      if (store_EF) EF[ins[q]] = -2.0 * Lt;
+    }
   }
   if (WANT_DT) {
+    double* o = out + (size_t)b0 * (1 + od.ndt);
     for (int d = 0; d < od.ndt; d++) {
       double t = block_sum(acc[d < 8 ? d : 0], red);
       if (threadIdx.x == 0) o[1 + d] = -2.0 * t;

@@ -30,7 +30,7 @@ METRIC = "particle-timesteps/sec batched vs plain GPU path; p50/p99 request late
 # by the ratio of known bytes to counter reading on a plain copy kernel:
 # the specialised kernel (profiles/beep_fused_spec_sim_spatial_abcsmc.md) and
 # the generic one (profiles/k_fused_spatial_abcsmc_traffic.md).
-SPEC_TRAFFIC_BYTES = 44_150_094_023
+SPEC_TRAFFIC_BYTES = 35_530_804_861
 GENERIC_TRAFFIC_BYTES = 41_895_375_565
 
 

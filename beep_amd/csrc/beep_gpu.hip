@@ -537,7 +537,7 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   const auto& m = c->m;
 
   // Topology.
-  int *d_from, *d_to, *d_in_ptr, *d_in_src, *d_pop_init;
+  int *d_from, *d_to, *d_in_ptr, *d_in_src, *d_pop_init, *d_order;
   float *d_diag, *d_fac, *d_in_w;
   std::vector<int> from(m.T), to(m.T);
   for (int t = 0; t < m.T; t++) { from[t] = m.trans[t].from; to[t] = m.trans[t].to; }
@@ -546,9 +546,17 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
     for (int co = 0; co < m.C; co++)
       for (int g = 0; g < m.G; g++)
         pop_dev[(size_t)co * m.AG + a * m.G + g] = m.pop_init[((size_t)a * m.C + co) * m.G + g];
+  // Areas by number of incoming mixing entries, most first (ties by index):
+  // the order in which the fused kernels' lanes take areas for the mixing.
+  std::vector<int> order(m.A);
+  for (int a = 0; a < m.A; a++) order[a] = a;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return m.in_ptr[x + 1] - m.in_ptr[x] > m.in_ptr[y + 1] - m.in_ptr[y];
+  });
   int rc;
   if ((rc = upload(c, &d_from, from)) || (rc = upload(c, &d_to, to)) || (rc = upload(c, &d_in_ptr, m.in_ptr)) ||
-      (rc = upload(c, &d_in_src, m.in_src)) || (rc = upload(c, &d_pop_init, pop_dev)))
+      (rc = upload(c, &d_in_src, m.in_src)) || (rc = upload(c, &d_pop_init, pop_dev)) ||
+      (rc = upload(c, &d_order, order)))
     return rc;
   std::vector<float> diag = to_f32(m.M_diag), fac = to_f32(m.M_factor), inw = to_f32(m.in_w);
   if ((rc = upload(c, &d_diag, diag)) || (rc = upload(c, &d_fac, fac)) || (rc = upload(c, &d_in_w, inw))) return rc;
@@ -571,7 +579,7 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   tp.off_beta = off; off += m.S * m.A;
   c->slot_stride = ((size_t)off + 63) / 64 * 64;
   tp.from = d_from; tp.to = d_to; tp.diag = d_diag; tp.fac = d_fac;
-  tp.in_ptr = d_in_ptr; tp.in_src = d_in_src; tp.in_w = d_in_w; tp.pop_init = d_pop_init;
+  tp.in_ptr = d_in_ptr; tp.in_src = d_in_src; tp.in_w = d_in_w; tp.mix_order = d_order; tp.pop_init = d_pop_init;
   if ((rc = dalloc(c, (void**)&c->d_topo, sizeof(TopoDev)))) return rc;
   HIPCHK(c, hipMemcpy(c->d_topo, &tp, sizeof tp, hipMemcpyHostToDevice));
 

@@ -25,6 +25,7 @@ struct TopoDev {
   const int* in_ptr;     // [A+1] incoming off-diagonal mixing (gather form)
   const int* in_src;     // [nnz]
   const float* in_w;     // [nnz]
+  const int* mix_order;  // [A] areas by number of incoming entries, most first
   const int* pop_init;   // [C][AG]
 };
 
@@ -74,19 +75,36 @@ __device__ inline gcbyte* gptr(const uint8_t* p) { return (gcbyte*)p; }
 // constant-address-space pointer, its fields are uniform scalar loads.
 typedef const __attribute__((address_space(4))) TopoDev ctopo;
 
-// Area mixing for this thread's cells (State::get_NMI, first half):
-// I = (Idiag + Imap) m + fac Idiag (1 - m), with Imap = M_off^T dinf and
-// Idiag = diag * dinf.  F and I are the mixing arrays' element types,
-// qualified with the address space they are read from (LDS when staged).
+// Area mixing (State::get_NMI, first half): I = (Idiag + Imap) m +
+// fac Idiag (1 - m), with Imap = M_off^T dinf and Idiag = diag * dinf.  F and
+// I are the mixing arrays' element types, qualified with the address space
+// they are read from (LDS when staged).
+//
+// Lanes take areas in the order `order` (most incoming entries first), so
+// the areas of one wavefront have similar row lengths and the gather loop,
+// whose trip count is the wavefront's longest row, wastes few iterations.
+// Each cell still sums its row in CSR order, four entries per iteration with
+// the loads issued together: the FP32 result is the same as a plain loop's.
 template <typename F, typename I>
-__device__ inline void mix_cells(F* diag, F* fac, I* in_ptr, I* in_src, F* in_w, const float* sdinf, float* sI,
-                                 float m, int G, int AG, int tid, int nt) {
-  for (int cell = tid; cell < AG; cell += nt) {
-    const int j = cell / G, g = cell - j * G;
+__device__ inline void mix_cells(F* diag, F* fac, I* in_ptr, I* in_src, F* in_w, I* order, const float* sdinf,
+                                 float* sI, float m, int G, int AG, int tid, int nt) {
+  for (int t = tid; t < AG; t += nt) {
+    const int jo = t / G, g = t - jo * G;
+    const int j = order[jo], cell = j * G + g;
     float idg = diag[j] * sdinf[cell];
     float imp = 0.f;
-    int e1 = in_ptr[j + 1];
-    for (int e = in_ptr[j]; e < e1; e++) imp += in_w[e] * sdinf[in_src[e] * G + g];
+    int e = in_ptr[j];
+    const int e1 = in_ptr[j + 1];
+    for (; e + 4 <= e1; e += 4) {
+      const int s0 = in_src[e], s1 = in_src[e + 1], s2 = in_src[e + 2], s3 = in_src[e + 3];
+      const float w0 = in_w[e], w1 = in_w[e + 1], w2 = in_w[e + 2], w3 = in_w[e + 3];
+      const float d0 = sdinf[s0 * G + g], d1 = sdinf[s1 * G + g], d2 = sdinf[s2 * G + g], d3 = sdinf[s3 * G + g];
+      imp += w0 * d0;
+      imp += w1 * d1;
+      imp += w2 * d2;
+      imp += w3 * d3;
+    }
+    for (; e < e1; e++) imp += in_w[e] * sdinf[in_src[e] * G + g];
     sI[cell] = (idg + imp) * m + fac[j] * idg * (1.f - m);
   }
 }
@@ -95,7 +113,7 @@ __device__ inline void mix_cells(F* diag, F* fac, I* in_ptr, I* in_src, F* in_w,
 __host__ __device__ inline int fused_qcap(int AG) { return AG < 32 ? 64 : (2 * AG < 1024 ? 2 * AG : 1024); }
 
 // Bytes of the mixing CSR when staged in LDS.
-__host__ __device__ inline size_t fused_mix_bytes(int A, int nnz) { return (size_t)4 * (3 * (size_t)A + 1 + 2 * (size_t)nnz); }
+__host__ __device__ inline size_t fused_mix_bytes(int A, int nnz) { return (size_t)4 * (4 * (size_t)A + 1 + 2 * (size_t)nnz); }
 
 // LDS bytes a job needs: populations double-buffered, infectivity, the mixed
 // infectivity I, the queue of large draws, the small per-slot tables and,

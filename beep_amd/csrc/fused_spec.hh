@@ -58,14 +58,16 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   gcint* in_ptr = gptr(tpp->in_ptr);
   gcint* in_src = gptr(tpp->in_src);
   gcfloat* in_w = gptr(tpp->in_w);
+  gcint* order = gptr(tpp->mix_order);
   const bool staged = tpp->stage_mix != 0;
   float* sdiag = sinf + C;
   float* sfac = sdiag + A;
   int* sptr = (int*)(sfac + A);
   int* ssrc = sptr + A + 1;
   float* sw = (float*)(ssrc + tpp->nnz);
+  int* sord = (int*)(sw + tpp->nnz);
   if (staged) {
-    for (int i = tid; i < A; i += nt) { sdiag[i] = diag[i]; sfac[i] = fac[i]; }
+    for (int i = tid; i < A; i += nt) { sdiag[i] = diag[i]; sfac[i] = fac[i]; sord[i] = order[i]; }
     for (int i = tid; i <= A; i += nt) sptr[i] = in_ptr[i];
     for (int i = tid; i < tpp->nnz; i += nt) { ssrc[i] = in_src[i]; sw[i] = in_w[i]; }
   }
@@ -115,9 +117,10 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     // (B) area mixing, from LDS when the CSR is staged there.
     const float m = tab[tpp->off_geo + s];
     if (staged)
-      mix_cells((lcfloat*)sdiag, (lcfloat*)sfac, (lcint*)sptr, (lcint*)ssrc, (lcfloat*)sw, sdinf, sI, m, G, AG, tid, nt);
+      mix_cells((lcfloat*)sdiag, (lcfloat*)sfac, (lcint*)sptr, (lcint*)ssrc, (lcfloat*)sw, (lcint*)sord, sdinf, sI, m,
+                G, AG, tid, nt);
     else
-      mix_cells(diag, fac, in_ptr, in_src, in_w, sdinf, sI, m, G, AG, tid, nt);
+      mix_cells(diag, fac, in_ptr, in_src, in_w, order, sdinf, sI, m, G, AG, tid, nt);
     __syncthreads();
     // (C) means, small draws, own-cell updates; large draws queued.
     int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;

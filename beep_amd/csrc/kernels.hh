@@ -52,14 +52,16 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
   gcint* in_ptr = gptr(tpp->in_ptr);
   gcint* in_src = gptr(tpp->in_src);
   gcfloat* in_w = gptr(tpp->in_w);
+  gcint* order = gptr(tpp->mix_order);
   const bool staged = tpp->stage_mix != 0;
   float* sdiag = (float*)(sto + T);
   float* sfac = sdiag + A;
   int* sptr = (int*)(sfac + A);
   int* ssrc = sptr + A + 1;
   float* sw = (float*)(ssrc + tpp->nnz);
+  int* sord = (int*)(sw + tpp->nnz);
   if (staged) {
-    for (int i = tid; i < A; i += nt) { sdiag[i] = diag[i]; sfac[i] = fac[i]; }
+    for (int i = tid; i < A; i += nt) { sdiag[i] = diag[i]; sfac[i] = fac[i]; sord[i] = order[i]; }
     for (int i = tid; i <= A; i += nt) sptr[i] = in_ptr[i];
     for (int i = tid; i < tpp->nnz; i += nt) { ssrc[i] = in_src[i]; sw[i] = in_w[i]; }
   }
@@ -97,9 +99,10 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
     // and I = (Idiag + Imap) m + fac Idiag (1 - m)  (State::get_NMI, first half).
     const float m = tab[tpp->off_geo + s];
     if (staged)
-      mix_cells((lcfloat*)sdiag, (lcfloat*)sfac, (lcint*)sptr, (lcint*)ssrc, (lcfloat*)sw, sdinf, sI, m, G, AG, tid, nt);
+      mix_cells((lcfloat*)sdiag, (lcfloat*)sfac, (lcint*)sptr, (lcint*)ssrc, (lcfloat*)sw, (lcint*)sord, sdinf, sI, m,
+                G, AG, tid, nt);
     else
-      mix_cells(diag, fac, in_ptr, in_src, in_w, sdinf, sI, m, G, AG, tid, nt);
+      mix_cells(diag, fac, in_ptr, in_src, in_w, order, sdinf, sI, m, G, AG, tid, nt);
     __syncthreads();
     // Phase 3a: age mixing, transition means (State::set_transmean), small
     // draws and the population update (State::update_pop) into the other buffer.

@@ -580,8 +580,6 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   c->slot_stride = ((size_t)off + 63) / 64 * 64;
   tp.from = d_from; tp.to = d_to; tp.diag = d_diag; tp.fac = d_fac;
   tp.in_ptr = d_in_ptr; tp.in_src = d_in_src; tp.in_w = d_in_w; tp.mix_order = d_order; tp.pop_init = d_pop_init;
-  if ((rc = dalloc(c, (void**)&c->d_topo, sizeof(TopoDev)))) return rc;
-  HIPCHK(c, hipMemcpy(c->d_topo, &tp, sizeof tp, hipMemcpyHostToDevice));
 
   // Path: the fused per-instance kernel when an instance fits in LDS and the
   // mixing is sparse; otherwise the step-wise path, with the contraction on
@@ -597,16 +595,29 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
     // The step-wise transition kernel's LDS: populations, queue, tables.
     size_t step_lds = 4 * ((size_t)16 * BEEP_STEP_BLOCK + 2 * (size_t)m.T * BEEP_STEP_BLOCK + 1 + (size_t)m.T * m.G +
                            (size_t)m.G * m.G + m.G + m.C);
-    size_t sl = beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, tp.stage_mix);
+    // The specialised kernel stages the mixing CSR in LDS whenever it fits:
+    // on the 100-area benchmark model, gathering from LDS at 3 resident
+    // instances per CU beat gathering through the caches at 4.
+    // BEEP_STAGE_MIX=0/1 forces the choice (a tuning knob).
+    int stage = tp.stage_mix;
+    if (!c->stepwise) {
+      const bool small = beepdev::fused_mix_bytes(m.A, tp.nnz) <= 32768;
+      stage = small && beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, 1) <= 160 * 1024;
+      if (const char* e = std::getenv("BEEP_STAGE_MIX")) stage = small && std::atoi(e) != 0;
+    }
+    size_t sl = beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, stage);
     if (c->stepwise ? step_lds <= 65536 : sl <= 160 * 1024) {
       SpecKernels* k = nullptr;
       if ((rc = spec_kernels(c, &k))) return rc;
       if (c->stepwise || sl <= (size_t)k->max_dyn_lds) {
         c->spec = k;
         c->spec_lds = c->stepwise ? 0 : sl;
+        if (!c->stepwise) tp.stage_mix = stage;
       }
     }
   }
+  if ((rc = dalloc(c, (void**)&c->d_topo, sizeof(TopoDev)))) return rc;
+  HIPCHK(c, hipMemcpy(c->d_topo, &tp, sizeof tp, hipMemcpyHostToDevice));
   c->Apad = (m.A + 63) / 64 * 64;
   if (c->dense) {
     std::vector<float> M((size_t)c->Apad * c->Apad, 0.f);

@@ -37,6 +37,7 @@
 #else
 // Run-time compilation provides the fixed-width types in its own namespace.
 typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::uint16_t uint16_t;
 typedef __hip_internal::uint32_t uint32_t;
 typedef __hip_internal::uint64_t uint64_t;
 #endif

@@ -157,14 +157,20 @@ __device__ inline __attribute__((always_inline)) int draw_item_inline(int kind, 
   return draw_item_body<true>(kind, val, ni, k0, k1, c0, cell, tr, key);
 }
 
+// The specialised kernel's queue of large draws names an item by
+// cell * T + tr, in 16 bits when every such index fits.
+__host__ __device__ inline bool spec_queue16(int T, int AG) { return (size_t)T * AG <= 65536; }
+
 // LDS bytes of the structure-specialised kernel: populations [AG][CP] (one
 // buffer, CP = C rounded up to 4), infectivity, mixed infectivity, the queue
-// of large draws (room for every (cell, transition)), the per-slot tables
-// and, when staged, the mixing CSR.
+// of large draws (room for every (cell, transition): a float value each, and
+// an index of 2 or 4 bytes), the per-slot tables and, when staged, the
+// mixing CSR.
 __host__ __device__ inline size_t spec_lds_bytes(int A, int G, int C, int T, int nnz, int stage_mix) {
   int AG = A * G, CP = (C + 3) / 4 * 4;
-  size_t b = (size_t)4 * ((size_t)AG * CP + 2 * (size_t)AG + 2 * (size_t)T * AG + 4 + (size_t)T * G +
-                          (size_t)G * G + G + C);
+  size_t q = (size_t)T * AG;
+  size_t b = (size_t)4 * ((size_t)AG * CP + 2 * (size_t)AG + q + 4 + (size_t)T * G + (size_t)G * G + G + C);
+  b += (q * (spec_queue16(T, AG) ? 2 : 4) + 3) / 4 * 4;
   if (stage_mix) b += fused_mix_bytes(A, nnz);
   return b;
 }

@@ -41,13 +41,13 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   const int nt = blockDim.x, tid = threadIdx.x;
   const float dt = tpp->dt;
   const int qcap = T * AG;                             // every (cell, transition): never full
+  const bool q16 = spec_queue16(T, AG);
 
   extern __shared__ int smem[];
   int* pop = smem;                                     // [CP/4][AG] chunks of 4 compartments
   float* sdinf = (float*)(pop + (size_t)AG * CP);      // [AG]
   float* sI = sdinf + AG;                              // [AG]
-  uint32_t* qcode = (uint32_t*)(sI + AG);              // [qcap]
-  float* qval = (float*)(qcode + qcap);                // [qcap]
+  float* qval = sI + AG;                               // [qcap] value; negative: ITEM_POIS_ADD
   int* qcount = (int*)(qval + qcap);                   // [4]
   float* srate = (float*)(qcount + 4);                 // [T][G]
   float* sN = srate + T * G;                           // [G][G]
@@ -66,6 +66,11 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   int* ssrc = sptr + A + 1;
   float* sw = (float*)(ssrc + tpp->nnz);
   int* sord = (int*)(sw + tpp->nnz);
+  // The queue's item indices (cell * T + tr), after the tables and, when
+  // staged, the mixing CSR: 16 bits each when they fit.
+  void* qidx = staged ? (void*)(sord + A) : (void*)(sinf + C);
+  uint16_t* qidx16 = (uint16_t*)qidx;
+  uint32_t* qidx32 = (uint32_t*)qidx;
   if (staged) {
     for (int i = tid; i < A; i += nt) { sdiag[i] = diag[i]; sfac[i] = fac[i]; sord[i] = order[i]; }
     for (int i = tid; i <= A; i += nt) sptr[i] = in_ptr[i];
@@ -198,8 +203,10 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
         }
         if (kind >= 0) {                     // drawn in (D)
           int slot = atomicAdd(qcount, 1);
-          qcode[slot] = (uint32_t)cell | ((uint32_t)tr << 20) | ((uint32_t)kind << 28);
-          qval[slot] = val;
+          const uint32_t idx = (uint32_t)cell * T + tr;
+          if (q16) qidx16[slot] = (uint16_t)idx;
+          else qidx32[slot] = idx;
+          qval[slot] = kind == ITEM_POIS_ADD ? -val : val;
         } else {
           if (tn_s && mode != MODE_REPLAY) st_nt(tn_s, tr * AG + cell, n);
           if (update) {
@@ -221,10 +228,15 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
       const int nq = qcount[0];
       if (nq > 0) {
         for (int i = tid; i < nq; i += nt) {
-          uint32_t code = qcode[i];
-          int cell = (int)(code & 0xFFFFFu), tr = (int)((code >> 20) & 0xFFu), kind = (int)(code >> 28);
+          const uint32_t idx = q16 ? (uint32_t)qidx16[i] : qidx32[i];
+          const int cell = (int)(idx / T), tr = (int)(idx - (uint32_t)cell * T);
+          // The kind from the value: Poisson means queued are >= the inline
+          // limit, binomial probabilities lie in (0, 1), coupled Poisson
+          // increments are stored negated.
+          const float v = qval[i];
+          const int kind = v < 0.f ? ITEM_POIS_ADD : (v < 1.f ? ITEM_BINOM : ITEM_POIS);
           int ni = mode == MODE_MBP ? itn[tr * AG + cell] : 0;
-          int n = draw_item_inline(kind, qval[i], ni, k0, k1, c0, (uint32_t)cell, (uint32_t)tr, jb.key);
+          int n = draw_item_inline(kind, fabsf(v), ni, k0, k1, c0, (uint32_t)cell, (uint32_t)tr, jb.key);
           if (tn_s) st_nt(tn_s, tr * AG + cell, n);
           if (update && n != 0) {
             int fr = 0, to = 0;

@@ -1,0 +1,538 @@
+"""BEEP model files (TOML + CSV tables) -> model spec for the device path.
+
+Reads the part of the reference's input format that the device path models
+and returns the spec dict of `beep_amd.spec`, the parameter values given in
+the file, and what was recorded without effect.  The rules follow the
+reference's readers:
+
+  time              src/details.cc:50-106 (time_format, start/end, gettime,
+                    steps_per_unit_time defaults to 2)
+  comps / trans     src/inputs_comps.cc:10-140,270-365 (Erlang stages named
+                    E, E1, ...; internal stage transitions first; a transition
+                    leaves the last stage of its source and enters the first
+                    stage of its destination)
+  parameters        src/inputs.cc:1146-1205,1284-1326, src/model.cc:836-935
+                    (value / prior / param strings, '|' lists, a single value
+                    repeated, default names, Fixed(value) when only a value
+                    is given; one parameter per distinct name)
+  splines           src/inputs.cc:1363-1474, 872-910 (bp defaults to
+                    start|end, times relative to start, factor -> multfac)
+  [col:file] values src/inputs.cc:1499-1551
+  initial state     src/data.cc:288-332,419-457 (everyone in the infection
+                    transition's source compartment, rounded to the nearest
+                    integer)
+  data tables       src/inputs.cc:273-475, src/data.cc:1119-1353,1981-2060,
+                    1589-1623, src/model.cc:1381-1425, src/utils.cc:575-583
+                    (rows in the analysis period, transition counts over
+                    consecutive windows of `timestep` days, populations at
+                    the start of a day, observations named 'X->Y' or 'X' by
+                    base compartment name, '.' unknown, '*' below threshold)
+
+Supported: one area without age groups (`areas` with a `population` column),
+`comps` with dist Exp / Erlang, `trans` with branching probabilities,
+`R_spline` and `efoi_spline` (one entry each), `data_tables` of type
+'transition' or 'population' with obsmodel 'normal <sd>', 'normal <P>%',
+'normal load' or 'poisson'.  Presentation keys are ignored; `modification`,
+`prediction_start` and `prediction_end` act only in the reference's
+prediction mode (src/model.cc:1349, src/state.cc:415,446) and are recorded
+in the returned metadata.  Everything else raises TomlError naming the key.
+"""
+import csv
+import datetime
+import math
+import os
+
+import numpy as np
+
+try:
+    import tomllib as _toml
+except ImportError:          # Python < 3.11
+    import tomli as _toml
+
+OBS_UNKNOWN = 999999997.0
+OBS_THRESH = 999999998.0
+
+IGNORED_KEYS = {"description", "outputdir", "state_outputs", "plot_param_values", "time_labels", "area_plot",
+                "output_prop", "mode"}
+PREDICTION_KEYS = {"modification", "prediction_start", "prediction_end"}
+SUPPORTED_KEYS = {"time_format", "start", "end", "datadir", "steps_per_unit_time", "comps", "trans", "R_spline",
+                  "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str"}
+
+
+class TomlError(ValueError):
+    """A model file the device path cannot take (unsupported key or bad value)."""
+
+
+def _s(v):
+    """The reference reads every field as a string (InputNode::stringfield)."""
+    if v is None:
+        return ""
+    if isinstance(v, bool):
+        return "true" if v else "false"
+    if isinstance(v, float) and v.is_integer():
+        return str(int(v))
+    return str(v).strip()
+
+
+def _split(s, d):
+    return [] if s == "" else [x.strip() for x in s.split(d)]
+
+
+class _Reader:
+    def __init__(self, doc, datadir):
+        self.doc = doc
+        self.datadir = datadir
+        fmt = _s(doc.get("time_format", ""))
+        if fmt == "number":
+            self.time_col = "time"
+        elif fmt == "year-month-day":
+            self.time_col = "date"
+        else:
+            raise TomlError(f"time_format '{fmt}' is not supported (number or year-month-day)")
+        self.fmt = fmt
+        self.labels = {_s(t.get("name")): _s(t.get("time")) for t in doc.get("time_labels", [])}
+        if "start" not in doc or "end" not in doc:
+            raise TomlError("'start' and 'end' must be set")
+        self.start = self._abs_time(_s(doc["start"]))
+        self.end = self._abs_time(_s(doc["end"]))
+        self.period = self.end - self.start
+        if self.period <= 0:
+            raise TomlError("'end' must be after 'start'")
+        spd = _s(doc.get("steps_per_unit_time", "2"))      # src/details.cc:97
+        self.spd = int(spd)
+        if self.spd <= 0 or str(self.spd) != spd:
+            raise TomlError(f"steps_per_unit_time '{spd}' must be a positive integer")
+
+    # --- time (Details::gettime, src/details.cc:130-175) ---
+    def _abs_time(self, st):
+        if st in self.labels:
+            st = self.labels[st]
+        if self.fmt == "number":
+            try:
+                return int(st)
+            except ValueError:
+                raise TomlError(f"time '{st}' is not an integer") from None
+        try:
+            return datetime.date.fromisoformat(st).toordinal()
+        except ValueError:
+            raise TomlError(f"time '{st}' is not in year-month-day format") from None
+
+    def time(self, st):
+        """Time relative to start, as 'start', 'end', a label or a value."""
+        if st == "start":
+            return 0
+        if st == "end":
+            return self.period
+        return self._abs_time(st) - self.start
+
+    # --- tables ---
+    def table(self, name):
+        path = os.path.join(self.datadir, name)
+        if not os.path.exists(path):
+            raise TomlError(f"cannot open the file '{path}'")
+        delim = "\t" if name.endswith(".txt") else ","
+        with open(path, newline="") as f:
+            rows = [[c.strip() for c in r] for r in csv.reader(f, delimiter=delim) if r]
+        if not rows:
+            raise TomlError(f"the file '{name}' contains no information")
+        return rows[0], rows[1:]
+
+    def from_table(self, s):
+        """'[column:file.csv]' -> the column's values joined by '|' (src/inputs.cc:1499-1551)."""
+        s = s.strip()
+        if len(s) > 6 and s[0] == "[" and s[-1] == "]":
+            spl = s[1:-1].split(":")
+            if len(spl) != 2 or not (spl[1].endswith(".csv") or spl[1].endswith(".txt")):
+                raise TomlError(f"'{s}' does not use the format '[column:file.csv]'")
+            head, rows = self.table(spl[1])
+            if spl[0] not in head:
+                raise TomlError(f"the heading '{spl[0]}' in file '{spl[1]}' could not be found")
+            c = head.index(spl[0])
+            return "|".join(r[c] for r in rows)
+        return s
+
+    def split_n(self, s, n, what):
+        """Inputs::split_string: n '|' items, one item repeated (src/inputs.cc:1554-1578)."""
+        spl = _split(self.from_table(s), "|")
+        if not spl:
+            return [""] * n
+        if len(spl) == 1 and n > 1:
+            spl = spl * n
+        if len(spl) != n:
+            raise TomlError(f"{what}: '{s}' has {len(spl)} elements instead of {n}")
+        return spl
+
+
+def _prior(st, value, name):
+    """A prior string as the spec's tuple (Model::add_parameter, src/model.cc:839-935)."""
+    if st == "":
+        if value == "":
+            raise TomlError(f"parameter '{name}': a prior or a value must be set")
+        st = f"Fixed({value})"
+    if "(" not in st or not st.endswith(")"):
+        raise TomlError(f"parameter '{name}': prior '{st}' is not recognised")
+    kind, args = st[:-1].split("(", 1)
+    vals = _split(args, ",")
+    kind = kind.strip().lower()
+    try:
+        if kind == "fixed" and len(vals) == 1:
+            return ("fixed", float(vals[0]))
+        if kind == "uniform" and len(vals) == 2:
+            return ("uniform", float(vals[0]), float(vals[1]))
+        if kind == "exp" and len(vals) == 1:
+            return ("exp", float(vals[0]))
+        if kind == "normal" and len(vals) == 2:
+            return ("normal", float(vals[0]), float(vals[1]))
+    except ValueError:
+        pass
+    # Dir / MDir priors tie parameters into Dirichlet groups
+    # (src/model.cc:2900-3090), which this reader does not build.
+    raise TomlError(f"parameter '{name}': prior '{st}' is not supported")
+
+
+class _Params:
+    """Parameters by name, in order of first use (one per distinct name)."""
+
+    def __init__(self):
+        self.specs, self.values, self.index = [], [], {}
+
+    def add(self, name, value, prior):
+        if name == "":
+            raise TomlError("a parameter must have a name")
+        pr = _prior(prior, value, name)
+        try:
+            v = float(value) if value != "" else math.nan
+        except ValueError:
+            raise TomlError(f"parameter '{name}': the value '{value}' is not supported") from None
+        if name in self.index:
+            i = self.index[name]
+            if self.specs[i]["prior"] != pr or not (self.values[i] == v or (math.isnan(v) and math.isnan(self.values[i]))):
+                raise TomlError(f"parameter '{name}' is given two different definitions")
+            return i
+        self.index[name] = len(self.specs)
+        self.specs.append({"name": name, "prior": pr})
+        self.values.append(v)
+        return self.index[name]
+
+
+def _no_dep(entry, keys, where):
+    for k in keys:
+        v = _s(entry.get(k, ""))
+        if ":" in v.split("[")[0]:
+            raise TomlError(f"{where}: the dependency in '{k}={v}' is not supported")
+
+
+def _check_keys(entry, allowed, where):
+    for k in entry:
+        if k not in allowed:
+            raise TomlError(f"{where}: '{k}' is not supported")
+
+
+def _compartments(doc, P):
+    """comps -> expanded compartments (src/inputs_comps.cc:22-63,303-365)."""
+    if "comps" not in doc:
+        raise TomlError("the file must define 'comps'")
+    comps = []          # {"base", "num", "shape", "mean", "inf", "name"}
+    bases = []
+    for e in doc["comps"]:
+        _check_keys(e, {"name", "dist", "k", "mean_param", "mean_value", "mean_prior", "inf_param", "inf_value",
+                        "inf_prior", "plot_name", "line_colour"}, "comps")
+        nam = _s(e.get("name"))
+        if nam == "":
+            raise TomlError("in 'comps' every compartment needs a 'name'")
+        if nam in bases:
+            raise TomlError(f"the compartment '{nam}' cannot be defined multiple times")
+        bases.append(nam)
+        _no_dep(e, ("mean_param", "mean_value", "mean_prior"), f"compartment '{nam}'")
+        dist = _s(e.get("dist", ""))
+        if dist == "":
+            k = None
+        elif dist == "Exp":
+            k = 1
+        elif dist == "Erlang":
+            ks = _s(e.get("k", ""))
+            if not ks.isdigit() or int(ks) < 1:
+                raise TomlError(f"compartment '{nam}': Erlang needs an integer shape 'k'")
+            k = int(ks)
+        else:
+            raise TomlError(f"compartment '{nam}': dist '{dist}' is not recognised")
+        mean = None
+        if any(x in e for x in ("mean_param", "mean_value", "mean_prior")):
+            pname = _s(e.get("mean_param", "")) or f"{nam} mean occupancy"
+            mean = P.add(pname, _s(e.get("mean_value", "")), _s(e.get("mean_prior", "")))
+        inf = -1
+        if any(x in e for x in ("inf_param", "inf_value", "inf_prior")):
+            pname = _s(e.get("inf_param", "")) or f"{nam} Infectivity"
+            inf = P.add(pname, _s(e.get("inf_value", "")), _s(e.get("inf_prior", "")))
+        stages = 1 if k is None else k
+        for num in range(stages):
+            comps.append({"base": nam, "num": num, "shape": k, "mean": mean, "inf": inf,
+                          "name": nam + (str(num) if num > 0 else "")})
+    return comps
+
+
+def _transitions(doc, comps, P):
+    """trans -> internal Erlang transitions first, then the file's (src/inputs_comps.cc:66-139,268-300)."""
+    if "trans" not in doc:
+        raise TomlError("the file must define 'trans'")
+    trans = []
+    for c, co in enumerate(comps):
+        if co["shape"] is not None and co["num"] < co["shape"] - 1:
+            trans.append({"from": c, "to": c + 1, "infection": False, "prob": None})
+
+    def find(name, last):
+        for c, co in enumerate(comps):
+            if co["base"] == name and (co["shape"] is None or co["num"] == ((co["shape"] - 1) if last else 0)):
+                return c
+        return None
+
+    ninf = 0
+    for e in doc["trans"]:
+        _check_keys(e, {"from", "to", "infection", "prob_param", "prob_value", "prob_prior"}, "trans")
+        fr, to = _s(e.get("from")), _s(e.get("to"))
+        if fr == to:
+            raise TomlError("in 'trans' the 'from' and 'to' values cannot be the same")
+        cf, ct = find(fr, True), find(to, False)
+        if cf is None:
+            raise TomlError(f"in 'trans' cannot find the 'from' compartment '{fr}'")
+        if ct is None:
+            raise TomlError(f"in 'trans' cannot find the 'to' compartment '{to}'")
+        inf = _s(e.get("infection", ""))
+        if inf not in ("", "yes", "no"):
+            raise TomlError("in 'trans' the value of 'infection' can be 'yes', 'no' or omitted")
+        _no_dep(e, ("prob_param", "prob_value", "prob_prior"), f"transition {fr}->{to}")
+        prob = None
+        if any(x in e for x in ("prob_param", "prob_value", "prob_prior")):
+            pname = _s(e.get("prob_param", "")) or f"{fr}->{to} branch prob."
+            prob = P.add(pname, _s(e.get("prob_value", "")), _s(e.get("prob_prior", "")))
+        ninf += inf == "yes"
+        trans.append({"from": cf, "to": ct, "infection": inf == "yes", "prob": prob, "base": f"{fr}->{to}"})
+    if ninf != 1:
+        raise TomlError("exactly one infection transition must exist")
+    return trans
+
+
+def _spline(rd, P, key, entry):
+    """One spline entry (src/inputs.cc:1363-1474): knots, parameters, multfac."""
+    _check_keys(entry, {"name", "bp", "param", "value", "prior", "factor", "factor_param", "factor_value",
+                        "factor_prior"}, key)
+    bp = [rd.time(t) for t in _split(rd.from_table(_s(entry.get("bp", "")) or "start|end"), "|")]
+    if not bp or bp[0] != 0:
+        raise TomlError(f"in '{key}' 'bp' must start at 'start'")
+    if bp[-1] != rd.period:
+        raise TomlError(f"in '{key}' 'bp' must end at 'end'")
+    if any(b > a for a, b in zip(bp[1:], bp[:-1])):
+        raise TomlError(f"in '{key}' times in 'bp' are not in the right order")
+    n = len(bp)
+    names = rd.split_n(_s(entry.get("param", "")), n, key)
+    vals = rd.split_n(_s(entry.get("value", "")), n, key)
+    pris = rd.split_n(_s(entry.get("prior", "")), n, key)
+    fac = rd.split_n(_s(entry.get("factor", "")) or "1", n, key)
+    params = []
+    for i in range(n):
+        nm = names[i]
+        if nm == "":
+            nm = key.replace("_", "-")
+            if n > 2:
+                nm += f" t={bp[i]}"
+        params.append(P.add(nm, vals[i], pris[i]))
+    factor = -1
+    if any(x in entry for x in ("factor_param", "factor_value", "factor_prior")):
+        factor = P.add(_s(entry.get("factor_param", "")) or f"{key} factor", _s(entry.get("factor_value", "")),
+                       _s(entry.get("factor_prior", "")))
+    return {"t": [float(b) for b in bp], "param": params, "multfac": [float(f) for f in fac], "factor": factor}
+
+
+def _observations(rd, doc, comps, trans, with_data):
+    """data_tables -> spec observations (single area, one age group)."""
+    thr_str = _s(doc.get("threshold_str", "*"))
+    nodata_str = _s(doc.get("nodata_str", "."))
+    obs = []
+    tables = doc.get("data_tables", [])
+    for d, e in enumerate(tables):
+        _check_keys(e, {"type", "observation", "timestep", "obsmodel", "file", "factor", "shift", "start", "end",
+                        "threshold", "epsilon_factor", "plot_name", "line_colour"}, "data_tables")
+        typ = _s(e.get("type"))
+        if typ not in ("transition", "population"):
+            raise TomlError(f"data table type '{typ}' is not supported (transition or population)")
+        file = _s(e.get("file"))
+        obsname = _s(e.get("observation"))
+        tr_list, co_list = [], []
+        for st in _split(obsname, ","):
+            cs = [c for c, co in enumerate(comps) if co["base"] == st]
+            if cs:
+                if typ != "population":
+                    raise TomlError(f"observation '{obsname}' names compartments in a transition table")
+                co_list += cs
+                continue
+            ts = [t for t, tr in enumerate(trans) if comps[tr["from"]]["base"] + "->" + comps[tr["to"]]["base"] == st]
+            if not ts:
+                raise TomlError(f"the observation '{st}' is not a compartment or a transition")
+            if typ != "transition":
+                raise TomlError(f"observation '{obsname}' names transitions in a population table")
+            tr_list += ts
+        # Observation model grammar (src/inputs.cc:412-474).
+        om = _s(e.get("obsmodel", ""))
+        spl = om.split(" ")
+        sd = percent = None
+        load_sd = False
+        if om == "":
+            raise TomlError("in 'data_tables' a value for 'obsmodel' must be set")
+        if len(spl) > 2:
+            raise TomlError(f"obsmodel '{om}' is invalid")
+        if spl[0].lower() == "normal":
+            otype = "normal"
+            if len(spl) == 2:
+                v = spl[1].lower()
+                if v == "load":
+                    load_sd = True
+                elif v.endswith("%"):
+                    otype, percent = "normal_percent", float(v[:-1])
+                else:
+                    sd = float(v)
+            if sd is None and percent is None and not load_sd:
+                raise TomlError(f"obsmodel '{om}' needs a standard deviation")
+        elif om == "poisson":
+            otype = "poisson"
+        elif om == "negbin":
+            # The reference asks for a shape here but compares the whole
+            # string with 'negbin', so no negative-binomial table loads.
+            raise TomlError("in 'data_tables' the value 'negbin' must contain a definition for the shape parameter")
+        else:
+            raise TomlError(f"obsmodel '{om}' is not recognised")
+        eps = float(_s(e.get("epsilon_factor", "")) or 0.01)
+        if eps != 0.01:
+            raise TomlError("'epsilon_factor' other than 0.01 is not supported")
+        factor = float(_s(e.get("factor", "")) or 1.0)
+        shift = int(_s(e.get("shift", "")) or 0)
+        thresh = _s(e.get("threshold", ""))
+        if not with_data:
+            continue
+        head, rows = rd.table(file)
+        if rd.time_col not in head or "Data" not in head:
+            raise TomlError(f"the file '{file}' needs columns '{rd.time_col}' and 'Data'")
+        tc, dc = head.index(rd.time_col), head.index("Data")
+        sc = head.index("Data SD") if load_sd and "Data SD" in head else None
+        if load_sd and sc is None:
+            raise TomlError(f"the file '{file}' needs a column 'Data SD' for 'normal load'")
+        # Rows in the analysis period (Data::table_select_dates, src/data.cc:1981-2060).
+        times = [rd.time(r[tc]) + shift for r in rows]
+        start = rd.time(_s(e["start"])) if "start" in e else None
+        end = rd.time(_s(e["end"])) if "end" in e else None
+        r0 = 0
+        while r0 < len(rows) and (times[r0] < 0 or (start is not None and times[r0] < start)):
+            r0 += 1
+        r1 = len(rows) - 1
+        while r1 >= 0 and (times[r1] >= rd.period or (end is not None and times[r1] >= end)):
+            r1 -= 1
+        if r0 > r1:
+            raise TomlError(f"the file '{file}' has no data in the analysis period")
+        timestep = _s(e.get("timestep", ""))
+        ts = int(timestep) if timestep else None
+        if typ == "transition":
+            if r0 == r1 and ts is None:
+                raise TomlError(f"the file '{file}' has one row: 'timestep' must be set")
+            if r1 > r0:
+                steps = {times[r + 1] - times[r] for r in range(r0, r1)}
+                if len(steps) != 1 or min(steps) <= 0:
+                    raise TomlError(f"in the file '{file}' measurements must be time ordered with an equal timestep")
+                tstep = steps.pop()
+                if ts is not None and ts != tstep:
+                    raise TomlError(f"'timestep' does not agree with the file '{file}'")
+                ts = tstep
+            if times[r1] + ts > rd.period:
+                r1 -= 1
+        graph = d
+        ti = times[r0] * rd.spd if typ == "transition" else None
+        for r in range(r0, r1 + 1):
+            raw = rows[r][dc]
+            if raw == thr_str:
+                if thresh == "":
+                    raise TomlError(f"the file '{file}' has thresholded data but no 'threshold' is set")
+                y = OBS_THRESH
+            elif raw == nodata_str:
+                y = OBS_UNKNOWN
+            else:
+                y = float(raw)
+            o = {"y": y, "type": otype, "factor": factor, "datatable": d, "graph": graph, "area": [0], "dp": [0],
+                 "thresh": float(thresh) if thresh else 0.0}
+            if sd is not None:
+                o["sd"] = sd
+            if percent is not None:
+                o["percent"] = percent
+            if load_sd:
+                o["sd"] = float(rows[r][sc])
+            if typ == "transition":
+                tf = (times[r] + ts) * rd.spd
+                o.update(sett_i=ti, sett_f=tf, tr=list(tr_list))
+                ti = tf
+            else:
+                t0 = times[r] * rd.spd
+                o.update(sett_i=t0, sett_f=t0 + 1, co=list(co_list))
+            obs.append(o)
+    return obs
+
+
+def load(path, datadir=None, with_data=True):
+    """Reads a BEEP model file.  Returns (spec, values, meta): the spec dict,
+    the parameter values the file gives (NaN where only a prior is given), and
+    {"ignored": [...], "prediction": {...}} for keys recorded without effect.
+    datadir overrides the file's 'datadir' (which the reference reads relative
+    to the working directory); with_data=False skips the data tables' files,
+    as the reference's simulation mode does."""
+    with open(path, "rb") as f:
+        doc = _toml.load(f)
+    meta = {"ignored": [], "prediction": {}}
+    for k in doc:
+        if k in IGNORED_KEYS:
+            meta["ignored"].append(k)
+        elif k in PREDICTION_KEYS:
+            meta["prediction"][k] = doc[k]
+        elif k not in SUPPORTED_KEYS:
+            raise TomlError(f"the key '{k}' is not supported by the device path")
+    if datadir is None:
+        datadir = _s(doc.get("datadir", ""))
+        if datadir == "":
+            raise TomlError("'datadir' is not set")
+    rd = _Reader(doc, datadir)
+    P = _Params()
+    comps = _compartments(doc, P)
+    trans = _transitions(doc, comps, P)
+
+    splines = []
+    if len(doc.get("R_spline", [])) != 1:
+        raise TomlError("exactly one 'R_spline' entry is supported")
+    splines.append(_spline(rd, P, "R_spline", doc["R_spline"][0]))
+    efoi = -1
+    if "efoi_spline" in doc:
+        if len(doc["efoi_spline"]) != 1:
+            raise TomlError("one 'efoi_spline' entry is supported")
+        splines.append(_spline(rd, P, "efoi_spline", doc["efoi_spline"][0]))
+        efoi = 1
+
+    # Areas and the initial population (src/data.cc:288-332,419-457).
+    if "areas" not in doc:
+        raise TomlError("'areas' must be set")
+    head, rows = rd.table(_s(doc["areas"]))
+    if "area" not in head or "population" not in head:
+        raise TomlError("the areas file needs columns 'area' and 'population'")
+    if len(rows) != 1:
+        raise TomlError("one area is supported (a geographic mixing matrix is needed for more)")
+    sus = trans[[t["infection"] for t in trans].index(True)]["from"]
+    popv = float(rows[0][head.index("population")])
+    if popv < 0:
+        raise TomlError("populations must be non-negative")
+    pop = [[[int(popv + 0.5) if c == sus else 0] for c in range(len(comps))]]
+
+    spec = {
+        "name": os.path.splitext(os.path.basename(path))[0], "A": 1, "G": 1, "period": rd.period, "spd": rd.spd,
+        "comps": [{"name": c["name"], "inf": c["inf"], "mean": [c["mean"]] if c["mean"] is not None else None,
+                   "shape": c["shape"] or 1} for c in comps],
+        "trans": [{"from": t["from"], "to": t["to"], "infection": t["infection"],
+                   **({"prob": [t["prob"]]} if t["prob"] is not None else {})} for t in trans],
+        "params": P.specs,
+        "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": [1.0],
+        "geo_spline": -1, "N": None, "M": None, "pop": pop,
+        "obs": _observations(rd, doc, comps, trans, with_data),
+    }
+    return spec, np.array(P.values), meta

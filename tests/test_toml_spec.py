@@ -1,0 +1,193 @@
+"""The TOML front end (beep_amd/toml_spec.py) against expectations worked by
+hand from the reference's input rules: Erlang stage names and order
+(src/inputs_comps.cc:22-139), default names and one parameter per name
+(src/inputs.cc:1146-1326), spline knots (src/inputs.cc:1363-1474,872-910),
+initial state (src/data.cc:419-457), observation windows and models
+(src/inputs.cc:273-475, src/data.cc:1119-1353,1981-2060)."""
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from beep_amd import spec as specmod
+from beep_amd import toml_spec as T
+
+HERE = os.path.dirname(__file__)
+FIX = os.path.join(HERE, "golden", "toml")
+REF_EXAMPLES = "/root/reference/examples"
+
+
+def _fixture():
+    return T.load(os.path.join(FIX, "seir_erlang.toml"), datadir=os.path.join(FIX, "data_seir_erlang"))
+
+
+def test_fixture_structure():
+    spec, values, meta = _fixture()
+    # Erlang(2) E becomes stages E, E1; the internal E->E1 transition comes
+    # first; E->I leaves the last stage, S->E enters the first.
+    assert [c["name"] for c in spec["comps"]] == ["S", "E", "E1", "I", "R"]
+    assert [(t["from"], t["to"], t["infection"]) for t in spec["trans"]] == [
+        (1, 2, False), (0, 1, True), (2, 3, False), (3, 4, False)]
+    assert [c["shape"] for c in spec["comps"]] == [1, 2, 2, 1, 1]
+    names = [p["name"] for p in spec["params"]]
+    assert names == ["E mean occupancy", "I mean occupancy", "I Infectivity", "R-spline", "efoi-spline"]
+    assert spec["comps"][1]["mean"] == spec["comps"][2]["mean"] == [0]
+    assert spec["comps"][3]["inf"] == 2 and spec["comps"][0]["inf"] == -1
+    assert spec["params"][0]["prior"] == ("uniform", 1.0, 10.0)
+    assert spec["params"][2]["prior"] == ("fixed", 1.0)          # value only -> Fixed(value)
+    assert np.array_equal(values, [3.0, 4.0, 1.0, 2.2, 2.0])
+    # One knot value for the default bp 'start|end': both knots share one parameter.
+    assert spec["splines"][0] == {"t": [0.0, 70.0], "param": [3, 3], "multfac": [1.0, 1.0], "factor": -1}
+    assert spec["efoi_spline"] == 1 and spec["R_spline"] == 0
+    # Everyone starts in the infection transition's source compartment.
+    assert spec["pop"] == [[[20000], [0], [0], [0], [0]]]
+    assert spec["spd"] == 2 and spec["period"] == 70 and specmod.nsteps(spec) == 140
+    assert sorted(meta["ignored"]) == ["description", "outputdir", "state_outputs"]
+    assert meta["prediction"]["modification"][0]["type"] == "beta_fac"
+
+
+def test_fixture_observations():
+    spec, _, _ = _fixture()
+    obs = spec["obs"]
+    ei = [o for o in obs if o["datatable"] == 0]
+    r = [o for o in obs if o["datatable"] == 1]
+    # Weekly E->I counts: consecutive 7-day windows of 2 steps per day; the
+    # transition is E1->I (index 2).
+    assert [(o["sett_i"], o["sett_f"]) for o in ei] == [(14 * k, 14 * k + 14) for k in range(10)]
+    assert all(o["tr"] == [2] and o["type"] == "normal_percent" and o["percent"] == 10.0 for o in ei)
+    assert [o["y"] for o in ei] == [3, 9, 31, 88, 230, 512, 890, 1104, 921, 602]
+    # Populations at the start of each listed day; the day-70 row lies past
+    # the end of the analysis and is dropped.
+    assert [(o["sett_i"], o["sett_f"]) for o in r] == [(14 * k, 14 * k + 1) for k in range(10)]
+    assert all(o["co"] == [4] and o["type"] == "normal" and o["sd"] == 20.0 for o in r)
+    assert {o["graph"] for o in ei} == {0} and {o["graph"] for o in r} == {1}
+
+
+def _write(d, name, text):
+    p = os.path.join(d, name)
+    with open(p, "w") as f:
+        f.write(text)
+    return p
+
+
+BASE = """
+time_format = "year-month-day"
+start = "2020-03-01"
+end = "2020-03-29"
+steps_per_unit_time = "1"
+datadir = "unused"
+comps = [{name="S"}, {name="I", dist="Exp", mean_value="5", mean_prior="Uniform(2,8)", inf_value="1"}, {name="H", mean_value="10"}, {name="R"}]
+trans = [{from="S", to="I", infection="yes"}, {from="I", to="H", prob_value="0.2", prob_prior="Uniform(0,0.5)"},
+         {from="I", to="R", prob_value="0.8", prob_prior="Uniform(0.5,1)"}, {from="H", to="R"}]
+R_spline = [{bp="[bp:R_spline.csv]", value="[value:R_spline.csv]", prior="[prior:R_spline.csv]"}]
+areas = "areas.csv"
+"""
+
+
+def _model(tmp_path, extra):
+    d = str(tmp_path)
+    _write(d, "areas.csv", "area,population\nX,1000.5\n")
+    _write(d, "R_spline.csv", 'bp,value,prior\nstart,2.5,"Uniform(1,4)"\n2020-03-15,1.0,"Uniform(0.2,2)"\n'
+                              '2020-03-29,1.2,"Uniform(0.2,2)"\n')
+    _write(d, "I-H.csv", "date,Data,Data SD\n2020-03-01,1,2\n2020-03-08,.,2\n2020-03-15,7,3\n2020-03-22,*,4\n"
+                         "2020-03-29,9,5\n")
+    _write(d, "H.csv", "date,Data\n2020-02-28,4\n2020-03-02,5\n2020-03-28,6\n")
+    p = _write(d, "m.toml", BASE + extra)
+    return T.load(p, datadir=d)
+
+
+def test_dates_tables_and_obsmodels(tmp_path):
+    spec, values, _ = _model(tmp_path, """
+data_tables = [{type="transition", observation="I->H", obsmodel="normal load", file="I-H.csv", threshold="3"},
+               {type="population", observation="H", obsmodel="poisson", file="H.csv"}]
+""")
+    # Knots from the table column, times in days from start; more than two
+    # knots: parameter names carry the knot time.
+    sp = spec["splines"][0]
+    assert sp["t"] == [0.0, 14.0, 28.0]
+    names = [p["name"] for p in spec["params"]]
+    assert [names[i] for i in sp["param"]] == ["R-spline t=0", "R-spline t=14", "R-spline t=28"]
+    assert [spec["params"][i]["prior"] for i in sp["param"]] == [("uniform", 1.0, 4.0), ("uniform", 0.2, 2.0),
+                                                                  ("uniform", 0.2, 2.0)]
+    assert [values[i] for i in sp["param"]] == [2.5, 1.0, 1.2]
+    assert spec["efoi_spline"] == -1
+    # H has a mean but no dist: one stage.  Branch probabilities by default name.
+    assert [c["shape"] for c in spec["comps"]] == [1, 1, 1, 1]
+    assert "I->H branch prob." in names and "I->R branch prob." in names
+    assert spec["pop"][0][0] == [1001]                               # (unsigned)(1000.5 + 0.5)
+    ih = [o for o in spec["obs"] if o["datatable"] == 0]
+    # Rows at days 0, 7, 14, 21, 28: day 28 is the end of the analysis, so
+    # four 7-day windows; '.' is unknown, '*' below the threshold; 'normal
+    # load' takes the SD column.
+    assert [(o["sett_i"], o["sett_f"]) for o in ih] == [(0, 7), (7, 14), (14, 21), (21, 28)]
+    assert [o["y"] for o in ih] == [1.0, T.OBS_UNKNOWN, 7.0, T.OBS_THRESH]
+    assert [o["sd"] for o in ih] == [2.0, 2.0, 3.0, 4.0] and ih[3]["thresh"] == 3.0
+    h = [o for o in spec["obs"] if o["datatable"] == 1]
+    # The row before start is dropped; populations on days 1 and 27.
+    assert [(o["sett_i"], o["y"], o["type"]) for o in h] == [(1, 5.0, "poisson"), (27, 6.0, "poisson")]
+
+
+@pytest.mark.parametrize("extra,msg", [
+    ('ages = [{cats="0-19|20+"}]\n', "'ages'"),
+    ('geo_mixing_matrix = "g.csv"\n', "'geo_mixing_matrix'"),
+    ('data_tables = [{type="transition", observation="I->H", obsmodel="negbin 2", file="I-H.csv"}]\n',
+     "not recognised"),
+    ('data_tables = [{type="transition", observation="I->X", obsmodel="poisson", file="I-H.csv"}]\n',
+     "not a compartment or a transition"),
+    ('data_tables = [{type="transition", observation="I->H", obsmodel="poisson", file="I-H.csv", geo_dep="r"}]\n',
+     "'geo_dep'"),
+])
+def test_rejections(tmp_path, extra, msg):
+    with pytest.raises(T.TomlError, match=msg):
+        _model(tmp_path, extra)
+
+
+def test_rejects_dirichlet_branches_and_smoothing(tmp_path):
+    d = str(tmp_path)
+    _write(d, "areas.csv", "area,population\nX,100\n")
+    base = BASE.replace('R_spline = [{bp="[bp:R_spline.csv]", value="[value:R_spline.csv]", '
+                        'prior="[prior:R_spline.csv]"}]', 'R_spline = [{value="2", prior="Uniform(1,3)"}]')
+    p = _write(d, "a.toml", base.replace('prob_value="0.8", prob_prior="Uniform(0.5,1)"',
+                                         'prob_value="*", prob_prior="Dir(1)"'))
+    with pytest.raises(T.TomlError, match="Dir"):
+        T.load(p, datadir=d)
+    p = _write(d, "b.toml", base.replace('R_spline = [{value="2", prior="Uniform(1,3)"}]',
+                                         'R_spline = [{value="2", prior="Uniform(1,3)", smooth="0.5"}]'))
+    with pytest.raises(T.TomlError, match="'smooth'"):
+        T.load(p, datadir=d)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="the reference's examples are not present")
+def test_reference_examples():
+    """Every example either loads or is refused with TomlError; EX.toml (SEIR
+    with Erlang E, weekly E->I counts and R populations) matches its files."""
+    loaded = []
+    for path in sorted(glob.glob(os.path.join(REF_EXAMPLES, "*.toml"))):
+        import tomli
+        with open(path, "rb") as f:
+            dd = tomli.load(f)
+        datadir = os.path.join(os.path.dirname(REF_EXAMPLES), dd.get("datadir", ""))
+        try:
+            spec, values, _ = T.load(path, datadir=datadir)
+        except T.TomlError:
+            continue
+        loaded.append(os.path.basename(path))
+        assert spec["obs"] and len(values) == len(spec["params"])
+    assert "EX.toml" in loaded
+    spec, values, _ = T.load(os.path.join(REF_EXAMPLES, "EX.toml"), datadir=os.path.join(REF_EXAMPLES, "Data_EX"))
+    assert [c["name"] for c in spec["comps"]] == ["S", "E", "E1", "I", "R"]
+    assert spec["period"] == 140 and spec["spd"] == 2
+    rows = {}
+    for f in ("E-I.csv", "R.csv", "population.csv"):
+        with open(os.path.join(REF_EXAMPLES, "Data_EX", f)) as fh:
+            rows[f] = [line.strip().split(",") for line in fh.read().strip().splitlines()[1:]]
+    ei_rows = [r for r in rows["E-I.csv"] if 0 <= int(r[0]) and int(r[0]) + 7 <= 140]
+    r_rows = [r for r in rows["R.csv"] if 0 <= int(r[0]) < 140]
+    ei = [o for o in spec["obs"] if o["datatable"] == 0]
+    r = [o for o in spec["obs"] if o["datatable"] == 1]
+    assert [o["y"] for o in ei] == [float(x[1]) for x in ei_rows]
+    assert [o["y"] for o in r] == [float(x[1]) for x in r_rows]
+    assert spec["pop"][0][0] == [int(float(rows["population.csv"][0][1]) + 0.5)]
+    assert not math.isnan(values[[p["name"] for p in spec["params"]].index("R-spline")])

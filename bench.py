@@ -30,7 +30,7 @@ METRIC = "particle-timesteps/sec batched vs plain GPU path; p50/p99 request late
 # by the ratio of known bytes to counter reading on a plain copy kernel:
 # the specialised kernel (profiles/beep_fused_spec_sim_spatial_abcsmc.md) and
 # the generic one (profiles/k_fused_spatial_abcsmc_traffic.md).
-SPEC_TRAFFIC_BYTES = 35_530_804_861
+SPEC_TRAFFIC_BYTES = 35_483_868_282
 GENERIC_TRAFFIC_BYTES = 41_895_375_565
 
 
@@ -104,8 +104,8 @@ def main():
     ap.add_argument("--instances", type=int, default=16384)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path shards by particle, "
-              f"see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path shards by particle: "
+              f"one GPU's share is what is measured here)", file=sys.stderr)
         sys.exit(2)
 
     from beep_amd import _lib, configs, spec as specmod

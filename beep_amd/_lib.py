@@ -53,7 +53,8 @@ class Trans(C.Structure):
 
 class Spline(C.Structure):
     _fields_ = [("nknot", C.c_int32), ("t", C.POINTER(C.c_double)), ("param", C.POINTER(C.c_int32)),
-                ("multfac", C.POINTER(C.c_double)), ("factor_param", C.c_int32)]
+                ("multfac", C.POINTER(C.c_double)), ("factor_param", C.c_int32), ("smooth_type", C.c_int32),
+                ("smooth", C.POINTER(C.c_double))]
 
 
 class Dirichlet(C.Structure):
@@ -221,6 +222,8 @@ class _SpecBuffers:
             ss[i].param = i32(sp["param"])
             ss[i].multfac = f64(sp.get("multfac", [1.0] * len(sp["t"])))
             ss[i].factor_param = sp.get("factor", -1)
+            ss[i].smooth_type = sp.get("smooth_type", 0)
+            ss[i].smooth = f64(sp["smooth"]) if sp.get("smooth_type", 0) else null_f64
         keep.append(ss)
         s.nspline, s.spline = len(spl), ss
         s.R_spline = spec["R_spline"]

@@ -125,6 +125,12 @@ std::string HostModel::build(const beep_spec& s) {
     spline[k].param.assign(x.param, x.param + x.nknot);
     spline[k].multfac.assign(x.multfac, x.multfac + x.nknot);
     spline[k].factor_param = x.factor_param;
+    spline[k].smooth_type = x.smooth_type;
+    if (x.smooth_type != BEEP_SMOOTH_NONE) {
+      if (x.smooth_type != BEEP_SMOOTH_NORMAL && x.smooth_type != BEEP_SMOOTH_LOG) return "unknown spline smoothing type";
+      if (!x.smooth) return "a smoothed spline needs its smoothing values";
+      spline[k].smooth.assign(x.smooth, x.smooth + x.nknot);
+    }
   }
   R_spline = s.R_spline; Rfactor_param = s.Rfactor_param;
   efoi_spline = s.efoi_spline; geo_spline = s.geo_spline;
@@ -461,6 +467,25 @@ double HostModel::prior_logpdf(const double* pv) const {   // Model::prior (src/
         break;
       case BEEP_PRIOR_DIRFLAT: Pr += -pv[th]; break;
       default: break;
+    }
+  }
+  // Model::spline_prior (src/model.cc:1662-1704), on the Dirichlet-corrected values.
+  std::vector<double> pd;
+  for (const Spline& spl : spline) {
+    if (spl.smooth_type == BEEP_SMOOTH_NONE) continue;
+    if (pd.empty()) pd = dirichlet_correct(pv);
+    for (size_t i = 0; i + 1 < spl.param.size(); i++) {
+      int p1 = spl.param[i], p2 = spl.param[i + 1];
+      if (p1 == p2 || spl.t[i] == spl.t[i + 1]) continue;
+      if (prior[p1].type == BEEP_PRIOR_FIXED && prior[p2].type == BEEP_PRIOR_FIXED) continue;
+      const double v = spl.smooth[i];
+      if (spl.smooth_type == BEEP_SMOOTH_LOG) {
+        const double mu = std::log(pd[p1] * spl.multfac[i]), logx = std::log(pd[p2] * spl.multfac[i + 1]);
+        Pr += -(logx - mu) * (logx - mu) / (2 * v * v) - logx;
+      } else {
+        const double d = pd[p1] - pd[p2];
+        Pr += -d * d / (2 * v * v);
+      }
     }
   }
   return Pr;

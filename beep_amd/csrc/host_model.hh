@@ -13,7 +13,7 @@ namespace beephost {
 
 struct Compartment { int inf_param; std::vector<int> mean_param; int shape; std::vector<int> exits; };
 struct Transition { int from, to, infection; std::vector<int> prob_param; };
-struct Spline { std::vector<double> t, multfac; std::vector<int> param; int factor_param; };
+struct Spline { std::vector<double> t, multfac, smooth; std::vector<int> param; int factor_param, smooth_type; };
 struct Dirichlet { std::vector<int> param; std::vector<double> frac; };
 struct Covar { int param; std::vector<double> value; /* [A][period], centred */ };
 

@@ -204,6 +204,9 @@ class _Params:
             v = float(value) if value != "" else math.nan
         except ValueError:
             raise TomlError(f"parameter '{name}': the value '{value}' is not supported") from None
+        return self.add_tuple(name, v, pr)
+
+    def add_tuple(self, name, v, pr):
         if name in self.index:
             i = self.index[name]
             if self.specs[i]["prior"] != pr or not (self.values[i] == v or (math.isnan(v) and math.isnan(self.values[i]))):
@@ -301,21 +304,82 @@ def _transitions(doc, comps, P):
         if inf not in ("", "yes", "no"):
             raise TomlError("in 'trans' the value of 'infection' can be 'yes', 'no' or omitted")
         _no_dep(e, ("prob_param", "prob_value", "prob_prior"), f"transition {fr}->{to}")
-        prob = None
+        branch = None
         if any(x in e for x in ("prob_param", "prob_value", "prob_prior")):
-            pname = _s(e.get("prob_param", "")) or f"{fr}->{to} branch prob."
-            prob = P.add(pname, _s(e.get("prob_value", "")), _s(e.get("prob_prior", "")))
+            branch = (_s(e.get("prob_param", "")) or f"{fr}->{to} branch prob.", _s(e.get("prob_value", "")),
+                      _s(e.get("prob_prior", "")))
         ninf += inf == "yes"
-        trans.append({"from": cf, "to": ct, "infection": inf == "yes", "prob": prob, "base": f"{fr}->{to}"})
+        trans.append({"from": cf, "to": ct, "infection": inf == "yes", "prob": None, "base": f"{fr}->{to}",
+                      "branch": branch})
     if ninf != 1:
         raise TomlError("exactly one infection transition must exist")
-    return trans
+    return trans, _branches(trans, comps, P)
+
+
+def _is_flat_dir(prior):
+    """Dir(*) and Dir(1): both give each branch parameter the gamma(1,1) log
+    density -x (Model::prior, src/model.cc:1626-1647)."""
+    t = prior.replace(" ", "").lower()
+    if not (t.startswith("dir(") and t.endswith(")")):
+        return False
+    a = t[4:-1]
+    if a == "*":
+        return True
+    try:
+        return float(a) == 1.0
+    except ValueError:
+        return False
+
+
+def _branches(trans, comps, P):
+    """Branch probabilities of the transitions leaving one compartment: one
+    value may be '*', one minus the others (src/model.cc:2894-2915); with
+    Dir(*) / Dir(1) priors they form a Dirichlet group that the model
+    normalises (Model::dirichlet_correct, src/model.cc:2685-2695)."""
+    groups = []
+    by_src = {}
+    for t in trans:
+        if t.get("branch") is not None:
+            by_src.setdefault(t["from"], []).append(t)
+    for src, ts in by_src.items():
+        priors = [t["branch"][2] for t in ts]
+        dirs = [p.replace(" ", "").lower().startswith(("dir(", "mdir(")) for p in priors]
+        if any(dirs) and not all(_is_flat_dir(p) for p in priors):
+            raise TomlError(f"branch probabilities out of '{comps[src]['base']}': only Dir(*) / Dir(1) priors on "
+                            "every branch are supported")
+        vals = [t["branch"][1] for t in ts]
+        star = [k for k, v in enumerate(vals) if v == "*"]
+        if len(star) > 1:
+            raise TomlError(f"branch probabilities out of '{comps[src]['base']}': at most one value can be '*'")
+        num = []
+        for k, v in enumerate(vals):
+            if k in star or v == "":
+                num.append(math.nan)
+            else:
+                try:
+                    num.append(float(v))
+                except ValueError:
+                    raise TomlError(f"branch probability value '{v}' is not a number") from None
+        if star:
+            rest = sum(x for k, x in enumerate(num) if k != star[0])
+            if math.isnan(rest) or rest > 1:
+                raise TomlError(f"branch probabilities out of '{comps[src]['base']}' cannot fill the value '*'")
+            num[star[0]] = 1.0 - rest
+        if all(dirs):
+            idx = [P.add_tuple(t["branch"][0], num[k], ("dirflat",)) for k, t in enumerate(ts)]
+            groups.append({"param": idx})
+        else:
+            idx = [P.add(t["branch"][0], "" if math.isnan(num[k]) else repr(num[k]), t["branch"][2])
+                   for k, t in enumerate(ts)]
+        for k, t in enumerate(ts):
+            t["prob"] = idx[k]
+    return groups
 
 
 def _spline(rd, P, key, entry):
     """One spline entry (src/inputs.cc:1363-1474): knots, parameters, multfac."""
     _check_keys(entry, {"name", "bp", "param", "value", "prior", "factor", "factor_param", "factor_value",
-                        "factor_prior"}, key)
+                        "factor_prior", "smooth", "smooth_type"}, key)
     bp = [rd.time(t) for t in _split(rd.from_table(_s(entry.get("bp", "")) or "start|end"), "|")]
     if not bp or bp[0] != 0:
         raise TomlError(f"in '{key}' 'bp' must start at 'start'")
@@ -340,7 +404,26 @@ def _spline(rd, P, key, entry):
     if any(x in entry for x in ("factor_param", "factor_value", "factor_prior")):
         factor = P.add(_s(entry.get("factor_param", "")) or f"{key} factor", _s(entry.get("factor_value", "")),
                        _s(entry.get("factor_prior", "")))
-    return {"t": [float(b) for b in bp], "param": params, "multfac": [float(f) for f in fac], "factor": factor}
+    out = {"t": [float(b) for b in bp], "param": params, "multfac": [float(f) for f in fac], "factor": factor}
+    # Smoothing prior between neighbouring knots (Inputs::find_smoothtype,
+    # src/inputs.cc:1343-1359; Model::spline_prior, src/model.cc:1662-1704).
+    st = _s(entry.get("smooth_type", ""))
+    kinds = {"": 0, "no_smooth": 0, "smooth": 1, "log_smooth": 2}
+    if st not in kinds:
+        raise TomlError(f"in '{key}' the value of 'smooth_type={st}' is not recognised")
+    if kinds[st]:
+        sm = rd.split_n(_s(entry.get("smooth", "")), n, key)
+        try:
+            smv = [float(x) if x != "" else math.nan for x in sm]
+        except ValueError:
+            raise TomlError(f"in '{key}' the smoothing values '{entry.get('smooth')}' are not numbers") from None
+        for i in range(n - 1):
+            pa, pb = P.specs[params[i]]["prior"], P.specs[params[i + 1]]["prior"]
+            if params[i] != params[i + 1] and bp[i] != bp[i + 1] and not (pa[0] == "fixed" and pb[0] == "fixed") \
+                    and math.isnan(smv[i]):
+                raise TomlError(f"for the spline '{key}' a value for 'smooth' must be set")
+        out.update(smooth_type=kinds[st], smooth=smv)
+    return out
 
 
 def _observations(rd, doc, comps, trans, with_data):
@@ -497,7 +580,7 @@ def load(path, datadir=None, with_data=True):
     rd = _Reader(doc, datadir)
     P = _Params()
     comps = _compartments(doc, P)
-    trans = _transitions(doc, comps, P)
+    trans, dirichlet = _transitions(doc, comps, P)
 
     splines = []
     if len(doc.get("R_spline", [])) != 1:
@@ -532,7 +615,7 @@ def load(path, datadir=None, with_data=True):
                    **({"prob": [t["prob"]]} if t["prob"] is not None else {})} for t in trans],
         "params": P.specs,
         "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": [1.0],
-        "geo_spline": -1, "N": None, "M": None, "pop": pop,
+        "geo_spline": -1, "N": None, "M": None, "pop": pop, "dirichlet": dirichlet,
         "obs": _observations(rd, doc, comps, trans, with_data),
     }
     return spec, np.array(P.values), meta

@@ -67,13 +67,20 @@ typedef struct {
   const int32_t* prob_param;    /* [G] branching weight parameters, NULL = single exit */
 } beep_trans;
 
-/* Piecewise-linear spline (Model::create_disc_spline, src/model.cc:1020-1053). */
+/* Piecewise-linear spline (Model::create_disc_spline, src/model.cc:1020-1053), with
+ * the smoothing prior of Model::spline_prior (src/model.cc:1662-1704). */
+#define BEEP_SMOOTH_NONE   0
+#define BEEP_SMOOTH_NORMAL 1            /* 'smooth': normal density on knot differences     */
+#define BEEP_SMOOTH_LOG    2            /* 'log_smooth': log-normal density on knot ratios  */
 typedef struct {
   int32_t nknot;
   const double* t;              /* knot times (days from start)                      */
   const int32_t* param;         /* [nknot] parameter at each knot                    */
   const double* multfac;        /* [nknot] multiplying factor at each knot           */
   int32_t factor_param;         /* overall factor parameter; -1 = 1                  */
+  int32_t smooth_type;          /* BEEP_SMOOTH_*                                     */
+  const double* smooth;         /* [nknot] smoothing sd between knot i and i+1, or null
+                                   with BEEP_SMOOTH_NONE                              */
 } beep_spline;
 
 /* Dirichlet group normalised by Model::dirichlet_correct (src/model.cc:2685-2695). */

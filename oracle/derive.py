@@ -10,6 +10,7 @@ Restates, for one strain with age as the only demographic category:
   Data::set_obs_sd                  src/data.cc:1589-1623
   Model::set_infected_uninfected    src/model.cc:110-166
   Model::dirichlet_correct          src/model.cc:2685-2695
+  Model::spline_prior               src/model.cc:1662-1704
   Model::create_transrate           src/model.cc:1116-1140
   Model::create_disc_spline         src/model.cc:1020-1053
   Model::create_susceptibility      src/model.cc:1144-1178
@@ -230,6 +231,38 @@ def prior(spec, pv):
             Pr += -0.5 * np.log(2 * np.pi * pr[2] ** 2) - (v - pr[1]) ** 2 / (2 * pr[2] ** 2)
         elif pr[0] == "dirflat":
             Pr += -v
+    return Pr + spline_prior(spec, pv)
+
+
+SMOOTH_NONE, SMOOTH_NORMAL, SMOOTH_LOG = 0, 1, 2
+
+
+def spline_prior(spec, pv):
+    """Model::spline_prior (src/model.cc:1662-1704): between neighbouring knots
+    with different parameters, times and not both fixed, a normal density on
+    the difference (smooth) or a log-normal one on the ratio (log_smooth),
+    on the Dirichlet-corrected values."""
+    pd = dirichlet_correct(spec, pv)
+    Pr = 0.0
+    for sp in spec["splines"]:
+        kind = sp.get("smooth_type", SMOOTH_NONE)
+        if kind == SMOOTH_NONE:
+            continue
+        par, t, mf, sm = sp["param"], sp["t"], sp.get("multfac", [1.0] * len(sp["t"])), sp["smooth"]
+        for i in range(len(par) - 1):
+            p1, p2 = par[i], par[i + 1]
+            if p1 == p2 or t[i] == t[i + 1]:
+                continue
+            if spec["params"][p1]["prior"][0] == "fixed" and spec["params"][p2]["prior"][0] == "fixed":
+                continue
+            v = sm[i]
+            if kind == SMOOTH_LOG:
+                mu = np.log(pd[p1] * mf[i])
+                logx = np.log(pd[p2] * mf[i + 1])
+                Pr += -(logx - mu) ** 2 / (2 * v * v) - logx
+            else:
+                d = pd[p1] - pd[p2]
+                Pr += -d * d / (2 * v * v)
     return Pr
 
 

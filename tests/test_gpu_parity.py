@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 from scipy import stats
 
+import toml_models
 from beep_amd import _lib, configs
 from oracle import oracle as O
 
@@ -43,17 +44,8 @@ def _transmean_magnitude(orc, tb, st):
     return out
 
 
-def toml_seir_erlang():
-    """SEIR with an Erlang(2) latent period, 2 steps per day, weekly transition
-    counts and population data, read from its model file by the TOML front end."""
-    import os
-    from beep_amd import toml_spec
-    d = os.path.join(os.path.dirname(__file__), "golden", "toml")
-    spec, values, _ = toml_spec.load(os.path.join(d, "seir_erlang.toml"), datadir=os.path.join(d, "data_seir_erlang"))
-    return spec, values
-
-
-@pytest.mark.parametrize("mk", [configs.seir_single, configs.spatial_abcsmc, toml_seir_erlang])
+@pytest.mark.parametrize("mk", [configs.seir_single, configs.spatial_abcsmc, toml_models.toml_seir_erlang,
+                                toml_models.toml_seird_branch])
 def test_fixed_trajectory_replay(gpu_lib, mk):
     spec, truth = mk()
     orc = O.Oracle(spec)

@@ -6,6 +6,7 @@ import re
 import numpy as np
 import pytest
 
+import toml_models
 from beep_amd import _lib, configs, spec as specmod
 from oracle import derive
 
@@ -20,7 +21,8 @@ def test_header_symbols_exported(gpu_lib):
         assert hasattr(gpu_lib, name), name
 
 
-@pytest.mark.parametrize("mk", [configs.seir_single, configs.spatial_abcsmc, configs.dense_pmcmc])
+@pytest.mark.parametrize("mk", [configs.seir_single, configs.spatial_abcsmc, configs.dense_pmcmc,
+                                toml_models.toml_seir_erlang, toml_models.toml_seird_branch])
 def test_host_tables_match_oracle(gpu_lib, mk):
     spec, truth = mk()
     nz = derive.normalise(spec)

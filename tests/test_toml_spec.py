@@ -144,19 +144,58 @@ def test_rejections(tmp_path, extra, msg):
         _model(tmp_path, extra)
 
 
-def test_rejects_dirichlet_branches_and_smoothing(tmp_path):
-    d = str(tmp_path)
+def _base_model(d, name, trans_I_R, spline):
     _write(d, "areas.csv", "area,population\nX,100\n")
-    base = BASE.replace('R_spline = [{bp="[bp:R_spline.csv]", value="[value:R_spline.csv]", '
-                        'prior="[prior:R_spline.csv]"}]', 'R_spline = [{value="2", prior="Uniform(1,3)"}]')
-    p = _write(d, "a.toml", base.replace('prob_value="0.8", prob_prior="Uniform(0.5,1)"',
-                                         'prob_value="*", prob_prior="Dir(1)"'))
+    text = BASE.replace('prob_value="0.8", prob_prior="Uniform(0.5,1)"', trans_I_R)
+    text = text.replace('R_spline = [{bp="[bp:R_spline.csv]", value="[value:R_spline.csv]", '
+                        'prior="[prior:R_spline.csv]"}]', spline)
+    return _write(d, name, text)
+
+
+def test_dirichlet_branches(tmp_path):
+    d = str(tmp_path)
+    spline = 'R_spline = [{value="2", prior="Uniform(1,3)"}]'
+    # A Dirichlet prior on one branch only: refused.
+    p = _base_model(d, "a.toml", 'prob_value="*", prob_prior="Dir(1)"', spline)
     with pytest.raises(T.TomlError, match="Dir"):
         T.load(p, datadir=d)
-    p = _write(d, "b.toml", base.replace('R_spline = [{value="2", prior="Uniform(1,3)"}]',
-                                         'R_spline = [{value="2", prior="Uniform(1,3)", smooth="0.5"}]'))
-    with pytest.raises(T.TomlError, match="'smooth'"):
-        T.load(p, datadir=d)
+    # Dir(*) / Dir(1) on every branch out of I: one Dirichlet group, flat
+    # priors, '*' filled with one minus the others.
+    text = open(p).read().replace('prob_value="0.2", prob_prior="Uniform(0,0.5)"', 'prob_value="0.3", prob_prior="Dir(*)"')
+    p = _write(d, "b.toml", text)
+    spec, values, _ = T.load(p, datadir=d)
+    names = [q["name"] for q in spec["params"]]
+    ih, ir = names.index("I->H branch prob."), names.index("I->R branch prob.")
+    assert spec["dirichlet"] == [{"param": [ih, ir]}]
+    assert spec["params"][ih]["prior"] == spec["params"][ir]["prior"] == ("dirflat",)
+    assert values[ih] == 0.3 and values[ir] == pytest.approx(0.7, abs=1e-15)
+    assert [t.get("prob") for t in spec["trans"]] == [None, [ih], [ir], None]
+    # Values only: fixed probabilities, '*' filled the same way, no group.
+    text = text.replace('prob_prior="Dir(*)"', "").replace('prob_prior="Dir(1)"', "")
+    spec, values, _ = T.load(_write(d, "c.toml", text), datadir=d)
+    assert spec["dirichlet"] == [] and spec["params"][ir]["prior"] == ("fixed", pytest.approx(0.7, abs=1e-15))
+
+
+def test_smoothing_prior(tmp_path):
+    """log_smooth between three knots: Pr gains, for each neighbouring pair,
+    -(log x2 - log x1)^2 / (2 s^2) - log x2 (src/model.cc:1684-1690)."""
+    from oracle import derive
+    d = str(tmp_path)
+    spline = ('R_spline = [{bp="start|2020-03-15|end", value="2|1|1.5", prior="Uniform(0.5,3)", '
+              'smooth_type="log_smooth", smooth="0.5"}]')
+    p = _base_model(d, "s.toml", 'prob_value="0.8", prob_prior="Uniform(0.5,1)"', spline)
+    spec, values, _ = T.load(p, datadir=d)
+    sp = spec["splines"][0]
+    assert sp["smooth_type"] == derive.SMOOTH_LOG and sp["smooth"] == [0.5, 0.5, 0.5]
+    want = sum(-(np.log(b) - np.log(a)) ** 2 / (2 * 0.25) - np.log(b) for a, b in ((2.0, 1.0), (1.0, 1.5)))
+    assert derive.spline_prior(spec, values) == pytest.approx(want, rel=1e-14)
+    # 'smooth' (normal on differences) and a fixed pair that contributes nothing.
+    spline2 = spline.replace('log_smooth', 'smooth').replace('prior="Uniform(0.5,3)"', 'prior="Fixed(2)|Fixed(1)|Uniform(0.5,3)"')
+    spec, values, _ = T.load(_base_model(d, "t.toml", 'prob_value="0.8", prob_prior="Uniform(0.5,1)"', spline2), datadir=d)
+    assert derive.spline_prior(spec, values) == pytest.approx(-(1.0 - 1.5) ** 2 / (2 * 0.25), rel=1e-14)
+    with pytest.raises(T.TomlError, match="smooth_type"):
+        T.load(_base_model(d, "u.toml", 'prob_value="0.8", prob_prior="Uniform(0.5,1)"',
+                           spline.replace("log_smooth", "wiggly")), datadir=d)
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="the reference's examples are not present")

@@ -1,0 +1,22 @@
+"""Model specs read by the TOML front end from the fixture model files under
+golden/toml (written in BEEP's input format for these tests)."""
+import os
+
+from beep_amd import toml_spec
+
+_DIR = os.path.join(os.path.dirname(__file__), "golden", "toml")
+
+
+def _load(name, data):
+    spec, values, _ = toml_spec.load(os.path.join(_DIR, name), datadir=os.path.join(_DIR, data))
+    return spec, values
+
+
+def toml_seir_erlang():
+    """SEIR, Erlang(2) latent period, 2 steps per day, weekly counts and populations."""
+    return _load("seir_erlang.toml", "data_seir_erlang")
+
+
+def toml_seird_branch():
+    """SEIRD, Erlang(3) infectious period, Dirichlet branches, log-smoothed 3-knot R(t)."""
+    return _load("seird_branch.toml", "data_seird_branch")

@@ -28,7 +28,8 @@ reference's readers:
                     the start of a day, observations named 'X->Y' or 'X' by
                     base compartment name, '.' unknown, '*' below threshold)
 
-Supported: one area without age groups (`areas` with a `population` column),
+Supported: one age group, one or more areas (`areas` with a `population` column, a
+`geo_mixing_matrix` for more than one, `geo_mixing_modify`, `geo_dep` tables),
 `comps` with dist Exp / Erlang, `trans` with branching probabilities,
 `R_spline` and `efoi_spline` (one entry each), `data_tables` of type
 'transition' or 'population' with obsmodel 'normal <sd>', 'normal <P>%',
@@ -56,7 +57,8 @@ IGNORED_KEYS = {"description", "outputdir", "state_outputs", "plot_param_values"
                 "output_prop", "mode"}
 PREDICTION_KEYS = {"modification", "prediction_start", "prediction_end"}
 SUPPORTED_KEYS = {"time_format", "start", "end", "datadir", "steps_per_unit_time", "comps", "trans", "R_spline",
-                  "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str"}
+                  "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str", "geo_mixing_matrix",
+                  "geo_mixing_modify"}
 
 
 class TomlError(ValueError):
@@ -126,7 +128,7 @@ class _Reader:
         return self._abs_time(st) - self.start
 
     # --- tables ---
-    def table(self, name):
+    def table(self, name, heading=True):
         path = os.path.join(self.datadir, name)
         if not os.path.exists(path):
             raise TomlError(f"cannot open the file '{path}'")
@@ -135,7 +137,7 @@ class _Reader:
             rows = [[c.strip() for c in r] for r in csv.reader(f, delimiter=delim) if r]
         if not rows:
             raise TomlError(f"the file '{name}' contains no information")
-        return rows[0], rows[1:]
+        return (rows[0], rows[1:]) if heading else (None, rows)
 
     def from_table(self, s):
         """'[column:file.csv]' -> the column's values joined by '|' (src/inputs.cc:1499-1551)."""
@@ -426,15 +428,31 @@ def _spline(rd, P, key, entry):
     return out
 
 
-def _observations(rd, doc, comps, trans, with_data):
-    """data_tables -> spec observations (single area, one age group)."""
+def _regions(area_head, area_rows, geo_dep):
+    """Data::create_geomap (src/data.cc:1830-1858): 'all', or the areas grouped
+    by their value in column geo_dep ('NA' leaves an area out)."""
+    if geo_dep == "all":
+        return [("all", list(range(len(area_rows))))]
+    if geo_dep not in area_head:
+        raise TomlError(f"'geo_dep={geo_dep}' is not a column of the areas file")
+    col = area_head.index(geo_dep)
+    out = {}
+    for r, row in enumerate(area_rows):
+        if row[col] != "NA":
+            out.setdefault(row[col], []).append(r)
+    return list(out.items())
+
+
+def _observations(rd, doc, comps, trans, with_data, area_head, area_rows):
+    """data_tables -> spec observations (one age group)."""
     thr_str = _s(doc.get("threshold_str", "*"))
     nodata_str = _s(doc.get("nodata_str", "."))
     obs = []
+    ngraph = 0                         # one graph per data column (Data::graph)
     tables = doc.get("data_tables", [])
     for d, e in enumerate(tables):
         _check_keys(e, {"type", "observation", "timestep", "obsmodel", "file", "factor", "shift", "start", "end",
-                        "threshold", "epsilon_factor", "plot_name", "line_colour"}, "data_tables")
+                        "threshold", "epsilon_factor", "plot_name", "line_colour", "geo_dep"}, "data_tables")
         typ = _s(e.get("type"))
         if typ not in ("transition", "population"):
             raise TomlError(f"data table type '{typ}' is not supported (transition or population)")
@@ -492,12 +510,23 @@ def _observations(rd, doc, comps, trans, with_data):
         if not with_data:
             continue
         head, rows = rd.table(file)
-        if rd.time_col not in head or "Data" not in head:
-            raise TomlError(f"the file '{file}' needs columns '{rd.time_col}' and 'Data'")
-        tc, dc = head.index(rd.time_col), head.index("Data")
-        sc = head.index("Data SD") if load_sd and "Data SD" in head else None
-        if load_sd and sc is None:
-            raise TomlError(f"the file '{file}' needs a column 'Data SD' for 'normal load'")
+        if rd.time_col not in head:
+            raise TomlError(f"the file '{file}' needs a column '{rd.time_col}'")
+        tc = head.index(rd.time_col)
+        # One column per region with geo_dep, else 'Data' over every area
+        # (Data::get_datafilter, src/data.cc:1460-1560); columns absent from
+        # the file are skipped, as the reference warns and skips them.
+        geo_dep = _s(e.get("geo_dep", ""))
+        if geo_dep:
+            filters = _regions(area_head, area_rows, geo_dep)
+        else:
+            filters = [("Data", list(range(len(area_rows))))]
+        cols = [(name, head.index(name), areas) for name, areas in filters if name in head]
+        if not cols:
+            raise TomlError(f"the data in '{file}' was not used: expected columns such as '{filters[0][0]}'")
+        for name, _, _ in cols:
+            if load_sd and name + " SD" not in head:
+                raise TomlError(f"the file '{file}' needs a column '{name} SD' for 'normal load'")
         # Rows in the analysis period (Data::table_select_dates, src/data.cc:1981-2060).
         times = [rd.time(r[tc]) + shift for r in rows]
         start = rd.time(_s(e["start"])) if "start" in e else None
@@ -525,34 +554,52 @@ def _observations(rd, doc, comps, trans, with_data):
                 ts = tstep
             if times[r1] + ts > rd.period:
                 r1 -= 1
-        graph = d
-        ti = times[r0] * rd.spd if typ == "transition" else None
-        for r in range(r0, r1 + 1):
-            raw = rows[r][dc]
-            if raw == thr_str:
-                if thresh == "":
-                    raise TomlError(f"the file '{file}' has thresholded data but no 'threshold' is set")
-                y = OBS_THRESH
-            elif raw == nodata_str:
-                y = OBS_UNKNOWN
-            else:
+        for name, dc, areas in cols:
+            sc = head.index(name + " SD") if load_sd else None
+            graph = ngraph
+            ngraph += 1
+            obs += _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, graph, d, otype, sd, percent,
+                                 factor, thresh, thr_str, nodata_str, tr_list, co_list, file)
+    return obs
+
+
+def _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, graph, d, otype, sd, percent, factor, thresh,
+                  thr_str, nodata_str, tr_list, co_list, file):
+    """The observations of one data column (Data::load_timeseries_datatable,
+    src/data.cc:1236-1310): populations at the start of a day, transition
+    counts over consecutive windows of `ts` days.  sc: the column of its
+    standard deviations ('normal load'), or None."""
+    obs = []
+    ti = times[r0] * rd.spd if typ == "transition" else None
+    for r in range(r0, r1 + 1):
+        raw = rows[r][dc]
+        if raw == thr_str:
+            if thresh == "":
+                raise TomlError(f"the file '{file}' has thresholded data but no 'threshold' is set")
+            y = OBS_THRESH
+        elif raw == nodata_str:
+            y = OBS_UNKNOWN
+        else:
+            try:
                 y = float(raw)
-            o = {"y": y, "type": otype, "factor": factor, "datatable": d, "graph": graph, "area": [0], "dp": [0],
-                 "thresh": float(thresh) if thresh else 0.0}
-            if sd is not None:
-                o["sd"] = sd
-            if percent is not None:
-                o["percent"] = percent
-            if load_sd:
-                o["sd"] = float(rows[r][sc])
-            if typ == "transition":
-                tf = (times[r] + ts) * rd.spd
-                o.update(sett_i=ti, sett_f=tf, tr=list(tr_list))
-                ti = tf
-            else:
-                t0 = times[r] * rd.spd
-                o.update(sett_i=t0, sett_f=t0 + 1, co=list(co_list))
-            obs.append(o)
+            except ValueError:
+                raise TomlError(f"in the file '{file}' the value '{raw}' is not a number") from None
+        o = {"y": y, "type": otype, "factor": factor, "datatable": d, "graph": graph, "area": list(areas),
+             "dp": [0], "thresh": float(thresh) if thresh else 0.0}
+        if sd is not None:
+            o["sd"] = sd
+        if percent is not None:
+            o["percent"] = percent
+        if sc is not None:
+            o["sd"] = float(rows[r][sc])
+        if typ == "transition":
+            tf = (times[r] + ts) * rd.spd
+            o.update(sett_i=ti, sett_f=tf, tr=list(tr_list))
+            ti = tf
+        else:
+            t0 = times[r] * rd.spd
+            o.update(sett_i=t0, sett_f=t0 + 1, co=list(co_list))
+        obs.append(o)
     return obs
 
 
@@ -593,29 +640,62 @@ def load(path, datadir=None, with_data=True):
         splines.append(_spline(rd, P, "efoi_spline", doc["efoi_spline"][0]))
         efoi = 1
 
+    geo = -1
+    if "geo_mixing_modify" in doc:
+        if len(doc["geo_mixing_modify"]) != 1:
+            raise TomlError("only one spline can represent 'geo_mixing'")
+        splines.append(_spline(rd, P, "geo_mixing_modify", doc["geo_mixing_modify"][0]))
+        geo = len(splines) - 1
+
     # Areas and the initial population (src/data.cc:288-332,419-457).
     if "areas" not in doc:
         raise TomlError("'areas' must be set")
     head, rows = rd.table(_s(doc["areas"]))
     if "area" not in head or "population" not in head:
         raise TomlError("the areas file needs columns 'area' and 'population'")
-    if len(rows) != 1:
-        raise TomlError("one area is supported (a geographic mixing matrix is needed for more)")
+    A = len(rows)
     sus = trans[[t["infection"] for t in trans].index(True)]["from"]
-    popv = float(rows[0][head.index("population")])
-    if popv < 0:
-        raise TomlError("populations must be non-negative")
-    pop = [[[int(popv + 0.5) if c == sus else 0] for c in range(len(comps))]]
+    pop = []
+    for row in rows:
+        try:
+            popv = float(row[head.index("population")])
+        except ValueError:
+            raise TomlError(f"the population '{row[head.index('population')]}' is not a number") from None
+        if popv < 0:
+            raise TomlError("populations must be non-negative")
+        pop.append([[int(popv + 0.5) if c == sus else 0] for c in range(len(comps))])
+
+    # Geographic mixing (Data::load_geo_mixing_matrix, src/data_matrix.cc:206-229):
+    # an A x A table without a heading; the diagonal and, per row, the
+    # non-zero off-diagonal entries.  Normalised later by the host model.
+    M = None
+    if "geo_mixing_matrix" in doc:
+        _, mrows = rd.table(_s(doc["geo_mixing_matrix"]), heading=False)
+        if len(mrows) != A or any(len(r) != A for r in mrows):
+            raise TomlError(f"the geographic mixing matrix must be {A} x {A} (one row and column per area)")
+        try:
+            mat = [[float(x) for x in r] for r in mrows]
+        except ValueError:
+            raise TomlError("the geographic mixing matrix holds a value that is not a number") from None
+        M = {"diag": [mat[j][j] for j in range(A)], "rowptr": [0], "to": [], "val": []}
+        for j in range(A):
+            for i in range(A):
+                if i != j and mat[j][i] != 0:
+                    M["to"].append(i)
+                    M["val"].append(mat[j][i])
+            M["rowptr"].append(len(M["to"]))
+    elif A != 1:
+        raise TomlError("a value for 'geo_mixing_matrix' must be set when there is more than one area")
 
     spec = {
-        "name": os.path.splitext(os.path.basename(path))[0], "A": 1, "G": 1, "period": rd.period, "spd": rd.spd,
+        "name": os.path.splitext(os.path.basename(path))[0], "A": A, "G": 1, "period": rd.period, "spd": rd.spd,
         "comps": [{"name": c["name"], "inf": c["inf"], "mean": [c["mean"]] if c["mean"] is not None else None,
                    "shape": c["shape"] or 1} for c in comps],
         "trans": [{"from": t["from"], "to": t["to"], "infection": t["infection"],
                    **({"prob": [t["prob"]]} if t["prob"] is not None else {})} for t in trans],
         "params": P.specs,
         "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": [1.0],
-        "geo_spline": -1, "N": None, "M": None, "pop": pop, "dirichlet": dirichlet,
-        "obs": _observations(rd, doc, comps, trans, with_data),
+        "geo_spline": geo, "N": None, "M": M, "pop": pop, "dirichlet": dirichlet,
+        "obs": _observations(rd, doc, comps, trans, with_data, head, rows),
     }
     return spec, np.array(P.values), meta

@@ -131,17 +131,48 @@ data_tables = [{type="transition", observation="I->H", obsmodel="normal load", f
 
 @pytest.mark.parametrize("extra,msg", [
     ('ages = [{cats="0-19|20+"}]\n', "'ages'"),
-    ('geo_mixing_matrix = "g.csv"\n', "'geo_mixing_matrix'"),
     ('data_tables = [{type="transition", observation="I->H", obsmodel="negbin 2", file="I-H.csv"}]\n',
      "not recognised"),
     ('data_tables = [{type="transition", observation="I->X", obsmodel="poisson", file="I-H.csv"}]\n',
      "not a compartment or a transition"),
     ('data_tables = [{type="transition", observation="I->H", obsmodel="poisson", file="I-H.csv", geo_dep="r"}]\n',
-     "'geo_dep'"),
+     "not a column of the areas file"),
 ])
 def test_rejections(tmp_path, extra, msg):
     with pytest.raises(T.TomlError, match=msg):
         _model(tmp_path, extra)
+
+
+def test_multi_area():
+    """Three areas: the raw matrix as diagonal + off-diagonal rows
+    (src/data_matrix.cc:206-229), the mixing spline as the geo spline, one
+    observation graph per data column, geo_dep by area and by region."""
+    import toml_models
+    spec, values = toml_models.toml_sir_3areas()
+    assert spec["A"] == 3 and spec["G"] == 1
+    assert spec["M"] == {"diag": [1.0, 1.5, 0.8], "rowptr": [0, 1, 3, 4], "to": [1, 0, 2, 1],
+                         "val": [0.2, 0.2, 0.3, 0.3]}
+    assert spec["pop"] == [[[30000], [0], [0]], [[80000], [0], [0]], [[45000], [0], [0]]]
+    names = [p["name"] for p in spec["params"]]
+    g = spec["splines"][spec["geo_spline"]]
+    assert [names[i] for i in g["param"]] == ["geo-mixing-modify"] * 2 and values[g["param"][0]] == 0.6
+    si = [o for o in spec["obs"] if o["datatable"] == 0]
+    ir = [o for o in spec["obs"] if o["datatable"] == 1]
+    assert len(si) == 3 * 8 and len(ir) == 2 * 8
+    assert [(o["area"], o["graph"]) for o in si[::8]] == [([0], 0), ([1], 1), ([2], 2)]
+    assert [(o["area"], o["graph"]) for o in ir[::8]] == [([0], 3), ([1, 2], 4)]
+
+
+def test_multi_area_needs_matrix(tmp_path):
+    d = str(tmp_path)
+    _write(d, "areas.csv", "area,population\nX,100\nY,200\n")
+    text = BASE.replace('R_spline = [{bp="[bp:R_spline.csv]", value="[value:R_spline.csv]", '
+                        'prior="[prior:R_spline.csv]"}]', 'R_spline = [{value="2", prior="Uniform(1,3)"}]')
+    with pytest.raises(T.TomlError, match="geo_mixing_matrix"):
+        T.load(_write(d, "m.toml", text), datadir=d)
+    _write(d, "g.csv", "1,0.5\n0.5\n")
+    with pytest.raises(T.TomlError, match="2 x 2"):
+        T.load(_write(d, "n.toml", text + 'geo_mixing_matrix = "g.csv"\n'), datadir=d)
 
 
 def _base_model(d, name, trans_I_R, spline):

@@ -20,3 +20,8 @@ def toml_seir_erlang():
 def toml_seird_branch():
     """SEIRD, Erlang(3) infectious period, Dirichlet branches, log-smoothed 3-knot R(t)."""
     return _load("seird_branch.toml", "data_seird_branch")
+
+
+def toml_sir_3areas():
+    """SIR over three areas: geographic mixing, mixing spline, per-area and per-region counts."""
+    return _load("sir_3areas.toml", "data_sir_3areas")

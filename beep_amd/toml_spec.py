@@ -28,7 +28,8 @@ reference's readers:
                     the start of a day, observations named 'X->Y' or 'X' by
                     base compartment name, '.' unknown, '*' below threshold)
 
-Supported: one age group, one or more areas (`areas` with a `population` column, a
+Supported: one or more age groups (`ages`, `age_mixing_matrix`, `age:` parameter lists,
+`democats_dep="age"` tables) and areas (`areas` with population columns, a
 `geo_mixing_matrix` for more than one, `geo_mixing_modify`, `geo_dep` tables),
 `comps` with dist Exp / Erlang, `trans` with branching probabilities,
 `R_spline` and `efoi_spline` (one entry each), `data_tables` of type
@@ -58,7 +59,7 @@ IGNORED_KEYS = {"description", "outputdir", "state_outputs", "plot_param_values"
 PREDICTION_KEYS = {"modification", "prediction_start", "prediction_end"}
 SUPPORTED_KEYS = {"time_format", "start", "end", "datadir", "steps_per_unit_time", "comps", "trans", "R_spline",
                   "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str", "geo_mixing_matrix",
-                  "geo_mixing_modify"}
+                  "geo_mixing_modify", "ages", "age_mixing_matrix"}
 
 
 class TomlError(ValueError):
@@ -227,13 +228,47 @@ def _no_dep(entry, keys, where):
             raise TomlError(f"{where}: the dependency in '{k}={v}' is not supported")
 
 
+def _ct_specs(rd, entry, keys, par_root, ages, where):
+    """Per age group, the (name, value, prior) strings of a compartment or
+    transition parameter (Inputs::get_dep and find_ct_param_spec,
+    src/inputs.cc:1107-1143,1284-1326): 'age: v0 | v1 | ...' gives one
+    parameter per age group, named '<age> <root>' unless named; without a
+    dependency one parameter serves every age group."""
+    strs, dep = [], ""
+    for k in keys:
+        st = _s(entry.get(k, ""))
+        spl = st.split("[")[0].split(":")
+        if len(spl) > 2:
+            raise TomlError(f"{where}: there was a problem with '{st}'")
+        if len(spl) == 2:
+            if "[" in st:
+                raise TomlError(f"{where}: a dependency with a table value ('{st}') is not supported")
+            d = spl[0].strip()
+            st = spl[1].strip()
+            if dep not in ("", d):
+                raise TomlError(f"{where}: conflicting dependencies")
+            dep = d
+        strs.append(st)
+    if dep == "":
+        return [tuple(strs)] * len(ages)
+    if dep != "age":
+        raise TomlError(f"{where}: the dependency '{dep}' is not supported")
+    if len(ages) == 1:
+        raise TomlError(f"{where}: the dependency 'age:' needs more than one age group")
+    G = len(ages)
+    names = rd.split_n(strs[0], G, where)
+    vals = rd.split_n(strs[1], G, where)
+    pris = rd.split_n(strs[2], G, where)
+    return [(names[g] or f"{ages[g]} {par_root}", vals[g], pris[g]) for g in range(G)]
+
+
 def _check_keys(entry, allowed, where):
     for k in entry:
         if k not in allowed:
             raise TomlError(f"{where}: '{k}' is not supported")
 
 
-def _compartments(doc, P):
+def _compartments(rd, doc, P, ages):
     """comps -> expanded compartments (src/inputs_comps.cc:22-63,303-365)."""
     if "comps" not in doc:
         raise TomlError("the file must define 'comps'")
@@ -248,7 +283,6 @@ def _compartments(doc, P):
         if nam in bases:
             raise TomlError(f"the compartment '{nam}' cannot be defined multiple times")
         bases.append(nam)
-        _no_dep(e, ("mean_param", "mean_value", "mean_prior"), f"compartment '{nam}'")
         dist = _s(e.get("dist", ""))
         if dist == "":
             k = None
@@ -263,9 +297,11 @@ def _compartments(doc, P):
             raise TomlError(f"compartment '{nam}': dist '{dist}' is not recognised")
         mean = None
         if any(x in e for x in ("mean_param", "mean_value", "mean_prior")):
-            pname = _s(e.get("mean_param", "")) or f"{nam} mean occupancy"
-            mean = P.add(pname, _s(e.get("mean_value", "")), _s(e.get("mean_prior", "")))
+            specs = _ct_specs(rd, e, ("mean_param", "mean_value", "mean_prior"), f"{nam} mean occupancy", ages,
+                              f"compartment '{nam}'")
+            mean = [P.add(nm or f"{nam} mean occupancy", v, pr) for nm, v, pr in specs]
         inf = -1
+        _no_dep(e, ("inf_param", "inf_value", "inf_prior"), f"compartment '{nam}'")
         if any(x in e for x in ("inf_param", "inf_value", "inf_prior")):
             pname = _s(e.get("inf_param", "")) or f"{nam} Infectivity"
             inf = P.add(pname, _s(e.get("inf_value", "")), _s(e.get("inf_prior", "")))
@@ -276,7 +312,7 @@ def _compartments(doc, P):
     return comps
 
 
-def _transitions(doc, comps, P):
+def _transitions(rd, doc, comps, P, ages):
     """trans -> internal Erlang transitions first, then the file's (src/inputs_comps.cc:66-139,268-300)."""
     if "trans" not in doc:
         raise TomlError("the file must define 'trans'")
@@ -305,17 +341,17 @@ def _transitions(doc, comps, P):
         inf = _s(e.get("infection", ""))
         if inf not in ("", "yes", "no"):
             raise TomlError("in 'trans' the value of 'infection' can be 'yes', 'no' or omitted")
-        _no_dep(e, ("prob_param", "prob_value", "prob_prior"), f"transition {fr}->{to}")
         branch = None
         if any(x in e for x in ("prob_param", "prob_value", "prob_prior")):
-            branch = (_s(e.get("prob_param", "")) or f"{fr}->{to} branch prob.", _s(e.get("prob_value", "")),
-                      _s(e.get("prob_prior", "")))
+            root = f"{fr}->{to} branch prob."
+            branch = [(nm or root, v, pr) for nm, v, pr in
+                      _ct_specs(rd, e, ("prob_param", "prob_value", "prob_prior"), root, ages, f"transition {fr}->{to}")]
         ninf += inf == "yes"
         trans.append({"from": cf, "to": ct, "infection": inf == "yes", "prob": None, "base": f"{fr}->{to}",
                       "branch": branch})
     if ninf != 1:
         raise TomlError("exactly one infection transition must exist")
-    return trans, _branches(trans, comps, P)
+    return trans, _branches(trans, comps, P, len(ages))
 
 
 def _is_flat_dir(prior):
@@ -333,55 +369,60 @@ def _is_flat_dir(prior):
         return False
 
 
-def _branches(trans, comps, P):
-    """Branch probabilities of the transitions leaving one compartment: one
-    value may be '*', one minus the others (src/model.cc:2894-2915); with
-    Dir(*) / Dir(1) priors they form a Dirichlet group that the model
-    normalises (Model::dirichlet_correct, src/model.cc:2685-2695)."""
+def _branches(trans, comps, P, G):
+    """Branch probabilities of the transitions leaving one compartment, per
+    age group: one value may be '*', one minus the others (src/model.cc:
+    2894-2915); with Dir(*) / Dir(1) priors they form a Dirichlet group that
+    the model normalises (Model::dirichlet_correct, src/model.cc:2685-2695).
+    Sets t["prob"] to the G parameter indices of each branching transition."""
     groups = []
     by_src = {}
     for t in trans:
-        if t.get("branch") is not None:
+        t["prob"] = [None] * G if t.get("branch") is not None else None
+        if t["prob"] is not None:
             by_src.setdefault(t["from"], []).append(t)
     for src, ts in by_src.items():
-        priors = [t["branch"][2] for t in ts]
-        dirs = [p.replace(" ", "").lower().startswith(("dir(", "mdir(")) for p in priors]
-        if any(dirs) and not all(_is_flat_dir(p) for p in priors):
-            raise TomlError(f"branch probabilities out of '{comps[src]['base']}': only Dir(*) / Dir(1) priors on "
-                            "every branch are supported")
-        vals = [t["branch"][1] for t in ts]
-        star = [k for k, v in enumerate(vals) if v == "*"]
-        if len(star) > 1:
-            raise TomlError(f"branch probabilities out of '{comps[src]['base']}': at most one value can be '*'")
-        num = []
-        for k, v in enumerate(vals):
-            if k in star or v == "":
-                num.append(math.nan)
+        where = f"branch probabilities out of '{comps[src]['base']}'"
+        for g in range(G):
+            priors = [t["branch"][g][2] for t in ts]
+            dirs = [p.replace(" ", "").lower().startswith(("dir(", "mdir(")) for p in priors]
+            if any(dirs) and not all(_is_flat_dir(p) for p in priors):
+                raise TomlError(f"{where}: only Dir(*) / Dir(1) priors on every branch are supported")
+            vals = [t["branch"][g][1] for t in ts]
+            star = [k for k, v in enumerate(vals) if v == "*"]
+            if len(star) > 1:
+                raise TomlError(f"{where}: at most one value can be '*'")
+            num = []
+            for k, v in enumerate(vals):
+                if k in star or v == "":
+                    num.append(math.nan)
+                else:
+                    try:
+                        num.append(float(v))
+                    except ValueError:
+                        raise TomlError(f"branch probability value '{v}' is not a number") from None
+            if star:
+                rest = sum(x for k, x in enumerate(num) if k != star[0])
+                if math.isnan(rest) or rest > 1:
+                    raise TomlError(f"{where} cannot fill the value '*'")
+                num[star[0]] = 1.0 - rest
+            if all(dirs):
+                idx = [P.add_tuple(t["branch"][g][0], num[k], ("dirflat",)) for k, t in enumerate(ts)]
+                if {"param": idx} not in groups:        # parameters shared by every age group: one group
+                    groups.append({"param": idx})
             else:
-                try:
-                    num.append(float(v))
-                except ValueError:
-                    raise TomlError(f"branch probability value '{v}' is not a number") from None
-        if star:
-            rest = sum(x for k, x in enumerate(num) if k != star[0])
-            if math.isnan(rest) or rest > 1:
-                raise TomlError(f"branch probabilities out of '{comps[src]['base']}' cannot fill the value '*'")
-            num[star[0]] = 1.0 - rest
-        if all(dirs):
-            idx = [P.add_tuple(t["branch"][0], num[k], ("dirflat",)) for k, t in enumerate(ts)]
-            groups.append({"param": idx})
-        else:
-            idx = [P.add(t["branch"][0], "" if math.isnan(num[k]) else repr(num[k]), t["branch"][2])
-                   for k, t in enumerate(ts)]
-        for k, t in enumerate(ts):
-            t["prob"] = idx[k]
+                idx = [P.add(t["branch"][g][0], "" if math.isnan(num[k]) else repr(num[k]), t["branch"][g][2])
+                       for k, t in enumerate(ts)]
+            for k, t in enumerate(ts):
+                t["prob"][g] = idx[k]
     return groups
 
 
 def _spline(rd, P, key, entry):
     """One spline entry (src/inputs.cc:1363-1474): knots, parameters, multfac."""
     _check_keys(entry, {"name", "bp", "param", "value", "prior", "factor", "factor_param", "factor_value",
-                        "factor_prior", "smooth", "smooth_type"}, key)
+                        "factor_prior", "smooth", "smooth_type"} | ({"age_dist"} if key == "efoi_spline" else set()),
+                key)
     bp = [rd.time(t) for t in _split(rd.from_table(_s(entry.get("bp", "")) or "start|end"), "|")]
     if not bp or bp[0] != 0:
         raise TomlError(f"in '{key}' 'bp' must start at 'start'")
@@ -443,7 +484,7 @@ def _regions(area_head, area_rows, geo_dep):
     return list(out.items())
 
 
-def _observations(rd, doc, comps, trans, with_data, area_head, area_rows):
+def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages):
     """data_tables -> spec observations (one age group)."""
     thr_str = _s(doc.get("threshold_str", "*"))
     nodata_str = _s(doc.get("nodata_str", "."))
@@ -452,7 +493,8 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows):
     tables = doc.get("data_tables", [])
     for d, e in enumerate(tables):
         _check_keys(e, {"type", "observation", "timestep", "obsmodel", "file", "factor", "shift", "start", "end",
-                        "threshold", "epsilon_factor", "plot_name", "line_colour", "geo_dep"}, "data_tables")
+                        "threshold", "epsilon_factor", "plot_name", "line_colour", "geo_dep", "democats_dep"},
+                     "data_tables")
         typ = _s(e.get("type"))
         if typ not in ("transition", "population"):
             raise TomlError(f"data table type '{typ}' is not supported (transition or population)")
@@ -516,15 +558,24 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows):
         # One column per region with geo_dep, else 'Data' over every area
         # (Data::get_datafilter, src/data.cc:1460-1560); columns absent from
         # the file are skipped, as the reference warns and skips them.
-        geo_dep = _s(e.get("geo_dep", ""))
-        if geo_dep:
-            filters = _regions(area_head, area_rows, geo_dep)
+        geo_dep, demo_dep = _s(e.get("geo_dep", "")), _s(e.get("democats_dep", ""))
+        every_area, every_age = list(range(len(area_rows))), list(range(len(ages)))
+        if geo_dep and demo_dep:
+            raise TomlError("in 'data_tables' values for 'democats_dep' and 'geo_dep' cannot both be set")
+        if demo_dep:
+            if demo_dep.lower() != "age":
+                raise TomlError(f"'democats_dep={demo_dep}' is not supported (only 'age')")
+            if len(ages) == 1:
+                raise TomlError("'democats_dep=\"age\"' can only be used with more than one age group")
+            filters = [(a, every_area, [g]) for g, a in enumerate(ages)]
+        elif geo_dep:
+            filters = [(name, areas, every_age) for name, areas in _regions(area_head, area_rows, geo_dep)]
         else:
-            filters = [("Data", list(range(len(area_rows))))]
-        cols = [(name, head.index(name), areas) for name, areas in filters if name in head]
+            filters = [("Data", every_area, every_age)]
+        cols = [(name, head.index(name), areas, dps) for name, areas, dps in filters if name in head]
         if not cols:
             raise TomlError(f"the data in '{file}' was not used: expected columns such as '{filters[0][0]}'")
-        for name, _, _ in cols:
+        for name, _, _, _ in cols:
             if load_sd and name + " SD" not in head:
                 raise TomlError(f"the file '{file}' needs a column '{name} SD' for 'normal load'")
         # Rows in the analysis period (Data::table_select_dates, src/data.cc:1981-2060).
@@ -554,17 +605,17 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows):
                 ts = tstep
             if times[r1] + ts > rd.period:
                 r1 -= 1
-        for name, dc, areas in cols:
+        for name, dc, areas, dps in cols:
             sc = head.index(name + " SD") if load_sd else None
             graph = ngraph
             ngraph += 1
-            obs += _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, graph, d, otype, sd, percent,
+            obs += _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, dps, graph, d, otype, sd, percent,
                                  factor, thresh, thr_str, nodata_str, tr_list, co_list, file)
     return obs
 
 
-def _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, graph, d, otype, sd, percent, factor, thresh,
-                  thr_str, nodata_str, tr_list, co_list, file):
+def _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, dps, graph, d, otype, sd, percent, factor,
+                  thresh, thr_str, nodata_str, tr_list, co_list, file):
     """The observations of one data column (Data::load_timeseries_datatable,
     src/data.cc:1236-1310): populations at the start of a day, transition
     counts over consecutive windows of `ts` days.  sc: the column of its
@@ -585,7 +636,7 @@ def _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, graph, d, oty
             except ValueError:
                 raise TomlError(f"in the file '{file}' the value '{raw}' is not a number") from None
         o = {"y": y, "type": otype, "factor": factor, "datatable": d, "graph": graph, "area": list(areas),
-             "dp": [0], "thresh": float(thresh) if thresh else 0.0}
+             "dp": list(dps), "thresh": float(thresh) if thresh else 0.0}
         if sd is not None:
             o["sd"] = sd
         if percent is not None:
@@ -626,8 +677,28 @@ def load(path, datadir=None, with_data=True):
             raise TomlError("'datadir' is not set")
     rd = _Reader(doc, datadir)
     P = _Params()
-    comps = _compartments(doc, P)
-    trans, dirichlet = _transitions(doc, comps, P)
+    # Age groups (Inputs::find_demographic_category, src/inputs.cc:617-640):
+    # without 'ages' one group whose population column is 'population'.
+    ages, sus_specs = ["population"], None
+    if "ages" in doc:
+        ag = doc["ages"]
+        if not isinstance(ag, dict):
+            raise TomlError("'ages' must be a table with 'cats'")
+        _check_keys(ag, {"cats", "sus_param", "sus_value", "sus_prior"}, "ages")
+        ages = _split(_s(ag.get("cats", "")), "|")
+        if not ages or "" in ages:
+            raise TomlError("in 'ages' the value of 'cats' must list the age groups")
+        if any(k in ag for k in ("sus_param", "sus_value", "sus_prior")):
+            G = len(ages)
+            sus_specs = list(zip(rd.split_n(_s(ag.get("sus_param", "")), G, "ages"),
+                                 rd.split_n(_s(ag.get("sus_value", "")), G, "ages"),
+                                 rd.split_n(_s(ag.get("sus_prior", "")), G, "ages")))
+    G = len(ages)
+    comps = _compartments(rd, doc, P, ages)
+    trans, dirichlet = _transitions(rd, doc, comps, P, ages)
+    sus_par = None
+    if sus_specs is not None:
+        sus_par = [P.add(nm, v, pr) for nm, v, pr in sus_specs]
 
     splines = []
     if len(doc.get("R_spline", [])) != 1:
@@ -647,23 +718,55 @@ def load(path, datadir=None, with_data=True):
         splines.append(_spline(rd, P, "geo_mixing_modify", doc["geo_mixing_modify"][0]))
         geo = len(splines) - 1
 
-    # Areas and the initial population (src/data.cc:288-332,419-457).
+    # Areas and the initial population (src/data.cc:288-332,419-457): one
+    # column per age group, everyone in the infection's source compartment.
     if "areas" not in doc:
         raise TomlError("'areas' must be set")
     head, rows = rd.table(_s(doc["areas"]))
-    if "area" not in head or "population" not in head:
-        raise TomlError("the areas file needs columns 'area' and 'population'")
+    for col in ["area"] + ages:
+        if col not in head:
+            raise TomlError(f"the areas file needs a column '{col}'")
     A = len(rows)
     sus = trans[[t["infection"] for t in trans].index(True)]["from"]
     pop = []
     for row in rows:
+        pa = []
+        for a in ages:
+            try:
+                v = float(row[head.index(a)])
+            except ValueError:
+                raise TomlError(f"the population '{row[head.index(a)]}' is not a number") from None
+            if v < 0:
+                raise TomlError("populations must be non-negative")
+            pa.append(int(v + 0.5))
+        pop.append([pa if c == sus else [0] * G for c in range(len(comps))])
+    pop_age = np.array(pop, dtype=np.float64).sum(axis=(0, 1))
+
+    # Age mixing (Data::age_mixing_matrix, src/data_matrix.cc:138-175): a
+    # G x G table without a heading, normalised later by the host model.
+    N = None
+    if "age_mixing_matrix" in doc:
+        if "|" in _s(doc["age_mixing_matrix"]):
+            raise TomlError("more than one 'age_mixing_matrix' (time-varying mixing) is not supported")
+        _, nrows = rd.table(_s(doc["age_mixing_matrix"]), heading=False)
+        if len(nrows) != G or any(len(r) != G for r in nrows):
+            raise TomlError(f"the age mixing matrix must be {G} x {G} (one row and column per age group)")
         try:
-            popv = float(row[head.index("population")])
+            N = [[float(x) for x in r] for r in nrows]
         except ValueError:
-            raise TomlError(f"the population '{row[head.index('population')]}' is not a number") from None
-        if popv < 0:
-            raise TomlError("populations must be non-negative")
-        pop.append([[int(popv + 0.5) if c == sus else 0] for c in range(len(comps))])
+            raise TomlError("the age mixing matrix holds a value that is not a number") from None
+    elif G != 1:
+        raise TomlError("when more than one age group exists a value for 'age_mixing_matrix' must be set")
+
+    # efoi by age (src/inputs.cc:1421-1442): 'age_dist' normalised by the
+    # population's age distribution, else 1 for every age group.
+    efoi_ad = [1.0] * G
+    if efoi >= 0 and "age_dist" in doc["efoi_spline"][0]:
+        if G <= 1:
+            raise TomlError("in 'efoi_spline' the value of 'age_dist' is not required")
+        ad = [float(x) for x in rd.split_n(_s(doc["efoi_spline"][0]["age_dist"]), G, "efoi_spline")]
+        tot, agedist = sum(ad), pop_age / pop_age.sum()
+        efoi_ad = [ad[a] / (tot * agedist[a]) for a in range(G)]
 
     # Geographic mixing (Data::load_geo_mixing_matrix, src/data_matrix.cc:206-229):
     # an A x A table without a heading; the diagonal and, per row, the
@@ -688,14 +791,13 @@ def load(path, datadir=None, with_data=True):
         raise TomlError("a value for 'geo_mixing_matrix' must be set when there is more than one area")
 
     spec = {
-        "name": os.path.splitext(os.path.basename(path))[0], "A": A, "G": 1, "period": rd.period, "spd": rd.spd,
-        "comps": [{"name": c["name"], "inf": c["inf"], "mean": [c["mean"]] if c["mean"] is not None else None,
-                   "shape": c["shape"] or 1} for c in comps],
+        "name": os.path.splitext(os.path.basename(path))[0], "A": A, "G": G, "period": rd.period, "spd": rd.spd,
+        "comps": [{"name": c["name"], "inf": c["inf"], "mean": c["mean"], "shape": c["shape"] or 1} for c in comps],
         "trans": [{"from": t["from"], "to": t["to"], "infection": t["infection"],
-                   **({"prob": [t["prob"]]} if t["prob"] is not None else {})} for t in trans],
+                   **({"prob": t["prob"]} if t["prob"] is not None else {})} for t in trans],
         "params": P.specs,
-        "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": [1.0],
-        "geo_spline": geo, "N": None, "M": M, "pop": pop, "dirichlet": dirichlet,
-        "obs": _observations(rd, doc, comps, trans, with_data, head, rows),
+        "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": efoi_ad,
+        "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet, "sus": sus_par,
+        "obs": _observations(rd, doc, comps, trans, with_data, head, rows, ages),
     }
     return spec, np.array(P.values), meta

@@ -163,6 +163,31 @@ def test_multi_area():
     assert [(o["area"], o["graph"]) for o in ir[::8]] == [([0], 3), ([1, 2], 4)]
 
 
+def test_age_groups():
+    """Three age groups: 'age:' lists give one parameter per group named
+    '<age> <root>'; one Dirichlet group per age; '*' per age; the contact
+    matrix raw; efoi weights age_dist / (sum * population share)
+    (src/inputs.cc:1421-1442); per-age columns select dp."""
+    import toml_models
+    spec, values = toml_models.toml_sird_ages()
+    names = [p["name"] for p in spec["params"]]
+    assert spec["G"] == 3 and spec["N"] == [[2.0, 0.8, 0.3], [0.8, 1.2, 0.4], [0.3, 0.4, 0.9]]
+    mean = spec["comps"][1]["mean"]
+    assert [names[i] for i in mean] == ["child I mean occupancy", "adult I mean occupancy", "elderly I mean occupancy"]
+    assert [values[i] for i in mean] == [4.0, 5.0, 7.0]
+    pr, pd = spec["trans"][1]["prob"], spec["trans"][2]["prob"]
+    assert [names[i] for i in pd] == ["child I->D branch prob.", "adult I->D branch prob.", "elderly I->D branch prob."]
+    assert spec["dirichlet"] == [{"param": [pr[g], pd[g]]} for g in range(3)]
+    assert [values[i] for i in pd] == pytest.approx([0.001, 0.01, 0.1], abs=1e-15)
+    assert spec["pop"] == [[[24000, 60000, 16000], [0, 0, 0], [0, 0, 0], [0, 0, 0]]]
+    share = np.array([24000, 60000, 16000]) / 100000
+    assert spec["efoi_agedist"] == pytest.approx(list(np.array([1, 2, 1]) / (4 * share)), rel=1e-15)
+    si = [o for o in spec["obs"] if o["datatable"] == 0]
+    assert [(o["dp"], o["graph"]) for o in si[::9]] == [([0], 0), ([1], 1), ([2], 2)]
+    d = [o for o in spec["obs"] if o["datatable"] == 1]
+    assert all(o["dp"] == [0, 1, 2] and o["co"] == [3] for o in d) and len(d) == 21
+
+
 def test_multi_area_needs_matrix(tmp_path):
     d = str(tmp_path)
     _write(d, "areas.csv", "area,population\nX,100\nY,200\n")

@@ -25,3 +25,8 @@ def toml_seird_branch():
 def toml_sir_3areas():
     """SIR over three areas: geographic mixing, mixing spline, per-area and per-region counts."""
     return _load("sir_3areas.toml", "data_sir_3areas")
+
+
+def toml_sird_ages():
+    """SIRD with three age groups: contact matrix, age-dependent periods and branches, efoi by age."""
+    return _load("sird_ages.toml", "data_sird_ages")

@@ -32,7 +32,8 @@ Supported: one or more age groups (`ages`, `age_mixing_matrix`, `age:` parameter
 `democats_dep="age"` tables) and areas (`areas` with population columns, a
 `geo_mixing_matrix` for more than one, `geo_mixing_modify`, `geo_dep` tables),
 `comps` with dist Exp / Erlang, `trans` with branching probabilities,
-`R_spline` and `efoi_spline` (one entry each), `data_tables` of type
+`R_spline` and `efoi_spline` (one entry each), `area_covars`, `tv_covars`
+and `area_tv_covars` (func log or linear), `data_tables` of type
 'transition' or 'population' with obsmodel 'normal <sd>', 'normal <P>%',
 'normal load' or 'poisson'.  Presentation keys are ignored; `modification`,
 `prediction_start` and `prediction_end` act only in the reference's
@@ -59,7 +60,7 @@ IGNORED_KEYS = {"description", "outputdir", "state_outputs", "plot_param_values"
 PREDICTION_KEYS = {"modification", "prediction_start", "prediction_end"}
 SUPPORTED_KEYS = {"time_format", "start", "end", "datadir", "steps_per_unit_time", "comps", "trans", "R_spline",
                   "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str", "geo_mixing_matrix",
-                  "geo_mixing_modify", "ages", "age_mixing_matrix"}
+                  "geo_mixing_modify", "ages", "age_mixing_matrix", "area_covars", "tv_covars", "area_tv_covars"}
 
 
 class TomlError(ValueError):
@@ -79,6 +80,13 @@ def _s(v):
 
 def _split(s, d):
     return [] if s == "" else [x.strip() for x in s.split(d)]
+
+
+def _col(head, name):
+    """Index of a table column, headings compared without case
+    (Data::find_column_noerror, src/data.cc:2080-2086); None when absent."""
+    low = name.lower()
+    return next((i for i, h in enumerate(head) if h.lower() == low), None)
 
 
 class _Reader:
@@ -474,14 +482,77 @@ def _regions(area_head, area_rows, geo_dep):
     by their value in column geo_dep ('NA' leaves an area out)."""
     if geo_dep == "all":
         return [("all", list(range(len(area_rows))))]
-    if geo_dep not in area_head:
+    col = _col(area_head, geo_dep)
+    if col is None:
         raise TomlError(f"'geo_dep={geo_dep}' is not a column of the areas file")
-    col = area_head.index(geo_dep)
     out = {}
     for r, row in enumerate(area_rows):
         if row[col] != "NA":
             out.setdefault(row[col], []).append(r)
     return list(out.items())
+
+
+def _covariates(rd, doc, P, area_head, area_rows):
+    """area_covars / tv_covars / area_tv_covars (Inputs::find_covariates,
+    src/inputs.cc:773-868; Data::read_covars, src/data.cc:461-556): the
+    values per area and day, log-transformed when func='log'.  Centring is
+    done by the host model.  A time-varying table's row r holds days t_r to
+    t_(r+1) inclusive, the next row overwriting the shared day; the last
+    row's own value is not used, as in the reference."""
+    A, period = len(area_rows), rd.period
+    out = []
+    kinds = (("area_covars", "Covariate: "), ("tv_covars", "TV Covariate: "),
+             ("area_tv_covars", "Area TV Covariate: "))
+    for key, prefix in kinds:
+        for e in doc.get(key, []):
+            allowed = {"name", "param", "value", "prior", "func"} | (set() if key == "area_covars" else {"file"})
+            _check_keys(e, allowed, key)
+            name = _s(e.get("name", ""))
+            if name == "":
+                raise TomlError(f"in '{key}' every covariate needs a 'name'")
+            func = _s(e.get("func", ""))
+            if func not in ("log", "linear"):
+                raise TomlError(f"in '{key}' the value 'func={func}' is not recognised")
+            pname = _s(e.get("param", "")) or prefix + name
+            par = P.add(pname, _s(e.get("value", "")), _s(e.get("prior", "")))
+            val = np.zeros((A, period))
+            if key == "area_covars":
+                col = _col(area_head, name)
+                if col is None:
+                    raise TomlError(f"the areas file has no column '{name}' for the covariate")
+                for c, row in enumerate(area_rows):
+                    val[c, :] = float(row[col])
+            else:
+                head, rows = rd.table(_s(e.get("file", "")))
+                tc = _col(head, rd.time_col)
+                if tc is None:
+                    raise TomlError(f"the covariate file '{e.get('file')}' needs a column '{rd.time_col}'")
+                names = [name] if key == "tv_covars" else [r[_col(area_head, "area")] for r in area_rows]
+                cols = [_col(head, nm) for nm in names]
+                for nm, c in zip(names, cols):
+                    if c is None:
+                        raise TomlError(f"the covariate file '{e.get('file')}' has no column '{nm}'")
+                times = [rd.time(r[tc]) for r in rows]
+                if not rows:
+                    raise TomlError(f"the covariate file '{e.get('file')}' has no rows")
+                # One row: no interval, the values stay zero (the reference's
+                # loop over row pairs does not run).
+                if len(rows) > 1 and (times[0] > 0 or times[-1] < period - 1):
+                    raise TomlError(f"the covariate file '{e.get('file')}' must cover the analysis period")
+                for r in range(len(rows) - 1):
+                    ti, tf = times[r], times[r + 1]
+                    if tf <= ti:
+                        raise TomlError(f"in the covariate file '{e.get('file')}' the times must increase")
+                    lo, hi = max(ti, 0), min(tf, period - 1)
+                    for c in range(A):
+                        v = float(rows[r][cols[0 if key == "tv_covars" else c]])
+                        val[c, lo:hi + 1] = v
+            if func == "log":
+                if (val <= 0).any():
+                    raise TomlError(f"the values of covariate '{name}' must be positive for log transformation")
+                val = np.log(val)
+            out.append({"param": par, "value": val.tolist()})
+    return out
 
 
 def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages):
@@ -552,9 +623,9 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages):
         if not with_data:
             continue
         head, rows = rd.table(file)
-        if rd.time_col not in head:
+        tc = _col(head, rd.time_col)
+        if tc is None:
             raise TomlError(f"the file '{file}' needs a column '{rd.time_col}'")
-        tc = head.index(rd.time_col)
         # One column per region with geo_dep, else 'Data' over every area
         # (Data::get_datafilter, src/data.cc:1460-1560); columns absent from
         # the file are skipped, as the reference warns and skips them.
@@ -572,11 +643,11 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages):
             filters = [(name, areas, every_age) for name, areas in _regions(area_head, area_rows, geo_dep)]
         else:
             filters = [("Data", every_area, every_age)]
-        cols = [(name, head.index(name), areas, dps) for name, areas, dps in filters if name in head]
+        cols = [(name, _col(head, name), areas, dps) for name, areas, dps in filters if _col(head, name) is not None]
         if not cols:
             raise TomlError(f"the data in '{file}' was not used: expected columns such as '{filters[0][0]}'")
         for name, _, _, _ in cols:
-            if load_sd and name + " SD" not in head:
+            if load_sd and _col(head, name + " SD") is None:
                 raise TomlError(f"the file '{file}' needs a column '{name} SD' for 'normal load'")
         # Rows in the analysis period (Data::table_select_dates, src/data.cc:1981-2060).
         times = [rd.time(r[tc]) + shift for r in rows]
@@ -606,7 +677,7 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages):
             if times[r1] + ts > rd.period:
                 r1 -= 1
         for name, dc, areas, dps in cols:
-            sc = head.index(name + " SD") if load_sd else None
+            sc = _col(head, name + " SD") if load_sd else None
             graph = ngraph
             ngraph += 1
             obs += _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, dps, graph, d, otype, sd, percent,
@@ -723,19 +794,20 @@ def load(path, datadir=None, with_data=True):
     if "areas" not in doc:
         raise TomlError("'areas' must be set")
     head, rows = rd.table(_s(doc["areas"]))
-    for col in ["area"] + ages:
-        if col not in head:
+    age_cols = [_col(head, a) for a in ages]
+    for col, c in zip(["area"] + ages, [_col(head, "area")] + age_cols):
+        if c is None:
             raise TomlError(f"the areas file needs a column '{col}'")
     A = len(rows)
     sus = trans[[t["infection"] for t in trans].index(True)]["from"]
     pop = []
     for row in rows:
         pa = []
-        for a in ages:
+        for c in age_cols:
             try:
-                v = float(row[head.index(a)])
+                v = float(row[c])
             except ValueError:
-                raise TomlError(f"the population '{row[head.index(a)]}' is not a number") from None
+                raise TomlError(f"the population '{row[c]}' is not a number") from None
             if v < 0:
                 raise TomlError("populations must be non-negative")
             pa.append(int(v + 0.5))
@@ -798,6 +870,7 @@ def load(path, datadir=None, with_data=True):
         "params": P.specs,
         "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": efoi_ad,
         "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet, "sus": sus_par,
+        "covars": _covariates(rd, doc, P, head, rows),
         "obs": _observations(rd, doc, comps, trans, with_data, head, rows, ages),
     }
     return spec, np.array(P.values), meta

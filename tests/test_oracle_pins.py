@@ -96,6 +96,29 @@ def test_normalisation_identities():
     assert sum(pop[aa] * N[aa, a] for a in range(G) for aa in range(G)) / (G * pop.sum()) == pytest.approx(1.0)
 
 
+def test_area_factor_from_raw_covariates():
+    """Model::create_areafactor (src/model.cc:1182-1249) with two covariates
+    on three areas over 56 days, recomputed by loops from the uncentred
+    values: the centring of Data::read_covars is a constant factor that the
+    population-weighted normalisation removes; 3 areas x 56 days also catches
+    a transposed table."""
+    import toml_models
+    spec, truth = toml_models.toml_sir_3areas()
+    nz = derive.normalise(spec)
+    A, T = spec["A"], spec["period"]
+    pop = [sum(sum(c) for c in spec["pop"][a]) for a in range(A)]
+    for pv in (truth, truth * np.where(np.arange(len(truth)) == spec["covars"][1]["param"], -2.0, 1.0)):
+        raw = np.ones((T, A))
+        for t in range(T):
+            for c in range(A):
+                for cv in spec["covars"]:
+                    raw[t, c] *= np.exp(pv[cv["param"]] * cv["value"][c][t])
+        av = sum(pop[c] * sum(raw[t, c] for t in range(T)) / T for c in range(A)) / sum(pop)
+        af = derive.derive(spec, nz, pv)["af"]
+        assert af.shape == (T, A)
+        assert np.allclose(af, raw / av, rtol=1e-13, atol=0)
+
+
 def test_deterministic_seir_is_forward_euler():
     """Deterministic dynamics with one area: the forward-Euler SEIR recursion
     with FOI = beta I / N (Idiag = I / N when A = 1)."""

@@ -161,6 +161,44 @@ def test_multi_area():
     assert len(si) == 3 * 8 and len(ir) == 2 * 8
     assert [(o["area"], o["graph"]) for o in si[::8]] == [([0], 0), ([1], 1), ([2], 2)]
     assert [(o["area"], o["graph"]) for o in ir[::8]] == [([0], 3), ([1, 2], 4)]
+    # Covariates (src/data.cc:461-556): the areas file's 'Density' column
+    # (headings match without case), log-transformed; area time-varying
+    # values from rows at days 0, 20, 35, 56, each row holding its days up to
+    # and including the next row's, the next row overwriting the shared day.
+    dens, mob = spec["covars"]
+    assert names[dens["param"]] == "Covariate: density" and values[dens["param"]] == 0.3
+    assert names[mob["param"]] == "mobility effect" and values[mob["param"]] == -0.5
+    assert np.array_equal(dens["value"], np.log([[0.5] * 56, [4.0] * 56, [1.5] * 56]))
+    want = [[v0] * 20 + [v1] * 15 + [v2] * 21 for v0, v1, v2 in ((1.0, 0.6, 0.8), (1.2, 0.5, 0.9), (0.9, 0.7, 0.85))]
+    assert np.array_equal(mob["value"], want)
+
+
+def test_time_varying_covariate(tmp_path):
+    """tv_covars with a 'Date' heading: the table's dates relative to start,
+    one value for every area; a log transform needs positive values; a
+    one-row table leaves the values zero, as the reference's loop over row
+    pairs does not run."""
+    d = str(tmp_path)
+    p = _base_model(d, "m.toml", 'prob_value="0.8", prob_prior="Uniform(0.5,1)"',
+                    'R_spline = [{value="2", prior="Uniform(1,3)"}]')
+    _write(d, "tv.csv", "Date,temp\n2020-02-28,2\n2020-03-05,4\n2020-03-30,8\n")
+    text = open(p).read() + 'tv_covars = [{name="temp", file="tv.csv", func="log", value="0.2", prior="Uniform(0,1)"}]\n'
+    spec, values, _ = T.load(_write(d, "a.toml", text), datadir=d)
+    (cv,) = spec["covars"]
+    assert [p["name"] for p in spec["params"]][cv["param"]] == "TV Covariate: temp" and values[cv["param"]] == 0.2
+    # Days -2..4 from the first row, 4..29 from the second, clipped to 0..27.
+    assert np.array_equal(cv["value"], [np.log([2.0] * 4 + [4.0] * 24)])
+    _write(d, "tv.csv", "Date,temp\n2020-02-28,2\n2020-03-05,-1\n2020-03-30,8\n")
+    with pytest.raises(T.TomlError, match="positive"):
+        T.load(_write(d, "b.toml", text), datadir=d)
+    _write(d, "tv.csv", "Date,temp\n2020-03-02,2\n2020-03-30,8\n")
+    with pytest.raises(T.TomlError, match="cover the analysis period"):
+        T.load(_write(d, "c.toml", text), datadir=d)
+    _write(d, "tv.csv", "Date,temp\n2020-03-05,3\n")
+    spec, _, _ = T.load(_write(d, "d.toml", text.replace('func="log"', 'func="linear"')), datadir=d)
+    assert np.array_equal(spec["covars"][0]["value"], [[0.0] * 28])
+    with pytest.raises(T.TomlError, match="func=exp"):
+        T.load(_write(d, "e.toml", text.replace('func="log"', 'func="exp"')), datadir=d)
 
 
 def test_age_groups():

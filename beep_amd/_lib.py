@@ -84,7 +84,9 @@ class Spec(C.Structure):
                 ("pop_init", C.POINTER(C.c_int32)), ("ncovar", C.c_int32), ("covar", C.POINTER(Covar)),
                 ("sus_param", C.POINTER(C.c_int32)), ("ndirichlet", C.c_int32), ("dirichlet", C.POINTER(Dirichlet)),
                 ("nobs", C.c_int32), ("obs", C.POINTER(Obs)), ("nsection", C.c_int32),
-                ("section_ti", C.POINTER(C.c_int32)), ("section_tf", C.POINTER(C.c_int32))]
+                ("section_ti", C.POINTER(C.c_int32)), ("section_tf", C.POINTER(C.c_int32)),
+                ("area_effect_param", C.POINTER(C.c_int32)), ("nlevel", C.c_int32),
+                ("level_param", C.POINTER(C.c_int32)), ("level_map", C.POINTER(C.c_int32))]
 
 
 _lib = None
@@ -279,6 +281,14 @@ class _SpecBuffers:
         s.nsection = len(secs)
         s.section_ti = i32([a for a, b in secs]) if secs else null_i32
         s.section_tf = i32([b for a, b in secs]) if secs else null_i32
+        ae = spec.get("area_effect")
+        s.area_effect_param = i32(ae) if ae is not None else null_i32
+        le = spec.get("level_effect")
+        if le is not None:
+            s.nlevel, s.level_param = len(le["param"]), i32(le["param"])
+            s.level_map = i32(np.asarray(le["map"]).reshape(-1))
+        else:
+            s.nlevel, s.level_param, s.level_map = 0, null_i32, null_i32
         self.spec = s
 
 

@@ -88,6 +88,10 @@ std::string HostModel::build(const beep_spec& s) {
   dt = (double)period / S;                                   // src/state.cc:384
   nparam = s.nparam;
   prior.assign(s.prior, s.prior + nparam);
+  for (const beep_prior& p : prior) {
+    if (p.type < BEEP_PRIOR_FIXED || p.type > BEEP_PRIOR_GAMMA) return "unknown prior type";
+    if (p.type == BEEP_PRIOR_GAMMA && (p.a <= 0 || p.b <= 0)) return "gamma prior needs a positive shape and rate";
+  }
 
   comp.resize(C);
   for (int c = 0; c < C; c++) {
@@ -232,6 +236,23 @@ std::string HostModel::build(const beep_spec& s) {
     for (double v : covar[k].value) av += v;
     av /= (double)A * period;
     for (double& v : covar[k].value) v -= av;
+    if (covar[k].param < 0 || covar[k].param >= nparam) return "covariate parameter out of range";
+  }
+  area_effect_param.clear();
+  if (s.area_effect_param) {
+    area_effect_param.assign(s.area_effect_param, s.area_effect_param + A);
+    for (int th : area_effect_param)
+      if (th < 0 || th >= nparam) return "area effect parameter out of range";
+  }
+  level_param.clear(); level_map.clear();
+  if (s.nlevel > 0) {
+    if (!s.level_param || !s.level_map) return "level effects need their parameters and map";
+    level_param.assign(s.level_param, s.level_param + s.nlevel);
+    level_map.assign(s.level_map, s.level_map + (size_t)period * A);
+    for (int th : level_param)
+      if (th < 0 || th >= nparam) return "level effect parameter out of range";
+    for (int l : level_map)
+      if (l < 0 || l >= s.nlevel) return "level effect map out of range";
   }
 
   // Infected states (Model::set_infected_uninfected, src/model.cc:110-166).
@@ -380,11 +401,16 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
 
   // Model::create_areafactor (src/model.cc:1182-1250).
   std::vector<double> af((size_t)period * A, 1.0);
+  if (!area_effect_param.empty())
+    for (int t = 0; t < period; t++)
+      for (int c = 0; c < A; c++) af[(size_t)t * A + c] = pd[area_effect_param[c]];
   for (const auto& cv : covar) {
     double b = pd[cv.param];
     for (int t = 0; t < period; t++)
       for (int c = 0; c < A; c++) af[(size_t)t * A + c] *= std::exp(b * cv.value[(size_t)c * period + t]);
   }
+  if (!level_param.empty())
+    for (size_t i = 0; i < af.size(); i++) af[i] *= pd[level_param[level_map[i]]];
   {
     double area_av = 0;
     for (int c = 0; c < A; c++) {
@@ -466,6 +492,9 @@ double HostModel::prior_logpdf(const double* pv) const {   // Model::prior (src/
         Pr += -0.5 * std::log(2 * M_PI * p.b * p.b) - (pv[th] - p.a) * (pv[th] - p.a) / (2 * p.b * p.b);
         break;
       case BEEP_PRIOR_DIRFLAT: Pr += -pv[th]; break;
+      case BEEP_PRIOR_GAMMA:                                 // gamma_probability, src/utils.cc:115-120
+        Pr += (p.a - 1) * std::log(pv[th]) - p.b * pv[th] + p.a * std::log(p.b) - std::lgamma(p.a);
+        break;
       default: break;
     }
   }
@@ -498,7 +527,7 @@ bool HostModel::inbounds(const double* pv) const {         // Model::inbounds (s
     switch (p.type) {
       case BEEP_PRIOR_FIXED: if (v != p.a) return false; break;
       case BEEP_PRIOR_UNIFORM: if (v < p.a || v > p.b) return false; break;
-      case BEEP_PRIOR_EXP: case BEEP_PRIOR_DIRFLAT: if (v < 0) return false; break;
+      case BEEP_PRIOR_EXP: case BEEP_PRIOR_DIRFLAT: case BEEP_PRIOR_GAMMA: if (v < 0) return false; break;
       default: break;
     }
   }

@@ -52,6 +52,8 @@ class HostModel {
   std::vector<Spline> spline;
   std::vector<Dirichlet> dirichlet;
   std::vector<Covar> covar;
+  std::vector<int> area_effect_param;                         // [A], empty = none
+  std::vector<int> level_param, level_map;                    // [nlevel], [period][A]
   std::vector<int> sus_param;
   int R_spline = -1, Rfactor_param = -1, efoi_spline = -1, geo_spline = -1;
   std::vector<double> efoi_agedist;

@@ -9,13 +9,15 @@ A spec is a plain dict (JSON-serialisable once arrays are lists):
   trans         [{"from", "to", "infection": bool, "prob": [G] params or None}]
                 internal Erlang transitions first (src/inputs_comps.cc:77-88)
   params        [{"name", "prior": ("fixed", v) | ("uniform", lo, hi) | ("exp", rate)
-                  | ("normal", mu, sd) | ("dirflat",)}]
+                  | ("normal", mu, sd) | ("dirflat",) | ("gamma", shape, rate)}]
   splines       [{"t": [...], "param": [...], "multfac": [...], "factor": param or -1}]
   R_spline, Rfactor, efoi_spline, efoi_agedist, geo_spline
   N             [G][G] age contact table or None (unit matrix for G == 1)
   M             {"diag": [A], "rowptr": [A+1], "to": [nnz], "val": [nnz]} raw, or None for A == 1
   pop           [A][C][G] initial populations
   covars        [{"param", "value": [A][period]}] (centred on load)
+  area_effect   [A] params multiplying each area's factor, or None
+  level_effect  {"param": [nlevel], "map": [period][A] level index}, or None
   sus           [G] params or None
   dirichlet     [{"param": [...], "frac": [...]}]
   obs           [{"y", "sd", "type", "shape", "factor", "thresh", "percent", "datatable",
@@ -26,7 +28,7 @@ A spec is a plain dict (JSON-serialisable once arrays are lists):
 """
 import numpy as np
 
-PRIOR_CODES = {"fixed": 0, "uniform": 1, "exp": 2, "normal": 3, "dirflat": 4}
+PRIOR_CODES = {"fixed": 0, "uniform": 1, "exp": 2, "normal": 3, "dirflat": 4, "gamma": 5}
 OBS_CODES = {"normal": 0, "normal_percent": 1, "poisson": 2, "negbin": 3}
 OBS_UNKNOWN = 999999997.0
 OBS_THRESH = 999999998.0
@@ -65,6 +67,8 @@ def sample_prior(spec, rng, n):
             out[:, i] = rng.normal(pr[1], pr[2], n)
         elif kind == "dirflat":
             out[:, i] = rng.gamma(1.0, 1.0, n)
+        elif kind == "gamma":
+            out[:, i] = rng.gamma(pr[1], 1.0 / pr[2], n)
         else:
             raise ValueError(kind)
     return out
@@ -83,6 +87,8 @@ def prior_mid(spec):
             v.append(1.0 / pr[1])
         elif pr[0] == "normal":
             v.append(pr[1])
+        elif pr[0] == "gamma":
+            v.append(pr[1] / pr[2])
         else:
             v.append(1.0)
     return np.array(v, dtype=np.float64)

@@ -33,7 +33,8 @@ Supported: one or more age groups (`ages`, `age_mixing_matrix`, `age:` parameter
 `geo_mixing_matrix` for more than one, `geo_mixing_modify`, `geo_dep` tables),
 `comps` with dist Exp / Erlang, `trans` with branching probabilities,
 `R_spline` and `efoi_spline` (one entry each), `area_covars`, `tv_covars`
-and `area_tv_covars` (func log or linear), `data_tables` of type
+and `area_tv_covars` (func log or linear), `area_effect` and `level_effect`
+(fixed or MDir(k) priors), `data_tables` of type
 'transition' or 'population' with obsmodel 'normal <sd>', 'normal <P>%',
 'normal load' or 'poisson'.  Presentation keys are ignored; `modification`,
 `prediction_start` and `prediction_end` act only in the reference's
@@ -60,7 +61,8 @@ IGNORED_KEYS = {"description", "outputdir", "state_outputs", "plot_param_values"
 PREDICTION_KEYS = {"modification", "prediction_start", "prediction_end"}
 SUPPORTED_KEYS = {"time_format", "start", "end", "datadir", "steps_per_unit_time", "comps", "trans", "R_spline",
                   "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str", "geo_mixing_matrix",
-                  "geo_mixing_modify", "ages", "age_mixing_matrix", "area_covars", "tv_covars", "area_tv_covars"}
+                  "geo_mixing_modify", "ages", "age_mixing_matrix", "area_covars", "tv_covars", "area_tv_covars",
+                  "area_effect", "level_effect"}
 
 
 class TomlError(ValueError):
@@ -424,6 +426,131 @@ def _branches(trans, comps, P, G):
             for k, t in enumerate(ts):
                 t["prob"][g] = idx[k]
     return groups
+
+
+def _modified_dirichlet(P, names, vals, priors, frac, where):
+    """A modified Dirichlet group (Model::add_dirichlet with DIR_MODIFIED,
+    src/model.cc:2699-2730,2838-2878,3025-3050): one parameter per distinct
+    name, its weight the sum of its items' fractions.  Values are on the
+    corrected scale: one must be '*', set so that sum(value * frac) = 1, and
+    the returned values are value * frac (Model::simulated_values,
+    src/model.cc:990-1003), which Model::dirichlet_correct maps back.  Priors
+    are all fixed or all MDir(k), the latter giving each parameter a
+    gamma(beta * frac, 1) prior with beta = (n - 1) / k^2 - 1.  Returns the
+    parameter index of each item and the group."""
+    order, info = [], {}
+    for nm, v, pr, f in zip(names, vals, priors, frac):
+        if nm in info:
+            if info[nm][:2] != [v, pr]:
+                raise TomlError(f"{where}: parameter '{nm}' has multiple definitions")
+            info[nm][2] += f
+        else:
+            order.append(nm)
+            info[nm] = [v, pr, f]
+    n = len(order)
+    mdir = [info[nm][1].replace(" ", "").lower().startswith("mdir(") for nm in order]
+    fixed = [info[nm][1] == "" for nm in order]
+    if not (all(mdir) or all(fixed)):
+        raise TomlError(f"{where}: the priors must all be MDir(k) or all fixed (a value and no prior)")
+    raw = [math.nan] * n
+    if info[order[0]][0] != "":
+        star = [i for i, nm in enumerate(order) if info[nm][0] == "*"]
+        if len(star) != 1:
+            raise TomlError(f"{where}: exactly one of the values must be '*'")
+        tot = 0.0
+        for i, nm in enumerate(order):
+            if i == star[0]:
+                continue
+            try:
+                raw[i] = float(info[nm][0]) * info[nm][2]
+            except ValueError:
+                raise TomlError(f"{where}: the value '{info[nm][0]}' is not a number") from None
+            tot += raw[i]
+        if tot > 1:
+            raise TomlError(f"{where}: the value '*' cannot be set (the weighted values sum to more than one)")
+        raw[star[0]] = 1.0 - tot
+    if all(mdir):
+        ks = {info[nm][1].replace(" ", "")[5:-1] for nm in order}
+        try:
+            kappa = float(info[order[0]][1].replace(" ", "")[5:-1])
+        except ValueError:
+            raise TomlError(f"{where}: the prior '{info[order[0]][1]}' is not recognised") from None
+        if len(ks) != 1:
+            raise TomlError(f"{where}: the MDir priors must share one argument")
+        beta = (n - 1) / (kappa * kappa) - 1
+        prs = []
+        for nm in order:
+            if beta * info[nm][2] < 0.1:
+                raise TomlError(f"{where}: the prior '{info[nm][1]}' is too diffuse")
+            prs.append(("gamma", beta * info[nm][2], 1.0))
+    else:
+        if any(math.isnan(r) for r in raw):
+            raise TomlError(f"{where}: a value must be set for every parameter")
+        prs = [("fixed", r) for r in raw]
+    idx = {nm: P.add_tuple(nm, raw[i], prs[i]) for i, nm in enumerate(order)}
+    return [idx[nm] for nm in names], {"param": [idx[nm] for nm in order], "frac": [info[nm][2] for nm in order]}
+
+
+def _effects(rd, doc, P, area_head, area_rows, pop):
+    """area_effect (Inputs::find_area_effect, src/inputs.cc:1075-1102: one
+    parameter per area, default names 'Area effect <code>', weights the areas'
+    population shares) and level_effect (Inputs::find_level_effect,
+    src/inputs.cc:746-768; Data::read_level_effect, src/data.cc:571-623: a
+    table of level names per day and area, each row holding from its day
+    until the next row's, level 0 before the first row; weights the levels'
+    shares of area-days).  Both form modified Dirichlet groups.  Returns the
+    area effect parameters, the level effect and the groups."""
+    A, period = len(area_rows), rd.period
+    codes = [r[_col(area_head, "area")] for r in area_rows]
+    groups, area_eff, level = [], None, None
+    if "area_effect" in doc:
+        e = doc["area_effect"]
+        _check_keys(e, {"param", "value", "prior"}, "area_effect")
+        _no_dep(e, ("param", "value", "prior"), "area_effect")
+        names = rd.split_n(_s(e.get("param", "")), A, "area_effect")
+        if not _s(e.get("param", "")):
+            names = ["Area effect " + c for c in codes]
+        vals = rd.split_n(_s(e.get("value", "")), A, "area_effect")
+        priors = rd.split_n(_s(e.get("prior", "")), A, "area_effect")
+        tot = [sum(sum(g) for g in pa) for pa in pop]
+        frac = [t / sum(tot) for t in tot]
+        area_eff, grp = _modified_dirichlet(P, names, vals, priors, frac, "area_effect")
+        groups.append(grp)
+    if "level_effect" in doc:
+        e = doc["level_effect"]
+        _check_keys(e, {"file", "param", "value", "prior"}, "level_effect")
+        names = _split(_s(e.get("param", "")), "|")
+        if not names:
+            raise TomlError("in 'level_effect' a value for 'param' must be set")
+        n = len(names)
+        vals = rd.split_n(_s(e.get("value", "")), n, "level_effect")
+        priors = rd.split_n(_s(e.get("prior", "")), n, "level_effect")
+        head, rows = rd.table(_s(e.get("file", "")))
+        tc = _col(head, rd.time_col)
+        if tc is None:
+            raise TomlError(f"the level effect file needs a column '{rd.time_col}'")
+        cols = [_col(head, c) for c in codes]
+        if None in cols:
+            raise TomlError(f"the level effect file has no column '{codes[cols.index(None)]}'")
+        lmap, cur, t = [], [0] * A, 0
+        for r in range(len(rows) + 1):
+            t_new = period if r == len(rows) else rd.time(rows[r][tc])
+            if t_new < t or t_new > period:
+                raise TomlError("in the level effect file the dates are out of the analysis range")
+            lmap += [list(cur)] * (t_new - t)
+            t = t_new
+            if r == len(rows):
+                break
+            for c in range(A):
+                v = rows[r][cols[c]]
+                if v not in names:
+                    raise TomlError(f"in 'level_effect' the value '{v}' is not contained in 'param'")
+                cur[c] = names.index(v)
+        cnt = np.bincount(np.asarray(lmap).reshape(-1), minlength=n)
+        idx, grp = _modified_dirichlet(P, names, vals, priors, [float(x) / (period * A) for x in cnt], "level_effect")
+        level = {"param": idx, "map": lmap}
+        groups.append(grp)
+    return area_eff, level, groups
 
 
 def _spline(rd, P, key, entry):
@@ -813,6 +940,7 @@ def load(path, datadir=None, with_data=True):
             pa.append(int(v + 0.5))
         pop.append([pa if c == sus else [0] * G for c in range(len(comps))])
     pop_age = np.array(pop, dtype=np.float64).sum(axis=(0, 1))
+    area_eff, level, eff_groups = _effects(rd, doc, P, head, rows, pop)
 
     # Age mixing (Data::age_mixing_matrix, src/data_matrix.cc:138-175): a
     # G x G table without a heading, normalised later by the host model.
@@ -869,8 +997,8 @@ def load(path, datadir=None, with_data=True):
                    **({"prob": t["prob"]} if t["prob"] is not None else {})} for t in trans],
         "params": P.specs,
         "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": efoi_ad,
-        "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet, "sus": sus_par,
-        "covars": _covariates(rd, doc, P, head, rows),
+        "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet + eff_groups, "sus": sus_par,
+        "covars": _covariates(rd, doc, P, head, rows), "area_effect": area_eff, "level_effect": level,
         "obs": _observations(rd, doc, comps, trans, with_data, head, rows, ages),
     }
     return spec, np.array(P.values), meta

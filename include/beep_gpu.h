@@ -49,6 +49,9 @@ extern "C" {
 #define BEEP_PRIOR_EXP      2   /* a = rate                          */
 #define BEEP_PRIOR_NORMAL   3   /* a = mean, b = sd                  */
 #define BEEP_PRIOR_DIRFLAT  4   /* flat Dirichlet weight, Pr = -value */
+#define BEEP_PRIOR_GAMMA    5   /* a = shape, b = rate: the MDir(k) weights of a modified
+                                   Dirichlet group (src/model.cc:3025-3050), gamma_probability
+                                   (src/utils.cc:115-120)                                    */
 
 typedef struct { int32_t type; double a, b; } beep_prior;
 
@@ -144,6 +147,12 @@ typedef struct {
   int32_t nsection;             /* PMCMC observation sections (0 = none); section k */
   const int32_t* section_ti;    /* covers steps [section_ti[k], section_tf[k])        */
   const int32_t* section_tf;    /* (ObservationModel::split_observations, 326-348)    */
+  /* Further area-factor terms (Model::create_areafactor, src/model.cc:1182-1216). */
+  const int32_t* area_effect_param; /* [A] parameter multiplying area c's factor
+                                   (area_effect), NULL = none                          */
+  int32_t nlevel;               /* level effects (level_effect), 0 = none             */
+  const int32_t* level_param;   /* [nlevel] parameter of each level                   */
+  const int32_t* level_map;     /* [period][A] level of area c on day t               */
 } beep_spec;
 
 /* Host only (no device is touched): State::set_param's FP64 tables for one

@@ -19,13 +19,19 @@ Restates, for one strain with age as the only demographic category:
   Model::calculate_beta_from_R      src/model.cc:1318-1362 with calculate_Vinv,
                                     calculate_F, calculate_NGM (2485-2576) and
                                     largest_eigenvalue (src/matrix.cc:1139-1219)
-  Model::prior / inbounds           src/model.cc:1590-1659, 2625-2669
+  Model::prior / inbounds           src/model.cc:1590-1659, 2625-2669, with
+                                    gamma_probability (src/utils.cc:115-120)
+                                    for MDir weights
 
-Pins: beta-from-R against the SEIR closed form rho = mean infectious period
+Pins: the gamma prior against the reference's own gamma_probability (golden
+vectors) and scipy; the area factor against loops over effects, levels and
+uncentred covariates; beta-from-R against the SEIR closed form rho = mean infectious period
 and against numpy.linalg.eig of the same next-generation matrix; the
 normalisations against their defining identities (row sums of the per-capita
 mixing equal one; population-weighted age mixing averages one).
 """
+import math
+
 import numpy as np
 
 EPSILON_FACTOR = 0.01
@@ -200,8 +206,13 @@ def derive(spec, nz, pv):
     sus = np.array([pd[p] if p >= 0 else 1.0 for p in spec["sus"]]) if spec.get("sus") is not None else np.ones(G)
     inf = np.array([pd[c["inf"]] if c.get("inf", -1) >= 0 else 0.0 for c in comps])
     af = np.ones((period, A))
+    if spec.get("area_effect") is not None:
+        af *= pd[np.asarray(spec["area_effect"])][None, :]
     for par, v in nz["covars"]:
         af *= np.exp(pd[par] * v.T)
+    le = spec.get("level_effect")
+    if le is not None:
+        af *= pd[np.asarray(le["param"])][np.asarray(le["map"])]
     area_av = (af.mean(axis=0) * nz["area_pop"]).sum() / nz["popsize"]
     af /= area_av
     ratio = largest_eigenvalue(ngm(spec, nz, rate, sus, inf))
@@ -231,6 +242,9 @@ def prior(spec, pv):
             Pr += -0.5 * np.log(2 * np.pi * pr[2] ** 2) - (v - pr[1]) ** 2 / (2 * pr[2] ** 2)
         elif pr[0] == "dirflat":
             Pr += -v
+        elif pr[0] == "gamma":          # gamma_probability, src/utils.cc:115-120
+            a, b = pr[1], pr[2]
+            Pr += (a - 1) * np.log(v) - b * v + a * np.log(b) - math.lgamma(a)
     return Pr + spline_prior(spec, pv)
 
 
@@ -273,7 +287,7 @@ def inbounds(spec, pv):
             return False
         if pr[0] == "uniform" and (v < pr[1] or v > pr[2]):
             return False
-        if pr[0] in ("exp", "dirflat") and v < 0:
+        if pr[0] in ("exp", "dirflat", "gamma") and v < 0:
             return False
     return True
 

@@ -389,5 +389,7 @@ def ref_lib():
         L.ref_poisson_probability.restype = C.c_double
         L.ref_negative_binomial_probability.argtypes = [C.c_uint, C.c_double, C.c_double]
         L.ref_negative_binomial_probability.restype = C.c_double
+        L.ref_gamma_probability.argtypes = [C.c_double] * 3
+        L.ref_gamma_probability.restype = C.c_double
         _ref = L
     return _ref

@@ -10,6 +10,7 @@ unsigned int binomial_sample(const double ratio, const unsigned int nn);
 double normal_probability(const double x, const double mean, const double var);
 double poisson_probability(const int i, const double lam);
 double negative_binomial_probability(const unsigned int val, const double m, const double shape);
+double gamma_probability(const double x, const double a, const double b);
 
 extern "C" {
 void ref_poisson_stream(int seed, double lam, int n, int* out) {
@@ -25,4 +26,5 @@ double ref_poisson_probability(int i, double lam) { return poisson_probability(i
 double ref_negative_binomial_probability(unsigned v, double m, double shape) {
   return negative_binomial_probability(v, m, shape);
 }
+double ref_gamma_probability(double x, double a, double b) { return gamma_probability(x, a, b); }
 }

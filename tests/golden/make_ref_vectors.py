@@ -1,7 +1,7 @@
 """Writes tests/golden/ref_utils_vectors.json: sample streams and log-pmf
 values produced by the reference's own src/utils.cc (poisson_sample,
 binomial_sample after sran(seed); normal_probability, poisson_probability,
-negative_binomial_probability), compiled unmodified into oracle/_ref by
+negative_binomial_probability, gamma_probability), compiled unmodified into oracle/_ref by
 oracle/Makefile.  Run in a container that has /root/reference:
 
     make -C oracle ref && python tests/golden/make_ref_vectors.py
@@ -24,6 +24,7 @@ BINOM = [(11, 0.3, 5), (12, 0.02, 1000), (13, 0.5, 100), (14, 0.93, 4000), (15, 
 NORMAL = [(3.0, 2.5, 4.0), (0.0, 100.0, 25.0), (1e5, 99990.0, 1e4)]
 POISP = [(0, 0.0), (3, 0.0), (0, 2.5), (7, 2.5), (120, 98.4), (-1, 3.0)]
 NEGBIN = [(0, 1.0, 2.0), (5, 3.7, 0.5), (40, 55.2, 10.0), (1000, 1234.5, 3.3)]
+GAMMA = [(0.37, 1.0, 1.0), (1.4, 3.2, 1.0), (0.05, 0.45, 1.0), (7.5, 12.0, 2.5), (2e-3, 0.11, 1.0)]
 
 
 def main(n=64):
@@ -31,7 +32,7 @@ def main(n=64):
     if L is None:
         raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
     out = {"poisson_streams": [], "binomial_streams": [], "normal_probability": [], "poisson_probability": [],
-           "negative_binomial_probability": []}
+           "negative_binomial_probability": [], "gamma_probability": []}
     for seed, lam in POIS:
         a = np.zeros(n, dtype=np.int32)
         L.ref_poisson_stream(seed, lam, n, a.ctypes.data_as(C.POINTER(C.c_int)))
@@ -47,6 +48,8 @@ def main(n=64):
     for v, m, shape in NEGBIN:
         out["negative_binomial_probability"].append(
             {"val": v, "m": m, "shape": shape, "value": L.ref_negative_binomial_probability(v, m, shape)})
+    for x, a, b in GAMMA:
+        out["gamma_probability"].append({"x": x, "a": a, "b": b, "value": L.ref_gamma_probability(x, a, b)})
     with open(os.path.join(HERE, "ref_utils_vectors.json"), "w") as f:
         json.dump(out, f, separators=(",", ":"))
 

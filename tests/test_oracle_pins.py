@@ -43,6 +43,20 @@ def test_logpmfs_match_reference_vectors_and_scipy():
         assert v == pytest.approx(stats.nbinom.logpmf(g["val"], r, r / (r + m)), rel=1e-10)
 
 
+def test_gamma_prior_matches_reference_vectors_and_scipy():
+    """The MDir weights' gamma prior (gamma_probability, src/utils.cc:115-120)
+    in the oracle's Model::prior: one parameter per golden case, the other
+    parameter fixed, so Pr is that case's density alone."""
+    for g in GOLD["gamma_probability"]:
+        spec = {"params": [{"name": "w", "prior": ("gamma", g["a"], g["b"])}, {"name": "f", "prior": ("fixed", 2.0)}],
+                "splines": []}
+        v = derive.prior(spec, np.array([g["x"], 2.0]))
+        assert v == pytest.approx(g["value"], rel=1e-14, abs=1e-14)
+        assert v == pytest.approx(stats.gamma.logpdf(g["x"], g["a"], scale=1.0 / g["b"]), rel=1e-12)
+        assert derive.inbounds(spec, np.array([g["x"], 2.0]))
+    assert not derive.inbounds(spec, np.array([-1e-9, 2.0]))
+
+
 def test_live_reference_build_when_present():
     L = O.ref_lib()
     if L is None:
@@ -96,21 +110,34 @@ def test_normalisation_identities():
     assert sum(pop[aa] * N[aa, a] for a in range(G) for aa in range(G)) / (G * pop.sum()) == pytest.approx(1.0)
 
 
-def test_area_factor_from_raw_covariates():
-    """Model::create_areafactor (src/model.cc:1182-1249) with two covariates
-    on three areas over 56 days, recomputed by loops from the uncentred
-    values: the centring of Data::read_covars is a constant factor that the
-    population-weighted normalisation removes; 3 areas x 56 days also catches
-    a transposed table."""
+def test_area_factor_by_loops():
+    """Model::create_areafactor (src/model.cc:1182-1249) on three areas over
+    56 days with an area effect, a level effect and two covariates, recomputed
+    by loops: effects normalised within their group as
+    value / (frac * sum of values) with frac the areas' population shares and
+    the levels' shares of area-days (src/model.cc:2685-2695, src/inputs.cc:
+    1091-1098, src/data.cc:613-622), counted here from the map; covariates
+    uncentred, since the centring of Data::read_covars is a constant factor
+    that the population-weighted normalisation removes; 3 areas x 56 days
+    also catches a transposed table."""
     import toml_models
     spec, truth = toml_models.toml_sir_3areas()
     nz = derive.normalise(spec)
     A, T = spec["A"], spec["period"]
     pop = [sum(sum(c) for c in spec["pop"][a]) for a in range(A)]
-    for pv in (truth, truth * np.where(np.arange(len(truth)) == spec["covars"][1]["param"], -2.0, 1.0)):
+    ae, le = spec["area_effect"], spec["level_effect"]
+    nlev = len(le["param"])
+    days = [sum(le["map"][t][c] == k for t in range(T) for c in range(A)) for k in range(nlev)]
+    rng = np.random.default_rng(5)
+    for pv in [truth] + [truth * rng.uniform(0.5, 1.5, len(truth)) for _ in range(2)]:
+        sa = sum(pv[ae[c]] for c in range(A))
+        area = [pv[ae[c]] / (pop[c] / sum(pop) * sa) for c in range(A)]
+        sl = sum(pv[le["param"][k]] for k in range(nlev))
+        lev = [pv[le["param"][k]] / (days[k] / (T * A) * sl) for k in range(nlev)]
         raw = np.ones((T, A))
         for t in range(T):
             for c in range(A):
+                raw[t, c] = area[c] * lev[le["map"][t][c]]
                 for cv in spec["covars"]:
                     raw[t, c] *= np.exp(pv[cv["param"]] * cv["value"][c][t])
         av = sum(pop[c] * sum(raw[t, c] for t in range(T)) / T for c in range(A)) / sum(pop)

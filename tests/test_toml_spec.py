@@ -173,6 +173,55 @@ def test_multi_area():
     assert np.array_equal(mob["value"], want)
 
 
+def test_area_and_level_effects():
+    """area_effect and level_effect of the three-area fixture, worked by hand:
+    weights are the areas' population shares (30/155, 80/155, 45/155) and the
+    levels' shares of the 3 x 56 area-days (closed: Centre and South on days
+    21-41, South on days 42-55, 56 of 168); '*' is set so that the weighted
+    values sum to one; values are returned times their weight
+    (src/model.cc:990-1003, 2838-2878); MDir(k) gives gamma(beta * frac, 1)
+    with beta = (n - 1) / k^2 - 1 (src/model.cc:3025-3050)."""
+    import toml_models
+    spec, values = toml_models.toml_sir_3areas()
+    names = [p["name"] for p in spec["params"]]
+    ae, le = spec["area_effect"], spec["level_effect"]
+    assert [names[i] for i in ae] == ["Area effect North", "Area effect Centre", "Area effect South"]
+    f = [30 / 155, 80 / 155, 45 / 155]
+    star = (1 - 1.2 * f[0] - 0.9 * f[1]) / f[2]
+    assert [values[i] for i in ae] == pytest.approx([1.2 * f[0], 0.9 * f[1], star * f[2]], rel=1e-14)
+    beta = 2 / 0.09 - 1
+    assert [spec["params"][i]["prior"] for i in ae] == [("gamma", pytest.approx(beta * x, rel=1e-14), 1.0) for x in f]
+    assert [names[i] for i in le["param"]] == ["open", "closed"]
+    want = [[0, 0, 0]] * 21 + [[0, 1, 1]] * 21 + [[0, 0, 1]] * 14
+    assert le["map"] == want
+    fl = [112 / 168, 56 / 168]
+    assert [values[i] for i in le["param"]] == pytest.approx([(1 - 0.6 * fl[1]), 0.6 * fl[1]], rel=1e-14)
+    beta = 1 / 0.16 - 1
+    assert [spec["params"][i]["prior"] for i in le["param"]] == [("gamma", pytest.approx(beta * x, rel=1e-14), 1.0)
+                                                                 for x in fl]
+    groups = spec["dirichlet"][-2:]
+    assert groups[0]["param"] == ae and groups[0]["frac"] == pytest.approx(f, rel=1e-15)
+    assert groups[1]["param"] == le["param"] and groups[1]["frac"] == pytest.approx(fl, rel=1e-15)
+
+
+@pytest.mark.parametrize("entry,msg", [
+    ('area_effect = {value="1.1|0.9", prior="MDir(0.5)"}', "exactly one of the values must be '\\*'"),
+    ('area_effect = {value="1.1|*", prior="MDir(0.5)|Fixed(1)"}', "all be MDir"),
+    ('area_effect = {value="1.1|*", prior="MDir(3)"}', "too diffuse"),
+    ('area_effect = {value="5|*", prior="MDir(0.5)"}', "sum to more than one"),
+    ('level_effect = {file="lev.csv", param="a|b", value="*|2", prior="MDir(0.5)"}', "not contained in 'param'"),
+])
+def test_effect_rejections(tmp_path, entry, msg):
+    d = str(tmp_path)
+    _write(d, "areas.csv", "area,population\nX,100\nY,300\n")
+    _write(d, "g.csv", "1,0.1\n0.1,1\n")
+    _write(d, "lev.csv", "date,X,Y\n2020-03-01,a,b\n2020-03-10,a,c\n")
+    text = BASE.replace('R_spline = [{bp="[bp:R_spline.csv]", value="[value:R_spline.csv]", '
+                        'prior="[prior:R_spline.csv]"}]', 'R_spline = [{value="2", prior="Uniform(1,3)"}]')
+    with pytest.raises(T.TomlError, match=msg):
+        T.load(_write(d, "m.toml", text + 'geo_mixing_matrix = "g.csv"\n' + entry + "\n"), datadir=d)
+
+
 def test_time_varying_covariate(tmp_path):
     """tv_covars with a 'Date' heading: the table's dates relative to start,
     one value for every area; a log transform needs positive values; a

@@ -73,6 +73,10 @@ class Obs(C.Structure):
                 ("ntr", C.c_int32), ("tr", C.POINTER(C.c_int32)), ("nco", C.c_int32), ("co", C.POINTER(C.c_int32))]
 
 
+class AgeMod(C.Structure):
+    _fields_ = [("spline", C.c_int32), ("nage", C.c_int32), ("ages", C.POINTER(C.c_int32))]
+
+
 class Spec(C.Structure):
     _fields_ = [("A", C.c_int32), ("G", C.c_int32), ("C", C.c_int32), ("T", C.c_int32),
                 ("period", C.c_int32), ("steps_per_day", C.c_int32), ("nparam", C.c_int32),
@@ -86,7 +90,8 @@ class Spec(C.Structure):
                 ("nobs", C.c_int32), ("obs", C.POINTER(Obs)), ("nsection", C.c_int32),
                 ("section_ti", C.POINTER(C.c_int32)), ("section_tf", C.POINTER(C.c_int32)),
                 ("area_effect_param", C.POINTER(C.c_int32)), ("nlevel", C.c_int32),
-                ("level_param", C.POINTER(C.c_int32)), ("level_map", C.POINTER(C.c_int32))]
+                ("level_param", C.POINTER(C.c_int32)), ("level_map", C.POINTER(C.c_int32)),
+                ("nagemod", C.c_int32), ("agemod", C.POINTER(AgeMod))]
 
 
 _lib = None
@@ -281,6 +286,12 @@ class _SpecBuffers:
         s.nsection = len(secs)
         s.section_ti = i32([a for a, b in secs]) if secs else null_i32
         s.section_tf = i32([b for a, b in secs]) if secs else null_i32
+        am = spec.get("age_mod") or []
+        ams = (AgeMod * max(1, len(am)))()
+        for i, x in enumerate(am):
+            ams[i].spline, ams[i].nage, ams[i].ages = x["spline"], len(x["ages"]), i32(x["ages"])
+        keep.append(ams)
+        s.nagemod, s.agemod = len(am), ams
         ae = spec.get("area_effect")
         s.area_effect_param = i32(ae) if ae is not None else null_i32
         le = spec.get("level_effect")
@@ -300,7 +311,8 @@ def spec_tables(spec, paramval):
     C_, T = len(spec["comps"]), len(spec["trans"])
     S = spec["period"] * spec["spd"]
     pv = _f64(paramval)
-    out = dict(rate=np.zeros((T, G)), sus=np.zeros(G), inf=np.zeros(C_), N=np.zeros((G, G)),
+    Nshape = (S, G, G) if spec.get("age_mod") else (G, G)      # a time-varying age matrix: one per step
+    out = dict(rate=np.zeros((T, G)), sus=np.zeros(G), inf=np.zeros(C_), N=np.zeros(Nshape),
                beta_af=np.zeros((S, A)), efoi=np.zeros((S, G)), geo=np.zeros(S))
     Pr = np.zeros(1)
     inb = np.zeros(1, dtype=np.int32)
@@ -394,7 +406,8 @@ class Context:
 
     def get_tables(self, slot):
         A, G, C_, T, S = self.A, self.G, self.C, self.T, self.S
-        out = dict(rate=np.zeros((T, G)), sus=np.zeros(G), inf=np.zeros(C_), N=np.zeros((G, G)),
+        Nshape = (S, G, G) if self.spec.get("age_mod") else (G, G)
+        out = dict(rate=np.zeros((T, G)), sus=np.zeros(G), inf=np.zeros(C_), N=np.zeros(Nshape),
                    beta_af=np.zeros((S, A)), efoi=np.zeros((S, G)), geo=np.zeros(S))
         self._chk(lib().beep_get_tables(self._h, slot, *[_p(out[k], C.c_double) for k in
                                                          ("rate", "sus", "inf", "N", "beta_af", "efoi", "geo")]))

@@ -573,7 +573,8 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   tp.off_rate = off; off += m.T * m.G;
   tp.off_sus = off; off += m.G;
   tp.off_inf = off; off += m.C;
-  tp.off_N = off; off += m.G * m.G;
+  tp.N_tv = m.agemod.empty() ? 0 : 1;
+  tp.off_N = off; off += m.G * m.G * (tp.N_tv ? m.S : 1);
   tp.off_efoi = off; off += m.S * m.G;
   tp.off_geo = off; off += m.S;
   tp.off_beta = off; off += m.S * m.A;

@@ -18,6 +18,7 @@ struct TopoDev {
   int stage_mix;         // 1: the mixing CSR is copied into LDS
   float dt;
   int off_rate, off_sus, off_inf, off_N, off_efoi, off_geo, off_beta;
+  int N_tv;              // 1: the age matrix varies by step, [S][G][G] at off_N
   const int* from;       // [T]
   const int* to;         // [T]
   const float* diag;     // [A]  normalised diagonal of the area mixing

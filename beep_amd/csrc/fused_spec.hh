@@ -119,6 +119,11 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   for (int s = jb.t0; s < jb.t1; s++) {
     gcfloat* beta_s = tab + tpp->off_beta + (size_t)s * A;
     gcfloat* efoi_s = tab + tpp->off_efoi + (size_t)s * G;
+    // A time-varying age matrix: this step's.  Step s-1's reads of sN in (C)
+    // ended at the barrier after (C); these writes are seen after (B)'s.
+    if (tpp->N_tv) {
+      for (int i = tid; i < G * G; i += nt) sN[i] = tab[tpp->off_N + (size_t)s * G * G + i];
+    }
     // (B) area mixing, from LDS when the CSR is staged there.
     const float m = tab[tpp->off_geo + s];
     if (staged)

@@ -244,6 +244,15 @@ std::string HostModel::build(const beep_spec& s) {
     for (int th : area_effect_param)
       if (th < 0 || th >= nparam) return "area effect parameter out of range";
   }
+  agemod.clear();
+  for (int k = 0; k < s.nagemod; k++) {
+    const beep_agemod& x = s.agemod[k];
+    if (x.spline < 0 || x.spline >= s.nspline) return "age matrix modification spline out of range";
+    AgeMod am{x.spline, std::vector<int>(x.ages, x.ages + x.nage)};
+    for (int a : am.ages)
+      if (a < 0 || a >= G) return "age matrix modification age group out of range";
+    agemod.push_back(am);
+  }
   level_param.clear(); level_map.clear();
   if (s.nlevel > 0) {
     if (!s.level_param || !s.level_map) return "level effects need their parameters and map";
@@ -397,7 +406,26 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
     for (int g = 0; g < G; g++) tb.sus[g] = sus_param[g] >= 0 ? pd[sus_param[g]] : 1.0;
   tb.inf.assign(C, 0.0);
   for (int c = 0; c < C; c++) tb.inf[c] = comp[c].inf_param >= 0 ? pd[comp[c].inf_param] : 0.0;
-  tb.Ntime = N;                                              // Model::create_Ntime without modifications
+  // Model::create_Ntime (src/model.cc:1254-1301): each modification
+  // multiplies its age groups' rows and columns by its spline's value.
+  tb.Ntime = N;
+  if (!agemod.empty()) {
+    std::vector<std::vector<double>> fac;
+    for (const AgeMod& am : agemod) fac.push_back(disc_spline(am.spline, pd));
+    tb.Ntime.assign((size_t)S * G * G, 0.0);
+    for (int s = 0; s < S; s++) {
+      double* Ns = tb.Ntime.data() + (size_t)s * G * G;
+      std::copy(N.begin(), N.end(), Ns);
+      for (size_t k = 0; k < agemod.size(); k++) {
+        const double f = fac[k][s];
+        for (int a : agemod[k].ages)
+          for (int i = 0; i < G; i++) {
+            Ns[(size_t)a * G + i] *= f;
+            Ns[(size_t)i * G + a] *= f;
+          }
+      }
+    }
+  }
 
   // Model::create_areafactor (src/model.cc:1182-1250).
   std::vector<double> af((size_t)period * A, 1.0);
@@ -442,30 +470,50 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
     }
   }
   if (!invert(V, NV, Vinv)) { err = "singular transition matrix V"; return BEEP_EINVAL; }
-  std::vector<double> F((size_t)G * NV, 0.0);
-  for (int si = 0; si < nis; si++) {
-    double inf = tb.inf[inf_state[si]];
-    if (inf == 0) continue;
-    for (int dp = 0; dp < G; dp++) {
-      int i = si * G + dp;
-      for (int q = 0; q < G; q++) F[(size_t)q * NV + i] += Rdist[q] * tb.sus[q] * inf * N[(size_t)q * G + dp];
+  // calculate_R_beta_ratio_using_NGM: dominant eigenvalue of F V^-1 for one
+  // age contact matrix Nm (src/model.cc:2485-2576).
+  auto ngm_ratio = [&](const double* Nm) {
+    std::vector<double> F((size_t)G * NV, 0.0);
+    for (int si = 0; si < nis; si++) {
+      double inf = tb.inf[inf_state[si]];
+      if (inf == 0) continue;
+      for (int dp = 0; dp < G; dp++) {
+        int i = si * G + dp;
+        for (int q = 0; q < G; q++) F[(size_t)q * NV + i] += Rdist[q] * tb.sus[q] * inf * Nm[(size_t)q * G + dp];
+      }
+    }
+    std::vector<double> NGM((size_t)G * G, 0.0);
+    for (int q = 0; q < G; q++)
+      for (int i = 0; i < G; i++) {
+        double sum = 0;
+        for (int j = 0; j < NV; j++) sum += F[(size_t)q * NV + j] * Vinv[(size_t)j * NV + i];
+        NGM[(size_t)q * G + i] = sum;
+      }
+    return dominant_eigenvalue(NGM, G);
+  };
+  // The ratio is recomputed only where the matrix differs from the previous
+  // step's by more than TINY = 1e-8 in some entry (src/model.cc:1337-1339,1303-1314).
+  const bool tv = !agemod.empty();
+  std::vector<double> ratio(tv ? S : 1);
+  for (int s = 0; s < (int)ratio.size(); s++) {
+    const double* Ns = tb.Ntime.data() + (size_t)s * G * G;
+    bool same = s > 0;
+    for (int k = 0; same && k < G * G; k++) {
+      const double v1 = Ns[k - G * G], v2 = Ns[k];
+      if (v1 < v2 - 1e-8 || v1 > v2 + 1e-8) same = false;
+    }
+    ratio[s] = same ? ratio[s - 1] : ngm_ratio(Ns);
+    if (!(ratio[s] > 0) || std::isnan(ratio[s])) {
+      err = "next-generation matrix has no positive dominant eigenvalue";
+      return BEEP_EINVAL;
     }
   }
-  std::vector<double> NGM((size_t)G * G, 0.0);
-  for (int q = 0; q < G; q++)
-    for (int i = 0; i < G; i++) {
-      double sum = 0;
-      for (int j = 0; j < NV; j++) sum += F[(size_t)q * NV + j] * Vinv[(size_t)j * NV + i];
-      NGM[(size_t)q * G + i] = sum;
-    }
-  double ratio = dominant_eigenvalue(NGM, G);
-  if (!(ratio > 0) || std::isnan(ratio)) { err = "next-generation matrix has no positive dominant eigenvalue"; return BEEP_EINVAL; }
   double Rfac = Rfactor_param >= 0 ? pd[Rfactor_param] : 1.0;
   std::vector<double> Rt = disc_spline(R_spline, pd);
 
   tb.beta_af.assign((size_t)S * A, 0.0);
   for (int s = 0; s < S; s++) {
-    double beta = Rfac * Rt[s] / ratio;
+    double beta = Rfac * Rt[s] / ratio[tv ? s : 0];
     int day = s / spd;
     for (int c = 0; c < A; c++) tb.beta_af[(size_t)s * A + c] = beta * af[(size_t)day * A + c];
   }

@@ -31,7 +31,7 @@ struct Tables {
   std::vector<double> rate;     // [T][G]
   std::vector<double> sus;      // [G]
   std::vector<double> inf;      // [C]
-  std::vector<double> Ntime;    // [G][G] (no age_matrix_perturb: constant in time)
+  std::vector<double> Ntime;    // [G][G], or [S][G][G] with age matrix modifications
   std::vector<double> beta_af;  // [S][A] beta * areafactor(day of s)
   std::vector<double> efoi;     // [S][G] efoi(t_s) / popsize * agedist
   std::vector<double> geo;      // [S]    m(t_s)
@@ -54,6 +54,8 @@ class HostModel {
   std::vector<Covar> covar;
   std::vector<int> area_effect_param;                         // [A], empty = none
   std::vector<int> level_param, level_map;                    // [nlevel], [period][A]
+  struct AgeMod { int spline; std::vector<int> ages; };
+  std::vector<AgeMod> agemod;                                 // empty: N constant in time
   std::vector<int> sus_param;
   int R_spline = -1, Rfactor_param = -1, efoi_spline = -1, geo_spline = -1;
   std::vector<double> efoi_agedist;

@@ -97,6 +97,12 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
     __syncthreads();
     // Phase 2: area-mixing contraction Imap = M_off^T dinf, Idiag = diag * dinf,
     // and I = (Idiag + Imap) m + fac Idiag (1 - m)  (State::get_NMI, first half).
+    // A time-varying age matrix: this step's.  Step s-1's reads of sN in
+    // phase 3a ended at the barrier after it; these writes are seen after
+    // the barrier that ends phase 2.
+    if (tpp->N_tv) {
+      for (int i = tid; i < G * G; i += nt) sN[i] = tab[tpp->off_N + (size_t)s * G * G + i];
+    }
     const float m = tab[tpp->off_geo + s];
     if (staged)
       mix_cells((lcfloat*)sdiag, (lcfloat*)sfac, (lcint*)sptr, (lcint*)ssrc, (lcfloat*)sw, (lcint*)sord, sdinf, sI, m,

@@ -222,7 +222,8 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
     }
     gcfloat* Ib = gptr(Ibuf) + (size_t)b * G * Apad;
     float nmi = 0.f;
-    for (int gg = 0; gg < G; gg++) nmi += tab[tp->off_N + g * G + gg] * Ib[(size_t)gg * Apad + j];
+    const size_t offN = tp->off_N + (tp->N_tv ? (size_t)s * G * G : 0);   // a time-varying age matrix: step s's
+    for (int gg = 0; gg < G; gg++) nmi += tab[offN + g * G + gg] * Ib[(size_t)gg * Apad + j];
     if (nmi < 0.f) nmi = 0.f;
     const float foi = tab[tp->off_beta + (size_t)s * A + j] * nmi + tab[tp->off_efoi + (size_t)s * G + g];
     const bool simu = MODE == MODE_MBP ? (jb.simu ? gptr(jb.simu)[(size_t)s * A + j] != 0 : false) : true;

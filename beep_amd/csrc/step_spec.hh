@@ -31,7 +31,8 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
   __shared__ float srate[T * G], sN[G * G], ssus[G], sinf[C];
   gcfloat* tab = gptr(jb.tab);
   for (int i = tid; i < T * G; i += BEEP_STEP_BLOCK) srate[i] = tab[tp->off_rate + i];
-  for (int i = tid; i < G * G; i += BEEP_STEP_BLOCK) sN[i] = tab[tp->off_N + i];
+  const size_t offN = tp->off_N + (tp->N_tv ? (size_t)s * G * G : 0);   // a time-varying age matrix: step s's
+  for (int i = tid; i < G * G; i += BEEP_STEP_BLOCK) sN[i] = tab[offN + i];
   for (int i = tid; i < G; i += BEEP_STEP_BLOCK) ssus[i] = tab[tp->off_sus + i];
   for (int i = tid; i < C; i += BEEP_STEP_BLOCK) sinf[i] = tab[tp->off_inf + i];
   if (tid == 0) qcount = 0;

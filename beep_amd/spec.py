@@ -18,6 +18,9 @@ A spec is a plain dict (JSON-serialisable once arrays are lists):
   covars        [{"param", "value": [A][period]}] (centred on load)
   area_effect   [A] params multiplying each area's factor, or None
   level_effect  {"param": [nlevel], "map": [period][A] level index}, or None
+  age_mod       [{"spline", "ages": [...]}]: at step s the listed age groups' rows and
+                columns of the normalised contact matrix are multiplied by the
+                spline's value (age_mixing_modify / age_mixing_perturb)
   sus           [G] params or None
   dirichlet     [{"param": [...], "frac": [...]}]
   obs           [{"y", "sd", "type", "shape", "factor", "thresh", "percent", "datatable",

@@ -34,7 +34,8 @@ Supported: one or more age groups (`ages`, `age_mixing_matrix`, `age:` parameter
 `comps` with dist Exp / Erlang, `trans` with branching probabilities,
 `R_spline` and `efoi_spline` (one entry each), `area_covars`, `tv_covars`
 and `area_tv_covars` (func log or linear), `area_effect` and `level_effect`
-(fixed or MDir(k) priors), `data_tables` of type
+(fixed or MDir(k) priors), `age_mixing_modify` (row_column) and
+`age_mixing_perturb` (a contact matrix that varies by step), `data_tables` of type
 'transition' or 'population' with obsmodel 'normal <sd>', 'normal <P>%',
 'normal load' or 'poisson'.  Presentation keys are ignored; `modification`,
 `prediction_start` and `prediction_end` act only in the reference's
@@ -62,7 +63,7 @@ PREDICTION_KEYS = {"modification", "prediction_start", "prediction_end"}
 SUPPORTED_KEYS = {"time_format", "start", "end", "datadir", "steps_per_unit_time", "comps", "trans", "R_spline",
                   "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str", "geo_mixing_matrix",
                   "geo_mixing_modify", "ages", "age_mixing_matrix", "area_covars", "tv_covars", "area_tv_covars",
-                  "area_effect", "level_effect"}
+                  "area_effect", "level_effect", "age_mixing_modify", "age_mixing_perturb"}
 
 
 class TomlError(ValueError):
@@ -553,11 +554,14 @@ def _effects(rd, doc, P, area_head, area_rows, pop):
     return area_eff, level, groups
 
 
-def _spline(rd, P, key, entry):
-    """One spline entry (src/inputs.cc:1363-1474): knots, parameters, multfac."""
-    _check_keys(entry, {"name", "bp", "param", "value", "prior", "factor", "factor_param", "factor_value",
-                        "factor_prior", "smooth", "smooth_type"} | ({"age_dist"} if key == "efoi_spline" else set()),
-                key)
+def _spline(rd, P, key, entry, default_name=None, params=None):
+    """One spline entry (src/inputs.cc:1363-1474): knots, parameters, multfac.
+    default_name(i, bp) names knot i's parameter when 'param' is not given;
+    params, when given, are the knots' parameter indices, already added."""
+    if default_name is None:
+        _check_keys(entry, {"name", "bp", "param", "value", "prior", "factor", "factor_param", "factor_value",
+                            "factor_prior", "smooth", "smooth_type"} | ({"age_dist"} if key == "efoi_spline" else set()),
+                    key)
     bp = [rd.time(t) for t in _split(rd.from_table(_s(entry.get("bp", "")) or "start|end"), "|")]
     if not bp or bp[0] != 0:
         raise TomlError(f"in '{key}' 'bp' must start at 'start'")
@@ -566,18 +570,22 @@ def _spline(rd, P, key, entry):
     if any(b > a for a, b in zip(bp[1:], bp[:-1])):
         raise TomlError(f"in '{key}' times in 'bp' are not in the right order")
     n = len(bp)
-    names = rd.split_n(_s(entry.get("param", "")), n, key)
-    vals = rd.split_n(_s(entry.get("value", "")), n, key)
-    pris = rd.split_n(_s(entry.get("prior", "")), n, key)
     fac = rd.split_n(_s(entry.get("factor", "")) or "1", n, key)
-    params = []
-    for i in range(n):
-        nm = names[i]
-        if nm == "":
-            nm = key.replace("_", "-")
-            if n > 2:
-                nm += f" t={bp[i]}"
-        params.append(P.add(nm, vals[i], pris[i]))
+    if params is None:
+        names = rd.split_n(_s(entry.get("param", "")), n, key)
+        vals = rd.split_n(_s(entry.get("value", "")), n, key)
+        pris = rd.split_n(_s(entry.get("prior", "")), n, key)
+        params = []
+        for i in range(n):
+            nm = names[i]
+            if nm == "":
+                if default_name is not None:
+                    nm = default_name(i, bp)
+                else:
+                    nm = key.replace("_", "-")
+                    if n > 2:
+                        nm += f" t={bp[i]}"
+            params.append(P.add(nm, vals[i], pris[i]))
     factor = -1
     if any(x in entry for x in ("factor_param", "factor_value", "factor_prior")):
         factor = P.add(_s(entry.get("factor_param", "")) or f"{key} factor", _s(entry.get("factor_value", "")),
@@ -602,6 +610,87 @@ def _spline(rd, P, key, entry):
                 raise TomlError(f"for the spline '{key}' a value for 'smooth' must be set")
         out.update(smooth_type=kinds[st], smooth=smv)
     return out
+
+
+def _bp(rd, entry, key):
+    """Knot times of a spline entry (Inputs::get_breakpoints, src/inputs.cc:872-905)."""
+    bp = [rd.time(t) for t in _split(rd.from_table(_s(entry.get("bp", "")) or "start|end"), "|")]
+    if not bp or bp[0] != 0:
+        raise TomlError(f"in '{key}' 'bp' must start at 'start'")
+    if bp[-1] != rd.period:
+        raise TomlError(f"in '{key}' 'bp' must end at 'end'")
+    return bp
+
+
+def _age_mods(rd, doc, P, ages, pop_age, splines):
+    """age_mixing_modify (type 'row_column') and age_mixing_perturb
+    (Inputs::find_genQ, src/inputs.cc:944-1011): one spline each, knot
+    parameters named 'MatMod <agecat> factor t=<bp>' by default, that
+    multiplies the rows and columns of its age groups in the contact matrix
+    (Model::create_Ntime, src/model.cc:1254-1301).  The perturbations'
+    parameters at each knot form a modified Dirichlet group weighted by the
+    groups' population shares, and every age group must be perturbed exactly
+    once (Model::set_dirichlet, src/model.cc:2786-2825).  Appends the splines
+    and returns the modifications and the Dirichlet groups."""
+    mods, groups = [], []
+    G = len(ages)
+
+    def agecats(e, key):
+        if "bp" not in e:
+            raise TomlError(f"in '{key}' a value for 'bp' must be set")
+        st = _s(e.get("agecat", ""))
+        out = []
+        for a in _split(st, "|"):
+            if a not in ages:
+                raise TomlError(f"in '{key}' the value '{a}' in 'agecat' does not specify a valid age classification")
+            out.append(ages.index(a))
+        if not out:
+            raise TomlError(f"in '{key}' a value for 'agecat' must be set")
+        return st, out
+
+    for e in doc.get("age_mixing_modify", []):
+        _check_keys(e, {"type", "agecat", "bp", "param", "value", "prior", "smooth", "factor"}, "age_mixing_modify")
+        if _s(e.get("type", "")) != "row_column":
+            raise TomlError(f"in 'age_mixing_modify' the value of 'type={_s(e.get('type', ''))}' is not recognised")
+        st, idx = agecats(e, "age_mixing_modify")
+        splines.append(_spline(rd, P, "age_mixing_modify", e, lambda i, bp, st=st: f"MatMod {st} factor t={bp[i]}"))
+        mods.append({"spline": len(splines) - 1, "ages": idx})
+    pert = doc.get("age_mixing_perturb", [])
+    if pert:
+        share = pop_age / pop_age.sum()
+        info, used = [], [0] * G
+        for e in pert:
+            _check_keys(e, {"agecat", "bp", "param", "value", "prior", "smooth", "smooth_type", "factor"},
+                        "age_mixing_perturb")
+            st, idx = agecats(e, "age_mixing_perturb")
+            for a in idx:
+                used[a] += 1
+            bp = _bp(rd, e, "age_mixing_perturb")
+            n = len(bp)
+            names = [nm or f"MatMod {st} factor t={bp[i]}"
+                     for i, nm in enumerate(rd.split_n(_s(e.get("param", "")), n, "age_mixing_perturb"))]
+            info.append((e, idx, names, rd.split_n(_s(e.get("value", "")), n, "age_mixing_perturb"),
+                         rd.split_n(_s(e.get("prior", "")), n, "age_mixing_perturb"),
+                         float(sum(share[a] for a in idx))))
+        if any(u != 1 for u in used):
+            a = next(a for a in range(G) if used[a] != 1)
+            raise TomlError(f"'age_mixing_perturb' must use age category '{ages[a]}' exactly once")
+        nk = len(info[0][2])
+        if any(len(x[2]) != nk for x in info):
+            raise TomlError("every 'age_mixing_perturb' entry must have the same number of knots")
+        params = [[None] * nk for _ in info]
+        for p in range(nk):
+            idx, grp = _modified_dirichlet(P, [x[2][p] for x in info], [x[3][p] for x in info],
+                                           [x[4][p] for x in info], [x[5] for x in info],
+                                           f"'age_mixing_perturb' at knot {p}")
+            if grp not in groups:
+                groups.append(grp)
+            for j in range(len(info)):
+                params[j][p] = idx[j]
+        for j, (e, idx, _, _, _, _) in enumerate(info):
+            splines.append(_spline(rd, P, "age_mixing_perturb", e, lambda i, bp: "", params[j]))
+            mods.append({"spline": len(splines) - 1, "ages": idx})
+    return mods, groups
 
 
 def _regions(area_head, area_rows, geo_dep):
@@ -941,6 +1030,7 @@ def load(path, datadir=None, with_data=True):
         pop.append([pa if c == sus else [0] * G for c in range(len(comps))])
     pop_age = np.array(pop, dtype=np.float64).sum(axis=(0, 1))
     area_eff, level, eff_groups = _effects(rd, doc, P, head, rows, pop)
+    age_mod, mod_groups = _age_mods(rd, doc, P, ages, pop_age, splines)
 
     # Age mixing (Data::age_mixing_matrix, src/data_matrix.cc:138-175): a
     # G x G table without a heading, normalised later by the host model.
@@ -997,8 +1087,9 @@ def load(path, datadir=None, with_data=True):
                    **({"prob": t["prob"]} if t["prob"] is not None else {})} for t in trans],
         "params": P.specs,
         "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": efoi_ad,
-        "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet + eff_groups, "sus": sus_par,
+        "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet + eff_groups + mod_groups, "sus": sus_par,
         "covars": _covariates(rd, doc, P, head, rows), "area_effect": area_eff, "level_effect": level,
+        "age_mod": age_mod,
         "obs": _observations(rd, doc, comps, trans, with_data, head, rows, ages),
     }
     return spec, np.array(P.values), meta

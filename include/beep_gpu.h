@@ -89,6 +89,12 @@ typedef struct {
 /* Dirichlet group normalised by Model::dirichlet_correct (src/model.cc:2685-2695). */
 typedef struct { int32_t n; const int32_t* param; const double* frac; } beep_dirichlet;
 
+/* Modification of the age contact matrix over time (Model::create_Ntime,
+ * src/model.cc:1254-1301; age_mixing_modify / age_mixing_perturb): at step s
+ * the rows and columns of the listed age groups are multiplied by the spline's
+ * value.  With any, the age matrix and the NGM ratio of beta vary by step. */
+typedef struct { int32_t spline; int32_t nage; const int32_t* ages; } beep_agemod;
+
 /* Linear covariate on the area factor (Model::create_areafactor, src/model.cc:1182-1235);
  * values are centred on load as Data::read_covars does (src/data.cc:557-565). */
 typedef struct { int32_t param; const double* value; /* [A][period], after any log transform */ } beep_covar;
@@ -153,6 +159,8 @@ typedef struct {
   int32_t nlevel;               /* level effects (level_effect), 0 = none             */
   const int32_t* level_param;   /* [nlevel] parameter of each level                   */
   const int32_t* level_map;     /* [period][A] level of area c on day t               */
+  int32_t nagemod;              /* age contact matrix modifications, 0 = none          */
+  const beep_agemod* agemod;
 } beep_spec;
 
 /* Host only (no device is touched): State::set_param's FP64 tables for one
@@ -213,7 +221,8 @@ int beep_set_param(beep_ctx* ctx, int n, const int32_t* slot, const double* para
                    double* Pr_out, int32_t* inb_out);
 
 /* Copies the FP64 tables derived for a slot (tests): rate[T][G], sus[G], inf[C],
- * Ntime[G][G], beta_af[S][A] (beta * areafactor), efoi[S][G], geo[S]. */
+ * Ntime[G][G] ([S][G][G] when the spec has age matrix modifications),
+ * beta_af[S][A] (beta * areafactor), efoi[S][G], geo[S]. */
 int beep_get_tables(beep_ctx* ctx, int slot, double* rate, double* sus, double* inf, double* Ntime,
                     double* beta_af, double* efoi, double* geo);
 

@@ -15,7 +15,7 @@ Restates, for one strain with age as the only demographic category:
   Model::create_disc_spline         src/model.cc:1020-1053
   Model::create_susceptibility      src/model.cc:1144-1178
   Model::create_areafactor          src/model.cc:1182-1250
-  Model::create_Ntime               src/model.cc:1254-1301 (no matrix modifications)
+  Model::create_Ntime               src/model.cc:1254-1301
   Model::calculate_beta_from_R      src/model.cc:1318-1362 with calculate_Vinv,
                                     calculate_F, calculate_NGM (2485-2576) and
                                     largest_eigenvalue (src/matrix.cc:1139-1219)
@@ -157,8 +157,10 @@ def largest_eigenvalue(M, rng=None):
     raise RuntimeError("eigen-vector convergence problem")
 
 
-def ngm(spec, nz, tr_rate, sus, inf):
-    """Next-generation matrix of calculate_R_beta_ratio_using_NGM."""
+def ngm(spec, nz, tr_rate, sus, inf, N=None):
+    """Next-generation matrix of calculate_R_beta_ratio_using_NGM for the
+    age contact matrix N (default the normalised one)."""
+    N = nz["N"] if N is None else N
     G = spec["G"]
     trans = spec["trans"]
     ist = nz["inf_state"]
@@ -181,7 +183,7 @@ def ngm(spec, nz, tr_rate, sus, inf):
             continue
         for dp in range(G):
             for q in range(G):
-                F[q, si * G + dp] += nz["Rdist"][q] * sus[q] * inf[co] * nz["N"][q, dp]
+                F[q, si * G + dp] += nz["Rdist"][q] * sus[q] * inf[co] * N[q, dp]
     return (F @ Vinv)[:, :G]
 
 
@@ -215,7 +217,23 @@ def derive(spec, nz, pv):
         af *= pd[np.asarray(le["param"])][np.asarray(le["map"])]
     area_av = (af.mean(axis=0) * nz["area_pop"]).sum() / nz["popsize"]
     af /= area_av
-    ratio = largest_eigenvalue(ngm(spec, nz, rate, sus, inf))
+    # Model::create_Ntime (src/model.cc:1254-1301): modifications multiply
+    # their age groups' rows and columns by the spline value at each step.
+    Ntime = np.repeat(nz["N"][None, :, :], S, axis=0)
+    for am in spec.get("age_mod") or []:
+        f = disc_spline(spec, am["spline"], pd)
+        for s in range(S):
+            for a in am["ages"]:
+                Ntime[s, a, :] *= f[s]
+                Ntime[s, :, a] *= f[s]
+    # The ratio is recomputed where the matrix changes by more than TINY
+    # (src/model.cc:1335-1340, Model::equal 1303-1314).
+    ratio = np.zeros(S)
+    for s in range(S):
+        if s == 0 or np.any(np.abs(Ntime[s] - Ntime[s - 1]) > 1e-8):
+            ratio[s] = largest_eigenvalue(ngm(spec, nz, rate, sus, inf, Ntime[s]))
+        else:
+            ratio[s] = ratio[s - 1]
     Rfac = pd[spec["Rfactor"]] if spec.get("Rfactor", -1) >= 0 else 1.0
     beta = Rfac * disc_spline(spec, spec["R_spline"], pd) / ratio          # [S]
     beta_SA = np.repeat(beta[:, None], A, axis=1)
@@ -225,9 +243,8 @@ def derive(spec, nz, pv):
         efoi = np.zeros(S)
     agedist = np.asarray(spec.get("efoi_agedist", [1.0 / G] * G), dtype=np.float64)
     geo = disc_spline(spec, spec["geo_spline"], pd) if spec.get("geo_spline", -1) >= 0 else np.ones(S)
-    Ntime = np.repeat(nz["N"][None, :, :], S, axis=0)
     return dict(rate=rate, sus=sus, inf=inf, af=af, beta=beta_SA, efoi=efoi, agedist=agedist, geo=geo,
-                Ntime=Ntime, ratio=ratio)
+                Ntime=Ntime, ratio=ratio[0] if not spec.get("age_mod") else ratio)
 
 
 def prior(spec, pv):

@@ -5,6 +5,7 @@ masks), replay and mixed batches -- on models with 1, 4 and 9 age groups."""
 import numpy as np
 import pytest
 
+import toml_models
 from beep_amd import _lib, configs, spec as specmod
 
 pytestmark = pytest.mark.gpu
@@ -31,6 +32,7 @@ MODELS = [
     ("spatial_100x4", lambda: configs.spatial_abcsmc()),
     ("seir_areas_30x9", lambda: configs.seir_areas(30, 9, 80, seed=3)),
     ("seir_areas_50x1", lambda: configs.seir_areas(50, 1, 120, seed=4)),
+    ("toml_sird_ages_tv_matrix", toml_models.toml_sird_ages),
 ]
 
 
@@ -104,6 +106,7 @@ STEP_MODELS = [
     ("spatial_100x4_sparse", lambda: configs.spatial_abcsmc(A=100, period=60)),
     ("seir_areas_30x9", lambda: configs.seir_areas(30, 9, 60, seed=3)),
     ("dense_40x9", lambda: configs.dense_pmcmc(A=40, period=40, with_obs=False)),
+    ("toml_sird_ages_tv_matrix", toml_models.toml_sird_ages),
 ]
 
 

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 from scipy import stats
 
+import toml_models
 from beep_amd import _lib, configs
 from oracle import oracle as O
 
@@ -22,12 +23,19 @@ def test_mfma_gemm_matches_numpy(gpu_lib, rows, K):
     assert np.all(np.abs(C - ref) <= 1e-5 * scale + 1e-6)
 
 
-def test_stepwise_sparse_equals_fused(gpu_lib):
-    """Same keys and seeds: the step-wise (sparse) path reproduces the fused
-    kernel's trajectories (same arithmetic order, same Philox streams)."""
+def _spatial_high_R():
     spec, truth = configs.spatial_abcsmc()
     th = truth.copy()
     th[:10] = 2.0
+    return spec, th
+
+
+@pytest.mark.parametrize("make", [_spatial_high_R, toml_models.toml_sird_ages], ids=["spatial_100x4", "sird_ages_tv"])
+def test_stepwise_sparse_equals_fused(gpu_lib, make):
+    """Same keys and seeds: the step-wise (sparse) path reproduces the fused
+    kernel's trajectories (same arithmetic order, same Philox streams), also
+    with an age matrix that varies by step."""
+    spec, th = make()
     a = _lib.Context(spec, capacity=4, nslot=1)
     b = _lib.Context(spec, capacity=4, nslot=1, keep=_lib.KEEP_ALL | _lib.STEPWISE)
     for ctx in (a, b):

@@ -36,7 +36,7 @@ def test_host_tables_match_oracle(gpu_lib, mk):
         assert np.allclose(t["beta_af"], ba, rtol=1e-12, atol=0)
         assert np.allclose(t["rate"], d["rate"], rtol=1e-14)
         assert np.allclose(t["sus"], d["sus"]) and np.allclose(t["inf"], d["inf"])
-        assert np.allclose(t["N"], nz["N"], rtol=1e-14)
+        assert np.allclose(t["N"], d["Ntime"] if spec.get("age_mod") else nz["N"], rtol=1e-14)
         assert np.allclose(t["efoi"], d["efoi"][:, None] * d["agedist"][None, :])
         assert np.allclose(t["geo"], d["geo"])
         assert t["Pr"] == pytest.approx(derive.prior(spec, pv), rel=1e-14)

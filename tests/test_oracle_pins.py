@@ -146,6 +146,36 @@ def test_area_factor_by_loops():
         assert np.allclose(af, raw / av, rtol=1e-13, atol=0)
 
 
+def test_time_varying_age_matrix_closed_form():
+    """Model::create_Ntime (src/model.cc:1254-1301) on the three-age fixture
+    (a row_column factor on 'elderly', one perturbation per age group): every
+    modification multiplies its groups' rows and columns, so at step s
+    Ntime[s] = N * outer(F_s, F_s) with F_s[a] the product of the factors on
+    age a, here from numpy.interp of the Dirichlet-corrected knot values at
+    t = s + 0.5.  The NGM ratio at each step equals numpy's dominant
+    eigenvalue for that step's matrix (src/model.cc:1335-1340)."""
+    import toml_models
+    spec, truth = toml_models.toml_sird_ages()
+    nz = derive.normalise(spec)
+    S, G = spec["period"] * spec["spd"], spec["G"]
+    t = (np.arange(S) + 0.5) * spec["period"] / S
+    for pv in (truth, truth * np.random.default_rng(2).uniform(0.7, 1.3, len(truth))):
+        pd = derive.dirichlet_correct(spec, pv)
+        F = np.ones((S, G))
+        for m in spec["age_mod"]:
+            sp = spec["splines"][m["spline"]]
+            f = np.interp(t, sp["t"], [pd[i] for i in sp["param"]])
+            for a in m["ages"]:
+                F[:, a] *= f
+        d = derive.derive(spec, nz, pv)
+        want = nz["N"][None, :, :] * F[:, :, None] * F[:, None, :]
+        assert np.allclose(d["Ntime"], want, rtol=1e-13, atol=0)
+        for s in (0, 20, 21, 40, S - 1):
+            K = derive.ngm(spec, nz, d["rate"], d["sus"], d["inf"], want[s])
+            assert d["ratio"][s] == pytest.approx(np.linalg.eigvals(K).real.max(), rel=1e-10)
+        assert np.ptp(d["ratio"]) > 0.01 * d["ratio"].mean()
+
+
 def test_deterministic_seir_is_forward_euler():
     """Deterministic dynamics with one area: the forward-Euler SEIR recursion
     with FOI = beta I / N (Idiag = I / N when A = 1)."""

@@ -264,7 +264,7 @@ def test_age_groups():
     assert [values[i] for i in mean] == [4.0, 5.0, 7.0]
     pr, pd = spec["trans"][1]["prob"], spec["trans"][2]["prob"]
     assert [names[i] for i in pd] == ["child I->D branch prob.", "adult I->D branch prob.", "elderly I->D branch prob."]
-    assert spec["dirichlet"] == [{"param": [pr[g], pd[g]]} for g in range(3)]
+    assert spec["dirichlet"][:3] == [{"param": [pr[g], pd[g]]} for g in range(3)]
     assert [values[i] for i in pd] == pytest.approx([0.001, 0.01, 0.1], abs=1e-15)
     assert spec["pop"] == [[[24000, 60000, 16000], [0, 0, 0], [0, 0, 0], [0, 0, 0]]]
     share = np.array([24000, 60000, 16000]) / 100000
@@ -273,6 +273,42 @@ def test_age_groups():
     assert [(o["dp"], o["graph"]) for o in si[::9]] == [([0], 0), ([1], 1), ([2], 2)]
     d = [o for o in spec["obs"] if o["datatable"] == 1]
     assert all(o["dp"] == [0, 1, 2] and o["co"] == [3] for o in d) and len(d) == 21
+
+
+def test_age_matrix_modifications(tmp_path):
+    """age_mixing_modify (row_column on 'elderly') and age_mixing_perturb (one
+    entry per age group) of the age fixture, worked by hand: one spline each,
+    knot parameters 'MatMod <agecat> factor t=<bp>' by default; at each knot
+    the perturbations form a modified Dirichlet group weighted by the age
+    shares (0.24, 0.6, 0.16) with gamma(beta * share, 1) priors,
+    beta = 2 / 0.3^2 - 1; '*' is filled so the weighted values sum to one and
+    values are returned times their weight (src/inputs.cc:944-1011,
+    src/model.cc:2786-2825,2838-2878,3025-3050)."""
+    import toml_models
+    spec, values = toml_models.toml_sird_ages()
+    names = [p["name"] for p in spec["params"]]
+    mods = spec["age_mod"]
+    assert [m["ages"] for m in mods] == [[2], [0], [1], [2]]
+    rc = spec["splines"][mods[0]["spline"]]
+    assert rc["t"] == [0.0, 28.0, 63.0] and [names[i] for i in rc["param"]] == ["shield 0", "shield 28", "shield 63"]
+    assert [values[i] for i in rc["param"]] == [1.0, 0.5, 0.7]
+    share = [0.24, 0.6, 0.16]
+    beta = 2 / 0.09 - 1
+    raw = [[0.24, 0.6, 0.16], [0.6 * 0.24, 1 - 0.6 * 0.24 - 1.2 * 0.16, 1.2 * 0.16], [0.8 * 0.24, 1 - 0.8 * 0.24 - 0.16, 0.16]]
+    for k, age in enumerate(("child", "adult", "elderly")):
+        sp = spec["splines"][mods[k + 1]["spline"]]
+        assert sp["t"] == [0.0, 21.0, 63.0] and sp["smooth_type"] == 2 and sp["smooth"] == [0.3] * 3
+        assert [names[i] for i in sp["param"]] == [f"MatMod {age} factor t={t}" for t in (0, 21, 63)]
+        assert [values[i] for i in sp["param"]] == pytest.approx([raw[p][k] for p in range(3)], rel=1e-14)
+        assert [spec["params"][i]["prior"] for i in sp["param"]] == [("gamma", pytest.approx(beta * share[k]), 1.0)] * 3
+    for p in range(3):
+        grp = spec["dirichlet"][3 + p]
+        assert grp["param"] == [spec["splines"][m["spline"]]["param"][p] for m in mods[1:]]
+        assert grp["frac"] == pytest.approx(share, rel=1e-15)
+    # Every age group perturbed exactly once (src/model.cc:2796-2813).
+    text = open(os.path.join(FIX, "sird_ages.toml")).read().replace('{agecat="adult"', '{agecat="child"')
+    with pytest.raises(T.TomlError, match="exactly once"):
+        T.load(_write(str(tmp_path), "m.toml", text), datadir=os.path.join(FIX, "data_sird_ages"))
 
 
 def test_multi_area_needs_matrix(tmp_path):

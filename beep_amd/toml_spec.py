@@ -39,8 +39,8 @@ and `area_tv_covars` (func log or linear), `area_effect` and `level_effect`
 'transition' or 'population' with obsmodel 'normal <sd>', 'normal <P>%',
 'normal load' or 'poisson'.  Presentation keys are ignored; `modification`,
 `prediction_start` and `prediction_end` act only in the reference's
-prediction mode (src/model.cc:1349, src/state.cc:415,446) and are recorded
-in the returned metadata.  Everything else raises TomlError naming the key.
+prediction mode (src/model.cc:1349, src/state.cc:415,446), and `prob_reach`
+is an output-only derived quantity; they are recorded in the returned metadata.  Everything else raises TomlError naming the key.
 """
 import csv
 import datetime
@@ -60,6 +60,9 @@ OBS_THRESH = 999999998.0
 IGNORED_KEYS = {"description", "outputdir", "state_outputs", "plot_param_values", "time_labels", "area_plot",
                 "output_prop", "mode"}
 PREDICTION_KEYS = {"modification", "prediction_start", "prediction_end"}
+# Quantities the reference derives from the posterior for its outputs only
+# (Model::calculate_probreach, src/model.cc:1932-1957,2140-2160).
+DERIVED_KEYS = {"prob_reach"}
 SUPPORTED_KEYS = {"time_format", "start", "end", "datadir", "steps_per_unit_time", "comps", "trans", "R_spline",
                   "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str", "geo_mixing_matrix",
                   "geo_mixing_modify", "ages", "age_mixing_matrix", "area_covars", "tv_covars", "area_tv_covars",
@@ -365,25 +368,28 @@ def _transitions(rd, doc, comps, P, ages):
     return trans, _branches(trans, comps, P, len(ages))
 
 
-def _is_flat_dir(prior):
-    """Dir(*) and Dir(1): both give each branch parameter the gamma(1,1) log
-    density -x (Model::prior, src/model.cc:1626-1647)."""
+def _dir_kind(prior):
+    """A branch prior's Dirichlet kind (Model::add_parameter, src/model.cc:
+    901-930): 'flat' for Dir(*) (density -x), ('alpha', a) for Dir(a) (a
+    gamma(a, 1) density), 'meansd' for Dir(mean,sd), None otherwise."""
     t = prior.replace(" ", "").lower()
     if not (t.startswith("dir(") and t.endswith(")")):
-        return False
+        return None
     a = t[4:-1]
     if a == "*":
-        return True
+        return "flat"
+    if "," in a:
+        return "meansd"
     try:
-        return float(a) == 1.0
+        return ("alpha", float(a))
     except ValueError:
-        return False
+        raise TomlError(f"the prior '{prior}' is not recognised") from None
 
 
 def _branches(trans, comps, P, G):
     """Branch probabilities of the transitions leaving one compartment, per
     age group: one value may be '*', one minus the others (src/model.cc:
-    2894-2915); with Dir(*) / Dir(1) priors they form a Dirichlet group that
+    2894-2915); with Dir(*) or Dir(alpha) priors they form a Dirichlet group that
     the model normalises (Model::dirichlet_correct, src/model.cc:2685-2695).
     Sets t["prob"] to the G parameter indices of each branching transition."""
     groups = []
@@ -396,9 +402,15 @@ def _branches(trans, comps, P, G):
         where = f"branch probabilities out of '{comps[src]['base']}'"
         for g in range(G):
             priors = [t["branch"][g][2] for t in ts]
-            dirs = [p.replace(" ", "").lower().startswith(("dir(", "mdir(")) for p in priors]
-            if any(dirs) and not all(_is_flat_dir(p) for p in priors):
-                raise TomlError(f"{where}: only Dir(*) / Dir(1) priors on every branch are supported")
+            if any(p.replace(" ", "").lower().startswith("mdir(") for p in priors):
+                raise TomlError(f"{where}: MDir priors are not used for branch probabilities")
+            kinds = [_dir_kind(p) for p in priors]
+            dirs = [k is not None for k in kinds]
+            if "meansd" in kinds:
+                raise TomlError(f"{where}: Dir(mean,sd) priors are not supported")
+            # All fixed, all Dir(*) or all Dir(a) (src/model.cc:2924-2931).
+            if any(dirs) and not (all(k == "flat" for k in kinds) or all(isinstance(k, tuple) for k in kinds)):
+                raise TomlError(f"{where}: the priors must all be fixed, all Dir(*) or all Dir(alpha)")
             vals = [t["branch"][g][1] for t in ts]
             star = [k for k, v in enumerate(vals) if v == "*"]
             if len(star) > 1:
@@ -418,7 +430,8 @@ def _branches(trans, comps, P, G):
                     raise TomlError(f"{where} cannot fill the value '*'")
                 num[star[0]] = 1.0 - rest
             if all(dirs):
-                idx = [P.add_tuple(t["branch"][g][0], num[k], ("dirflat",)) for k, t in enumerate(ts)]
+                prs = [("dirflat",) if k == "flat" else ("gamma", k[1], 1.0) for k in kinds]
+                idx = [P.add_tuple(t["branch"][g][0], num[k], prs[k]) for k, t in enumerate(ts)]
                 if {"param": idx} not in groups:        # parameters shared by every age group: one group
                     groups.append({"param": idx})
             else:
@@ -944,18 +957,21 @@ def _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, dps, graph, d
 def load(path, datadir=None, with_data=True):
     """Reads a BEEP model file.  Returns (spec, values, meta): the spec dict,
     the parameter values the file gives (NaN where only a prior is given), and
-    {"ignored": [...], "prediction": {...}} for keys recorded without effect.
+    {"ignored": [...], "prediction": {...}, "derived": {...}} for keys
+    recorded without effect on the simulation or the likelihood.
     datadir overrides the file's 'datadir' (which the reference reads relative
     to the working directory); with_data=False skips the data tables' files,
     as the reference's simulation mode does."""
     with open(path, "rb") as f:
         doc = _toml.load(f)
-    meta = {"ignored": [], "prediction": {}}
+    meta = {"ignored": [], "prediction": {}, "derived": {}}
     for k in doc:
         if k in IGNORED_KEYS:
             meta["ignored"].append(k)
         elif k in PREDICTION_KEYS:
             meta["prediction"][k] = doc[k]
+        elif k in DERIVED_KEYS:
+            meta["derived"][k] = doc[k]
         elif k not in SUPPORTED_KEYS:
             raise TomlError(f"the key '{k}' is not supported by the device path")
     if datadir is None:
@@ -1029,6 +1045,10 @@ def load(path, datadir=None, with_data=True):
             pa.append(int(v + 0.5))
         pop.append([pa if c == sus else [0] * G for c in range(len(comps))])
     pop_age = np.array(pop, dtype=np.float64).sum(axis=(0, 1))
+    for e in doc.get("prob_reach", []):          # Inputs::find_probreach, src/inputs.cc:1328-1340
+        _check_keys(e, {"name", "comp"}, "prob_reach")
+        if _s(e.get("name", "")) == "" or _s(e.get("comp", "")) not in {c["base"] for c in comps}:
+            raise TomlError(f"in 'prob_reach' cannot find the compartment '{_s(e.get('comp', ''))}'")
     area_eff, level, eff_groups = _effects(rd, doc, P, head, rows, pop)
     age_mod, mod_groups = _age_mods(rd, doc, P, ages, pop_age, splines)
 

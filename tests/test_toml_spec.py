@@ -143,6 +143,15 @@ def test_rejections(tmp_path, extra, msg):
         _model(tmp_path, extra)
 
 
+def test_prob_reach_recorded(tmp_path):
+    """prob_reach is an output-only derived quantity (src/model.cc:1932-1957):
+    recorded in the metadata, its compartment checked."""
+    _, _, meta = _model(tmp_path, 'prob_reach = [{name="ihr", comp="H"}]\n')
+    assert meta["derived"]["prob_reach"] == [{"name": "ihr", "comp": "H"}]
+    with pytest.raises(T.TomlError, match="cannot find the compartment 'X'"):
+        _model(tmp_path, 'prob_reach = [{name="ihr", comp="X"}]\n')
+
+
 def test_multi_area():
     """Three areas: the raw matrix as diagonal + off-diagonal rows
     (src/data_matrix.cc:206-229), the mixing spline as the geo spline, one
@@ -338,9 +347,14 @@ def test_dirichlet_branches(tmp_path):
     p = _base_model(d, "a.toml", 'prob_value="*", prob_prior="Dir(1)"', spline)
     with pytest.raises(T.TomlError, match="Dir"):
         T.load(p, datadir=d)
-    # Dir(*) / Dir(1) on every branch out of I: one Dirichlet group, flat
-    # priors, '*' filled with one minus the others.
+    # Dir(*) with Dir(1): a flat and an alpha prior in one group, refused as
+    # the reference does (src/model.cc:2924-2931).
     text = open(p).read().replace('prob_value="0.2", prob_prior="Uniform(0,0.5)"', 'prob_value="0.3", prob_prior="Dir(*)"')
+    with pytest.raises(T.TomlError, match="all Dir"):
+        T.load(_write(d, "m.toml", text), datadir=d)
+    # Dir(*) on every branch out of I: one Dirichlet group, flat priors, '*'
+    # filled with one minus the others.
+    text = text.replace('prob_prior="Dir(1)"', 'prob_prior="Dir(*)"')
     p = _write(d, "b.toml", text)
     spec, values, _ = T.load(p, datadir=d)
     names = [q["name"] for q in spec["params"]]
@@ -349,8 +363,15 @@ def test_dirichlet_branches(tmp_path):
     assert spec["params"][ih]["prior"] == spec["params"][ir]["prior"] == ("dirflat",)
     assert values[ih] == 0.3 and values[ir] == pytest.approx(0.7, abs=1e-15)
     assert [t.get("prob") for t in spec["trans"]] == [None, [ih], [ir], None]
+    # Dir(alpha): a gamma(alpha, 1) prior per branch (src/model.cc:907-913,1624-1637).
+    spec, _, _ = T.load(_write(d, "g.toml", text.replace('prob_value="0.3", prob_prior="Dir(*)"',
+                                                          'prob_value="0.3", prob_prior="Dir(0.4)"')
+                                                  .replace('prob_prior="Dir(*)"', 'prob_prior="Dir(2.5)"')), datadir=d)
+    assert spec["params"][ih]["prior"] == ("gamma", 0.4, 1.0) and spec["params"][ir]["prior"] == ("gamma", 2.5, 1.0)
+    with pytest.raises(T.TomlError, match="mean,sd"):
+        T.load(_write(d, "h.toml", text.replace('prob_prior="Dir(*)"', 'prob_prior="Dir(0.5,0.1)"')), datadir=d)
     # Values only: fixed probabilities, '*' filled the same way, no group.
-    text = text.replace('prob_prior="Dir(*)"', "").replace('prob_prior="Dir(1)"', "")
+    text = text.replace('prob_prior="Dir(*)"', "")
     spec, values, _ = T.load(_write(d, "c.toml", text), datadir=d)
     assert spec["dirichlet"] == [] and spec["params"][ir]["prior"] == ("fixed", pytest.approx(0.7, abs=1e-15))
 

@@ -70,7 +70,8 @@ class Obs(C.Structure):
                 ("thresh", C.c_double), ("percent", C.c_double), ("type", C.c_int32), ("datatable", C.c_int32),
                 ("graph", C.c_int32), ("sett_i", C.c_int32), ("sett_f", C.c_int32),
                 ("narea", C.c_int32), ("area", C.POINTER(C.c_int32)), ("ndp", C.c_int32), ("dp", C.POINTER(C.c_int32)),
-                ("ntr", C.c_int32), ("tr", C.POINTER(C.c_int32)), ("nco", C.c_int32), ("co", C.POINTER(C.c_int32))]
+                ("ntr", C.c_int32), ("tr", C.POINTER(C.c_int32)), ("nco", C.c_int32), ("co", C.POINTER(C.c_int32)),
+                ("factor_spline", C.c_int32)]
 
 
 class AgeMod(C.Structure):
@@ -280,6 +281,7 @@ class _SpecBuffers:
             tr, co = o.get("tr", []), o.get("co", [])
             x.ntr, x.tr = len(tr), i32(tr) if tr else null_i32
             x.nco, x.co = len(co), i32(co) if co else null_i32
+            x.factor_spline = o.get("spline", -1) + 1       # 0 = none
         keep.append(os_)
         s.nobs, s.obs = len(obs), os_
         secs = spec.get("sections", [])

@@ -106,6 +106,13 @@ struct beep_ctx {
   hipEvent_t bstart = nullptr;
   bool obs_general = false;        // observations other than plain normal ones
   bool obs_pop = false;            // observations counting population elements
+  // Observation splines: their values per slot, [nslot][nospl][S] (FP64), and
+  // the slot each instance was last simulated, proposed or replayed with.
+  int nospl = 0;
+  double* d_ospl = nullptr;
+  std::vector<int> inst_slot;
+  int* d_inst_slot = nullptr;
+  bool inst_slot_dirty = false;
 };
 
 namespace {
@@ -421,15 +428,31 @@ int launch_fused(beep_ctx* c, int mode, const Job* djobs, int njobs, uint64_t se
 // (all when null), into out[b * (1 + ndatatable)].
 void launch_obs_loglik(beep_ctx* c, int n, const int* d_inst, const int* sel, int nsel, double* out, int store_EF,
                        bool want_dt) {
-  using K = decltype(&beepdev::k_obs_loglik<true, true, true>);
-  static const K ks[8] = {beepdev::k_obs_loglik<false, false, false>, beepdev::k_obs_loglik<false, false, true>,
-                          beepdev::k_obs_loglik<false, true, false>,  beepdev::k_obs_loglik<false, true, true>,
-                          beepdev::k_obs_loglik<true, false, false>,  beepdev::k_obs_loglik<true, false, true>,
-                          beepdev::k_obs_loglik<true, true, false>,   beepdev::k_obs_loglik<true, true, true>};
-  K k = ks[(want_dt ? 4 : 0) + (c->obs_general ? 2 : 0) + (c->obs_pop ? 1 : 0)];
+  using K = decltype(&beepdev::k_obs_loglik<true, true, true, false>);
+  // With observation splines only the general variants are used.
+  static const K ks[12] = {
+      beepdev::k_obs_loglik<false, false, false, false>, beepdev::k_obs_loglik<false, false, true, false>,
+      beepdev::k_obs_loglik<false, true, false, false>,  beepdev::k_obs_loglik<false, true, true, false>,
+      beepdev::k_obs_loglik<true, false, false, false>,  beepdev::k_obs_loglik<true, false, true, false>,
+      beepdev::k_obs_loglik<true, true, false, false>,   beepdev::k_obs_loglik<true, true, true, false>,
+      beepdev::k_obs_loglik<false, true, false, true>,   beepdev::k_obs_loglik<false, true, true, true>,
+      beepdev::k_obs_loglik<true, true, false, true>,    beepdev::k_obs_loglik<true, true, true, true>};
+  K k = c->nospl ? ks[8 + (want_dt ? 2 : 0) + (c->obs_pop ? 1 : 0)]
+                 : ks[(want_dt ? 4 : 0) + (c->obs_general ? 2 : 0) + (c->obs_pop ? 1 : 0)];
   const int ni = want_dt ? beepdev::OBS_INSTANCES<true> : beepdev::OBS_INSTANCES<false>;
   hipLaunchKernelGGL(k, dim3((n + ni - 1) / ni), dim3(256), 0, c->stream, c->d_obs, n, d_inst, c->d_tn, c->d_pt,
-                     c->traj_sz, c->pop_traj_sz, sel, nsel, out, c->d_EF, store_EF);
+                     c->traj_sz, c->pop_traj_sz, sel, nsel, out, c->d_EF, store_EF, c->d_ospl, c->d_inst_slot);
+}
+
+// Before an observation kernel reads inst_slot: the host copy, uploaded when
+// it changed (a synchronous copy: later jobs change the host copy).
+int sync_inst_slot(beep_ctx* c) {
+  if (!c->nospl || !c->inst_slot_dirty) return BEEP_OK;
+  HIPCHK(c, hipMemcpyAsync(c->d_inst_slot, c->inst_slot.data(), c->inst_slot.size() * sizeof(int),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->inst_slot_dirty = false;
+  return BEEP_OK;
 }
 
 Job make_job(beep_ctx* c, int inst, int slot, uint32_t key, int t0, int t1) {
@@ -447,6 +470,8 @@ Job make_job(beep_ctx* c, int inst, int slot, uint32_t key, int t0, int t1) {
   j.t1 = t1;
   j.from_init = t0 == 0 ? 1 : 0;
   j.mode = beepdev::MODE_SIM;
+  c->inst_slot[inst] = slot;
+  c->inst_slot_dirty = true;
   return j;
 }
 
@@ -639,6 +664,15 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   HIPCHK(c, hipMemset(c->d_tab, 0, c->slot_stride * nslot * sizeof(float)));
   c->slot_param.assign((size_t)nslot * m.nparam, 0.0);
   c->slot_set.assign(nslot, 0);
+  c->inst_slot.assign(capacity, -1);
+  c->nospl = (int)m.obs_splines.size();
+  if (c->nospl) {
+    size_t sz = (size_t)nslot * c->nospl * m.S * sizeof(double);
+    if ((rc = dalloc(c, (void**)&c->d_ospl, sz))) return rc;
+    HIPCHK(c, hipMemset(c->d_ospl, 0, sz));
+    if ((rc = dalloc(c, (void**)&c->d_inst_slot, capacity * sizeof(int)))) return rc;
+    c->inst_slot_dirty = true;
+  }
   c->state_sz = (size_t)m.C * m.AG;
   c->traj_sz = (size_t)m.S * m.T * m.AG;
   c->pop_traj_sz = (size_t)m.S * m.C * m.AG;
@@ -679,7 +713,7 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
     for (int i = 0; i < nobs; i++) ord[i] = i;
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return first_elem(a) < first_elem(b); });
     for (int k = 0; k < nobs; k++) pos[ord[k]] = k;
-    std::vector<int> optr(1, 0), type, dtab;
+    std::vector<int> optr(1, 0), type, dtab, ospl;
     std::vector<uint32_t> elem;
     std::vector<double> y, sd, shape, factor, thresh, lc, iv;
     for (int k = 0; k < nobs; k++) {
@@ -688,7 +722,7 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
       for (uint32_t x : o.elem)
         if (x & 0x80000000u) c->obs_pop = true;
       optr.push_back((int)elem.size());
-      type.push_back(o.type); dtab.push_back(o.datatable);
+      type.push_back(o.type); dtab.push_back(o.datatable); ospl.push_back(o.ospl);
       y.push_back(o.y); sd.push_back(o.sd); shape.push_back(o.shape); factor.push_back(o.factor); thresh.push_back(o.thresh);
       // normal_probability (src/utils.cc:107-111) split into per-observation constants
       const double var = o.sd * o.sd;
@@ -707,8 +741,11 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
         (rc = upload(c, (double**)&od.y, y)) || (rc = upload(c, (double**)&od.sd, sd)) ||
         (rc = upload(c, (double**)&od.shape, shape)) || (rc = upload(c, (double**)&od.factor, factor)) ||
         (rc = upload(c, (double**)&od.thresh, thresh)) || (rc = upload(c, (int**)&od.orig, ord)) ||
-        (rc = upload(c, (double**)&od.lc, lc)) || (rc = upload(c, (double**)&od.iv, iv)))
+        (rc = upload(c, (double**)&od.lc, lc)) || (rc = upload(c, (double**)&od.iv, iv)) ||
+        (rc = upload(c, (int**)&od.ospl, ospl)))
       return rc;
+    od.nospl = c->nospl;
+    od.S = m.S;
     if ((rc = dalloc(c, (void**)&c->d_obs, sizeof(beepdev::ObsDev)))) return rc;
     HIPCHK(c, hipMemcpy(c->d_obs, &od, sizeof od, hipMemcpyHostToDevice));
     // Sections as lists of stored positions, ascending.
@@ -787,6 +824,9 @@ int beep_set_param(beep_ctx* c, int n, const int32_t* slot, const double* paramv
     for (size_t k = 0; k < tb.beta_af.size(); k++) block[tp.off_beta + k] = (float)tb.beta_af[k];
     HIPCHK(c, hipMemcpyAsync(c->d_tab + (size_t)slot[i] * c->slot_stride, block.data(), block.size() * sizeof(float),
                              hipMemcpyHostToDevice, c->stream));
+    if (c->nospl)
+      HIPCHK(c, hipMemcpyAsync(c->d_ospl + (size_t)slot[i] * c->nospl * m.S, tb.obs_spl.data(),
+                               tb.obs_spl.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     std::copy(pv, pv + m.nparam, c->slot_param.begin() + (size_t)slot[i] * m.nparam);
     c->slot_set[slot[i]] = 1;
@@ -922,6 +962,12 @@ int beep_obs_loglik(beep_ctx* c, int n, const int32_t* inst, int section, double
     sobs = c->d_sec_obs + c->sec_ptr[section];
     nsel = c->sec_ptr[section + 1] - c->sec_ptr[section];
   }
+  if (c->nospl) {
+    for (int i = 0; i < n; i++)
+      if (c->inst_slot[inst[i]] < 0)
+        return fail(c, BEEP_EINVAL, "instance %d has no parameter slot for its observation splines", inst[i]);
+    if ((rc = sync_inst_slot(c))) return rc;
+  }
   record(c, 2);
   launch_obs_loglik(c, n, d_inst, sobs, nsel, d_out, section < 0 ? 1 : 0, EF_dt_out != nullptr);
   HIPCHK(c, hipGetLastError());
@@ -947,9 +993,12 @@ int beep_obs_values(beep_ctx* c, int inst, double* value_out) {
   int rc;
   if ((rc = ensure_scratch(c, (size_t)nobs * sizeof(double)))) return rc;
   double* d_out = (double*)c->d_scratch;
+  if (c->nospl && c->inst_slot[inst] < 0)
+    return fail(c, BEEP_EINVAL, "instance %d has no parameter slot for its observation splines", inst);
+  const double* splv = c->nospl ? c->d_ospl + (size_t)c->inst_slot[inst] * c->nospl * c->m.S : nullptr;
   hipLaunchKernelGGL(beepdev::k_obs_values, dim3((nobs + 255) / 256), dim3(256), 0, c->stream, c->d_obs,
                      c->d_tn + (size_t)inst * c->traj_sz, c->d_pt ? c->d_pt + (size_t)inst * c->pop_traj_sz : nullptr,
-                     d_out);
+                     splv, d_out);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(value_out, d_out, nobs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1170,6 +1219,10 @@ int beep_derive_slots(beep_ctx* c, int n, const int32_t* dst, const int32_t* src
   hipLaunchKernelGGL(beepdev::k_derive_slot, dim3(bx, n), dim3(256), 0, c->stream, c->d_tab, c->slot_stride, d_dst,
                      d_src, d_f, d_from, c->h_topo.off_beta, c->m.A);
   HIPCHK(c, hipGetLastError());
+  const size_t osz = (size_t)c->nospl * c->m.S;     // observation splines: the source slot's
+  for (int i = 0; c->nospl && i < n; i++)
+    HIPCHK(c, hipMemcpyAsync(c->d_ospl + (size_t)dst[i] * osz, c->d_ospl + (size_t)src[i] * osz, osz * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < n; i++) {
     c->slot_set[dst[i]] = 1;
@@ -1265,6 +1318,7 @@ int beep_pmcmc_obs_prob(beep_ctx* c, int n, const int32_t* inst, int slot, const
   for (int k = 0; k < nsec; k++)
     for (int i = 0; i < n; i++) jobs.push_back(make_job(c, inst[i], slot, key[i], m.section_ti[k], m.section_tf[k]));
   if ((rc = put_jobs(c, jobs))) return rc;
+  if ((rc = sync_inst_slot(c))) return rc;
   const unsigned mv_x = (unsigned)std::min<size_t>(16, (c->state_sz + 255) / 256);
   record(c, 0);
   for (int k = 0; k < nsec; k++) {

@@ -298,6 +298,7 @@ std::string HostModel::build(const beep_spec& s) {
   // Observations: ObservationModel::initialise_obs_change (src/obsmodel.cc:174-217)
   // flattened per observation, and Data::set_obs_sd (src/data.cc:1589-1623).
   obs.resize(s.nobs);
+  obs_splines.clear();
   ndatatable = 1;
   std::vector<double> graph_max;
   for (int i = 0; i < s.nobs; i++) {
@@ -315,6 +316,16 @@ std::string HostModel::build(const beep_spec& s) {
     if (o.datatable < 0 || o.datatable >= 8) return "at most 8 data tables are supported";
     if (o.datatable + 1 > ndatatable) ndatatable = o.datatable + 1;
     ob.sd = o.sd;
+    ob.ospl = -1;
+    if (o.factor_spline != 0) {
+      const int sp = o.factor_spline - 1;
+      if (sp < 0 || sp >= s.nspline) return "observation spline out of range";
+      if (o.nco > 0) return "an observation spline on a population observation";   // emsgEC("ObsModel",3)
+      size_t k = 0;
+      while (k < obs_splines.size() && obs_splines[k] != sp) k++;
+      if (k == obs_splines.size()) obs_splines.push_back(sp);
+      ob.ospl = (int)k;
+    }
     if (o.type == BEEP_OBS_NORMAL_PERCENT && !(o.sd > 0))
       ob.sd = (o.y * (1 - EPSILON_FACTOR) + graph_max[o.graph] * EPSILON_FACTOR) * 0.01 * o.percent;
     if ((o.type == BEEP_OBS_NORMAL || o.type == BEEP_OBS_NORMAL_PERCENT) && !(ob.sd > 0) &&
@@ -525,6 +536,12 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
   }
   tb.geo.assign(S, 1.0);
   if (geo_spline >= 0) tb.geo = disc_spline(geo_spline, pd);
+  // State::disc_spline of the observation splines (src/state.cc:76).
+  tb.obs_spl.clear();
+  for (int sp : obs_splines) {
+    std::vector<double> v = disc_spline(sp, pd);
+    tb.obs_spl.insert(tb.obs_spl.end(), v.begin(), v.end());
+  }
   for (double v : tb.beta_af) if (std::isnan(v)) { err = "NaN transmission rate"; return BEEP_ENAN; }
   return BEEP_OK;
 }

@@ -23,6 +23,7 @@ struct Covar { int param; std::vector<double> value; /* [A][period], centred */ 
 struct Obs {
   double y, sd, shape, factor, thresh;
   int type, datatable, sett_i, sett_f;
+  int ospl = -1;                // index into obs_splines, -1 = none
   std::vector<uint32_t> elem;
 };
 
@@ -35,6 +36,7 @@ struct Tables {
   std::vector<double> beta_af;  // [S][A] beta * areafactor(day of s)
   std::vector<double> efoi;     // [S][G] efoi(t_s) / popsize * agedist
   std::vector<double> geo;      // [S]    m(t_s)
+  std::vector<double> obs_spl;  // [nobs_spline][S] observation spline values
 };
 
 class HostModel {
@@ -74,6 +76,7 @@ class HostModel {
   std::vector<int> inf_state;         // infected compartments (Model::set_infected_uninfected)
 
   std::vector<Obs> obs;
+  std::vector<int> obs_splines;       // the splines observations use, in first-use order
   int ndatatable = 1;
   std::vector<int> section_ti, section_tf;
   std::vector<std::vector<int>> section_obs;

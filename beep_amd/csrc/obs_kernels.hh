@@ -23,6 +23,8 @@ struct ObsDev {
   const int* orig;           // the model's index of each stored observation
   const double* lc;          // normal types: -log(2 pi sd^2) / 2
   const double* iv;          // normal types: 1 / (2 sd^2)
+  const int* ospl;           // observation spline of each stored observation, -1 = none
+  int nospl, S;              // observation splines; steps (a slot holds [nospl][S] values)
 };
 
 // log Gamma(x) for x > 0, FP64: Stirling's series
@@ -77,12 +79,16 @@ __device__ inline double obs_prob(double value, double y, double sd, int type, d
   return negbin_lpmf(k, lam, shape);
 }
 
-__device__ inline double obs_value(const ObsDev& od, int i, const int* tn, const int* pt) {
+// splv: the instance's slot's observation spline values [nospl][S], or null
+// when the model has none.  With a spline, each transition count is weighted
+// by the spline's value at its step (src/obsmodel.cc:85-88).
+__device__ inline double obs_value(const ObsDev& od, int i, const int* tn, const int* pt, const double* splv) {
   double v = 0, f = od.factor[i];
+  const int os = splv ? od.ospl[i] : -1;
   for (int e = od.ptr[i]; e < od.ptr[i + 1]; e++) {
     uint32_t x = od.elem[e];
     int num = (x & 0x80000000u) ? (pt ? pt[x & 0x7FFFFFFFu] : 0) : tn[x];
-    if (num != 0) v += f * (double)num;
+    if (num != 0) v += os < 0 ? f * (double)num : f * (double)num * splv[(size_t)os * od.S + x / od.TAG];
   }
   return v;
 }
@@ -128,13 +134,16 @@ constexpr int OBS_INSTANCES = WANT_DT ? 1 : 4;
 // other than plain normal ones (Poisson, negative binomial, thresholds,
 // unknowns); without it the kernel holds only the normal path, which keeps
 // its register use, and so its occupancy, low.  POP: some observation counts
-// population elements.
-template <bool WANT_DT, bool GENERAL, bool POP>
+// population elements.  SPL: some observations carry an observation spline;
+// their counts are weighted per step by the values of the instance's slot,
+// ospl_tab + inst_slot[instance] * nospl * S.
+template <bool WANT_DT, bool GENERAL, bool POP, bool SPL>
 __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ odp, int n, const int* __restrict__ inst,
                                                     const int* __restrict__ tn, const int* __restrict__ pt,
                                                     size_t traj_sz, size_t pop_traj_sz, const int* __restrict__ sel,
                                                     int nsel, double* __restrict__ out, double* __restrict__ EF,
-                                                    int store_EF) {
+                                                    int store_EF, const double* __restrict__ ospl_tab,
+                                                    const int* __restrict__ inst_slot) {
   // Instances per workgroup: they share the observation structure, so one
   // thread reads an observation's elements once and gathers the counts of
   // all NI instances with independent loads.
@@ -151,6 +160,9 @@ __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ o
     tnb[q] = tn + (size_t)ins[q] * traj_sz;
     ptb[q] = pt ? pt + (size_t)ins[q] * pop_traj_sz : nullptr;
   }
+  const double* splv[NI];
+#pragma unroll
+  for (int q = 0; q < NI; q++) splv[q] = SPL ? ospl_tab + (size_t)inst_slot[ins[q]] * od.nospl * od.S : nullptr;
   const auto optr = gview(od.ptr);
   const auto elem = gview(od.elem);
   const auto factor = gview(od.factor);
@@ -168,6 +180,22 @@ __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ o
     for (int q = 0; q < NI; q++) cnt[q] = 0;
     int e = optr[i];
     const int e1 = optr[i + 1];
+    const int os = SPL ? gview(od.ospl)[i] : -1;
+    double wsum[NI];                       // SPL: sum of count x spline value
+#pragma unroll
+    for (int q = 0; q < NI; q++) wsum[q] = 0;
+    if (SPL && os >= 0) {
+      // Transition elements only; the step of element x is x / TAG.
+      for (; e < e1; e++) {
+        const uint32_t x = elem[e];
+        const size_t k = (size_t)os * od.S + x / od.TAG;
+#pragma unroll
+        for (int q = 0; q < NI; q++) {
+          const int num = tnb[q][x];
+          if (num != 0) wsum[q] += (double)num * splv[q][k];
+        }
+      }
+    }
     // Four elements at a time: without population elements the trajectory
     // loads are unconditional and in flight together.
     for (; e + 4 <= e1; e += 4) {
@@ -190,7 +218,7 @@ __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ o
     const bool plain = !GENERAL || (type <= 1 && y != 999999997.0 && y != 999999998.0);
 #pragma unroll
     for (int q = 0; q < NI; q++) {
-      const double v = f * (double)cnt[q];
+      const double v = SPL && os >= 0 ? f * wsum[q] : f * (double)cnt[q];
       double lp;
       if (plain) {
         const double d = y - (v < 0 ? 0.0 : v);
@@ -225,10 +253,10 @@ __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ o
 }
 
 __global__ void k_obs_values(const ObsDev* __restrict__ odp, const int* __restrict__ tn, const int* __restrict__ pt,
-                             double* __restrict__ out) {
+                             const double* __restrict__ splv, double* __restrict__ out) {
   const ObsDev od = *odp;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < od.nobs) out[od.orig[i]] = obs_value(od, i, tn, pt);
+  if (i < od.nobs) out[od.orig[i]] = obs_value(od, i, tn, pt, splv);
 }
 
 // State::likelihood (src/state.cc:812-826): one workgroup per instance.

@@ -24,7 +24,8 @@ A spec is a plain dict (JSON-serialisable once arrays are lists):
   sus           [G] params or None
   dirichlet     [{"param": [...], "frac": [...]}]
   obs           [{"y", "sd", "type", "shape", "factor", "thresh", "percent", "datatable",
-                  "graph", "sett_i", "sett_f", "area": [...], "dp": [...], "tr": [...], "co": [...]}]
+                  "graph", "sett_i", "sett_f", "area": [...], "dp": [...], "tr": [...], "co": [...],
+                  "spline": obs_spline weighting each step's transition counts, or -1}]
   sections      [(ti, tf), ...] PMCMC observation sections
 
 `param_index(spec, name)` finds parameters by name.

@@ -35,7 +35,8 @@ Supported: one or more age groups (`ages`, `age_mixing_matrix`, `age:` parameter
 `R_spline` and `efoi_spline` (one entry each), `area_covars`, `tv_covars`
 and `area_tv_covars` (func log or linear), `area_effect` and `level_effect`
 (fixed or MDir(k) priors), `age_mixing_modify` (row_column) and
-`age_mixing_perturb` (a contact matrix that varies by step), `data_tables` of type
+`age_mixing_perturb` (a contact matrix that varies by step), `obs_spline`
+(named by a table's `factor_spline`), `data_tables` of type
 'transition' or 'population' with obsmodel 'normal <sd>', 'normal <P>%',
 'normal load' or 'poisson'.  Presentation keys are ignored; `modification`,
 `prediction_start` and `prediction_end` act only in the reference's
@@ -66,7 +67,7 @@ DERIVED_KEYS = {"prob_reach"}
 SUPPORTED_KEYS = {"time_format", "start", "end", "datadir", "steps_per_unit_time", "comps", "trans", "R_spline",
                   "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str", "geo_mixing_matrix",
                   "geo_mixing_modify", "ages", "age_mixing_matrix", "area_covars", "tv_covars", "area_tv_covars",
-                  "area_effect", "level_effect", "age_mixing_modify", "age_mixing_perturb"}
+                  "area_effect", "level_effect", "age_mixing_modify", "age_mixing_perturb", "obs_spline"}
 
 
 class TomlError(ValueError):
@@ -784,8 +785,32 @@ def _covariates(rd, doc, P, area_head, area_rows):
     return out
 
 
-def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages):
-    """data_tables -> spec observations (one age group)."""
+def _obs_splines(rd, doc, P, splines):
+    """obs_spline entries (Model::add_splines, src/model.cc:611-629): a
+    spline per entry that some data table names in 'factor_spline'; its knot
+    parameters are named after the spline ('<name> t=<bp>' with more than two
+    knots, src/inputs.cc:1388-1401).  Appends the splines; returns name ->
+    spline index."""
+    used = {_s(e.get("factor_spline", "")) for e in doc.get("data_tables", [])} - {""}
+    out = {}
+    for e in doc.get("obs_spline", []):
+        _check_keys(e, {"name", "bp", "param", "value", "prior", "smooth", "smooth_type", "factor", "factor_param",
+                        "factor_value", "factor_prior"}, "obs_spline")
+        name = _s(e.get("name", ""))
+        if name == "":
+            raise TomlError("each spline in 'obs_spline' must have 'name' set")
+        if name in used:
+            splines.append(_spline(rd, P, "obs_spline", e,
+                                   lambda i, bp, name=name: name + (f" t={bp[i]}" if len(bp) > 2 else "")))
+            out[name] = len(splines) - 1
+    missing = used - set(out)
+    if missing:
+        raise TomlError(f"in 'data_tables' cannot find the observation spline '{sorted(missing)[0]}'")
+    return out
+
+
+def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages, obs_spl):
+    """data_tables -> spec observations."""
     thr_str = _s(doc.get("threshold_str", "*"))
     nodata_str = _s(doc.get("nodata_str", "."))
     obs = []
@@ -793,8 +818,8 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages):
     tables = doc.get("data_tables", [])
     for d, e in enumerate(tables):
         _check_keys(e, {"type", "observation", "timestep", "obsmodel", "file", "factor", "shift", "start", "end",
-                        "threshold", "epsilon_factor", "plot_name", "line_colour", "geo_dep", "democats_dep"},
-                     "data_tables")
+                        "threshold", "epsilon_factor", "plot_name", "line_colour", "geo_dep", "democats_dep",
+                        "factor_spline"}, "data_tables")
         typ = _s(e.get("type"))
         if typ not in ("transition", "population"):
             raise TomlError(f"data table type '{typ}' is not supported (transition or population)")
@@ -909,8 +934,16 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages):
             sc = _col(head, name + " SD") if load_sd else None
             graph = ngraph
             ngraph += 1
-            obs += _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, dps, graph, d, otype, sd, percent,
-                                 factor, thresh, thr_str, nodata_str, tr_list, co_list, file)
+            col = _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, dps, graph, d, otype, sd, percent,
+                                factor, thresh, thr_str, nodata_str, tr_list, co_list, file)
+            fs = _s(e.get("factor_spline", ""))
+            if fs:
+                # Population observations cannot take a spline (emsgEC("ObsModel",3), src/obsmodel.cc:100).
+                if typ != "transition":
+                    raise TomlError(f"'factor_spline' in the population table '{file}' is not supported")
+                for o in col:
+                    o["spline"] = obs_spl[fs]
+            obs += col
     return obs
 
 
@@ -1051,6 +1084,7 @@ def load(path, datadir=None, with_data=True):
             raise TomlError(f"in 'prob_reach' cannot find the compartment '{_s(e.get('comp', ''))}'")
     area_eff, level, eff_groups = _effects(rd, doc, P, head, rows, pop)
     age_mod, mod_groups = _age_mods(rd, doc, P, ages, pop_age, splines)
+    obs_spl = _obs_splines(rd, doc, P, splines)
 
     # Age mixing (Data::age_mixing_matrix, src/data_matrix.cc:138-175): a
     # G x G table without a heading, normalised later by the host model.
@@ -1110,6 +1144,6 @@ def load(path, datadir=None, with_data=True):
         "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet + eff_groups + mod_groups, "sus": sus_par,
         "covars": _covariates(rd, doc, P, head, rows), "area_effect": area_eff, "level_effect": level,
         "age_mod": age_mod,
-        "obs": _observations(rd, doc, comps, trans, with_data, head, rows, ages),
+        "obs": _observations(rd, doc, comps, trans, with_data, head, rows, ages, obs_spl),
     }
     return spec, np.array(P.values), meta

@@ -119,6 +119,10 @@ typedef struct {
   int32_t ndp;   const int32_t* dp;
   int32_t ntr;   const int32_t* tr;
   int32_t nco;   const int32_t* co;
+  int32_t factor_spline;        /* 1 + the index of the obs_spline multiplying each step's
+                                   count by its value at that step (src/obsmodel.cc:85-88);
+                                   0 = none, so a zero-initialised observation has none;
+                                   transition observations only                         */
 } beep_obs;
 
 typedef struct {

@@ -349,6 +349,7 @@ struct OrObs {
   const int* el_c;
   const int* el_x;
   const int* el_dp;
+  const double* el_w;  // [nel] obs_spline value at the element's step, or null (no spline)
 };
 
 // get_obs_value_section restricted to steps [ti, tf).
@@ -360,7 +361,8 @@ void or_obs_values(const OrModel* m, const OrObs* ob, const OrState* st, int ti,
       if (s < ti || s >= tf) continue;
       double num = ob->el_kind[e] == 0 ? st->transnum[ix_tr(m, s, ob->el_c[e], ob->el_x[e], ob->el_dp[e])]
                                        : st->pop[ix_pop(m, s, ob->el_c[e], ob->el_x[e], ob->el_dp[e])];
-      if (num != 0) v += ob->factor[i] * num;
+      // With an observation spline: factor * num * spline(s) (src/obsmodel.cc:85-88).
+      if (num != 0) v += ob->el_w ? ob->factor[i] * num * ob->el_w[e] : ob->factor[i] * num;
     }
     value[i] = v;
   }

@@ -81,7 +81,8 @@ class _OrState(C.Structure):
 class _OrObs(C.Structure):
     _fields_ = [("nobs", C.c_int)] + [(n, C.POINTER(C.c_double)) for n in ("y", "sd")] + \
                [("type", C.POINTER(C.c_int))] + [(n, C.POINTER(C.c_double)) for n in ("shape", "factor", "thresh")] + \
-               [(n, C.POINTER(C.c_int)) for n in ("el_ptr", "el_kind", "el_s", "el_c", "el_x", "el_dp")]
+               [(n, C.POINTER(C.c_int)) for n in ("el_ptr", "el_kind", "el_s", "el_c", "el_x", "el_dp")] + \
+               [("el_w", C.POINTER(C.c_double))]
 
 
 def _d(a):
@@ -105,6 +106,7 @@ class State:
         self.Imap = np.zeros((S, A * G))
         self.Idiag = np.zeros((S, A * G))
         self._c = _OrState(_d(self.pop), _d(self.transnum), _d(self.transmean), _d(self.Imap), _d(self.Idiag))
+        self.tb = None          # the tables the trajectory was made with
 
 
 class Oracle:
@@ -133,8 +135,9 @@ class Oracle:
         spec = self.spec
         obs = spec.get("obs", [])
         sd = derive.obs_sd(spec)
-        ptr, kind, s_, c_, x_, dp_ = [0], [], [], [], [], []
+        ptr, kind, s_, c_, x_, dp_, sp_ = [0], [], [], [], [], [], []
         for o in obs:
+            n0 = len(kind)
             for s in range(o["sett_i"], o["sett_f"]):
                 for c in o["area"]:
                     for tr in o.get("tr", []):
@@ -143,7 +146,9 @@ class Oracle:
                     for co in o.get("co", []):
                         for dp in o["dp"]:
                             kind.append(1); s_.append(s); c_.append(c); x_.append(co); dp_.append(dp)
+            sp_ += [o.get("spline", -1)] * (len(kind) - n0)
             ptr.append(len(kind))
+        self._el_s, self._el_sp = np.array(s_, dtype=np.int64), np.array(sp_, dtype=np.int64)
         self._obs_arrays = [
             np.array([o["y"] for o in obs] or [0.0]), np.ascontiguousarray(sd if len(sd) else np.zeros(1)),
             np.array([OBS_TYPES[o["type"]] if isinstance(o["type"], str) else o["type"] for o in obs] or [0],
@@ -155,7 +160,7 @@ class Oracle:
         ]
         a = self._obs_arrays
         self.obs = _OrObs(len(obs), _d(a[0]), _d(a[1]), _i(a[2]), _d(a[3]), _d(a[4]), _d(a[5]),
-                          _i(a[6]), _i(a[7]), _i(a[8]), _i(a[9]), _i(a[10]), _i(a[11]))
+                          _i(a[6]), _i(a[7]), _i(a[8]), _i(a[9]), _i(a[10]), _i(a[11]), None)
         self.obs_sd = sd
 
     def tables(self, pv):
@@ -166,6 +171,7 @@ class Oracle:
         t = _OrTables(*[_d(a) for a in arrs])
         t._keep = arrs
         t.derived = d
+        t.pv = np.asarray(pv, dtype=np.float64)
         return t
 
     def new_state(self):
@@ -180,6 +186,7 @@ class Oracle:
                            self.S if tf is None else tf)
         if rc:
             raise RuntimeError(f"oracle simulate failed: {rc}")
+        st.tb = tb
         return st
 
     def replay(self, tb, transnum):
@@ -188,13 +195,28 @@ class Oracle:
         rc = lib().or_replay(C.byref(self.model), C.byref(tb), _d(self.pop_init), C.byref(st._c))
         if rc:
             raise RuntimeError(f"oracle replay failed: {rc}")
+        st.tb = tb
         return st
 
     def likelihood(self, st):
         return lib().or_likelihood(C.byref(self.model), C.byref(st._c))
 
-    def obs_loglik(self, st, ti=0, tf=None, sel=None):
+    def obs_loglik(self, st, ti=0, tf=None, sel=None, tb=None):
+        """ObservationModel::calculate over steps [ti, tf); observation
+        splines take their values from tb (default the tables the state was
+        simulated or replayed with)."""
         values = np.zeros(max(1, len(self.spec.get("obs", []))))
+        w = None
+        if len(self._el_sp) and (self._el_sp >= 0).any():
+            tb = tb if tb is not None else st.tb
+            spl = {sp: derive.disc_spline(self.spec, sp, derive.dirichlet_correct(self.spec, tb.pv))
+                   for sp in set(self._el_sp[self._el_sp >= 0].tolist())}
+            w = np.ones(len(self._el_sp))
+            for sp, v in spl.items():
+                sel_e = self._el_sp == sp
+                w[sel_e] = v[self._el_s[sel_e]]
+        self._w = w
+        self.obs.el_w = _d(w) if w is not None else None
         if sel is not None:
             sel = np.ascontiguousarray(sel, dtype=np.int32)
         L = lib().or_obs_loglik(C.byref(self.model), C.byref(self.obs), C.byref(st._c), ti,
@@ -214,6 +236,7 @@ class Oracle:
                       inf_update)
         if rc:
             raise RuntimeError(f"oracle mbp failed: {rc}")
+        pro.tb = tbp
         return pro
 
 

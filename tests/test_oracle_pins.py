@@ -176,6 +176,35 @@ def test_time_varying_age_matrix_closed_form():
         assert np.ptp(d["ratio"]) > 0.01 * d["ratio"].mean()
 
 
+def test_observation_spline_values_by_loops():
+    """ObservationModel::get_obs_value_section with an obs_spline
+    (src/obsmodel.cc:74-109): each step's transition count times factor
+    times the spline at that step, here from numpy.interp of the knot values
+    at t = s + 0.5, summed by loops over a simulated trajectory; the other
+    observations are plain sums."""
+    import toml_models
+    spec, truth = toml_models.toml_seird_branch()
+    orc = O.Oracle(spec)
+    for pv in (truth, truth * np.random.default_rng(4).uniform(0.8, 1.2, len(truth))):
+        tb = orc.tables(pv)
+        st = orc.simulate(tb, seed=17)
+        _, values = orc.obs_loglik(st)
+        pd = derive.dirichlet_correct(spec, pv)
+        for i, o in enumerate(spec["obs"]):
+            w = np.ones(orc.S)
+            if o.get("spline", -1) >= 0:
+                sp = spec["splines"][o["spline"]]
+                w = np.interp(np.arange(orc.S) + 0.5, sp["t"], [pd[k] for k in sp["param"]])
+            v = 0.0
+            for s in range(o["sett_i"], o["sett_f"]):
+                for c in o["area"]:
+                    for tr in o["tr"]:
+                        for dp in o["dp"]:
+                            v += o.get("factor", 1.0) * st.transnum[s, c, tr, dp] * w[s]
+            assert values[i] == pytest.approx(v, rel=1e-13, abs=1e-12)
+        assert any(o.get("spline", -1) >= 0 and values[i] > 0 for i, o in enumerate(spec["obs"]))
+
+
 def test_deterministic_seir_is_forward_euler():
     """Deterministic dynamics with one area: the forward-Euler SEIR recursion
     with FOI = beta I / N (Idiag = I / N when A = 1)."""

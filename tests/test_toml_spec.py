@@ -143,6 +143,34 @@ def test_rejections(tmp_path, extra, msg):
         _model(tmp_path, extra)
 
 
+def test_observation_spline(tmp_path):
+    """obs_spline named by a data table's factor_spline (src/model.cc:611-629,
+    1429-1439): a spline whose knot parameters carry the spline's name and
+    knot time, attached to every observation of that table; none on the
+    others; refused on a population table (src/obsmodel.cc:100) or when the
+    name is not found."""
+    import toml_models
+    spec, values = toml_models.toml_seird_branch()
+    names = [p["name"] for p in spec["params"]]
+    se = [o for o in spec["obs"] if o["datatable"] == 1]
+    k = se[0]["spline"]
+    sp = spec["splines"][k]
+    assert sp["t"] == [0.0, 42.0, 84.0]
+    assert [names[i] for i in sp["param"]] == ["ascertainment t=0", "ascertainment t=42", "ascertainment t=84"]
+    assert [values[i] for i in sp["param"]] == [0.6, 0.9, 0.8]
+    assert all(o["spline"] == k for o in se)
+    assert all(o.get("spline", -1) == -1 for o in spec["obs"] if o["datatable"] == 0)
+    text = open(os.path.join(FIX, "seird_branch.toml")).read()
+    bad = text.replace('factor_spline="ascertainment"', 'factor_spline="detection"')
+    with pytest.raises(T.TomlError, match="cannot find the observation spline 'detection'"):
+        T.load(_write(str(tmp_path), "a.toml", bad), datadir=os.path.join(FIX, "data_seird_branch"))
+    bad = text.replace('obsmodel="poisson", file="I-D.csv"',
+                       'obsmodel="poisson", file="I-D.csv"}, {type="population", observation="D", '
+                       'obsmodel="poisson", factor_spline="ascertainment", file="I-D.csv"')
+    with pytest.raises(T.TomlError, match="population table"):
+        T.load(_write(str(tmp_path), "b.toml", bad), datadir=os.path.join(FIX, "data_seird_branch"))
+
+
 def test_prob_reach_recorded(tmp_path):
     """prob_reach is an output-only derived quantity (src/model.cc:1932-1957):
     recorded in the metadata, its compartment checked."""

@@ -92,7 +92,9 @@ class Spec(C.Structure):
                 ("section_ti", C.POINTER(C.c_int32)), ("section_tf", C.POINTER(C.c_int32)),
                 ("area_effect_param", C.POINTER(C.c_int32)), ("nlevel", C.c_int32),
                 ("level_param", C.POINTER(C.c_int32)), ("level_map", C.POINTER(C.c_int32)),
-                ("nagemod", C.c_int32), ("agemod", C.POINTER(AgeMod))]
+                ("nagemod", C.c_int32), ("agemod", C.POINTER(AgeMod)),
+                ("nage", C.c_int32), ("dp_age", C.POINTER(C.c_int32)),
+                ("nsus_extra", C.c_int32), ("sus_extra", C.POINTER(C.c_int32))]
 
 
 _lib = None
@@ -294,6 +296,13 @@ class _SpecBuffers:
             ams[i].spline, ams[i].nage, ams[i].ages = x["spline"], len(x["ages"]), i32(x["ages"])
         keep.append(ams)
         s.nagemod, s.agemod = len(am), ams
+        if spec.get("dp_age") is not None:
+            s.nage, s.dp_age = spec["nage"], i32(spec["dp_age"])
+        else:
+            s.nage, s.dp_age = G, null_i32
+        se = spec.get("sus_extra") or []
+        s.nsus_extra = len(se)
+        s.sus_extra = i32(np.asarray(se, dtype=np.int32).reshape(-1)) if se else null_i32
         ae = spec.get("area_effect")
         s.area_effect_param = i32(ae) if ae is not None else null_i32
         le = spec.get("level_effect")

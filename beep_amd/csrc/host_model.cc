@@ -142,6 +142,23 @@ std::string HostModel::build(const beep_spec& s) {
   efoi_agedist.assign(G, 0.0);
   if (s.efoi_agedist) efoi_agedist.assign(s.efoi_agedist, s.efoi_agedist + G);
   if (s.sus_param) sus_param.assign(s.sus_param, s.sus_param + G);
+  sus_extra.clear();
+  for (int k = 0; k < s.nsus_extra; k++) {
+    sus_extra.emplace_back(s.sus_extra + (size_t)k * G, s.sus_extra + (size_t)(k + 1) * G);
+    for (int th : sus_extra.back())
+      if (th >= nparam) return "susceptibility parameter out of range";
+  }
+  if (s.dp_age) {
+    nage = s.nage;
+    if (nage <= 0 || nage > G) return "bad number of age groups";
+    dp_age.assign(s.dp_age, s.dp_age + G);
+    for (int a : dp_age)
+      if (a < 0 || a >= nage) return "dp age group out of range";
+  } else {
+    nage = G;
+    dp_age.resize(G);
+    for (int g = 0; g < G; g++) dp_age[g] = g;
+  }
   dirichlet.resize(s.ndirichlet);
   for (int k = 0; k < s.ndirichlet; k++) {
     dirichlet[k].param.assign(s.dirichlet[k].param, s.dirichlet[k].param + s.dirichlet[k].n);
@@ -168,25 +185,33 @@ std::string HostModel::build(const beep_spec& s) {
   Rdist.assign(G, 0.0);                                       // SplineInfo::democatpos_dist (src/model.cc:409-424)
   for (int g = 0; g < G; g++) Rdist[g] = pop_age[g] / popsize;
 
-  // Age mixing: Data::age_mixing_matrix + agematrix_normalise (src/data_matrix.cc:100-202).
-  N.assign((size_t)G * G, 0.0);
+  // Age mixing: Data::age_mixing_matrix + agematrix_normalise
+  // (src/data_matrix.cc:100-202), over the age groups' populations.
+  std::vector<double> pop_a(nage, 0.0);
+  for (int g = 0; g < G; g++) pop_a[dp_age[g]] += pop_age[g];
+  N_age.assign((size_t)nage * nage, 0.0);
   if (s.N_raw) {
-    std::vector<double> m((size_t)G * G);
-    for (int j = 0; j < G; j++)
-      for (int i = 0; i < G; i++) m[(size_t)j * G + i] = s.N_raw[(size_t)j * G + i] / (pop_age[j] / popsize);
-    for (int j = 0; j < G; j++)
-      for (int i = 0; i < G; i++) N[(size_t)j * G + i] = 0.5 * (m[(size_t)j * G + i] + m[(size_t)i * G + j]);
+    std::vector<double> m((size_t)nage * nage);
+    for (int j = 0; j < nage; j++)
+      for (int i = 0; i < nage; i++) m[(size_t)j * nage + i] = s.N_raw[(size_t)j * nage + i] / (pop_a[j] / popsize);
+    for (int j = 0; j < nage; j++)
+      for (int i = 0; i < nage; i++)
+        N_age[(size_t)j * nage + i] = 0.5 * (m[(size_t)j * nage + i] + m[(size_t)i * nage + j]);
   } else {
-    if (G != 1) return "an age mixing matrix is needed when there is more than one age group";
-    N[0] = 1;
+    if (nage != 1) return "an age mixing matrix is needed when there is more than one age group";
+    N_age[0] = 1;
   }
   {
     double sum = 0, sum2 = 0;
-    for (int a = 0; a < G; a++)
-      for (int aa = 0; aa < G; aa++) { sum += pop_age[aa] * N[(size_t)aa * G + a]; sum2 += pop_age[aa]; }
+    for (int a = 0; a < nage; a++)
+      for (int aa = 0; aa < nage; aa++) { sum += pop_a[aa] * N_age[(size_t)aa * nage + a]; sum2 += pop_a[aa]; }
     sum /= sum2;
-    for (auto& v : N) v /= sum;
+    for (auto& v : N_age) v /= sum;
   }
+  // Between dps, the mixing of their age groups (State::get_NMI, src/state.cc:547-578).
+  N.assign((size_t)G * G, 0.0);
+  for (int q = 0; q < G; q++)
+    for (int d = 0; d < G; d++) N[(size_t)q * G + d] = N_age[(size_t)dp_age[q] * nage + dp_age[d]];
 
   // Geographic mixing: Data::geo_normalise (src/data_matrix.cc:55-96).
   M_rowptr.assign(A + 1, 0);
@@ -250,7 +275,7 @@ std::string HostModel::build(const beep_spec& s) {
     if (x.spline < 0 || x.spline >= s.nspline) return "age matrix modification spline out of range";
     AgeMod am{x.spline, std::vector<int>(x.ages, x.ages + x.nage)};
     for (int a : am.ages)
-      if (a < 0 || a >= G) return "age matrix modification age group out of range";
+      if (a < 0 || a >= nage) return "age matrix modification age group out of range";
     agemod.push_back(am);
   }
   level_param.clear(); level_map.clear();
@@ -411,30 +436,37 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
       tb.rate[(size_t)t * G + g] = r;
     }
   }
-  // Model::create_susceptibility (src/model.cc:1144-1178), age the only category.
+  // Model::create_susceptibility (src/model.cc:1144-1178): the product over categories.
   tb.sus.assign(G, 1.0);
   if (!sus_param.empty())
     for (int g = 0; g < G; g++) tb.sus[g] = sus_param[g] >= 0 ? pd[sus_param[g]] : 1.0;
+  for (const auto& f : sus_extra)
+    for (int g = 0; g < G; g++)
+      if (f[g] >= 0) tb.sus[g] *= pd[f[g]];
   tb.inf.assign(C, 0.0);
   for (int c = 0; c < C; c++) tb.inf[c] = comp[c].inf_param >= 0 ? pd[comp[c].inf_param] : 0.0;
   // Model::create_Ntime (src/model.cc:1254-1301): each modification
   // multiplies its age groups' rows and columns by its spline's value.
+  // The age matrix is modified, then spread over the dps.
   tb.Ntime = N;
   if (!agemod.empty()) {
     std::vector<std::vector<double>> fac;
     for (const AgeMod& am : agemod) fac.push_back(disc_spline(am.spline, pd));
     tb.Ntime.assign((size_t)S * G * G, 0.0);
+    std::vector<double> Na(N_age.size());
     for (int s = 0; s < S; s++) {
-      double* Ns = tb.Ntime.data() + (size_t)s * G * G;
-      std::copy(N.begin(), N.end(), Ns);
+      Na = N_age;
       for (size_t k = 0; k < agemod.size(); k++) {
         const double f = fac[k][s];
         for (int a : agemod[k].ages)
-          for (int i = 0; i < G; i++) {
-            Ns[(size_t)a * G + i] *= f;
-            Ns[(size_t)i * G + a] *= f;
+          for (int i = 0; i < nage; i++) {
+            Na[(size_t)a * nage + i] *= f;
+            Na[(size_t)i * nage + a] *= f;
           }
       }
+      double* Ns = tb.Ntime.data() + (size_t)s * G * G;
+      for (int q = 0; q < G; q++)
+        for (int d = 0; d < G; d++) Ns[(size_t)q * G + d] = Na[(size_t)dp_age[q] * nage + dp_age[d]];
     }
   }
 

@@ -59,6 +59,9 @@ class HostModel {
   struct AgeMod { int spline; std::vector<int> ages; };
   std::vector<AgeMod> agemod;                                 // empty: N constant in time
   std::vector<int> sus_param;
+  std::vector<std::vector<int>> sus_extra;                   // further factors, [G] each
+  int nage = 0;
+  std::vector<int> dp_age;                                    // [G] age group of each dp
   int R_spline = -1, Rfactor_param = -1, efoi_spline = -1, geo_spline = -1;
   std::vector<double> efoi_agedist;
   std::vector<double> Rdist;          // [G] population fraction of the R-spline area group
@@ -70,7 +73,8 @@ class HostModel {
   // incoming (src, w) of w * dinf[src].
   std::vector<int> in_ptr, in_src;
   std::vector<double> in_w;
-  std::vector<double> N;              // [G][G] normalised age mixing
+  std::vector<double> N_age;          // [nage][nage] normalised age mixing
+  std::vector<double> N;              // [G][G] the same for each pair of dps
   std::vector<int> pop_init;          // [A][C][G]
   std::vector<double> area_pop;       // [A]
   std::vector<int> inf_state;         // infected compartments (Model::set_infected_uninfected)

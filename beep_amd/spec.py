@@ -2,7 +2,8 @@
 
 A spec is a plain dict (JSON-serialisable once arrays are lists):
 
-  A, G          areas, age groups (dp == age group, one strain)
+  A, G          areas, demographic groups (dp; the age groups unless democats)
+  nage, dp_age  age groups and each dp's age group (absent: dp == age group)
   period, spd   days, steps per day (details.division_per_time)
   comps         [{"name", "inf": param or -1, "mean": [G] params or None, "shape": int}]
                 after Erlang expansion (src/inputs_comps.cc:33-46)
@@ -12,7 +13,7 @@ A spec is a plain dict (JSON-serialisable once arrays are lists):
                   | ("normal", mu, sd) | ("dirflat",) | ("gamma", shape, rate)}]
   splines       [{"t": [...], "param": [...], "multfac": [...], "factor": param or -1}]
   R_spline, Rfactor, efoi_spline, efoi_agedist, geo_spline
-  N             [G][G] age contact table or None (unit matrix for G == 1)
+  N             [nage][nage] age contact table or None (unit matrix for one age group)
   M             {"diag": [A], "rowptr": [A+1], "to": [nnz], "val": [nnz]} raw, or None for A == 1
   pop           [A][C][G] initial populations
   covars        [{"param", "value": [A][period]}] (centred on load)
@@ -22,6 +23,7 @@ A spec is a plain dict (JSON-serialisable once arrays are lists):
                 columns of the normalised contact matrix are multiplied by the
                 spline's value (age_mixing_modify / age_mixing_perturb)
   sus           [G] params or None
+  sus_extra     [[G] params]: further susceptibility factors (one per category)
   dirichlet     [{"param": [...], "frac": [...]}]
   obs           [{"y", "sd", "type", "shape", "factor", "thresh", "percent", "datatable",
                   "graph", "sett_i", "sett_f", "area": [...], "dp": [...], "tr": [...], "co": [...],

@@ -67,7 +67,8 @@ DERIVED_KEYS = {"prob_reach"}
 SUPPORTED_KEYS = {"time_format", "start", "end", "datadir", "steps_per_unit_time", "comps", "trans", "R_spline",
                   "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str", "geo_mixing_matrix",
                   "geo_mixing_modify", "ages", "age_mixing_matrix", "area_covars", "tv_covars", "area_tv_covars",
-                  "area_effect", "level_effect", "age_mixing_modify", "age_mixing_perturb", "obs_spline"}
+                  "area_effect", "level_effect", "age_mixing_modify", "age_mixing_perturb", "obs_spline",
+                  "democats", "init_pop"}
 
 
 class TomlError(ValueError):
@@ -243,12 +244,44 @@ def _no_dep(entry, keys, where):
             raise TomlError(f"{where}: the dependency in '{k}={v}' is not supported")
 
 
-def _ct_specs(rd, entry, keys, par_root, ages, where):
-    """Per age group, the (name, value, prior) strings of a compartment or
-    transition parameter (Inputs::get_dep and find_ct_param_spec,
-    src/inputs.cc:1107-1143,1284-1326): 'age: v0 | v1 | ...' gives one
-    parameter per age group, named '<age> <root>' unless named; without a
-    dependency one parameter serves every age group."""
+class _Demo:
+    """Demographic categories (Inputs::find_demographic_category,
+    src/inputs.cc:616-660): age first, then the 'democats' entries.  dp
+    enumerates their combinations with the first category varying fastest
+    (src/data.cc:55-73); dp_age is each dp's age group."""
+
+    def __init__(self, cats):
+        self.cats = cats                       # [(name, [values])]
+        self.dps = []
+        idx = [0] * len(cats)
+        while True:
+            self.dps.append(tuple(idx))
+            k = 0
+            while k < len(cats):
+                idx[k] += 1
+                if idx[k] < len(cats[k][1]):
+                    break
+                idx[k] = 0
+                k += 1
+            if k == len(cats):
+                break
+        self.G = len(self.dps)
+        self.ages = cats[0][1]
+        self.nage = len(self.ages)
+        self.dp_age = [t[0] for t in self.dps]
+
+    def find(self, name):
+        """A category by name, without case (Data::get_datafilter, src/data.cc:1491-1493)."""
+        return next((k for k, (n, _) in enumerate(self.cats) if n.lower() == name.lower()), None)
+
+
+def _ct_specs(rd, entry, keys, par_root, demo, where):
+    """Per dp, the (name, value, prior) strings of a compartment or transition
+    parameter (Inputs::get_dep, find_demo_ref and find_ct_param_spec,
+    src/inputs.cc:1107-1143,1217-1326): 'cat: v0 | v1 | ...' (or 'c1,c2: ...',
+    the first category varying fastest) gives one parameter per value, named
+    '<value> <root>' unless named; without a dependency one parameter serves
+    every dp."""
     strs, dep = [], ""
     for k in keys:
         st = _s(entry.get(k, ""))
@@ -265,16 +298,30 @@ def _ct_specs(rd, entry, keys, par_root, ages, where):
             dep = d
         strs.append(st)
     if dep == "":
-        return [tuple(strs)] * len(ages)
-    if dep != "age":
-        raise TomlError(f"{where}: the dependency '{dep}' is not supported")
-    if len(ages) == 1:
-        raise TomlError(f"{where}: the dependency 'age:' needs more than one age group")
-    G = len(ages)
-    names = rd.split_n(strs[0], G, where)
-    vals = rd.split_n(strs[1], G, where)
-    pris = rd.split_n(strs[2], G, where)
-    return [(names[g] or f"{ages[g]} {par_root}", vals[g], pris[g]) for g in range(G)]
+        return [tuple(strs)] * demo.G
+    deps = []
+    for d in (x.strip() for x in dep.split(",")):
+        k = next((k for k, (n, _) in enumerate(demo.cats) if n == d), None)
+        if k is None:
+            raise TomlError(f"{where}: the dependency '{d}' cannot be found")
+        if len(demo.cats[k][1]) == 1:
+            raise TomlError(f"{where}: the dependency '{dep}:' cannot be used because '{d}' only contains one value")
+        if k in deps:
+            raise TomlError(f"{where}: the value '{dep}' contains multiple dependencies")
+        deps.append(k)
+    combos = [()]
+    for k in deps:                                 # deps[0] varies fastest
+        combos = [c + (v,) for v in range(len(demo.cats[k][1])) for c in combos]
+    n = len(combos)
+    names = rd.split_n(strs[0], n, where)
+    vals = rd.split_n(strs[1], n, where)
+    pris = rd.split_n(strs[2], n, where)
+    out = []
+    for t in demo.dps:
+        i = combos.index(tuple(t[k] for k in deps))
+        dflt = ",".join(demo.cats[k][1][combos[i][j]] for j, k in enumerate(deps)) + " " + par_root
+        out.append((names[i] or dflt, vals[i], pris[i]))
+    return out
 
 
 def _check_keys(entry, allowed, where):
@@ -283,7 +330,7 @@ def _check_keys(entry, allowed, where):
             raise TomlError(f"{where}: '{k}' is not supported")
 
 
-def _compartments(rd, doc, P, ages):
+def _compartments(rd, doc, P, demo):
     """comps -> expanded compartments (src/inputs_comps.cc:22-63,303-365)."""
     if "comps" not in doc:
         raise TomlError("the file must define 'comps'")
@@ -312,7 +359,7 @@ def _compartments(rd, doc, P, ages):
             raise TomlError(f"compartment '{nam}': dist '{dist}' is not recognised")
         mean = None
         if any(x in e for x in ("mean_param", "mean_value", "mean_prior")):
-            specs = _ct_specs(rd, e, ("mean_param", "mean_value", "mean_prior"), f"{nam} mean occupancy", ages,
+            specs = _ct_specs(rd, e, ("mean_param", "mean_value", "mean_prior"), f"{nam} mean occupancy", demo,
                               f"compartment '{nam}'")
             mean = [P.add(nm or f"{nam} mean occupancy", v, pr) for nm, v, pr in specs]
         inf = -1
@@ -327,7 +374,7 @@ def _compartments(rd, doc, P, ages):
     return comps
 
 
-def _transitions(rd, doc, comps, P, ages):
+def _transitions(rd, doc, comps, P, demo):
     """trans -> internal Erlang transitions first, then the file's (src/inputs_comps.cc:66-139,268-300)."""
     if "trans" not in doc:
         raise TomlError("the file must define 'trans'")
@@ -360,13 +407,13 @@ def _transitions(rd, doc, comps, P, ages):
         if any(x in e for x in ("prob_param", "prob_value", "prob_prior")):
             root = f"{fr}->{to} branch prob."
             branch = [(nm or root, v, pr) for nm, v, pr in
-                      _ct_specs(rd, e, ("prob_param", "prob_value", "prob_prior"), root, ages, f"transition {fr}->{to}")]
+                      _ct_specs(rd, e, ("prob_param", "prob_value", "prob_prior"), root, demo, f"transition {fr}->{to}")]
         ninf += inf == "yes"
         trans.append({"from": cf, "to": ct, "infection": inf == "yes", "prob": None, "base": f"{fr}->{to}",
                       "branch": branch})
     if ninf != 1:
         raise TomlError("exactly one infection transition must exist")
-    return trans, _branches(trans, comps, P, len(ages))
+    return trans, _branches(trans, comps, P, demo.G)
 
 
 def _dir_kind(prior):
@@ -785,6 +832,92 @@ def _covariates(rd, doc, P, area_head, area_rows):
     return out
 
 
+def _initial_population(rd, doc, demo, comps, trans, head, rows):
+    """Initial populations [A][C][G] (Data::read_initial_population,
+    src/data.cc:288-457).  From the areas file: everyone in the infection's
+    source compartment, one column per age group times, for every other
+    category, the fraction of its value in the area (Data::get_demo_frac,
+    src/data.cc:821-870: one column per value, each divided by their sum),
+    rounded to the nearest integer.  From an 'init_pop' file: one row per
+    area, compartment and category value (columns 'area', 'compartment',
+    'population' and one per category with more than one value), the number
+    placed in the first stage of the compartment; every combination must be
+    given."""
+    A, G, C = len(rows), demo.G, len(comps)
+    codes = [r[_col(head, "area")] for r in rows]
+    if "init_pop" in doc:
+        ph, prow = rd.table(_s(doc["init_pop"]))
+        cols = {k: _col(ph, k) for k in ("area", "compartment", "population")}
+        for k, c in cols.items():
+            if c is None:
+                raise TomlError(f"the 'init_pop' file needs a column '{k}'")
+        dcol = [(_col(ph, n) if len(v) > 1 else None) for n, v in demo.cats]
+        for (n, v), c in zip(demo.cats, dcol):
+            if len(v) > 1 and c is None:
+                raise TomlError(f"the 'init_pop' file needs a column '{n}'")
+        pop = [[[None] * G for _ in range(C)] for _ in range(A)]
+        for r in prow:
+            if r[cols["area"]] not in codes:
+                raise TomlError(f"in the 'init_pop' file the area '{r[cols['area']]}' is not recognised")
+            a = codes.index(r[cols["area"]])
+            t = []
+            for (n, v), c in zip(demo.cats, dcol):
+                if c is None:
+                    t.append(0)
+                elif r[c] not in v:
+                    raise TomlError(f"in the 'init_pop' file '{r[c]}' is not a value of '{n}'")
+                else:
+                    t.append(v.index(r[c]))
+            dp = demo.dps.index(tuple(t))
+            stages = [c for c, co in enumerate(comps) if co["base"] == r[cols["compartment"]]]
+            if not stages:
+                raise TomlError(f"in the 'init_pop' file the compartment '{r[cols['compartment']]}' is not recognised")
+            try:
+                v = float(r[cols["population"]])
+            except ValueError:
+                raise TomlError(f"the population '{r[cols['population']]}' is not a number") from None
+            if v < 0:
+                raise TomlError("populations must be non-negative")
+            for i, c in enumerate(stages):
+                pop[a][c][dp] = v if i == 0 else 0
+        for c in range(C):
+            if any(pop[a][c][dp] is None for a in range(A) for dp in range(G)):
+                raise TomlError(f"the initial populations for '{comps[c]['base']}' are not all specified")
+        return [[[int(v) for v in row] for row in pa] for pa in pop]
+    age_cols = [_col(head, a) for a in demo.ages]
+    for a, c in zip(demo.ages, age_cols):
+        if c is None:
+            raise TomlError(f"the areas file needs a column '{a}'")
+
+    def num(x):
+        try:
+            v = float(x)
+        except ValueError:
+            raise TomlError(f"the population '{x}' is not a number") from None
+        if v < 0:
+            raise TomlError("populations must be non-negative")
+        return v
+
+    fracs = {}
+    for k in range(1, len(demo.cats)):
+        vc = [_col(head, v) for v in demo.cats[k][1]]
+        if None in vc:
+            raise TomlError(f"the areas file needs a column for each value of '{demo.cats[k][0]}'")
+        vals = [[num(r[c]) for c in vc] for r in rows]
+        fracs[k] = [[x / sum(v) for x in v] for v in vals]
+    sus = trans[[t["infection"] for t in trans].index(True)]["from"]
+    pop = []
+    for a, row in enumerate(rows):
+        pa = []
+        for t in demo.dps:
+            v = num(row[age_cols[t[0]]])
+            for k in range(1, len(demo.cats)):
+                v *= fracs[k][a][t[k]]
+            pa.append(int(v + 0.5))
+        pop.append([pa if c == sus else [0] * G for c in range(C)])
+    return pop
+
+
 def _obs_splines(rd, doc, P, splines):
     """obs_spline entries (Model::add_splines, src/model.cc:611-629): a
     spline per entry that some data table names in 'factor_spline'; its knot
@@ -809,7 +942,7 @@ def _obs_splines(rd, doc, P, splines):
     return out
 
 
-def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages, obs_spl):
+def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, demo, obs_spl):
     """data_tables -> spec observations."""
     thr_str = _s(doc.get("threshold_str", "*"))
     nodata_str = _s(doc.get("nodata_str", "."))
@@ -884,19 +1017,23 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, ages, 
         # (Data::get_datafilter, src/data.cc:1460-1560); columns absent from
         # the file are skipped, as the reference warns and skips them.
         geo_dep, demo_dep = _s(e.get("geo_dep", "")), _s(e.get("democats_dep", ""))
-        every_area, every_age = list(range(len(area_rows))), list(range(len(ages)))
+        every_area, every_dp = list(range(len(area_rows))), list(range(demo.G))
         if geo_dep and demo_dep:
             raise TomlError("in 'data_tables' values for 'democats_dep' and 'geo_dep' cannot both be set")
         if demo_dep:
-            if demo_dep.lower() != "age":
-                raise TomlError(f"'democats_dep={demo_dep}' is not supported (only 'age')")
-            if len(ages) == 1:
-                raise TomlError("'democats_dep=\"age\"' can only be used with more than one age group")
-            filters = [(a, every_area, [g]) for g, a in enumerate(ages)]
+            # One column per value of the category, over the dps holding it
+            # (Data::get_datafilter, src/data.cc:1491-1520).
+            k = demo.find(demo_dep)
+            if k is None:
+                raise TomlError(f"'democats_dep={demo_dep}' is not a demographic category")
+            if len(demo.cats[k][1]) == 1:
+                raise TomlError(f"'democats_dep=\"{demo_dep}\"' can only be used if '{demo_dep}' has more than one value")
+            filters = [(v, every_area, [dp for dp, t in enumerate(demo.dps) if t[k] == i])
+                       for i, v in enumerate(demo.cats[k][1])]
         elif geo_dep:
-            filters = [(name, areas, every_age) for name, areas in _regions(area_head, area_rows, geo_dep)]
+            filters = [(name, areas, every_dp) for name, areas in _regions(area_head, area_rows, geo_dep)]
         else:
-            filters = [("Data", every_area, every_age)]
+            filters = [("Data", every_area, every_dp)]
         cols = [(name, _col(head, name), areas, dps) for name, areas, dps in filters if _col(head, name) is not None]
         if not cols:
             raise TomlError(f"the data in '{file}' was not used: expected columns such as '{filters[0][0]}'")
@@ -1013,28 +1150,45 @@ def load(path, datadir=None, with_data=True):
             raise TomlError("'datadir' is not set")
     rd = _Reader(doc, datadir)
     P = _Params()
-    # Age groups (Inputs::find_demographic_category, src/inputs.cc:617-640):
-    # without 'ages' one group whose population column is 'population'.
-    ages, sus_specs = ["population"], None
+    # Demographic categories (Inputs::find_demographic_category,
+    # src/inputs.cc:617-660): 'ages' (without it one group whose population
+    # column is 'population'), then 'democats'; each may carry susceptibility
+    # parameters.
+    cats, sus_specs = [("age", ["population"])], {}
+    entries = []
     if "ages" in doc:
         ag = doc["ages"]
         if not isinstance(ag, dict):
             raise TomlError("'ages' must be a table with 'cats'")
         _check_keys(ag, {"cats", "sus_param", "sus_value", "sus_prior"}, "ages")
-        ages = _split(_s(ag.get("cats", "")), "|")
-        if not ages or "" in ages:
-            raise TomlError("in 'ages' the value of 'cats' must list the age groups")
-        if any(k in ag for k in ("sus_param", "sus_value", "sus_prior")):
-            G = len(ages)
-            sus_specs = list(zip(rd.split_n(_s(ag.get("sus_param", "")), G, "ages"),
-                                 rd.split_n(_s(ag.get("sus_value", "")), G, "ages"),
-                                 rd.split_n(_s(ag.get("sus_prior", "")), G, "ages")))
-    G = len(ages)
-    comps = _compartments(rd, doc, P, ages)
-    trans, dirichlet = _transitions(rd, doc, comps, P, ages)
-    sus_par = None
-    if sus_specs is not None:
-        sus_par = [P.add(nm, v, pr) for nm, v, pr in sus_specs]
+        entries.append(("age", ag, "ages"))
+    dcs = doc.get("democats", [])
+    if isinstance(dcs, dict):
+        dcs = [dcs]
+    for e in dcs:
+        _check_keys(e, {"name", "cats", "sus_param", "sus_value", "sus_prior"}, "democats")
+        if _s(e.get("name", "")) == "":
+            raise TomlError("in 'democats' every category needs a 'name'")
+        entries.append((_s(e["name"]), e, "democats"))
+    for name, e, key in entries:
+        vals = _split(_s(e.get("cats", "")), "|")
+        if not vals or "" in vals:
+            raise TomlError(f"in '{key}' the value of 'cats' must list the categories")
+        if name == "age":
+            cats[0] = ("age", vals)
+        else:
+            if any(n.lower() == name.lower() for n, _ in cats):
+                raise TomlError(f"the demographic category '{name}' is defined twice")
+            cats.append((name, vals))
+        if any(k in e for k in ("sus_param", "sus_value", "sus_prior")):
+            n = len(vals)
+            sus_specs[len(cats) - 1 if name != "age" else 0] = (
+                rd.split_n(_s(e.get("sus_param", "")), n, key), rd.split_n(_s(e.get("sus_value", "")), n, key),
+                rd.split_n(_s(e.get("sus_prior", "")), n, key))
+    demo = _Demo(cats)
+    ages, G = demo.ages, demo.G
+    comps = _compartments(rd, doc, P, demo)
+    trans, dirichlet = _transitions(rd, doc, comps, P, demo)
 
     splines = []
     if len(doc.get("R_spline", [])) != 1:
@@ -1054,30 +1208,30 @@ def load(path, datadir=None, with_data=True):
         splines.append(_spline(rd, P, "geo_mixing_modify", doc["geo_mixing_modify"][0]))
         geo = len(splines) - 1
 
-    # Areas and the initial population (src/data.cc:288-332,419-457): one
-    # column per age group, everyone in the infection's source compartment.
+    # Areas and the initial population (src/data.cc:288-457).
     if "areas" not in doc:
         raise TomlError("'areas' must be set")
     head, rows = rd.table(_s(doc["areas"]))
-    age_cols = [_col(head, a) for a in ages]
-    for col, c in zip(["area"] + ages, [_col(head, "area")] + age_cols):
-        if c is None:
-            raise TomlError(f"the areas file needs a column '{col}'")
+    if _col(head, "area") is None:
+        raise TomlError("the areas file needs a column 'area'")
     A = len(rows)
-    sus = trans[[t["infection"] for t in trans].index(True)]["from"]
-    pop = []
-    for row in rows:
-        pa = []
-        for c in age_cols:
-            try:
-                v = float(row[c])
-            except ValueError:
-                raise TomlError(f"the population '{row[c]}' is not a number") from None
-            if v < 0:
-                raise TomlError("populations must be non-negative")
-            pa.append(int(v + 0.5))
-        pop.append([pa if c == sus else [0] * G for c in range(len(comps))])
-    pop_age = np.array(pop, dtype=np.float64).sum(axis=(0, 1))
+    pop = _initial_population(rd, doc, demo, comps, trans, head, rows)
+    pop_dp = np.array(pop, dtype=np.float64).sum(axis=(0, 1))              # [G]
+    pop_age = np.array([pop_dp[[dp for dp in range(G) if demo.dp_age[dp] == a]].sum()
+                        for a in range(demo.nage)])
+    # Susceptibility (Model::set_dirichlet, src/model.cc:2768-2784): per
+    # category a modified Dirichlet group weighted by the category's
+    # population shares; a dp's susceptibility is the product over categories
+    # (Model::create_susceptibility, src/model.cc:1144-1178).
+    sus_terms = []
+    for k, (nm, vl, pr) in sorted(sus_specs.items()):
+        share = [float(pop_dp[[dp for dp in range(G) if demo.dps[dp][k] == v]].sum() / pop_dp.sum())
+                 for v in range(len(demo.cats[k][1]))]
+        if any(x == "" for x in nm):
+            raise TomlError(f"the susceptibility parameters of '{demo.cats[k][0]}' need names ('sus_param')")
+        idx, grp = _modified_dirichlet(P, nm, vl, pr, share, f"susceptibility of '{demo.cats[k][0]}'")
+        dirichlet.append(grp)
+        sus_terms.append([idx[t[k]] for t in demo.dps])
     for e in doc.get("prob_reach", []):          # Inputs::find_probreach, src/inputs.cc:1328-1340
         _check_keys(e, {"name", "comp"}, "prob_reach")
         if _s(e.get("name", "")) == "" or _s(e.get("comp", "")) not in {c["base"] for c in comps}:
@@ -1086,8 +1240,9 @@ def load(path, datadir=None, with_data=True):
     age_mod, mod_groups = _age_mods(rd, doc, P, ages, pop_age, splines)
     obs_spl = _obs_splines(rd, doc, P, splines)
 
-    # Age mixing (Data::age_mixing_matrix, src/data_matrix.cc:138-175): a
-    # G x G table without a heading, normalised later by the host model.
+    # Age mixing (Data::age_mixing_matrix, src/data_matrix.cc:138-175): an
+    # age x age table without a heading, normalised later by the host model.
+    G, nage = demo.nage, demo.nage
     N = None
     if "age_mixing_matrix" in doc:
         if "|" in _s(doc["age_mixing_matrix"]):
@@ -1101,16 +1256,19 @@ def load(path, datadir=None, with_data=True):
             raise TomlError("the age mixing matrix holds a value that is not a number") from None
     elif G != 1:
         raise TomlError("when more than one age group exists a value for 'age_mixing_matrix' must be set")
+    G = demo.G
 
     # efoi by age (src/inputs.cc:1421-1442): 'age_dist' normalised by the
-    # population's age distribution, else 1 for every age group.
-    efoi_ad = [1.0] * G
+    # population's age distribution, else 1 for every age group; each dp
+    # takes its age group's (src/state.cc:415-427).
+    efoi_ad = [1.0] * nage
     if efoi >= 0 and "age_dist" in doc["efoi_spline"][0]:
-        if G <= 1:
+        if nage <= 1:
             raise TomlError("in 'efoi_spline' the value of 'age_dist' is not required")
-        ad = [float(x) for x in rd.split_n(_s(doc["efoi_spline"][0]["age_dist"]), G, "efoi_spline")]
+        ad = [float(x) for x in rd.split_n(_s(doc["efoi_spline"][0]["age_dist"]), nage, "efoi_spline")]
         tot, agedist = sum(ad), pop_age / pop_age.sum()
-        efoi_ad = [ad[a] / (tot * agedist[a]) for a in range(G)]
+        efoi_ad = [float(ad[a] / (tot * agedist[a])) for a in range(nage)]
+    efoi_ad = [efoi_ad[a] for a in demo.dp_age]
 
     # Geographic mixing (Data::load_geo_mixing_matrix, src/data_matrix.cc:206-229):
     # an A x A table without a heading; the diagonal and, per row, the
@@ -1141,9 +1299,11 @@ def load(path, datadir=None, with_data=True):
                    **({"prob": t["prob"]} if t["prob"] is not None else {})} for t in trans],
         "params": P.specs,
         "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": efoi_ad,
-        "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet + eff_groups + mod_groups, "sus": sus_par,
+        "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet + eff_groups + mod_groups,
+        "sus": sus_terms[0] if sus_terms else None, "sus_extra": sus_terms[1:],
+        "nage": nage, "dp_age": demo.dp_age,
         "covars": _covariates(rd, doc, P, head, rows), "area_effect": area_eff, "level_effect": level,
         "age_mod": age_mod,
-        "obs": _observations(rd, doc, comps, trans, with_data, head, rows, ages, obs_spl),
+        "obs": _observations(rd, doc, comps, trans, with_data, head, rows, demo, obs_spl),
     }
     return spec, np.array(P.values), meta

@@ -140,8 +140,8 @@ typedef struct {
   int32_t efoi_spline;          /* external force of infection, -1 = none             */
   const double* efoi_agedist;   /* [G]                                               */
   int32_t geo_spline;           /* m(t) of geo_mixing_modify, -1 = unity              */
-  const double* N_raw;          /* [G][G] age contact table (Data::age_mixing_matrix
-                                   input), NULL = unit matrix for G == 1              */
+  const double* N_raw;          /* [nage][nage] age contact table (Data::age_mixing_matrix
+                                   input), NULL = unit matrix for one age group       */
   const double* M_diag;         /* [A]   raw geographic mixing (before geo_normalise) */
   const int32_t* M_rowptr;      /* [A+1] row c lists off-diagonal destinations        */
   const int32_t* M_to;          /* [nnz]                                             */
@@ -165,6 +165,15 @@ typedef struct {
   const int32_t* level_map;     /* [period][A] level of area c on day t               */
   int32_t nagemod;              /* age contact matrix modifications, 0 = none          */
   const beep_agemod* agemod;
+  /* Demographic categories beyond age (democats, src/inputs.cc:642-660): the G
+   * groups are then the combinations ("dp", age fastest, src/data.cc:55-73). */
+  int32_t nage;                 /* age groups; N_raw is [nage][nage]                   */
+  const int32_t* dp_age;        /* [G] age group of each dp; NULL = dp is the age group
+                                   (nage = G)                                           */
+  int32_t nsus_extra;           /* further susceptibility factors, 0 = none            */
+  const int32_t* sus_extra;     /* [nsus_extra][G] parameters (-1 = 1); a dp's
+                                   susceptibility is the product with sus_param's
+                                   (Model::create_susceptibility, src/model.cc:1144-1178) */
 } beep_spec;
 
 /* Host only (no device is touched): State::set_param's FP64 tables for one

@@ -46,16 +46,21 @@ def normalise(spec):
     pop = np.asarray(spec["pop"], dtype=np.float64).reshape(A, C, G)
     area_pop = pop.sum(axis=(1, 2))
     popsize = area_pop.sum()
-    pop_age = pop.sum(axis=(0, 1))
-    # Age mixing.
+    pop_age = pop.sum(axis=(0, 1))                     # per dp
+    # Age mixing over the age groups' populations, then between dps the
+    # mixing of their age groups.
+    dp_age = np.asarray(spec["dp_age"]) if spec.get("dp_age") is not None else np.arange(G)
+    nage = spec.get("nage", G) if spec.get("dp_age") is not None else G
+    pop_a = np.array([pop_age[dp_age == a].sum() for a in range(nage)])
     if spec.get("N") is not None:
         raw = np.asarray(spec["N"], dtype=np.float64)
-        m = raw / (pop_age[:, None] / popsize)
-        N = 0.5 * (m + m.T)
+        m = raw / (pop_a[:, None] / popsize)
+        N_age = 0.5 * (m + m.T)
     else:
-        N = np.ones((1, 1))
-    s = sum(pop_age[aa] * N[aa, a] for a in range(G) for aa in range(G)) / (G * pop_age.sum())
-    N = N / s
+        N_age = np.ones((1, 1))
+    s = sum(pop_a[aa] * N_age[aa, a] for a in range(nage) for aa in range(nage)) / (nage * pop_a.sum())
+    N_age = N_age / s
+    N = N_age[np.ix_(dp_age, dp_age)]
     # Geographic mixing.
     M = spec.get("M")
     if M is None:
@@ -104,8 +109,9 @@ def normalise(spec):
     for cv in spec.get("covars", []):
         v = np.asarray(cv["value"], dtype=np.float64).reshape(A, spec["period"])
         covars.append((cv["param"], v - v.mean()))
-    return dict(N=N, diag=diag, rowptr=rowptr, to=to, val=val, factor=factor, area_pop=area_pop,
-                popsize=popsize, Rdist=pop_age / popsize, inf_state=inf_state, itr=itr, exits=exits, covars=covars)
+    return dict(N=N, N_age=N_age, dp_age=dp_age, diag=diag, rowptr=rowptr, to=to, val=val, factor=factor,
+                area_pop=area_pop, popsize=popsize, Rdist=pop_age / popsize, inf_state=inf_state, itr=itr, exits=exits,
+                covars=covars)
 
 
 def dirichlet_correct(spec, pv):
@@ -206,6 +212,8 @@ def derive(spec, nz, pv):
                 r *= pd[t["prob"][g]]
             rate[k, g] = r
     sus = np.array([pd[p] if p >= 0 else 1.0 for p in spec["sus"]]) if spec.get("sus") is not None else np.ones(G)
+    for f in spec.get("sus_extra") or []:          # a product over categories (src/model.cc:1160-1163)
+        sus = sus * np.array([pd[p] if p >= 0 else 1.0 for p in f])
     inf = np.array([pd[c["inf"]] if c.get("inf", -1) >= 0 else 0.0 for c in comps])
     af = np.ones((period, A))
     if spec.get("area_effect") is not None:
@@ -219,13 +227,14 @@ def derive(spec, nz, pv):
     af /= area_av
     # Model::create_Ntime (src/model.cc:1254-1301): modifications multiply
     # their age groups' rows and columns by the spline value at each step.
-    Ntime = np.repeat(nz["N"][None, :, :], S, axis=0)
+    Na = np.repeat(nz["N_age"][None, :, :], S, axis=0)
     for am in spec.get("age_mod") or []:
         f = disc_spline(spec, am["spline"], pd)
         for s in range(S):
             for a in am["ages"]:
-                Ntime[s, a, :] *= f[s]
-                Ntime[s, :, a] *= f[s]
+                Na[s, a, :] *= f[s]
+                Na[s, :, a] *= f[s]
+    Ntime = Na[:, nz["dp_age"][:, None], nz["dp_age"][None, :]]
     # The ratio is recomputed where the matrix changes by more than TINY
     # (src/model.cc:1335-1340, Model::equal 1303-1314).
     ratio = np.zeros(S)

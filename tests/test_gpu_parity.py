@@ -45,7 +45,8 @@ def _transmean_magnitude(orc, tb, st):
 
 
 @pytest.mark.parametrize("mk", [configs.seir_single, configs.spatial_abcsmc, toml_models.toml_seir_erlang,
-                                toml_models.toml_seird_branch, toml_models.toml_sir_3areas, toml_models.toml_sird_ages])
+                                toml_models.toml_seird_branch, toml_models.toml_sir_3areas, toml_models.toml_sird_ages,
+                                toml_models.toml_seir_sex_ages])
 def test_fixed_trajectory_replay(gpu_lib, mk):
     spec, truth = mk()
     orc = O.Oracle(spec)

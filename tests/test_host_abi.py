@@ -23,7 +23,8 @@ def test_header_symbols_exported(gpu_lib):
 
 @pytest.mark.parametrize("mk", [configs.seir_single, configs.spatial_abcsmc, configs.dense_pmcmc,
                                 toml_models.toml_seir_erlang, toml_models.toml_seird_branch,
-                                toml_models.toml_sir_3areas, toml_models.toml_sird_ages])
+                                toml_models.toml_sir_3areas, toml_models.toml_sird_ages,
+                                toml_models.toml_seir_sex_ages])
 def test_host_tables_match_oracle(gpu_lib, mk):
     spec, truth = mk()
     nz = derive.normalise(spec)

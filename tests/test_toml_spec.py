@@ -348,6 +348,46 @@ def test_age_matrix_modifications(tmp_path):
         T.load(_write(str(tmp_path), "m.toml", text), datadir=os.path.join(FIX, "data_sird_ages"))
 
 
+def test_demographic_categories():
+    """Two age groups and two sexes (democats): dp enumerates the
+    combinations with age fastest (src/data.cc:55-73); populations are the
+    age columns times each area's sex fractions, the columns divided by their
+    sum (0.48/0.52 and 49/51), rounded (src/data.cc:419-457,821-870);
+    'Sex:' and 'age,Sex:' lists name their parameters '<values> <root>'
+    (src/inputs.cc:1217-1326); susceptibility by age and by sex are modified
+    Dirichlet groups weighted by population shares, a dp's the product
+    (src/model.cc:1144-1178,2768-2784); efoi age weights and data columns by
+    sex or age select dps (src/data.cc:1491-1520)."""
+    import toml_models
+    spec, values = toml_models.toml_seir_sex_ages()
+    names = [p["name"] for p in spec["params"]]
+    assert spec["G"] == 4 and spec["nage"] == 2 and spec["dp_age"] == [0, 1, 0, 1]
+    assert spec["N"] == [[1.5, 0.5], [0.5, 1.0]]
+    north = [round(30000 * 0.48), round(10000 * 0.48), round(30000 * 0.52), round(10000 * 0.52)]
+    south = [round(20000 * 0.49), round(25000 * 0.49), round(20000 * 0.51), round(25000 * 0.51)]
+    assert [spec["pop"][0][0], spec["pop"][1][0]] == [north, south]
+    assert [names[i] for i in spec["comps"][1]["mean"]] == ["M E mean occupancy"] * 2 + ["F E mean occupancy"] * 2
+    assert [names[i] for i in spec["comps"][2]["mean"]] == [
+        "young,M I mean occupancy", "old,M I mean occupancy", "young,F I mean occupancy", "old,F I mean occupancy"]
+    assert [values[i] for i in spec["comps"][2]["mean"]] == [4.0, 6.0, 5.0, 7.0]
+    pop = np.array(north) + np.array(south)
+    age_share = [(pop[0] + pop[2]) / pop.sum(), (pop[1] + pop[3]) / pop.sum()]
+    sex_share = [(pop[0] + pop[1]) / pop.sum(), (pop[2] + pop[3]) / pop.sum()]
+    sy, so = names.index("sus young"), names.index("sus old")
+    sm, sf = names.index("sus M"), names.index("sus F")
+    assert spec["sus"] == [sy, so, sy, so] and spec["sus_extra"] == [[sm, sm, sf, sf]]
+    assert spec["dirichlet"][0] == {"param": [sy, so], "frac": pytest.approx(age_share, rel=1e-15)}
+    assert spec["dirichlet"][1] == {"param": [sm, sf], "frac": pytest.approx(sex_share, rel=1e-15)}
+    assert values[sy] == pytest.approx(0.8 * age_share[0]) and values[so] == pytest.approx(1 - 0.8 * age_share[0])
+    assert spec["params"][sm]["prior"] == ("gamma", pytest.approx((1 / 0.09 - 1) * sex_share[0]), 1.0)
+    w = [1 / (4 * age_share[0]), 3 / (4 * age_share[1])]
+    assert spec["efoi_agedist"] == pytest.approx([w[0], w[1], w[0], w[1]], rel=1e-14)
+    se = [o for o in spec["obs"] if o["datatable"] == 0]
+    r = [o for o in spec["obs"] if o["datatable"] == 1]
+    assert {tuple(o["dp"]) for o in se[:8]} == {(0, 1)} and {tuple(o["dp"]) for o in se[8:]} == {(2, 3)}
+    assert {tuple(o["dp"]) for o in r[:8]} == {(0, 2)} and {tuple(o["dp"]) for o in r[8:]} == {(1, 3)}
+
+
 def test_multi_area_needs_matrix(tmp_path):
     d = str(tmp_path)
     _write(d, "areas.csv", "area,population\nX,100\nY,200\n")

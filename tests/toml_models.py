@@ -30,3 +30,8 @@ def toml_sir_3areas():
 def toml_sird_ages():
     """SIRD with three age groups: contact matrix, age-dependent periods and branches, efoi by age."""
     return _load("sird_ages.toml", "data_sird_ages")
+
+
+def toml_seir_sex_ages():
+    """SEIR over two areas with two age groups and two sexes: susceptibility by age and sex, per-sex counts."""
+    return _load("seir_sex_ages.toml", "data_seir_sex_ages")

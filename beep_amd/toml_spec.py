@@ -942,8 +942,76 @@ def _obs_splines(rd, doc, P, splines):
     return out
 
 
-def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, demo, obs_spl):
-    """data_tables -> spec observations."""
+def _filters(e, demo, area_head, area_rows):
+    """A data table's columns as (name, areas, dps) (Data::get_datafilter,
+    src/data.cc:1460-1560): one per region with geo_dep, one per value of a
+    category with democats_dep (over the dps holding it), else 'Data' over
+    every area and dp."""
+    geo_dep, demo_dep = _s(e.get("geo_dep", "")), _s(e.get("democats_dep", ""))
+    every_area, every_dp = list(range(len(area_rows))), list(range(demo.G))
+    if geo_dep and demo_dep:
+        raise TomlError("in 'data_tables' values for 'democats_dep' and 'geo_dep' cannot both be set")
+    if demo_dep:
+        k = demo.find(demo_dep)
+        if k is None:
+            raise TomlError(f"'democats_dep={demo_dep}' is not a demographic category")
+        if len(demo.cats[k][1]) == 1:
+            raise TomlError(f"'democats_dep=\"{demo_dep}\"' can only be used if '{demo_dep}' has more than one value")
+        return [(v, every_area, [dp for dp, t in enumerate(demo.dps) if t[k] == i])
+                for i, v in enumerate(demo.cats[k][1])]
+    if geo_dep:
+        return [(name, areas, every_dp) for name, areas in _regions(area_head, area_rows, geo_dep)]
+    return [("Data", every_area, every_dp)]
+
+
+def _obs_y(raw, thresh, thr_str, nodata_str, file):
+    """A data value: a number, '*' below the threshold, '.' unknown (src/utils.cc:575-583)."""
+    if raw == thr_str:
+        if thresh == "":
+            raise TomlError(f"the file '{file}' has thresholded data but no 'threshold' is set")
+        return OBS_THRESH
+    if raw == nodata_str:
+        return OBS_UNKNOWN
+    try:
+        return float(raw)
+    except ValueError:
+        raise TomlError(f"in the file '{file}' the value '{raw}' is not a number") from None
+
+
+def _marginal(rd, e, head, rows, file, demo, area_head, area_rows, tr_list, co_list, d, ngraph, otype, sd, percent,
+              factor, thresh, thr_str, nodata_str):
+    """A marginal table (Data::load_marginal_datatable, src/data.cc:1355-1455):
+    for each value of the category (or region) a row naming it in the first
+    column and its total in the second, observed over the steps from 'start'
+    (default the analysis start) to 'end' (default its end); one graph for
+    the table.  Returns the observations and the next graph index."""
+    if not (_s(e.get("geo_dep", "")) or _s(e.get("democats_dep", ""))):
+        raise TomlError("for marginal distributions in 'data_tables' either 'democats_dep' or 'geo_dep' must be set")
+    t0 = rd.time(_s(e["start"])) if "start" in e else 0
+    t1 = rd.time(_s(e["end"])) if "end" in e else rd.period
+    ti, tf = t0 * rd.spd, t1 * rd.spd
+    if ti < 0 or tf > rd.period * rd.spd or ti >= tf:
+        raise TomlError(f"in 'data_tables' the file '{file}' covers times outside the analysis")
+    names = [head[0]] + [r[0] for r in rows]         # the heading row names no value
+    obs = []
+    for name, areas, dps in _filters(e, demo, area_head, area_rows):
+        if name not in names[1:]:
+            raise TomlError(f"in 'data_tables' cannot find the value '{name}' in the first column of '{file}'")
+        row = rows[names[1:].index(name)]
+        o = {"y": _obs_y(row[1], thresh, thr_str, nodata_str, file), "type": otype, "factor": factor, "datatable": d,
+             "graph": ngraph, "area": list(areas), "dp": list(dps), "thresh": float(thresh) if thresh else 0.0,
+             "sett_i": ti, "sett_f": tf, "tr": list(tr_list), "co": list(co_list)}
+        if sd is not None:
+            o["sd"] = sd
+        if percent is not None:
+            o["percent"] = percent
+        obs.append(o)
+    return obs, ngraph + 1
+
+
+def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, demo, obs_spl, pop):
+    """data_tables -> spec observations: time series of transitions,
+    populations and population fractions, and marginal totals."""
     thr_str = _s(doc.get("threshold_str", "*"))
     nodata_str = _s(doc.get("nodata_str", "."))
     obs = []
@@ -954,22 +1022,22 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, demo, 
                         "threshold", "epsilon_factor", "plot_name", "line_colour", "geo_dep", "democats_dep",
                         "factor_spline"}, "data_tables")
         typ = _s(e.get("type"))
-        if typ not in ("transition", "population"):
-            raise TomlError(f"data table type '{typ}' is not supported (transition or population)")
+        if typ not in ("transition", "population", "population_fraction", "marginal"):
+            raise TomlError(f"in 'data_tables' the value of 'type={typ}' is not recognised")
         file = _s(e.get("file"))
         obsname = _s(e.get("observation"))
         tr_list, co_list = [], []
         for st in _split(obsname, ","):
             cs = [c for c, co in enumerate(comps) if co["base"] == st]
             if cs:
-                if typ != "population":
+                if typ == "transition":
                     raise TomlError(f"observation '{obsname}' names compartments in a transition table")
                 co_list += cs
                 continue
             ts = [t for t, tr in enumerate(trans) if comps[tr["from"]]["base"] + "->" + comps[tr["to"]]["base"] == st]
             if not ts:
                 raise TomlError(f"the observation '{st}' is not a compartment or a transition")
-            if typ != "transition":
+            if typ in ("population", "population_fraction"):
                 raise TomlError(f"observation '{obsname}' names transitions in a population table")
             tr_list += ts
         # Observation model grammar (src/inputs.cc:412-474).
@@ -1007,33 +1075,20 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, demo, 
         factor = float(_s(e.get("factor", "")) or 1.0)
         shift = int(_s(e.get("shift", "")) or 0)
         thresh = _s(e.get("threshold", ""))
+        if typ == "marginal" and "shift" in e:
+            raise TomlError("in 'data_tables' for 'marginal' data the property 'shift' cannot be used")
         if not with_data:
             continue
         head, rows = rd.table(file)
+        if typ == "marginal":
+            o_list, ngraph = _marginal(rd, e, head, rows, file, demo, area_head, area_rows, tr_list, co_list, d,
+                                       ngraph, otype, sd, percent, factor, thresh, thr_str, nodata_str)
+            obs += o_list
+            continue
         tc = _col(head, rd.time_col)
         if tc is None:
             raise TomlError(f"the file '{file}' needs a column '{rd.time_col}'")
-        # One column per region with geo_dep, else 'Data' over every area
-        # (Data::get_datafilter, src/data.cc:1460-1560); columns absent from
-        # the file are skipped, as the reference warns and skips them.
-        geo_dep, demo_dep = _s(e.get("geo_dep", "")), _s(e.get("democats_dep", ""))
-        every_area, every_dp = list(range(len(area_rows))), list(range(demo.G))
-        if geo_dep and demo_dep:
-            raise TomlError("in 'data_tables' values for 'democats_dep' and 'geo_dep' cannot both be set")
-        if demo_dep:
-            # One column per value of the category, over the dps holding it
-            # (Data::get_datafilter, src/data.cc:1491-1520).
-            k = demo.find(demo_dep)
-            if k is None:
-                raise TomlError(f"'democats_dep={demo_dep}' is not a demographic category")
-            if len(demo.cats[k][1]) == 1:
-                raise TomlError(f"'democats_dep=\"{demo_dep}\"' can only be used if '{demo_dep}' has more than one value")
-            filters = [(v, every_area, [dp for dp, t in enumerate(demo.dps) if t[k] == i])
-                       for i, v in enumerate(demo.cats[k][1])]
-        elif geo_dep:
-            filters = [(name, areas, every_dp) for name, areas in _regions(area_head, area_rows, geo_dep)]
-        else:
-            filters = [("Data", every_area, every_dp)]
+        filters = _filters(e, demo, area_head, area_rows)
         cols = [(name, _col(head, name), areas, dps) for name, areas, dps in filters if _col(head, name) is not None]
         if not cols:
             raise TomlError(f"the data in '{file}' was not used: expected columns such as '{filters[0][0]}'")
@@ -1071,8 +1126,13 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, demo, 
             sc = _col(head, name + " SD") if load_sd else None
             graph = ngraph
             ngraph += 1
-            col = _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, dps, graph, d, otype, sd, percent,
-                                factor, thresh, thr_str, nodata_str, tr_list, co_list, file)
+            fac = factor
+            if typ == "population_fraction":
+                # The column's initial population over its areas and dps (src/data.cc:1226-1235).
+                fac *= 1.0 / sum(pop[a][c][dp] for a in areas for c in range(len(comps)) for dp in dps)
+            col = _table_column(rd, rows, times, r0, r1, ts, "transition" if typ == "transition" else "population",
+                                dc, sc, areas, dps, graph, d, otype, sd, percent, fac, thresh, thr_str, nodata_str,
+                                tr_list, co_list, file)
             fs = _s(e.get("factor_spline", ""))
             if fs:
                 # Population observations cannot take a spline (emsgEC("ObsModel",3), src/obsmodel.cc:100).
@@ -1304,6 +1364,6 @@ def load(path, datadir=None, with_data=True):
         "nage": nage, "dp_age": demo.dp_age,
         "covars": _covariates(rd, doc, P, head, rows), "area_effect": area_eff, "level_effect": level,
         "age_mod": age_mod,
-        "obs": _observations(rd, doc, comps, trans, with_data, head, rows, demo, obs_spl),
+        "obs": _observations(rd, doc, comps, trans, with_data, head, rows, demo, obs_spl, pop),
     }
     return spec, np.array(P.values), meta

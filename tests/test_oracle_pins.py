@@ -176,14 +176,17 @@ def test_time_varying_age_matrix_closed_form():
         assert np.ptp(d["ratio"]) > 0.01 * d["ratio"].mean()
 
 
-def test_observation_spline_values_by_loops():
-    """ObservationModel::get_obs_value_section with an obs_spline
-    (src/obsmodel.cc:74-109): each step's transition count times factor
-    times the spline at that step, here from numpy.interp of the knot values
-    at t = s + 0.5, summed by loops over a simulated trajectory; the other
-    observations are plain sums."""
+@pytest.mark.parametrize("name", ["toml_seird_branch", "toml_seir_sex_ages"])
+def test_observation_values_by_loops(name):
+    """ObservationModel::get_obs_value_section (src/obsmodel.cc:74-109),
+    recomputed by loops over a simulated trajectory: each step's transition
+    count times the factor, times an obs_spline's value at that step where
+    the table names one (numpy.interp of the knot values at t = s + 0.5), and
+    populations times the factor; this covers weekly windows, per-sex and
+    per-age dp lists, marginal totals over days 14-42 and population
+    fractions (factor one over the initial population)."""
     import toml_models
-    spec, truth = toml_models.toml_seird_branch()
+    spec, truth = getattr(toml_models, name)()
     orc = O.Oracle(spec)
     for pv in (truth, truth * np.random.default_rng(4).uniform(0.8, 1.2, len(truth))):
         tb = orc.tables(pv)
@@ -198,44 +201,13 @@ def test_observation_spline_values_by_loops():
             v = 0.0
             for s in range(o["sett_i"], o["sett_f"]):
                 for c in o["area"]:
-                    for tr in o["tr"]:
-                        for dp in o["dp"]:
+                    for dp in o["dp"]:
+                        for tr in o.get("tr", []):
                             v += o.get("factor", 1.0) * st.transnum[s, c, tr, dp] * w[s]
+                        for co in o.get("co", []):
+                            v += o.get("factor", 1.0) * st.pop[s, c, co, dp]
             assert values[i] == pytest.approx(v, rel=1e-13, abs=1e-12)
-        assert any(o.get("spline", -1) >= 0 and values[i] > 0 for i, o in enumerate(spec["obs"]))
-
-
-def test_demographic_groups_by_hand():
-    """Two age groups x two sexes: the contact matrix is normalised over the
-    age groups' populations (src/data_matrix.cc:100-202, worked here by
-    hand) and each pair of dps takes their age groups' entry; a dp's
-    susceptibility is the product of its age's and its sex's corrected
-    weights (src/model.cc:1144-1178,2685-2695); beta's NGM ratio equals
-    numpy's dominant eigenvalue of the dp-level next-generation matrix."""
-    import toml_models
-    spec, truth = toml_models.toml_seir_sex_ages()
-    nz = derive.normalise(spec)
-    pop = np.asarray(spec["pop"], dtype=np.float64).sum(axis=(0, 1))        # per dp
-    pa = np.array([pop[0] + pop[2], pop[1] + pop[3]])
-    raw = np.array(spec["N"])
-    m = raw / (pa[:, None] / pop.sum())
-    m = 0.5 * (m + m.T)
-    m /= sum(pa[aa] * m[aa, a] for a in range(2) for aa in range(2)) / (2 * pa.sum())
-    ages = [0, 1, 0, 1]
-    assert np.allclose(nz["N"], [[m[ages[q], ages[d]] for d in range(4)] for q in range(4)], rtol=1e-14)
-    for pv in (truth, truth * np.random.default_rng(6).uniform(0.8, 1.2, len(truth))):
-        d = derive.derive(spec, nz, pv)
-        names = [p["name"] for p in spec["params"]]
-        v = {n: pv[names.index(n)] for n in ("sus young", "sus old", "sus M", "sus F")}
-        fa = [pa[0] / pa.sum(), pa[1] / pa.sum()]
-        fs = [(pop[0] + pop[1]) / pop.sum(), (pop[2] + pop[3]) / pop.sum()]
-        age = [v["sus young"] / (fa[0] * (v["sus young"] + v["sus old"])),
-               v["sus old"] / (fa[1] * (v["sus young"] + v["sus old"]))]
-        sex = [v["sus M"] / (fs[0] * (v["sus M"] + v["sus F"])), v["sus F"] / (fs[1] * (v["sus M"] + v["sus F"]))]
-        assert d["sus"] == pytest.approx([age[0] * sex[0], age[1] * sex[0], age[0] * sex[1], age[1] * sex[1]],
-                                         rel=1e-14)
-        K = derive.ngm(spec, nz, d["rate"], d["sus"], d["inf"])
-        assert d["ratio"] == pytest.approx(np.linalg.eigvals(K).real.max(), rel=1e-10)
+        assert np.count_nonzero(values) > len(values) // 2
 
 
 def test_deterministic_seir_is_forward_euler():

@@ -386,6 +386,16 @@ def test_demographic_categories():
     r = [o for o in spec["obs"] if o["datatable"] == 1]
     assert {tuple(o["dp"]) for o in se[:8]} == {(0, 1)} and {tuple(o["dp"]) for o in se[8:]} == {(2, 3)}
     assert {tuple(o["dp"]) for o in r[:8]} == {(0, 2)} and {tuple(o["dp"]) for o in r[8:]} == {(1, 3)}
+    # Marginal totals per sex over days 14-42, one graph (src/data.cc:1355-1455).
+    mg = [o for o in spec["obs"] if o["datatable"] == 2]
+    assert [(o["y"], o["dp"], o["sett_i"], o["sett_f"], o["tr"]) for o in mg] == [(2900.0, [0, 1], 14, 42, [1]),
+                                                                                  (3100.0, [2, 3], 14, 42, [1])]
+    assert len({o["graph"] for o in mg}) == 1
+    # Infected fractions per area: the factor is one over the area's initial
+    # population (src/data.cc:1226-1235).
+    fr = [o for o in spec["obs"] if o["datatable"] == 3]
+    assert [o["factor"] for o in fr[::8]] == [1 / sum(north), 1 / sum(south)]
+    assert all(o["co"] == [2] and o["sett_f"] == o["sett_i"] + 1 for o in fr)
 
 
 def test_multi_area_needs_matrix(tmp_path):

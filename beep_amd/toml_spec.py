@@ -419,7 +419,8 @@ def _transitions(rd, doc, comps, P, demo):
 def _dir_kind(prior):
     """A branch prior's Dirichlet kind (Model::add_parameter, src/model.cc:
     901-930): 'flat' for Dir(*) (density -x), ('alpha', a) for Dir(a) (a
-    gamma(a, 1) density), 'meansd' for Dir(mean,sd), None otherwise."""
+    gamma(a, 1) density), ('meansd', mean, sd) for Dir(mean,sd) (strings,
+    either may be '*'), None otherwise."""
     t = prior.replace(" ", "").lower()
     if not (t.startswith("dir(") and t.endswith(")")):
         return None
@@ -427,11 +428,42 @@ def _dir_kind(prior):
     if a == "*":
         return "flat"
     if "," in a:
-        return "meansd"
+        m, sd = a.split(",", 1)
+        return ("meansd", m, sd)
     try:
         return ("alpha", float(a))
     except ValueError:
         raise TomlError(f"the prior '{prior}' is not recognised") from None
+
+
+def _dir_meansd_alpha(means, sds, where):
+    """Dir(mean,sd) priors of a branch group (src/model.cc:2936-3020, each
+    weight 1): one mean may be '*' (one minus the others); exactly one sd is
+    set, and with that branch's mean m and sd s, alpha0 = m(1-m)/s^2 - 1;
+    branch i gets a gamma(alpha0 * mean_i, 1) prior."""
+    star = [i for i, m in enumerate(means) if m == "*"]
+    if len(star) != 1:
+        raise TomlError(f"{where}: exactly one Dirichlet mean must be '*'")
+    try:
+        mv = [math.nan if m == "*" else float(m) for m in means]
+    except ValueError:
+        raise TomlError(f"{where}: a Dirichlet mean is not a number") from None
+    rest = sum(m for i, m in enumerate(mv) if i != star[0])
+    if rest > 1:
+        raise TomlError(f"{where}: the Dirichlet mean '*' cannot be set")
+    mv[star[0]] = 1.0 - rest
+    given = [i for i, x in enumerate(sds) if x != "*"]
+    if len(given) != 1:
+        raise TomlError(f"{where}: exactly one Dirichlet standard deviation must be set")
+    try:
+        sd = float(sds[given[0]])
+    except ValueError:
+        raise TomlError(f"{where}: the Dirichlet standard deviation is not a number") from None
+    mu = mv[given[0]]
+    alpha0 = mu * (1 - mu) / (sd * sd) - 1
+    if not alpha0 > 0:
+        raise TomlError(f"{where}: the Dirichlet standard deviation is too large for its mean")
+    return [alpha0 * m / sum(mv) for m in mv]
 
 
 def _branches(trans, comps, P, G):
@@ -454,11 +486,13 @@ def _branches(trans, comps, P, G):
                 raise TomlError(f"{where}: MDir priors are not used for branch probabilities")
             kinds = [_dir_kind(p) for p in priors]
             dirs = [k is not None for k in kinds]
-            if "meansd" in kinds:
-                raise TomlError(f"{where}: Dir(mean,sd) priors are not supported")
-            # All fixed, all Dir(*) or all Dir(a) (src/model.cc:2924-2931).
-            if any(dirs) and not (all(k == "flat" for k in kinds) or all(isinstance(k, tuple) for k in kinds)):
-                raise TomlError(f"{where}: the priors must all be fixed, all Dir(*) or all Dir(alpha)")
+            alpha = [k[0] == "alpha" if isinstance(k, tuple) else False for k in kinds]
+            meansd = [k[0] == "meansd" if isinstance(k, tuple) else False for k in kinds]
+            # All fixed, all Dir(*), all Dir(a) or all Dir(mean,sd) (src/model.cc:2924-2931).
+            if any(dirs) and not (all(k == "flat" for k in kinds) or all(alpha) or all(meansd)):
+                raise TomlError(f"{where}: the priors must all be fixed, all Dir(*), all Dir(alpha) or all Dir(mean,sd)")
+            if all(meansd):
+                kinds = [("alpha", a) for a in _dir_meansd_alpha([k[1] for k in kinds], [k[2] for k in kinds], where)]
             vals = [t["branch"][g][1] for t in ts]
             star = [k for k, v in enumerate(vals) if v == "*"]
             if len(star) > 1:

@@ -446,8 +446,18 @@ def test_dirichlet_branches(tmp_path):
                                                           'prob_value="0.3", prob_prior="Dir(0.4)"')
                                                   .replace('prob_prior="Dir(*)"', 'prob_prior="Dir(2.5)"')), datadir=d)
     assert spec["params"][ih]["prior"] == ("gamma", 0.4, 1.0) and spec["params"][ir]["prior"] == ("gamma", 2.5, 1.0)
-    with pytest.raises(T.TomlError, match="mean,sd"):
-        T.load(_write(d, "h.toml", text.replace('prob_prior="Dir(*)"', 'prob_prior="Dir(0.5,0.1)"')), datadir=d)
+    # Dir(mean,sd): one mean '*', one sd; alpha0 = m(1-m)/sd^2 - 1 from the
+    # branch whose sd is set, then gamma(alpha0 * mean_i, 1) (src/model.cc:2936-3020).
+    spec, _, _ = T.load(_write(d, "h.toml", text.replace('prob_value="0.3", prob_prior="Dir(*)"',
+                                                          'prob_value="0.3", prob_prior="Dir(0.25,0.05)"')
+                                                  .replace('prob_prior="Dir(*)"', 'prob_prior="Dir(*,*)"')), datadir=d)
+    a0 = 0.25 * 0.75 / 0.05 ** 2 - 1
+    assert spec["params"][ih]["prior"] == ("gamma", pytest.approx(a0 * 0.25), 1.0)
+    assert spec["params"][ir]["prior"] == ("gamma", pytest.approx(a0 * 0.75), 1.0)
+    with pytest.raises(T.TomlError, match="standard deviation"):
+        T.load(_write(d, "i.toml", text.replace('prob_value="0.3", prob_prior="Dir(*)"',
+                                                 'prob_value="0.3", prob_prior="Dir(0.25,0.05)"')
+                                         .replace('prob_prior="Dir(*)"', 'prob_prior="Dir(*,0.1)"')), datadir=d)
     # Values only: fixed probabilities, '*' filled the same way, no group.
     text = text.replace('prob_prior="Dir(*)"', "")
     spec, values, _ = T.load(_write(d, "c.toml", text), datadir=d)

@@ -97,6 +97,7 @@ struct beep_ctx {
   int* d_pf = nullptr;
   size_t pf_cap = 0;
   int pf_n = 0;
+  int pf_slot = -1;                // the parameter slot of the last particle filter run
   // structure-specialised fused kernel (null: generic kernel or step-wise path)
   SpecKernels* spec = nullptr;
   size_t spec_lds = 0;
@@ -1174,6 +1175,8 @@ int beep_copy_instances(beep_ctx* c, int n, const int32_t* dst, const int32_t* s
       HIPCHK(c, hipMemcpyAsync(c->d_pt + (size_t)dst[i] * c->pop_traj_sz, c->d_pt + (size_t)src[i] * c->pop_traj_sz,
                                c->pop_traj_sz * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_EF + dst[i], c->d_EF + src[i], sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    c->inst_slot[dst[i]] = c->inst_slot[src[i]];     // the trajectory's parameters go with it
+    c->inst_slot_dirty = true;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return BEEP_OK;
@@ -1350,6 +1353,7 @@ int beep_pmcmc_obs_prob(beep_ctx* c, int n, const int32_t* inst, int slot, const
   if (rc) return rc;
   *obprob_out = ob;
   c->pf_n = n;
+  c->pf_slot = slot;
   return BEEP_OK;
 }
 
@@ -1369,6 +1373,8 @@ int beep_pmcmc_particle_sample(beep_ctx* c, int dst) {
   HIPCHK(c, hipGetLastError());
   // EF of the sampled particle is not defined by the section run.
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->inst_slot[dst] = c->pf_slot;                   // the filter's parameters
+  c->inst_slot_dirty = true;
   return BEEP_OK;
 }
 

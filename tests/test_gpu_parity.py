@@ -168,6 +168,35 @@ def test_simulate_distribution_spatial(gpu_lib):
     ctx.close()
 
 
+def test_observation_spline_follows_the_instance_slot(gpu_lib):
+    """An observation spline takes its values from the parameter slot an
+    instance was last simulated or replayed with, and a copied trajectory
+    takes its source's: two slots with different spline knots give the
+    oracle's likelihood for each, a copy reproduces its source's, and an
+    instance never run with a slot is refused."""
+    spec, truth = toml_models.toml_seird_branch()
+    names = [p["name"] for p in spec["params"]]
+    th2 = truth.copy()
+    for k in ("ascertainment t=0", "ascertainment t=42", "ascertainment t=84"):
+        th2[names.index(k)] = 0.35
+    orc = O.Oracle(spec)
+    ctx = _lib.Context(spec, capacity=4, nslot=2)
+    ctx.set_param([0, 1], np.stack([truth, th2]))
+    tn = orc.simulate(orc.tables(truth), seed=77).transnum
+    L = []
+    for inst, (slot, th) in enumerate(((0, truth), (1, th2))):
+        ctx.put_transnum(inst, slot, tn.astype(np.int32))
+        L.append(orc.obs_loglik(orc.replay(orc.tables(th), tn))[0])
+    Lg = ctx.obs_loglik([0, 1])
+    assert Lg[0] == pytest.approx(L[0], rel=1e-9) and Lg[1] == pytest.approx(L[1], rel=1e-9)
+    assert abs(L[0] - L[1]) > 1.0
+    ctx.copy_instances([2], [1])
+    assert ctx.obs_loglik([2])[0] == pytest.approx(L[1], rel=1e-9)
+    with pytest.raises(_lib.BeepError, match="no parameter slot"):
+        ctx.obs_loglik([3])
+    ctx.close()
+
+
 def test_batch_independence(gpu_lib):
     """An instance's trajectory does not depend on its slot or its batch."""
     spec, truth = configs.spatial_abcsmc()

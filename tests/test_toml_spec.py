@@ -487,6 +487,43 @@ def test_smoothing_prior(tmp_path):
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="the reference's examples are not present")
+def test_reference_examples_host_tables(gpu_lib):
+    """Every reference example the reader loads: the C++ host's State::set_param
+    tables, prior and bounds equal the FP64 oracle's for the file's values and
+    two prior draws (the same check as the fixtures', on the reference's own
+    models: Erlang stages, branches, ages, democats, effects, covariates,
+    perturbed mixing, observation splines)."""
+    import tomli
+    from beep_amd import _lib
+    from oracle import derive
+    n = 0
+    for path in sorted(glob.glob(os.path.join(REF_EXAMPLES, "*.toml"))):
+        with open(path, "rb") as f:
+            dd = tomli.load(f)
+        try:
+            spec, values, _ = T.load(path, datadir=os.path.join(os.path.dirname(REF_EXAMPLES), dd.get("datadir", "")))
+        except T.TomlError:
+            continue
+        nz = derive.normalise(spec)
+        rng = np.random.default_rng(11)
+        draws = list(specmod.sample_prior(spec, rng, 2))
+        if not np.isnan(values).any():
+            draws.insert(0, values)
+        S = spec["period"] * spec["spd"]
+        for pv in draws:
+            t = _lib.spec_tables(spec, pv)
+            d = derive.derive(spec, nz, pv)
+            ba = d["beta"] * d["af"][np.arange(S) // spec["spd"]]
+            assert np.allclose(t["beta_af"], ba, rtol=1e-10, atol=0), path
+            assert np.allclose(t["rate"], d["rate"], rtol=1e-14), path
+            assert np.allclose(t["sus"], d["sus"], rtol=1e-14), path
+            assert t["Pr"] == pytest.approx(derive.prior(spec, pv), rel=1e-12, abs=1e-9), path
+            assert t["inbounds"] == derive.inbounds(spec, pv), path
+        n += 1
+    assert n >= 30
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="the reference's examples are not present")
 def test_reference_examples():
     """Every example either loads or is refused with TomlError; EX.toml (SEIR
     with Erlang E, weekly E->I counts and R populations) matches its files."""

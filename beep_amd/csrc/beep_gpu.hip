@@ -368,6 +368,7 @@ int launch_stepwise(beep_ctx* c, int mode, const Job* djobs, int n, int t0, int 
   const int rows = n * m.G;
   dim3 gcell((m.AG + BEEP_STEP_BLOCK - 1) / BEEP_STEP_BLOCK, n);
   hipGraph_t graph;
+  (void)hipGetLastError();               // a stale error of an earlier call is not this loop's
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(beepdev::k_step_init, gcell, dim3(BEEP_STEP_BLOCK), 0, c->stream, djobs, c->d_D, c->Apad);
   for (int s = t0; s < t1; s++) {
@@ -400,6 +401,15 @@ int launch_stepwise(beep_ctx* c, int mode, const Job* djobs, int n, int t0, int 
     else
       hipLaunchKernelGGL(beepdev::k_step_trans<beepdev::MODE_REPLAY>, gcell, dim3(BEEP_STEP_BLOCK), 0, c->stream, djobs,
                          c->d_I, c->d_D, c->Apad, s, c->d_rngp);
+    // A launch refused while capturing (resources, configuration) is an
+    // error of the call, not a silently empty step.
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+      hipGraph_t g2 = nullptr;
+      (void)hipStreamEndCapture(c->stream, &g2);
+      if (g2) (void)hipGraphDestroy(g2);
+      return fail(c, BEEP_EHIP, "step %d launch under capture: %s", s, hipGetErrorString(le));
+    }
   }
   hipError_t e = hipStreamEndCapture(c->stream, &graph);
   if (e != hipSuccess) return fail(c, BEEP_EHIP, "graph capture: %s", hipGetErrorString(e));

@@ -94,7 +94,8 @@ class Spec(C.Structure):
                 ("level_param", C.POINTER(C.c_int32)), ("level_map", C.POINTER(C.c_int32)),
                 ("nagemod", C.c_int32), ("agemod", C.POINTER(AgeMod)),
                 ("nage", C.c_int32), ("dp_age", C.POINTER(C.c_int32)),
-                ("nsus_extra", C.c_int32), ("sus_extra", C.POINTER(C.c_int32))]
+                ("nsus_extra", C.c_int32), ("sus_extra", C.POINTER(C.c_int32)),
+                ("nstrain", C.c_int32), ("strain_Rfactor", C.POINTER(C.c_int32))]
 
 
 _lib = None
@@ -300,6 +301,8 @@ class _SpecBuffers:
             s.nage, s.dp_age = spec["nage"], i32(spec["dp_age"])
         else:
             s.nage, s.dp_age = G, null_i32
+        sr = spec.get("strain_Rfactor")
+        s.nstrain, s.strain_Rfactor = (len(sr), i32(sr)) if sr else (1, null_i32)
         se = spec.get("sus_extra") or []
         s.nsus_extra = len(se)
         s.sus_extra = i32(np.asarray(se, dtype=np.int32).reshape(-1)) if se else null_i32

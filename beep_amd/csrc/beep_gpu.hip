@@ -271,7 +271,8 @@ std::string spec_prelude(const beephost::HostModel& m) {
   std::ostringstream o;
   o << "namespace beepspec {\n";
   o << "constexpr int C = " << m.C << ", T = " << m.T << ", G = " << m.G << ", CP = " << (m.C + 3) / 4 * 4
-    << ", INF_TR = " << m.inf_tr << ", MIN_WAVES = " << waves << ", STEP_WAVES = " << step_waves << ";\n";
+    << ", INF_TR = " << m.inf_tr << ", MIN_WAVES = " << waves << ", STEP_WAVES = " << step_waves
+    << ", NSTRAIN = " << m.nstrain << ", DPM = " << m.G / m.nstrain << ";\n";
   o << "constexpr int FROM[" << m.T << "] = {";
   for (int t = 0; t < m.T; t++) o << (t ? ", " : "") << m.trans[t].from;
   o << "};\nconstexpr int TO[" << m.T << "] = {";
@@ -639,10 +640,10 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
     int stage = tp.stage_mix;
     if (!c->stepwise) {
       const bool small = beepdev::fused_mix_bytes(m.A, tp.nnz) <= 32768;
-      stage = small && beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, 1) <= 160 * 1024;
+      stage = small && beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, 1, m.nstrain) <= 160 * 1024;
       if (const char* e = std::getenv("BEEP_STAGE_MIX")) stage = small && std::atoi(e) != 0;
     }
-    size_t sl = beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, stage);
+    size_t sl = beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, stage, m.nstrain);
     if (c->stepwise ? step_lds <= 65536 : sl <= 160 * 1024) {
       SpecKernels* k = nullptr;
       if ((rc = spec_kernels(c, &k))) return rc;
@@ -653,6 +654,11 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
       }
     }
   }
+  // Strains pool susceptibles across cells, which only the specialised fused
+  // kernel does.
+  if (m.nstrain > 1 && (c->stepwise || !c->spec))
+    return fail(c, BEEP_EINVAL, "a model with strains runs on the specialised fused kernel only (an instance in LDS, "
+                                "sparse mixing, not BEEP_GENERIC_KERNEL or BEEP_STEPWISE)");
   if ((rc = dalloc(c, (void**)&c->d_topo, sizeof(TopoDev)))) return rc;
   HIPCHK(c, hipMemcpy(c->d_topo, &tp, sizeof tp, hipMemcpyHostToDevice));
   c->Apad = (m.A + 63) / 64 * 64;

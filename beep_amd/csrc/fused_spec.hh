@@ -7,6 +7,7 @@
 //   constexpr int INF_TR;             the infection transition
 //   constexpr int FROM[T], TO[T];     transition source / destination
 //   constexpr bool INFC[C];           compartments with an infectivity weight
+//   constexpr int NSTRAIN, DPM;       strains, and groups per strain (G = NSTRAIN * DPM)
 //
 // It computes exactly what k_fused (kernels.hh) computes -- the same FP32
 // expressions in the same order and the same uniforms consumed in the same
@@ -30,6 +31,23 @@ static_assert(beepspec::CP % 4 == 0 && beepspec::CP >= beepspec::C, "CP");
 // cell; chunk k of all cells is contiguous, so one wavefront's 16-byte
 // accesses to its cells' chunk are consecutive in LDS.
 __device__ inline int pidx(int c, int cell, int AG) { return ((c >> 2) * AG + cell) * 4 + (c & 3); }
+
+// Strains: for each strain-0 cell, the susceptibles of its dp summed over
+// the strains (reads pop only; the caller's barrier publishes spool).
+__device__ inline void pool_susceptibles(const int* pop, int* spool, int AG, int tid, int nt) {
+  if constexpr (beepspec::NSTRAIN > 1) {
+    constexpr int G = beepspec::G, DPM = beepspec::DPM, SUS = beepspec::FROM[beepspec::INF_TR];
+    for (int cell = tid; cell < AG; cell += nt) {
+      const int j = cell / G, g = cell - j * G;
+      if (g < DPM) {
+        int sum = 0;
+#pragma unroll
+        for (int st = 0; st < beepspec::NSTRAIN; st++) sum += pop[pidx(SUS, j * G + st * DPM + g, AG)];
+        spool[cell] = sum;
+      }
+    }
+  }
+}
 
 template <int MODE>
 __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t seed, uint32_t stream) {
@@ -71,6 +89,9 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   void* qidx = staged ? (void*)(sord + A) : (void*)(sinf + C);
   uint16_t* qidx16 = (uint16_t*)qidx;
   uint32_t* qidx32 = (uint32_t*)qidx;
+  // Strains: each strain-0 cell's susceptibles pooled over the strains of its
+  // dp (src/state.cc:395-404), after the queue's indices.
+  int* spool = (int*)((char*)qidx + ((size_t)qcap * (q16 ? 2 : 4) + 3) / 4 * 4);
   if (staged) {
     for (int i = tid; i < A; i += nt) { sdiag[i] = diag[i]; sfac[i] = fac[i]; sord[i] = order[i]; }
     for (int i = tid; i <= A; i += nt) sptr[i] = in_ptr[i];
@@ -111,6 +132,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     }
     sdinf[cell] = d;
   }
+  pool_susceptibles(pop, spool, AG, tid, nt);
   __syncthreads();
 
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
@@ -153,6 +175,14 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
           p[4 * c4] = v.x; p[4 * c4 + 1] = v.y; p[4 * c4 + 2] = v.z; p[4 * c4 + 3] = v.w;
         }
       }
+      int pool = 0;
+      if constexpr (NSTRAIN > 1) {
+        // Strain st's infections draw on the pooled susceptibles, which the
+        // reference keeps in strain 0's dp (src/state.cc:395-425).
+        const int st = g / DPM;
+        pool = spool[j * G + (g - st * DPM)];
+        p[FROM[INF_TR]] = st == 0 ? pool : 0;
+      }
       if (pt_s) {
 #pragma unroll
         for (int c = 0; c < C; c++) st_nt(pt_s, c * AG + cell, p[c]);
@@ -173,6 +203,9 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
       for (int tr = 0; tr < T; tr++) {
         const int fr = FROM[tr], to = TO[tr];
         float popu = (float)p[fr];
+        if constexpr (NSTRAIN > 1) {
+          if (tr == INF_TR) popu = (float)pool;
+        }
         float lp = 0.f;
         if (popu > 0.f) lp = tr == INF_TR ? dt * popu * ssus[g] * foi : dt * popu * srate[tr * G + g];
         // src/state.cc:429 (NaN mean) and src/utils.cc:150 (mean > 1e9) stop the reference.
@@ -275,6 +308,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
       }
       sdinf[cell] = d;
     }
+    pool_susceptibles(pop, spool, AG, tid, nt);
     __syncthreads();
   }
   for (int cell = tid; cell < AG; cell += nt) {

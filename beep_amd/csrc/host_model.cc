@@ -148,6 +148,15 @@ std::string HostModel::build(const beep_spec& s) {
     for (int th : sus_extra.back())
       if (th >= nparam) return "susceptibility parameter out of range";
   }
+  nstrain = s.nstrain > 1 ? s.nstrain : 1;
+  strain_Rfactor.assign(nstrain, -1);
+  if (nstrain > 1) {
+    if (G % nstrain) return "the groups are not a whole number per strain";
+    if (!s.strain_Rfactor) return "strains need their Rfactor parameters";
+    strain_Rfactor.assign(s.strain_Rfactor, s.strain_Rfactor + nstrain);
+    for (int th : strain_Rfactor)
+      if (th >= nparam) return "strain Rfactor parameter out of range";
+  }
   if (s.dp_age) {
     nage = s.nage;
     if (nage <= 0 || nage > G) return "bad number of age groups";
@@ -210,8 +219,11 @@ std::string HostModel::build(const beep_spec& s) {
   }
   // Between dps, the mixing of their age groups (State::get_NMI, src/state.cc:547-578).
   N.assign((size_t)G * G, 0.0);
+  // With strains, a strain's infections mix with that strain's only.
+  const int dpm = G / nstrain;
   for (int q = 0; q < G; q++)
-    for (int d = 0; d < G; d++) N[(size_t)q * G + d] = N_age[(size_t)dp_age[q] * nage + dp_age[d]];
+    for (int d = 0; d < G; d++)
+      N[(size_t)q * G + d] = q / dpm == d / dpm ? N_age[(size_t)dp_age[q] * nage + dp_age[d]] : 0.0;
 
   // Geographic mixing: Data::geo_normalise (src/data_matrix.cc:55-96).
   M_rowptr.assign(A + 1, 0);
@@ -465,8 +477,10 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
           }
       }
       double* Ns = tb.Ntime.data() + (size_t)s * G * G;
+      const int dpm = G / nstrain;
       for (int q = 0; q < G; q++)
-        for (int d = 0; d < G; d++) Ns[(size_t)q * G + d] = Na[(size_t)dp_age[q] * nage + dp_age[d]];
+        for (int d = 0; d < G; d++)
+          Ns[(size_t)q * G + d] = q / dpm == d / dpm ? Na[(size_t)dp_age[q] * nage + dp_age[d]] : 0.0;
     }
   }
 
@@ -496,69 +510,74 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
 
   // Model::calculate_beta_from_R (src/model.cc:1318-1362) with calculate_Vinv,
   // calculate_F, calculate_NGM and the dominant eigenvalue (2485-2621).
-  const int nis = (int)inf_state.size();
-  if (nis == 0) { err = "no infected states"; return BEEP_EINVAL; }
-  std::vector<int> ref(C, -1);
-  for (int i = 0; i < nis; i++) ref[inf_state[i]] = i;
-  const int NV = nis * G;
-  std::vector<double> V((size_t)NV * NV, 0.0), Vinv;
-  for (int t = 0; t < T; t++) {
-    int fi = ref[trans[t].from], ti = ref[trans[t].to];
-    if (fi < 0) continue;
-    for (int g = 0; g < G; g++) {
-      double r = tb.rate[(size_t)t * G + g];
-      int j = fi * G + g;
-      V[(size_t)j * NV + j] += r;
-      if (ti >= 0) V[(size_t)(ti * G + g) * NV + j] -= r;
-    }
-  }
-  if (!invert(V, NV, Vinv)) { err = "singular transition matrix V"; return BEEP_EINVAL; }
-  // calculate_R_beta_ratio_using_NGM: dominant eigenvalue of F V^-1 for one
-  // age contact matrix Nm (src/model.cc:2485-2576).
-  auto ngm_ratio = [&](const double* Nm) {
-    std::vector<double> F((size_t)G * NV, 0.0);
-    for (int si = 0; si < nis; si++) {
-      double inf = tb.inf[inf_state[si]];
-      if (inf == 0) continue;
-      for (int dp = 0; dp < G; dp++) {
-        int i = si * G + dp;
-        for (int q = 0; q < G; q++) F[(size_t)q * NV + i] += Rdist[q] * tb.sus[q] * inf * Nm[(size_t)q * G + dp];
+  if (nstrain > 1) {
+    int rc = strain_beta(pd, af, tb, err);
+    if (rc) return rc;
+  } else {
+    const int nis = (int)inf_state.size();
+    if (nis == 0) { err = "no infected states"; return BEEP_EINVAL; }
+    std::vector<int> ref(C, -1);
+    for (int i = 0; i < nis; i++) ref[inf_state[i]] = i;
+    const int NV = nis * G;
+    std::vector<double> V((size_t)NV * NV, 0.0), Vinv;
+    for (int t = 0; t < T; t++) {
+      int fi = ref[trans[t].from], ti = ref[trans[t].to];
+      if (fi < 0) continue;
+      for (int g = 0; g < G; g++) {
+        double r = tb.rate[(size_t)t * G + g];
+        int j = fi * G + g;
+        V[(size_t)j * NV + j] += r;
+        if (ti >= 0) V[(size_t)(ti * G + g) * NV + j] -= r;
       }
     }
-    std::vector<double> NGM((size_t)G * G, 0.0);
-    for (int q = 0; q < G; q++)
-      for (int i = 0; i < G; i++) {
-        double sum = 0;
-        for (int j = 0; j < NV; j++) sum += F[(size_t)q * NV + j] * Vinv[(size_t)j * NV + i];
-        NGM[(size_t)q * G + i] = sum;
+    if (!invert(V, NV, Vinv)) { err = "singular transition matrix V"; return BEEP_EINVAL; }
+    // calculate_R_beta_ratio_using_NGM: dominant eigenvalue of F V^-1 for one
+    // age contact matrix Nm (src/model.cc:2485-2576).
+    auto ngm_ratio = [&](const double* Nm) {
+      std::vector<double> F((size_t)G * NV, 0.0);
+      for (int si = 0; si < nis; si++) {
+        double inf = tb.inf[inf_state[si]];
+        if (inf == 0) continue;
+        for (int dp = 0; dp < G; dp++) {
+          int i = si * G + dp;
+          for (int q = 0; q < G; q++) F[(size_t)q * NV + i] += Rdist[q] * tb.sus[q] * inf * Nm[(size_t)q * G + dp];
+        }
       }
-    return dominant_eigenvalue(NGM, G);
-  };
-  // The ratio is recomputed only where the matrix differs from the previous
-  // step's by more than TINY = 1e-8 in some entry (src/model.cc:1337-1339,1303-1314).
-  const bool tv = !agemod.empty();
-  std::vector<double> ratio(tv ? S : 1);
-  for (int s = 0; s < (int)ratio.size(); s++) {
-    const double* Ns = tb.Ntime.data() + (size_t)s * G * G;
-    bool same = s > 0;
-    for (int k = 0; same && k < G * G; k++) {
-      const double v1 = Ns[k - G * G], v2 = Ns[k];
-      if (v1 < v2 - 1e-8 || v1 > v2 + 1e-8) same = false;
+      std::vector<double> NGM((size_t)G * G, 0.0);
+      for (int q = 0; q < G; q++)
+        for (int i = 0; i < G; i++) {
+          double sum = 0;
+          for (int j = 0; j < NV; j++) sum += F[(size_t)q * NV + j] * Vinv[(size_t)j * NV + i];
+          NGM[(size_t)q * G + i] = sum;
+        }
+      return dominant_eigenvalue(NGM, G);
+    };
+    // The ratio is recomputed only where the matrix differs from the previous
+    // step's by more than TINY = 1e-8 in some entry (src/model.cc:1337-1339,1303-1314).
+    const bool tv = !agemod.empty();
+    std::vector<double> ratio(tv ? S : 1);
+    for (int s = 0; s < (int)ratio.size(); s++) {
+      const double* Ns = tb.Ntime.data() + (size_t)s * G * G;
+      bool same = s > 0;
+      for (int k = 0; same && k < G * G; k++) {
+        const double v1 = Ns[k - G * G], v2 = Ns[k];
+        if (v1 < v2 - 1e-8 || v1 > v2 + 1e-8) same = false;
+      }
+      ratio[s] = same ? ratio[s - 1] : ngm_ratio(Ns);
+      if (!(ratio[s] > 0) || std::isnan(ratio[s])) {
+        err = "next-generation matrix has no positive dominant eigenvalue";
+        return BEEP_EINVAL;
+      }
     }
-    ratio[s] = same ? ratio[s - 1] : ngm_ratio(Ns);
-    if (!(ratio[s] > 0) || std::isnan(ratio[s])) {
-      err = "next-generation matrix has no positive dominant eigenvalue";
-      return BEEP_EINVAL;
-    }
-  }
-  double Rfac = Rfactor_param >= 0 ? pd[Rfactor_param] : 1.0;
-  std::vector<double> Rt = disc_spline(R_spline, pd);
+    double Rfac = Rfactor_param >= 0 ? pd[Rfactor_param] : 1.0;
+    std::vector<double> Rt = disc_spline(R_spline, pd);
 
-  tb.beta_af.assign((size_t)S * A, 0.0);
-  for (int s = 0; s < S; s++) {
-    double beta = Rfac * Rt[s] / ratio[tv ? s : 0];
-    int day = s / spd;
-    for (int c = 0; c < A; c++) tb.beta_af[(size_t)s * A + c] = beta * af[(size_t)day * A + c];
+    tb.beta_af.assign((size_t)S * A, 0.0);
+    for (int s = 0; s < S; s++) {
+      double beta = Rfac * Rt[s] / ratio[tv ? s : 0];
+      int day = s / spd;
+      for (int c = 0; c < A; c++) tb.beta_af[(size_t)s * A + c] = beta * af[(size_t)day * A + c];
+    }
   }
   tb.efoi.assign((size_t)S * G, 0.0);
   if (efoi_spline >= 0) {
@@ -575,6 +594,85 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
     tb.obs_spl.insert(tb.obs_spl.end(), v.begin(), v.end());
   }
   for (double v : tb.beta_af) if (std::isnan(v)) { err = "NaN transmission rate"; return BEEP_ENAN; }
+  return BEEP_OK;
+}
+
+// Strains (Model::calculate_beta_from_R, src/model.cc:1318-1348, with
+// calculate_Vinv and calculate_F per strain): each strain's NGM ratio over its
+// own dps, with the population fractions pooled over strains (democatpos_dist,
+// src/data.cc:154) and that strain's susceptibility; beta_st = Rfactor_st *
+// R(t) / ratio_st.  The device takes beta * areafactor per area and step, so
+// the strain factor Rfactor_st / ratio_st goes into the strain's block of the
+// (block-diagonal) contact matrix, and beta_af holds R(t) * areafactor.
+int HostModel::strain_beta(const std::vector<double>& pd, const std::vector<double>& af, Tables& tb,
+                           std::string& err) const {
+  const int nis = (int)inf_state.size();
+  if (nis == 0) { err = "no infected states"; return BEEP_EINVAL; }
+  const int dpm = G / nstrain, NV = nis * dpm;
+  std::vector<int> ref(C, -1);
+  for (int i = 0; i < nis; i++) ref[inf_state[i]] = i;
+  std::vector<double> Rd(dpm, 0.0);
+  for (int g = 0; g < G; g++) Rd[g % dpm] += Rdist[g];
+  const bool tv = !agemod.empty();
+  const int nsteps = tv ? S : 1;
+  std::vector<double> fold((size_t)nsteps * nstrain);
+  for (int st = 0; st < nstrain; st++) {
+    std::vector<double> V((size_t)NV * NV, 0.0), Vinv;
+    for (int t = 0; t < T; t++) {
+      int fi = ref[trans[t].from], ti = ref[trans[t].to];
+      if (fi < 0) continue;
+      for (int q = 0; q < dpm; q++) {
+        double r = tb.rate[(size_t)t * G + st * dpm + q];
+        int j = fi * dpm + q;
+        V[(size_t)j * NV + j] += r;
+        if (ti >= 0) V[(size_t)(ti * dpm + q) * NV + j] -= r;
+      }
+    }
+    if (!invert(V, NV, Vinv)) { err = "singular transition matrix V"; return BEEP_EINVAL; }
+    const double Rfac = strain_Rfactor[st] >= 0 ? pd[strain_Rfactor[st]] : 1.0;
+    double ratio = 0;
+    for (int s = 0; s < nsteps; s++) {
+      const double* Ns = tb.Ntime.data() + (size_t)s * G * G;
+      bool same = s > 0;
+      for (int k = 0; same && k < G * G; k++) {
+        const double v1 = Ns[k - G * G], v2 = Ns[k];
+        if (v1 < v2 - 1e-8 || v1 > v2 + 1e-8) same = false;
+      }
+      if (!same) {
+        std::vector<double> F((size_t)dpm * NV, 0.0);
+        for (int si = 0; si < nis; si++) {
+          double inf = tb.inf[inf_state[si]];
+          if (inf == 0) continue;
+          for (int d = 0; d < dpm; d++)
+            for (int q = 0; q < dpm; q++)
+              F[(size_t)q * NV + si * dpm + d] +=
+                  Rd[q] * tb.sus[st * dpm + q] * inf * Ns[(size_t)(st * dpm + q) * G + st * dpm + d];
+        }
+        std::vector<double> NGM((size_t)dpm * dpm, 0.0);
+        for (int q = 0; q < dpm; q++)
+          for (int i = 0; i < dpm; i++) {
+            double sum = 0;
+            for (int j = 0; j < NV; j++) sum += F[(size_t)q * NV + j] * Vinv[(size_t)j * NV + i];
+            NGM[(size_t)q * dpm + i] = sum;
+          }
+        ratio = dominant_eigenvalue(NGM, dpm);
+        if (!(ratio > 0) || std::isnan(ratio)) {
+          err = "next-generation matrix has no positive dominant eigenvalue";
+          return BEEP_EINVAL;
+        }
+      }
+      fold[(size_t)s * nstrain + st] = Rfac / ratio;
+    }
+  }
+  for (int s = 0; s < nsteps; s++) {
+    double* Ns = tb.Ntime.data() + (size_t)s * G * G;
+    for (int q = 0; q < G; q++)
+      for (int d = 0; d < G; d++) Ns[(size_t)q * G + d] *= fold[(size_t)s * nstrain + q / dpm];
+  }
+  std::vector<double> Rt = disc_spline(R_spline, pd);
+  tb.beta_af.assign((size_t)S * A, 0.0);
+  for (int s = 0; s < S; s++)
+    for (int c = 0; c < A; c++) tb.beta_af[(size_t)s * A + c] = Rt[s] * af[(size_t)(s / spd) * A + c];
   return BEEP_OK;
 }
 

@@ -62,6 +62,8 @@ class HostModel {
   std::vector<std::vector<int>> sus_extra;                   // further factors, [G] each
   int nage = 0;
   std::vector<int> dp_age;                                    // [G] age group of each dp
+  int nstrain = 1;                                            // strains: G = nstrain x dps per strain
+  std::vector<int> strain_Rfactor;                            // [nstrain] parameter, -1 = 1
   int R_spline = -1, Rfactor_param = -1, efoi_spline = -1, geo_spline = -1;
   std::vector<double> efoi_agedist;
   std::vector<double> Rdist;          // [G] population fraction of the R-spline area group
@@ -93,6 +95,7 @@ class HostModel {
 
  private:
   std::vector<double> disc_spline(int sp, const std::vector<double>& pdir) const;
+  int strain_beta(const std::vector<double>& pd, const std::vector<double>& af, Tables& tb, std::string& err) const;
 };
 
 }  // namespace beephost

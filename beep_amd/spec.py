@@ -24,6 +24,8 @@ A spec is a plain dict (JSON-serialisable once arrays are lists):
                 spline's value (age_mixing_modify / age_mixing_perturb)
   sus           [G] params or None
   sus_extra     [[G] params]: further susceptibility factors (one per category)
+  strain_Rfactor [nstrain] params (-1 = 1), or absent for one strain; strains are the
+                last category (dp = strain * G / nstrain + dp within the strain)
   dirichlet     [{"param": [...], "frac": [...]}]
   obs           [{"y", "sd", "type", "shape", "factor", "thresh", "percent", "datatable",
                   "graph", "sett_i", "sett_f", "area": [...], "dp": [...], "tr": [...], "co": [...],

@@ -36,7 +36,8 @@ Supported: one or more age groups (`ages`, `age_mixing_matrix`, `age:` parameter
 and `area_tv_covars` (func log or linear), `area_effect` and `level_effect`
 (fixed or MDir(k) priors), `age_mixing_modify` (row_column) and
 `age_mixing_perturb` (a contact matrix that varies by step), `obs_spline`
-(named by a table's `factor_spline`), `data_tables` of type
+(named by a table's `factor_spline`), `democats`, `init_pop` and `strains`,
+`data_tables` of type
 'transition' or 'population' with obsmodel 'normal <sd>', 'normal <P>%',
 'normal load' or 'poisson'.  Presentation keys are ignored; `modification`,
 `prediction_start` and `prediction_end` act only in the reference's
@@ -68,7 +69,7 @@ SUPPORTED_KEYS = {"time_format", "start", "end", "datadir", "steps_per_unit_time
                   "efoi_spline", "areas", "data_tables", "threshold_str", "nodata_str", "geo_mixing_matrix",
                   "geo_mixing_modify", "ages", "age_mixing_matrix", "area_covars", "tv_covars", "area_tv_covars",
                   "area_effect", "level_effect", "age_mixing_modify", "age_mixing_perturb", "obs_spline",
-                  "democats", "init_pop"}
+                  "democats", "init_pop", "strains"}
 
 
 class TomlError(ValueError):
@@ -866,7 +867,7 @@ def _covariates(rd, doc, P, area_head, area_rows):
     return out
 
 
-def _initial_population(rd, doc, demo, comps, trans, head, rows):
+def _initial_population(rd, doc, demo, comps, trans, head, rows, strain_k=None):
     """Initial populations [A][C][G] (Data::read_initial_population,
     src/data.cc:288-457).  From the areas file: everyone in the infection's
     source compartment, one column per age group times, for every other
@@ -934,6 +935,8 @@ def _initial_population(rd, doc, demo, comps, trans, head, rows):
 
     fracs = {}
     for k in range(1, len(demo.cats)):
+        if k == strain_k:
+            continue
         vc = [_col(head, v) for v in demo.cats[k][1]]
         if None in vc:
             raise TomlError(f"the areas file needs a column for each value of '{demo.cats[k][0]}'")
@@ -944,9 +947,13 @@ def _initial_population(rd, doc, demo, comps, trans, head, rows):
     for a, row in enumerate(rows):
         pa = []
         for t in demo.dps:
+            if strain_k is not None and t[strain_k] > 0:      # everyone starts in strain 0's dps
+                pa.append(0)
+                continue
             v = num(row[age_cols[t[0]]])
             for k in range(1, len(demo.cats)):
-                v *= fracs[k][a][t[k]]
+                if k != strain_k:
+                    v *= fracs[k][a][t[k]]
             pa.append(int(v + 0.5))
         pop.append([pa if c == sus else [0] * G for c in range(C)])
     return pop
@@ -1264,6 +1271,15 @@ def load(path, datadir=None, with_data=True):
         if _s(e.get("name", "")) == "":
             raise TomlError("in 'democats' every category needs a 'name'")
         entries.append((_s(e["name"]), e, "democats"))
+    # Strains (src/inputs.cc:662-690): the last category.
+    strain_rf = None
+    if "strains" in doc:
+        e = doc["strains"]
+        if not isinstance(e, dict):
+            raise TomlError("'strains' must be a table with 'cats'")
+        _check_keys(e, {"cats", "sus_param", "sus_value", "sus_prior", "Rfactor_param", "Rfactor_value",
+                        "Rfactor_prior"}, "strains")
+        entries.append(("strain", e, "strains"))
     for name, e, key in entries:
         vals = _split(_s(e.get("cats", "")), "|")
         if not vals or "" in vals:
@@ -1281,6 +1297,16 @@ def load(path, datadir=None, with_data=True):
                 rd.split_n(_s(e.get("sus_prior", "")), n, key))
     demo = _Demo(cats)
     ages, G = demo.ages, demo.G
+    strain_k = len(cats) - 1 if "strains" in doc else None
+    if strain_k is not None:
+        e, names = doc["strains"], cats[strain_k][1]
+        n = len(names)
+        if n > 1:
+            pn = rd.split_n(_s(e.get("Rfactor_param", "")), n, "strains")
+            pv = rd.split_n(_s(e.get("Rfactor_value", "")), n, "strains")
+            pp = rd.split_n(_s(e.get("Rfactor_prior", "")), n, "strains")
+            strain_rf = [P.add(pn[i] or f"Rfactor_{names[i]}", pv[i], pp[i]) if (pv[i] or pp[i]) else -1
+                         for i in range(n)]
     comps = _compartments(rd, doc, P, demo)
     trans, dirichlet = _transitions(rd, doc, comps, P, demo)
 
@@ -1309,7 +1335,7 @@ def load(path, datadir=None, with_data=True):
     if _col(head, "area") is None:
         raise TomlError("the areas file needs a column 'area'")
     A = len(rows)
-    pop = _initial_population(rd, doc, demo, comps, trans, head, rows)
+    pop = _initial_population(rd, doc, demo, comps, trans, head, rows, strain_k)
     pop_dp = np.array(pop, dtype=np.float64).sum(axis=(0, 1))              # [G]
     pop_age = np.array([pop_dp[[dp for dp in range(G) if demo.dp_age[dp] == a]].sum()
                         for a in range(demo.nage)])
@@ -1321,6 +1347,8 @@ def load(path, datadir=None, with_data=True):
     for k, (nm, vl, pr) in sorted(sus_specs.items()):
         share = [float(pop_dp[[dp for dp in range(G) if demo.dps[dp][k] == v]].sum() / pop_dp.sum())
                  for v in range(len(demo.cats[k][1]))]
+        if k == strain_k:                         # the last category's weights are 1 (src/model.cc:2778-2779)
+            share = [1.0] * len(share)
         if any(x == "" for x in nm):
             raise TomlError(f"the susceptibility parameters of '{demo.cats[k][0]}' need names ('sus_param')")
         idx, grp = _modified_dirichlet(P, nm, vl, pr, share, f"susceptibility of '{demo.cats[k][0]}'")
@@ -1395,7 +1423,7 @@ def load(path, datadir=None, with_data=True):
         "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": efoi_ad,
         "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet + eff_groups + mod_groups,
         "sus": sus_terms[0] if sus_terms else None, "sus_extra": sus_terms[1:],
-        "nage": nage, "dp_age": demo.dp_age,
+        "nage": nage, "dp_age": demo.dp_age, "strain_Rfactor": strain_rf,
         "covars": _covariates(rd, doc, P, head, rows), "area_effect": area_eff, "level_effect": level,
         "age_mod": age_mod,
         "obs": _observations(rd, doc, comps, trans, with_data, head, rows, demo, obs_spl, pop),

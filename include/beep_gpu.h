@@ -174,6 +174,12 @@ typedef struct {
   const int32_t* sus_extra;     /* [nsus_extra][G] parameters (-1 = 1); a dp's
                                    susceptibility is the product with sus_param's
                                    (Model::create_susceptibility, src/model.cc:1144-1178) */
+  /* Strains (strains, src/inputs.cc:662-690): the last category, so dp =
+   * strain * (G / nstrain) + the dp within a strain.  Each strain infects with
+   * its own force of infection, beta = Rfactor * R(t) / (its NGM ratio), and
+   * draws on the susceptibles of every strain's dp, pooled (src/state.cc:380-430). */
+  int32_t nstrain;              /* 0 or 1 = one strain                                 */
+  const int32_t* strain_Rfactor;/* [nstrain] Rfactor parameter of each strain, -1 = 1  */
 } beep_spec;
 
 /* Host only (no device is touched): State::set_param's FP64 tables for one

@@ -101,6 +101,7 @@ struct OrModel {
   const int* Mto;      // [nnz]
   const double* Mval;  // [nnz]
   const double* Mfac;  // [A]   genQ.factor
+  int nstrain;         // data.nstrain (1 = one strain); strains are the last category
 };
 
 // Quantities State::set_param derives from one parameter vector.
@@ -237,8 +238,23 @@ int or_set_transmean(const OrModel* m, const OrTables* tb, OrState* st, int s, i
   double be = tb->beta[(size_t)s * m->A + c];
   be *= tb->af[(size_t)(s / m->spd) * m->A + c];
   double et = tb->efoi[s];
+  // Strains (src/state.cc:395-404): the susceptibles of a dp are pooled into
+  // strain 0 (in the state itself) and every strain's infection draws on the
+  // pool.  (The tables carry each strain's Rfactor / ratio in its block of
+  // Ntime, so be is R(t) * areafactor here.)
+  const int ns = m->nstrain > 1 ? m->nstrain : 1, dpm = G / ns;
+  if (ns > 1)
+    for (int dp = 0; dp < dpm; dp++) {
+      double sum = 0;
+      for (int k = 1; k < ns; k++) {
+        double& v = st->pop[ix_pop(m, s, c, m->from[tr], k * dpm + dp)];
+        sum += v;
+        v = 0;
+      }
+      st->pop[ix_pop(m, s, c, m->from[tr], dp)] += sum;
+    }
   for (int dp = 0; dp < G; dp++) {
-    double popu = st->pop[ix_pop(m, s, c, m->from[tr], dp)];
+    double popu = st->pop[ix_pop(m, s, c, m->from[tr], dp % dpm)];
     double& out = st->transmean[ix_tr(m, s, c, tr, dp)];
     if (popu <= 0) out = 0;
     else {

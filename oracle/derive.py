@@ -61,6 +61,9 @@ def normalise(spec):
     s = sum(pop_a[aa] * N_age[aa, a] for a in range(nage) for aa in range(nage)) / (nage * pop_a.sum())
     N_age = N_age / s
     N = N_age[np.ix_(dp_age, dp_age)]
+    ns = len(spec.get("strain_Rfactor") or [1])          # strains mix within a strain only
+    strain = np.arange(G) // (G // ns)
+    N = N * (strain[:, None] == strain[None, :])
     # Geographic mixing.
     M = spec.get("M")
     if M is None:
@@ -109,7 +112,7 @@ def normalise(spec):
     for cv in spec.get("covars", []):
         v = np.asarray(cv["value"], dtype=np.float64).reshape(A, spec["period"])
         covars.append((cv["param"], v - v.mean()))
-    return dict(N=N, N_age=N_age, dp_age=dp_age, diag=diag, rowptr=rowptr, to=to, val=val, factor=factor,
+    return dict(N=N, N_age=N_age, dp_age=dp_age, strain=strain, nstrain=ns, diag=diag, rowptr=rowptr, to=to, val=val, factor=factor,
                 area_pop=area_pop, popsize=popsize, Rdist=pop_age / popsize, inf_state=inf_state, itr=itr, exits=exits,
                 covars=covars)
 
@@ -235,6 +238,9 @@ def derive(spec, nz, pv):
                 Na[s, a, :] *= f[s]
                 Na[s, :, a] *= f[s]
     Ntime = Na[:, nz["dp_age"][:, None], nz["dp_age"][None, :]]
+    Ntime = Ntime * (nz["strain"][:, None] == nz["strain"][None, :])[None, :, :]
+    if nz["nstrain"] > 1:
+        return _derive_strains(spec, nz, pd, rate, sus, inf, af, Ntime, S, A, G)
     # The ratio is recomputed where the matrix changes by more than TINY
     # (src/model.cc:1335-1340, Model::equal 1303-1314).
     ratio = np.zeros(S)
@@ -254,6 +260,60 @@ def derive(spec, nz, pv):
     geo = disc_spline(spec, spec["geo_spline"], pd) if spec.get("geo_spline", -1) >= 0 else np.ones(S)
     return dict(rate=rate, sus=sus, inf=inf, af=af, beta=beta_SA, efoi=efoi, agedist=agedist, geo=geo,
                 Ntime=Ntime, ratio=ratio[0] if not spec.get("age_mod") else ratio)
+
+
+def _derive_strains(spec, nz, pd, rate, sus, inf, af, Ntime, S, A, G):
+    """Strains (Model::calculate_beta_from_R, src/model.cc:1318-1348; calculate_Vinv
+    and calculate_F per strain, src/model.cc:2485-2556): each strain's NGM over
+    its own dps with the population fractions pooled over strains and that
+    strain's susceptibility; beta_st = Rfactor_st * R(t) / ratio_st.  Returned
+    in the device's form: Rfactor_st / ratio_st folded into the strain's block
+    of Ntime and beta = R(t) (times the area factor by the caller)."""
+    ns = nz["nstrain"]
+    dpm = G // ns
+    Rd = np.zeros(dpm)
+    for g in range(G):
+        Rd[g % dpm] += nz["Rdist"][g]
+    ist = nz["inf_state"]
+    ref = {c: i for i, c in enumerate(ist)}
+    NV = len(ist) * dpm
+    ratio = np.zeros((S, ns))
+    for st in range(ns):
+        V = np.zeros((NV, NV))
+        for k, t in enumerate(spec["trans"]):
+            fi, ti = ref.get(t["from"]), ref.get(t["to"])
+            if fi is None:
+                continue
+            for q in range(dpm):
+                V[fi * dpm + q, fi * dpm + q] += rate[k, st * dpm + q]
+                if ti is not None:
+                    V[ti * dpm + q, fi * dpm + q] -= rate[k, st * dpm + q]
+        Vinv = np.linalg.inv(V)
+        for s in range(S):
+            if s > 0 and not np.any(np.abs(Ntime[s] - Ntime[s - 1]) > 1e-8):
+                ratio[s, st] = ratio[s - 1, st]
+                continue
+            F = np.zeros((dpm, NV))
+            for si, co in enumerate(ist):
+                if inf[co] == 0:
+                    continue
+                for d in range(dpm):
+                    for q in range(dpm):
+                        F[q, si * dpm + d] += Rd[q] * sus[st * dpm + q] * inf[co] * Ntime[s, st * dpm + q, st * dpm + d]
+            ratio[s, st] = largest_eigenvalue((F @ Vinv)[:, :dpm])
+    Rfac = np.array([pd[p] if p >= 0 else 1.0 for p in spec["strain_Rfactor"]])
+    fold = (Rfac[None, :] / ratio)[:, nz["strain"]]                    # [S][G]
+    Ntime = Ntime * fold[:, :, None]
+    beta = disc_spline(spec, spec["R_spline"], pd)
+    beta_SA = np.repeat(beta[:, None], A, axis=1)
+    if spec.get("efoi_spline", -1) >= 0:
+        efoi = disc_spline(spec, spec["efoi_spline"], pd) / nz["popsize"]
+    else:
+        efoi = np.zeros(S)
+    agedist = np.asarray(spec.get("efoi_agedist", [1.0 / G] * G), dtype=np.float64)
+    geo = disc_spline(spec, spec["geo_spline"], pd) if spec.get("geo_spline", -1) >= 0 else np.ones(S)
+    return dict(rate=rate, sus=sus, inf=inf, af=af, beta=beta_SA, efoi=efoi, agedist=agedist, geo=geo,
+                Ntime=Ntime, ratio=ratio)
 
 
 def prior(spec, pv):

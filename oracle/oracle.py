@@ -66,7 +66,7 @@ class _OrModel(C.Structure):
                 ("inf_tr", C.c_int), ("stochastic", C.c_int), ("dt", C.c_double),
                 ("from_", C.POINTER(C.c_int)), ("to", C.POINTER(C.c_int)), ("Mdiag", C.POINTER(C.c_double)),
                 ("Mptr", C.POINTER(C.c_int)), ("Mto", C.POINTER(C.c_int)), ("Mval", C.POINTER(C.c_double)),
-                ("Mfac", C.POINTER(C.c_double))]
+                ("Mfac", C.POINTER(C.c_double)), ("nstrain", C.c_int)]
 
 
 class _OrTables(C.Structure):
@@ -127,7 +127,7 @@ class Oracle:
         self._fac = np.ascontiguousarray(nz["factor"], dtype=np.float64)
         self.model = _OrModel(A, G, C_, T, S, spec["spd"], nz["itr"], 1 if stochastic else 0,
                               spec["period"] / S, _i(self._from), _i(self._to), _d(self._diag), _i(self._ptr),
-                              _i(self._mto), _d(self._val), _d(self._fac))
+                              _i(self._mto), _d(self._val), _d(self._fac), len(spec.get("strain_Rfactor") or [1]))
         self.pop_init = np.ascontiguousarray(np.asarray(spec["pop"], dtype=np.float64).reshape(A, C_, G))
         self._build_obs()
 

@@ -46,7 +46,7 @@ def _transmean_magnitude(orc, tb, st):
 
 @pytest.mark.parametrize("mk", [configs.seir_single, configs.spatial_abcsmc, toml_models.toml_seir_erlang,
                                 toml_models.toml_seird_branch, toml_models.toml_sir_3areas, toml_models.toml_sird_ages,
-                                toml_models.toml_seir_sex_ages])
+                                toml_models.toml_seir_sex_ages, toml_models.toml_sird_strains])
 def test_fixed_trajectory_replay(gpu_lib, mk):
     spec, truth = mk()
     orc = O.Oracle(spec)
@@ -139,6 +139,35 @@ def test_simulate_distribution_seir(gpu_lib):
     for t in (10, 30, 60, 99):
         for c, name in enumerate("SEIR"):
             _ks_compare(gpu[:, c, t], ref[:, c, t], f"{name}@{t}")
+    ctx.close()
+
+
+def test_simulate_distribution_strains(gpu_lib):
+    """Two strains drawing on pooled susceptibles: 5000 device replicates vs
+    5000 oracle replicates; per strain, the cumulative infections and the
+    infected at days 30 and 69, and the pooled susceptibles."""
+    spec, truth = toml_models.toml_sird_strains()
+    n = 5000
+    orc = O.Oracle(spec)
+    tb = orc.tables(truth)
+    itr = [t["infection"] for t in spec["trans"]].index(True)
+    S = spec["period"] * spec["spd"]
+
+    def summary(pop, tn):
+        out = []
+        for t in (30, S - 1):
+            out += [tn[:t, 0, itr, 0:2].sum(), tn[:t, 0, itr, 2:4].sum(), pop[t, 0, 1, 0:2].sum(),
+                    pop[t, 0, 1, 2:4].sum(), pop[t, 0, 0, 0:2].sum()]
+        return out
+
+    ref = np.array([summary((st := orc.simulate(tb, seed=3000 + r)).pop, st.transnum) for r in range(n)])
+    ctx = _ctx(spec, n, keep=_lib.KEEP_ALL)
+    ctx.set_param([0], truth[None, :])
+    ctx.simulate(np.arange(n), np.zeros(n), np.arange(n) + 91, seed=2025)
+    gpu = np.array([summary(ctx.get_pop(r), ctx.get_transnum(r)) for r in range(n)])
+    assert np.all(gpu[:, 4] > 0) and ref[:, 1].mean() > 10
+    for k in range(ref.shape[1]):
+        _ks_compare(gpu[:, k], ref[:, k], f"summary {k}")
     ctx.close()
 
 

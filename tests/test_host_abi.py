@@ -24,7 +24,7 @@ def test_header_symbols_exported(gpu_lib):
 @pytest.mark.parametrize("mk", [configs.seir_single, configs.spatial_abcsmc, configs.dense_pmcmc,
                                 toml_models.toml_seir_erlang, toml_models.toml_seird_branch,
                                 toml_models.toml_sir_3areas, toml_models.toml_sird_ages,
-                                toml_models.toml_seir_sex_ages])
+                                toml_models.toml_seir_sex_ages, toml_models.toml_sird_strains])
 def test_host_tables_match_oracle(gpu_lib, mk):
     spec, truth = mk()
     nz = derive.normalise(spec)
@@ -37,7 +37,7 @@ def test_host_tables_match_oracle(gpu_lib, mk):
         assert np.allclose(t["beta_af"], ba, rtol=1e-12, atol=0)
         assert np.allclose(t["rate"], d["rate"], rtol=1e-14)
         assert np.allclose(t["sus"], d["sus"]) and np.allclose(t["inf"], d["inf"])
-        assert np.allclose(t["N"], d["Ntime"] if spec.get("age_mod") else nz["N"], rtol=1e-14)
+        assert np.allclose(t["N"], d["Ntime"] if spec.get("age_mod") else d["Ntime"][0], rtol=1e-12)
         assert np.allclose(t["efoi"], d["efoi"][:, None] * d["agedist"][None, :])
         assert np.allclose(t["geo"], d["geo"])
         assert t["Pr"] == pytest.approx(derive.prior(spec, pv), rel=1e-14)

@@ -210,6 +210,36 @@ def test_observation_values_by_loops(name):
         assert np.count_nonzero(values) > len(values) // 2
 
 
+def test_strains_beta_and_pooled_susceptibles():
+    """Two strains with the same rates (src/state.cc:380-430, src/model.cc:
+    1318-1348): each strain's NGM ratio is proportional to its corrected
+    susceptibility, so ratio_s2 / ratio_s1 = sus_s2 / sus_s1, and its block of
+    the contact matrix carries Rfactor_st / ratio_st; no mixing between
+    strains.  In a simulation the susceptibles stay pooled in strain 1's dps
+    (the other strain's read zero at every step) and fall by both strains'
+    infections."""
+    import toml_models
+    spec, truth = toml_models.toml_sird_strains()
+    nz = derive.normalise(spec)
+    d = derive.derive(spec, nz, truth)
+    names = [p["name"] for p in spec["params"]]
+    s1, s2 = truth[names.index("sus s1")], truth[names.index("sus s2")]
+    assert d["ratio"][0, 1] / d["ratio"][0, 0] == pytest.approx(s2 / s1, rel=1e-9)
+    N = d["Ntime"][0]
+    assert np.all(N[:2, 2:] == 0) and np.all(N[2:, :2] == 0)
+    rf2 = truth[names.index("Rfactor_s2")]
+    assert np.allclose(N[2:, 2:] / N[:2, :2], rf2 * d["ratio"][0, 0] / d["ratio"][0, 1], rtol=1e-12)
+    orc = O.Oracle(spec)
+    st = orc.simulate(orc.tables(truth), seed=5)
+    S_idx, itr = 0, nz["itr"]
+    assert np.all(st.pop[:, 0, S_idx, 2:] == 0)
+    inf = st.transnum[:, 0, itr, :]
+    assert inf[:, 2:].sum() > 0 and inf[:, :2].sum() > 0
+    for s in range(orc.S - 1):
+        for dp in range(2):
+            assert st.pop[s + 1, 0, S_idx, dp] == st.pop[s, 0, S_idx, dp] - inf[s, dp] - inf[s, dp + 2]
+
+
 def test_deterministic_seir_is_forward_euler():
     """Deterministic dynamics with one area: the forward-Euler SEIR recursion
     with FOI = beta I / N (Idiag = I / N when A = 1)."""

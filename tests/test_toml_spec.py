@@ -398,6 +398,26 @@ def test_demographic_categories():
     assert all(o["co"] == [2] and o["sett_f"] == o["sett_i"] + 1 for o in fr)
 
 
+def test_strains():
+    """strains (src/inputs.cc:662-690): the last category, dp = strain x 2 +
+    age; Rfactor parameters named 'Rfactor_<strain>' by default; strain
+    susceptibility a modified Dirichlet group with weights 1
+    (src/model.cc:2768-2784); everyone starts in the first strain's dps
+    (src/data.cc:440-456); data columns per strain."""
+    import toml_models
+    spec, values = toml_models.toml_sird_strains()
+    names = [p["name"] for p in spec["params"]]
+    assert spec["G"] == 4 and spec["dp_age"] == [0, 1, 0, 1]
+    assert [names[i] for i in spec["strain_Rfactor"]] == ["Rfactor_s1", "Rfactor_s2"]
+    assert [spec["params"][i]["prior"] for i in spec["strain_Rfactor"]] == [("fixed", 1.0), ("uniform", 0.5, 2.0)]
+    assert spec["pop"][0][0] == [60000, 40000, 0, 0]
+    s1, s2 = names.index("sus s1"), names.index("sus s2")
+    assert spec["sus"] == [s1, s1, s2, s2] and {"param": [s1, s2], "frac": [1.0, 1.0]} in spec["dirichlet"]
+    assert values[s2] == pytest.approx(0.3) and spec["params"][s1]["prior"] == ("gamma", 3.0, 1.0)
+    si = [o for o in spec["obs"] if o["datatable"] == 0]
+    assert {tuple(o["dp"]) for o in si[:10]} == {(0, 1)} and {tuple(o["dp"]) for o in si[10:]} == {(2, 3)}
+
+
 def test_multi_area_needs_matrix(tmp_path):
     d = str(tmp_path)
     _write(d, "areas.csv", "area,population\nX,100\nY,200\n")

@@ -35,3 +35,8 @@ def toml_sird_ages():
 def toml_seir_sex_ages():
     """SEIR over two areas with two age groups and two sexes: susceptibility by age and sex, per-sex counts."""
     return _load("seir_sex_ages.toml", "data_seir_sex_ages")
+
+
+def toml_sird_strains():
+    """SIRD with two strains and two age groups: strain susceptibility and Rfactor, pooled susceptibles."""
+    return _load("sird_strains.toml", "data_sird_strains")

@@ -28,7 +28,8 @@ SYMBOLS = [
     "beep_put_transnum", "beep_copy_instances", "beep_synchronize", "beep_debug_philox", "beep_debug_poisson",
     "beep_debug_binomial", "beep_timing", "beep_debug_gemm", "beep_run_batch", "beep_copy_states",
     "beep_derive_slots", "beep_sum_transnum", "beep_pmcmc_obs_prob", "beep_pmcmc_particle_sample", "beep_bootstrap", "beep_path",
-    "beep_abcsmc_weights", "beep_mc3_swap",
+    "beep_abcsmc_weights", "beep_mc3_swap", "beep_debug_check_launch", "beep_batch_accept", "beep_batch_mc3_swap",
+    "beep_batch_set_param",
 ]
 
 
@@ -141,7 +142,12 @@ def lib():
             "beep_debug_poisson": (C.c_int, [C.c_int, C.c_int, f32p, C.c_uint64, C.c_uint32, i32p]),
             "beep_debug_binomial": (C.c_int, [C.c_int, C.c_int, i32p, f64p, C.c_uint64, C.c_uint32, i32p]),
             "beep_debug_gemm": (C.c_int, [C.c_int, C.c_int, C.c_int, f32p, f32p, f32p]),
-            "beep_run_batch": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, f64p]),
+            "beep_run_batch": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, f64p, f64p]),
+            "beep_batch_accept": (C.c_int, [C.c_int, C.c_void_p, C.c_int, f64p, C.c_double, f64p, C.c_uint64,
+                                            C.c_uint32, f64p, u8p]),
+            "beep_batch_mc3_swap": (C.c_int, [C.c_int, i32p, C.c_void_p, i32p, i32p, f64p, C.c_uint64, C.c_uint32,
+                                              i32p, i32p, i32p]),
+            "beep_batch_set_param": (C.c_int, [C.c_int, C.c_void_p, i32p, C.c_void_p, f64p, i32p]),
             "beep_copy_states": (C.c_int, [P, C.c_int, i32p, i32p]),
             "beep_derive_slots": (C.c_int, [P, C.c_int, i32p, i32p, f64p, i32p]),
             "beep_sum_transnum": (C.c_int, [P, C.c_int, i32p, C.c_int, C.c_int, C.c_int, f64p]),
@@ -154,6 +160,8 @@ def lib():
             "beep_mc3_swap": (C.c_int, [P, C.c_int, i32p, i32p, f64p, C.c_uint64, C.c_uint32, i32p, i32p, i32p]),
             "beep_bootstrap": (C.c_int, [C.c_int, C.c_int, f64p, C.c_double, C.c_uint64, C.c_uint32, C.c_int, i32p,
                                          f64p]),
+            "beep_debug_check_launch": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int64),
+                                                  C.c_char_p, C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -266,8 +274,16 @@ class _SpecBuffers:
         keep.append(ds)
         s.ndirichlet, s.dirichlet = len(dirs), ds
         obs = spec.get("obs", [])
-        os_ = (Obs * max(1, len(obs)))()
-        for i, o in enumerate(obs):
+        if spec.get("obs_table") is not None:
+            if obs:
+                raise ValueError("a spec holds obs or obs_table, not both")
+            os_, nobs = _obs_from_table(spec["obs_table"], keep)
+            keep.append(os_)
+            s.nobs, s.obs = nobs, os_
+            obs = None
+        else:
+            os_ = (Obs * max(1, len(obs)))()
+        for i, o in enumerate(obs or []):
             x = os_[i]
             x.y = o["y"]
             x.sd = o.get("sd", 0.0)
@@ -285,8 +301,9 @@ class _SpecBuffers:
             x.ntr, x.tr = len(tr), i32(tr) if tr else null_i32
             x.nco, x.co = len(co), i32(co) if co else null_i32
             x.factor_spline = o.get("spline", -1) + 1       # 0 = none
-        keep.append(os_)
-        s.nobs, s.obs = len(obs), os_
+        if obs is not None:
+            keep.append(os_)
+            s.nobs, s.obs = len(obs), os_
         secs = spec.get("sections", [])
         s.nsection = len(secs)
         s.section_ti = i32([a for a, b in secs]) if secs else null_i32
@@ -315,6 +332,41 @@ class _SpecBuffers:
         else:
             s.nlevel, s.level_param, s.level_map = 0, null_i32, null_i32
         self.spec = s
+
+
+def _obs_from_table(tab, keep):
+    """The ctypes observation array of a compact obs_table, filled column by
+    column through numpy views of the struct array (no per-observation Python
+    work): one area each, shared dp and transition lists."""
+    y = np.asarray(tab["y"], dtype=np.float64)
+    n = len(y)
+    arr = (Obs * max(1, n))()
+    size = C.sizeof(Obs)
+
+    def col(name, dt):
+        return np.ndarray((n,), dtype=dt, buffer=arr, offset=getattr(Obs, name).offset, strides=(size,))
+
+    area = _i32(tab["area"])
+    dp, tr = _i32(tab["dp"]), _i32(tab["tr"])
+    areas = np.arange(int(area.max()) + 1 if n else 1, dtype=np.int32)
+    keep.extend([area, dp, tr, areas])
+    col("y", np.float64)[:] = y
+    col("sd", np.float64)[:] = np.asarray(tab["sd"], dtype=np.float64)
+    col("shape", np.float64)[:] = 1.0
+    col("factor", np.float64)[:] = 1.0
+    t = tab["type"]
+    col("type", np.int32)[:] = specmod.OBS_CODES[t] if isinstance(t, str) else t
+    col("datatable", np.int32)[:] = int(tab.get("datatable", 0))
+    col("graph", np.int32)[:] = _i32(tab["graph"])
+    col("sett_i", np.int32)[:] = _i32(tab["sett_i"])
+    col("sett_f", np.int32)[:] = _i32(tab["sett_f"])
+    col("narea", np.int32)[:] = 1
+    col("area", np.uint64)[:] = np.uint64(areas.ctypes.data) + area.astype(np.uint64) * np.uint64(4)
+    col("ndp", np.int32)[:] = len(dp)
+    col("dp", np.uint64)[:] = dp.ctypes.data
+    col("ntr", np.int32)[:] = len(tr)
+    col("tr", np.uint64)[:] = tr.ctypes.data
+    return arr, n
 
 
 def spec_tables(spec, paramval):
@@ -350,19 +402,103 @@ WORK_SIMULATE, WORK_MBP = 0, 1
 BATCH_AUTOTUNE = 1
 
 
-def run_batch(items, seed=1, stream=0, autotune=False):
+def _work_array(items):
+    """The beep_work array of (ctx, kind, inst, init, slot, key, t0, t1)
+    items, filled column by column."""
+    n = len(items)
+    arr = (Work * max(1, n))()
+    size = C.sizeof(Work)
+
+    def col(name, dt):
+        return np.ndarray((n,), dtype=dt, buffer=arr, offset=getattr(Work, name).offset, strides=(size,))
+
+    cols = list(zip(*items)) if n else [()] * 8
+    col("ctx", np.uint64)[:] = np.array([c if isinstance(c, (int, np.integer)) else c._h.value for c in cols[0]],
+                                        dtype=np.uint64)
+    for k, (name, dt) in enumerate((("kind", np.int32), ("inst", np.int32), ("init", np.int32), ("slot", np.int32),
+                                    ("key", np.uint32), ("t0", np.int32), ("t1", np.int32)), start=1):
+        col(name, dt)[:] = np.asarray(cols[k], dtype=np.int64).astype(dt)
+    return arr
+
+
+BATCH_LOGLIK = 2
+
+
+def run_batch(items, seed=1, stream=0, autotune=False, loglik=False):
     """Runs (ctx, kind, inst, init, slot, key, t0, t1) work items across
     contexts in one batch; returns (wall ms, fused launches, step-wise
-    contexts, cell-steps)."""
+    contexts, cell-steps), and with loglik=True also each item's observation
+    log-likelihood."""
     n = len(items)
-    arr = (Work * n)()
-    for i, (ctx, kind, inst, init, slot, key, t0, t1) in enumerate(items):
-        arr[i] = Work(ctx._h.value, kind, inst, init, slot, key, t0, t1)
+    arr = _work_array(items)
     stats = np.zeros(4)
-    rc = lib().beep_run_batch(n, arr, seed, stream, BATCH_AUTOTUNE if autotune else 0, _p(stats, C.c_double))
+    L = np.zeros(max(1, n))
+    flags = (BATCH_AUTOTUNE if autotune else 0) | (BATCH_LOGLIK if loglik else 0)
+    rc = lib().beep_run_batch(n, arr, seed, stream, flags, _p(L, C.c_double) if loglik else None,
+                              _p(stats, C.c_double))
     if rc != BEEP_OK:
         raise BeepError(rc, lib().beep_last_error(C.c_void_p(items[0][0]._h.value)).decode())
-    return stats
+    return (stats, L[:n]) if loglik else stats
+
+
+def handles(ctxs):
+    """The contexts' handles as a uint64 array (reusable across batch calls)."""
+    return np.array([c._h.value for c in ctxs], dtype=np.uint64)
+
+
+def batch_set_param(ctxs, slots, paramvals):
+    """State::set_param for slots of many contexts in one call: row i of the
+    2-D array paramvals holds the parameter vector of slot slots[i] of context
+    ctxs[i] (its first nparam entries).  ctxs: Context objects or handles().
+    Returns (Pr, inbounds)."""
+    h = ctxs if isinstance(ctxs, np.ndarray) else handles(ctxs)
+    n = len(h)
+    pv = _f64(paramvals)
+    assert pv.ndim == 2 and pv.shape[0] == n
+    ptrs = np.uint64(pv.ctypes.data) + np.arange(n, dtype=np.uint64) * np.uint64(pv.strides[0])
+    slots = _i32(slots)
+    Pr = np.zeros(n)
+    inb = np.zeros(n, dtype=np.int32)
+    rc = lib().beep_batch_set_param(n, h.ctypes.data, _p(slots, C.c_int32), ptrs.ctypes.data, _p(Pr, C.c_double),
+                                    _p(inb, C.c_int32))
+    if rc != BEEP_OK:
+        raise BeepError(rc, lib().beep_last_error(C.c_void_p(int(h[0]))).decode())
+    return Pr, inb.astype(bool)
+
+
+def batch_accept(items, invT, dlogq, mode=1, EFcut=0.0, seed=1, stream=0):
+    """Metropolis-Hastings acceptance of MBP items across contexts on the
+    device: (al, accepted)."""
+    n = len(items)
+    arr = _work_array(items)
+    invT, dlogq = _f64(np.broadcast_to(invT, (n,))), _f64(np.broadcast_to(dlogq, (n,)))
+    al = np.zeros(n)
+    acc = np.zeros(n, dtype=np.uint8)
+    rc = lib().beep_batch_accept(n, arr, mode, _p(invT, C.c_double), float(EFcut), _p(dlogq, C.c_double), seed, stream,
+                                 _p(al, C.c_double), _p(acc, C.c_uint8))
+    if rc != BEEP_OK:
+        raise BeepError(rc, lib().beep_last_error(C.c_void_p(items[0][0]._h.value)).decode())
+    return al, acc.astype(bool)
+
+
+def batch_mc3_swap(ladders, seed=1, stream=0):
+    """MC3 swaps over many ladders at once: ladders is a list of lists of
+    (ctx, inst, run, invT) chains in temperature order.  Returns per ladder
+    (order, ntr, nac)."""
+    ptr = np.cumsum([0] + [len(l) for l in ladders]).astype(np.int32)
+    chains = [ch for l in ladders for ch in l]
+    n = len(chains)
+    ctxs = (C.c_void_p * n)(*[ch[0]._h.value for ch in chains])
+    inst = _i32([ch[1] for ch in chains])
+    run = _i32([ch[2] for ch in chains])
+    invT = _f64([ch[3] for ch in chains])
+    order, ntr, nac = (np.zeros(n, dtype=np.int32) for _ in range(3))
+    rc = lib().beep_batch_mc3_swap(len(ladders), _p(ptr, C.c_int32), ctxs, _p(inst, C.c_int32), _p(run, C.c_int32),
+                                   _p(invT, C.c_double), seed, stream, _p(order, C.c_int32), _p(ntr, C.c_int32),
+                                   _p(nac, C.c_int32))
+    if rc != BEEP_OK:
+        raise BeepError(rc, lib().beep_last_error(C.c_void_p(chains[0][0]._h.value)).decode())
+    return [(order[a:b], ntr[a:b], nac[a:b]) for a, b in zip(ptr[:-1], ptr[1:])]
 
 
 class Context:
@@ -590,6 +726,22 @@ def bootstrap(L, invT=1.0, seed=1, stream=0, sec=0, device=0):
     if rc:
         raise BeepError(rc, "beep_bootstrap")
     return bp, float(ob[0])
+
+
+CHECK_FUSED, CHECK_STEPWISE = 0, 1
+
+
+def check_launch(kind, args):
+    """The host-side launch checks (no device needed): the grid a fused launch
+    (args njobs, block, lds, lds limit) or a step-wise launch (args A, G, Apad,
+    n, capacity) would use; raises BeepError(EINVAL) with the reason."""
+    a = np.ascontiguousarray(np.asarray(args, dtype=np.int64))
+    g = np.zeros(6, dtype=np.int64)
+    err = C.create_string_buffer(256)
+    rc = lib().beep_debug_check_launch(kind, _p(a, C.c_int64), len(a), _p(g, C.c_int64), err, 256)
+    if rc != BEEP_OK:
+        raise BeepError(rc, err.value.decode())
+    return g[:1] if kind == CHECK_FUSED else g
 
 
 def debug_philox(ctr, k0, k1, device=0):

@@ -271,8 +271,12 @@ def daily_observations(transnum, tr, percent=10.0, datatable=0):
             for c in range(A) if y[:, c].max() > 0 for d in range(S)]
 
 
-def seir_areas(A, G, period, seed, obs_every=0):
-    """SEIR in every area with 10-NN mixing, G in {1, 4, 9} age groups."""
+def seir_areas(A, G, period, seed, obs_every=0, daily_obs=False):
+    """SEIR in every area with 10-NN mixing, G in {1, 4, 9} age groups.
+    obs_every: Poisson I->R observations per area every that many days.
+    daily_obs: daily I->R counts per area (all age groups) with normal errors
+    of sd max(5, 0.1 count), as the single-area SEIR's, stored as a compact
+    observation table."""
     rng = np.random.default_rng(seed)
     fr = {1: (1.0,), 4: AGE_FRAC4, 9: (0.11, 0.12, 0.13, 0.13, 0.13, 0.13, 0.11, 0.09, 0.05)}[G]
     ages = _age_pops(rng, A, fr, 5e4, 0.8)
@@ -302,6 +306,11 @@ def seir_areas(A, G, period, seed, obs_every=0):
         spec["obs"] = [{"y": float(y[c, d]), "type": "poisson", "datatable": 0, "graph": c, "sett_i": d,
                         "sett_f": d + 1, "area": [c], "dp": list(range(G)), "tr": [2]}
                        for c in range(A) for d in range(0, period, obs_every)]
+    elif daily_obs:
+        y = _epi_curves(rng, A, period, 30.0).reshape(-1)          # [A][period], area-major
+        c, d = np.divmod(np.arange(A * period), period)
+        spec["obs_table"] = {"y": y, "sd": np.maximum(5.0, 0.1 * y), "type": "normal", "datatable": 0,
+                             "graph": c, "area": c, "sett_i": d, "sett_f": d + 1, "dp": list(range(G)), "tr": [2]}
     truth = np.array([2.2, 4.0, 6.0, 1.0])
     return spec, truth
 
@@ -310,24 +319,28 @@ def steps(spec):
     return nsteps(spec)
 
 
-def uneven_batch(n=16384, seed=4, frac_mc3=0.5):
-    """Instances of the SEIR-in-every-area model with area counts uniform in
-    10..400, horizons uniform in 50..400 days and age groups in {1, 4, 9}.
-    Instances with the same (areas, age groups) share one topology.  Returns
-    a list of groups {A, G, S, sim_horizons, n_mc3}: the ABC-SMC particles
-    (fresh simulations over their own horizons) and the MC3 chains (MBP
-    proposals over the group's horizon S, the longest in the group)."""
+def uneven_batch(n=16384, seed=4, frac_mc3=0.5, ladder=8):
+    """The uneven workload: n instances of the SEIR-in-every-area model with
+    area counts uniform in 10..400, horizons uniform in 50..400 days and age
+    groups in {1, 4, 9}, half ABC-SMC particles and half MC3 chains.
+    Returns {"abc": groups, "mc3": ladders}:
+      groups   ABC-SMC particles with the same (areas, age groups) share one
+               model: {A, G, S (the longest horizon), horizons}; each particle
+               is simulated over its own horizon;
+      ladders  MC3 chains in ladders of `ladder` temperatures, each ladder one
+               model {A, G, S} (the draw of its first chain)."""
     rng = np.random.default_rng(seed)
     A = rng.integers(10, 401, n)
     S = rng.integers(50, 401, n)
     G = rng.choice([1, 4, 9], n)
     mc3 = rng.random(n) < frac_mc3
     groups = {}
-    for a, s, g, m in zip(A, S, G, mc3):
-        d = groups.setdefault((int(a), int(g)), {"A": int(a), "G": int(g), "S": 0, "sim_horizons": [], "n_mc3": 0})
+    for a, s, g in zip(A[~mc3], S[~mc3], G[~mc3]):
+        d = groups.setdefault((int(a), int(g)), {"A": int(a), "G": int(g), "S": 0, "horizons": []})
         d["S"] = max(d["S"], int(s))
-        if m:
-            d["n_mc3"] += 1
-        else:
-            d["sim_horizons"].append(int(s))
-    return [groups[k] for k in sorted(groups)]
+        d["horizons"].append(int(s))
+    idx = np.flatnonzero(mc3)
+    nl = len(idx) // ladder
+    ladders = [{"A": int(A[idx[l * ladder]]), "G": int(G[idx[l * ladder]]), "S": int(S[idx[l * ladder]])}
+               for l in range(nl)]
+    return {"abc": [groups[k] for k in sorted(groups)], "mc3": ladders, "ladder": ladder}

@@ -16,10 +16,12 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "beep_gpu.h"
 #include "host_model.hh"
+#include "launch_geometry.hh"
 #include "kernels.hh"
 #include "obs_kernels.hh"
 #include "pmcmc_kernels.hh"
@@ -89,7 +91,13 @@ struct beep_ctx {
   float* d_D = nullptr;            // [capacity*G][Apad]
   float* d_I = nullptr;            // [capacity*G][Apad]
   uint64_t* d_rngp = nullptr;      // seed, stream read by the captured step kernels
-  struct Graph { int mode = -1, n = 0, t0 = 0, t1 = 0; const Job* jobs = nullptr; hipGraphExec_t exec = nullptr; };
+  struct Graph {
+    int mode = -1, n = 0, t0 = 0, t1 = 0;
+    const Job* jobs = nullptr;
+    long long rows = 0, cells_y = 0;   // launch geometry captured
+    const void* spec = nullptr;        // kernels captured (specialised or generic)
+    hipGraphExec_t exec = nullptr;
+  };
   std::vector<Graph> graphs;       // captured step loops, most recently made last
   // particle filter: ancestry rows [(nsection+1)][pf_n] then the instances [pf_n]
   int* d_sec_ti = nullptr;
@@ -105,6 +113,7 @@ struct beep_ctx {
   // size classes (created on first use), and the event they start from.
   hipStream_t bstream[4] = {};
   hipEvent_t bstart = nullptr;
+  hipEvent_t bend[4] = {};         // end of each batch stream's work
   bool obs_general = false;        // observations other than plain normal ones
   bool obs_pop = false;            // observations counting population elements
   // Observation splines: their values per slot, [nslot][nospl][S] (FP64), and
@@ -114,6 +123,12 @@ struct beep_ctx {
   std::vector<int> inst_slot;
   int* d_inst_slot = nullptr;
   bool inst_slot_dirty = false;
+  // Pinned host staging of beep_set_param: the slot blocks derived in
+  // parallel, uploaded with one copy.
+  float* h_stage = nullptr;
+  size_t h_stage_cap = 0;          // floats
+  double* h_stage64 = nullptr;
+  size_t h_stage64_cap = 0;        // doubles
 };
 
 namespace {
@@ -209,6 +224,24 @@ double elapsed(beep_ctx* c, int k0, int k1) {
 bool valid_inst(const beep_ctx* c, int i) { return i >= 0 && i < c->capacity; }
 bool valid_slot(const beep_ctx* c, int s) { return s >= 0 && s < c->nslot; }
 
+// The instances one call writes must be in range and distinct: each is one
+// trajectory and one state, and two jobs writing the same one would race
+// (and, on the step-wise path, index infectivity rows past the context's
+// capacity).  `read`, when given, lists instances the call only reads (the
+// initial states of MBPs): none of them may be written by the same call.
+int check_instances(beep_ctx* c, int n, const int32_t* written, const int32_t* read, const char* what) {
+  std::vector<char> seen(c->capacity, 0);
+  for (int i = 0; i < n; i++) {
+    if (!valid_inst(c, written[i])) return fail(c, BEEP_EINVAL, "%s: instance %d out of range", what, written[i]);
+    if (seen[written[i]]++) return fail(c, BEEP_EINVAL, "%s: instance %d written twice", what, written[i]);
+  }
+  for (int i = 0; read && i < n; i++) {
+    if (!valid_inst(c, read[i])) return fail(c, BEEP_EINVAL, "%s: instance %d out of range", what, read[i]);
+    if (seen[read[i]]) return fail(c, BEEP_EINVAL, "%s: instance %d is both read and written", what, read[i]);
+  }
+  return BEEP_OK;
+}
+
 int check_device_error(beep_ctx* c) {
   int e = 0;
   HIPCHK(c, hipMemcpyAsync(&e, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -244,6 +277,10 @@ template <int MODE>
 int launch_fused_t(beep_ctx* c, const Job* djobs, int njobs, size_t lds, int block, uint64_t seed, uint32_t stream,
                    hipStream_t st = nullptr) {
   if (njobs == 0) return BEEP_OK;
+  {
+    std::string e = beephost::fused_geometry(njobs, block, lds, 160 * 1024, BEEP_FUSED_MAX_BLOCK);
+    if (!e.empty()) return fail(c, BEEP_EINVAL, "%s", e.c_str());
+  }
   auto kfn = beepdev::k_fused<MODE>;
   if (lds > 65536) HIPCHK(c, hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(kfn, dim3(njobs), dim3(block), lds, st ? st : c->stream, djobs, seed, stream);
@@ -340,6 +377,10 @@ int spec_kernels(beep_ctx* c, SpecKernels** out) {
 int launch_spec(beep_ctx* c, const SpecKernels* k, int mode, const Job* djobs, int njobs, size_t lds, int block,
                 uint64_t seed, uint32_t stream, hipStream_t st = nullptr) {
   if (njobs == 0) return BEEP_OK;
+  {
+    std::string e = beephost::fused_geometry(njobs, block, lds, (size_t)k->max_dyn_lds, BEEP_FUSED_MAX_BLOCK);
+    if (!e.empty()) return fail(c, BEEP_EINVAL, "%s", e.c_str());
+  }
   void* args[3] = {(void*)&djobs, (void*)&seed, (void*)&stream};
   HIPCHK(c, hipModuleLaunchKernel(k->fn[mode], (unsigned)njobs, 1, 1, (unsigned)block, 1, 1, (unsigned)lds,
                                   st ? st : c->stream, args, nullptr));
@@ -353,21 +394,36 @@ int launch_spec(beep_ctx* c, const SpecKernels* k, int mode, const Job* djobs, i
 // read from device memory so one graph serves every call.
 int launch_stepwise(beep_ctx* c, int mode, const Job* djobs, int n, int t0, int t1, uint64_t seed, uint32_t stream) {
   const auto& m = c->m;
+  beephost::StepGeometry geo;
+  {
+    std::string e = beephost::stepwise_geometry(m.A, m.G, c->Apad, n, c->capacity, BEEP_STEP_BLOCK, &geo);
+    if (!e.empty()) return fail(c, BEEP_EINVAL, "%s", e.c_str());
+  }
+  // The job descriptors must lie inside the context's job buffer: the graph
+  // reads them there at every replay.
+  if (!c->d_jobs || djobs < c->d_jobs || djobs + n > c->d_jobs + c->jobs_cap)
+    return fail(c, BEEP_EINVAL, "step-wise launch: job descriptors outside the context's job buffer");
   uint64_t rp[2] = {seed, (uint64_t)stream};
   HIPCHK(c, hipMemcpyAsync(c->d_rngp, rp, sizeof rp, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (const auto& g : c->graphs)
     if (g.mode == mode && g.n == n && g.t0 == t0 && g.t1 == t1 && g.jobs == djobs) {
+      // A cached graph is replayed only with the geometry it was captured with.
+      if (g.rows != geo.rows || g.cells_y != geo.cell.y || g.spec != c->spec)
+        return fail(c, BEEP_EINVAL, "step-wise launch: cached graph does not match the launch geometry");
       HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
       return BEEP_OK;
     }
   if (c->graphs.size() >= 32) {
+    // The oldest graph may still be running on the stream: it is destroyed
+    // only after the stream has drained.
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipGraphExecDestroy(c->graphs.front().exec));
     c->graphs.erase(c->graphs.begin());
   }
   beep_ctx::Graph G;
-  const int rows = n * m.G;
-  dim3 gcell((m.AG + BEEP_STEP_BLOCK - 1) / BEEP_STEP_BLOCK, n);
+  const int rows = (int)geo.rows;
+  const dim3 gcell((unsigned)geo.cell.x, (unsigned)geo.cell.y);
   hipGraph_t graph;
   (void)hipGetLastError();               // a stale error of an earlier call is not this loop's
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -376,11 +432,13 @@ int launch_stepwise(beep_ctx* c, int mode, const Job* djobs, int n, int t0, int 
     if (c->dense) {
       beepdev::DenseMix p{c->d_D, c->d_M, c->d_I, c->h_topo.diag, c->h_topo.fac, djobs, rows, m.A, c->Apad, m.G, s,
                           c->h_topo.off_geo};
-      hipLaunchKernelGGL(beepdev::k_dense_contract, dim3(c->Apad / 64, (rows + 63) / 64), dim3(256), 0, c->stream, p);
+      hipLaunchKernelGGL(beepdev::k_dense_contract, dim3((unsigned)geo.dense.x, (unsigned)geo.dense.y), dim3(256), 0,
+                         c->stream, p);
     } else {
       beepdev::SparseMix p{c->d_D, c->d_I, c->h_topo.diag, c->h_topo.fac, c->h_topo.in_ptr, c->h_topo.in_src,
                            c->h_topo.in_w, djobs, rows, m.A, c->Apad, m.G, s, c->h_topo.off_geo};
-      hipLaunchKernelGGL(beepdev::k_sparse_contract, dim3((m.A + 255) / 256, rows), dim3(256), 0, c->stream, p);
+      hipLaunchKernelGGL(beepdev::k_sparse_contract, dim3((unsigned)geo.sparse.x, (unsigned)geo.sparse.y), dim3(256),
+                         0, c->stream, p);
     }
     if (c->spec) {
       int Apad = c->Apad, ss = s;
@@ -418,6 +476,7 @@ int launch_stepwise(beep_ctx* c, int mode, const Job* djobs, int n, int t0, int 
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return fail(c, BEEP_EHIP, "graph instantiate: %s", hipGetErrorString(e));
   G.mode = mode; G.n = n; G.t0 = t0; G.t1 = t1; G.jobs = djobs;
+  G.rows = geo.rows; G.cells_y = geo.cell.y; G.spec = c->spec;
   c->graphs.push_back(G);
   HIPCHK(c, hipGraphLaunch(G.exec, c->stream));
   return BEEP_OK;
@@ -439,7 +498,7 @@ int launch_fused(beep_ctx* c, int mode, const Job* djobs, int njobs, uint64_t se
 // The observation log-likelihood of n instances over the observations sel
 // (all when null), into out[b * (1 + ndatatable)].
 void launch_obs_loglik(beep_ctx* c, int n, const int* d_inst, const int* sel, int nsel, double* out, int store_EF,
-                       bool want_dt) {
+                       bool want_dt, hipStream_t st = nullptr) {
   using K = decltype(&beepdev::k_obs_loglik<true, true, true, false>);
   // With observation splines only the general variants are used.
   static const K ks[12] = {
@@ -452,8 +511,9 @@ void launch_obs_loglik(beep_ctx* c, int n, const int* d_inst, const int* sel, in
   K k = c->nospl ? ks[8 + (want_dt ? 2 : 0) + (c->obs_pop ? 1 : 0)]
                  : ks[(want_dt ? 4 : 0) + (c->obs_general ? 2 : 0) + (c->obs_pop ? 1 : 0)];
   const int ni = want_dt ? beepdev::OBS_INSTANCES<true> : beepdev::OBS_INSTANCES<false>;
-  hipLaunchKernelGGL(k, dim3((n + ni - 1) / ni), dim3(256), 0, c->stream, c->d_obs, n, d_inst, c->d_tn, c->d_pt,
-                     c->traj_sz, c->pop_traj_sz, sel, nsel, out, c->d_EF, store_EF, c->d_ospl, c->d_inst_slot);
+  hipLaunchKernelGGL(k, dim3((n + ni - 1) / ni), dim3(256), 0, st ? st : c->stream, c->d_obs, n, d_inst, c->d_tn,
+                     c->d_pt, c->traj_sz, c->pop_traj_sz, sel, nsel, out, c->d_EF, store_EF, c->d_ospl,
+                     c->d_inst_slot);
 }
 
 // Before an observation kernel reads inst_slot: the host copy, uploaded when
@@ -542,8 +602,14 @@ int beep_spec_tables(const beep_spec* spec, const double* paramval, double* rate
   if (sus) std::copy(tb.sus.begin(), tb.sus.end(), sus);
   if (inf) std::copy(tb.inf.begin(), tb.inf.end(), inf);
   if (Ntime) std::copy(tb.Ntime.begin(), tb.Ntime.end(), Ntime);
-  if (beta_af) std::copy(tb.beta_af.begin(), tb.beta_af.end(), beta_af);
-  if (efoi) std::copy(tb.efoi.begin(), tb.efoi.end(), efoi);
+  if (beta_af) {
+    std::vector<double> ba = m.beta_af(tb);
+    std::copy(ba.begin(), ba.end(), beta_af);
+  }
+  if (efoi) {
+    std::vector<double> eg = m.efoi_groups(tb);
+    std::copy(eg.begin(), eg.end(), efoi);
+  }
   if (geo) std::copy(tb.geo.begin(), tb.geo.end(), geo);
   if (Pr) *Pr = m.prior_logpdf(paramval);
   if (inb) *inb = m.inbounds(paramval) ? 1 : 0;
@@ -612,9 +678,26 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   tp.off_inf = off; off += m.C;
   tp.N_tv = m.agemod.empty() ? 0 : 1;
   tp.off_N = off; off += m.G * m.G * (tp.N_tv ? m.S : 1);
-  tp.off_efoi = off; off += m.S * m.G;
+  tp.off_efoi = off; off += m.S;
   tp.off_geo = off; off += m.S;
-  tp.off_beta = off; off += m.S * m.A;
+  tp.off_beta = off; off += m.S;
+  // The area factor: in each slot when it depends on the parameters, else one
+  // table for the context.
+  {
+    float* d_agedist = nullptr;
+    if ((rc = upload(c, &d_agedist, to_f32(m.efoi_agedist)))) return rc;
+    tp.agedist = d_agedist;
+  }
+  tp.off_af = -1;
+  tp.af = nullptr;
+  if (m.af_param) {
+    tp.off_af = off;
+    off += m.period * m.A;
+  } else {
+    float* d_af = nullptr;
+    if ((rc = upload(c, &d_af, to_f32(m.af_const)))) return rc;
+    tp.af = d_af;
+  }
   c->slot_stride = ((size_t)off + 63) / 64 * 64;
   tp.from = d_from; tp.to = d_to; tp.diag = d_diag; tp.fac = d_fac;
   tp.in_ptr = d_in_ptr; tp.in_src = d_in_src; tp.in_w = d_in_w; tp.mix_order = d_order; tp.pop_init = d_pop_init;
@@ -795,7 +878,11 @@ int beep_destroy(beep_ctx* c) {
       (void)hipStreamDestroy(st);
     }
   if (c->bstart) (void)hipEventDestroy(c->bstart);
+  for (auto& e : c->bend)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
+  if (c->h_stage64) (void)hipHostFree(c->h_stage64);
   delete c;
   return BEEP_OK;
 }
@@ -816,36 +903,209 @@ int beep_get_dims(const beep_ctx* c, int32_t* d) {
 
 size_t beep_device_bytes(const beep_ctx* c) { return c ? c->allocated : 0; }
 
+}  // extern "C"
+
+namespace {
+
+// The FP32 slot block of derived tables (layout of TopoDev's offsets).
+void fill_slot_block(const beep_ctx* c, const beephost::Tables& tb, float* block) {
+  const TopoDev& tp = c->h_topo;
+  std::fill(block, block + c->slot_stride, 0.f);
+  for (size_t k = 0; k < tb.rate.size(); k++) block[tp.off_rate + k] = (float)tb.rate[k];
+  for (size_t k = 0; k < tb.sus.size(); k++) block[tp.off_sus + k] = (float)tb.sus[k];
+  for (size_t k = 0; k < tb.inf.size(); k++) block[tp.off_inf + k] = (float)tb.inf[k];
+  for (size_t k = 0; k < tb.Ntime.size(); k++) block[tp.off_N + k] = (float)tb.Ntime[k];
+  for (size_t k = 0; k < tb.efoi.size(); k++) block[tp.off_efoi + k] = (float)tb.efoi[k];
+  for (size_t k = 0; k < tb.geo.size(); k++) block[tp.off_geo + k] = (float)tb.geo[k];
+  for (size_t k = 0; k < tb.beta.size(); k++) block[tp.off_beta + k] = (float)tb.beta[k];
+  if (tp.off_af >= 0)
+    for (size_t k = 0; k < tb.af.size(); k++) block[tp.off_af + k] = (float)tb.af[k];
+}
+
+// Pinned host staging of at least nf floats and nd doubles.
+int ensure_host_stage(beep_ctx* c, size_t nf, size_t nd) {
+  if (nf > c->h_stage_cap) {
+    if (c->h_stage) HIPCHK(c, hipHostFree(c->h_stage));
+    c->h_stage = nullptr;
+    c->h_stage_cap = 0;
+    HIPCHK(c, hipHostMalloc((void**)&c->h_stage, nf * sizeof(float), hipHostMallocDefault));
+    c->h_stage_cap = nf;
+  }
+  if (nd > c->h_stage64_cap) {
+    if (c->h_stage64) HIPCHK(c, hipHostFree(c->h_stage64));
+    c->h_stage64 = nullptr;
+    c->h_stage64_cap = 0;
+    HIPCHK(c, hipHostMalloc((void**)&c->h_stage64, nd * sizeof(double), hipHostMallocDefault));
+    c->h_stage64_cap = nd;
+  }
+  return BEEP_OK;
+}
+
+// Runs f(i0, i1) over [0, n) on up to the host's hardware threads, in
+// contiguous chunks of at least `grain` items (inline when one chunk).
+template <class F>
+void parallel_chunks(int n, int grain, F f) {
+  int nt = (int)std::thread::hardware_concurrency();
+  nt = std::max(1, std::min(nt, std::min(64, (n + grain - 1) / grain)));
+  if (nt <= 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; t++) {
+    const int i0 = (int)((long long)n * t / nt), i1 = (int)((long long)n * (t + 1) / nt);
+    th.emplace_back([=, &f] { f(i0, i1); });
+  }
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+// State::set_param (src/state.cc:66-91) for n parameter vectors into their
+// slots.  The derivations (NGM, splines, area factors) run in parallel on the
+// host threads into pinned staging; the slot blocks go to the device with one
+// copy (slots contiguous and ascending) or one copy and a scatter kernel, and
+// one synchronisation.  On any derivation error nothing is uploaded and the
+// first failing vector's error is returned.
 int beep_set_param(beep_ctx* c, int n, const int32_t* slot, const double* paramval, double* Pr_out,
                    int32_t* inb_out) {
   if (!c || n < 0 || (n > 0 && (!slot || !paramval))) return fail(c, BEEP_EINVAL, "beep_set_param: bad arguments");
+  {
+    std::vector<char> seen(c->nslot, 0);
+    for (int i = 0; i < n; i++) {
+      if (!valid_slot(c, slot[i])) return fail(c, BEEP_EINVAL, "slot %d out of range", slot[i]);
+      if (seen[slot[i]]++) return fail(c, BEEP_EINVAL, "slot %d set twice in one call", slot[i]);
+    }
+  }
+  if (n == 0) return BEEP_OK;
   const auto& m = c->m;
-  std::vector<float> block(c->slot_stride);
-  beephost::Tables tb;
-  for (int i = 0; i < n; i++) {
-    if (!valid_slot(c, slot[i])) return fail(c, BEEP_EINVAL, "slot %d out of range", slot[i]);
-    const double* pv = paramval + (size_t)i * m.nparam;
-    if (Pr_out) Pr_out[i] = m.prior_logpdf(pv);
-    if (inb_out) inb_out[i] = m.inbounds(pv) ? 1 : 0;
-    std::string e;
-    int rc = m.derive(pv, tb, e);
-    if (rc) return fail(c, rc, "set_param: %s", e.c_str());
-    const TopoDev& tp = c->h_topo;
-    std::fill(block.begin(), block.end(), 0.f);
-    for (size_t k = 0; k < tb.rate.size(); k++) block[tp.off_rate + k] = (float)tb.rate[k];
-    for (size_t k = 0; k < tb.sus.size(); k++) block[tp.off_sus + k] = (float)tb.sus[k];
-    for (size_t k = 0; k < tb.inf.size(); k++) block[tp.off_inf + k] = (float)tb.inf[k];
-    for (size_t k = 0; k < tb.Ntime.size(); k++) block[tp.off_N + k] = (float)tb.Ntime[k];
-    for (size_t k = 0; k < tb.efoi.size(); k++) block[tp.off_efoi + k] = (float)tb.efoi[k];
-    for (size_t k = 0; k < tb.geo.size(); k++) block[tp.off_geo + k] = (float)tb.geo[k];
-    for (size_t k = 0; k < tb.beta_af.size(); k++) block[tp.off_beta + k] = (float)tb.beta_af[k];
-    HIPCHK(c, hipMemcpyAsync(c->d_tab + (size_t)slot[i] * c->slot_stride, block.data(), block.size() * sizeof(float),
+  const size_t stride = c->slot_stride, ostride = (size_t)c->nospl * m.S;
+  int rc;
+  if ((rc = ensure_host_stage(c, (size_t)n * stride, (size_t)n * ostride))) return rc;
+  float* blk = c->h_stage;
+  double* ob = c->h_stage64;
+  std::vector<int> code(n, 0);
+  std::vector<std::string> msg(n);
+  parallel_chunks(n, 16, [&](int i0, int i1) {
+    beephost::Tables tb;
+    for (int i = i0; i < i1; i++) {
+      const double* pv = paramval + (size_t)i * m.nparam;
+      if (Pr_out) Pr_out[i] = m.prior_logpdf(pv);
+      if (inb_out) inb_out[i] = m.inbounds(pv) ? 1 : 0;
+      code[i] = m.derive(pv, tb, msg[i]);
+      if (code[i]) continue;
+      fill_slot_block(c, tb, blk + (size_t)i * stride);
+      if (ostride) std::copy(tb.obs_spl.begin(), tb.obs_spl.end(), ob + (size_t)i * ostride);
+    }
+  });
+  for (int i = 0; i < n; i++)
+    if (code[i]) return fail(c, code[i], "set_param: %s", msg[i].c_str());
+  bool contiguous = true;
+  for (int i = 1; i < n && contiguous; i++) contiguous = slot[i] == slot[0] + i;
+  if (contiguous) {
+    HIPCHK(c, hipMemcpyAsync(c->d_tab + (size_t)slot[0] * stride, blk, (size_t)n * stride * sizeof(float),
                              hipMemcpyHostToDevice, c->stream));
-    if (c->nospl)
-      HIPCHK(c, hipMemcpyAsync(c->d_ospl + (size_t)slot[i] * c->nospl * m.S, tb.obs_spl.data(),
-                               tb.obs_spl.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (ostride)
+      HIPCHK(c, hipMemcpyAsync(c->d_ospl + (size_t)slot[0] * ostride, ob, (size_t)n * ostride * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+  } else {
+    const size_t fbytes = ((size_t)n * stride * sizeof(float) + 255) / 256 * 256;
+    const size_t obytes = ((size_t)n * ostride * sizeof(double) + 255) / 256 * 256;
+    if ((rc = ensure_scratch(c, fbytes + obytes + (size_t)n * sizeof(int)))) return rc;
+    float* d_blk = (float*)c->d_scratch;
+    double* d_ob = (double*)((char*)c->d_scratch + fbytes);
+    int* d_slot = (int*)((char*)c->d_scratch + fbytes + obytes);
+    HIPCHK(c, hipMemcpyAsync(d_blk, blk, (size_t)n * stride * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_slot, slot, (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const unsigned by = (unsigned)std::min<size_t>(64, (stride + 255) / 256);
+    hipLaunchKernelGGL(beepdev::k_scatter_slots, dim3((unsigned)n, by), dim3(256), 0, c->stream, c->d_tab, stride,
+                       d_blk, d_slot);
+    HIPCHK(c, hipGetLastError());
+    if (ostride) {
+      HIPCHK(c, hipMemcpyAsync(d_ob, ob, (size_t)n * ostride * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      const unsigned oy = (unsigned)std::min<size_t>(64, (ostride + 255) / 256);
+      hipLaunchKernelGGL(beepdev::k_scatter_rows64, dim3((unsigned)n, oy), dim3(256), 0, c->stream, c->d_ospl,
+                         ostride, d_ob, d_slot);
+      HIPCHK(c, hipGetLastError());
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));   // the staging is reused by the next call
+  for (int i = 0; i < n; i++) {
+    const double* pv = paramval + (size_t)i * m.nparam;
     std::copy(pv, pv + m.nparam, c->slot_param.begin() + (size_t)slot[i] * m.nparam);
+    c->slot_set[slot[i]] = 1;
+  }
+  return BEEP_OK;
+}
+
+int beep_batch_set_param(int n, beep_ctx* const* ctx, const int32_t* slot, const double* const* paramval,
+                         double* Pr_out, int32_t* inb_out) {
+  if (n <= 0 || !ctx || !ctx[0] || !slot || !paramval) return BEEP_EINVAL;
+  beep_ctx* c0 = ctx[0];
+  std::vector<size_t> off(n + 1, 0);
+  {
+    std::map<beep_ctx*, std::vector<char>> seen;
+    for (int i = 0; i < n; i++) {
+      beep_ctx* c = ctx[i];
+      if (!c || c->device != c0->device || !paramval[i]) return fail(c0, BEEP_EINVAL, "item %d: bad context or values", i);
+      if (c->nospl) return fail(c0, BEEP_EINVAL, "item %d: a model with observation splines (use beep_set_param)", i);
+      if (!valid_slot(c, slot[i])) return fail(c0, BEEP_EINVAL, "item %d: slot %d out of range", i, slot[i]);
+      auto& sv = seen[c];
+      if (sv.empty()) sv.assign(c->nslot, 0);
+      if (sv[slot[i]]++) return fail(c0, BEEP_EINVAL, "item %d: slot %d set twice in one call", i, slot[i]);
+      off[i + 1] = off[i] + c->slot_stride;
+    }
+  }
+  int rc;
+  const size_t total = off[n];
+  if ((rc = ensure_host_stage(c0, total, 0))) return rc;
+  float* blk = c0->h_stage;
+  std::vector<int> code(n, 0);
+  std::vector<std::string> msg(n);
+  parallel_chunks(n, 16, [&](int i0, int i1) {
+    beephost::Tables tb;
+    for (int i = i0; i < i1; i++) {
+      const auto& m = ctx[i]->m;
+      const double* pv = paramval[i];
+      if (Pr_out) Pr_out[i] = m.prior_logpdf(pv);
+      if (inb_out) inb_out[i] = m.inbounds(pv) ? 1 : 0;
+      code[i] = m.derive(pv, tb, msg[i]);
+      if (!code[i]) fill_slot_block(ctx[i], tb, blk + off[i]);
+    }
+  });
+  for (int i = 0; i < n; i++)
+    if (code[i]) return fail(c0, code[i], "set_param (item %d): %s", i, msg[i].c_str());
+  // Device staging: the packed blocks, then per item the source offset, the
+  // destination slot and the length.
+  const size_t pbytes = (total * sizeof(float) + 255) / 256 * 256;
+  const size_t mbytes = (size_t)n * (8 + 8 + 4);
+  if ((rc = ensure_scratch(c0, pbytes + mbytes))) return rc;
+  std::vector<char> meta(mbytes);
+  uint64_t* hs = (uint64_t*)meta.data();
+  float** hd = (float**)(hs + n);
+  uint32_t* hl = (uint32_t*)(hd + n);
+  for (int i = 0; i < n; i++) {
+    hs[i] = off[i];
+    hd[i] = ctx[i]->d_tab + (size_t)slot[i] * ctx[i]->slot_stride;
+    hl[i] = (uint32_t)ctx[i]->slot_stride;
+  }
+  char* d = (char*)c0->d_scratch;
+  HIPCHK(c0, hipMemcpyAsync(d, blk, total * sizeof(float), hipMemcpyHostToDevice, c0->stream));
+  HIPCHK(c0, hipMemcpyAsync(d + pbytes, meta.data(), mbytes, hipMemcpyHostToDevice, c0->stream));
+  size_t maxlen = 0;
+  for (int i = 0; i < n; i++) maxlen = std::max(maxlen, (size_t)ctx[i]->slot_stride);
+  const unsigned by = (unsigned)std::min<size_t>(64, (maxlen + 255) / 256);
+  hipLaunchKernelGGL(beepdev::k_scatter_blocks, dim3((unsigned)n, by), dim3(256), 0, c0->stream, (const float*)d,
+                     (const uint64_t*)(d + pbytes), (float* const*)(d + pbytes + (size_t)8 * n),
+                     (const uint32_t*)(d + pbytes + (size_t)16 * n));
+  HIPCHK(c0, hipGetLastError());
+  HIPCHK(c0, hipStreamSynchronize(c0->stream));   // staging reused by the next call
+  for (int i = 0; i < n; i++) {
+    beep_ctx* c = ctx[i];
+    std::copy(paramval[i], paramval[i] + c->m.nparam, c->slot_param.begin() + (size_t)slot[i] * c->m.nparam);
     c->slot_set[slot[i]] = 1;
   }
   return BEEP_OK;
@@ -862,8 +1122,14 @@ int beep_get_tables(beep_ctx* c, int slot, double* rate, double* sus, double* in
   if (sus) std::copy(tb.sus.begin(), tb.sus.end(), sus);
   if (inf) std::copy(tb.inf.begin(), tb.inf.end(), inf);
   if (Ntime) std::copy(tb.Ntime.begin(), tb.Ntime.end(), Ntime);
-  if (beta_af) std::copy(tb.beta_af.begin(), tb.beta_af.end(), beta_af);
-  if (efoi) std::copy(tb.efoi.begin(), tb.efoi.end(), efoi);
+  if (beta_af) {
+    std::vector<double> ba = c->m.beta_af(tb);
+    std::copy(ba.begin(), ba.end(), beta_af);
+  }
+  if (efoi) {
+    std::vector<double> eg = c->m.efoi_groups(tb);
+    std::copy(eg.begin(), eg.end(), efoi);
+  }
   if (geo) std::copy(tb.geo.begin(), tb.geo.end(), geo);
   return BEEP_OK;
 }
@@ -874,12 +1140,11 @@ int beep_simulate(beep_ctx* c, int n, const int32_t* inst, const int32_t* slot, 
   if (n < 0 || (n > 0 && (!inst || !slot || !key))) return fail(c, BEEP_EINVAL, "beep_simulate: bad arguments");
   if (t0 < 0 || t1 > c->m.S || t0 > t1) return fail(c, BEEP_EINVAL, "beep_simulate: bad step range [%d,%d)", t0, t1);
   if (stream > 0xFFFF) return fail(c, BEEP_EINVAL, "stream must be below 65536");
-  for (int i = 0; i < n; i++) {
-    if (!valid_inst(c, inst[i])) return fail(c, BEEP_EINVAL, "instance %d out of range", inst[i]);
-    if (!valid_slot(c, slot[i]) || !c->slot_set[slot[i]]) return fail(c, BEEP_EINVAL, "slot %d not set", slot[i]);
-  }
-  if (n == 0 || t0 == t1) return BEEP_OK;
   int rc;
+  if ((rc = check_instances(c, n, inst, nullptr, "beep_simulate"))) return rc;
+  for (int i = 0; i < n; i++)
+    if (!valid_slot(c, slot[i]) || !c->slot_set[slot[i]]) return fail(c, BEEP_EINVAL, "slot %d not set", slot[i]);
+  if (n == 0 || t0 == t1) return BEEP_OK;
   if (flags & BEEP_SIM_PLAIN) {
     // The unbatched path: one launch per instance per step.
     std::vector<Job> jobs;
@@ -924,14 +1189,14 @@ int beep_mbp(beep_ctx* c, int n, const int32_t* init, const int32_t* prop, const
   if (c->m.nstrain > 1) return fail(c, BEEP_EINVAL, "MBP proposals are not supported for models with strains");
   if (n < 0 || (n > 0 && (!init || !prop || !slot || !key))) return fail(c, BEEP_EINVAL, "beep_mbp: bad arguments");
   if (stream > 0xFFFF) return fail(c, BEEP_EINVAL, "stream must be below 65536");
-  for (int i = 0; i < n; i++) {
-    if (!valid_inst(c, init[i]) || !valid_inst(c, prop[i]) || init[i] == prop[i])
-      return fail(c, BEEP_EINVAL, "bad instance pair (%d,%d)", init[i], prop[i]);
+  int rc;
+  // Proposals are distinct, and no proposal overwrites an initial state
+  // another proposal of the call couples to.
+  if ((rc = check_instances(c, n, prop, init, "beep_mbp"))) return rc;
+  for (int i = 0; i < n; i++)
     if (!valid_slot(c, slot[i]) || !c->slot_set[slot[i]]) return fail(c, BEEP_EINVAL, "slot %d not set", slot[i]);
-  }
   if (n == 0) return BEEP_OK;
   const auto& m = c->m;
-  int rc;
   uint8_t* d_mask = nullptr;
   size_t mask_sz = (size_t)m.S * m.A;
   if (simu_mask) {
@@ -1238,8 +1503,8 @@ int beep_derive_slots(beep_ctx* c, int n, const int32_t* dst, const int32_t* src
   HIPCHK(c, hipMemcpyAsync(d_from, from_step, n * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(d_f, f.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
   int bx = (int)std::min<size_t>(64, (c->slot_stride + 255) / 256);
-  hipLaunchKernelGGL(beepdev::k_derive_slot, dim3(bx, n), dim3(256), 0, c->stream, c->d_tab, c->slot_stride, d_dst,
-                     d_src, d_f, d_from, c->h_topo.off_beta, c->m.A);
+  hipLaunchKernelGGL(beepdev::k_derive_slot, dim3(n, bx), dim3(256), 0, c->stream, c->d_tab, c->slot_stride, d_dst,
+                     d_src, d_f, d_from, c->h_topo.off_beta, c->m.S);
   HIPCHK(c, hipGetLastError());
   const size_t osz = (size_t)c->nospl * c->m.S;     // observation splines: the source slot's
   for (int i = 0; c->nospl && i < n; i++)
@@ -1547,25 +1812,149 @@ int pow2_ceil(int x) {
   while (p < x) p <<= 1;
   return p;
 }
+
+// The streams a batch launched from c0 spreads its work over (made once).
+int ensure_bstreams(beep_ctx* c0) {
+  if (c0->bstart) return BEEP_OK;
+  HIPCHK(c0, hipEventCreateWithFlags(&c0->bstart, hipEventDisableTiming));
+  for (auto& e : c0->bend) HIPCHK(c0, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (auto& st : c0->bstream) HIPCHK(c0, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  return BEEP_OK;
+}
+
+// ObservationModel::calculate (src/obsmodel.cc:22-109) of every item's
+// instance after a batch: per context one observation-kernel launch over its
+// items' instances, the launches spread over the batch streams, one upload of
+// the instance lists and one download of the sums.  Each instance's EF is
+// kept for the acceptance steps.
+int batch_loglik(beep_ctx* c0, int n, const beep_work* w, double* L_out) {
+  std::vector<beep_ctx*> ctxs;
+  std::map<beep_ctx*, std::vector<int>> items;
+  for (int i = 0; i < n; i++) {
+    auto it = items.find(w[i].ctx);
+    if (it == items.end()) {
+      ctxs.push_back(w[i].ctx);
+      items[w[i].ctx].push_back(i);
+    } else {
+      it->second.push_back(i);
+    }
+  }
+  std::vector<int> hinst;
+  hinst.reserve(n);
+  std::vector<size_t> ioff, ooff;
+  size_t nout = 0;
+  for (beep_ctx* c : ctxs) {
+    if (!c->d_tn) return fail(c0, BEEP_EINVAL, "BEEP_BATCH_LOGLIK: a context does not keep transnum");
+    ioff.push_back(hinst.size());
+    ooff.push_back(nout);
+    for (int i : items[c]) hinst.push_back(w[i].inst);
+    nout += items[c].size() * (size_t)(1 + c->m.ndatatable);
+  }
+  const size_t obytes = (nout * sizeof(double) + 255) / 256 * 256;
+  int rc;
+  if ((rc = ensure_scratch(c0, obytes + (size_t)n * sizeof(int)))) return rc;
+  double* d_out = (double*)c0->d_scratch;
+  int* d_inst = (int*)((char*)c0->d_scratch + obytes);
+  HIPCHK(c0, hipMemcpyAsync(d_inst, hinst.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, c0->stream));
+  HIPCHK(c0, hipStreamSynchronize(c0->stream));
+  if ((rc = ensure_bstreams(c0))) return rc;
+  const int nbs = (int)(sizeof c0->bstream / sizeof c0->bstream[0]);
+  for (size_t k = 0; k < ctxs.size(); k++)        // splines' slot lists first (synchronous copies)
+    if (ctxs[k]->nospl) {
+      for (int i : items[ctxs[k]])
+        if (ctxs[k]->inst_slot[w[i].inst] < 0)
+          return fail(c0, BEEP_EINVAL, "work %d: no parameter slot for the observation splines", i);
+      if ((rc = sync_inst_slot(ctxs[k]))) {
+        c0->err = ctxs[k]->err;
+        return rc;
+      }
+    }
+  // Timed on the context stream from the start of the batch streams to the
+  // end of their last launch (timing[1]).
+  record(c0, 2);
+  HIPCHK(c0, hipEventRecord(c0->bstart, c0->stream));
+  for (auto& st : c0->bstream) HIPCHK(c0, hipStreamWaitEvent(st, c0->bstart, 0));
+  for (size_t k = 0; k < ctxs.size(); k++) {
+    beep_ctx* c = ctxs[k];
+    launch_obs_loglik(c, (int)items[c].size(), d_inst + ioff[k], nullptr, (int)c->m.obs.size(), d_out + ooff[k], 1,
+                      false, c0->bstream[k % nbs]);
+    HIPCHK(c0, hipGetLastError());
+  }
+  for (int k = 0; k < nbs; k++) {
+    HIPCHK(c0, hipEventRecord(c0->bend[k], c0->bstream[k]));
+    HIPCHK(c0, hipStreamWaitEvent(c0->stream, c0->bend[k], 0));
+  }
+  record(c0, 3);
+  HIPCHK(c0, hipStreamSynchronize(c0->stream));
+  c0->timing[1] = elapsed(c0, 2, 3);
+  std::vector<double> h(nout);
+  HIPCHK(c0, hipMemcpy(h.data(), d_out, nout * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < ctxs.size(); k++) {
+    beep_ctx* c = ctxs[k];
+    const auto& its = items[c];
+    for (size_t j = 0; j < its.size(); j++) {
+      const double L = h[ooff[k] + j * (size_t)(1 + c->m.ndatatable)];
+      if (std::isnan(L)) return fail(c0, BEEP_ENAN, "NaN observation log-likelihood (work %d)", its[j]);
+      L_out[its[j]] = L;
+    }
+  }
+  return BEEP_OK;
+}
 }  // namespace
 
-int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, int flags, double* stats) {
+int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, int flags, double* L_out,
+                   double* stats) {
   if (n <= 0 || !w || !w[0].ctx) return BEEP_EINVAL;
-  if (stream > 0xFFFF) return BEEP_EINVAL;
   beep_ctx* c0 = w[0].ctx;
+  if (stream > 0xFFFF) return fail(c0, BEEP_EINVAL, "stream must be below 65536");
+  if ((flags & BEEP_BATCH_LOGLIK) && !L_out) return fail(c0, BEEP_EINVAL, "BEEP_BATCH_LOGLIK needs L_out");
   auto t_start = std::chrono::steady_clock::now();
   // Validate and build jobs.
   struct Item { Job job; beep_ctx* ctx; size_t lds; double cost; int AG; SpecKernels* spec; };
   std::vector<Item> fused;
-  std::map<beep_ctx*, std::vector<Job>> step_sim, step_mbp;
   double cellsteps = 0;
+  // Per context: the instances the batch writes (distinct) and the initial
+  // states its MBPs read (none of them written by the batch).
+  {
+    std::map<beep_ctx*, std::pair<std::vector<int32_t>, std::vector<int32_t>>> wr;
+    for (int i = 0; i < n; i++) {
+      beep_ctx* c = w[i].ctx;
+      if (!c) return fail(c0, BEEP_EINVAL, "work %d has no context", i);
+      if (c->device != c0->device) return fail(c0, BEEP_EINVAL, "batch contexts must share one device");
+      // Step-wise contexts (dense mixing, or an instance beyond LDS) are not
+      // batched: their work goes through beep_simulate / beep_mbp.
+      if (c->stepwise)
+        return fail(c0, BEEP_EINVAL, "work %d: a step-wise context (dense mixing or an instance beyond LDS) cannot be "
+                                     "batched; call beep_simulate / beep_mbp on it", i);
+      auto& e = wr[c];
+      e.first.push_back(w[i].inst);
+      if (w[i].kind == BEEP_WORK_MBP) e.second.push_back(w[i].init);
+    }
+    for (auto& kv : wr) {
+      beep_ctx* c = kv.first;
+      const auto& written = kv.second.first;
+      const auto& read = kv.second.second;
+      int rc0 = check_instances(c, (int)written.size(), written.data(), nullptr, "beep_run_batch");
+      if (!rc0) {
+        std::vector<char> isw(c->capacity, 0);
+        for (int32_t x : written) isw[x] = 1;
+        for (int32_t x : read)
+          if (!valid_inst(c, x) || isw[x]) {
+            rc0 = fail(c, BEEP_EINVAL, "beep_run_batch: MBP initial instance %d out of range or written by the batch", x);
+            break;
+          }
+      }
+      if (rc0) {
+        if (c != c0) c0->err = c->err;
+        return rc0;
+      }
+    }
+  }
   for (int i = 0; i < n; i++) {
     beep_ctx* c = w[i].ctx;
-    if (!c) return fail(c0, BEEP_EINVAL, "work %d has no context", i);
-    if (c->device != c0->device) return fail(c0, BEEP_EINVAL, "batch contexts must share one device");
     const auto& m = c->m;
-    if (!valid_inst(c, w[i].inst) || !valid_slot(c, w[i].slot) || !c->slot_set[w[i].slot])
-      return fail(c0, BEEP_EINVAL, "work %d: bad instance or slot", i);
+    if (!valid_slot(c, w[i].slot) || !c->slot_set[w[i].slot])
+      return fail(c0, BEEP_EINVAL, "work %d: bad slot", i);
     Job j;
     if (w[i].kind == BEEP_WORK_SIMULATE) {
       if (w[i].t0 < 0 || w[i].t1 > m.S || w[i].t0 > w[i].t1) return fail(c0, BEEP_EINVAL, "work %d: bad steps", i);
@@ -1582,37 +1971,15 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
       return fail(c0, BEEP_EINVAL, "work %d: unknown kind", i);
     }
     cellsteps += (double)m.AG * (j.t1 - j.t0);
+    j.err = c0->d_err;                       // one error word for the whole batch
     if (j.t0 == j.t1) continue;
-    if (c->stepwise) {
-      (j.mode == beepdev::MODE_MBP ? step_mbp : step_sim)[c].push_back(j);
-    } else {
-      SpecKernels* sk = c->stepwise ? nullptr : c->spec;
-      size_t lds = sk ? c->spec_lds : beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, c->h_topo.nnz, c->h_topo.stage_mix);
-      fused.push_back({j, c, lds, (double)m.AG * m.T * (j.t1 - j.t0), m.AG, sk});
-    }
+    SpecKernels* sk = c->spec;
+    size_t lds = sk ? c->spec_lds : beepdev::fused_lds_bytes(m.A, m.G, m.C, m.T, c->h_topo.nnz, c->h_topo.stage_mix);
+    fused.push_back({j, c, lds, (double)m.AG * m.T * (j.t1 - j.t0), m.AG, sk});
   }
   int rc;
   int nlaunch = 0;
-  // Step-wise contexts first, each on its own stream so they overlap with one
-  // another and with the fused launches below.
-  int nstep = 0;
-  for (int pass = 0; pass < 2; pass++) {
-    auto& mp = pass == 0 ? step_sim : step_mbp;
-    for (auto& kv : mp) {
-      beep_ctx* c = kv.first;
-      auto& jobs = kv.second;
-      int t0 = jobs[0].t0, t1 = jobs[0].t1;
-      for (auto& j : jobs) { t0 = std::min(t0, j.t0); t1 = std::max(t1, j.t1); }
-      if (pass == 1 && step_sim.count(c)) HIPCHK(c0, hipStreamSynchronize(c->stream));   // job buffer reuse
-      if ((rc = put_jobs(c, jobs)) ||
-          (rc = launch_stepwise(c, pass == 0 ? beepdev::MODE_SIM : beepdev::MODE_MBP, c->d_jobs, (int)jobs.size(), t0,
-                                t1, seed, stream))) {
-        if (c != c0) c0->err = "step-wise context: " + c->err;   // the message goes back through the first context
-        return rc;
-      }
-      nstep++;
-    }
-  }
+  const int nstep = 0;
   if (!fused.empty()) {
     // Size classes (and kernel: one specialised kernel per compartment
     // structure), each sorted longest first.
@@ -1620,15 +1987,20 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
     for (auto& it : fused)
       groups[{{pow2_ceil(it.AG), it.ctx->m.C, it.ctx->m.T, it.spec ? 1 : 0}, it.spec}].push_back(it);
     std::vector<Job> all;
-    // key, offset, count, lds, maxAG, kernel
-    std::vector<std::tuple<TuneKey, size_t, size_t, size_t, int, SpecKernels*>> launches;
+    // key, offset, count, lds, maxAG, kernel, every job starts from the initial state
+    std::vector<std::tuple<TuneKey, size_t, size_t, size_t, int, SpecKernels*, bool>> launches;
     for (auto& kv : groups) {
       auto& v = kv.second;
       std::stable_sort(v.begin(), v.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
       size_t lds = 0;
       int maxAG = 0;
-      for (auto& it : v) { lds = std::max(lds, it.lds); maxAG = std::max(maxAG, it.AG); }
-      launches.emplace_back(kv.first.first, all.size(), v.size(), lds, maxAG, kv.first.second);
+      bool rerunnable = true;
+      for (auto& it : v) {
+        lds = std::max(lds, it.lds);
+        maxAG = std::max(maxAG, it.AG);
+        rerunnable = rerunnable && it.job.from_init;
+      }
+      launches.emplace_back(kv.first.first, all.size(), v.size(), lds, maxAG, kv.first.second, rerunnable);
       for (auto& it : v) all.push_back(it.job);
     }
     if (all.size() > c0->bjobs_cap) {
@@ -1650,10 +2022,10 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
     // device idle; each stream starts after the job upload.  (Autotuning
     // launches run alone on the context stream to be timed.)
     const int nbs = (int)(sizeof c0->bstream / sizeof c0->bstream[0]);
-    if (!c0->bstart) {
-      HIPCHK(c0, hipEventCreateWithFlags(&c0->bstart, hipEventDisableTiming));
-      for (auto& st : c0->bstream) HIPCHK(c0, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    }
+    if ((rc = ensure_bstreams(c0))) return rc;
+    // The fused phase is timed on the context stream from before the batch
+    // streams start to after they have all ended (timing[0]).
+    record(c0, 0);
     HIPCHK(c0, hipEventRecord(c0->bstart, c0->stream));
     for (auto& st : c0->bstream) HIPCHK(c0, hipStreamWaitEvent(st, c0->bstart, 0));
     int next_stream = 0;
@@ -1665,6 +2037,7 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
       size_t off = std::get<1>(L), cnt = std::get<2>(L), lds = std::get<3>(L);
       int maxAG = std::get<4>(L);
       SpecKernels* sk = std::get<5>(L);
+      const bool rerunnable = std::get<6>(L);
       auto launch_any = [&](int blk, hipStream_t st) {
         return sk ? launch_spec(c0, sk, beepdev::MODE_ANY, c0->d_bjobs + off, (int)cnt, lds, blk, seed, stream, st)
                   : launch_fused_t<beepdev::MODE_ANY>(c0, c0->d_bjobs + off, (int)cnt, lds, blk, seed, stream, st);
@@ -1672,9 +2045,14 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
       int block = sk ? spec_block(maxAG) : fused_block(maxAG);
       auto f = g_tune.find(key);
       if (f != g_tune.end()) block = f->second;
-      else if (flags & BEEP_BATCH_AUTOTUNE) {
-        // Same keys give the same trajectories whatever the workgroup size, so
-        // every candidate run leaves valid results; the fastest is kept.
+      else if ((flags & BEEP_BATCH_AUTOTUNE) && rerunnable) {
+        // Same keys give the same trajectories whatever the workgroup size,
+        // and every job of the class starts from the initial state (t0 = 0:
+        // simulations from the start, MBPs), so each candidate run recomputes
+        // the same results from the same inputs; the fastest is kept.  A class
+        // with a job that continues from its current state (t0 > 0) would
+        // advance that state once per candidate: it is not tuned here and
+        // runs once with the default size.
         float best = 1e30f;
         for (int cand = 64; cand <= BEEP_FUSED_MAX_BLOCK; cand *= 2) {
           if (cand > pow2_ceil(fused_block(maxAG))) break;
@@ -1707,21 +2085,23 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
       nlaunch++;
     }
     if (tuned_any) tune_save();
-    for (auto& st : c0->bstream) HIPCHK(c0, hipStreamSynchronize(st));
-    HIPCHK(c0, hipStreamSynchronize(c0->stream));   // `all` is the source of an async copy
-  }
-  for (auto& kv : step_sim) HIPCHK(c0, hipStreamSynchronize(kv.first->stream));
-  for (auto& kv : step_mbp) HIPCHK(c0, hipStreamSynchronize(kv.first->stream));
-  HIPCHK(c0, hipStreamSynchronize(c0->stream));   // also keeps `all` alive until its copy is done
-  // Errors raised by any context's kernels.
-  std::map<beep_ctx*, int> seen;
-  for (int i = 0; i < n; i++) {
-    if (seen.count(w[i].ctx)) continue;
-    seen[w[i].ctx] = 1;
-    if ((rc = check_device_error(w[i].ctx))) {
-      if (w[i].ctx != c0) c0->err = w[i].ctx->err;
-      return rc;
+    // Join: the context stream waits for every batch stream, then records the
+    // end of the fused phase.
+    for (int k = 0; k < nbs; k++) {
+      HIPCHK(c0, hipEventRecord(c0->bend[k], c0->bstream[k]));
+      HIPCHK(c0, hipStreamWaitEvent(c0->stream, c0->bend[k], 0));
     }
+    record(c0, 1);
+    HIPCHK(c0, hipStreamSynchronize(c0->stream));   // `all` is the source of an async copy
+    c0->timing[0] = (flags & BEEP_BATCH_AUTOTUNE) ? -1 : elapsed(c0, 0, 1);
+    c0->timing[3] = nlaunch;
+  }
+  HIPCHK(c0, hipStreamSynchronize(c0->stream));   // also keeps `all` alive until its copy is done
+  // Errors raised by the batch's kernels: every job reported into the first
+  // context's error word.
+  if ((rc = check_device_error(c0))) return rc;
+  if (flags & BEEP_BATCH_LOGLIK) {
+    if ((rc = batch_loglik(c0, n, w, L_out))) return rc;
   }
   if (stats) {
     stats[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -1729,6 +2109,97 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
     stats[2] = nstep;
     stats[3] = cellsteps;
   }
+  return BEEP_OK;
+}
+
+int beep_batch_accept(int n, const beep_work* w, int mode, const double* invT, double EFcut, const double* dlogq,
+                      uint64_t seed, uint32_t stream, double* al_out, uint8_t* accepted) {
+  if (n <= 0 || !w || !w[0].ctx) return BEEP_EINVAL;
+  beep_ctx* c0 = w[0].ctx;
+  if (!invT || !dlogq || !al_out || !accepted || (mode != 0 && mode != 1) || stream > 0xFFFF)
+    return fail(c0, BEEP_EINVAL, "beep_batch_accept: bad arguments");
+  // Packed upload: EF pointers of the proposed and initial instances, the
+  // temperatures, the log proposal ratios and the keys.
+  std::vector<char> h((size_t)n * (8 + 8 + 8 + 8 + 4));
+  const double** efp = (const double**)h.data();
+  const double** efi = efp + n;
+  double* hT = (double*)(efi + n);
+  double* hq = hT + n;
+  uint32_t* hk = (uint32_t*)(hq + n);
+  for (int i = 0; i < n; i++) {
+    beep_ctx* c = w[i].ctx;
+    if (!c || c->device != c0->device) return fail(c0, BEEP_EINVAL, "work %d: no context or another device", i);
+    if (w[i].kind != BEEP_WORK_MBP || !valid_inst(c, w[i].inst) || !valid_inst(c, w[i].init))
+      return fail(c0, BEEP_EINVAL, "work %d: not an MBP proposal with valid instances", i);
+    efp[i] = c->d_EF + w[i].inst;
+    efi[i] = c->d_EF + w[i].init;
+    hT[i] = invT[i];
+    hq[i] = dlogq[i];
+    hk[i] = w[i].key;
+  }
+  int rc;
+  const size_t inb = (h.size() + 255) / 256 * 256;
+  if ((rc = ensure_scratch(c0, inb + (size_t)n * 9))) return rc;
+  char* d = (char*)c0->d_scratch;
+  double* d_al = (double*)(d + inb);
+  uint8_t* d_acc = (uint8_t*)(d_al + n);
+  HIPCHK(c0, hipMemcpyAsync(d, h.data(), h.size(), hipMemcpyHostToDevice, c0->stream));
+  hipLaunchKernelGGL(beepdev::k_accept_ptrs, dim3((n + 255) / 256), dim3(256), 0, c0->stream, n,
+                     (const double* const*)d, (const double* const*)(d + (size_t)8 * n), (const double*)(d + (size_t)16 * n),
+                     (const double*)(d + (size_t)24 * n), (const uint32_t*)(d + (size_t)32 * n), mode, EFcut, seed, stream,
+                     d_al, d_acc);
+  HIPCHK(c0, hipGetLastError());
+  HIPCHK(c0, hipMemcpyAsync(al_out, d_al, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c0->stream));
+  HIPCHK(c0, hipMemcpyAsync(accepted, d_acc, (size_t)n, hipMemcpyDeviceToHost, c0->stream));
+  HIPCHK(c0, hipStreamSynchronize(c0->stream));
+  for (int i = 0; i < n; i++)
+    if (std::isnan(al_out[i])) return fail(c0, BEEP_ENAN, "NaN acceptance probability (work %d)", i);   // src/mbp.cc:353
+  return BEEP_OK;
+}
+
+int beep_batch_mc3_swap(int nladder, const int32_t* ladder_ptr, beep_ctx* const* ctx, const int32_t* inst,
+                        const int32_t* run, const double* invT, uint64_t seed, uint32_t stream, int32_t* order_out,
+                        int32_t* ntr_out, int32_t* nac_out) {
+  if (nladder <= 0 || !ladder_ptr || !ctx || !ctx[0] || !inst || !run || !invT || !order_out || !ntr_out || !nac_out)
+    return BEEP_EINVAL;
+  beep_ctx* c0 = ctx[0];
+  if (stream > 0xFFFF || ladder_ptr[0] != 0) return fail(c0, BEEP_EINVAL, "beep_batch_mc3_swap: bad arguments");
+  for (int l = 0; l < nladder; l++)
+    if (ladder_ptr[l + 1] <= ladder_ptr[l]) return fail(c0, BEEP_EINVAL, "ladder %d is empty or out of order", l);
+  const int n = ladder_ptr[nladder];
+  // Packed upload: ladder offsets, EF pointers, temperatures, run labels.
+  std::vector<char> h((size_t)(nladder + 1) * 4 + (size_t)n * (8 + 8 + 4) + 16);
+  int* hp = (int*)h.data();
+  const size_t off_ef = ((size_t)(nladder + 1) * 4 + 7) / 8 * 8;
+  const double** hef = (const double**)(h.data() + off_ef);
+  double* hT = (double*)(hef + n);
+  int* hr = (int*)(hT + n);
+  for (int l = 0; l <= nladder; l++) hp[l] = ladder_ptr[l];
+  for (int i = 0; i < n; i++) {
+    beep_ctx* c = ctx[i];
+    if (!c || c->device != c0->device || !valid_inst(c, inst[i]))
+      return fail(c0, BEEP_EINVAL, "chain %d: no context, another device or instance out of range", i);
+    hef[i] = c->d_EF + inst[i];
+    hT[i] = invT[i];
+    hr[i] = run[i];
+  }
+  int rc;
+  const size_t inb = (h.size() + 255) / 256 * 256;
+  if ((rc = ensure_scratch(c0, inb + (size_t)n * (8 + 3 * 4)))) return rc;
+  char* d = (char*)c0->d_scratch;
+  double* d_EF = (double*)(d + inb);
+  int* d_order = (int*)(d_EF + n);
+  int* d_ntr = d_order + n;
+  int* d_nac = d_ntr + n;
+  HIPCHK(c0, hipMemcpyAsync(d, h.data(), h.size(), hipMemcpyHostToDevice, c0->stream));
+  hipLaunchKernelGGL(beepdev::k_mc3_swap_ladders, dim3((nladder + 63) / 64), dim3(64), 0, c0->stream, nladder,
+                     (const int*)d, (const int*)(d + off_ef + (size_t)16 * n), (const double*)(d + off_ef + (size_t)8 * n),
+                     (const double* const*)(d + off_ef), d_EF, d_order, d_ntr, d_nac, seed, stream);
+  HIPCHK(c0, hipGetLastError());
+  HIPCHK(c0, hipMemcpyAsync(order_out, d_order, (size_t)n * 4, hipMemcpyDeviceToHost, c0->stream));
+  HIPCHK(c0, hipMemcpyAsync(ntr_out, d_ntr, (size_t)n * 4, hipMemcpyDeviceToHost, c0->stream));
+  HIPCHK(c0, hipMemcpyAsync(nac_out, d_nac, (size_t)n * 4, hipMemcpyDeviceToHost, c0->stream));
+  HIPCHK(c0, hipStreamSynchronize(c0->stream));
   return BEEP_OK;
 }
 
@@ -1766,6 +2237,32 @@ int beep_debug_gemm(int device, int rows, int K, const float* A, const float* B,
                      hipLaunchKernelGGL(beepdev::k_dbg_gemm, dim3(K / 64, (rows + 63) / 64), dim3(256), 0, 0,
                                         (const float*)a, (const float*)b, (float*)o, rows, K);
                    });
+}
+
+int beep_debug_check_launch(int kind, const int64_t* args, int nargs, int64_t* grid_out, char* err, int errlen) {
+  std::string e;
+  if (!args) {
+    e = "no arguments";
+  } else if (kind == BEEP_CHECK_FUSED && nargs == 4) {
+    beephost::Dim g;
+    e = beephost::fused_geometry(args[0], (int)args[1], (size_t)args[2], (size_t)args[3], BEEP_FUSED_MAX_BLOCK, &g);
+    if (e.empty() && grid_out) grid_out[0] = g.x;
+  } else if (kind == BEEP_CHECK_STEPWISE && nargs == 5) {
+    beephost::StepGeometry g;
+    e = beephost::stepwise_geometry((int)args[0], (int)args[1], (int)args[2], args[3], args[4], BEEP_STEP_BLOCK, &g);
+    if (e.empty() && grid_out) {
+      const long long v[6] = {g.cell.x, g.cell.y, g.dense.x, g.dense.y, g.sparse.x, g.sparse.y};
+      for (int k = 0; k < 6; k++) grid_out[k] = v[k];
+    }
+  } else {
+    e = "unknown check or wrong argument count";
+  }
+  if (e.empty()) return BEEP_OK;
+  if (err && errlen > 0) {
+    std::strncpy(err, e.c_str(), errlen - 1);
+    err[errlen - 1] = 0;
+  }
+  return BEEP_EINVAL;
 }
 
 int beep_debug_binomial(int device, int n, const int32_t* nn, const double* p, uint64_t seed, uint32_t key,

@@ -28,6 +28,11 @@ struct TopoDev {
   const float* in_w;     // [nnz]
   const int* mix_order;  // [A] areas by number of incoming entries, most first
   const int* pop_init;   // [C][AG]
+  // Area factor [period][A]: in each parameter slot at off_af when it depends
+  // on the parameters (off_af >= 0), else this context-wide table.
+  const float* af;
+  int off_af;
+  const float* agedist;  // [G] age distribution of the external force of infection (efoi[S] per slot)
 };
 
 // One instance's work in a launch.
@@ -108,6 +113,26 @@ __device__ inline void mix_cells(F* diag, F* fac, I* in_ptr, I* in_src, F* in_w,
     for (; e < e1; e++) imp += in_w[e] * sdinf[in_src[e] * G + g];
     sI[cell] = (idg + imp) * m + fac[j] * idg * (1.f - m);
   }
+}
+
+// The area-factor row of step s (day s / steps per day) for a slot's tables:
+// the slot's own when the factor depends on the parameters, else the
+// context's.  The infection mean uses beta[s] * af[day][area]
+// (State::set_transmean, src/state.cc:410-415).
+__device__ inline gcfloat* af_row(ctopo* tp, gcfloat* tab, int s) {
+  const int day = s / tp->spd;
+  gcfloat* base = tp->off_af >= 0 ? tab + tp->off_af : gptr(tp->af);
+  return base + (size_t)day * tp->A;
+}
+
+// The force of infection per susceptible, beta[s] af[day][area] NMI +
+// efoi[s] agedist[group] (State::set_transmean, src/state.cc:410-424), with
+// its roundings fixed (two rounded products, one fused multiply-add) so that
+// every kernel and compiler computes the same FP32 value.
+__device__ inline float foi_mean(float beta, float af, float nmi, float efoi, float agedist) {
+  const float b = beta * af;
+  const float e = efoi * agedist;
+  return __builtin_fmaf(b, nmi, e);
 }
 
 // Queue capacity of large draws per step (items beyond it are drawn in place).

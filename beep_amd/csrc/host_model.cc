@@ -400,6 +400,11 @@ std::string HostModel::build(const beep_spec& s) {
       if (obs[i].sett_i < tf && obs[i].sett_f >= ti && obs[i].sett_i >= ti && obs[i].sett_f <= tf)
         section_obs[k].push_back(i);
   }
+  // The area factor depends on the parameters only through area effects,
+  // covariates and level effects; without them it is one table for all.
+  af_param = !area_effect_param.empty() || !covar.empty() || !level_param.empty();
+  af_const.clear();
+  if (!af_param) af_const = area_factor(std::vector<double>(nparam, 0.0));
   return "";
 }
 
@@ -484,34 +489,13 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
     }
   }
 
-  // Model::create_areafactor (src/model.cc:1182-1250).
-  std::vector<double> af((size_t)period * A, 1.0);
-  if (!area_effect_param.empty())
-    for (int t = 0; t < period; t++)
-      for (int c = 0; c < A; c++) af[(size_t)t * A + c] = pd[area_effect_param[c]];
-  for (const auto& cv : covar) {
-    double b = pd[cv.param];
-    for (int t = 0; t < period; t++)
-      for (int c = 0; c < A; c++) af[(size_t)t * A + c] *= std::exp(b * cv.value[(size_t)c * period + t]);
-  }
-  if (!level_param.empty())
-    for (size_t i = 0; i < af.size(); i++) af[i] *= pd[level_param[level_map[i]]];
-  {
-    double area_av = 0;
-    for (int c = 0; c < A; c++) {
-      double av = 0;
-      for (int t = 0; t < period; t++) av += af[(size_t)t * A + c];
-      av /= period;
-      area_av += av * area_pop[c];
-    }
-    area_av /= popsize;
-    for (auto& v : af) v /= area_av;
-  }
+  if (af_param) tb.af = area_factor(pd);
+  else tb.af.clear();
 
   // Model::calculate_beta_from_R (src/model.cc:1318-1362) with calculate_Vinv,
   // calculate_F, calculate_NGM and the dominant eigenvalue (2485-2621).
   if (nstrain > 1) {
-    int rc = strain_beta(pd, af, tb, err);
+    int rc = strain_beta(pd, tb, err);
     if (rc) return rc;
   } else {
     const int nis = (int)inf_state.size();
@@ -572,18 +556,13 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
     double Rfac = Rfactor_param >= 0 ? pd[Rfactor_param] : 1.0;
     std::vector<double> Rt = disc_spline(R_spline, pd);
 
-    tb.beta_af.assign((size_t)S * A, 0.0);
-    for (int s = 0; s < S; s++) {
-      double beta = Rfac * Rt[s] / ratio[tv ? s : 0];
-      int day = s / spd;
-      for (int c = 0; c < A; c++) tb.beta_af[(size_t)s * A + c] = beta * af[(size_t)day * A + c];
-    }
+    tb.beta.assign(S, 0.0);
+    for (int s = 0; s < S; s++) tb.beta[s] = Rfac * Rt[s] / ratio[tv ? s : 0];
   }
-  tb.efoi.assign((size_t)S * G, 0.0);
+  tb.efoi.assign(S, 0.0);
   if (efoi_spline >= 0) {
     std::vector<double> e = disc_spline(efoi_spline, pd);
-    for (int s = 0; s < S; s++)
-      for (int g = 0; g < G; g++) tb.efoi[(size_t)s * G + g] = e[s] / popsize * efoi_agedist[g];
+    for (int s = 0; s < S; s++) tb.efoi[s] = e[s] / popsize;
   }
   tb.geo.assign(S, 1.0);
   if (geo_spline >= 0) tb.geo = disc_spline(geo_spline, pd);
@@ -593,19 +572,60 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
     std::vector<double> v = disc_spline(sp, pd);
     tb.obs_spl.insert(tb.obs_spl.end(), v.begin(), v.end());
   }
-  for (double v : tb.beta_af) if (std::isnan(v)) { err = "NaN transmission rate"; return BEEP_ENAN; }
+  for (double v : tb.beta) if (std::isnan(v)) { err = "NaN transmission rate"; return BEEP_ENAN; }
+  for (double v : tb.af) if (std::isnan(v)) { err = "NaN area factor"; return BEEP_ENAN; }
   return BEEP_OK;
+}
+
+// Model::create_areafactor (src/model.cc:1182-1250): area effects, covariates
+// and level effects, normalised by the population-weighted time average.
+std::vector<double> HostModel::area_factor(const std::vector<double>& pd) const {
+  std::vector<double> af((size_t)period * A, 1.0);
+  if (!area_effect_param.empty())
+    for (int t = 0; t < period; t++)
+      for (int c = 0; c < A; c++) af[(size_t)t * A + c] = pd[area_effect_param[c]];
+  for (const auto& cv : covar) {
+    double b = pd[cv.param];
+    for (int t = 0; t < period; t++)
+      for (int c = 0; c < A; c++) af[(size_t)t * A + c] *= std::exp(b * cv.value[(size_t)c * period + t]);
+  }
+  if (!level_param.empty())
+    for (size_t i = 0; i < af.size(); i++) af[i] *= pd[level_param[level_map[i]]];
+  double area_av = 0;
+  for (int c = 0; c < A; c++) {
+    double av = 0;
+    for (int t = 0; t < period; t++) av += af[(size_t)t * A + c];
+    av /= period;
+    area_av += av * area_pop[c];
+  }
+  area_av /= popsize;
+  for (auto& v : af) v /= area_av;
+  return af;
+}
+
+std::vector<double> HostModel::efoi_groups(const Tables& tb) const {
+  std::vector<double> out((size_t)S * G);
+  for (int s = 0; s < S; s++)
+    for (int g = 0; g < G; g++) out[(size_t)s * G + g] = tb.efoi[s] * efoi_agedist[g];
+  return out;
+}
+
+std::vector<double> HostModel::beta_af(const Tables& tb) const {
+  const std::vector<double>& af = af_param ? tb.af : af_const;
+  std::vector<double> out((size_t)S * A, 0.0);
+  for (int s = 0; s < S; s++)
+    for (int c = 0; c < A; c++) out[(size_t)s * A + c] = tb.beta[s] * af[(size_t)(s / spd) * A + c];
+  return out;
 }
 
 // Strains (Model::calculate_beta_from_R, src/model.cc:1318-1348, with
 // calculate_Vinv and calculate_F per strain): each strain's NGM ratio over its
 // own dps, with the population fractions pooled over strains (democatpos_dist,
 // src/data.cc:154) and that strain's susceptibility; beta_st = Rfactor_st *
-// R(t) / ratio_st.  The device takes beta * areafactor per area and step, so
+// R(t) / ratio_st.  The device takes one beta per step for all groups, so
 // the strain factor Rfactor_st / ratio_st goes into the strain's block of the
-// (block-diagonal) contact matrix, and beta_af holds R(t) * areafactor.
-int HostModel::strain_beta(const std::vector<double>& pd, const std::vector<double>& af, Tables& tb,
-                           std::string& err) const {
+// (block-diagonal) contact matrix, and beta holds R(t).
+int HostModel::strain_beta(const std::vector<double>& pd, Tables& tb, std::string& err) const {
   const int nis = (int)inf_state.size();
   if (nis == 0) { err = "no infected states"; return BEEP_EINVAL; }
   const int dpm = G / nstrain, NV = nis * dpm;
@@ -669,10 +689,7 @@ int HostModel::strain_beta(const std::vector<double>& pd, const std::vector<doub
     for (int q = 0; q < G; q++)
       for (int d = 0; d < G; d++) Ns[(size_t)q * G + d] *= fold[(size_t)s * nstrain + q / dpm];
   }
-  std::vector<double> Rt = disc_spline(R_spline, pd);
-  tb.beta_af.assign((size_t)S * A, 0.0);
-  for (int s = 0; s < S; s++)
-    for (int c = 0; c < A; c++) tb.beta_af[(size_t)s * A + c] = Rt[s] * af[(size_t)(s / spd) * A + c];
+  tb.beta = disc_spline(R_spline, pd);
   return BEEP_OK;
 }
 

@@ -33,8 +33,10 @@ struct Tables {
   std::vector<double> sus;      // [G]
   std::vector<double> inf;      // [C]
   std::vector<double> Ntime;    // [G][G], or [S][G][G] with age matrix modifications
-  std::vector<double> beta_af;  // [S][A] beta * areafactor(day of s)
-  std::vector<double> efoi;     // [S][G] efoi(t_s) / popsize * agedist
+  std::vector<double> beta;     // [S] Rfactor * R(t_s) / NGM ratio (strains: R(t_s))
+  std::vector<double> af;       // [period][A] area factor; empty when it does not depend
+                                // on the parameters (HostModel::af_const holds it)
+  std::vector<double> efoi;     // [S] efoi(t_s) / popsize (times efoi_agedist[g] per group)
   std::vector<double> geo;      // [S]    m(t_s)
   std::vector<double> obs_spl;  // [nobs_spline][S] observation spline values
 };
@@ -87,15 +89,25 @@ class HostModel {
   std::vector<int> section_ti, section_tf;
   std::vector<std::vector<int>> section_obs;
 
+  // The area factor depends on the parameters (area effects, covariates,
+  // level effects); otherwise it is computed once, into af_const [period][A].
+  bool af_param = false;
+  std::vector<double> af_const;
+
   std::vector<double> dirichlet_correct(const double* paramv) const;
   // Returns 0 or a BEEP_* error code.
   int derive(const double* paramv, Tables& out, std::string& err) const;
+  // beta[s] * areafactor[day of s][area], [S][A] (FP64), of derived tables.
+  std::vector<double> beta_af(const Tables& tb) const;
+  // efoi[s] * efoi_agedist[g], [S][G] (FP64): the external force per group.
+  std::vector<double> efoi_groups(const Tables& tb) const;
   double prior_logpdf(const double* paramv) const;
   bool inbounds(const double* paramv) const;
 
  private:
   std::vector<double> disc_spline(int sp, const std::vector<double>& pdir) const;
-  int strain_beta(const std::vector<double>& pd, const std::vector<double>& af, Tables& tb, std::string& err) const;
+  std::vector<double> area_factor(const std::vector<double>& pdir) const;
+  int strain_beta(const std::vector<double>& pd, Tables& tb, std::string& err) const;
 };
 
 }  // namespace beephost

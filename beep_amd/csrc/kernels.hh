@@ -6,8 +6,10 @@
 //   transnum   [S][T][AG]  int32  realised transitions
 //   transmean  [S][T][AG]  fp32   Poisson means
 //   poptraj    [S][C][AG]  int32  populations at the start of each step
-// Per parameter slot (fp32, one block): rate[T][G], sus[G], inf[C], N[G][G],
-// efoi[S][G], geo[S], beta_af[S][A].
+// Per parameter slot (fp32, one block): rate[T][G], sus[G], inf[C], N[G][G]
+// (or [S][G][G]), efoi[S], geo[S], beta[S], and af[period][A] when the area
+// factor depends on the parameters (else one table per context); the efoi age
+// distribution [G] is the context's.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -112,8 +114,10 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
     __syncthreads();
     // Phase 3a: age mixing, transition means (State::set_transmean), small
     // draws and the population update (State::update_pop) into the other buffer.
-    gcfloat* beta_s = tab + tpp->off_beta + (size_t)s * A;
-    gcfloat* efoi_s = tab + tpp->off_efoi + (size_t)s * G;
+    const float beta_s = tab[tpp->off_beta + s];
+    gcfloat* af_s = af_row(tpp, tab, s);
+    const float efoi_s = tab[tpp->off_efoi + s];
+    gcfloat* agedist = gptr(tpp->agedist);
     int* tn_s = jb.tn ? jb.tn + (size_t)s * TAG : nullptr;
     float* tm_s = jb.tm ? jb.tm + (size_t)s * TAG : nullptr;
     int* pt_s = jb.pt ? jb.pt + (size_t)s * CAG : nullptr;
@@ -133,7 +137,7 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
       float nmi = 0.f;
       for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * sI[j * G + gg];
       if (nmi < 0.f) nmi = 0.f;
-      const float foi = beta_s[j] * nmi + efoi_s[g];
+      const float foi = foi_mean(beta_s, af_s[j], nmi, efoi_s, agedist[g]);
       const bool simu = mode == MODE_MBP ? (simu_s ? simu_s[j] != 0 : false) : true;
       CellRng rng;
       rng.init(k0, k1, c0, (uint32_t)cell, jb.key);

@@ -274,42 +274,91 @@ __global__ void __launch_bounds__(256) k_pois_loglik(const int* __restrict__ ins
 }
 
 // Mbp::get_al + MVN::MH (src/mbp.cc:334-361, src/mvn.cc:233-249), FP64.
+// One acceptance: al and the decision with the proposal key's uniform.
+__device__ inline void accept_one(double EFp, double EFi, double dlogq, uint32_t key, int mode, double invT,
+                                  double EFcut, uint64_t seed, uint32_t stream, double* al_out, uint8_t* acc) {
+  double al = 0;
+  if (mode == 0) {
+    if (EFp < EFcut) al = exp(dlogq);
+  } else {
+    al = exp(-invT * 0.5 * (EFp - EFi) + dlogq);
+  }
+  if (al > 1) al = 1;
+  u4 blk = philox4x32_10(u4{0xFFFFu | (stream << 16), 0xFFFFFFFFu, 0x7FFFFFFFu, key}, (uint32_t)seed,
+                         (uint32_t)(seed >> 32));
+  double u = u01d(blk.x);
+  *al_out = al;
+  *acc = u < al ? 1 : 0;
+}
+
 __global__ void k_accept(int n, const int* __restrict__ init, const int* __restrict__ prop, const double* __restrict__ EF,
                          const double* __restrict__ dlogq, const uint32_t* __restrict__ key, int mode, double invT,
                          double EFcut, uint64_t seed, uint32_t stream, double* __restrict__ al_out,
                          uint8_t* __restrict__ acc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double EFp = EF[prop[i]], EFi = EF[init[i]], al = 0;
-  if (mode == 0) {
-    if (EFp < EFcut) al = exp(dlogq[i]);
-  } else {
-    al = exp(-invT * 0.5 * (EFp - EFi) + dlogq[i]);
-  }
-  if (al > 1) al = 1;
-  u4 blk = philox4x32_10(u4{0xFFFFu | (stream << 16), 0xFFFFFFFFu, 0x7FFFFFFFu, key[i]}, (uint32_t)seed,
-                         (uint32_t)(seed >> 32));
-  double u = u01d(blk.x);
-  al_out[i] = al;
-  acc[i] = u < al ? 1 : 0;
+  accept_one(EF[prop[i]], EF[init[i]], dlogq[i], key[i], mode, invT, EFcut, seed, stream, al_out + i, acc + i);
+}
+
+// The same over proposals of many contexts: the EF of each proposed and
+// initial instance through a pointer into its context's EF array, and a
+// temperature per proposal.
+__global__ void k_accept_ptrs(int n, const double* const* __restrict__ efp, const double* const* __restrict__ efi,
+                              const double* __restrict__ invT, const double* __restrict__ dlogq,
+                              const uint32_t* __restrict__ key, int mode, double EFcut, uint64_t seed, uint32_t stream,
+                              double* __restrict__ al_out, uint8_t* __restrict__ acc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  accept_one(*efp[i], *efi[i], dlogq[i], key[i], mode, invT[i], EFcut, seed, stream, al_out + i, acc + i);
 }
 
 // ---- prediction helpers ----
 
-// tab[dst] = tab[src] with beta_af rows s >= from scaled by factor.
+// tab[dst] = tab[src] with beta[s] for steps s >= from scaled by factor (R(t)
+// multiplied after a change point: beta is proportional to R(t_s)).
 __global__ void k_derive_slot(float* __restrict__ tab, size_t stride, const int* __restrict__ dst,
                               const int* __restrict__ src, const float* __restrict__ factor,
-                              const int* __restrict__ from, int off_beta, int A) {
-  const int i = blockIdx.y;
+                              const int* __restrict__ from, int off_beta, int S) {
+  const int i = blockIdx.x;        // grid (n, chunks)
   const float* s_ = tab + (size_t)src[i] * stride;
   float* d_ = tab + (size_t)dst[i] * stride;
   const float f = factor[i];
-  const size_t first = (size_t)off_beta + (size_t)from[i] * A;
-  for (size_t k = blockIdx.x * blockDim.x + threadIdx.x; k < stride; k += (size_t)gridDim.x * blockDim.x) {
+  const size_t first = (size_t)off_beta + (size_t)from[i], end = (size_t)off_beta + (size_t)S;
+  for (size_t k = blockIdx.y * blockDim.x + threadIdx.x; k < stride; k += (size_t)gridDim.y * blockDim.x) {
     float v = s_[k];
-    if (k >= first && k >= (size_t)off_beta) v *= f;
+    if (k >= first && k < end) v *= f;
     d_[k] = v;
   }
+}
+
+// Parameter slots written from a packed upload: tab[slot[i]] = packed[i]
+// (n blocks of `stride` floats); grid (n, chunks): slot on x.
+__global__ void k_scatter_slots(float* __restrict__ tab, size_t stride, const float* __restrict__ packed,
+                                const int* __restrict__ slot) {
+  const int i = blockIdx.x;
+  const float* s_ = packed + (size_t)i * stride;
+  float* d_ = tab + (size_t)slot[i] * stride;
+  for (size_t k = blockIdx.y * blockDim.x + threadIdx.x; k < stride; k += (size_t)gridDim.y * blockDim.x) d_[k] = s_[k];
+}
+
+// Blocks of a packed upload into slots of many contexts: dst[i][k] =
+// packed[src_off[i] + k] for k < len[i]; grid (n, chunks).
+__global__ void k_scatter_blocks(const float* __restrict__ packed, const uint64_t* __restrict__ src_off,
+                                 float* const* __restrict__ dst, const uint32_t* __restrict__ len) {
+  const int i = blockIdx.x;
+  const float* s_ = packed + src_off[i];
+  float* d_ = dst[i];
+  const uint32_t L = len[i];
+  for (uint32_t k = blockIdx.y * blockDim.x + threadIdx.x; k < L; k += gridDim.y * blockDim.x) d_[k] = s_[k];
+}
+
+// The same for FP64 rows (observation spline values per slot).
+__global__ void k_scatter_rows64(double* __restrict__ dst, size_t stride, const double* __restrict__ packed,
+                                 const int* __restrict__ slot) {
+  const int i = blockIdx.x;
+  const double* s_ = packed + (size_t)i * stride;
+  double* d_ = dst + (size_t)slot[i] * stride;
+  for (size_t k = blockIdx.y * blockDim.x + threadIdx.x; k < stride; k += (size_t)gridDim.y * blockDim.x) d_[k] = s_[k];
 }
 
 // out[b][s - t0] = sum over cells of transnum[s][tr][cell]; one workgroup per (b, s).

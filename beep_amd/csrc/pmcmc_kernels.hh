@@ -267,13 +267,13 @@ __global__ void __launch_bounds__(BEEP_ABC_BLOCK) k_abcsmc_weights(int nvar, int
 // and swapping the chain labels (order) and EF.  One thread per launch, as
 // the sweeps are sequential; draw k uses the Philox counter
 // {0xFFFD | stream << 16, k, 0, 0x7FFFFFFD}.
-__global__ void k_mc3_swap(int n, const int* __restrict__ run, const double* __restrict__ invT,
-                           const double* __restrict__ EFall, const int* __restrict__ inst, double* __restrict__ EF,
-                           int* __restrict__ order, int* __restrict__ ntr, int* __restrict__ nac, uint64_t seed,
-                           uint32_t stream) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+// The five sweeps of neighbour swaps over one ladder of n chains (EF already
+// gathered); the uniforms are Philox blocks (0xFFFD | stream << 16, draw,
+// ladder, 0x7FFFFFFD), draw counting the ladder's swap trials.
+__device__ inline void mc3_sweeps(int n, const int* __restrict__ run, const double* __restrict__ invT,
+                                  double* __restrict__ EF, int* __restrict__ order, int* __restrict__ ntr,
+                                  int* __restrict__ nac, uint64_t seed, uint32_t stream, uint32_t ladder) {
   for (int i = 0; i < n; i++) {
-    EF[i] = EFall[inst[i]];
     order[i] = i;
     ntr[i] = 0;
     nac[i] = 0;
@@ -285,7 +285,7 @@ __global__ void k_mc3_swap(int n, const int* __restrict__ run, const double* __r
       if (run[i] != run[i + 1]) continue;
       const double al = exp(0.5 * (invT[i] - invT[i + 1]) * (EF[i] - EF[i + 1]));
       ntr[i]++;
-      u4 r = philox4x32_10(u4{0xFFFDu | (stream << 16), draw++, 0u, 0x7FFFFFFDu}, k0, k1);
+      u4 r = philox4x32_10(u4{0xFFFDu | (stream << 16), draw++, ladder, 0x7FFFFFFDu}, k0, k1);
       if (u01d(r.x) < al) {
         nac[i]++;
         int to = order[i];
@@ -297,6 +297,29 @@ __global__ void k_mc3_swap(int n, const int* __restrict__ run, const double* __r
       }
     }
   }
+}
+
+__global__ void k_mc3_swap(int n, const int* __restrict__ run, const double* __restrict__ invT,
+                           const double* __restrict__ EFall, const int* __restrict__ inst, double* __restrict__ EF,
+                           int* __restrict__ order, int* __restrict__ ntr, int* __restrict__ nac, uint64_t seed,
+                           uint32_t stream) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int i = 0; i < n; i++) EF[i] = EFall[inst[i]];
+  mc3_sweeps(n, run, invT, EF, order, ntr, nac, seed, stream, 0u);
+}
+
+// Many ladders at once, one thread each: ladder l holds the chains
+// ptr[l] .. ptr[l+1]-1, whose EF are read through per-chain pointers (their
+// contexts' EF arrays); order is local to the ladder.
+__global__ void k_mc3_swap_ladders(int nladder, const int* __restrict__ ptr, const int* __restrict__ run,
+                                   const double* __restrict__ invT, const double* const* __restrict__ efp,
+                                   double* __restrict__ EF, int* __restrict__ order, int* __restrict__ ntr,
+                                   int* __restrict__ nac, uint64_t seed, uint32_t stream) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= nladder) return;
+  const int b = ptr[l], e = ptr[l + 1];
+  for (int i = b; i < e; i++) EF[i] = *efp[i];
+  mc3_sweeps(e - b, run + b, invT + b, EF + b, order + b, ntr + b, nac + b, seed, stream, (uint32_t)l);
 }
 
 }  // namespace beepdev

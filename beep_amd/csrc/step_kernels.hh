@@ -135,8 +135,8 @@ struct SparseMix {
 };
 
 __global__ void __launch_bounds__(256) k_sparse_contract(SparseMix p) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  int row = blockIdx.y;
+  int j = blockIdx.y * blockDim.x + threadIdx.x;
+  int row = blockIdx.x;
   if (j >= p.A) return;
   const float* Dr = p.D + (size_t)row * p.Apad;
   float idg = p.diag[j] * Dr[j];
@@ -148,12 +148,12 @@ __global__ void __launch_bounds__(256) k_sparse_contract(SparseMix p) {
 
 // Start of a step-wise run: the state at t0 (the spec's initial populations
 // when t0 == 0) and its infectivity D[(b,g)][j] = sum_c inf_c pop_c.
-// Grid (ceil(AG/256), nb).
+// Grid (nb, ceil(AG/256)): job on x, block of cells on y.
 __global__ void __launch_bounds__(256) k_step_init(const Job* __restrict__ jobs, float* __restrict__ D, int Apad) {
-  const Job jb = jobs[blockIdx.y];
+  const Job jb = jobs[blockIdx.x];
   ctopo* tp = (ctopo*)jb.topo;
   const int AG = tp->AG, C = tp->C, G = tp->G;
-  int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  int cell = blockIdx.y * blockDim.x + threadIdx.x;
   if (cell >= AG) return;
   gint* state = (gint*)jb.state;
   float d = 0.f;
@@ -164,13 +164,13 @@ __global__ void __launch_bounds__(256) k_step_init(const Job* __restrict__ jobs,
     if (w != 0.f) d += w * (float)v;
   }
   int j = cell / G, g = cell - j * G;
-  ((gfloat*)D)[((size_t)blockIdx.y * G + g) * Apad + j] = d;
+  ((gfloat*)D)[((size_t)blockIdx.x * G + g) * Apad + j] = d;
 }
 
 #define BEEP_STEP_QCAP 512
 
 // One step s of the tau-leap for a batch, one thread per cell, grid
-// (ceil(AG/256), nb): means from the mixed infectivity I of the contraction,
+// (nb, ceil(AG/256)): means from the mixed infectivity I of the contraction,
 // small draws in place, large ones through an LDS queue, populations updated
 // in HBM, and the next step's infectivity D written.
 template <int MODE>
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
                                                                const uint64_t* __restrict__ rngp) {
   const uint64_t seed = rngp[0];
   const uint32_t stream = (uint32_t)rngp[1];
-  const int b = blockIdx.y;
+  const int b = blockIdx.x;
   const Job jb = jobs[b];
   ctopo* __restrict__ tp = (ctopo*)jb.topo;
   const int A = tp->A, G = tp->G, C = tp->C, T = tp->T, S = tp->S, AG = tp->AG, inf_tr = tp->inf_tr;
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
   gcint* tto = gptr(tp->to);
   const float dt = tp->dt;
   const int tid = threadIdx.x;
-  const int cell = blockIdx.x * BEEP_STEP_BLOCK + tid;
+  const int cell = blockIdx.y * BEEP_STEP_BLOCK + tid;
   const bool valid = cell < AG && s >= jb.t0 && s < jb.t1;
 
   __shared__ int spop0[16][BEEP_STEP_BLOCK];     // populations at step s of this workgroup's cells, C <= 16
@@ -225,7 +225,8 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
     const size_t offN = tp->off_N + (tp->N_tv ? (size_t)s * G * G : 0);   // a time-varying age matrix: step s's
     for (int gg = 0; gg < G; gg++) nmi += tab[offN + g * G + gg] * Ib[(size_t)gg * Apad + j];
     if (nmi < 0.f) nmi = 0.f;
-    const float foi = tab[tp->off_beta + (size_t)s * A + j] * nmi + tab[tp->off_efoi + (size_t)s * G + g];
+    const float foi = foi_mean(tab[tp->off_beta + s], af_row(tp, tab, s)[j], nmi, tab[tp->off_efoi + s],
+                               gptr(tp->agedist)[g]);
     const bool simu = MODE == MODE_MBP ? (jb.simu ? gptr(jb.simu)[(size_t)s * A + j] != 0 : false) : true;
     CellRng rng;
     rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
@@ -287,7 +288,7 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
   for (int i = tid; i < nq; i += BEEP_STEP_BLOCK) {
     uint32_t code = qcode[i];
     int t2 = (int)(code & 0xFFFFFu), tr = (int)((code >> 20) & 0xFFu), kind = (int)(code >> 28);
-    int c2 = blockIdx.x * BEEP_STEP_BLOCK + t2;
+    int c2 = blockIdx.y * BEEP_STEP_BLOCK + t2;
     int ni = MODE == MODE_MBP ? itn[tr * AG + c2] : 0;
     int n = draw_item(kind, qval[i], ni, k0, k1, c0, (uint32_t)c2, (uint32_t)tr, jb.key);
     if (tn_s) st_nt(tn_s, tr * AG + c2, n);

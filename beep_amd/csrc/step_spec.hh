@@ -15,13 +15,13 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
   using namespace beepspec;
   const uint64_t seed = rngp[0];
   const uint32_t stream = (uint32_t)rngp[1];
-  const int b = blockIdx.y;
+  const int b = blockIdx.x;
   const Job jb = jobs[b];
   ctopo* __restrict__ tp = (ctopo*)jb.topo;
   const int A = tp->A, S = tp->S, AG = tp->AG;
   const float dt = tp->dt;
   const int tid = threadIdx.x;
-  const int cell = blockIdx.x * BEEP_STEP_BLOCK + tid;
+  const int cell = blockIdx.y * BEEP_STEP_BLOCK + tid;
   const bool valid = cell < AG && s >= jb.t0 && s < jb.t1;
 
   __shared__ int spop[CP][BEEP_STEP_BLOCK];          // populations after the step's own-lane draws
@@ -68,7 +68,8 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
 #pragma unroll
     for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * Ib[(size_t)gg * Apad + j];
     if (nmi < 0.f) nmi = 0.f;
-    const float foi = tab[tp->off_beta + (size_t)s * A + j] * nmi + tab[tp->off_efoi + (size_t)s * G + g];
+    const float foi = foi_mean(tab[tp->off_beta + s], af_row(tp, tab, s)[j], nmi, tab[tp->off_efoi + s],
+                               gptr(tp->agedist)[g]);
     const bool simu = MODE == MODE_MBP ? (jb.simu ? gptr(jb.simu)[(size_t)s * A + j] != 0 : false) : true;
     CellRng rng;
     rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
@@ -127,7 +128,7 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
   for (int i = tid; i < nq; i += BEEP_STEP_BLOCK) {
     uint32_t code = qcode[i];
     int t2 = (int)(code & 0xFFFFFu), tr = (int)((code >> 20) & 0xFFu), kind = (int)(code >> 28);
-    int c2 = blockIdx.x * BEEP_STEP_BLOCK + t2;
+    int c2 = blockIdx.y * BEEP_STEP_BLOCK + t2;
     int ni = MODE == MODE_MBP ? itn[tr * AG + c2] : 0;
     int n = draw_item_inline(kind, qval[i], ni, k0, k1, c0, (uint32_t)c2, (uint32_t)tr, jb.key);
     if (tn_s) st_nt(tn_s, tr * AG + c2, n);

@@ -30,6 +30,10 @@ A spec is a plain dict (JSON-serialisable once arrays are lists):
   obs           [{"y", "sd", "type", "shape", "factor", "thresh", "percent", "datatable",
                   "graph", "sett_i", "sett_f", "area": [...], "dp": [...], "tr": [...], "co": [...],
                   "spline": obs_spline weighting each step's transition counts, or -1}]
+  obs_table     a compact alternative to obs for many one-area observations sharing
+                their dp and transition lists: {"y", "sd", "sett_i", "sett_f", "area",
+                "graph": arrays [nobs]; "type": one obs type; "datatable": int;
+                "dp": [...], "tr": [...]}.  `expand_obs` turns it into obs entries.
   sections      [(ti, tf), ...] PMCMC observation sections
 
 `param_index(spec, name)` finds parameters by name.
@@ -100,6 +104,22 @@ def prior_mid(spec):
         else:
             v.append(1.0)
     return np.array(v, dtype=np.float64)
+
+
+def expand_obs(spec):
+    """The spec with an obs_table turned into the equivalent obs entries (a
+    shallow copy; specs without a table are returned as they are)."""
+    tab = spec.get("obs_table")
+    if tab is None:
+        return spec
+    out = dict(spec)
+    dp, tr = list(tab["dp"]), list(tab["tr"])
+    out["obs"] = [{"y": float(y), "sd": float(sd), "type": tab["type"], "datatable": int(tab.get("datatable", 0)),
+                   "graph": int(gr), "sett_i": int(a), "sett_f": int(b), "area": [int(c)], "dp": dp, "tr": tr}
+                  for y, sd, gr, a, b, c in zip(tab["y"], tab["sd"], tab["graph"], tab["sett_i"], tab["sett_f"],
+                                                tab["area"])]
+    del out["obs_table"]
+    return out
 
 
 def obs_elements(spec):

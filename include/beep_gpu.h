@@ -303,10 +303,16 @@ int beep_accept(beep_ctx* ctx, int n, const int32_t* init, const int32_t* prop, 
  * belong to different contexts (different area counts, age groups, horizons):
  * the batch-submitting counterpart of the reference drivers' one-particle-at-
  * a-time loops (ABC-SMC particles, MC3 chains, src/abcsmc.cc:42-81,
- * src/mc3.cc:60-66).  Fused-kernel work is grouped by size class, each group
- * sorted longest first and launched as one kernel whose workgroups each take
- * one instance; step-wise work runs per context.  Results are those of the
- * single-context calls with the same keys. */
+ * src/mc3.cc:60-66).  Work is grouped by size class, each group sorted
+ * longest first and launched as one fused kernel whose workgroups each take
+ * one instance.  Results are those of the single-context calls with the same
+ * keys.  Per context, the instances written (simulated, proposed) must be
+ * distinct and no MBP's initial instance may be written by the same batch.
+ * Contexts on the step-wise path (dense mixing, or an instance beyond LDS)
+ * are refused with BEEP_EINVAL: their work goes through beep_simulate /
+ * beep_mbp.  Autotuning times only size classes whose jobs all start from
+ * the initial state (t0 = 0 simulations and MBPs), which a rerun recomputes
+ * identically; other classes run once at the default size. */
 typedef struct {
   beep_ctx* ctx;
   int32_t kind;      /* BEEP_WORK_SIMULATE or BEEP_WORK_MBP                       */
@@ -323,9 +329,44 @@ typedef struct {
                                     class not yet tuned and keep the fastest (cached
                                     in memory and in the file named by the
                                     BEEP_TUNE_CACHE environment variable)           */
-/* stats_out (may be NULL): [0] wall ms, [1] fused launches, [2] step-wise
- * contexts, [3] cell-steps (instances x areas x age groups x steps). */
-int beep_run_batch(int n, const beep_work* work, uint64_t seed, uint32_t stream, int flags, double* stats_out4);
+#define BEEP_BATCH_LOGLIK    2   /* then each item's observation log-likelihood
+                                    (ObservationModel::calculate, src/obsmodel.cc:
+                                    22-109, FP64) into L_out[i]; the instance's EF
+                                    (-2 L) is kept for beep_accept / beep_batch_accept */
+/* stats_out (may be NULL): [0] wall ms, [1] fused launches, [2] 0 (step-wise
+ * contexts are not batched), [3] cell-steps (instances x areas x age groups x
+ * steps). */
+int beep_run_batch(int n, const beep_work* work, uint64_t seed, uint32_t stream, int flags, double* L_out,
+                   double* stats_out4);
+/* State::set_param (src/state.cc:66-91) for parameter slots of many contexts
+ * at once, item i writing slot[i] of ctx[i] from the nparam values at
+ * paramval[i]: the derivations run in parallel on the host threads, then one
+ * pinned upload and one scatter kernel place every slot's tables.  Pr_out /
+ * inb_out (may be NULL) as beep_set_param.  A slot may not appear twice for
+ * one context; models with observation splines go through beep_set_param.
+ * On a derivation error nothing is written. */
+int beep_batch_set_param(int n, beep_ctx* const* ctx, const int32_t* slot, const double* const* paramval,
+                         double* Pr_out, int32_t* inb_out);
+/* Metropolis-Hastings acceptance (Mbp::get_al with MVN::MH, src/mbp.cc:334-361,
+ * src/mvn.cc:233-249) of MBP proposals across contexts, FP64 on the device: for
+ * work item i (kind BEEP_WORK_MBP), with the EF its proposed (inst) and initial
+ * (init) instances last got from the observation model, al = min(1,
+ * exp(-invT[i]/2 (EF_p - EF_i) + dlogq[i])) (mode 1) or exp(dlogq[i]) when
+ * EF_p < EFcut (mode 0); accepted[i] = u < al with the uniform beep_accept
+ * draws for the item's key. */
+int beep_batch_accept(int n, const beep_work* work, int mode, const double* invT, double EFcut, const double* dlogq,
+                      uint64_t seed, uint32_t stream, double* al_out, uint8_t* accepted);
+/* MC3::swap_states (src/mc3.cc:153-215) for many chain ladders at once: ladder
+ * l holds the chains ladder_ptr[l] .. ladder_ptr[l+1]-1 (ladder_ptr[0] = 0),
+ * chain i being instance inst[i] of context ctx[i] (its EF from the last
+ * observation-model evaluation) at inverse temperature invT[i], run label
+ * run[i].  order_out[i]: the chain (index within its ladder) whose state chain
+ * i takes; ntr/nac per chain as beep_mc3_swap.  Ladder l draws its uniforms
+ * from Philox blocks (0xFFFD | stream << 16, trial, l, 0x7FFFFFFD): ladder 0
+ * draws what beep_mc3_swap draws. */
+int beep_batch_mc3_swap(int nladder, const int32_t* ladder_ptr, beep_ctx* const* ctx, const int32_t* inst,
+                        const int32_t* run, const double* invT, uint64_t seed, uint32_t stream, int32_t* order_out,
+                        int32_t* ntr_out, int32_t* nac_out);
 
 /* ---- state transfer (State::save / load, src/state.cc:709-808; PMCMC moves) ---- */
 int beep_get_transnum(beep_ctx* ctx, int inst, int32_t* out);   /* [S][A][T][G] */
@@ -434,6 +475,15 @@ int beep_debug_binomial(int device, int n, const int32_t* nn, const double* p, u
 /* C[rows][K] = A[rows][K] x B[K][K] with the dense contraction's MFMA tiling
  * (K a multiple of 64), FP32. */
 int beep_debug_gemm(int device, int rows, int K, const float* A, const float* B, float* C);
+/* The host-side launch checks every fused and step-wise launch passes before
+ * it is enqueued (no device is touched).  kind BEEP_CHECK_FUSED, args
+ * {njobs, block, lds bytes, lds limit}; kind BEEP_CHECK_STEPWISE, args {A, G,
+ * Apad, n jobs, capacity}.  Returns BEEP_OK, or BEEP_EINVAL with the reason in
+ * err.  On success grid_out (may be NULL) receives the grid: fused {x}, step-
+ * wise {cell x, cell y, dense x, dense y, sparse x, sparse y}. */
+#define BEEP_CHECK_FUSED     0
+#define BEEP_CHECK_STEPWISE  1
+int beep_debug_check_launch(int kind, const int64_t* args, int nargs, int64_t* grid_out, char* err, int errlen);
 
 #ifdef __cplusplus
 }

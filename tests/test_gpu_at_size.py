@@ -1,0 +1,154 @@
+"""Parity with the FP64 oracle at the benchmark workloads' real sizes.
+
+  dense 317 x 9 x 300 (10 compartments, dense gravity mixing, covariates):
+      oracle trajectories replayed on the device's step-wise path, whose
+      area-mixing contraction is the FP32 MFMA GEMM over the zero-padded
+      320-area operands; several instances at once so that the GEMM's rows
+      span more than one 64-row tile;
+  uneven batch: items sampled from the 16384-instance workload (including a
+      400-area, 9-age-group model), simulated and proposed in one mixed batch
+      with their log-likelihoods, each device trajectory replayed through the
+      oracle.
+
+The bar: populations bit-exact, transition means within the FP32 bound
+(relative to the magnitudes of their summands), observation log-likelihood
+within 1e-6 relative."""
+import numpy as np
+import pytest
+
+from beep_amd import _lib, configs, spec as specmod
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def transmean_scale(orc, tb, st):
+    """Per (step, area, transition, group), the mean recomputed from the
+    absolute values of all its summands (|inf * pop|, |mixing weights|,
+    |contact matrix|): the scale of its FP32 rounding error.  Matrix form,
+    so that it runs at hundreds of areas."""
+    d, nz = tb.derived, orc.nz
+    A, G, S = orc.A, orc.G, orc.S
+    itr = nz["itr"]
+    M = np.zeros((A, A))
+    for c in range(A):
+        for k in range(nz["rowptr"][c], nz["rowptr"][c + 1]):
+            M[c, nz["to"][k]] += abs(nz["val"][k])
+    out = np.abs(st.transmean).copy()
+    inf = np.abs(d["inf"])
+    fr = orc.spec["trans"][itr]["from"]
+    dt = orc.spec["period"] / S
+    spd = orc.spec["spd"]
+    for s in range(S):
+        pop = st.pop[s]
+        dinf = np.einsum("c,acg->ag", inf, np.abs(pop))
+        idg = np.abs(nz["diag"])[:, None] * dinf
+        imp = M.T @ dinf
+        m = d["geo"][s]
+        I = (idg + imp) * abs(m) + np.abs(nz["factor"])[:, None] * idg * abs(1 - m)
+        nmi = I @ np.abs(d["Ntime"][s]).T
+        Sus = np.maximum(pop[:, fr, :], 0)
+        lam = dt * Sus * np.abs(d["sus"])[None, :] * (np.abs(d["beta"][s])[:, None] * np.abs(d["af"][s // spd])[:, None]
+                                                     * nmi + abs(d["efoi"][s]))
+        out[s][:, itr, :] = np.maximum(out[s][:, itr, :], lam)
+    return out
+
+
+def check_replay(ctx, inst, orc, tb, st, label, obs=True, t1=None):
+    """Device trajectory of instance inst over steps [0, t1) against the
+    oracle state st (the oracle's replay of, or own draw of, the same
+    transition counts)."""
+    t1 = orc.S if t1 is None else t1
+    pop = ctx.get_pop(inst)[:t1]
+    assert np.array_equal(pop.astype(np.float64), st.pop[:t1]), f"{label}: populations"
+    tm = ctx.get_transmean(inst).astype(np.float64)[:t1]
+    ref = st.transmean[:t1]
+    mag = transmean_scale(orc, tb, st)[:t1]
+    err = np.abs(tm - ref)
+    assert (err <= 2e-5 * ref + 1e-5 * mag + 1e-12).all(), (label, (err - 2e-5 * ref - 1e-5 * mag).max())
+    if obs:
+        L = ctx.obs_loglik([inst])[0]
+        Lr, _ = orc.obs_loglik(st)
+        assert abs(L / Lr - 1) < 1e-6, (label, L, Lr)
+    return True
+
+
+def test_dense_317x9_replay_against_oracle(gpu_lib):
+    """Config 3's model: 8 oracle trajectories (two seeds at the true
+    parameters, six at prior draws) replayed together on the device."""
+    spec, truth = configs.dense_pmcmc()
+    orc = O.Oracle(spec)
+    rng = np.random.default_rng(3)
+    params = np.vstack([truth, truth, specmod.sample_prior(spec, rng, 6)])
+    n = len(params)
+    ctx = _lib.Context(spec, capacity=n, nslot=n)
+    assert ctx.path in (_lib.PATH_STEPWISE, _lib.PATH_STEPWISE_SPEC)
+    assert (ctx.A, ctx.G, ctx.S) == (317, 9, 300)
+    ctx.set_param(range(n), params)
+    states = []
+    for i, pv in enumerate(params):
+        tb = orc.tables(pv)
+        st = orc.simulate(tb, seed=4000 + i)
+        ctx.put_transnum(i, i, st.transnum.astype(np.int32))
+        states.append((tb, st))
+    for i, (tb, st) in enumerate(states):
+        check_replay(ctx, i, orc, tb, st, f"dense instance {i}")
+    # The device's own simulations of the same instances, replayed through the
+    # oracle: the populations and means the device computed are the oracle's.
+    ctx.simulate(range(n), range(n), 77 + np.arange(n), seed=5, stream=6)
+    for i in (0, 5):
+        tb = states[i][0]
+        st = orc.replay(tb, ctx.get_transnum(i).astype(np.float64))
+        check_replay(ctx, i, orc, tb, st, f"dense simulation {i}")
+    ctx.close()
+
+
+def _sample_uneven(n_items, seed):
+    """Items of the 16384-instance uneven workload: the largest 9-age-group
+    model, then a random sample of ABC-SMC particles and MC3 chains."""
+    w = configs.uneven_batch(16384, seed=4)
+    abc = [(gi, g, h) for gi, g in enumerate(w["abc"]) for h in g["horizons"]]
+    mc3 = list(enumerate(w["mc3"]))
+    big = max((x for x in abc if x[1]["G"] == 9), key=lambda x: x[1]["A"])
+    rng = np.random.default_rng(seed)
+    pick_abc = [big] + [abc[i] for i in rng.choice(len(abc), n_items // 2, replace=False)]
+    pick_mc3 = [mc3[i] for i in rng.choice(len(mc3), n_items // 2, replace=False)]
+    return pick_abc, pick_mc3
+
+
+def test_uneven_sample_against_oracle(gpu_lib):
+    """34 items of the uneven workload (a 400-area x 9-group model among them)
+    in one mixed batch with log-likelihoods: every simulated and proposed
+    trajectory replayed through the oracle at its parameters."""
+    pick_abc, pick_mc3 = _sample_uneven(32, seed=8)
+    rng = np.random.default_rng(9)
+    items, checks = [], []
+    for k, (gi, g, h) in enumerate(pick_abc):
+        spec, _ = configs.seir_areas(g["A"], g["G"], g["S"], seed=4 * 100003 + gi, daily_obs=True)
+        ctx = _lib.Context(spec, capacity=1, nslot=1, keep=_lib.KEEP_ALL)
+        pv = specmod.sample_prior(spec, rng, 1)[0]
+        ctx.set_param([0], pv[None, :])
+        items.append((ctx, _lib.WORK_SIMULATE, 0, 0, 0, 10 + k, 0, h))
+        checks.append((ctx, 0, spec, pv, h))
+    for k, (li, ld) in enumerate(pick_mc3):
+        spec, _ = configs.seir_areas(ld["A"], ld["G"], ld["S"], seed=4 * 100003 + 50000 + li, daily_obs=True)
+        ctx = _lib.Context(spec, capacity=2, nslot=2, keep=_lib.KEEP_ALL)
+        pv = specmod.sample_prior(spec, rng, 2)
+        ctx.set_param([0, 1], pv)
+        ctx.simulate([0], [0], [900 + k], seed=3, stream=2)          # the chain's current state
+        items.append((ctx, _lib.WORK_MBP, 1, 0, 1, 500 + k, 0, ctx.S))
+        checks.append((ctx, 1, spec, pv[1], ctx.S))
+    assert max(c[0].A * c[0].G for c in checks) >= 9 * 390
+    _stats, L = _lib.run_batch(items, seed=6, stream=7, loglik=True)
+    for i, (ctx, inst, spec, pv, t1) in enumerate(checks):
+        orc = O.Oracle(specmod.expand_obs(spec))
+        tb = orc.tables(pv)
+        st = orc.replay(tb, ctx.get_transnum(inst).astype(np.float64))
+        check_replay(ctx, inst, orc, tb, st, f"uneven item {i} ({ctx.A}x{ctx.G}x{ctx.S}, steps {t1})", obs=False,
+                     t1=t1)
+        # The log-likelihood covers the model's observations; steps past a
+        # particle's horizon hold zero counts on both sides.
+        Lr, _ = orc.obs_loglik(st)
+        assert abs(L[i] / Lr - 1) < 1e-6, (i, L[i], Lr)
+    for c in checks:
+        c[0].close()

@@ -60,3 +60,71 @@ def test_batch_equals_single_context_calls(gpu_lib, tmp_path):
     assert (tmp_path / "tune.txt").exists()
     for ctx in a + b:
         ctx.close()
+
+
+def test_autotune_leaves_continued_sections_untouched(gpu_lib, tmp_path):
+    """A class with a job continuing from its state (t0 > 0) is not re-run by
+    the autotuner: its result equals the single-context call."""
+    os.environ["BEEP_TUNE_CACHE"] = str(tmp_path / "tune2.txt")
+    out = []
+    for batched in (False, True):
+        spec, truth = configs.seir_areas(200, 9, 60, seed=5)     # a size class no other test tunes
+        ctx = _lib.Context(spec, capacity=3, nslot=1)
+        ctx.set_param([0], truth[None, :])
+        ctx.simulate([0, 1], [0, 0], [7, 8], 0, 30, seed=5, stream=1)
+        if batched:
+            items = [(ctx, _lib.WORK_SIMULATE, 0, 0, 0, 7, 30, 60), (ctx, _lib.WORK_SIMULATE, 1, 0, 0, 8, 30, 60)]
+            _lib.run_batch(items, seed=5, stream=2, autotune=True)
+        else:
+            ctx.simulate([0, 1], [0, 0], [7, 8], 30, 60, seed=5, stream=2)
+        out.append([(ctx.get_transnum(i), ctx.get_state(i)) for i in (0, 1)])
+        ctx.close()
+    for (ta, sa), (tb, sb) in zip(*out):
+        assert np.array_equal(ta, tb) and np.array_equal(sa, sb)
+
+
+def test_written_instances_must_be_distinct(gpu_lib):
+    spec, truth = configs.seir_areas(12, 1, 40, seed=6)
+    ctx = _lib.Context(spec, capacity=4, nslot=1)
+    ctx.set_param([0], truth[None, :])
+    with pytest.raises(_lib.BeepError, match="written twice") as e:
+        ctx.simulate([1, 1], [0, 0], [1, 2])
+    assert e.value.code == -1
+    ctx.simulate([0, 1], [0, 0], [1, 2])
+    with pytest.raises(_lib.BeepError, match="both read and written"):
+        ctx.mbp([0, 2], [2, 3], [0, 0], [3, 4])          # the second proposal's initial state is the first's proposal
+    with pytest.raises(_lib.BeepError, match="written twice"):
+        _lib.run_batch([(ctx, _lib.WORK_SIMULATE, 2, 0, 0, 5, 0, ctx.S), (ctx, _lib.WORK_MBP, 2, 0, 0, 6, 0, ctx.S)])
+    with pytest.raises(_lib.BeepError, match="written by the batch"):
+        _lib.run_batch([(ctx, _lib.WORK_SIMULATE, 0, 0, 0, 5, 0, ctx.S), (ctx, _lib.WORK_MBP, 2, 0, 0, 6, 0, ctx.S)])
+    ctx.close()
+
+
+def test_stepwise_contexts_run_outside_the_batch(gpu_lib):
+    """beep_run_batch refuses step-wise contexts; batch.run sends their work
+    through the context's own calls, with the single-call results."""
+    from beep_amd import batch
+    spec, truth = configs.seir_areas(40, 4, 50, seed=7)
+    outs = []
+    for mode in ("single", "run"):
+        fused = _lib.Context(spec, capacity=2, nslot=1)
+        step = _lib.Context(spec, capacity=3, nslot=1, keep=_lib.KEEP_ALL | _lib.STEPWISE)
+        for c in (fused, step):
+            c.set_param([0], truth[None, :])
+        assert step.path in (_lib.PATH_STEPWISE, _lib.PATH_STEPWISE_SPEC)
+        if mode == "single":
+            fused.simulate([0], [0], [11], seed=3, stream=9)
+            step.simulate([0, 1], [0, 0], [12, 13], seed=3, stream=9)
+            step.mbp([0], [2], [0], [14], seed=3, stream=9)
+        else:
+            with pytest.raises(_lib.BeepError, match="step-wise context"):
+                _lib.run_batch([(step, _lib.WORK_SIMULATE, 0, 0, 0, 12, 0, step.S)], seed=3, stream=9)
+            batch.run([(fused, _lib.WORK_SIMULATE, 0, 0, 0, 11, 0, fused.S),
+                       (step, _lib.WORK_SIMULATE, 0, 0, 0, 12, 0, step.S),
+                       (step, _lib.WORK_SIMULATE, 1, 0, 0, 13, 0, step.S)], seed=3, stream=9)
+            batch.run([(step, _lib.WORK_MBP, 2, 0, 0, 14, 0, step.S)], seed=3, stream=9)
+        outs.append([fused.get_transnum(0)] + [step.get_transnum(i) for i in (0, 1, 2)])
+        fused.close()
+        step.close()
+    for x, y in zip(*outs):
+        assert np.array_equal(x, y)

@@ -1,0 +1,148 @@
+"""The batched steps of an uneven ABC-SMC / MC3 sweep across contexts: the
+log-likelihoods a mixed batch returns (against the FP64 oracle on the same
+trajectories), parameter slots set for many contexts at once, the
+Metropolis-Hastings acceptance at per-chain temperatures and the ladder
+swaps (against the oracle's MC3::swap_states on the same uniforms), and a
+small uneven workload sweeping end to end."""
+import numpy as np
+import pytest
+
+from beep_amd import _lib, configs, spec as specmod
+from beep_amd.batch import UnevenBatch
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _models():
+    out = []
+    for (A, G, S, seed) in ((12, 1, 40, 1), (30, 4, 55, 2), (17, 9, 33, 3)):
+        spec, truth = configs.seir_areas(A, G, S, seed=seed, daily_obs=True)
+        out.append((spec, truth))
+    return out
+
+
+def test_batch_loglik_matches_oracle(gpu_lib):
+    """Simulations and MBP proposals of three contexts in one batch with
+    their log-likelihoods: each equals the oracle's FP64 log-likelihood of
+    the same device trajectory to 1e-6 relative (FP32 means do not enter:
+    the observation model reads transition counts)."""
+    ctxs, items, params = [], [], {}
+    for ci, (spec, truth) in enumerate(_models()):
+        ctx = _lib.Context(spec, capacity=4, nslot=4)
+        pv = np.stack([truth * [f, 1, 1, 1] for f in (1.0, 0.9, 1.1, 1.2)])
+        ctx.set_param(range(4), pv)
+        params[id(ctx)] = pv
+        ctx.simulate([0], [0], [10 + ci], seed=2, stream=1)
+        ctxs.append(ctx)
+        items += [(ctx, _lib.WORK_SIMULATE, 1, 0, 1, 20 + ci, 0, ctx.S),
+                  (ctx, _lib.WORK_SIMULATE, 2, 0, 2, 30 + ci, 0, ctx.S // 2),
+                  (ctx, _lib.WORK_MBP, 3, 0, 3, 40 + ci, 0, ctx.S)]
+    stats, L = _lib.run_batch(items, seed=3, stream=2, loglik=True)
+    assert np.isfinite(L).all()
+    for i, (ctx, kind, inst, init, slot, key, a, b) in enumerate(items):
+        orc = O.Oracle(specmod.expand_obs(ctx.spec))
+        tb = orc.tables(params[id(ctx)][slot])
+        st = orc.replay(tb, ctx.get_transnum(inst).astype(np.float64))
+        want, _ = orc.obs_loglik(st)
+        assert abs(L[i] / want - 1) < 1e-6, (i, L[i], want)
+        assert L[i] == ctx.obs_loglik([inst])[0]          # the same kernel and summation order
+    for ctx in ctxs:
+        ctx.close()
+
+
+def test_batch_set_param_equals_set_param(gpu_lib):
+    """Slots of three contexts set in one call hold the tables beep_set_param
+    writes, and simulations from them are bit-identical."""
+    models = _models()
+    rng = np.random.default_rng(4)
+    a = [_lib.Context(s, capacity=3, nslot=3) for s, _ in models]
+    b = [_lib.Context(s, capacity=3, nslot=3) for s, _ in models]
+    pvs = [specmod.sample_prior(s, rng, 3) for s, _ in models]
+    for ctx, pv in zip(a, pvs):
+        ctx.set_param([2, 0, 1], pv)                 # not contiguous: the scatter path
+    hs, slots, rows = [], [], []
+    for ctx, pv in zip(b, pvs):
+        for k, sl in enumerate((2, 0, 1)):
+            hs.append(ctx)
+            slots.append(sl)
+            rows.append(pv[k])
+    Pr, inb = _lib.batch_set_param(hs, slots, np.stack(rows))
+    assert inb.all() and np.isfinite(Pr).all()
+    for ca, cb in zip(a, b):
+        for sl in range(3):
+            ta, tb = ca.get_tables(sl), cb.get_tables(sl)
+            for k in ta:
+                assert np.array_equal(ta[k], tb[k]), k
+        for c in (ca, cb):
+            c.simulate([0, 1, 2], [0, 1, 2], [5, 6, 7], seed=9, stream=4)
+        for i in range(3):
+            assert np.array_equal(ca.get_transnum(i), cb.get_transnum(i))
+    with pytest.raises(_lib.BeepError, match="set twice"):
+        _lib.batch_set_param([b[0], b[0]], [1, 1], np.stack([pvs[0][0], pvs[0][1]]))
+    for c in a + b:
+        c.close()
+
+
+def test_batch_accept_and_swaps_match_oracle(gpu_lib):
+    """Acceptance at per-chain temperatures over three ladders (contexts)
+    equals the closed form and the single-context beep_accept; the ladder
+    swaps follow the oracle's MC3::swap_states on each ladder's uniforms."""
+    L = 5
+    ctxs, mbp, invT, dlogq = [], [], [], []
+    rng = np.random.default_rng(6)
+    temps = (1.0 - np.arange(L) / L) ** 3
+    for li, (spec, truth) in enumerate(_models()):
+        ctx = _lib.Context(spec, capacity=2 * L, nslot=2 * L)
+        ctx.set_param(range(2 * L), specmod.sample_prior(spec, rng, 2 * L))
+        ctx.simulate(range(L), range(L), 100 * li + np.arange(L), seed=1, stream=1)
+        ctx.obs_loglik(range(L))
+        ctxs.append(ctx)
+        for k in range(L):
+            mbp.append((ctx, _lib.WORK_MBP, L + k, k, L + k, 500 + 10 * li + k, 0, ctx.S))
+        invT += list(temps)
+        dlogq += list(rng.normal(0, 1, L))
+    _stats, Lik = _lib.run_batch(mbp, seed=2, stream=3, loglik=True)
+    al, acc = _lib.batch_accept(mbp, np.array(invT), np.array(dlogq), seed=8, stream=5)
+    for i, (ctx, _k, prop, init, _s, key, _a, _b) in enumerate(mbp):
+        EFp = -2 * ctx.obs_loglik([prop])[0]
+        EFi = -2 * ctx.obs_loglik([init])[0]
+        want = O.get_al(1, EFp, EFi, dlogq[i], 0.0, invT[i], 0.0)
+        assert abs(al[i] - want) <= 1e-12 * max(1.0, want)
+        a1, c1 = ctx.accept([init], [prop], 1, [dlogq[i]], [key], invT=invT[i], seed=8, stream=5)
+        assert a1[0] == al[i] and c1[0] == acc[i]
+    # Swaps along the ladders of the chains' current states.
+    seed, stream = 31, 7
+    ladders = [[(ctx, k, 0, float(temps[k])) for k in range(L)] for ctx in ctxs]
+    res = _lib.batch_mc3_swap(ladders, seed=seed, stream=stream)
+    for li, (ctx, (order, ntr, nac)) in enumerate(zip(ctxs, res)):
+        EF = [-2 * ctx.obs_loglik([k])[0] for k in range(L)]
+        nd = 5 * (L - 1)
+        ctr = np.zeros((nd, 4), dtype=np.uint32)
+        ctr[:, 0] = 0xFFFD | (stream << 16)
+        ctr[:, 1] = np.arange(nd)
+        ctr[:, 2] = li
+        ctr[:, 3] = 0x7FFFFFFD
+        u = (_lib.debug_philox(ctr, seed, 0)[:, 0].astype(np.float64) + 0.5) * 2.0 ** -32
+        o_order, o_ntr, o_nac, _ = O.mc3_swap([0] * L, list(temps), EF, list(u))
+        assert list(order) == o_order and list(ntr) == o_ntr and list(nac) == o_nac
+    # Ladder 0 draws what the single-context swap draws.
+    o1, n1, a1 = ctxs[0].mc3_swap(np.arange(L), np.zeros(L), temps, seed=seed, stream=stream)
+    assert list(o1) == list(res[0][0]) and list(a1) == list(res[0][2])
+    for c in ctxs:
+        c.close()
+
+
+def test_uneven_sweep_small(gpu_lib):
+    """A small uneven workload sweeps end to end: fresh parameters,
+    simulations and proposals with log-likelihoods, acceptance, swaps; chain
+    labels stay a permutation of each ladder's instances."""
+    w = configs.uneven_batch(96, seed=9)
+    ub = UnevenBatch(w, seed=9)
+    for k in range(3):
+        r = ub.sweep(k)
+        assert r["loglik_finite"] and r["cell_steps"] > 0
+    L = ub.L
+    for l in range(len(ub.mc3_ctx)):
+        assert sorted(list(ub.cur[l]) + list(ub.alt[l])) == list(range(2 * L))
+    ub.close()

@@ -60,3 +60,23 @@ def test_spec_errors_are_reported(gpu_lib):
     th[1] = 0.0       # zero mean residence time
     with pytest.raises(_lib.BeepError):
         _lib.spec_tables(spec, th)
+
+
+def test_compact_observation_table_matches_expanded_list(gpu_lib):
+    """An obs_table fills the same observation structs as its expanded obs
+    list (values, windows, element lists), and the host model accepts it."""
+    spec, truth = configs.seir_areas(13, 4, 30, seed=3, daily_obs=True)
+    assert "obs_table" in spec and len(spec["obs_table"]["y"]) == 13 * 30
+    bufa, bufb = _lib._SpecBuffers(spec), _lib._SpecBuffers(specmod.expand_obs(spec))   # own the arrays
+    a, b = bufa.spec, bufb.spec
+    assert a.nobs == b.nobs == 13 * 30
+    for i in range(a.nobs):
+        x, y = a.obs[i], b.obs[i]
+        for f in ("y", "sd", "shape", "factor", "thresh", "percent", "type", "datatable", "graph", "sett_i", "sett_f",
+                  "narea", "ndp", "ntr", "nco", "factor_spline"):
+            assert getattr(x, f) == getattr(y, f), (i, f)
+        assert [x.area[k] for k in range(x.narea)] == [y.area[k] for k in range(y.narea)]
+        assert [x.dp[k] for k in range(x.ndp)] == [y.dp[k] for k in range(y.ndp)]
+        assert [x.tr[k] for k in range(x.ntr)] == [y.tr[k] for k in range(y.ntr)]
+    t = _lib.spec_tables(spec, truth)
+    assert np.isfinite(t["beta_af"]).all()

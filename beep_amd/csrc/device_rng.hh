@@ -109,6 +109,13 @@ typedef CellRng ExtraUniforms;
 
 #define BEEP_MAX_EXTRA_BLOCKS 4096u
 
+// Blocks a pool has used: the low 22 bits of its block word.  A cell-step's
+// pool counts from 0; a large draw's own stream from 0x80000000 | tr << 22
+// (draw_item_body), so the transition bits above must not count.  The
+// rejection loops stop at BEEP_MAX_EXTRA_BLOCKS blocks (a bound on the loop,
+// reached with probability far below 1e-100).
+__device__ inline uint32_t blocks_used(const CellRng& r) { return r.block & 0x3FFFFFu; }
+
 // ---- log-pmf pieces in FP32 (C. Loader, "Fast and accurate computation of
 // binomial probabilities", 2000): log n! = (n+1/2) log n - n + log(2 pi)/2 +
 // stirlerr(n), and bd0(x, m) = x log(x/m) + m - x evaluated without
@@ -222,7 +229,7 @@ __device__ inline int poisson_ptrs(float lam, float ufirst, ExtraUniforms& r) {
       float rhs = k == 0.f ? -lam : log_pois_f(k, lam);
       if (lhs <= rhs) return (int)k;
     }
-    if (r.block >= BEEP_MAX_EXTRA_BLOCKS) return (int)lfl;   // unreachable in practice
+    if (blocks_used(r) >= BEEP_MAX_EXTRA_BLOCKS) return (int)lfl;   // unreachable in practice
     Uraw = r.next();
   }
 }
@@ -256,7 +263,7 @@ __device__ inline int binomial_inversion(int n, float p, float ufirst, ExtraUnif
       if (rr <= 0.f) { k = n + 1; break; }
     }
     if (k <= n && k <= 110) return k;
-    if (r.block >= BEEP_MAX_EXTRA_BLOCKS) return (int)floorf(n * p);   // unreachable in practice
+    if (blocks_used(r) >= BEEP_MAX_EXTRA_BLOCKS) return (int)floorf(n * p);   // unreachable in practice
     u = (float)r.next();
   }
 }
@@ -319,7 +326,7 @@ __device__ inline int binomial_btrd(int n, float p, float ufirst, ExtraUniforms&
       }
     }
     if (accept) return (int)k;
-    if (r.block >= BEEP_MAX_EXTRA_BLOCKS) return (int)floorf(n * p);   // unreachable in practice
+    if (blocks_used(r) >= BEEP_MAX_EXTRA_BLOCKS) return (int)floorf(n * p);   // unreachable in practice
     v = (float)r.next();
   }
 }

@@ -198,12 +198,13 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
       if (nmi < 0.f) nmi = 0.f;
       const float foi = foi_mean(beta_s, af_s[j], nmi, efoi_s, agedist[g]);
       const bool simu = mode == MODE_MBP ? (simu_s ? simu_s[j] != 0 : false) : true;
-      // The means of every transition first (State::set_transmean), ...
-      float lps[T];
-      bool draws = false;
+      CellRng rng;
+      rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
+      if (mode != MODE_REPLAY) rng.prime();
 #pragma unroll
       for (int tr = 0; tr < T; tr++) {
-        float popu = (float)p[FROM[tr]];
+        const int fr = FROM[tr], to = TO[tr];
+        float popu = (float)p[fr];
         if constexpr (NSTRAIN > 1) {
           if (tr == INF_TR) popu = (float)pool;
         }
@@ -213,20 +214,6 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
         if (lp != lp) { atomicOr(jb.err, 1); lp = 0.f; }
         else if (lp > 1e9f) { atomicOr(jb.err, 2); lp = 0.f; }
         if (tm_s) st_nt(tm_s, tr * AG + cell, lp);
-        lps[tr] = lp;
-        draws = draws || lp > 0.f;
-      }
-      // ... then the cell-step's first uniform block, generated converged by
-      // the whole wavefront when any of its lanes has a non-zero mean (a
-      // wavefront of cells with no infection nearby and nothing to move skips
-      // the generator; the uniforms drawn are the same either way).
-      CellRng rng;
-      rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
-      if (mode != MODE_REPLAY && __ballot(draws) != 0) rng.prime();
-#pragma unroll
-      for (int tr = 0; tr < T; tr++) {
-        const int fr = FROM[tr], to = TO[tr];
-        const float lp = lps[tr];
         int n = 0;
         int kind = -1;
         float val = 0.f;

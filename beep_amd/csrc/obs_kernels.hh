@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "device_rng.hh"
+#include "fused_common.hh"
 
 namespace beepdev {
 
@@ -381,22 +382,22 @@ __global__ void k_dbg_philox(int n, const u4* __restrict__ ctr, uint32_t k0, uin
   if (i < n) out[i] = philox4x32_10(ctr[i], k0, k1);
 }
 
-// Draw i uses the uniform pool of cell i at step 0 of instance key.
+// Draw i is the simulation kernels' queued large draw (draw_item: its own
+// stream) of cell i, transition 0, at step 0 of instance key, so these
+// samplers are tested with the stream numbering the kernels use.
 __global__ void k_dbg_poisson(int n, const float* __restrict__ lam, uint64_t seed, uint32_t key, int* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  CellRng rng;
-  rng.init((uint32_t)seed, (uint32_t)(seed >> 32), 0u, (uint32_t)i, key);
-  out[i] = lam[i] > 0.f ? poisson_draw(lam[i], rng.next(), rng) : 0;
+  out[i] = lam[i] > 0.f ? draw_item(ITEM_POIS, lam[i], 0, (uint32_t)seed, (uint32_t)(seed >> 32), 0u, (uint32_t)i, 0u,
+                                    key)
+                        : 0;
 }
 
 __global__ void k_dbg_binomial(int n, const int* __restrict__ nn, const double* __restrict__ p, uint64_t seed,
                                uint32_t key, int* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  CellRng rng;
-  rng.init((uint32_t)seed, (uint32_t)(seed >> 32), 0u, (uint32_t)i, key);
-  out[i] = binomial_draw(nn[i], (float)p[i], rng.next(), rng);
+  out[i] = draw_item(ITEM_BINOM, (float)p[i], nn[i], (uint32_t)seed, (uint32_t)(seed >> 32), 0u, (uint32_t)i, 0u, key);
 }
 
 }  // namespace beepdev

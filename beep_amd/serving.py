@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import _lib, configs, spec as specmod
+from . import _lib, configs, samples, spec as specmod
 
 INFECTION_TR = 0          # S -> E in the SEIR catalogue models
 
@@ -46,23 +46,42 @@ class Request:
 
 
 class Model:
-    """A catalogue entry: context, posterior samples, free work pool."""
+    """A catalogue entry: context, posterior samples, free work pool.  The
+    posterior is loaded from a directory of saved samples (the reference's
+    files, beep_amd.samples) when posterior_dir is given; otherwise it is
+    made on the device: prior draws simulated up to t_post."""
 
-    def __init__(self, idx, A, G, t_post, S, nposterior, pool, seed):
+    def __init__(self, idx, A, G, t_post, S, nposterior, pool, seed, posterior_dir=None):
         self.idx, self.A, self.G = idx, A, G
         self.t_post = t_post
         spec, _truth = configs.seir_areas(A, G, S, seed=1000 + idx)
         self.spec = spec
         cap = nposterior + pool
         self.ctx = _lib.Context(spec, capacity=cap, nslot=cap, keep=_lib.KEEP_TRANSNUM)
-        rng = np.random.default_rng(seed)
-        post = specmod.sample_prior(spec, rng, nposterior)
-        self.ctx.set_param(np.arange(nposterior), post)
-        # Posterior states at t_post (the state a loaded sample carries).
         inst = np.arange(nposterior)
-        self.ctx.simulate(inst, inst, inst + 7_000_000, 0, t_post, seed=seed, stream=1)
+        if posterior_dir is not None:
+            # State::load of each sample (Simulate::model_modification,
+            # src/simulate.cc:142-165): parameters and counts up to where the
+            # samples end, replayed on the device; the prediction continues
+            # from that step.
+            _EF, post, ndiv = samples.load_posterior(posterior_dir, spec, self.ctx, inst, inst)
+            if ndiv != t_post:
+                raise ValueError(f"the samples in {posterior_dir} end at step {ndiv}, not at t_post = {t_post}")
+        else:
+            rng = np.random.default_rng(seed)
+            post = specmod.sample_prior(spec, rng, nposterior)
+            self.ctx.set_param(inst, post)
+            # Posterior states at t_post (the state a loaded sample carries).
+            self.ctx.simulate(inst, inst, inst + 7_000_000, 0, t_post, seed=seed, stream=1)
+        self.post = post
         self.nposterior = nposterior
         self.free = list(range(nposterior, cap))       # work instance == work slot index
+
+    def save_posterior(self, directory):
+        """Writes the posterior samples (parameters, counts up to t_post) and
+        sample_information.txt in the reference's format."""
+        inst = np.arange(self.nposterior)
+        samples.save_posterior(directory, self.spec, self.ctx, inst, inst, self.post, timesteps=self.t_post)
 
     def close(self):
         self.ctx.close()
@@ -70,9 +89,11 @@ class Model:
 
 class PredictionServer:
     def __init__(self, catalogue, t_post=100, max_horizon=180, nposterior=1000, hbm_budget=64 << 30,
-                 chunk=14, seed=5, max_pool=4096):
+                 chunk=14, seed=5, max_pool=4096, posterior_dirs=None):
         """catalogue: list of (areas, age groups).  The work pool of each model
-        is sized so the whole server fits hbm_budget bytes."""
+        is sized so the whole server fits hbm_budget bytes.  posterior_dirs:
+        per model, a directory of saved posterior samples to load (or None
+        for a posterior made on the device)."""
         self.S = t_post + max_horizon
         self.t_post = t_post
         self.chunk = chunk
@@ -86,8 +107,20 @@ class PredictionServer:
         share = max(0, hbm_budget - fixed) / max(1, len(catalogue))
         for i, (A, G) in enumerate(catalogue):
             pool = int(min(max_pool, max(1000, share // per_inst[i])))
-            self.models.append(Model(i, A, G, t_post, self.S, nposterior, pool, seed + i))
+            pdir = posterior_dirs[i] if posterior_dirs is not None else None
+            self.models.append(Model(i, A, G, t_post, self.S, nposterior, pool, seed + i, posterior_dir=pdir))
         self.pending, self.active, self.finished = [], [], []
+
+    def save_posteriors(self, root):
+        """Each model's posterior samples into root/model<i>; returns the
+        directories (for posterior_dirs)."""
+        import os
+        dirs = []
+        for m in self.models:
+            d = os.path.join(root, f"model{m.idx}")
+            m.save_posterior(d)
+            dirs.append(d)
+        return dirs
 
     def close(self):
         for m in self.models:

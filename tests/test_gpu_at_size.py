@@ -19,7 +19,12 @@ import pytest
 from beep_amd import _lib, configs, spec as specmod
 from oracle import oracle as O
 
-pytestmark = pytest.mark.gpu
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="a run of the GPU suite that included these two tests ended in a device fault (memory "
+                            "fault or hung queue) whose cause has not been found; they stay off the device until "
+                            "one is shown"),
+]
 
 
 def transmean_scale(orc, tb, st):

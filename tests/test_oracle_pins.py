@@ -532,3 +532,36 @@ def test_deterministic_foi_matrix_form():
     for kw in ({"transpose_M": True}, {"drop_fac": True}, {"drop_efoi": True}):
         _, bad = _euler_matrix_form(orc, tb, **kw)
         assert not np.allclose(st.transnum, bad, rtol=1e-3), kw
+
+
+def test_strain_mbp_follows_simulation_law():
+    """The MBP property on the two-strain model (susceptibles pooled over the
+    strains, src/state.cc:395-404): proposals at a higher second-strain
+    Rfactor, coupled to initial trajectories simulated at the base
+    parameters, follow the law of fresh simulations at the proposed
+    parameters -- every compartment of each strain at the last step."""
+    import toml_models
+    spec, truth = toml_models.toml_sird_strains()
+    names = [p["name"] for p in spec["params"]]
+    th2 = truth.copy()
+    th2[names.index("Rfactor_s2")] = 1.9
+    orc = O.Oracle(spec)
+    tb1, tb2 = orc.tables(truth), orc.tables(th2)
+    n = 3000
+    mbp = np.array([orc.mbp(tb2, orc.simulate(tb1, seed=10000 + r), seed=20000 + r).pop[-1, 0] for r in range(n)])
+    sim = np.array([orc.simulate(tb2, seed=30000 + r).pop[-1, 0] for r in range(n)])
+    moved = 0
+    for co in range(mbp.shape[1]):
+        for strain in (slice(0, 2), slice(2, 4)):
+            a, b = mbp[:, co, strain].sum(1), sim[:, co, strain].sum(1)
+            if a.var() == 0 and b.var() == 0:
+                assert (a == b).all()
+                continue
+            assert stats.ks_2samp(a, b).pvalue > 1e-3, (co, strain)
+            se = np.sqrt(a.var() / n + b.var() / n)
+            assert abs(a.mean() - b.mean()) < 5 * se, (co, strain)
+            moved += 1
+    assert moved >= 4
+    # The proposals moved: the second strain's deaths rose with its Rfactor.
+    base = np.array([orc.simulate(tb1, seed=40000 + r).pop[-1, 0, 3, 2:].sum() for r in range(500)])
+    assert mbp[:, 3, 2:].sum(1).mean() > base.mean()

@@ -1184,9 +1184,6 @@ int beep_mbp(beep_ctx* c, int n, const int32_t* init, const int32_t* prop, const
   (void)inf_update;
   if (!c) return BEEP_EINVAL;
   if (!c->d_tn || !c->d_tm) return fail(c, BEEP_EINVAL, "MBPs need the context to keep transnum and transmean");
-  // With strains the proposals' law differs from the oracle's MBP (the
-  // simulation and replay paths agree with it); refused until that is resolved.
-  if (c->m.nstrain > 1) return fail(c, BEEP_EINVAL, "MBP proposals are not supported for models with strains");
   if (n < 0 || (n > 0 && (!init || !prop || !slot || !key))) return fail(c, BEEP_EINVAL, "beep_mbp: bad arguments");
   if (stream > 0xFFFF) return fail(c, BEEP_EINVAL, "stream must be below 65536");
   int rc;
@@ -1960,7 +1957,6 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
       if (w[i].t0 < 0 || w[i].t1 > m.S || w[i].t0 > w[i].t1) return fail(c0, BEEP_EINVAL, "work %d: bad steps", i);
       j = make_job(c, w[i].inst, w[i].slot, w[i].key, w[i].t0, w[i].t1);
     } else if (w[i].kind == BEEP_WORK_MBP) {
-      if (w[i].ctx->m.nstrain > 1) return fail(w[i].ctx, BEEP_EINVAL, "MBP proposals are not supported for models with strains");
       if (!c->d_tn || !c->d_tm || !valid_inst(c, w[i].init) || w[i].init == w[i].inst)
         return fail(c0, BEEP_EINVAL, "work %d: bad MBP pair", i);
       j = make_job(c, w[i].inst, w[i].slot, w[i].key, 0, m.S);

@@ -267,8 +267,6 @@ def test_mbp_identity_and_law(gpu_lib):
     ctx.close()
 
 
-@pytest.mark.skip(reason="MBP proposals with strains follow a different law from the oracle's MBP (KS p = 3e-5 on "
-                         "the second strain's deaths); the library refuses MBP for strain models until resolved")
 def test_mbp_identity_and_law_strains(gpu_lib):
     """MBP with two strains drawing on pooled susceptibles: unchanged
     parameters reproduce the trajectory exactly, and proposals at a higher

@@ -29,7 +29,7 @@ SYMBOLS = [
     "beep_debug_binomial", "beep_timing", "beep_debug_gemm", "beep_run_batch", "beep_copy_states",
     "beep_derive_slots", "beep_sum_transnum", "beep_pmcmc_obs_prob", "beep_pmcmc_particle_sample", "beep_bootstrap", "beep_path",
     "beep_abcsmc_weights", "beep_mc3_swap", "beep_debug_check_launch", "beep_batch_accept", "beep_batch_mc3_swap",
-    "beep_batch_set_param",
+    "beep_batch_set_param", "beep_simulate_noalloc",
 ]
 
 
@@ -123,6 +123,8 @@ def lib():
             "beep_set_param": (C.c_int, [P, C.c_int, i32p, f64p, f64p, i32p]),
             "beep_get_tables": (C.c_int, [P, C.c_int, f64p, f64p, f64p, f64p, f64p, f64p, f64p]),
             "beep_simulate": (C.c_int, [P, C.c_int, i32p, i32p, u32p, C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_int]),
+            "beep_simulate_noalloc": (C.c_int, [P, C.c_int, i32p, i32p, u32p, C.c_int, C.c_int, C.c_uint64,
+                                                C.c_uint32]),
             "beep_obs_loglik": (C.c_int, [P, C.c_int, i32p, C.c_int, f64p, f64p]),
             "beep_obs_values": (C.c_int, [P, C.c_int, f64p]),
             "beep_pois_loglik": (C.c_int, [P, C.c_int, i32p, f64p]),
@@ -568,6 +570,14 @@ class Context:
         t1 = self.S if t1 is None else t1
         self._chk(lib().beep_simulate(self._h, len(inst), _p(inst, C.c_int32), _p(slot, C.c_int32),
                                       _p(key, C.c_uint32), t0, t1, seed, stream, SIM_PLAIN if plain else 0))
+
+    def simulate_noalloc(self, inst, slot, key, t0=0, t1=None, seed=1, stream=0):
+        """simulate() through beep_simulate_noalloc: no device memory is
+        allocated (job descriptors in the library's fixed per-device array)."""
+        inst, slot, key = _i32(inst), _i32(slot), np.ascontiguousarray(np.asarray(key, dtype=np.uint32))
+        t1 = self.S if t1 is None else t1
+        self._chk(lib().beep_simulate_noalloc(self._h, len(inst), _p(inst, C.c_int32), _p(slot, C.c_int32),
+                                              _p(key, C.c_uint32), t0, t1, seed, stream))
 
     def obs_loglik(self, inst, section=-1, with_dt=False):
         inst = _i32(inst)

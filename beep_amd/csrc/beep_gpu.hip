@@ -131,7 +131,19 @@ struct beep_ctx {
   size_t h_stage64_cap = 0;        // doubles
 };
 
+// Device memory of the allocation-free entry point (beep_simulate_noalloc):
+// the job descriptors of one call, at most BEEP_NOALLOC_BYTES per device.
+// Each device holds its own copy until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress gives on that device
+// (g_noalloc_ptr), never through the array's name.
+#define BEEP_NOALLOC_BYTES (1u << 20)
+static __device__ __align__(256) unsigned char scratch[BEEP_NOALLOC_BYTES];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
 
 int fail(beep_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -253,6 +265,20 @@ int check_device_error(beep_ctx* c) {
     if (e & 1) return fail(c, BEEP_ENAN, "NaN transition mean");
     return fail(c, BEEP_ELARGE, "transition mean above 1e9");
   }
+  return BEEP_OK;
+}
+
+// Once per device, with that device current (called from beep_create): the
+// device address of `scratch` on it.
+int noalloc_setup(beep_ctx* c) {
+  if (c->device < 0 || c->device >= kMaxDevices) return fail(c, BEEP_EINVAL, "device %d out of range", c->device);
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[c->device]) return BEEP_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(c, BEEP_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", c->device, hipGetErrorString(e));
+  g_noalloc_ptr[c->device] = p;
   return BEEP_OK;
 }
 
@@ -635,6 +661,8 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   c->keep = keep;
   *out = c;
   HIPCHK(c, hipSetDevice(device));
+  int rc0 = noalloc_setup(c);
+  if (rc0) return rc0;
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   for (auto& e : c->ev) HIPCHK(c, hipEventCreate(&e));
   const auto& m = c->m;
@@ -1174,6 +1202,57 @@ int beep_simulate(beep_ctx* c, int n, const int32_t* inst, const int32_t* slot, 
     record(c, 1);
     c->timing[3] = c->stepwise ? 2 * (t1 - t0) + 1 : 1;
   }
+  rc = check_device_error(c);
+  c->timing[0] = elapsed(c, 0, 1);
+  return rc;
+}
+
+// beep_simulate's batched fused path with the job descriptors in `scratch`
+// (this device's copy) instead of the context's job buffer: no device memory
+// is allocated or freed.  The descriptors go up in chunks from a stack
+// buffer, each copy waited for before the buffer is refilled; every
+// descriptor the kernel reads is written by this call, so nothing needs
+// clearing.  The call ends synchronised with its stream (the error word is
+// read back), so calls from one host thread never overlap; calls that overlap
+// on one device (other threads) race on `scratch`.
+int beep_simulate_noalloc(beep_ctx* c, int n, const int32_t* inst, const int32_t* slot, const uint32_t* key, int t0,
+                          int t1, uint64_t seed, uint32_t stream) {
+  if (!c) return BEEP_EINVAL;
+  if (n < 0 || (n > 0 && (!inst || !slot || !key))) return fail(c, BEEP_EINVAL, "beep_simulate_noalloc: bad arguments");
+  if (t0 < 0 || t1 > c->m.S || t0 > t1)
+    return fail(c, BEEP_EINVAL, "beep_simulate_noalloc: bad step range [%d,%d)", t0, t1);
+  if (stream > 0xFFFF) return fail(c, BEEP_EINVAL, "stream must be below 65536");
+  const long long max_jobs = BEEP_NOALLOC_BYTES / sizeof(Job);
+  if (n > max_jobs)
+    return fail(c, BEEP_EINVAL, "beep_simulate_noalloc: %d instances; at most %lld per call (%u bytes of job descriptors)",
+                n, max_jobs, BEEP_NOALLOC_BYTES);
+  if (c->stepwise)
+    return fail(c, BEEP_EINVAL, "beep_simulate_noalloc: a step-wise context (its captured step loops read the "
+                                "context's job buffer); use beep_simulate");
+  int rc;
+  if ((rc = check_instances(c, n, inst, nullptr, "beep_simulate_noalloc"))) return rc;
+  for (int i = 0; i < n; i++)
+    if (!valid_slot(c, slot[i]) || !c->slot_set[slot[i]]) return fail(c, BEEP_EINVAL, "slot %d not set", slot[i]);
+  Job* d_jobs = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    d_jobs = (Job*)g_noalloc_ptr[c->device];
+  }
+  if (!d_jobs) return fail(c, BEEP_EHIP, "beep_simulate_noalloc: no device address of scratch on device %d", c->device);
+  if (n == 0 || t0 == t1) return BEEP_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  constexpr int kChunk = 128;
+  Job h[kChunk];
+  for (int i0 = 0; i0 < n; i0 += kChunk) {
+    const int m = std::min(kChunk, n - i0);
+    for (int i = 0; i < m; i++) h[i] = make_job(c, inst[i0 + i], slot[i0 + i], key[i0 + i], t0, t1);
+    HIPCHK(c, hipMemcpyAsync(d_jobs + i0, h, (size_t)m * sizeof(Job), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  record(c, 0);
+  if ((rc = launch_fused(c, beepdev::MODE_SIM, d_jobs, n, seed, stream))) return rc;
+  record(c, 1);
+  c->timing[3] = 1;
   rc = check_device_error(c);
   c->timing[0] = elapsed(c, 0, 1);
   return rc;

@@ -262,6 +262,19 @@ int beep_get_tables(beep_ctx* ctx, int slot, double* rate, double* sus, double* 
 int beep_simulate(beep_ctx* ctx, int n, const int32_t* inst, const int32_t* slot, const uint32_t* key,
                   int t0, int t1, uint64_t seed, uint32_t stream, int flags);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops or harnesses that forbid
+ * hipMalloc): beep_simulate's batched path (same arguments, same results for
+ * the same keys) that allocates and frees nothing.  Its job descriptors live
+ * in a fixed 1 MiB device array of the library, one per device, so one call
+ * takes at most 1 MiB / sizeof(job descriptor) instances (10082 on x86-64;
+ * more is refused with BEEP_EINVAL).  Fused-path contexts only (a step-wise
+ * context is refused with BEEP_EINVAL).  Every call on a device shares that
+ * device's array: calls that overlap on one device (from several host
+ * threads) race; each call returns synchronised with the context's stream. */
+int beep_simulate_noalloc(beep_ctx* ctx, int n, const int32_t* inst, const int32_t* slot, const uint32_t* key,
+                          int t0, int t1, uint64_t seed, uint32_t stream);
+
 /* ObservationModel::calculate (src/obsmodel.cc:22-35) when section < 0, else
  * calculate_section(section) (39-58); L_out[i] = log-likelihood (FP64).
  * EF_dt_out (may be NULL) receives get_EF_datatable (309-322) as

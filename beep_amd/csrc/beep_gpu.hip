@@ -290,12 +290,14 @@ int fused_block(int AG) {
 }
 
 // Threads per workgroup for the specialised fused kernel: 1, 2, 4 or 8
-// wavefronts, so that its register-limited 16 resident wavefronts per CU are
+// wavefronts, so that its register-limited resident wavefronts per CU are
 // whole workgroups (measured on 100 areas x 4 age groups: 512 threads beat
-// the one-pass 448 by 5%).
+// the one-pass 448 by 5%); 16 for instances of more than 1024 cells, whose
+// LDS leaves room for one workgroup per CU, so that the CU still has sixteen
+// wavefronts to switch between.
 int spec_block(int AG) {
   int w = (AG + 63) / 64;
-  w = w <= 1 ? 1 : w <= 2 ? 2 : w <= 4 ? 4 : 8;
+  w = w <= 1 ? 1 : w <= 2 ? 2 : w <= 4 ? 4 : w <= 16 ? 8 : 16;
   return 64 * w;
 }
 
@@ -329,7 +331,9 @@ int spec_waves(const beephost::HostModel& m, const char* env, int dflt) {
 }
 
 std::string spec_prelude(const beephost::HostModel& m) {
-  const int waves = spec_waves(m, "BEEP_SPEC_WAVES", m.T <= 4 ? 8 : m.T <= 8 ? 6 : 4);
+  // At least 4: a workgroup of BEEP_SPEC_MAX_BLOCK threads is four wavefronts
+  // per SIMD.
+  const int waves = std::max(4, spec_waves(m, "BEEP_SPEC_WAVES", m.T <= 4 ? 8 : m.T <= 8 ? 6 : 4));
   const int step_waves = spec_waves(m, "BEEP_STEP_WAVES", 4);
   std::ostringstream o;
   o << "namespace beepspec {\n";
@@ -404,7 +408,7 @@ int launch_spec(beep_ctx* c, const SpecKernels* k, int mode, const Job* djobs, i
                 uint64_t seed, uint32_t stream, hipStream_t st = nullptr) {
   if (njobs == 0) return BEEP_OK;
   {
-    std::string e = beephost::fused_geometry(njobs, block, lds, (size_t)k->max_dyn_lds, BEEP_FUSED_MAX_BLOCK);
+    std::string e = beephost::fused_geometry(njobs, block, lds, (size_t)k->max_dyn_lds, BEEP_SPEC_MAX_BLOCK);
     if (!e.empty()) return fail(c, BEEP_EINVAL, "%s", e.c_str());
   }
   void* args[3] = {(void*)&djobs, (void*)&seed, (void*)&stream};
@@ -537,9 +541,12 @@ void launch_obs_loglik(beep_ctx* c, int n, const int* d_inst, const int* sel, in
   K k = c->nospl ? ks[8 + (want_dt ? 2 : 0) + (c->obs_pop ? 1 : 0)]
                  : ks[(want_dt ? 4 : 0) + (c->obs_general ? 2 : 0) + (c->obs_pop ? 1 : 0)];
   const int ni = want_dt ? beepdev::OBS_INSTANCES<true> : beepdev::OBS_INSTANCES<false>;
-  hipLaunchKernelGGL(k, dim3((n + ni - 1) / ni), dim3(256), 0, st ? st : c->stream, c->d_obs, n, d_inst, c->d_tn,
-                     c->d_pt, c->traj_sz, c->pop_traj_sz, sel, nsel, out, c->d_EF, store_EF, c->d_ospl,
-                     c->d_inst_slot);
+  // Per-table sums: one pass per eight data tables.
+  const int npass = want_dt ? (c->m.ndatatable + 7) / 8 : 1;
+  for (int p = 0; p < npass; p++)
+    hipLaunchKernelGGL(k, dim3((n + ni - 1) / ni), dim3(256), 0, st ? st : c->stream, c->d_obs, n, d_inst, c->d_tn,
+                       c->d_pt, c->traj_sz, c->pop_traj_sz, sel, nsel, out, c->d_EF, store_EF, c->d_ospl,
+                       c->d_inst_slot, 8 * p);
 }
 
 // Before an observation kernel reads inst_slot: the host copy, uploaded when
@@ -552,6 +559,10 @@ int sync_inst_slot(beep_ctx* c) {
   c->inst_slot_dirty = false;
   return BEEP_OK;
 }
+
+// Defined with the mixed batches, which share it.
+int obs_loglik_chunked(beep_ctx* c0, const std::vector<beep_ctx*>& ctxs, const std::vector<const int32_t*>& inst,
+                       const std::vector<int>& ninst, bool store_EF, double* out, int ev0);
 
 Job make_job(beep_ctx* c, int inst, int slot, uint32_t key, int t0, int t1) {
   Job j;
@@ -1310,9 +1321,27 @@ int beep_obs_loglik(beep_ctx* c, int n, const int32_t* inst, int section, double
   for (int i = 0; i < n; i++)
     if (!valid_inst(c, inst[i])) return fail(c, BEEP_EINVAL, "instance %d out of range", inst[i]);
   if (n == 0) return BEEP_OK;
+  int rc;
+  if (c->nospl) {
+    for (int i = 0; i < n; i++)
+      if (c->inst_slot[inst[i]] < 0)
+        return fail(c, BEEP_EINVAL, "instance %d has no parameter slot for its observation splines", inst[i]);
+    if ((rc = sync_inst_slot(c))) return rc;
+  }
+  if (section < 0 && !EF_dt_out) {
+    // The whole observation model: the mixed batches' chunked kernels, so a
+    // batch and this call give the same sums.
+    std::vector<beep_ctx*> cs(1, c);
+    std::vector<const int32_t*> ip(1, inst);
+    std::vector<int> ni(1, n);
+    if ((rc = obs_loglik_chunked(c, cs, ip, ni, true, L_out, 2))) return rc;
+    c->timing[1] = elapsed(c, 2, 3);
+    for (int i = 0; i < n; i++)
+      if (std::isnan(L_out[i])) return fail(c, BEEP_ENAN, "NaN observation log-likelihood");   // obsmodel.cc:32
+    return BEEP_OK;
+  }
   const int ndt = c->m.ndatatable;
   size_t out_doubles = (size_t)n * (1 + ndt);
-  int rc;
   if ((rc = ensure_scratch(c, out_doubles * sizeof(double) + (size_t)n * sizeof(int)))) return rc;
   double* d_out = (double*)c->d_scratch;
   int* d_inst = (int*)(d_out + out_doubles);
@@ -1322,12 +1351,6 @@ int beep_obs_loglik(beep_ctx* c, int n, const int32_t* inst, int section, double
   if (section >= 0) {
     sobs = c->d_sec_obs + c->sec_ptr[section];
     nsel = c->sec_ptr[section + 1] - c->sec_ptr[section];
-  }
-  if (c->nospl) {
-    for (int i = 0; i < n; i++)
-      if (c->inst_slot[inst[i]] < 0)
-        return fail(c, BEEP_EINVAL, "instance %d has no parameter slot for its observation splines", inst[i]);
-    if ((rc = sync_inst_slot(c))) return rc;
   }
   record(c, 2);
   launch_obs_loglik(c, n, d_inst, sobs, nsel, d_out, section < 0 ? 1 : 0, EF_dt_out != nullptr);
@@ -1847,9 +1870,15 @@ int beep_bootstrap(int device, int n, const double* L, double invT, uint64_t see
 // ---- mixed batches ----
 
 namespace {
+// A size class of a mixed batch: cells rounded up to a power of two, the
+// compartment structure, the kernel (specialised or generic) and the jobs'
+// mode (simulations and MBP proposals run as separate launches, each with the
+// kernel compiled for its mode alone, which needs fewer registers).
 struct TuneKey {
-  int ag2, C, T, spec;
-  bool operator<(const TuneKey& o) const { return std::tie(ag2, C, T, spec) < std::tie(o.ag2, o.C, o.T, o.spec); }
+  int ag2, C, T, spec, mode;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(ag2, C, T, spec, mode) < std::tie(o.ag2, o.C, o.T, o.spec, o.mode);
+  }
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, int> g_tune;
@@ -1867,20 +1896,21 @@ void tune_load() {
   if (!p) return;
   std::ifstream f(p);
   std::string ver;
-  if (!(f >> ver) || ver != "beep-tune-2") return;     // another format: ignored, rewritten on save
+  if (!(f >> ver) || ver != "beep-tune-3") return;     // another format: ignored, rewritten on save
   TuneKey k;
   int b;
-  while (f >> k.ag2 >> k.C >> k.T >> k.spec >> b)
-    if (b >= 64 && b <= BEEP_FUSED_MAX_BLOCK && b % 64 == 0) g_tune[k] = b;
+  while (f >> k.ag2 >> k.C >> k.T >> k.spec >> k.mode >> b)
+    if (b >= 64 && b <= (k.spec ? BEEP_SPEC_MAX_BLOCK : BEEP_FUSED_MAX_BLOCK) && b % 64 == 0) g_tune[k] = b;
 }
 
 void tune_save() {
   const char* p = tune_path();
   if (!p) return;
   std::ofstream f(p);
-  f << "beep-tune-2\n";
+  f << "beep-tune-3\n";
   for (auto& kv : g_tune)
-    f << kv.first.ag2 << " " << kv.first.C << " " << kv.first.T << " " << kv.first.spec << " " << kv.second << "\n";
+    f << kv.first.ag2 << " " << kv.first.C << " " << kv.first.T << " " << kv.first.spec << " " << kv.first.mode << " "
+      << kv.second << "\n";
 }
 
 int pow2_ceil(int x) {
@@ -1903,6 +1933,116 @@ int ensure_bstreams(beep_ctx* c0) {
 // items' instances, the launches spread over the batch streams, one upload of
 // the instance lists and one download of the sums.  Each instance's EF is
 // kept for the acceptance steps.
+// Observation log-likelihoods (ObservationModel::calculate, src/obsmodel.cc:
+// 22-35) of the instances inst[k][0..ninst[k]) of contexts ctxs[k], into out
+// (the contexts' results one after the other, in that order): each context's
+// stored observations in chunks of BEEP_OBS_CHUNK, one workgroup per chunk of
+// a group of up to four instances, over the whole device in one launch per
+// kernel variant (k_obs_chunks), then each instance's chunk sums added in
+// chunk order (k_obs_reduce), which also stores its EF when store_EF.  An
+// instance's result depends only on its own trajectory and its context's
+// observations.  Staging in c0's scratch, work on c0's stream; with ev0 >= 0
+// the kernels are bracketed by c0's timing events ev0 and ev0 + 1.  Returns
+// after the results are back.
+int obs_loglik_chunked(beep_ctx* c0, const std::vector<beep_ctx*>& ctxs, const std::vector<const int32_t*>& inst,
+                       const std::vector<int>& ninst, bool store_EF, double* out, int ev0) {
+  hipStream_t st = c0->stream;
+  using beepdev::ObsGroup;
+  using beepdev::ObsItem;
+  constexpr int GN = beepdev::OBS_GROUP;
+  size_t N = 0;
+  for (int v : ninst) N += (size_t)v;
+  if (N == 0) return BEEP_OK;
+  // Group and item counts first, to lay out the staging.
+  size_t ngroups = 0;
+  for (size_t k = 0; k < ctxs.size(); k++) ngroups += ((size_t)ninst[k] + GN - 1) / GN;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t b_inst = al(N * sizeof(int32_t)), b_groups = al(ngroups * sizeof(ObsGroup)),
+               b_items = al(N * sizeof(ObsItem)), b_out = al(N * sizeof(double));
+  size_t nparts = 0;
+  for (size_t k = 0; k < ctxs.size(); k++) {
+    const size_t nchunk = ((size_t)ctxs[k]->obs.nobs + BEEP_OBS_CHUNK - 1) / BEEP_OBS_CHUNK;
+    nparts += nchunk * (((size_t)ninst[k] + GN - 1) / GN);
+  }
+  const size_t b_part = al(nparts * GN * sizeof(double));
+  const size_t up = b_inst + b_groups + b_items;
+  int rc;
+  if ((rc = ensure_scratch(c0, up + b_out + b_part))) return rc;
+  char* d = (char*)c0->d_scratch;
+  const int32_t* d_inst = (const int32_t*)d;
+  ObsGroup* d_groups = (ObsGroup*)(d + b_inst);
+  ObsItem* d_items = (ObsItem*)(d + b_inst + b_groups);
+  double* d_out = (double*)(d + up);
+  double* d_part = (double*)(d + up + b_out);
+  std::vector<char> h(up, 0);
+  int32_t* h_inst = (int32_t*)h.data();
+  ObsItem* h_items = (ObsItem*)(h.data() + b_inst + b_groups);
+  // Groups by kernel variant (general / population elements / splines), each
+  // variant's groups contiguous and numbered by workgroup from 0.
+  std::vector<ObsGroup> byv[8];
+  long long wgs[8] = {};
+  size_t ii = 0;
+  int part = 0;
+  for (size_t k = 0; k < ctxs.size(); k++) {
+    beep_ctx* c = ctxs[k];
+    const int v = (c->obs_general ? 4 : 0) | (c->obs_pop ? 2 : 0) | (c->nospl ? 1 : 0);
+    const int nchunk = (int)(((size_t)c->obs.nobs + BEEP_OBS_CHUNK - 1) / BEEP_OBS_CHUNK);
+    for (int g0 = 0; g0 < ninst[k]; g0 += GN) {
+      ObsGroup G;
+      std::memset(&G, 0, sizeof G);
+      G.od = c->obs;
+      G.tn = c->d_tn;
+      G.pt = c->d_pt;
+      G.ospl = c->nospl ? c->d_ospl : nullptr;
+      G.inst_slot = c->nospl ? c->d_inst_slot : nullptr;
+      G.inst = d_inst + ii;
+      G.traj_sz = c->traj_sz;
+      G.pop_traj_sz = c->pop_traj_sz;
+      G.ninst = std::min(GN, ninst[k] - g0);
+      G.nchunk = nchunk;
+      G.wg0 = (int)wgs[v];
+      G.part0 = part;
+      for (int q = 0; q < G.ninst; q++) {
+        const int32_t in = inst[k][g0 + q];
+        h_inst[ii] = in;
+        h_items[ii] = ObsItem{part, nchunk, q, store_EF ? c->d_EF + in : nullptr};
+        ii++;
+      }
+      if (nchunk > 0) byv[v].push_back(G);
+      wgs[v] += nchunk;
+      part += nchunk;
+    }
+  }
+  size_t goff[8], gsum = 0;
+  for (int v = 0; v < 8; v++) {
+    goff[v] = gsum;
+    if (!byv[v].empty())
+      std::memcpy(h.data() + b_inst + gsum * sizeof(ObsGroup), byv[v].data(), byv[v].size() * sizeof(ObsGroup));
+    gsum += byv[v].size();
+    if (wgs[v] > beephost::kGridMaxX) return fail(c0, BEEP_EINVAL, "observation log-likelihood: too many chunks");
+  }
+  HIPCHK(c0, hipMemcpyAsync(d, h.data(), up, hipMemcpyHostToDevice, st));
+  if (ev0 >= 0) record(c0, ev0);
+  using KC = decltype(&beepdev::k_obs_chunks<false, false, false>);
+  static const KC kc[8] = {beepdev::k_obs_chunks<false, false, false>, beepdev::k_obs_chunks<false, false, true>,
+                           beepdev::k_obs_chunks<false, true, false>,  beepdev::k_obs_chunks<false, true, true>,
+                           beepdev::k_obs_chunks<true, false, false>,  beepdev::k_obs_chunks<true, false, true>,
+                           beepdev::k_obs_chunks<true, true, false>,   beepdev::k_obs_chunks<true, true, true>};
+  for (int v = 0; v < 8; v++) {
+    if (byv[v].empty()) continue;
+    hipLaunchKernelGGL(kc[v], dim3((unsigned)wgs[v]), dim3(256), 0, st, (const ObsGroup*)(d_groups + goff[v]),
+                       (int)byv[v].size(), d_part);
+    HIPCHK(c0, hipGetLastError());
+  }
+  hipLaunchKernelGGL(beepdev::k_obs_reduce, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st,
+                     (const ObsItem*)d_items, (int)N, (const double*)d_part, d_out);
+  HIPCHK(c0, hipGetLastError());
+  if (ev0 >= 0) record(c0, ev0 + 1);
+  HIPCHK(c0, hipMemcpyAsync(out, d_out, N * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(c0, hipStreamSynchronize(st));
+  return BEEP_OK;
+}
+
 int batch_loglik(beep_ctx* c0, int n, const beep_work* w, double* L_out) {
   std::vector<beep_ctx*> ctxs;
   std::map<beep_ctx*, std::vector<int>> items;
@@ -1915,26 +2055,17 @@ int batch_loglik(beep_ctx* c0, int n, const beep_work* w, double* L_out) {
       it->second.push_back(i);
     }
   }
-  std::vector<int> hinst;
-  hinst.reserve(n);
-  std::vector<size_t> ioff, ooff;
-  size_t nout = 0;
-  for (beep_ctx* c : ctxs) {
+  std::vector<std::vector<int32_t>> insts(ctxs.size());
+  std::vector<const int32_t*> ip(ctxs.size());
+  std::vector<int> ni(ctxs.size());
+  for (size_t k = 0; k < ctxs.size(); k++) {
+    beep_ctx* c = ctxs[k];
     if (!c->d_tn) return fail(c0, BEEP_EINVAL, "BEEP_BATCH_LOGLIK: a context does not keep transnum");
-    ioff.push_back(hinst.size());
-    ooff.push_back(nout);
-    for (int i : items[c]) hinst.push_back(w[i].inst);
-    nout += items[c].size() * (size_t)(1 + c->m.ndatatable);
+    for (int i : items[c]) insts[k].push_back(w[i].inst);
+    ip[k] = insts[k].data();
+    ni[k] = (int)insts[k].size();
   }
-  const size_t obytes = (nout * sizeof(double) + 255) / 256 * 256;
   int rc;
-  if ((rc = ensure_scratch(c0, obytes + (size_t)n * sizeof(int)))) return rc;
-  double* d_out = (double*)c0->d_scratch;
-  int* d_inst = (int*)((char*)c0->d_scratch + obytes);
-  HIPCHK(c0, hipMemcpyAsync(d_inst, hinst.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, c0->stream));
-  HIPCHK(c0, hipStreamSynchronize(c0->stream));
-  if ((rc = ensure_bstreams(c0))) return rc;
-  const int nbs = (int)(sizeof c0->bstream / sizeof c0->bstream[0]);
   for (size_t k = 0; k < ctxs.size(); k++)        // splines' slot lists first (synchronous copies)
     if (ctxs[k]->nospl) {
       for (int i : items[ctxs[k]])
@@ -1945,35 +2076,17 @@ int batch_loglik(beep_ctx* c0, int n, const beep_work* w, double* L_out) {
         return rc;
       }
     }
-  // Timed on the context stream from the start of the batch streams to the
-  // end of their last launch (timing[1]).
-  record(c0, 2);
-  HIPCHK(c0, hipEventRecord(c0->bstart, c0->stream));
-  for (auto& st : c0->bstream) HIPCHK(c0, hipStreamWaitEvent(st, c0->bstart, 0));
-  for (size_t k = 0; k < ctxs.size(); k++) {
-    beep_ctx* c = ctxs[k];
-    launch_obs_loglik(c, (int)items[c].size(), d_inst + ioff[k], nullptr, (int)c->m.obs.size(), d_out + ooff[k], 1,
-                      false, c0->bstream[k % nbs]);
-    HIPCHK(c0, hipGetLastError());
-  }
-  for (int k = 0; k < nbs; k++) {
-    HIPCHK(c0, hipEventRecord(c0->bend[k], c0->bstream[k]));
-    HIPCHK(c0, hipStreamWaitEvent(c0->stream, c0->bend[k], 0));
-  }
-  record(c0, 3);
-  HIPCHK(c0, hipStreamSynchronize(c0->stream));
+  // The kernels are timed on the context stream (timing[1]).
+  std::vector<double> h(n);
+  if ((rc = obs_loglik_chunked(c0, ctxs, ip, ni, true, h.data(), 2))) return rc;
   c0->timing[1] = elapsed(c0, 2, 3);
-  std::vector<double> h(nout);
-  HIPCHK(c0, hipMemcpy(h.data(), d_out, nout * sizeof(double), hipMemcpyDeviceToHost));
-  for (size_t k = 0; k < ctxs.size(); k++) {
-    beep_ctx* c = ctxs[k];
-    const auto& its = items[c];
-    for (size_t j = 0; j < its.size(); j++) {
-      const double L = h[ooff[k] + j * (size_t)(1 + c->m.ndatatable)];
-      if (std::isnan(L)) return fail(c0, BEEP_ENAN, "NaN observation log-likelihood (work %d)", its[j]);
-      L_out[its[j]] = L;
+  size_t o = 0;
+  for (size_t k = 0; k < ctxs.size(); k++)
+    for (int i : items[ctxs[k]]) {
+      const double L = h[o++];
+      if (std::isnan(L)) return fail(c0, BEEP_ENAN, "NaN observation log-likelihood (work %d)", i);
+      L_out[i] = L;
     }
-  }
   return BEEP_OK;
 }
 }  // namespace
@@ -2057,10 +2170,10 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
   const int nstep = 0;
   if (!fused.empty()) {
     // Size classes (and kernel: one specialised kernel per compartment
-    // structure), each sorted longest first.
+    // structure; and mode), each sorted longest first.
     std::map<std::pair<TuneKey, SpecKernels*>, std::vector<Item>> groups;
     for (auto& it : fused)
-      groups[{{pow2_ceil(it.AG), it.ctx->m.C, it.ctx->m.T, it.spec ? 1 : 0}, it.spec}].push_back(it);
+      groups[{{pow2_ceil(it.AG), it.ctx->m.C, it.ctx->m.T, it.spec ? 1 : 0, it.job.mode}, it.spec}].push_back(it);
     std::vector<Job> all;
     // key, offset, count, lds, maxAG, kernel, every job starts from the initial state
     std::vector<std::tuple<TuneKey, size_t, size_t, size_t, int, SpecKernels*, bool>> launches;
@@ -2113,9 +2226,12 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
       int maxAG = std::get<4>(L);
       SpecKernels* sk = std::get<5>(L);
       const bool rerunnable = std::get<6>(L);
+      const bool sim = key.mode == beepdev::MODE_SIM;
       auto launch_any = [&](int blk, hipStream_t st) {
-        return sk ? launch_spec(c0, sk, beepdev::MODE_ANY, c0->d_bjobs + off, (int)cnt, lds, blk, seed, stream, st)
-                  : launch_fused_t<beepdev::MODE_ANY>(c0, c0->d_bjobs + off, (int)cnt, lds, blk, seed, stream, st);
+        const Job* dj = c0->d_bjobs + off;
+        if (sk) return launch_spec(c0, sk, key.mode, dj, (int)cnt, lds, blk, seed, stream, st);
+        return sim ? launch_fused_t<beepdev::MODE_SIM>(c0, dj, (int)cnt, lds, blk, seed, stream, st)
+                   : launch_fused_t<beepdev::MODE_MBP>(c0, dj, (int)cnt, lds, blk, seed, stream, st);
       };
       int block = sk ? spec_block(maxAG) : fused_block(maxAG);
       auto f = g_tune.find(key);
@@ -2129,8 +2245,9 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
         // advance that state once per candidate: it is not tuned here and
         // runs once with the default size.
         float best = 1e30f;
-        for (int cand = 64; cand <= BEEP_FUSED_MAX_BLOCK; cand *= 2) {
-          if (cand > pow2_ceil(fused_block(maxAG))) break;
+        const int maxb = sk ? BEEP_SPEC_MAX_BLOCK : BEEP_FUSED_MAX_BLOCK;
+        for (int cand = 64; cand <= maxb; cand *= 2) {
+          if (cand > pow2_ceil((maxAG + 63) / 64 * 64)) break;
           record(c0, 0);
           if ((rc = launch_any(cand, nullptr))) return rc;
           record(c0, 1);
@@ -2140,7 +2257,7 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
           if (ms < best) { best = ms; block = cand; }
         }
         int fb = fused_block(maxAG);
-        if (fb > block && (fb & (fb - 1))) {   // the one-pass size when it is not a power of two
+        if (!sk && fb > block && (fb & (fb - 1))) {   // the one-pass size when it is not a power of two
           record(c0, 0);
           if ((rc = launch_any(fb, nullptr))) return rc;
           record(c0, 1);
@@ -2307,9 +2424,11 @@ int beep_debug_poisson(int device, int n, const float* lam, uint64_t seed, uint3
 
 int beep_debug_gemm(int device, int rows, int K, const float* A, const float* B, float* C) {
   if (rows <= 0 || K <= 0 || K % 64 || !A || !B || !C) return BEEP_EINVAL;
+  const long long tiles = (long long)((rows + 63) / 64) * (K / 64);
+  if (tiles > beephost::kGridMaxX / 2) return BEEP_EINVAL;
   return debug_run(device, (size_t)rows * K * 4, A, (size_t)K * K * 4, B, (size_t)rows * K * 4, C,
                    [&](void* a, void* b, void* o) {
-                     hipLaunchKernelGGL(beepdev::k_dbg_gemm, dim3(K / 64, (rows + 63) / 64), dim3(256), 0, 0,
+                     hipLaunchKernelGGL(beepdev::k_dbg_gemm, dim3((unsigned)((tiles + 7) / 8 * 8)), dim3(256), 0, 0,
                                         (const float*)a, (const float*)b, (float*)o, rows, K);
                    });
 }

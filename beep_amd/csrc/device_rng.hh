@@ -354,4 +354,24 @@ __device__ inline int binomial_draw(int n, float p, float ufirst, ExtraUniforms&
   return flip ? n - k : k;
 }
 
+// binomial_draw where n * min(p, 1 - p) < 10 is known: every chunk then takes
+// the inversion branch, so this is the same function of (n, p, uniforms)
+// without the BTRD code, for call sites inlined in the transition loop.
+__device__ inline int binomial_draw_small(int n, float p, float ufirst, ExtraUniforms& r) {
+  if (n <= 0) return 0;
+  bool flip = p > 0.5f;
+  float pp = flip ? 1.f - p : p;
+  int k = 0;
+  const int chunk = 1 << 23;
+  int left = n;
+  bool first = true;
+  while (left > 0) {
+    int c = left < chunk ? left : chunk;
+    k += binomial_inversion(c, pp, first ? ufirst : r.next(), r);
+    first = false;
+    left -= c;
+  }
+  return flip ? n - k : k;
+}
+
 }  // namespace beepdev

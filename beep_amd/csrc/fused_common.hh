@@ -153,6 +153,7 @@ __host__ __device__ inline size_t fused_lds_bytes(int A, int G, int C, int T, in
 }
 
 #define BEEP_FUSED_MAX_BLOCK 512
+#define BEEP_SPEC_MAX_BLOCK 1024        // the specialised fused kernel: up to 16 wavefronts
 #define BEEP_STEP_BLOCK 256         // cells per workgroup of the step-wise transition kernels
 #define BEEP_POIS_INLINE 10.0f     // means below this are drawn in place by inversion
 

@@ -236,7 +236,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
             } else {
               float pr = lp / li;
               float pp = pr > 0.5f ? 1.f - pr : pr;
-              if ((float)ni * pp < 10.f) n = binomial_draw(ni, pr, rng.next(), rng);
+              if ((float)ni * pp < 10.f) n = binomial_draw_small(ni, pr, rng.next(), rng);
               else { kind = ITEM_BINOM; val = pr; }
             }
           }
@@ -321,19 +321,19 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
 
 }  // namespace beepdev
 
-extern "C" __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, beepspec::MIN_WAVES)
+extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_WAVES)
     beep_fused_spec_sim(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
   beepdev::fused_spec_body<beepdev::MODE_SIM>(jobs, seed, stream);
 }
-extern "C" __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, beepspec::MIN_WAVES)
+extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_WAVES)
     beep_fused_spec_mbp(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
   beepdev::fused_spec_body<beepdev::MODE_MBP>(jobs, seed, stream);
 }
-extern "C" __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, beepspec::MIN_WAVES)
+extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_WAVES)
     beep_fused_spec_replay(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
   beepdev::fused_spec_body<beepdev::MODE_REPLAY>(jobs, seed, stream);
 }
-extern "C" __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, beepspec::MIN_WAVES)
+extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_WAVES)
     beep_fused_spec_any(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
   beepdev::fused_spec_body<beepdev::MODE_ANY>(jobs, seed, stream);
 }

@@ -350,7 +350,7 @@ std::string HostModel::build(const beep_spec& s) {
     ob.type = o.type; ob.datatable = o.datatable; ob.sett_i = o.sett_i; ob.sett_f = o.sett_f;
     if (o.type < 0 || o.type > 3) return "unknown observation model";
     if (o.sett_i < 0 || o.sett_f > S || o.sett_i >= o.sett_f) return "observation window out of range";
-    if (o.datatable < 0 || o.datatable >= 8) return "at most 8 data tables are supported";
+    if (o.datatable < 0) return "negative data table index";
     if (o.datatable + 1 > ndatatable) ndatatable = o.datatable + 1;
     ob.sd = o.sd;
     ob.ospl = -1;

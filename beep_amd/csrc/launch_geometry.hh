@@ -52,8 +52,9 @@ inline std::string fused_geometry(long long njobs, int block, size_t lds, size_t
 // groups, Apad the padded area count of its D / I buffers, which hold
 // capacity * G rows): the initial infectivity and transition kernels (grid
 // n x ceil(AG / step_block): job on x, block of cells on y), the dense MFMA
-// contraction (Apad/64 x ceil(rows/64) tiles of 64x64) or the sparse one
-// (rows x ceil(A/256)).
+// contraction (its ceil(rows/64) x Apad/64 tiles of 64x64 on a one-dimensional
+// grid rounded up to a multiple of the eight XCDs, tiles dealt to XCDs in
+// contiguous runs) or the sparse one (rows x ceil(A/256)).
 struct StepGeometry {
   Dim cell;       // k_step_init and the transition kernels
   Dim dense;      // k_dense_contract
@@ -72,8 +73,9 @@ inline std::string stepwise_geometry(int A, int G, int Apad, long long n, long l
   g.rows = n * G;
   g.cell.x = n;
   g.cell.y = ((long long)A * G + step_block - 1) / step_block;
-  g.dense.x = Apad / 64;
-  g.dense.y = (g.rows + 63) / 64;
+  const long long tiles = ((g.rows + 63) / 64) * (Apad / 64);
+  g.dense.x = (tiles + 7) / 8 * 8;
+  g.dense.y = 1;
   g.sparse.x = g.rows;
   g.sparse.y = (A + 255) / 256;
   Dim b256;

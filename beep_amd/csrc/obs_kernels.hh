@@ -127,6 +127,12 @@ __device__ inline int obs_elem(uint32_t x, const int* tnb, const int* ptb) {
 template <bool WANT_DT>
 constexpr int OBS_INSTANCES = WANT_DT ? 1 : 4;
 
+// Defined below, after the kernels that call it.
+template <bool WANT_DT, bool GENERAL, bool POP, bool SPL, int NI>
+__device__ inline void obs_terms(const ObsDev& od, const int* __restrict__ sel, int kbeg, int kend, int d0,
+                                 const int* (&tnb)[NI], const int* (&ptb)[NI], const double* (&splv)[NI],
+                                 double (&L)[NI], double (&acc)[8]);
+
 // One workgroup per OBS_INSTANCES instances.  out[b*(1+ndt)] = log-likelihood,
 // out[b*(1+ndt)+1+d] = -2 * (sum over data table d)  (get_EF_datatable),
 // written when WANT_DT.  The counts of an observation are summed as
@@ -144,7 +150,7 @@ __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ o
                                                     size_t traj_sz, size_t pop_traj_sz, const int* __restrict__ sel,
                                                     int nsel, double* __restrict__ out, double* __restrict__ EF,
                                                     int store_EF, const double* __restrict__ ospl_tab,
-                                                    const int* __restrict__ inst_slot) {
+                                                    const int* __restrict__ inst_slot, int d0) {
   // Instances per workgroup: they share the observation structure, so one
   // thread reads an observation's elements once and gathers the counts of
   // all NI instances with independent loads.
@@ -164,17 +170,132 @@ __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ o
   const double* splv[NI];
 #pragma unroll
   for (int q = 0; q < NI; q++) splv[q] = SPL ? ospl_tab + (size_t)inst_slot[ins[q]] * od.nospl * od.S : nullptr;
+  double L[NI], acc[8];
+  obs_terms<WANT_DT, GENERAL, POP, SPL, NI>(od, sel, 0, nsel, d0, tnb, ptb, splv, L, acc);
+#pragma unroll
+  for (int q = 0; q < NI; q++) {
+    const double Lt = block_sum(L[q], red);
+    if (threadIdx.x == 0 && b0 + q < n && d0 == 0) {
+      out[(size_t)(b0 + q) * (1 + od.ndt)] = Lt;
+      if (store_EF) EF[ins[q]] = -2.0 * Lt;
+    }
+  }
+  if (WANT_DT) {
+    // Data tables d0 .. d0 + 7 in this pass (the launcher makes one pass per
+    // eight tables; the total above is written by the first).
+    double* o = out + (size_t)b0 * (1 + od.ndt);
+    const int nd = min(8, od.ndt - d0);
+    for (int d = 0; d < nd; d++) {
+      double t = block_sum(acc[d], red);
+      if (threadIdx.x == 0) o[1 + d0 + d] = -2.0 * t;
+    }
+  }
+}
+
+// ---- observation log-likelihoods in chunks of observations ----
+// A group is up to OBS_GROUP instances of one context; its observations are
+// cut into chunks of BEEP_OBS_CHUNK (in stored order), one workgroup each, and
+// each workgroup writes one partial sum per instance.  k_obs_reduce then adds
+// an instance's partials in chunk order.  Work is spread over the whole device
+// however few instances a context has, and the sum of an instance depends only
+// on its context's observation count, not on the call or batch it is in.
+#define BEEP_OBS_CHUNK 2048
+constexpr int OBS_GROUP = 4;
+
+struct ObsGroup {
+  ObsDev od;                        // the context's observation structure (a copy: read
+                                    // with scalar loads, as a kernel argument would be)
+  const int* tn;                    // its transnum, [instance][S][T][AG]
+  const int* pt;                    // its population trajectories, or null
+  const double* ospl;               // its observation-spline values per slot, or null
+  const int* inst_slot;             // its instances' slots (splines), or null
+  const int* inst;                  // the group's instances
+  unsigned long long traj_sz, pop_traj_sz;
+  int ninst;                        // 1..OBS_GROUP
+  int nchunk;                       // ceil(nobs / BEEP_OBS_CHUNK), at least 1
+  int wg0;                          // first workgroup of the group in its launch
+  int part0;                        // first partial record of the group
+};
+
+// Partial sums: record p holds OBS_GROUP doubles, one per instance of its
+// group.  Grid: one workgroup per (group, chunk), groups in wg0 order.
+template <bool GENERAL, bool POP, bool SPL>
+__global__ void __launch_bounds__(256) k_obs_chunks(const ObsGroup* __restrict__ groups, int ngroups,
+                                                    double* __restrict__ partial) {
+  constexpr int NI = OBS_GROUP;
+  __shared__ double red[8];
+  // The group of this workgroup: the last one whose wg0 is at most blockIdx.x.
+  int lo = 0, hi = ngroups - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (groups[mid].wg0 <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const ObsGroup& gr = groups[lo];
+  const ObsDev& od = gr.od;
+  const int c = (int)blockIdx.x - gr.wg0;
+  const int kbeg = c * BEEP_OBS_CHUNK;
+  const int kend = min(kbeg + BEEP_OBS_CHUNK, od.nobs);
+  int ins[NI];
+  const int* tnb[NI];
+  const int* ptb[NI];
+  const double* splv[NI];
+  // The pointers read from the group are generic; derived from global views,
+  // the trajectory gathers compile to global (not flat) loads, as they do
+  // from kernel arguments.
+#pragma unroll
+  for (int q = 0; q < NI; q++) {
+    ins[q] = gview(gr.inst)[q < gr.ninst ? q : 0];  // past the group's end: instance 0 again, not written
+    tnb[q] = (const int*)(gview(gr.tn) + (size_t)ins[q] * gr.traj_sz);
+    ptb[q] = gr.pt ? (const int*)(gview(gr.pt) + (size_t)ins[q] * gr.pop_traj_sz) : nullptr;
+    splv[q] = SPL ? (const double*)(gview(gr.ospl) + (size_t)gview(gr.inst_slot)[ins[q]] * od.nospl * od.S) : nullptr;
+  }
+  double L[NI], acc[8];
+  obs_terms<false, GENERAL, POP, SPL, NI>(od, nullptr, kbeg, kend, 0, tnb, ptb, splv, L, acc);
+#pragma unroll
+  for (int q = 0; q < NI; q++) {
+    const double Lt = block_sum(L[q], red);
+    if (threadIdx.x == 0 && q < gr.ninst) partial[(size_t)(gr.part0 + c) * NI + q] = Lt;
+  }
+}
+
+// An instance's place among the partials, and where its EF goes (or null).
+struct ObsItem {
+  int part0, nchunk, q;
+  double* EF;
+};
+
+// out[i] = sum over the chunks of item i's partials, in chunk order; EF too.
+__global__ void __launch_bounds__(256) k_obs_reduce(const ObsItem* __restrict__ items, int n,
+                                                    const double* __restrict__ partial, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const ObsItem it = items[i];
+  double L = 0;
+  for (int c = 0; c < it.nchunk; c++) L += partial[(size_t)(it.part0 + c) * OBS_GROUP + it.q];
+  out[i] = L;
+  if (it.EF) *it.EF = -2.0 * L;
+}
+
+// The log-likelihood terms of observations [kbeg, kend) of the list sel (all
+// stored observations when null), thread t taking every blockDim.x-th from
+// kbeg + t: L[q] for the NI instances whose transnum, population trajectories
+// and spline rows are tnb[q], ptb[q], splv[q]; with WANT_DT also acc[d], the
+// sums per data table.
+template <bool WANT_DT, bool GENERAL, bool POP, bool SPL, int NI>
+__device__ inline void obs_terms(const ObsDev& od, const int* __restrict__ sel, int kbeg, int kend, int d0,
+                                 const int* (&tnb)[NI], const int* (&ptb)[NI], const double* (&splv)[NI],
+                                 double (&L)[NI], double (&acc)[8]) {
   const auto optr = gview(od.ptr);
   const auto elem = gview(od.elem);
   const auto factor = gview(od.factor);
   const auto oy = gview(od.y);
   const auto otype = gview(od.type);
-  double L[NI], acc[8];
 #pragma unroll
   for (int q = 0; q < NI; q++) L[q] = 0;
 #pragma unroll
   for (int d = 0; d < 8; d++) acc[d] = 0;
-  for (int k = threadIdx.x; k < nsel; k += blockDim.x) {
+  for (int k = kbeg + (int)threadIdx.x; k < kend; k += blockDim.x) {
     const int i = sel ? sel[k] : k;
     long long cnt[NI];
 #pragma unroll
@@ -229,26 +350,11 @@ __global__ void __launch_bounds__(256) k_obs_loglik(const ObsDev* __restrict__ o
       }
       L[q] += lp;
       if (WANT_DT) {
-        const int dt = gview(od.dt)[i];
+        const int dt = gview(od.dt)[i] - d0;
 #pragma unroll
         for (int d = 0; d < 8; d++)
           if (d == dt) acc[d] += lp;
       }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < NI; q++) {
-    const double Lt = block_sum(L[q], red);
-    if (threadIdx.x == 0 && b0 + q < n) {
-      out[(size_t)(b0 + q) * (1 + od.ndt)] = Lt;
-      if (store_EF) EF[ins[q]] = -2.0 * Lt;
-    }
-  }
-  if (WANT_DT) {
-    double* o = out + (size_t)b0 * (1 + od.ndt);
-    for (int d = 0; d < od.ndt; d++) {
-      double t = block_sum(acc[d < 8 ? d : 0], red);
-      if (threadIdx.x == 0) o[1 + d] = -2.0 * t;
     }
   }
 }

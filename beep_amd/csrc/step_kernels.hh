@@ -84,10 +84,29 @@ __device__ inline f16v mfma_tile_64x64(const float* __restrict__ A, int lda, int
   return acc;
 }
 
-// The dense contraction with its FOI epilogue, grid (Apad / 64, ceil(rows / 64)).
+// Tiles of a 64x64-tiled product for a one-dimensional grid of
+// 8 * ceil(tiles / 8) workgroups.  Workgroups are dealt to the eight XCDs in
+// turn (workgroup w runs on XCD w % 8), so XCD x is given the contiguous run
+// of tiles [x * per, (x + 1) * per) in row-major tile order: the column tiles
+// of one row tile share an XCD and its L2, which then reads that row tile of
+// the left operand once instead of once per column tile.  Returns false for
+// the few workgroups past the last tile (the whole workgroup returns).
+__device__ inline bool xcd_tile(int ntm, int ntn, int* tm, int* tn) {
+  const int total = ntm * ntn;
+  const int per = (total + 7) >> 3;
+  const int t = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+  if (t >= total) return false;
+  *tm = (t / ntn) * 64;
+  *tn = (t - (t / ntn) * ntn) * 64;
+  return true;
+}
+
+// The dense contraction with its FOI epilogue, one-dimensional grid of
+// 8 * ceil(ceil(rows / 64) * (Apad / 64) / 8) workgroups (xcd_tile).
 __global__ void __launch_bounds__(256) k_dense_contract(DenseMix p) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tm = blockIdx.y * 64, tn = blockIdx.x * 64;
+  int tm, tn;
+  if (!xcd_tile((p.rows + 63) / 64, p.Apad / 64, &tm, &tn)) return;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   f16v acc = mfma_tile_64x64(p.D, p.Apad, p.rows, p.M, p.Apad, p.Apad, tm, tn);
   const int col = tn + wn + (lane & 31);
@@ -109,7 +128,8 @@ __global__ void __launch_bounds__(256) k_dense_contract(DenseMix p) {
 // accumulator layout: C[rows][Apad] = A[rows][Apad] x B[Apad][Apad].
 __global__ void __launch_bounds__(256) k_dbg_gemm(const float* A_, const float* B_, float* C_, int rows, int Apad) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tm = blockIdx.y * 64, tn = blockIdx.x * 64;
+  int tm, tn;
+  if (!xcd_tile((rows + 63) / 64, Apad / 64, &tm, &tn)) return;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   f16v acc = mfma_tile_64x64(A_, Apad, rows, B_, Apad, Apad, tm, tn);
   const int col = tn + wn + (lane & 31);

@@ -103,7 +103,7 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
           } else {
             float pr = lp / li;
             float pp = pr > 0.5f ? 1.f - pr : pr;
-            if ((float)ni * pp < 10.f) n = binomial_draw(ni, pr, rng.next(), rng);
+            if ((float)ni * pp < 10.f) n = binomial_draw_small(ni, pr, rng.next(), rng);
             else { kind = ITEM_BINOM; val = pr; }
           }
         }

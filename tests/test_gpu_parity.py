@@ -110,6 +110,30 @@ def test_obs_sections_and_datatables(gpu_lib):
     ctx.close()
 
 
+def test_more_than_eight_datatables(gpu_lib):
+    """Eleven data tables (the reference loops over data.ndatatable without a
+    bound, src/obsmodel.cc:309-322): every table's -2 log-likelihood against
+    the oracle's sum over that table's observations, and the total."""
+    spec, truth = configs.spatial_abcsmc()
+    spec["obs"] = [dict(o, datatable=i % 11) for i, o in enumerate(spec["obs"])]
+    orc = O.Oracle(spec)
+    st = orc.simulate(orc.tables(truth), seed=8)
+    ctx = _ctx(spec, 2)
+    assert ctx.ndt == 11
+    ctx.set_param([0], truth[None, :])
+    for inst in (0, 1):
+        ctx.put_transnum(inst, 0, st.transnum.astype(np.int32))
+    L, EFdt = ctx.obs_loglik([0, 1], with_dt=True)
+    Lr, _ = orc.obs_loglik(st)
+    assert abs(L[0] / Lr - 1) < 1e-12 and L[1] == L[0]
+    for d in range(11):
+        sel = [i for i, o in enumerate(spec["obs"]) if o["datatable"] == d]
+        Ld, _ = orc.obs_loglik(st, sel=sel)
+        assert abs(EFdt[0, d] / (-2 * Ld) - 1) < 1e-12, d
+        assert EFdt[1, d] == EFdt[0, d]
+    ctx.close()
+
+
 def _ks_compare(a, b, label):
     d, p = stats.ks_2samp(a, b)
     assert p > 1e-4, (label, d, p)

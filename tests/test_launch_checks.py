@@ -23,7 +23,11 @@ def test_stepwise_geometry(gpu_lib):
     g = _lib.check_launch(_lib.CHECK_STEPWISE, [317, 9, 320, 1024, 1024])
     cell_x, cell_y, dense_x, dense_y, sparse_x, sparse_y = g
     assert (cell_x, cell_y) == (1024, (317 * 9 + 255) // 256)
-    assert (dense_x, dense_y) == (5, (1024 * 9 + 63) // 64)
+    # Dense tiles on a one-dimensional grid, a whole number of XCD rounds:
+    # 144 row tiles x 5 column tiles = 720 = 8 x 90.
+    assert (dense_x, dense_y) == (720, 1)
+    g = _lib.check_launch(_lib.CHECK_STEPWISE, [317, 9, 320, 3, 3])     # 1 x 5 tiles: 8 workgroups, 3 empty
+    assert (g[2], g[3]) == (8, 1)
     assert (sparse_x, sparse_y) == (1024 * 9, 2)
     # Jobs on x: more than 65535 jobs or rows are allowed.
     g = _lib.check_launch(_lib.CHECK_STEPWISE, [40, 9, 64, 100000, 100000])

@@ -79,6 +79,15 @@ class AgeMod(C.Structure):
     _fields_ = [("spline", C.c_int32), ("nage", C.c_int32), ("ages", C.POINTER(C.c_int32))]
 
 
+class Mod(C.Structure):
+    _fields_ = [("type", C.c_int32), ("start", C.c_int32), ("end", C.c_int32), ("factor", C.c_double),
+                ("ntr", C.c_int32), ("tr", C.POINTER(C.c_int32)), ("narea", C.c_int32), ("area", C.POINTER(C.c_int32)),
+                ("ndp", C.c_int32), ("dp", C.POINTER(C.c_int32)), ("spline", C.c_int32), ("strain", C.c_int32)]
+
+
+MOD_TYPES = {"trans_rate_fac": 0, "beta_fac": 1, "efoi_fac": 2, "spline_fac": 3, "spline_set": 4}
+
+
 class Spec(C.Structure):
     _fields_ = [("A", C.c_int32), ("G", C.c_int32), ("C", C.c_int32), ("T", C.c_int32),
                 ("period", C.c_int32), ("steps_per_day", C.c_int32), ("nparam", C.c_int32),
@@ -96,7 +105,8 @@ class Spec(C.Structure):
                 ("nagemod", C.c_int32), ("agemod", C.POINTER(AgeMod)),
                 ("nage", C.c_int32), ("dp_age", C.POINTER(C.c_int32)),
                 ("nsus_extra", C.c_int32), ("sus_extra", C.POINTER(C.c_int32)),
-                ("nstrain", C.c_int32), ("strain_Rfactor", C.POINTER(C.c_int32))]
+                ("nstrain", C.c_int32), ("strain_Rfactor", C.POINTER(C.c_int32)),
+                ("nmod", C.c_int32), ("mod", C.POINTER(Mod))]
 
 
 _lib = None
@@ -333,6 +343,19 @@ class _SpecBuffers:
             s.level_map = i32(np.asarray(le["map"]).reshape(-1))
         else:
             s.nlevel, s.level_param, s.level_map = 0, null_i32, null_i32
+        mods = spec.get("modifications") or []
+        ma = (Mod * max(1, len(mods)))()
+        for i, md in enumerate(mods):
+            ma[i].type = MOD_TYPES[md["type"]]
+            ma[i].start, ma[i].end, ma[i].factor = md["start"], md["end"], float(md["factor"])
+            for n, k in (("ntr", "trans"), ("narea", "areas"), ("ndp", "dps")):
+                v = md.get(k) or []
+                setattr(ma[i], n, len(v))
+                setattr(ma[i], {"ntr": "tr", "narea": "area", "ndp": "dp"}[n], i32(v) if v else null_i32)
+            ma[i].spline = md.get("spline", -1)
+            ma[i].strain = md.get("strain", 0)
+        keep.append(ma)
+        s.nmod, s.mod = len(mods), ma
         self.spec = s
 
 

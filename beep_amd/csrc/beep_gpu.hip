@@ -727,6 +727,19 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
     if ((rc = upload(c, &d_agedist, to_f32(m.efoi_agedist)))) return rc;
     tp.agedist = d_agedist;
   }
+  // Prediction-mode modifications: context-wide multiplier tables, null when
+  // the model has none of a kind (the kernels then read nothing more).
+  tp.mod_beta = tp.mod_efoi = tp.mod_trans = nullptr;
+  {
+    const std::vector<double>* mt[3] = {&m.mod_beta, &m.mod_efoi, &m.mod_trans};
+    const float** dst[3] = {&tp.mod_beta, &tp.mod_efoi, &tp.mod_trans};
+    for (int k = 0; k < 3; k++)
+      if (!mt[k]->empty()) {
+        float* d = nullptr;
+        if ((rc = upload(c, &d, to_f32(*mt[k])))) return rc;
+        *dst[k] = d;
+      }
+  }
   tp.off_af = -1;
   tp.af = nullptr;
   if (m.af_param) {

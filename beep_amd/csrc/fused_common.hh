@@ -33,6 +33,12 @@ struct TopoDev {
   const float* af;
   int off_af;
   const float* agedist;  // [G] age distribution of the external force of infection (efoi[S] per slot)
+  // Prediction-mode modifications, null when absent: multipliers of beta and
+  // of the external force per step and cell, [S][AG], and of every transition
+  // mean, [S][T][AG].
+  const float* mod_beta;
+  const float* mod_efoi;
+  const float* mod_trans;
 };
 
 // One instance's work in a launch.
@@ -133,6 +139,21 @@ __device__ inline float foi_mean(float beta, float af, float nmi, float efoi, fl
   const float b = beta * af;
   const float e = efoi * agedist;
   return __builtin_fmaf(b, nmi, e);
+}
+
+// Prediction-mode modifications (src/model.cc:1349-1356, src/state.cc:415,
+// 446-453).  The force of infection's factors of cell `cell` at step s: af
+// times beta's multiplier and agedist times the external force's; without
+// modifications they stay af and agedist.
+__device__ inline void mod_foi_factors(ctopo* tp, int s, int cell, float& af, float& agedist) {
+  if (tp->mod_beta) af *= gptr(tp->mod_beta)[(size_t)s * tp->AG + cell];
+  if (tp->mod_efoi) agedist *= gptr(tp->mod_efoi)[(size_t)s * tp->AG + cell];
+}
+
+// A transition mean times its trans_rate_fac multiplier.
+__device__ inline float mod_mean(ctopo* tp, int s, int tr, int cell, float lp) {
+  if (tp->mod_trans) lp *= gptr(tp->mod_trans)[((size_t)s * tp->T + tr) * tp->AG + cell];
+  return lp;
 }
 
 // Queue capacity of large draws per step (items beyond it are drawn in place).

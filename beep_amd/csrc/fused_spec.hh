@@ -196,7 +196,9 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
 #pragma unroll
       for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * sI[j * G + gg];
       if (nmi < 0.f) nmi = 0.f;
-      const float foi = foi_mean(beta_s, af_s[j], nmi, efoi_s, agedist[g]);
+      float afc = af_s[j], agc = agedist[g];
+      mod_foi_factors(tpp, s, cell, afc, agc);
+      const float foi = foi_mean(beta_s, afc, nmi, efoi_s, agc);
       const bool simu = mode == MODE_MBP ? (simu_s ? simu_s[j] != 0 : false) : true;
       CellRng rng;
       rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
@@ -210,6 +212,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
         }
         float lp = 0.f;
         if (popu > 0.f) lp = tr == INF_TR ? dt * popu * ssus[g] * foi : dt * popu * srate[tr * G + g];
+        lp = mod_mean(tpp, s, tr, cell, lp);
         // src/state.cc:429 (NaN mean) and src/utils.cc:150 (mean > 1e9) stop the reference.
         if (lp != lp) { atomicOr(jb.err, 1); lp = 0.f; }
         else if (lp > 1e9f) { atomicOr(jb.err, 2); lp = 0.f; }
@@ -241,13 +244,25 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
             }
           }
         }
-        if (kind >= 0) {                     // drawn in (D)
-          int slot = atomicAdd(qcount, 1);
-          const uint32_t idx = (uint32_t)cell * T + tr;
-          if (q16) qidx16[slot] = (uint16_t)idx;
-          else qidx32[slot] = idx;
-          qval[slot] = kind == ITEM_POIS_ADD ? -val : val;
-        } else {
+        // Draws queued for (D): the wavefront's lanes that queue this
+        // transition take consecutive slots from one LDS atomic (the draws are
+        // keyed by cell and transition, so the slots' order is immaterial).
+        const unsigned long long qm = __ballot(kind >= 0);
+        if (qm) {
+          const int lane = (int)__lane_id();
+          const int leader = __ffsll((long long)qm) - 1;
+          int base = 0;
+          if (lane == leader) base = atomicAdd(qcount, __popcll(qm));
+          base = __shfl(base, leader);
+          if (kind >= 0) {
+            const int slot = base + __popcll(qm & ((1ull << lane) - 1ull));
+            const uint32_t idx = (uint32_t)cell * T + tr;
+            if (q16) qidx16[slot] = (uint16_t)idx;
+            else qidx32[slot] = idx;
+            qval[slot] = kind == ITEM_POIS_ADD ? -val : val;
+          }
+        }
+        if (kind < 0) {
           if (tn_s && mode != MODE_REPLAY) st_nt(tn_s, tr * AG + cell, n);
           if (update) {
             q[fr] -= n;
@@ -325,7 +340,9 @@ extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_
     beep_fused_spec_sim(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
   beepdev::fused_spec_body<beepdev::MODE_SIM>(jobs, seed, stream);
 }
-extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_WAVES)
+// The coupled proposal needs more registers than a simulation: at most six
+// resident wavefronts per SIMD (80 registers) keep it from spilling.
+extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_WAVES < 6 ? beepspec::MIN_WAVES : 6)
     beep_fused_spec_mbp(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
   beepdev::fused_spec_body<beepdev::MODE_MBP>(jobs, seed, stream);
 }

@@ -405,6 +405,91 @@ std::string HostModel::build(const beep_spec& s) {
   af_param = !area_effect_param.empty() || !covar.empty() || !level_param.empty();
   af_const.clear();
   if (!af_param) af_const = area_factor(std::vector<double>(nparam, 0.0));
+  return build_modifications(s);
+}
+
+// The tables of Model::model_modification's loop (src/model.cc:1495-1575):
+// each modification multiplies its cells' factors, or sets a spline's steps;
+// a spline step cannot be both set and multiplied (the reference's error).
+std::string HostModel::build_modifications(const beep_spec& s) {
+  mod_beta.clear();
+  mod_efoi.clear();
+  mod_trans.clear();
+  spline_mult.clear();
+  spline_set.clear();
+  if (s.nmod < 0 || (s.nmod > 0 && !s.mod)) return "bad modification list";
+  const int dpm = G / nstrain;
+  for (int k = 0; k < s.nmod; k++) {
+    const beep_mod& md = s.mod[k];
+    if (md.start < 0 || md.end > S || md.start > md.end) return "modification: step range out of range";
+    if (!(md.factor >= 0) || std::isinf(md.factor)) return "modification: the factor must be non-negative";
+    std::vector<int> areas, dps;
+    if (md.narea > 0) {
+      if (!md.area) return "modification: no area list";
+      for (int i = 0; i < md.narea; i++) {
+        if (md.area[i] < 0 || md.area[i] >= A) return "modification: area out of range";
+        areas.push_back(md.area[i]);
+      }
+    } else {
+      for (int c = 0; c < A; c++) areas.push_back(c);
+    }
+    if (md.ndp > 0) {
+      if (!md.dp) return "modification: no dp list";
+      for (int i = 0; i < md.ndp; i++) {
+        if (md.dp[i] < 0 || md.dp[i] >= G) return "modification: dp out of range";
+        dps.push_back(md.dp[i]);
+      }
+    } else {
+      for (int g = 0; g < G; g++) dps.push_back(g);
+    }
+    switch (md.type) {
+      case BEEP_MOD_TRANS_RATE: {
+        if (md.ntr <= 0 || !md.tr) return "trans_rate_fac: no transitions";
+        if (mod_trans.empty()) mod_trans.assign((size_t)S * T * AG, 1.0);
+        for (int i = 0; i < md.ntr; i++) {
+          const int tr = md.tr[i];
+          if (tr < 0 || tr >= T) return "trans_rate_fac: transition out of range";
+          for (int st = md.start; st < md.end; st++)
+            for (int c : areas)
+              for (int g : dps) mod_trans[((size_t)st * T + tr) * AG + (size_t)c * G + g] *= md.factor;
+        }
+        break;
+      }
+      case BEEP_MOD_BETA:
+      case BEEP_MOD_EFOI: {
+        if (md.strain < 0 || md.strain >= nstrain) return "beta_fac / efoi_fac: strain out of range";
+        std::vector<double>& tab = md.type == BEEP_MOD_BETA ? mod_beta : mod_efoi;
+        if (tab.empty()) tab.assign((size_t)S * AG, 1.0);
+        // The strain's dps: its infections (src/state.cc:409-430).
+        for (int st = md.start; st < md.end; st++)
+          for (int c : areas)
+            for (int g = md.strain * dpm; g < (md.strain + 1) * dpm; g++) tab[(size_t)st * AG + (size_t)c * G + g] *= md.factor;
+        break;
+      }
+      case BEEP_MOD_SPLINE_FAC:
+      case BEEP_MOD_SPLINE_SET: {
+        if (md.spline < 0 || md.spline >= (int)spline.size()) return "spline_fac / spline_set: spline out of range";
+        if (spline_mult.empty()) {
+          spline_mult.assign(spline.size(), std::vector<double>(S, 1.0));
+          spline_set.assign(spline.size(), std::vector<double>(S, std::nan("")));
+        }
+        for (int st = md.start; st < md.end; st++) {
+          double& mult = spline_mult[md.spline][st];
+          double& set = spline_set[md.spline][st];
+          if (md.type == BEEP_MOD_SPLINE_FAC) {
+            if (!std::isnan(set)) return "In 'modification' a spline cannot both be directly set and have a factor change";
+            mult *= md.factor;
+          } else {
+            if (mult != 1) return "In 'modification' a spline cannot both be directly set and have a factor change";
+            set = md.factor;
+          }
+        }
+        break;
+      }
+      default:
+        return "modification: unknown type";
+    }
+  }
   return "";
 }
 
@@ -434,6 +519,12 @@ std::vector<double> HostModel::disc_spline(int sp, const std::vector<double>& pd
       out[s] = factor * (pd[spl.param[p]] * spl.multfac[p] * (1 - fac) + pd[spl.param[p + 1]] * spl.multfac[p + 1] * fac);
     }
   }
+  // Prediction-mode modifications (src/model.cc:1044-1049).
+  if (!spline_mult.empty())
+    for (int s = 0; s < S; s++) {
+      out[s] *= spline_mult[sp][s];
+      if (!std::isnan(spline_set[sp][s])) out[s] = spline_set[sp][s];
+    }
   return out;
 }
 

@@ -94,6 +94,16 @@ class HostModel {
   bool af_param = false;
   std::vector<double> af_const;
 
+  // Prediction-mode modifications (src/model.cc:1440-1580), as multipliers per
+  // step and cell (dp of an area): of beta (beta_fac), of the external force
+  // (efoi_fac) and of every transition mean (trans_rate_fac), each empty when
+  // no modification of its kind is given; per spline and step, the factor of
+  // spline_fac and the value of spline_set (NaN = not set), empty = none.
+  std::vector<double> mod_beta;                               // [S][AG]
+  std::vector<double> mod_efoi;                               // [S][AG]
+  std::vector<double> mod_trans;                              // [S][T][AG]
+  std::vector<std::vector<double>> spline_mult, spline_set;   // [nspline][S]
+
   std::vector<double> dirichlet_correct(const double* paramv) const;
   // Returns 0 or a BEEP_* error code.
   int derive(const double* paramv, Tables& out, std::string& err) const;
@@ -105,6 +115,7 @@ class HostModel {
   bool inbounds(const double* paramv) const;
 
  private:
+  std::string build_modifications(const beep_spec& s);
   std::vector<double> disc_spline(int sp, const std::vector<double>& pdir) const;
   std::vector<double> area_factor(const std::vector<double>& pdir) const;
   int strain_beta(const std::vector<double>& pd, Tables& tb, std::string& err) const;

@@ -137,7 +137,9 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
       float nmi = 0.f;
       for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * sI[j * G + gg];
       if (nmi < 0.f) nmi = 0.f;
-      const float foi = foi_mean(beta_s, af_s[j], nmi, efoi_s, agedist[g]);
+      float afc = af_s[j], agc = agedist[g];
+      mod_foi_factors(tpp, s, cell, afc, agc);
+      const float foi = foi_mean(beta_s, afc, nmi, efoi_s, agc);
       const bool simu = mode == MODE_MBP ? (simu_s ? simu_s[j] != 0 : false) : true;
       CellRng rng;
       rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
@@ -148,6 +150,7 @@ __global__ void __launch_bounds__(BEEP_FUSED_MAX_BLOCK, 4)
         float popu = (float)cur[fr * AG + cell];
         float lp = 0.f;
         if (popu > 0.f) lp = tr == inf_tr ? dt * popu * ssus[g] * foi : dt * popu * srate[tr * G + g];
+        lp = mod_mean(tpp, s, tr, cell, lp);
         // src/state.cc:429 (NaN mean) and src/utils.cc:150 (mean > 1e9) stop the reference.
         if (lp != lp) { atomicOr(jb.err, 1); lp = 0.f; }
         else if (lp > 1e9f) { atomicOr(jb.err, 2); lp = 0.f; }

@@ -68,8 +68,9 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
 #pragma unroll
     for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * Ib[(size_t)gg * Apad + j];
     if (nmi < 0.f) nmi = 0.f;
-    const float foi = foi_mean(tab[tp->off_beta + s], af_row(tp, tab, s)[j], nmi, tab[tp->off_efoi + s],
-                               gptr(tp->agedist)[g]);
+    float afc = af_row(tp, tab, s)[j], agc = gptr(tp->agedist)[g];
+    mod_foi_factors(tp, s, cell, afc, agc);
+    const float foi = foi_mean(tab[tp->off_beta + s], afc, nmi, tab[tp->off_efoi + s], agc);
     const bool simu = MODE == MODE_MBP ? (jb.simu ? gptr(jb.simu)[(size_t)s * A + j] != 0 : false) : true;
     CellRng rng;
     rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
@@ -80,6 +81,7 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
       float popu = (float)p[fr];
       float lp = 0.f;
       if (popu > 0.f) lp = tr == INF_TR ? dt * popu * ssus[g] * foi : dt * popu * srate[tr * G + g];
+      lp = mod_mean(tp, s, tr, cell, lp);
       if (lp != lp) { atomicOr(jb.err, 1); lp = 0.f; }
       else if (lp > 1e9f) { atomicOr(jb.err, 2); lp = 0.f; }
       if (tm_s) st_nt(tm_s, tr * AG + cell, lp);

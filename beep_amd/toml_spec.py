@@ -39,10 +39,15 @@ and `area_tv_covars` (func log or linear), `area_effect` and `level_effect`
 (named by a table's `factor_spline`), `democats`, `init_pop` and `strains`,
 `data_tables` of type
 'transition' or 'population' with obsmodel 'normal <sd>', 'normal <P>%',
-'normal load' or 'poisson'.  Presentation keys are ignored; `modification`,
+'normal load' or 'poisson'.  Presentation keys are ignored.  `modification`,
 `prediction_start` and `prediction_end` act only in the reference's
-prediction mode (src/model.cc:1349, src/state.cc:415,446), and `prob_reach`
-is an output-only derived quantity; they are recorded in the returned metadata.  Everything else raises TomlError naming the key.
+prediction mode (src/details.cc:80-92, src/model.cc:1044-1049,1349-1356,
+1445-1585, src/state.cc:415,446): load(..., prediction=True) reads them
+(every modification type, with geo_filt and democats_filt; the period
+extended to prediction_end, the parameter splines extended to it),
+otherwise they are only recorded in the returned metadata, as is
+`prob_reach`, an output-only derived quantity.  Everything else raises
+TomlError naming the key.
 """
 import csv
 import datetime
@@ -99,7 +104,7 @@ def _col(head, name):
 
 
 class _Reader:
-    def __init__(self, doc, datadir):
+    def __init__(self, doc, datadir, prediction=False):
         self.doc = doc
         self.datadir = datadir
         fmt = _s(doc.get("time_format", ""))
@@ -118,6 +123,12 @@ class _Reader:
         self.period = self.end - self.start
         if self.period <= 0:
             raise TomlError("'end' must be after 'start'")
+        # Prediction mode simulates on to 'prediction_end' when that is later
+        # (src/details.cc:80-92); 'end' keeps naming the end of the data.
+        self.data_period = self.period
+        self.prediction = prediction
+        if prediction and "prediction_end" in doc:
+            self.period = max(self.period, self._abs_time(_s(doc["prediction_end"])) - self.start)
         spd = _s(doc.get("steps_per_unit_time", "2"))      # src/details.cc:97
         self.spd = int(spd)
         if self.spd <= 0 or str(self.spd) != spd:
@@ -142,7 +153,7 @@ class _Reader:
         if st == "start":
             return 0
         if st == "end":
-            return self.period
+            return self.data_period
         return self._abs_time(st) - self.start
 
     # --- tables ---
@@ -658,19 +669,28 @@ def _spline(rd, P, key, entry, default_name=None, params=None):
         _check_keys(entry, {"name", "bp", "param", "value", "prior", "factor", "factor_param", "factor_value",
                             "factor_prior", "smooth", "smooth_type"} | ({"age_dist"} if key == "efoi_spline" else set()),
                     key)
-    bp = [rd.time(t) for t in _split(rd.from_table(_s(entry.get("bp", "")) or "start|end"), "|")]
-    if not bp or bp[0] != 0:
-        raise TomlError(f"in '{key}' 'bp' must start at 'start'")
-    if bp[-1] != rd.period:
-        raise TomlError(f"in '{key}' 'bp' must end at 'end'")
+    bp = _bp(rd, entry, key, extend=params is None)
     if any(b > a for a, b in zip(bp[1:], bp[:-1])):
         raise TomlError(f"in '{key}' times in 'bp' are not in the right order")
     n = len(bp)
     fac = rd.split_n(_s(entry.get("factor", "")) or "1", n, key)
+    sm = rd.split_n(_s(entry.get("smooth", "")), n, key)
     if params is None:
         names = rd.split_n(_s(entry.get("param", "")), n, key)
         vals = rd.split_n(_s(entry.get("value", "")), n, key)
         pris = rd.split_n(_s(entry.get("prior", "")), n, key)
+        if bp[-1] < rd.period:
+            # Inputs::extend_spline_for_prediction (src/inputs.cc:1475-1493):
+            # the last knot moves to the prediction's end when the last two
+            # knots are specified alike, else a copy of the last is added.
+            lists = (names, vals, pris, fac, sm)
+            if n >= 2 and all(x[-2] == x[-1] for x in lists):
+                bp[-1] = rd.period
+            else:
+                bp.append(rd.period)
+                for x in lists:
+                    x.append(x[-1])
+                n += 1
         params = []
         for i in range(n):
             nm = names[i]
@@ -694,7 +714,6 @@ def _spline(rd, P, key, entry, default_name=None, params=None):
     if st not in kinds:
         raise TomlError(f"in '{key}' the value of 'smooth_type={st}' is not recognised")
     if kinds[st]:
-        sm = rd.split_n(_s(entry.get("smooth", "")), n, key)
         try:
             smv = [float(x) if x != "" else math.nan for x in sm]
         except ValueError:
@@ -708,13 +727,19 @@ def _spline(rd, P, key, entry, default_name=None, params=None):
     return out
 
 
-def _bp(rd, entry, key):
-    """Knot times of a spline entry (Inputs::get_breakpoints, src/inputs.cc:872-905)."""
+def _bp(rd, entry, key, extend=False):
+    """Knot times of a spline entry (Inputs::get_breakpoints, src/inputs.cc:872-905):
+    from 'start' to 'end'; in prediction mode the last may end before the
+    prediction's end when the spline is extended there (extend)."""
     bp = [rd.time(t) for t in _split(rd.from_table(_s(entry.get("bp", "")) or "start|end"), "|")]
     if not bp or bp[0] != 0:
         raise TomlError(f"in '{key}' 'bp' must start at 'start'")
-    if bp[-1] != rd.period:
-        raise TomlError(f"in '{key}' 'bp' must end at 'end'")
+    if any(b > rd.data_period for b in bp):
+        raise TomlError(f"in '{key}' a time in 'bp' is outside of the analysis time period")
+    if bp[-1] != rd.period and not (extend and rd.prediction):
+        raise TomlError(f"in '{key}' 'bp' must end at 'end'" +
+                        (" (the prediction's end: only splines of parameters are extended there)"
+                         if rd.period != rd.data_period else ""))
     return bp
 
 
@@ -983,6 +1008,115 @@ def _obs_splines(rd, doc, P, splines):
     return out
 
 
+def _area_sel(head, rows, expr, where):
+    """Data::create_area_sel (src/data.cc:1783-1826) over the areas file:
+    'col:v1|v2,col2:v3' -> the areas that every comma-separated term selects
+    (a term takes the areas whose value in the column is listed; column
+    'all' has the one value 'all'); '' -> every area."""
+    if expr == "":
+        return list(range(len(rows)))
+    parts = expr.split(",")
+    flag = [0] * len(rows)
+    for part in parts:
+        sp = part.split(":")
+        if len(sp) != 2:
+            raise TomlError(f"{where} there was a problem with the expression '{expr}'")
+        vals = sp[1].split("|")
+        if len(set(vals)) != len(vals):
+            raise TomlError(f"{where} a value in '{expr}' must not be repeated.")
+        regions = dict(_regions(head, rows, sp[0]))
+        for v in vals:
+            if v not in regions:
+                raise TomlError(f"{where} the value '{v}' is not found in '{sp[0]}'.")
+            for i in regions[v]:
+                flag[i] += 1
+    return [i for i in range(len(rows)) if flag[i] == len(parts)]
+
+
+def _dp_sel(demo, expr, where):
+    """Data::create_dp_sel (src/data.cc:1718-1781): 'cat:v1|v2,cat2:v3' ->
+    the dps whose value of each named category is listed; '' -> all."""
+    if expr == "":
+        return list(range(demo.G))
+    parts = expr.split(",")
+    flag = [[0] * len(v) for _, v in demo.cats]
+    for part in parts:
+        sp = part.split(":")
+        if len(sp) != 2:
+            raise TomlError(f"{where} the expression '{expr}' must specify the demographic category and its value")
+        k = next((k for k, (n, _) in enumerate(demo.cats) if n == sp[0]), None)
+        if k is None:
+            raise TomlError(f"{where} cannot find the value '{sp[0]}' in 'ages' or 'democats'")
+        vals = sp[1].split("|")
+        if len(set(vals)) != len(vals):
+            raise TomlError(f"{where} a value in '{expr}' must not be repeated.")
+        for v in vals:
+            if v not in demo.cats[k][1]:
+                raise TomlError(f"{where} cannot find the value '{v}' in 'name=\"{sp[0]}\"'")
+            flag[k][demo.cats[k][1].index(v)] += 1
+        for kk in range(len(demo.cats)):
+            if kk != k:
+                flag[kk] = [f + 1 for f in flag[kk]]
+    return [dp for dp, t in enumerate(demo.dps) if all(flag[k][t[k]] == len(parts) for k in range(len(demo.cats)))]
+
+
+MOD_KEYS = {"start", "end", "strain", "geo_filt", "democats_filt", "type", "trans", "factor", "value", "name"}
+
+
+def _modifications(rd, entries, area_head, area_rows, demo, comps, trans, spline_names, strain_names):
+    """'modification' entries (Inputs::find_modification,
+    src/inputs.cc:1986-2061; filters resolved as src/data.cc:2142-2150) as
+    the host model's modification dicts: type, the steps [start, end) (the
+    entry's times relative to 'start', by default the whole period, times
+    steps per day), factor (spline_set's 'value'), and the transitions,
+    areas, dps, spline and strain it acts on (Model::setup_modification,
+    src/model.cc:1445-1577)."""
+    out = []
+    for e in entries:
+        _check_keys(e, MOD_KEYS, "modification")
+        where = "In 'modification'"
+        start = rd.time(_s(e["start"])) if _s(e.get("start", "")) else 0
+        end = rd.time(_s(e["end"])) if _s(e.get("end", "")) else rd.period
+        if not 0 <= start <= end <= rd.period:
+            raise TomlError(f"{where} 'start' and 'end' must lie in the period, in that order")
+        typ = _s(e.get("type", ""))
+        md = {"type": typ, "start": start * rd.spd, "end": end * rd.spd,
+              "areas": _area_sel(area_head, area_rows, _s(e.get("geo_filt", "")), f"{where} for 'geo_filt'"),
+              "dps": _dp_sel(demo, _s(e.get("democats_filt", "")), f"{where} for 'democats_filt'")}
+        key = "value" if typ == "spline_set" else "factor"
+        if typ not in ("trans_rate_fac", "beta_fac", "efoi_fac", "spline_fac", "spline_set"):
+            raise TomlError(f"'{typ}' in 'modification' is not recognised (it should be 'trans_rate_fac' or 'efoi_fac').")
+        try:
+            md["factor"] = float(_s(e[key]))
+        except (KeyError, ValueError):
+            raise TomlError(f"{where} a numeric '{key}' must be set") from None
+        if md["factor"] < 0:
+            raise TomlError(f"{where} for '{key}' the expression '{_s(e[key])}' is not positive.")
+        if typ == "trans_rate_fac":
+            name = _s(e.get("trans", ""))
+            md["trans"] = [k for k, t in enumerate(trans)
+                           if comps[t["from"]]["name"] + "->" + comps[t["to"]]["name"] == name]
+            if not md["trans"]:
+                raise TomlError(f"The transition '{name}' is not recognised")
+        if typ in ("beta_fac", "efoi_fac"):
+            st = _s(e.get("strain", ""))
+            if st == "":
+                if strain_names and len(strain_names) > 1:
+                    raise TomlError("In 'modification' a value for 'strain' must be set")
+                md["strain"] = 0
+            else:
+                if not strain_names or st not in strain_names:
+                    raise TomlError(f"In 'modification' the value 'strain=\"{st}\"' is not recognised")
+                md["strain"] = strain_names.index(st)
+        if typ in ("spline_fac", "spline_set"):
+            name = _s(e.get("name", ""))
+            if name not in spline_names:
+                raise TomlError(f"In 'modification' cannot find spline with the name '{name}'")
+            md["spline"] = spline_names[name]
+        out.append(md)
+    return out
+
+
 def _filters(e, demo, area_head, area_rows):
     """A data table's columns as (name, areas, dps) (Data::get_datafilter,
     src/data.cc:1460-1560): one per region with geo_dep, one per value of a
@@ -1225,14 +1359,18 @@ def _table_column(rd, rows, times, r0, r1, ts, typ, dc, sc, areas, dps, graph, d
     return obs
 
 
-def load(path, datadir=None, with_data=True):
+def load(path, datadir=None, with_data=True, prediction=False):
     """Reads a BEEP model file.  Returns (spec, values, meta): the spec dict,
     the parameter values the file gives (NaN where only a prior is given), and
     {"ignored": [...], "prediction": {...}, "derived": {...}} for keys
     recorded without effect on the simulation or the likelihood.
     datadir overrides the file's 'datadir' (which the reference reads relative
     to the working directory); with_data=False skips the data tables' files,
-    as the reference's simulation mode does."""
+    as the reference's simulation mode does.  prediction=True reads the model
+    as the reference's prediction mode does: the 'modification' entries go
+    into spec["modifications"] and meta["prediction"]["pred_start"] is the
+    day (relative to 'start') the modified simulations resume from
+    (src/model.cc:1493-1585)."""
     with open(path, "rb") as f:
         doc = _toml.load(f)
     meta = {"ignored": [], "prediction": {}, "derived": {}}
@@ -1249,7 +1387,7 @@ def load(path, datadir=None, with_data=True):
         datadir = _s(doc.get("datadir", ""))
         if datadir == "":
             raise TomlError("'datadir' is not set")
-    rd = _Reader(doc, datadir)
+    rd = _Reader(doc, datadir, prediction)
     P = _Params()
     # Demographic categories (Inputs::find_demographic_category,
     # src/inputs.cc:617-660): 'ages' (without it one group whose population
@@ -1310,16 +1448,20 @@ def load(path, datadir=None, with_data=True):
     comps = _compartments(rd, doc, P, demo)
     trans, dirichlet = _transitions(rd, doc, comps, P, demo)
 
-    splines = []
+    # Splines by the names modifications use (Model::add_splines,
+    # src/model.cc:518-628): 'name', else R_t, efoi(t), m(t).
+    splines, spline_names = [], {}
     if len(doc.get("R_spline", [])) != 1:
         raise TomlError("exactly one 'R_spline' entry is supported")
     splines.append(_spline(rd, P, "R_spline", doc["R_spline"][0]))
+    spline_names.setdefault(_s(doc["R_spline"][0].get("name", "")) or "R_t", 0)
     efoi = -1
     if "efoi_spline" in doc:
         if len(doc["efoi_spline"]) != 1:
             raise TomlError("one 'efoi_spline' entry is supported")
         splines.append(_spline(rd, P, "efoi_spline", doc["efoi_spline"][0]))
         efoi = 1
+        spline_names.setdefault(_s(doc["efoi_spline"][0].get("name", "")) or "efoi(t)", efoi)
 
     geo = -1
     if "geo_mixing_modify" in doc:
@@ -1327,6 +1469,7 @@ def load(path, datadir=None, with_data=True):
             raise TomlError("only one spline can represent 'geo_mixing'")
         splines.append(_spline(rd, P, "geo_mixing_modify", doc["geo_mixing_modify"][0]))
         geo = len(splines) - 1
+        spline_names.setdefault(_s(doc["geo_mixing_modify"][0].get("name", "")) or "m(t)", geo)
 
     # Areas and the initial population (src/data.cc:288-457).
     if "areas" not in doc:
@@ -1360,7 +1503,11 @@ def load(path, datadir=None, with_data=True):
             raise TomlError(f"in 'prob_reach' cannot find the compartment '{_s(e.get('comp', ''))}'")
     area_eff, level, eff_groups = _effects(rd, doc, P, head, rows, pop)
     age_mod, mod_groups = _age_mods(rd, doc, P, ages, pop_age, splines)
+    for m, e in zip(age_mod, doc.get("age_mixing_modify", []) + doc.get("age_mixing_perturb", [])):
+        spline_names.setdefault(f"MatMod {_s(e.get('agecat', ''))} factor", m["spline"])
     obs_spl = _obs_splines(rd, doc, P, splines)
+    for name, k in obs_spl.items():
+        spline_names.setdefault(name, k)
 
     # Age mixing (Data::age_mixing_matrix, src/data_matrix.cc:138-175): an
     # age x age table without a heading, normalised later by the host model.
@@ -1428,4 +1575,23 @@ def load(path, datadir=None, with_data=True):
         "age_mod": age_mod,
         "obs": _observations(rd, doc, comps, trans, with_data, head, rows, demo, obs_spl, pop),
     }
+    if prediction:
+        mods = doc.get("modification", [])
+        if isinstance(mods, dict):
+            mods = [mods]
+        strain_names = cats[strain_k][1] if strain_k is not None else ["all"]
+        spec["modifications"] = _modifications(rd, mods, head, rows, demo, comps, trans, spline_names,
+                                               strain_names)
+        # The day the simulations resume from (src/model.cc:1493-1585,
+        # src/details.cc:80-85): the earliest modification, else the end;
+        # 'prediction_start' when given, which no modification may precede.
+        ps = min((m["start"] // rd.spd for m in spec["modifications"]), default=rd.period)
+        if "prediction_start" in doc:
+            given = rd.time(_s(doc["prediction_start"]))
+            if given > rd.period:
+                raise TomlError("'prediction_start' cannot be after 'end'")
+            if spec["modifications"] and ps < given:
+                raise TomlError("Dates in 'modification' cannot be before 'pred_start'")
+            ps = given
+        meta["prediction"]["pred_start"] = ps
     return spec, np.array(P.values), meta

@@ -99,6 +99,25 @@ typedef struct { int32_t spline; int32_t nage; const int32_t* ages; } beep_agemo
  * values are centred on load as Data::read_covars does (src/data.cc:557-565). */
 typedef struct { int32_t param; const double* value; /* [A][period], after any log transform */ } beep_covar;
 
+/* Prediction-mode model modifications (Inputs 'modification', src/inputs.cc:
+ * 1990-2060; their tables Model::model_modification... src/model.cc:1440-1580;
+ * their use src/model.cc:1044-1049,1349-1356, src/state.cc:415,446-453). */
+#define BEEP_MOD_TRANS_RATE 0   /* trans_rate_fac: every mean of the listed transitions x factor */
+#define BEEP_MOD_BETA       1   /* beta_fac: beta x factor                                  */
+#define BEEP_MOD_EFOI       2   /* efoi_fac: the external force of infection x factor       */
+#define BEEP_MOD_SPLINE_FAC 3   /* spline_fac: the spline's value x factor                  */
+#define BEEP_MOD_SPLINE_SET 4   /* spline_set: the spline's value set to factor             */
+typedef struct {
+  int32_t type;
+  int32_t start, end;           /* steps [start, end)                                  */
+  double factor;                /* the factor, or spline_set's value                    */
+  int32_t ntr;   const int32_t* tr;    /* trans_rate_fac: the transitions               */
+  int32_t narea; const int32_t* area;  /* geo_filt: the areas; 0 = every area          */
+  int32_t ndp;   const int32_t* dp;    /* democats_filt: the dps; 0 = every dp          */
+  int32_t spline;               /* spline_fac / spline_set: the spline                  */
+  int32_t strain;               /* beta_fac / efoi_fac: the strain (0 with one strain)  */
+} beep_mod;
+
 /* Observation model functions (src/consts.hh ObsModelFunc). */
 #define BEEP_OBS_NORMAL          0  /* sd given                                            */
 #define BEEP_OBS_NORMAL_PERCENT  1  /* sd = (y(1-eps) + max_graph*eps)*percent/100          */
@@ -180,6 +199,10 @@ typedef struct {
    * draws on the susceptibles of every strain's dp, pooled (src/state.cc:380-430). */
   int32_t nstrain;              /* 0 or 1 = one strain                                 */
   const int32_t* strain_Rfactor;/* [nstrain] Rfactor parameter of each strain, -1 = 1  */
+  /* Prediction-mode modifications (the reference applies them in its
+   * prediction mode only); 0 = none. */
+  int32_t nmod;
+  const beep_mod* mod;
 } beep_spec;
 
 /* Host only (no device is touched): State::set_param's FP64 tables for one

@@ -115,6 +115,12 @@ struct OrTables {
   const double* agedist; // [G]      efoi_agedist
   const double* geo;     // [S]      disc_spline[geo_spline_ref][s]
   const double* Ntime;   // [S][G][G]
+  // Prediction-mode modifications (Model::modelmod, src/struct.hh:488-495),
+  // null when absent: beta_mult and efoi_mult of each dp's strain, [S][A][G],
+  // and transmean_mult, [S][A][T][G].
+  const double* beta_mult;
+  const double* efoi_mult;
+  const double* transmean_mult;
 };
 
 struct OrState {
@@ -235,8 +241,9 @@ int or_set_transmean(const OrModel* m, const OrTables* tb, OrState* st, int s, i
   std::vector<double> NMI(G);
   get_NMI(m, tb, st, s, c, NMI.data());
   int tr = m->inf_tr;
-  double be = tb->beta[(size_t)s * m->A + c];
-  be *= tb->af[(size_t)(s / m->spd) * m->A + c];
+  const double beta_sc = tb->beta[(size_t)s * m->A + c];
+  const double af_sc = tb->af[(size_t)(s / m->spd) * m->A + c];
+  double be = beta_sc * af_sc;
   double et = tb->efoi[s];
   // Strains (src/state.cc:395-404): the susceptibles of a dp are pooled into
   // strain 0 (in the state itself) and every strain's infection draws on the
@@ -258,7 +265,12 @@ int or_set_transmean(const OrModel* m, const OrTables* tb, OrState* st, int s, i
     double& out = st->transmean[ix_tr(m, s, c, tr, dp)];
     if (popu <= 0) out = 0;
     else {
-      out = dt * popu * tb->sus[dp] * (be * NMI[dp] + et * tb->agedist[dp]);
+      // beta[st][c][sett] carries beta_mult (src/model.cc:1349-1356) and et
+      // efoi_mult (src/state.cc:415).
+      double b = be, e = et;
+      if (tb->beta_mult) b = beta_sc * tb->beta_mult[((size_t)s * m->A + c) * G + dp] * af_sc;
+      if (tb->efoi_mult) e = et * tb->efoi_mult[((size_t)s * m->A + c) * G + dp];
+      out = dt * popu * tb->sus[dp] * (b * NMI[dp] + e * tb->agedist[dp]);
       if (std::isnan(out)) return -4;
     }
   }
@@ -270,6 +282,10 @@ int or_set_transmean(const OrModel* m, const OrTables* tb, OrState* st, int s, i
       st->transmean[ix_tr(m, s, c, t, dp)] = popu <= 0 ? 0 : dt * popu * tb->rate[t * G + dp];
     }
   }
+  // transmean_mult (src/state.cc:446-453).
+  if (tb->transmean_mult)
+    for (int t = 0; t < T; t++)
+      for (int dp = 0; dp < G; dp++) st->transmean[ix_tr(m, s, c, t, dp)] *= tb->transmean_mult[ix_tr(m, s, c, t, dp)];
   return 0;
 }
 

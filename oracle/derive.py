@@ -144,7 +144,69 @@ def disc_spline(spec, sp, pd):
         else:
             f = (t - t_k[p]) / (t_k[p + 1] - t_k[p])
             out[s] = factor * (pd[par[p]] * mf[p] * (1 - f) + pd[par[p + 1]] * mf[p + 1] * f)
+    if spec.get("modifications"):
+        # Prediction mode (src/model.cc:1044-1049): spline_mult, then spline_set.
+        mult, setv = spline_modifications(spec)
+        out = out * mult[sp]
+        out = np.where(np.isnan(setv[sp]), out, setv[sp])
     return out
+
+
+def spline_modifications(spec):
+    """Model::model_modification's spline tables (src/model.cc:1484-1490,
+    1559-1575): per spline and step the product of the spline_fac factors and
+    the spline_set value (NaN = not set); a step both set and multiplied is
+    the reference's error."""
+    S = spec["period"] * spec["spd"]
+    ns = len(spec["splines"])
+    mult, setv = np.ones((ns, S)), np.full((ns, S), np.nan)
+    for md in spec.get("modifications") or []:
+        if md["type"] not in ("spline_fac", "spline_set"):
+            continue
+        sp = md["spline"]
+        for s in range(md["start"], md["end"]):
+            if md["type"] == "spline_fac":
+                if not np.isnan(setv[sp, s]):
+                    raise ValueError("In 'modification' a spline cannot both be directly set and have a factor change")
+                mult[sp, s] *= md["factor"]
+            else:
+                if mult[sp, s] != 1:
+                    raise ValueError("In 'modification' a spline cannot both be directly set and have a factor change")
+                setv[sp, s] = md["factor"]
+    return mult, setv
+
+
+def modification_tables(spec):
+    """Model::model_modification's multipliers (src/model.cc:1446-1556):
+    beta_mult and efoi_mult [S][A][G] (each dp takes its strain's), and
+    transmean_mult [S][A][T][G]; None for a kind with no modification."""
+    A, G, T = spec["A"], spec["G"], len(spec["trans"])
+    S = spec["period"] * spec["spd"]
+    ns = max(1, len(spec.get("strain_Rfactor") or [1]))
+    dpm = G // ns
+    bm = em = tm = None
+    for md in spec.get("modifications") or []:
+        areas = md.get("areas") or list(range(A))
+        if md["type"] == "trans_rate_fac":
+            tm = np.ones((S, A, T, G)) if tm is None else tm
+            dps = md.get("dps") or list(range(G))
+            for tr in md["trans"]:
+                for s in range(md["start"], md["end"]):
+                    for c in areas:
+                        for dp in dps:
+                            tm[s, c, tr, dp] *= md["factor"]
+        elif md["type"] in ("beta_fac", "efoi_fac"):
+            st = md.get("strain", 0)
+            if md["type"] == "beta_fac":
+                bm = np.ones((S, A, G)) if bm is None else bm
+                tab = bm
+            else:
+                em = np.ones((S, A, G)) if em is None else em
+                tab = em
+            for s in range(md["start"], md["end"]):
+                for c in areas:
+                    tab[s, c, st * dpm:(st + 1) * dpm] *= md["factor"]
+    return bm, em, tm
 
 
 def largest_eigenvalue(M, rng=None):
@@ -258,8 +320,10 @@ def derive(spec, nz, pv):
         efoi = np.zeros(S)
     agedist = np.asarray(spec.get("efoi_agedist", [1.0 / G] * G), dtype=np.float64)
     geo = disc_spline(spec, spec["geo_spline"], pd) if spec.get("geo_spline", -1) >= 0 else np.ones(S)
+    bm, em, tm = modification_tables(spec)
     return dict(rate=rate, sus=sus, inf=inf, af=af, beta=beta_SA, efoi=efoi, agedist=agedist, geo=geo,
-                Ntime=Ntime, ratio=ratio[0] if not spec.get("age_mod") else ratio)
+                Ntime=Ntime, ratio=ratio[0] if not spec.get("age_mod") else ratio, beta_mult=bm, efoi_mult=em,
+                transmean_mult=tm)
 
 
 def _derive_strains(spec, nz, pd, rate, sus, inf, af, Ntime, S, A, G):
@@ -312,8 +376,9 @@ def _derive_strains(spec, nz, pd, rate, sus, inf, af, Ntime, S, A, G):
         efoi = np.zeros(S)
     agedist = np.asarray(spec.get("efoi_agedist", [1.0 / G] * G), dtype=np.float64)
     geo = disc_spline(spec, spec["geo_spline"], pd) if spec.get("geo_spline", -1) >= 0 else np.ones(S)
+    bm, em, tm = modification_tables(spec)
     return dict(rate=rate, sus=sus, inf=inf, af=af, beta=beta_SA, efoi=efoi, agedist=agedist, geo=geo,
-                Ntime=Ntime, ratio=ratio)
+                Ntime=Ntime, ratio=ratio, beta_mult=bm, efoi_mult=em, transmean_mult=tm)
 
 
 def prior(spec, pv):

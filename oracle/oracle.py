@@ -71,7 +71,7 @@ class _OrModel(C.Structure):
 
 class _OrTables(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in ("inf", "rate", "sus", "beta", "af", "efoi", "agedist", "geo",
-                                                     "Ntime")]
+                                                     "Ntime", "beta_mult", "efoi_mult", "transmean_mult")]
 
 
 class _OrState(C.Structure):
@@ -168,8 +168,11 @@ class Oracle:
         d = derive.derive(self.spec, self.nz, np.asarray(pv, dtype=np.float64))
         arrs = [np.ascontiguousarray(d[k], dtype=np.float64) for k in
                 ("inf", "rate", "sus", "beta", "af", "efoi", "agedist", "geo", "Ntime")]
-        t = _OrTables(*[_d(a) for a in arrs])
-        t._keep = arrs
+        # Prediction-mode modifications: null pointers when the spec has none.
+        mods = [None if d.get(k) is None else np.ascontiguousarray(d[k], dtype=np.float64)
+                for k in ("beta_mult", "efoi_mult", "transmean_mult")]
+        t = _OrTables(*[_d(a) for a in arrs], *[None if a is None else _d(a) for a in mods])
+        t._keep = arrs + [a for a in mods if a is not None]
         t.derived = d
         t.pv = np.asarray(pv, dtype=np.float64)
         return t

@@ -25,6 +25,7 @@ STRUCTS = {
     _lib.Covar: ("beep_covar", {}),
     _lib.Obs: ("beep_obs", {}),
     _lib.AgeMod: ("beep_agemod", {}),
+    _lib.Mod: ("beep_mod", {}),
     _lib.Spec: ("beep_spec", {}),
     _lib.Work: ("beep_work", {}),
 }

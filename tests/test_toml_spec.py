@@ -575,3 +575,140 @@ def test_reference_examples():
     assert [o["y"] for o in r] == [float(x[1]) for x in r_rows]
     assert spec["pop"][0][0] == [int(float(rows["population.csv"][0][1]) + 0.5)]
     assert not math.isnan(values[[p["name"] for p in spec["params"]].index("R-spline")])
+
+
+def _sex_ages(tmp_path, extra, areas=None):
+    """The two-area, age-by-sex fixture read in prediction mode with extra
+    top-level keys; areas replaces its areas file."""
+    import shutil
+    data = tmp_path / "data"
+    shutil.copytree(os.path.join(FIX, "data_seir_sex_ages"), data)
+    if areas is not None:
+        (data / "areas.csv").write_text(areas)
+    p = tmp_path / "m.toml"
+    p.write_text(open(os.path.join(FIX, "seir_sex_ages.toml")).read() + "\n" + extra + "\n")
+    return T.load(str(p), datadir=str(data), prediction=True)
+
+
+MODS_TOML = '''modification = [
+{type="trans_rate_fac", trans="S->E", start="7", end="21", factor="1.5", geo_filt="area:South", democats_filt="Sex:F"},
+{type="beta_fac", start="28", factor="0.5"},
+{type="efoi_fac", end="14", factor="2", geo_filt="region:Coast|Hills,area:North"},
+{type="spline_fac", name="R_t", start="35", end="42", factor="0.8"},
+{type="spline_set", name="efoi(t)", start="42", value="0"},
+{type="trans_rate_fac", trans="I->R", factor="1.2", democats_filt="age:old,Sex:M"}
+]'''
+AREAS_REGION = "area,region,young,old,M,F\nNorth,Coast,30000,10000,0.48,0.52\nSouth,Hills,20000,25000,49,51\n"
+
+
+def test_prediction_modifications(tmp_path):
+    """Modifications read in prediction mode (src/inputs.cc:1986-2061,
+    src/data.cc:1716-1826, src/model.cc:1495-1575), worked by hand: dps
+    enumerate (age, Sex) with age fastest, so Sex:F is dps 2 and 3 and
+    old-and-M is dp 1; a comma joins terms that must all hold; times are in
+    days (one step per day here); the beta and efoi factors act on the one
+    strain 'all'; splines are found by their default names."""
+    spec, values, meta = _sex_ages(tmp_path, MODS_TOML, AREAS_REGION)
+    tr = {(spec["comps"][t["from"]]["name"], spec["comps"][t["to"]]["name"]): k for k, t in enumerate(spec["trans"])}
+    m = spec["modifications"]
+    assert [x["type"] for x in m] == ["trans_rate_fac", "beta_fac", "efoi_fac", "spline_fac", "spline_set",
+                                      "trans_rate_fac"]
+    assert (m[0]["start"], m[0]["end"], m[0]["factor"]) == (7, 21, 1.5)
+    assert m[0]["trans"] == [tr[("S", "E")]] and m[0]["areas"] == [1] and m[0]["dps"] == [2, 3]
+    assert (m[1]["start"], m[1]["end"], m[1]["strain"], m[1]["areas"]) == (28, 56, 0, [0, 1])
+    assert (m[2]["start"], m[2]["end"], m[2]["factor"], m[2]["areas"]) == (0, 14, 2.0, [0])
+    assert (m[3]["spline"], m[3]["start"], m[3]["end"]) == (spec["R_spline"], 35, 42)
+    assert (m[4]["spline"], m[4]["factor"], m[4]["start"], m[4]["end"]) == (spec["efoi_spline"], 0.0, 42, 56)
+    assert m[5]["trans"] == [tr[("I", "R")]] and m[5]["dps"] == [1] and m[5]["areas"] == [0, 1]
+    assert meta["prediction"]["pred_start"] == 0
+    # Without prediction mode the entries are only recorded.
+    spec0, _, meta0 = T.load(str(tmp_path / "m.toml"), datadir=str(tmp_path / "data"))
+    assert "modifications" not in spec0 and len(meta0["prediction"]["modification"]) == 6
+    # The host model takes the list as it is: its tables equal the oracle's.
+    from oracle import derive
+    from beep_amd import _lib
+    t = _lib.spec_tables(spec, values)
+    d = derive.derive(spec, derive.normalise(spec), values)
+    assert np.allclose(t["beta_af"], d["beta"] * d["af"][np.arange(56)], rtol=1e-12, atol=0)
+    bm, em, tm = derive.modification_tables(spec)
+    assert tm[7:21, 1, tr[("S", "E")]][:, [2, 3]].min() == tm[7:21, 1, tr[("S", "E")]][:, [2, 3]].max() == 1.5
+    assert tm[:, :, tr[("I", "R")], 1].min() == 1.2 and (tm[:, 0, tr[("S", "E")]] == 1).all()
+    assert (em[:14, 0] == 2).all() and (em[:14, 1] == 1).all() and (em[14:] == 1).all()
+
+
+@pytest.mark.parametrize("entry, msg", [
+    ('{type="pop_fac", factor="2"}', "not recognised"),
+    ('{type="trans_rate_fac", trans="S->I", factor="2"}', "'S->I' is not recognised"),
+    ('{type="trans_rate_fac", trans="S->E"}', "numeric 'factor'"),
+    ('{type="beta_fac", factor="-1"}', "not positive"),
+    ('{type="beta_fac", factor="2", geo_filt="area:West"}', "'West' is not found"),
+    ('{type="beta_fac", factor="2", geo_filt="area:North|North"}', "must not be repeated"),
+    ('{type="beta_fac", factor="2", democats_filt="Sex:X"}', "cannot find the value 'X'"),
+    ('{type="beta_fac", factor="2", democats_filt="Gender:F"}', "cannot find the value 'Gender'"),
+    ('{type="beta_fac", factor="2", strain="delta"}', "'strain=\"delta\"' is not recognised"),
+    ('{type="spline_fac", name="R(t)", factor="2"}', "cannot find spline with the name 'R\\(t\\)'"),
+    ('{type="spline_set", name="R_t", factor="2"}', "numeric 'value'"),
+    ('{type="beta_fac", start="30", end="20", factor="2"}', "in that order"),
+    ('{type="beta_fac", factor="2", colour="red"}', "colour"),
+])
+def test_modification_rejections(tmp_path, entry, msg):
+    with pytest.raises(T.TomlError, match=msg):
+        _sex_ages(tmp_path, f"modification = [{entry}]")
+
+
+def test_modification_start_before_prediction_start(tmp_path):
+    with pytest.raises(T.TomlError, match="cannot be before 'pred_start'"):
+        _sex_ages(tmp_path, 'prediction_start = "30"\nmodification = [{type="beta_fac", start="20", factor="2"}]')
+    (tmp_path / "b").mkdir()
+    _, _, meta = _sex_ages(tmp_path / "b", 'prediction_start = "10"\n'
+                                          'modification = [{type="beta_fac", start="20", factor="2"}]')
+    assert meta["prediction"]["pred_start"] == 10
+
+
+def test_prediction_end_extends_the_period(tmp_path):
+    """prediction_end beyond 'end' lengthens the period (src/details.cc:92)
+    and the parameter splines reach it (src/inputs.cc:1475-1493): the last
+    knot moves when the last two knots are specified alike (one value for
+    both), else a knot copying the last is added at the new end, as its
+    own parameter when the knots are unnamed."""
+    spec, values, _ = _sex_ages(tmp_path, 'prediction_end = "70"')
+    assert spec["period"] == 70
+    assert spec["splines"][0]["t"] == [0.0, 70.0] and spec["splines"][0]["param"][0] == spec["splines"][0]["param"][1]
+    assert spec["splines"][1]["t"] == [0.0, 70.0]
+    (tmp_path / "b").mkdir()
+    spec2, _, _ = _sex_ages(tmp_path / "b", 'prediction_end = "50"')       # before 'end': no change
+    assert spec2["period"] == 56 and spec2["splines"][0]["t"] == [0.0, 56.0]
+    # A 3-knot R(t) whose last two knots differ gains a fourth.
+    p = tmp_path / "c.toml"
+    src = open(os.path.join(FIX, "seir_sex_ages.toml")).read()
+    src = src.replace('R_spline = [{ value="2.1", prior="Uniform(0.5,4)"}]',
+                      'R_spline = [{ bp="start|28|end", value="2.1|1.5|1.2", prior="Uniform(0.5,4)"}]')
+    p.write_text(src + '\nprediction_end = "70"\n')
+    spec3, values3, _ = T.load(str(p), datadir=os.path.join(FIX, "data_seir_sex_ages"), prediction=True)
+    sp = spec3["splines"][0]
+    names = [q["name"] for q in spec3["params"]]
+    assert sp["t"] == [0.0, 28.0, 56.0, 70.0]
+    assert [names[k] for k in sp["param"]] == ["R-spline t=0", "R-spline t=28", "R-spline t=56", "R-spline t=70"]
+    assert values3[sp["param"][3]] == 1.2 and spec3["params"][sp["param"][3]]["prior"] == ("uniform", 0.5, 4.0)
+    # Without prediction mode prediction_end changes nothing.
+    spec4, _, _ = T.load(str(p), datadir=os.path.join(FIX, "data_seir_sex_ages"))
+    assert spec4["period"] == 56 and spec4["splines"][0]["t"] == [0.0, 28.0, 56.0]
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="the reference's examples are not present")
+def test_reference_prediction_examples():
+    """EX_F3 and EX_F4 read in prediction mode: a doubled (EX_F3) or reduced
+    (EX_F4) S->E rate from day 80 (40), on a period extended to
+    prediction_end (EX_F3: 100 days)."""
+    f3, _, meta = T.load(os.path.join(REF_EXAMPLES, "EX_F3.toml"), datadir=os.path.join(REF_EXAMPLES, "Data_EX_F3"),
+                         prediction=True)
+    spd = f3["spd"]
+    assert f3["period"] == 100
+    (m,) = f3["modifications"]
+    se = next(k for k, t in enumerate(f3["trans"]) if f3["comps"][t["from"]]["name"] == "S")
+    assert (m["type"], m["trans"], m["start"], m["end"], m["factor"]) == ("trans_rate_fac", [se], 80 * spd,
+                                                                         100 * spd, 2.0)
+    assert meta["prediction"]["pred_start"] == 80
+    f4, _, _ = T.load(os.path.join(REF_EXAMPLES, "EX_F4.toml"), datadir=os.path.join(REF_EXAMPLES, "Data_EX_F4"),
+                      prediction=True)
+    assert [(x["start"], x["factor"]) for x in f4["modifications"]] == [(40 * f4["spd"], 0.3)]

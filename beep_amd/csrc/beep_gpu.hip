@@ -87,8 +87,8 @@ struct beep_ctx {
   double timing[4] = {0, 0, 0, 0};
   // step-wise path (populations in HBM, one launch per kernel per step)
   int stepwise = 0, dense = 0, Apad = 0;
-  float* d_M = nullptr;            // [Apad][Apad] dense off-diagonal mixing
-  float* d_D = nullptr;            // [capacity*G][Apad]
+  float* d_M = nullptr;            // [Apad][Apad] dense off-diagonal mixing, transposed, fragment order
+  float* d_D = nullptr;            // [capacity*G padded to 64][Apad], fragment order
   float* d_I = nullptr;            // [capacity*G][Apad]
   uint64_t* d_rngp = nullptr;      // seed, stream read by the captured step kernels
   struct Graph {
@@ -798,15 +798,20 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   HIPCHK(c, hipMemcpy(c->d_topo, &tp, sizeof tp, hipMemcpyHostToDevice));
   c->Apad = (m.A + 63) / 64 * 64;
   if (c->dense) {
+    // The GEMM's right operand: M transposed, in fragment order (Mt[j][a] = M[a][j]).
     std::vector<float> M((size_t)c->Apad * c->Apad, 0.f);
     for (int a = 0; a < m.A; a++)
-      for (int k = m.M_rowptr[a]; k < m.M_rowptr[a + 1]; k++) M[(size_t)a * c->Apad + m.M_to[k]] += (float)m.M_val[k];
+      for (int k = m.M_rowptr[a]; k < m.M_rowptr[a + 1]; k++)
+        M[beepdev::frag_at(m.M_to[k], a, c->Apad)] += (float)m.M_val[k];
     if ((rc = upload(c, &c->d_M, M))) return rc;
   }
   if (c->stepwise) {
-    size_t sz = (size_t)capacity * m.G * c->Apad * sizeof(float);
-    if ((rc = dalloc(c, (void**)&c->d_D, sz)) || (rc = dalloc(c, (void**)&c->d_I, sz))) return rc;
-    HIPCHK(c, hipMemset(c->d_D, 0, sz));
+    // D in fragment order over rows padded to the GEMM's 64-row tiles (the
+    // padding rows and columns stay zero); I row-major.
+    const size_t rows_pad = ((size_t)capacity * m.G + 63) / 64 * 64;
+    size_t szD = rows_pad * c->Apad * sizeof(float), sz = (size_t)capacity * m.G * c->Apad * sizeof(float);
+    if ((rc = dalloc(c, (void**)&c->d_D, szD)) || (rc = dalloc(c, (void**)&c->d_I, sz))) return rc;
+    HIPCHK(c, hipMemset(c->d_D, 0, szD));
     HIPCHK(c, hipMemset(c->d_I, 0, sz));
     if ((rc = dalloc(c, (void**)&c->d_rngp, 2 * sizeof(uint64_t)))) return rc;
   }
@@ -2439,7 +2444,15 @@ int beep_debug_gemm(int device, int rows, int K, const float* A, const float* B,
   if (rows <= 0 || K <= 0 || K % 64 || !A || !B || !C) return BEEP_EINVAL;
   const long long tiles = (long long)((rows + 63) / 64) * (K / 64);
   if (tiles > beephost::kGridMaxX / 2) return BEEP_EINVAL;
-  return debug_run(device, (size_t)rows * K * 4, A, (size_t)K * K * 4, B, (size_t)rows * K * 4, C,
+  // The operands as the dense contraction holds them: A over rows padded to
+  // 64, B transposed, both in fragment order.
+  const size_t rows_pad = ((size_t)rows + 63) / 64 * 64;
+  std::vector<float> Af(rows_pad * K, 0.f), Btf((size_t)K * K);
+  for (int r = 0; r < rows; r++)
+    for (int k = 0; k < K; k++) Af[beepdev::frag_at(r, k, K)] = A[(size_t)r * K + k];
+  for (int k = 0; k < K; k++)
+    for (int j = 0; j < K; j++) Btf[beepdev::frag_at(j, k, K)] = B[(size_t)k * K + j];
+  return debug_run(device, Af.size() * 4, Af.data(), Btf.size() * 4, Btf.data(), (size_t)rows * K * 4, C,
                    [&](void* a, void* b, void* o) {
                      hipLaunchKernelGGL(beepdev::k_dbg_gemm, dim3((unsigned)((tiles + 7) / 8 * 8)), dim3(256), 0, 0,
                                         (const float*)a, (const float*)b, (float*)o, rows, K);

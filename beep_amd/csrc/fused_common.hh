@@ -156,6 +156,19 @@ __device__ inline float mod_mean(ctopo* tp, int s, int tr, int cell, float lp) {
   return lp;
 }
 
+// Element (row, k) of a matrix with K columns (K a multiple of 8) stored in
+// MFMA fragment order: blocks of 32 rows x 8 columns, 1 KB each, ordered by
+// row block then column group; in a block, lane l = 32 h + r of a wavefront
+// holds row r's columns 4h..4h+3 as one float4.  One coalesced 16-byte load
+// per lane brings a whole block into registers as the operands of four
+// v_mfma_f32_32x32x2f32: column 8q + 4h + i feeds MFMA 4q + i.  The step-wise
+// path keeps the infectivity D in this order, and the dense mixing matrix
+// transposed in it.
+__host__ __device__ inline size_t frag_at(int row, int k, int K) {
+  return ((((size_t)(row >> 5) * (size_t)(K >> 3) + (size_t)(k >> 3)) * 64 + (size_t)(((k >> 2) & 1) * 32 + (row & 31)))
+          << 2) + (size_t)(k & 3);
+}
+
 // Queue capacity of large draws per step (items beyond it are drawn in place).
 __host__ __device__ inline int fused_qcap(int AG) { return AG < 32 ? 64 : (2 * AG < 1024 ? 2 * AG : 1024); }
 

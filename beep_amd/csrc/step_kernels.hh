@@ -16,9 +16,9 @@ namespace beepdev {
 
 // Dense contraction operands of one launch.
 struct DenseMix {
-  const float* D;      // [rows][Apad] infectivity, rows = b*G + g
-  const float* M;      // [Apad][Apad] off-diagonal mixing, M[a][j] (zero padded)
-  float* I;            // [rows][Apad] output
+  const float* D;      // infectivity, rows b*G + g (rows padded to 64), Apad columns, fragment order
+  const float* Mt;     // [Apad][Apad] off-diagonal mixing transposed, Mt[j][a] = M[a][j], fragment order
+  float* I;            // [rows][Apad] output, row-major
   const float* diag;   // [A]
   const float* fac;    // [A]
   const Job* jobs;     // row b*G+g belongs to jobs[b] (m(t) of its slot)
@@ -27,61 +27,95 @@ struct DenseMix {
 
 typedef float f16v __attribute__((ext_vector_type(16)));
 
-// One 64x64 output tile of C = A x B (A [rows][lda], B [K][ldb], K a multiple
-// of 32, tile origin (tm, tn)) per workgroup of 4 wavefronts; each wavefront
-// owns a 32x32 quadrant computed with v_mfma_f32_32x32x2f32 over K in chunks
-// of 32 staged through LDS, double-buffered: the global loads of chunk k+1 are
-// issued before the matrix instructions of chunk k, and one barrier separates
-// chunks.  The A tile is row-padded to 33 floats against bank conflicts.
+// One 64x64 output tile of C = A x B (origin (tm, tn)) per workgroup of four
+// wavefronts.  A (rows padded to a multiple of 64) and B transposed are in
+// fragment order (frag_at), K a multiple of 64.  K is split four ways: each
+// wavefront computes the whole 64x64 tile over its quarter of K with four
+// accumulators (each 16-byte operand load feeds two MFMAs, and the four
+// accumulation chains interleave), streaming its row blocks of A and column
+// blocks of B from L2 straight into registers, the next block's four loads in
+// flight while the current block's 16 MFMAs run.  The four partial tiles are
+// then summed through LDS (48 KB): wavefront w returns the sum of quadrant w
+// (rows 32 (w / 2), columns 32 (w % 2) of the tile).  2880 wavefronts for the
+// 9216 x 320 x 320 product of 1024 particles of 9 groups, 2.8 per SIMD.
 // Accumulator layout of the 32x32 MFMA: lane l holds column l % 32 and, in
 // register v, row 8*(v/4) + 4*(l/32) + v%4.
-__device__ inline f16v mfma_tile_64x64(const float* __restrict__ A, int lda, int rows, const float* __restrict__ B,
-                                       int ldb, int K, int tm, int tn) {
-  __shared__ float As[2][64][33];
-  __shared__ float Bs[2][32][64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  f16v acc;
+__device__ inline f16v mfma_tile_ksplit(const float* __restrict__ Af, const float* __restrict__ Btf, int K, int tm,
+                                        int tn) {
+  __shared__ float red[4][3][16][64];       // [writer][its other quadrants][register][lane]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nq = K >> 3, nqw = nq >> 2, q0 = wave * nqw, qe = q0 + nqw;
+  const float4* __restrict__ a4[2];
+  const float4* __restrict__ b4[2];
 #pragma unroll
-  for (int v = 0; v < 16; v++) acc[v] = 0.f;
-  // Each thread stages 8 elements of A and 8 of B per chunk.
-  float ra[8], rb[8];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      int i = tid + q * 256;
-      int r = i >> 5, k = i & 31;
-      int gr = tm + r;
-      ra[q] = gr < rows ? A[(size_t)gr * lda + k0 + k] : 0.f;
-      int kb = i >> 6, c = i & 63;
-      rb[q] = B[(size_t)(k0 + kb) * ldb + tn + c];
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      int i = tid + q * 256;
-      As[buf][i >> 5][i & 31] = ra[q];
-      Bs[buf][i >> 6][i & 63] = rb[q];
-    }
-  };
-  load(0);
-  store(0);
-  __syncthreads();
-  const int nk = K / 32;
-  for (int kc = 0; kc < nk; kc++) {
-    const int buf = kc & 1;
-    if (kc + 1 < nk) load((kc + 1) * 32);
-#pragma unroll
-    for (int kk = 0; kk < 32; kk += 2) {
-      float a = As[buf][wm + (lane & 31)][kk + (lane >> 5)];
-      float b = Bs[buf][kk + (lane >> 5)][wn + (lane & 31)];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
-    }
-    if (kc + 1 < nk) store(buf ^ 1);
-    __syncthreads();
+  for (int i = 0; i < 2; i++) {
+    a4[i] = (const float4*)Af + (size_t)((tm >> 5) + i) * nq * 64 + lane;
+    b4[i] = (const float4*)Btf + (size_t)((tn >> 5) + i) * nq * 64 + lane;
   }
-  return acc;
+  f16v acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int v = 0; v < 16; v++) acc[i][j][v] = 0.f;
+  float4 a0[2], b0[2], a1[2], b1[2];
+  // A scheduling barrier after each group of loads keeps the compiler from
+  // sinking them next to their MFMAs (which would wait out every load's
+  // latency).  The loop body has no condition around its loads, so each
+  // wait before a block's MFMAs leaves the next block's loads in flight.
+  auto load = [&](float4(&a)[2], float4(&b)[2], int q) {
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      a[i] = a4[i][q * 64];
+      b[i] = b4[i][q * 64];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  auto mfma4 = [&](float x0, float x1, float y0, float y1) {
+    acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0, y0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0, y1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1, y0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1, y1, acc[1][1], 0, 0, 0);
+  };
+  // Block component i is the operand of the block's MFMA i.
+  auto compute = [&](const float4(&a)[2], const float4(&b)[2]) {
+    mfma4(a[0].x, a[1].x, b[0].x, b[1].x);
+    mfma4(a[0].y, a[1].y, b[0].y, b[1].y);
+    mfma4(a[0].z, a[1].z, b[0].z, b[1].z);
+    mfma4(a[0].w, a[1].w, b[0].w, b[1].w);
+  };
+  load(a0, b0, q0);
+  int q = q0;
+  for (; q + 2 < qe; q += 2) {
+    load(a1, b1, q + 1);
+    compute(a0, b0);
+    load(a0, b0, q + 2);
+    compute(a1, b1);
+  }
+  load(a1, b1, q + 1);
+  compute(a0, b0);
+  compute(a1, b1);
+  // Quadrant qd of writer w goes to slot qd - (qd > w) of its three.
+#pragma unroll
+  for (int qd = 0; qd < 4; qd++) {
+    if (qd == wave) continue;
+#pragma unroll
+    for (int v = 0; v < 16; v++) red[wave][qd - (qd > wave)][v][lane] = acc[qd >> 1][qd & 1][v];
+  }
+  __syncthreads();
+  f16v s;
+#pragma unroll
+  for (int qd = 0; qd < 4; qd++)
+    if (qd == wave) s = acc[qd >> 1][qd & 1];
+#pragma unroll
+  for (int w2 = 0; w2 < 4; w2++) {
+    if (w2 == wave) continue;
+    const int slot = wave - (wave > w2);
+#pragma unroll
+    for (int v = 0; v < 16; v++) s[v] += red[w2][slot][v][lane];
+  }
+  return s;
 }
 
 // Tiles of a 64x64-tiled product for a one-dimensional grid of
@@ -103,35 +137,49 @@ __device__ inline bool xcd_tile(int ntm, int ntn, int* tm, int* tn) {
 
 // The dense contraction with its FOI epilogue, one-dimensional grid of
 // 8 * ceil(ceil(rows / 64) * (Apad / 64) / 8) workgroups (xcd_tile).
-__global__ void __launch_bounds__(256) k_dense_contract(DenseMix p) {
+// Three workgroups per CU (48 KB of LDS each) hold the grid of 720
+// workgroups of the 9216-row product at once: registers are capped for three
+// wavefronts per SIMD.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4))) k_dense_contract(DenseMix p) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int tm, tn;
   if (!xcd_tile((p.rows + 63) / 64, p.Apad / 64, &tm, &tn)) return;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  f16v acc = mfma_tile_64x64(p.D, p.Apad, p.rows, p.M, p.Apad, p.Apad, tm, tn);
+  f16v acc = mfma_tile_ksplit(p.D, p.Mt, p.Apad, tm, tn);
   const int col = tn + wn + (lane & 31);
   if (col >= p.A) return;
   const float dg = p.diag[col], fc = p.fac[col];
+  // Every load of the epilogue is issued before the first store: the stores
+  // to I may alias the job table and D as far as the compiler knows, so
+  // loads interleaved with them would wait out one latency per row.
+  const int rbase = tm + wm + 4 * (lane >> 5);
+  float mv[16], dv[16];
 #pragma unroll
   for (int v = 0; v < 16; v++) {
-    int row = tm + wm + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+    const int row = min(rbase + 8 * (v >> 2) + (v & 3), p.rows - 1);
+    mv[v] = p.jobs[row / p.G].tab[p.off_geo + p.s];
+    dv[v] = p.D[frag_at(row, col, p.Apad)];
+  }
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int row = rbase + 8 * (v >> 2) + (v & 3);
     if (row < p.rows) {
-      int b = row / p.G;
-      float m = p.jobs[b].tab[p.off_geo + p.s];
-      float idg = dg * p.D[(size_t)row * p.Apad + col];
-      p.I[(size_t)row * p.Apad + col] = (idg + acc[v]) * m + fc * idg * (1.f - m);
+      const float idg = dg * dv[v];
+      p.I[(size_t)row * p.Apad + col] = (idg + acc[v]) * mv[v] + fc * idg * (1.f - mv[v]);
     }
   }
 }
 
-// Plain GEMM with the same tiling (no epilogue) for the tests of the
-// accumulator layout: C[rows][Apad] = A[rows][Apad] x B[Apad][Apad].
-__global__ void __launch_bounds__(256) k_dbg_gemm(const float* A_, const float* B_, float* C_, int rows, int Apad) {
+// Plain GEMM with the same tiling (no epilogue) for the tests of the operand
+// order and accumulator layout: C[rows][Apad] = A x B, A and B transposed in
+// fragment order.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 4)))
+k_dbg_gemm(const float* Af, const float* Btf, float* C_, int rows, int Apad) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int tm, tn;
   if (!xcd_tile((rows + 63) / 64, Apad / 64, &tm, &tn)) return;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  f16v acc = mfma_tile_64x64(A_, Apad, rows, B_, Apad, Apad, tm, tn);
+  f16v acc = mfma_tile_ksplit(Af, Btf, Apad, tm, tn);
   const int col = tn + wn + (lane & 31);
 #pragma unroll
   for (int v = 0; v < 16; v++) {
@@ -158,10 +206,9 @@ __global__ void __launch_bounds__(256) k_sparse_contract(SparseMix p) {
   int j = blockIdx.y * blockDim.x + threadIdx.x;
   int row = blockIdx.x;
   if (j >= p.A) return;
-  const float* Dr = p.D + (size_t)row * p.Apad;
-  float idg = p.diag[j] * Dr[j];
+  float idg = p.diag[j] * p.D[frag_at(row, j, p.Apad)];
   float imp = 0.f;
-  for (int e = p.in_ptr[j]; e < p.in_ptr[j + 1]; e++) imp += p.in_w[e] * Dr[p.in_src[e]];
+  for (int e = p.in_ptr[j]; e < p.in_ptr[j + 1]; e++) imp += p.in_w[e] * p.D[frag_at(row, p.in_src[e], p.Apad)];
   float m = p.jobs[row / p.G].tab[p.off_geo + p.s];
   p.I[(size_t)row * p.Apad + j] = (idg + imp) * m + p.fac[j] * idg * (1.f - m);
 }
@@ -184,7 +231,7 @@ __global__ void __launch_bounds__(256) k_step_init(const Job* __restrict__ jobs,
     if (w != 0.f) d += w * (float)v;
   }
   int j = cell / G, g = cell - j * G;
-  ((gfloat*)D)[((size_t)blockIdx.x * G + g) * Apad + j] = d;
+  ((gfloat*)D)[frag_at(blockIdx.x * G + g, j, Apad)] = d;
 }
 
 #define BEEP_STEP_QCAP 512
@@ -245,8 +292,9 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
     const size_t offN = tp->off_N + (tp->N_tv ? (size_t)s * G * G : 0);   // a time-varying age matrix: step s's
     for (int gg = 0; gg < G; gg++) nmi += tab[offN + g * G + gg] * Ib[(size_t)gg * Apad + j];
     if (nmi < 0.f) nmi = 0.f;
-    const float foi = foi_mean(tab[tp->off_beta + s], af_row(tp, tab, s)[j], nmi, tab[tp->off_efoi + s],
-                               gptr(tp->agedist)[g]);
+    float afc = af_row(tp, tab, s)[j], agc = gptr(tp->agedist)[g];
+    mod_foi_factors(tp, s, cell, afc, agc);
+    const float foi = foi_mean(tab[tp->off_beta + s], afc, nmi, tab[tp->off_efoi + s], agc);
     const bool simu = MODE == MODE_MBP ? (jb.simu ? gptr(jb.simu)[(size_t)s * A + j] != 0 : false) : true;
     CellRng rng;
     rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
@@ -258,6 +306,7 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
       float lp = 0.f;
       if (popu > 0.f)
         lp = tr == inf_tr ? dt * popu * tab[tp->off_sus + g] * foi : dt * popu * tab[tp->off_rate + tr * G + g];
+      lp = mod_mean(tp, s, tr, cell, lp);
       if (lp != lp) { atomicOr(jb.err, 1); lp = 0.f; }
       else if (lp > 1e9f) { atomicOr(jb.err, 2); lp = 0.f; }
       if (tm_s) st_nt(tm_s, tr * AG + cell, lp);
@@ -326,7 +375,7 @@ __global__ void __launch_bounds__(BEEP_STEP_BLOCK) k_step_trans(const Job* __res
       float w = tab[tp->off_inf + c];
       if (w != 0.f) d += w * (float)v;
     }
-    ((gfloat*)D)[((size_t)b * G + g) * Apad + j] = d;
+    ((gfloat*)D)[frag_at(b * G + g, j, Apad)] = d;
   }
 }
 

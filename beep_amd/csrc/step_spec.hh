@@ -155,7 +155,7 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
         if (w != 0.f) d += w * (float)v;
       }
     }
-    ((gfloat*)D)[((size_t)b * G + g) * Apad + j] = d;
+    ((gfloat*)D)[frag_at(b * G + g, j, Apad)] = d;
   }
 }
 

@@ -821,7 +821,7 @@ def _regions(area_head, area_rows, geo_dep):
         return [("all", list(range(len(area_rows))))]
     col = _col(area_head, geo_dep)
     if col is None:
-        raise TomlError(f"'geo_dep={geo_dep}' is not a column of the areas file")
+        raise TomlError(f"cannot find the column '{geo_dep}' in the areas file")
     out = {}
     for r, row in enumerate(area_rows):
         if row[col] != "NA":
@@ -1119,24 +1119,43 @@ def _modifications(rd, entries, area_head, area_rows, demo, comps, trans, spline
 
 def _filters(e, demo, area_head, area_rows):
     """A data table's columns as (name, areas, dps) (Data::get_datafilter,
-    src/data.cc:1460-1560): one per region with geo_dep, one per value of a
-    category with democats_dep (over the dps holding it), else 'Data' over
-    every area and dp."""
+    src/data.cc:1460-1575): one per value of a category with democats_dep
+    (its dps), one per region with geo_dep (its areas), else 'Data'; each
+    narrowed by the table's geo_filt (areas) and democats_filt (dps) as
+    Data::create_area_sel / create_dp_sel read them, and a column that keeps
+    no area or no dp dropped."""
     geo_dep, demo_dep = _s(e.get("geo_dep", "")), _s(e.get("democats_dep", ""))
-    every_area, every_dp = list(range(len(area_rows))), list(range(demo.G))
+    geo_filt, demo_filt = _s(e.get("geo_filt", "")), _s(e.get("democats_filt", ""))
+    file = _s(e.get("file", ""))
     if geo_dep and demo_dep:
         raise TomlError("in 'data_tables' values for 'democats_dep' and 'geo_dep' cannot both be set")
+
+    def join(a, b):
+        return a + ("," + b if b else "")
+
+    cols = []
     if demo_dep:
         k = demo.find(demo_dep)
         if k is None:
             raise TomlError(f"'democats_dep={demo_dep}' is not a demographic category")
-        if len(demo.cats[k][1]) == 1:
+        name, vals = demo.cats[k]
+        if len(vals) == 1:
             raise TomlError(f"'democats_dep=\"{demo_dep}\"' can only be used if '{demo_dep}' has more than one value")
-        return [(v, every_area, [dp for dp, t in enumerate(demo.dps) if t[k] == i])
-                for i, v in enumerate(demo.cats[k][1])]
-    if geo_dep:
-        return [(name, areas, every_dp) for name, areas in _regions(area_head, area_rows, geo_dep)]
-    return [("Data", every_area, every_dp)]
+        areas = _area_sel(area_head, area_rows, geo_filt, f"In 'geo_filt' for '{file}'")
+        for v in vals:
+            cols.append((v, areas, _dp_sel(demo, join(f"{name}:{v}", demo_filt), f"In 'democats_filt' for '{file}'")))
+    elif geo_dep:
+        dps = _dp_sel(demo, demo_filt, f"In 'democats_filt' for '{file}'")
+        for region, _ in _regions(area_head, area_rows, geo_dep):
+            cols.append((region, _area_sel(area_head, area_rows, join(f"{geo_dep}:{region}", geo_filt),
+                                           f"In 'geo_filt' for '{file}'"), dps))
+    else:
+        cols.append(("Data", _area_sel(area_head, area_rows, geo_filt, f"In 'geo_filt' for '{file}'"),
+                     _dp_sel(demo, demo_filt, f"In 'democats_filt' for '{file}'")))
+    cols = [c for c in cols if c[1] and c[2]]
+    if not cols:
+        raise TomlError(f"In 'data_tables' the filter for '{file}' does not generate any data columns")
+    return cols
 
 
 def _obs_y(raw, thresh, thr_str, nodata_str, file):
@@ -1195,7 +1214,7 @@ def _observations(rd, doc, comps, trans, with_data, area_head, area_rows, demo, 
     for d, e in enumerate(tables):
         _check_keys(e, {"type", "observation", "timestep", "obsmodel", "file", "factor", "shift", "start", "end",
                         "threshold", "epsilon_factor", "plot_name", "line_colour", "geo_dep", "democats_dep",
-                        "factor_spline"}, "data_tables")
+                        "geo_filt", "democats_filt", "factor_spline"}, "data_tables")
         typ = _s(e.get("type"))
         if typ not in ("transition", "population", "population_fraction", "marginal"):
             raise TomlError(f"in 'data_tables' the value of 'type={typ}' is not recognised")

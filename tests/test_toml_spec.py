@@ -136,7 +136,7 @@ data_tables = [{type="transition", observation="I->H", obsmodel="normal load", f
     ('data_tables = [{type="transition", observation="I->X", obsmodel="poisson", file="I-H.csv"}]\n',
      "not a compartment or a transition"),
     ('data_tables = [{type="transition", observation="I->H", obsmodel="poisson", file="I-H.csv", geo_dep="r"}]\n',
-     "not a column of the areas file"),
+     "cannot find the column 'r' in the areas file"),
 ])
 def test_rejections(tmp_path, extra, msg):
     with pytest.raises(T.TomlError, match=msg):
@@ -712,3 +712,40 @@ def test_reference_prediction_examples():
     f4, _, _ = T.load(os.path.join(REF_EXAMPLES, "EX_F4.toml"), datadir=os.path.join(REF_EXAMPLES, "Data_EX_F4"),
                       prediction=True)
     assert [(x["start"], x["factor"]) for x in f4["modifications"]] == [(40 * f4["spd"], 0.3)]
+
+
+def test_data_table_filters(tmp_path):
+    """geo_filt and democats_filt on data tables (Data::get_datafilter,
+    src/data.cc:1460-1575): a per-sex table over the South only, a per-area
+    table of the old only, a plain table of the young women of the Coast
+    region; columns are read by their names, the selections worked by hand
+    (dps (age, Sex) with age fastest: young-M 0, old-M 1, young-F 2, old-F 3)."""
+    import shutil
+    data = tmp_path / "data"
+    shutil.copytree(os.path.join(FIX, "data_seir_sex_ages"), data)
+    (data / "areas.csv").write_text(AREAS_REGION)
+    src = open(os.path.join(FIX, "seir_sex_ages.toml")).read()
+    start = src.index("data_tables = [")
+    src = src[:start] + '''data_tables = [
+{type="transition", observation="S->E", timestep="7", obsmodel="normal 10%", democats_dep="Sex", geo_filt="area:South", file="S-E.csv"},
+{type="population", observation="R", obsmodel="normal 20", geo_dep="area", democats_filt="age:old", file="R-area.csv"},
+{type="population", observation="R", obsmodel="normal 20", geo_filt="region:Coast", democats_filt="age:young,Sex:F", file="R-all.csv"}
+]
+'''
+    (data / "R-area.csv").write_text("time,North,South\n0,1,2\n7,3,4\n")
+    (data / "R-all.csv").write_text("time,Data\n0,5\n14,6\n")
+    p = tmp_path / "f.toml"
+    p.write_text(src)
+    spec, _, _ = T.load(str(p), datadir=str(data))
+    t0, t1, t2 = ([o for o in spec["obs"] if o["datatable"] == d] for d in range(3))
+    # Per-sex columns over the South's cells only.
+    assert {(tuple(o["area"]), tuple(o["dp"])) for o in t0} == {((1,), (0, 1)), ((1,), (2, 3))}
+    # Per-area columns of the old; the young women of the Coast.
+    assert [(o["area"], o["dp"], o["y"]) for o in t1] == [([0], [1, 3], 1.0), ([0], [1, 3], 3.0),
+                                                           ([1], [1, 3], 2.0), ([1], [1, 3], 4.0)]
+    assert [(o["area"], o["dp"], o["y"]) for o in t2] == [([0], [2], 5.0), ([0], [2], 6.0)]
+    # A filter that leaves a column with no cells drops it; one that leaves none is an error.
+    bad = src.replace('geo_filt="region:Coast", democats_filt="age:young,Sex:F"', 'geo_filt="region:Coast,area:South"')
+    p.write_text(bad)
+    with pytest.raises(T.TomlError, match="does not generate any data columns"):
+        T.load(str(p), datadir=str(data))

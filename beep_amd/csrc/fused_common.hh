@@ -232,7 +232,7 @@ __host__ __device__ inline bool spec_queue16(int T, int AG) { return (size_t)T *
 __host__ __device__ inline size_t spec_lds_bytes(int A, int G, int C, int T, int nnz, int stage_mix, int nstrain = 1) {
   int AG = A * G, CP = (C + 3) / 4 * 4;
   size_t q = (size_t)T * AG;
-  size_t b = (size_t)4 * ((size_t)AG * CP + 2 * (size_t)AG + q + 4 + (size_t)T * G + (size_t)G * G + G + C);
+  size_t b = (size_t)4 * ((size_t)AG * CP + 2 * (size_t)AG + q + 4 + (size_t)T * G + (size_t)G * G + G + C + A + G);
   b += (q * (spec_queue16(T, AG) ? 2 : 4) + 3) / 4 * 4;
   if (stage_mix) b += fused_mix_bytes(A, nnz);
   if (nstrain > 1) b += (size_t)4 * AG;

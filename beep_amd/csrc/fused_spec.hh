@@ -78,7 +78,12 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   gcfloat* in_w = gptr(tpp->in_w);
   gcint* order = gptr(tpp->mix_order);
   const bool staged = tpp->stage_mix != 0;
-  float* sdiag = sinf + C;
+  // This step's area factor and the external force's age weights, read by
+  // every cell in (C): from LDS, so that no global load there waits behind the
+  // trajectory stores still in flight (loads and stores share one counter).
+  float* saf = sinf + C;                               // [A]
+  float* sage = saf + A;                               // [G]
+  float* sdiag = sage + G;
   float* sfac = sdiag + A;
   int* sptr = (int*)(sfac + A);
   int* ssrc = sptr + A + 1;
@@ -86,7 +91,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   int* sord = (int*)(sw + tpp->nnz);
   // The queue's item indices (cell * T + tr), after the tables and, when
   // staged, the mixing CSR: 16 bits each when they fit.
-  void* qidx = staged ? (void*)(sord + A) : (void*)(sinf + C);
+  void* qidx = staged ? (void*)(sord + A) : (void*)(sage + G);
   uint16_t* qidx16 = (uint16_t*)qidx;
   uint32_t* qidx32 = (uint32_t*)qidx;
   // Strains: each strain-0 cell's susceptibles pooled over the strains of its
@@ -109,6 +114,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     for (int i = tid; i < G * G; i += nt) sN[i] = tab[tpp->off_N + i];
     for (int i = tid; i < G; i += nt) ssus[i] = tab[tpp->off_sus + i];
     for (int i = tid; i < C; i += nt) sinf[i] = tab[tpp->off_inf + i];
+    for (int i = tid; i < G; i += nt) sage[i] = gptr(tpp->agedist)[i];
     if (tid == 0) qcount[0] = 0;
   }
   __syncthreads();
@@ -142,7 +148,9 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     const float beta_s = tab[tpp->off_beta + s];
     gcfloat* af_s = af_row(tpp, tab, s);
     const float efoi_s = tab[tpp->off_efoi + s];
-    gcfloat* agedist = gptr(tpp->agedist);
+    // Read in (C) after the barrier that ends (B); step s-1's reads ended at
+    // the barriers of (C), (D) and (E).
+    for (int i = tid; i < A; i += nt) saf[i] = af_s[i];
     // A time-varying age matrix: this step's.  Step s-1's reads of sN in (C)
     // ended at the barrier after (C); these writes are seen after (B)'s.
     if (tpp->N_tv) {
@@ -168,6 +176,22 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     const uint32_t c0 = (uint32_t)s | (stream << 16);
     for (int cell = tid; cell < AG; cell += nt) {
       const int j = cell / G, g = cell - j * G;
+      // The cell's global reads are issued before its first store: a load
+      // issued after a store waits for that store too.
+      const bool simu = mode == MODE_MBP ? (simu_s ? simu_s[j] != 0 : false) : true;
+      int nrep[T], nini[T];
+      float lini[T];
+      if (mode == MODE_REPLAY) {
+#pragma unroll
+        for (int tr = 0; tr < T; tr++) nrep[tr] = tn_r[tr * AG + cell];
+      }
+      if (mode == MODE_MBP && !simu) {
+#pragma unroll
+        for (int tr = 0; tr < T; tr++) {
+          nini[tr] = itn[tr * AG + cell];
+          lini[tr] = itm[tr * AG + cell];
+        }
+      }
       int p[CP];
       {
         const int4* pv = (const int4*)pop;
@@ -196,10 +220,9 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
 #pragma unroll
       for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * sI[j * G + gg];
       if (nmi < 0.f) nmi = 0.f;
-      float afc = af_s[j], agc = agedist[g];
+      float afc = saf[j], agc = sage[g];
       mod_foi_factors(tpp, s, cell, afc, agc);
       const float foi = foi_mean(beta_s, afc, nmi, efoi_s, agc);
-      const bool simu = mode == MODE_MBP ? (simu_s ? simu_s[j] != 0 : false) : true;
       CellRng rng;
       rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
       if (mode != MODE_REPLAY) rng.prime();
@@ -221,15 +244,15 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
         int kind = -1;
         float val = 0.f;
         if (mode == MODE_REPLAY) {
-          n = tn_r[tr * AG + cell];
+          n = nrep[tr];
         } else if (lp > 0.f) {
           if (mode == MODE_SIM || simu) {
             if (lp < BEEP_POIS_INLINE) n = poisson_inversion(lp, rng.next());
             else { kind = ITEM_POIS; val = lp; }
           } else {
             // Coupled proposal (src/mbp.cc:286-311).
-            int ni = itn[tr * AG + cell];
-            float li = itm[tr * AG + cell];
+            const int ni = nini[tr];
+            const float li = lini[tr];
             if (lp > li) {
               float dl = lp - li;
               if (dl < BEEP_POIS_INLINE) n = ni + poisson_inversion(dl, rng.next());
@@ -342,7 +365,7 @@ extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_
 }
 // The coupled proposal needs more registers than a simulation: at most six
 // resident wavefronts per SIMD (80 registers) keep it from spilling.
-extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_WAVES < 6 ? beepspec::MIN_WAVES : 6)
+extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_WAVES < 5 ? beepspec::MIN_WAVES : 5)
     beep_fused_spec_mbp(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
   beepdev::fused_spec_body<beepdev::MODE_MBP>(jobs, seed, stream);
 }

@@ -56,12 +56,16 @@ def test_stepwise_sparse_equals_fused(gpu_lib, make):
     b.close()
 
 
-def _small_dense():
-    return configs.dense_pmcmc(A=64, period=40, with_obs=True)
+def _small_dense(A=64):
+    return configs.dense_pmcmc(A=A, period=40, with_obs=True)
 
 
-def test_dense_replay_matches_oracle(gpu_lib):
-    spec, truth = _small_dense()
+@pytest.mark.parametrize("A", [64, 100], ids=["A64", "A100_padded_to_128"])
+def test_dense_replay_matches_oracle(gpu_lib, A):
+    """A replay on the dense (MFMA) path against the FP64 oracle; at A = 100
+    the GEMM's K and N are padded to 128, the padding rows and columns of the
+    operands zero."""
+    spec, truth = _small_dense(A)
     orc = O.Oracle(spec)
     th = truth.copy()
     th[0] = 2.5

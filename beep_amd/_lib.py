@@ -106,7 +106,10 @@ class Spec(C.Structure):
                 ("nage", C.c_int32), ("dp_age", C.POINTER(C.c_int32)),
                 ("nsus_extra", C.c_int32), ("sus_extra", C.POINTER(C.c_int32)),
                 ("nstrain", C.c_int32), ("strain_Rfactor", C.POINTER(C.c_int32)),
-                ("nmod", C.c_int32), ("mod", C.POINTER(Mod))]
+                ("nmod", C.c_int32), ("mod", C.POINTER(Mod)),
+                ("nrgroup", C.c_int32), ("rgroup_spline", C.POINTER(C.c_int32)), ("rgroup_area", C.POINTER(C.c_int32)),
+                ("negroup", C.c_int32), ("egroup_spline", C.POINTER(C.c_int32)), ("egroup_cell", C.POINTER(C.c_int32)),
+                ("egroup_agedist", C.POINTER(C.c_double))]
 
 
 _lib = None
@@ -356,6 +359,27 @@ class _SpecBuffers:
             ma[i].strain = md.get("strain", 0)
         keep.append(ma)
         s.nmod, s.mod = len(mods), ma
+        # R(t) per area group and the external force per (strain, area) group.
+        rs = spec.get("R_splines")
+        if rs:
+            rarea = np.full(spec["A"], -1, dtype=np.int32)
+            for k, e in enumerate(rs):
+                rarea[np.asarray(e["areas"], dtype=np.int64)] = k
+            s.nrgroup, s.rgroup_spline, s.rgroup_area = len(rs), i32([e["spline"] for e in rs]), i32(rarea)
+        else:
+            s.nrgroup, s.rgroup_spline, s.rgroup_area = 0, null_i32, null_i32
+        es = spec.get("efoi_splines")
+        if es:
+            ns = s.nstrain
+            ecell = np.full((ns, spec["A"]), -1, dtype=np.int32)
+            for k, e in enumerate(es):
+                for st in e.get("strains", range(ns)):
+                    ecell[st, np.asarray(e["areas"], dtype=np.int64)] = k
+            s.negroup, s.egroup_spline = len(es), i32([e["spline"] for e in es])
+            s.egroup_cell = i32(ecell.reshape(-1))
+            s.egroup_agedist = f64(np.concatenate([np.asarray(e["agedist"], dtype=np.float64) for e in es]))
+        else:
+            s.negroup, s.egroup_spline, s.egroup_cell, s.egroup_agedist = 0, null_i32, null_i32, null_f64
         self.spec = s
 
 
@@ -404,7 +428,7 @@ def spec_tables(spec, paramval):
     pv = _f64(paramval)
     Nshape = (S, G, G) if spec.get("age_mod") else (G, G)      # a time-varying age matrix: one per step
     out = dict(rate=np.zeros((T, G)), sus=np.zeros(G), inf=np.zeros(C_), N=np.zeros(Nshape),
-               beta_af=np.zeros((S, A)), efoi=np.zeros((S, G)), geo=np.zeros(S))
+               beta_af=np.zeros((S, A)), efoi=np.zeros((S, A, G)), geo=np.zeros(S))
     Pr = np.zeros(1)
     inb = np.zeros(1, dtype=np.int32)
     err = C.create_string_buffer(256)
@@ -583,7 +607,7 @@ class Context:
         A, G, C_, T, S = self.A, self.G, self.C, self.T, self.S
         Nshape = (S, G, G) if self.spec.get("age_mod") else (G, G)
         out = dict(rate=np.zeros((T, G)), sus=np.zeros(G), inf=np.zeros(C_), N=np.zeros(Nshape),
-                   beta_af=np.zeros((S, A)), efoi=np.zeros((S, G)), geo=np.zeros(S))
+                   beta_af=np.zeros((S, A)), efoi=np.zeros((S, A, G)), geo=np.zeros(S))
         self._chk(lib().beep_get_tables(self._h, slot, *[_p(out[k], C.c_double) for k in
                                                          ("rate", "sus", "inf", "N", "beta_af", "efoi", "geo")]))
         return out

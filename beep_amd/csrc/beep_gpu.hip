@@ -644,7 +644,7 @@ int beep_spec_tables(const beep_spec* spec, const double* paramval, double* rate
     std::copy(ba.begin(), ba.end(), beta_af);
   }
   if (efoi) {
-    std::vector<double> eg = m.efoi_groups(tb);
+    std::vector<double> eg = m.efoi_cells(tb);
     std::copy(eg.begin(), eg.end(), efoi);
   }
   if (geo) std::copy(tb.geo.begin(), tb.geo.end(), geo);
@@ -717,15 +717,21 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   tp.off_inf = off; off += m.C;
   tp.N_tv = m.agemod.empty() ? 0 : 1;
   tp.off_N = off; off += m.G * m.G * (tp.N_tv ? m.S : 1);
-  tp.off_efoi = off; off += m.S;
+  tp.off_efoi = off; off += m.S * m.nE;
   tp.off_geo = off; off += m.S;
-  tp.off_beta = off; off += m.S;
+  tp.off_beta = off; off += m.S * m.nR;
   // The area factor: in each slot when it depends on the parameters, else one
   // table for the context.
   {
     float* d_agedist = nullptr;
-    if ((rc = upload(c, &d_agedist, to_f32(m.efoi_agedist)))) return rc;
+    if ((rc = upload(c, &d_agedist, to_f32(m.egrp_agedist)))) return rc;
     tp.agedist = d_agedist;
+    int *d_rgrp = nullptr, *d_egrp = nullptr;
+    if ((rc = upload(c, &d_rgrp, m.rgrp_area)) || (rc = upload(c, &d_egrp, m.egrp_cell))) return rc;
+    tp.nR = m.nR;
+    tp.nE = m.nE;
+    tp.rgrp = d_rgrp;
+    tp.egrp = d_egrp;
   }
   // Prediction-mode modifications: context-wide multiplier tables, null when
   // the model has none of a kind (the kernels then read nothing more).
@@ -773,12 +779,13 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
     // instances per CU beat gathering through the caches at 4.
     // BEEP_STAGE_MIX=0/1 forces the choice (a tuning knob).
     int stage = tp.stage_mix;
+    const int grouped = m.nR > 1 || m.nE > 1;
     if (!c->stepwise) {
       const bool small = beepdev::fused_mix_bytes(m.A, tp.nnz) <= 32768;
-      stage = small && beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, 1, m.nstrain) <= 160 * 1024;
+      stage = small && beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, 1, m.nstrain, grouped, m.nE) <= 160 * 1024;
       if (const char* e = std::getenv("BEEP_STAGE_MIX")) stage = small && std::atoi(e) != 0;
     }
-    size_t sl = beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, stage, m.nstrain);
+    size_t sl = beepdev::spec_lds_bytes(m.A, m.G, m.C, m.T, tp.nnz, stage, m.nstrain, grouped, m.nE);
     if (c->stepwise ? step_lds <= 65536 : sl <= 160 * 1024) {
       SpecKernels* k = nullptr;
       if ((rc = spec_kernels(c, &k))) return rc;
@@ -794,6 +801,9 @@ int beep_create(const beep_spec* spec, int device, size_t hbm_budget, int capaci
   if (m.nstrain > 1 && (c->stepwise || !c->spec))
     return fail(c, BEEP_EINVAL, "a model with strains runs on the specialised fused kernel only (an instance in LDS, "
                                 "sparse mixing, not BEEP_GENERIC_KERNEL or BEEP_STEPWISE)");
+  if ((m.nR > 1 || m.nE > 1) && !c->spec)
+    return fail(c, BEEP_EINVAL, "R or external-force splines per area or strain run on the specialised kernels only "
+                                "(not BEEP_GENERIC_KERNEL)");
   if ((rc = dalloc(c, (void**)&c->d_topo, sizeof(TopoDev)))) return rc;
   HIPCHK(c, hipMemcpy(c->d_topo, &tp, sizeof tp, hipMemcpyHostToDevice));
   c->Apad = (m.A + 63) / 64 * 64;
@@ -1184,7 +1194,7 @@ int beep_get_tables(beep_ctx* c, int slot, double* rate, double* sus, double* in
     std::copy(ba.begin(), ba.end(), beta_af);
   }
   if (efoi) {
-    std::vector<double> eg = c->m.efoi_groups(tb);
+    std::vector<double> eg = c->m.efoi_cells(tb);
     std::copy(eg.begin(), eg.end(), efoi);
   }
   if (geo) std::copy(tb.geo.begin(), tb.geo.end(), geo);
@@ -1621,7 +1631,7 @@ int beep_derive_slots(beep_ctx* c, int n, const int32_t* dst, const int32_t* src
   HIPCHK(c, hipMemcpyAsync(d_f, f.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
   int bx = (int)std::min<size_t>(64, (c->slot_stride + 255) / 256);
   hipLaunchKernelGGL(beepdev::k_derive_slot, dim3(n, bx), dim3(256), 0, c->stream, c->d_tab, c->slot_stride, d_dst,
-                     d_src, d_f, d_from, c->h_topo.off_beta, c->m.S);
+                     d_src, d_f, d_from, c->h_topo.off_beta, c->m.S, c->m.nR);
   HIPCHK(c, hipGetLastError());
   const size_t osz = (size_t)c->nospl * c->m.S;     // observation splines: the source slot's
   for (int i = 0; c->nospl && i < n; i++)

@@ -32,13 +32,19 @@ struct TopoDev {
   // on the parameters (off_af >= 0), else this context-wide table.
   const float* af;
   int off_af;
-  const float* agedist;  // [G] age distribution of the external force of infection (efoi[S] per slot)
+  const float* agedist;  // [nE][G] age weights of the external force of each group (efoi[S][nE] per slot)
   // Prediction-mode modifications, null when absent: multipliers of beta and
   // of the external force per step and cell, [S][AG], and of every transition
   // mean, [S][T][AG].
   const float* mod_beta;
   const float* mod_efoi;
   const float* mod_trans;
+  // R(t) per area group and the external force per (strain, area) group: the
+  // slot holds beta[S][nR] and efoi[S][nE]; rgrp[A] and egrp[nstrain][A] name
+  // each area's and (strain, area)'s group.  nR = nE = 1: one of each.
+  int nR, nE;
+  const int* rgrp;
+  const int* egrp;
 };
 
 // One instance's work in a launch.
@@ -150,6 +156,21 @@ __device__ inline void mod_foi_factors(ctopo* tp, int s, int cell, float& af, fl
   if (tp->mod_efoi) agedist *= gptr(tp->mod_efoi)[(size_t)s * tp->AG + cell];
 }
 
+// The group-dependent factors of the force of infection of cell (area j,
+// dp g) at step s, read from the slot tables: beta of the area's R group, the
+// external force of its (strain, area) group and that group's age weight
+// (State::set_transmean, src/state.cc:410-424, with
+// beta[st][c][s] and efoi_spl_ref[st][c]).  dpm = dps per strain.
+__device__ inline void foi_groups(ctopo* tp, gcfloat* tab, int s, int j, int g, int dpm, float& beta, float& efoi,
+                                  float& agedist) {
+  const int nR = tp->nR, nE = tp->nE;
+  const int r = nR > 1 ? gptr(tp->rgrp)[j] : 0;
+  const int e = nE > 1 ? gptr(tp->egrp)[(g / dpm) * tp->A + j] : 0;
+  beta = tab[tp->off_beta + (size_t)s * nR + r];
+  efoi = tab[tp->off_efoi + (size_t)s * nE + e];
+  agedist = gptr(tp->agedist)[(size_t)e * tp->G + g];
+}
+
 // A transition mean times its trans_rate_fac multiplier.
 __device__ inline float mod_mean(ctopo* tp, int s, int tr, int cell, float lp) {
   if (tp->mod_trans) lp *= gptr(tp->mod_trans)[((size_t)s * tp->T + tr) * tp->AG + cell];
@@ -229,10 +250,16 @@ __host__ __device__ inline bool spec_queue16(int T, int AG) { return (size_t)T *
 // mixing CSR.
 // With strains, the susceptibles pooled over strains take one int per cell
 // more, after the queue's indices.
-__host__ __device__ inline size_t spec_lds_bytes(int A, int G, int C, int T, int nnz, int stage_mix, int nstrain = 1) {
+// With R or external-force groups (grouped), each step's beta of every area
+// and external force of every (strain, area) are staged too, with each
+// (strain, area)'s group, and the age weights of every external-force group
+// (nE of them).
+__host__ __device__ inline size_t spec_lds_bytes(int A, int G, int C, int T, int nnz, int stage_mix, int nstrain = 1,
+                                                 int grouped = 0, int nE = 1) {
   int AG = A * G, CP = (C + 3) / 4 * 4;
   size_t q = (size_t)T * AG;
-  size_t b = (size_t)4 * ((size_t)AG * CP + 2 * (size_t)AG + q + 4 + (size_t)T * G + (size_t)G * G + G + C + A + G);
+  size_t b = (size_t)4 * ((size_t)AG * CP + 2 * (size_t)AG + q + 4 + (size_t)T * G + (size_t)G * G + G + C + A +
+                          (size_t)nE * G + (grouped ? (size_t)A * (1 + 2 * (size_t)nstrain) : 0));
   b += (q * (spec_queue16(T, AG) ? 2 : 4) + 3) / 4 * 4;
   if (stage_mix) b += fused_mix_bytes(A, nnz);
   if (nstrain > 1) b += (size_t)4 * AG;

@@ -82,8 +82,14 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   // every cell in (C): from LDS, so that no global load there waits behind the
   // trajectory stores still in flight (loads and stores share one counter).
   float* saf = sinf + C;                               // [A]
-  float* sage = saf + A;                               // [G]
-  float* sdiag = sage + G;
+  const int nE = tpp->nE, grouped = tpp->nR > 1 || nE > 1;
+  float* sage = saf + A;                               // [nE][G] age weights of each external-force group
+  // With R or external-force groups: this step's beta of every area and
+  // external force of every (strain, area), and each (strain, area)'s group.
+  float* sbeta = sage + (size_t)nE * G;                // [A]
+  float* sefoi = sbeta + (grouped ? A : 0);            // [NSTRAIN][A]
+  int* segrp = (int*)(sefoi + (grouped ? NSTRAIN * A : 0));   // [NSTRAIN][A]
+  float* sdiag = (float*)(segrp + (grouped ? NSTRAIN * A : 0));
   float* sfac = sdiag + A;
   int* sptr = (int*)(sfac + A);
   int* ssrc = sptr + A + 1;
@@ -91,7 +97,7 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
   int* sord = (int*)(sw + tpp->nnz);
   // The queue's item indices (cell * T + tr), after the tables and, when
   // staged, the mixing CSR: 16 bits each when they fit.
-  void* qidx = staged ? (void*)(sord + A) : (void*)(sage + G);
+  void* qidx = staged ? (void*)(sord + A) : (void*)sdiag;
   uint16_t* qidx16 = (uint16_t*)qidx;
   uint32_t* qidx32 = (uint32_t*)qidx;
   // Strains: each strain-0 cell's susceptibles pooled over the strains of its
@@ -114,7 +120,9 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     for (int i = tid; i < G * G; i += nt) sN[i] = tab[tpp->off_N + i];
     for (int i = tid; i < G; i += nt) ssus[i] = tab[tpp->off_sus + i];
     for (int i = tid; i < C; i += nt) sinf[i] = tab[tpp->off_inf + i];
-    for (int i = tid; i < G; i += nt) sage[i] = gptr(tpp->agedist)[i];
+    for (int i = tid; i < nE * G; i += nt) sage[i] = gptr(tpp->agedist)[i];
+    if (grouped)
+      for (int i = tid; i < NSTRAIN * A; i += nt) segrp[i] = tpp->nE > 1 ? gptr(tpp->egrp)[i] : 0;
     if (tid == 0) qcount[0] = 0;
   }
   __syncthreads();
@@ -151,6 +159,11 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
     // Read in (C) after the barrier that ends (B); step s-1's reads ended at
     // the barriers of (C), (D) and (E).
     for (int i = tid; i < A; i += nt) saf[i] = af_s[i];
+    if (grouped) {
+      const int nR = tpp->nR;
+      for (int i = tid; i < A; i += nt) sbeta[i] = tab[tpp->off_beta + (size_t)s * nR + (nR > 1 ? gptr(tpp->rgrp)[i] : 0)];
+      for (int i = tid; i < NSTRAIN * A; i += nt) sefoi[i] = tab[tpp->off_efoi + (size_t)s * nE + segrp[i]];
+    }
     // A time-varying age matrix: this step's.  Step s-1's reads of sN in (C)
     // ended at the barrier after (C); these writes are seen after (B)'s.
     if (tpp->N_tv) {
@@ -220,9 +233,17 @@ __device__ inline void fused_spec_body(const Job* __restrict__ jobs, uint64_t se
 #pragma unroll
       for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * sI[j * G + gg];
       if (nmi < 0.f) nmi = 0.f;
-      float afc = saf[j], agc = sage[g];
+      float afc = saf[j], agc, bet = beta_s, ef = efoi_s;
+      if (grouped) {
+        const int k = (g / DPM) * A + j;
+        bet = sbeta[j];
+        ef = sefoi[k];
+        agc = sage[segrp[k] * G + g];
+      } else {
+        agc = sage[g];
+      }
       mod_foi_factors(tpp, s, cell, afc, agc);
-      const float foi = foi_mean(beta_s, afc, nmi, efoi_s, agc);
+      const float foi = foi_mean(bet, afc, nmi, ef, agc);
       CellRng rng;
       rng.init(k0, k1, c0, (uint32_t)cell, jb.key);
       if (mode != MODE_REPLAY) rng.prime();

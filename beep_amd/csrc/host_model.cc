@@ -405,7 +405,73 @@ std::string HostModel::build(const beep_spec& s) {
   af_param = !area_effect_param.empty() || !covar.empty() || !level_param.empty();
   af_const.clear();
   if (!af_param) af_const = area_factor(std::vector<double>(nparam, 0.0));
+  const std::string eg = build_groups(s);
+  if (!eg.empty()) return eg;
   return build_modifications(s);
+}
+
+// The groups of R(t) and of the external force (Model::update_R_ref and
+// update_efoi_ref, src/model.cc:384-480): each R group's dp distribution over
+// its areas' initial populations (SplineInfo::democatpos_dist, the ratio's
+// weights), each external-force group's spline and age weights.  Without
+// groups in the spec: one of each, R_spline and efoi_spline everywhere.
+std::string HostModel::build_groups(const beep_spec& s) {
+  const int nspl = (int)spline.size();
+  if (s.nrgroup < 0 || (s.nrgroup > 0 && (!s.rgroup_spline || !s.rgroup_area))) return "bad R-spline groups";
+  if (s.negroup < 0 || (s.negroup > 0 && (!s.egroup_spline || !s.egroup_cell || !s.egroup_agedist)))
+    return "bad external-force groups";
+  if (s.nrgroup > 1 && nstrain > 1) return "R splines per area are not supported with strains";
+  nR = s.nrgroup > 0 ? s.nrgroup : 1;
+  if (s.nrgroup > 0) {
+    rgrp_spline.assign(s.rgroup_spline, s.rgroup_spline + nR);
+    rgrp_area.assign(s.rgroup_area, s.rgroup_area + A);
+  } else {
+    rgrp_spline.assign(1, R_spline);
+    rgrp_area.assign(A, 0);
+  }
+  for (int sp : rgrp_spline)
+    if (sp < 0 || sp >= nspl) return "R-spline group: spline out of range";
+  std::vector<int> nareas(nR, 0);
+  for (int r : rgrp_area) {
+    if (r < 0 || r >= nR) return "R-spline group of an area out of range";
+    nareas[r]++;
+  }
+  for (int n : nareas)
+    if (n == 0) return "an R-spline group has no area";
+  if (nR == 1) {
+    rgrp_dist.assign(1, Rdist);
+  } else {
+    rgrp_dist.assign(nR, std::vector<double>(G, 0.0));
+    std::vector<double> tot(nR, 0.0);
+    for (int c = 0; c < A; c++)
+      for (int co = 0; co < C; co++)
+        for (int g = 0; g < G; g++) {
+          const double p = pop_init[((size_t)c * C + co) * G + g];
+          rgrp_dist[rgrp_area[c]][g] += p;
+          tot[rgrp_area[c]] += p;
+        }
+    for (int r = 0; r < nR; r++) {
+      if (!(tot[r] > 0)) return "an R-spline group's areas have no population";
+      for (double& v : rgrp_dist[r]) v /= tot[r];
+    }
+  }
+  nE = s.negroup > 0 ? s.negroup : 1;
+  if (s.negroup > 0) {
+    egrp_spline.assign(s.egroup_spline, s.egroup_spline + nE);
+    egrp_cell.assign(s.egroup_cell, s.egroup_cell + (size_t)nstrain * A);
+    egrp_agedist.assign(s.egroup_agedist, s.egroup_agedist + (size_t)nE * G);
+    for (int sp : egrp_spline)
+      if (sp < 0 || sp >= nspl) return "external-force group: spline out of range";
+    for (int e : egrp_cell)
+      if (e < 0 || e >= nE) return "external-force group of a cell out of range";
+    for (double v : egrp_agedist)
+      if (!(v >= 0) || std::isinf(v)) return "external-force age weights must be finite and non-negative";
+  } else {
+    egrp_spline.assign(1, efoi_spline);
+    egrp_cell.assign((size_t)nstrain * A, 0);
+    egrp_agedist = efoi_agedist;
+  }
+  return "";
 }
 
 // The tables of Model::model_modification's loop (src/model.cc:1495-1575):
@@ -608,14 +674,14 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
     if (!invert(V, NV, Vinv)) { err = "singular transition matrix V"; return BEEP_EINVAL; }
     // calculate_R_beta_ratio_using_NGM: dominant eigenvalue of F V^-1 for one
     // age contact matrix Nm (src/model.cc:2485-2576).
-    auto ngm_ratio = [&](const double* Nm) {
+    auto ngm_ratio = [&](const double* Nm, const std::vector<double>& dist) {
       std::vector<double> F((size_t)G * NV, 0.0);
       for (int si = 0; si < nis; si++) {
         double inf = tb.inf[inf_state[si]];
         if (inf == 0) continue;
         for (int dp = 0; dp < G; dp++) {
           int i = si * G + dp;
-          for (int q = 0; q < G; q++) F[(size_t)q * NV + i] += Rdist[q] * tb.sus[q] * inf * Nm[(size_t)q * G + dp];
+          for (int q = 0; q < G; q++) F[(size_t)q * NV + i] += dist[q] * tb.sus[q] * inf * Nm[(size_t)q * G + dp];
         }
       }
       std::vector<double> NGM((size_t)G * G, 0.0);
@@ -630,30 +696,33 @@ int HostModel::derive(const double* paramv, Tables& tb, std::string& err) const 
     // The ratio is recomputed only where the matrix differs from the previous
     // step's by more than TINY = 1e-8 in some entry (src/model.cc:1337-1339,1303-1314).
     const bool tv = !agemod.empty();
-    std::vector<double> ratio(tv ? S : 1);
-    for (int s = 0; s < (int)ratio.size(); s++) {
-      const double* Ns = tb.Ntime.data() + (size_t)s * G * G;
-      bool same = s > 0;
-      for (int k = 0; same && k < G * G; k++) {
-        const double v1 = Ns[k - G * G], v2 = Ns[k];
-        if (v1 < v2 - 1e-8 || v1 > v2 + 1e-8) same = false;
+    const double Rfac = Rfactor_param >= 0 ? pd[Rfactor_param] : 1.0;
+    tb.beta.assign((size_t)S * nR, 0.0);
+    // Each R group's ratio over its own areas' dp distribution.
+    for (int r = 0; r < nR; r++) {
+      std::vector<double> ratio(tv ? S : 1);
+      for (int s = 0; s < (int)ratio.size(); s++) {
+        const double* Ns = tb.Ntime.data() + (size_t)s * G * G;
+        bool same = s > 0;
+        for (int k = 0; same && k < G * G; k++) {
+          const double v1 = Ns[k - G * G], v2 = Ns[k];
+          if (v1 < v2 - 1e-8 || v1 > v2 + 1e-8) same = false;
+        }
+        ratio[s] = same ? ratio[s - 1] : ngm_ratio(Ns, rgrp_dist[r]);
+        if (!(ratio[s] > 0) || std::isnan(ratio[s])) {
+          err = "next-generation matrix has no positive dominant eigenvalue";
+          return BEEP_EINVAL;
+        }
       }
-      ratio[s] = same ? ratio[s - 1] : ngm_ratio(Ns);
-      if (!(ratio[s] > 0) || std::isnan(ratio[s])) {
-        err = "next-generation matrix has no positive dominant eigenvalue";
-        return BEEP_EINVAL;
-      }
+      const std::vector<double> Rt = disc_spline(rgrp_spline[r], pd);
+      for (int s = 0; s < S; s++) tb.beta[(size_t)s * nR + r] = Rfac * Rt[s] / ratio[tv ? s : 0];
     }
-    double Rfac = Rfactor_param >= 0 ? pd[Rfactor_param] : 1.0;
-    std::vector<double> Rt = disc_spline(R_spline, pd);
-
-    tb.beta.assign(S, 0.0);
-    for (int s = 0; s < S; s++) tb.beta[s] = Rfac * Rt[s] / ratio[tv ? s : 0];
   }
-  tb.efoi.assign(S, 0.0);
-  if (efoi_spline >= 0) {
-    std::vector<double> e = disc_spline(efoi_spline, pd);
-    for (int s = 0; s < S; s++) tb.efoi[s] = e[s] / popsize;
+  tb.efoi.assign((size_t)S * nE, 0.0);
+  for (int e = 0; e < nE; e++) {
+    if (egrp_spline[e] < 0) continue;
+    const std::vector<double> v = disc_spline(egrp_spline[e], pd);
+    for (int s = 0; s < S; s++) tb.efoi[(size_t)s * nE + e] = v[s] / popsize;
   }
   tb.geo.assign(S, 1.0);
   if (geo_spline >= 0) tb.geo = disc_spline(geo_spline, pd);
@@ -694,10 +763,15 @@ std::vector<double> HostModel::area_factor(const std::vector<double>& pd) const 
   return af;
 }
 
-std::vector<double> HostModel::efoi_groups(const Tables& tb) const {
-  std::vector<double> out((size_t)S * G);
+std::vector<double> HostModel::efoi_cells(const Tables& tb) const {
+  const int dpm = G / nstrain;
+  std::vector<double> out((size_t)S * A * G);
   for (int s = 0; s < S; s++)
-    for (int g = 0; g < G; g++) out[(size_t)s * G + g] = tb.efoi[s] * efoi_agedist[g];
+    for (int c = 0; c < A; c++)
+      for (int g = 0; g < G; g++) {
+        const int e = egrp_cell[(size_t)(g / dpm) * A + c];
+        out[((size_t)s * A + c) * G + g] = tb.efoi[(size_t)s * nE + e] * egrp_agedist[(size_t)e * G + g];
+      }
   return out;
 }
 
@@ -705,7 +779,8 @@ std::vector<double> HostModel::beta_af(const Tables& tb) const {
   const std::vector<double>& af = af_param ? tb.af : af_const;
   std::vector<double> out((size_t)S * A, 0.0);
   for (int s = 0; s < S; s++)
-    for (int c = 0; c < A; c++) out[(size_t)s * A + c] = tb.beta[s] * af[(size_t)(s / spd) * A + c];
+    for (int c = 0; c < A; c++)
+      out[(size_t)s * A + c] = tb.beta[(size_t)s * nR + rgrp_area[c]] * af[(size_t)(s / spd) * A + c];
   return out;
 }
 

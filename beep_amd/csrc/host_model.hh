@@ -33,10 +33,10 @@ struct Tables {
   std::vector<double> sus;      // [G]
   std::vector<double> inf;      // [C]
   std::vector<double> Ntime;    // [G][G], or [S][G][G] with age matrix modifications
-  std::vector<double> beta;     // [S] Rfactor * R(t_s) / NGM ratio (strains: R(t_s))
+  std::vector<double> beta;     // [S][nR] Rfactor * R(t_s) / NGM ratio of each R group (strains: R(t_s))
   std::vector<double> af;       // [period][A] area factor; empty when it does not depend
                                 // on the parameters (HostModel::af_const holds it)
-  std::vector<double> efoi;     // [S] efoi(t_s) / popsize (times efoi_agedist[g] per group)
+  std::vector<double> efoi;     // [S][nE] efoi(t_s) / popsize of each external-force group
   std::vector<double> geo;      // [S]    m(t_s)
   std::vector<double> obs_spl;  // [nobs_spline][S] observation spline values
 };
@@ -68,7 +68,15 @@ class HostModel {
   std::vector<int> strain_Rfactor;                            // [nstrain] parameter, -1 = 1
   int R_spline = -1, Rfactor_param = -1, efoi_spline = -1, geo_spline = -1;
   std::vector<double> efoi_agedist;
-  std::vector<double> Rdist;          // [G] population fraction of the R-spline area group
+  std::vector<double> Rdist;          // [G] population fraction of every area's dps
+  // R(t) per area group and the external force per (strain, area) group
+  // (Model::update_R_ref / update_efoi_ref, src/model.cc:384-480): one group
+  // each unless the spec names more.
+  int nR = 1, nE = 1;
+  std::vector<int> rgrp_spline, rgrp_area;                    // [nR], [A]
+  std::vector<std::vector<double>> rgrp_dist;                 // [nR][G] dp fractions over the group's areas
+  std::vector<int> egrp_spline, egrp_cell;                    // [nE], [nstrain][A]
+  std::vector<double> egrp_agedist;                           // [nE][G]
 
   // Data::geo_normalise (src/data_matrix.cc:55-96), reference orientation ...
   std::vector<double> M_diag, M_val, M_factor;
@@ -107,15 +115,17 @@ class HostModel {
   std::vector<double> dirichlet_correct(const double* paramv) const;
   // Returns 0 or a BEEP_* error code.
   int derive(const double* paramv, Tables& out, std::string& err) const;
-  // beta[s] * areafactor[day of s][area], [S][A] (FP64), of derived tables.
+  // beta[s][group of area] * areafactor[day of s][area], [S][A] (FP64).
   std::vector<double> beta_af(const Tables& tb) const;
-  // efoi[s] * efoi_agedist[g], [S][G] (FP64): the external force per group.
-  std::vector<double> efoi_groups(const Tables& tb) const;
+  // The external force of every cell, efoi[s][its group] * the group's age
+  // weight, [S][A][G] (FP64).
+  std::vector<double> efoi_cells(const Tables& tb) const;
   double prior_logpdf(const double* paramv) const;
   bool inbounds(const double* paramv) const;
 
  private:
   std::string build_modifications(const beep_spec& s);
+  std::string build_groups(const beep_spec& s);
   std::vector<double> disc_spline(int sp, const std::vector<double>& pdir) const;
   std::vector<double> area_factor(const std::vector<double>& pdir) const;
   int strain_beta(const std::vector<double>& pd, Tables& tb, std::string& err) const;

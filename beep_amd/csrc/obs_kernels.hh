@@ -425,12 +425,13 @@ __global__ void k_accept_ptrs(int n, const double* const* __restrict__ efp, cons
 // multiplied after a change point: beta is proportional to R(t_s)).
 __global__ void k_derive_slot(float* __restrict__ tab, size_t stride, const int* __restrict__ dst,
                               const int* __restrict__ src, const float* __restrict__ factor,
-                              const int* __restrict__ from, int off_beta, int S) {
+                              const int* __restrict__ from, int off_beta, int S, int nR) {
   const int i = blockIdx.x;        // grid (n, chunks)
   const float* s_ = tab + (size_t)src[i] * stride;
   float* d_ = tab + (size_t)dst[i] * stride;
   const float f = factor[i];
-  const size_t first = (size_t)off_beta + (size_t)from[i], end = (size_t)off_beta + (size_t)S;
+  // beta[S][nR]: every group's beta from step from[i] on.
+  const size_t first = (size_t)off_beta + (size_t)from[i] * nR, end = (size_t)off_beta + (size_t)S * nR;
   for (size_t k = blockIdx.y * blockDim.x + threadIdx.x; k < stride; k += (size_t)gridDim.y * blockDim.x) {
     float v = s_[k];
     if (k >= first && k < end) v *= f;

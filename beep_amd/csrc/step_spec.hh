@@ -68,9 +68,11 @@ __device__ inline void step_spec_body(const Job* __restrict__ jobs, const float*
 #pragma unroll
     for (int gg = 0; gg < G; gg++) nmi += sN[g * G + gg] * Ib[(size_t)gg * Apad + j];
     if (nmi < 0.f) nmi = 0.f;
-    float afc = af_row(tp, tab, s)[j], agc = gptr(tp->agedist)[g];
+    float bet, ef, agc;
+    foi_groups(tp, tab, s, j, g, DPM, bet, ef, agc);
+    float afc = af_row(tp, tab, s)[j];
     mod_foi_factors(tp, s, cell, afc, agc);
-    const float foi = foi_mean(tab[tp->off_beta + s], afc, nmi, tab[tp->off_efoi + s], agc);
+    const float foi = foi_mean(bet, afc, nmi, ef, agc);
     const bool simu = MODE == MODE_MBP ? (jb.simu ? gptr(jb.simu)[(size_t)s * A + j] != 0 : false) : true;
     CellRng rng;
     rng.init(k0, k1, c0, (uint32_t)cell, jb.key);

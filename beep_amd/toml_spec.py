@@ -32,7 +32,9 @@ Supported: one or more age groups (`ages`, `age_mixing_matrix`, `age:` parameter
 `democats_dep="age"` tables) and areas (`areas` with population columns, a
 `geo_mixing_matrix` for more than one, `geo_mixing_modify`, `geo_dep` tables),
 `comps` with dist Exp / Erlang, `trans` with branching probabilities,
-`R_spline` and `efoi_spline` (one entry each), `area_covars`, `tv_covars`
+`R_spline` (one entry, or one per area by `geo_filt`) and `efoi_spline`
+(one entry, or one per area and/or strain, each with its own `age_dist`),
+`area_covars`, `tv_covars`
 and `area_tv_covars` (func log or linear), `area_effect` and `level_effect`
 (fixed or MDir(k) priors), `age_mixing_modify` (row_column) and
 `age_mixing_perturb` (a contact matrix that varies by step), `obs_spline`
@@ -661,14 +663,15 @@ def _effects(rd, doc, P, area_head, area_rows, pop):
     return area_eff, level, groups
 
 
-def _spline(rd, P, key, entry, default_name=None, params=None):
+def _spline(rd, P, key, entry, default_name=None, params=None, suffix=""):
     """One spline entry (src/inputs.cc:1363-1474): knots, parameters, multfac.
     default_name(i, bp) names knot i's parameter when 'param' is not given;
-    params, when given, are the knots' parameter indices, already added."""
+    params, when given, are the knots' parameter indices, already added;
+    suffix (' <area>', ' <strain>') follows the default name's root."""
     if default_name is None:
         _check_keys(entry, {"name", "bp", "param", "value", "prior", "factor", "factor_param", "factor_value",
-                            "factor_prior", "smooth", "smooth_type"} | ({"age_dist"} if key == "efoi_spline" else set()),
-                    key)
+                            "factor_prior", "smooth", "smooth_type"} |
+                    ({"age_dist", "geo_filt", "strain"} if key in ("efoi_spline", "R_spline") else set()), key)
     bp = _bp(rd, entry, key, extend=params is None)
     if any(b > a for a, b in zip(bp[1:], bp[:-1])):
         raise TomlError(f"in '{key}' times in 'bp' are not in the right order")
@@ -698,7 +701,7 @@ def _spline(rd, P, key, entry, default_name=None, params=None):
                 if default_name is not None:
                     nm = default_name(i, bp)
                 else:
-                    nm = key.replace("_", "-")
+                    nm = key.replace("_", "-") + suffix
                     if n > 2:
                         nm += f" t={bp[i]}"
             params.append(P.add(nm, vals[i], pris[i]))
@@ -1467,20 +1470,83 @@ def load(path, datadir=None, with_data=True, prediction=False):
     comps = _compartments(rd, doc, P, demo)
     trans, dirichlet = _transitions(rd, doc, comps, P, demo)
 
+    # The areas file (src/data.cc:288-457), read first: spline entries name
+    # areas by code.
+    if "areas" not in doc:
+        raise TomlError("'areas' must be set")
+    head, rows = rd.table(_s(doc["areas"]))
+    if _col(head, "area") is None:
+        raise TomlError("the areas file needs a column 'area'")
+    A = len(rows)
+    codes = [r[_col(head, "area")] for r in rows]
+    strain_names = cats[strain_k][1] if strain_k is not None else ["all"]
+    nst = len(strain_names) if strain_k is not None else 1
+
+    def spline_areas(e, key):
+        area = _s(e.get("geo_filt", ""))
+        if area == "":
+            return "", list(range(A))
+        if area not in codes:
+            raise TomlError(f"In '{key}' cannot find 'area={area}'")
+        return area, [codes.index(area)]
+
     # Splines by the names modifications use (Model::add_splines,
-    # src/model.cc:518-628): 'name', else R_t, efoi(t), m(t).
+    # src/model.cc:505-628): 'name', else R_t, efoi(t), m(t), followed by the
+    # area (and the strain) an entry is for.  R(t) (Model::update_R_ref,
+    # src/model.cc:384-430): one entry per area or one for every area, each
+    # area exactly once.
     splines, spline_names = [], {}
-    if len(doc.get("R_spline", [])) != 1:
-        raise TomlError("exactly one 'R_spline' entry is supported")
-    splines.append(_spline(rd, P, "R_spline", doc["R_spline"][0]))
-    spline_names.setdefault(_s(doc["R_spline"][0].get("name", "")) or "R_t", 0)
-    efoi = -1
-    if "efoi_spline" in doc:
-        if len(doc["efoi_spline"]) != 1:
-            raise TomlError("one 'efoi_spline' entry is supported")
-        splines.append(_spline(rd, P, "efoi_spline", doc["efoi_spline"][0]))
-        efoi = 1
-        spline_names.setdefault(_s(doc["efoi_spline"][0].get("name", "")) or "efoi(t)", efoi)
+    r_entries = doc.get("R_spline", [])
+    if isinstance(r_entries, dict):
+        r_entries = [r_entries]
+    if not r_entries:
+        raise TomlError("A value for 'R_spline' must be set.")
+    rgroups, rseen = [], [False] * A
+    for e in r_entries:
+        if _s(e.get("strain", "")):
+            raise TomlError("In 'R_spline' a value for 'strain' cannot be set")
+        if "age_dist" in e:
+            raise TomlError("In 'R_spline' a value for 'age_dist' cannot be set")
+        area, areas = spline_areas(e, "R_spline")
+        name = (_s(e.get("name", "")) or "R_t") + (f" {area}" if area and A > 1 else "")
+        for c in areas:
+            if rseen[c]:
+                raise TomlError(f"For '{name}' the area '{codes[c]}' already has a spline specified")
+            rseen[c] = True
+        splines.append(_spline(rd, P, "R_spline", e, suffix=f" {area}" if area else ""))
+        spline_names.setdefault(name, len(splines) - 1)
+        rgroups.append({"spline": len(splines) - 1, "areas": areas})
+    if not all(rseen):
+        raise TomlError("'R_spline' must be set for all areas and infection transitions.")
+    # The external force (Model::update_efoi_ref, src/model.cc:432-480): one
+    # entry per (strain, area), per area, per strain or for everything, each
+    # (strain, area) exactly once; its own age_dist.
+    e_entries = doc.get("efoi_spline", [])
+    if isinstance(e_entries, dict):
+        e_entries = [e_entries]
+    egroups, eseen = [], [[False] * A for _ in range(nst)]
+    for e in e_entries:
+        area, areas = spline_areas(e, "efoi_spline")
+        st = _s(e.get("strain", ""))
+        if st:
+            if strain_k is None or st not in strain_names:
+                raise TomlError(f"In 'efoi_spline' cannot find 'strain={st}'")
+            sts = [strain_names.index(st)]
+        else:
+            sts = list(range(nst))
+        name = (_s(e.get("name", "")) or "efoi(t)") + (f" {area}" if area and A > 1 else "") + \
+            (f" {st}" if st and nst > 1 else "")
+        for k in sts:
+            for c in areas:
+                if eseen[k][c]:
+                    raise TomlError(f"For '{name}' the area '{codes[c]}' already has a spline specified")
+                eseen[k][c] = True
+        splines.append(_spline(rd, P, "efoi_spline", e, suffix=(f" {area}" if area else "") + (f" {st}" if st else "")))
+        spline_names.setdefault(name, len(splines) - 1)
+        egroups.append({"spline": len(splines) - 1, "areas": areas, "strains": sts, "entry": e})
+    if e_entries and not all(all(x) for x in eseen):
+        raise TomlError("'efoi_spline' must be set for all areas and strains.")
+    efoi = egroups[0]["spline"] if egroups else -1
 
     geo = -1
     if "geo_mixing_modify" in doc:
@@ -1490,13 +1556,7 @@ def load(path, datadir=None, with_data=True, prediction=False):
         geo = len(splines) - 1
         spline_names.setdefault(_s(doc["geo_mixing_modify"][0].get("name", "")) or "m(t)", geo)
 
-    # Areas and the initial population (src/data.cc:288-457).
-    if "areas" not in doc:
-        raise TomlError("'areas' must be set")
-    head, rows = rd.table(_s(doc["areas"]))
-    if _col(head, "area") is None:
-        raise TomlError("the areas file needs a column 'area'")
-    A = len(rows)
+    # The initial population (src/data.cc:288-457).
     pop = _initial_population(rd, doc, demo, comps, trans, head, rows, strain_k)
     pop_dp = np.array(pop, dtype=np.float64).sum(axis=(0, 1))              # [G]
     pop_age = np.array([pop_dp[[dp for dp in range(G) if demo.dp_age[dp] == a]].sum()
@@ -1549,14 +1609,19 @@ def load(path, datadir=None, with_data=True, prediction=False):
     # efoi by age (src/inputs.cc:1421-1442): 'age_dist' normalised by the
     # population's age distribution, else 1 for every age group; each dp
     # takes its age group's (src/state.cc:415-427).
-    efoi_ad = [1.0] * nage
-    if efoi >= 0 and "age_dist" in doc["efoi_spline"][0]:
-        if nage <= 1:
-            raise TomlError("in 'efoi_spline' the value of 'age_dist' is not required")
-        ad = [float(x) for x in rd.split_n(_s(doc["efoi_spline"][0]["age_dist"]), nage, "efoi_spline")]
-        tot, agedist = sum(ad), pop_age / pop_age.sum()
-        efoi_ad = [float(ad[a] / (tot * agedist[a])) for a in range(nage)]
-    efoi_ad = [efoi_ad[a] for a in demo.dp_age]
+    def age_weights(e):
+        w = [1.0] * nage
+        if "age_dist" in e:
+            if nage <= 1:
+                raise TomlError("in 'efoi_spline' the value of 'age_dist' is not required")
+            ad = [float(x) for x in rd.split_n(_s(e["age_dist"]), nage, "efoi_spline")]
+            tot, agedist = sum(ad), pop_age / pop_age.sum()
+            w = [float(ad[a] / (tot * agedist[a])) for a in range(nage)]
+        return [w[a] for a in demo.dp_age]
+
+    for eg in egroups:
+        eg["agedist"] = age_weights(eg.pop("entry"))
+    efoi_ad = egroups[0]["agedist"] if egroups else [1.0] * G
 
     # Geographic mixing (Data::load_geo_mixing_matrix, src/data_matrix.cc:206-229):
     # an A x A table without a heading; the diagonal and, per row, the
@@ -1586,7 +1651,8 @@ def load(path, datadir=None, with_data=True, prediction=False):
         "trans": [{"from": t["from"], "to": t["to"], "infection": t["infection"],
                    **({"prob": t["prob"]} if t["prob"] is not None else {})} for t in trans],
         "params": P.specs,
-        "splines": splines, "R_spline": 0, "Rfactor": -1, "efoi_spline": efoi, "efoi_agedist": efoi_ad,
+        "splines": splines, "R_spline": rgroups[0]["spline"], "Rfactor": -1, "efoi_spline": efoi,
+        "efoi_agedist": efoi_ad,
         "geo_spline": geo, "N": N, "M": M, "pop": pop, "dirichlet": dirichlet + eff_groups + mod_groups,
         "sus": sus_terms[0] if sus_terms else None, "sus_extra": sus_terms[1:],
         "nage": nage, "dp_age": demo.dp_age, "strain_Rfactor": strain_rf,
@@ -1594,6 +1660,10 @@ def load(path, datadir=None, with_data=True, prediction=False):
         "age_mod": age_mod,
         "obs": _observations(rd, doc, comps, trans, with_data, head, rows, demo, obs_spl, pop),
     }
+    if len(rgroups) > 1:
+        spec["R_splines"] = rgroups
+    if len(egroups) > 1:
+        spec["efoi_splines"] = egroups
     if prediction:
         mods = doc.get("modification", [])
         if isinstance(mods, dict):

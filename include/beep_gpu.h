@@ -203,6 +203,25 @@ typedef struct {
    * prediction mode only); 0 = none. */
   int32_t nmod;
   const beep_mod* mod;
+  /* R(t) per area (R_spline entries naming an area, Model::update_R_ref,
+   * src/model.cc:384-430; beta per area, Model::calculate_beta_from_R,
+   * src/model.cc:1318-1347): area c follows spline rgroup_spline[g] of its
+   * group g = rgroup_area[c], with beta = Rfactor * R(t) / (the NGM ratio over
+   * the populations of the group's areas).  0 = R_spline for every area.  One
+   * strain only. */
+  int32_t nrgroup;
+  const int32_t* rgroup_spline; /* [nrgroup]                                        */
+  const int32_t* rgroup_area;   /* [A] group of each area                          */
+  /* The external force per strain and area (efoi_spline entries naming a
+   * strain or an area, Model::update_efoi_ref, src/model.cc:432-480, used at
+   * src/state.cc:410-424): (strain st, area c) follows group
+   * e = egroup_cell[st * A + c], efoi(t) = spline egroup_spline[e] / popsize
+   * times the age weights egroup_agedist[e][dp].  0 = efoi_spline with
+   * efoi_agedist everywhere. */
+  int32_t negroup;
+  const int32_t* egroup_spline; /* [negroup]                                        */
+  const int32_t* egroup_cell;   /* [nstrain][A]                                     */
+  const double* egroup_agedist; /* [negroup][G]                                     */
 } beep_spec;
 
 /* Host only (no device is touched): State::set_param's FP64 tables for one
@@ -264,7 +283,8 @@ int beep_set_param(beep_ctx* ctx, int n, const int32_t* slot, const double* para
 
 /* Copies the FP64 tables derived for a slot (tests): rate[T][G], sus[G], inf[C],
  * Ntime[G][G] ([S][G][G] when the spec has age matrix modifications),
- * beta_af[S][A] (beta * areafactor), efoi[S][G], geo[S]. */
+ * beta_af[S][A] (beta * areafactor), efoi[S][A][G] (the external force of each
+ * cell: efoi(t) of its strain and area times its age weight), geo[S]. */
 int beep_get_tables(beep_ctx* ctx, int slot, double* rate, double* sus, double* inf, double* Ntime,
                     double* beta_af, double* efoi, double* geo);
 

@@ -121,6 +121,10 @@ struct OrTables {
   const double* beta_mult;
   const double* efoi_mult;
   const double* transmean_mult;
+  // External-force splines per strain and area (efoi_spl_ref[st][c],
+  // src/state.cc:410-424): each cell's efoi times its age weight, [S][A][G];
+  // null = efoi[s] * agedist[dp] everywhere.
+  const double* efoi_cell;
 };
 
 struct OrState {
@@ -270,7 +274,12 @@ int or_set_transmean(const OrModel* m, const OrTables* tb, OrState* st, int s, i
       double b = be, e = et;
       if (tb->beta_mult) b = beta_sc * tb->beta_mult[((size_t)s * m->A + c) * G + dp] * af_sc;
       if (tb->efoi_mult) e = et * tb->efoi_mult[((size_t)s * m->A + c) * G + dp];
-      out = dt * popu * tb->sus[dp] * (b * NMI[dp] + e * tb->agedist[dp]);
+      double eterm = e * tb->agedist[dp];
+      if (tb->efoi_cell) {
+        eterm = tb->efoi_cell[((size_t)s * m->A + c) * G + dp];
+        if (tb->efoi_mult) eterm *= tb->efoi_mult[((size_t)s * m->A + c) * G + dp];
+      }
+      out = dt * popu * tb->sus[dp] * (b * NMI[dp] + eterm);
       if (std::isnan(out)) return -4;
     }
   }

@@ -228,10 +228,12 @@ def largest_eigenvalue(M, rng=None):
     raise RuntimeError("eigen-vector convergence problem")
 
 
-def ngm(spec, nz, tr_rate, sus, inf, N=None):
+def ngm(spec, nz, tr_rate, sus, inf, N=None, dist=None):
     """Next-generation matrix of calculate_R_beta_ratio_using_NGM for the
-    age contact matrix N (default the normalised one)."""
+    age contact matrix N (default the normalised one) and the dp weights dist
+    of the R spline's areas (default every area's, nz["Rdist"])."""
     N = nz["N"] if N is None else N
+    dist = nz["Rdist"] if dist is None else dist
     G = spec["G"]
     trans = spec["trans"]
     ist = nz["inf_state"]
@@ -254,7 +256,7 @@ def ngm(spec, nz, tr_rate, sus, inf, N=None):
             continue
         for dp in range(G):
             for q in range(G):
-                F[q, si * G + dp] += nz["Rdist"][q] * sus[q] * inf[co] * N[q, dp]
+                F[q, si * G + dp] += dist[q] * sus[q] * inf[co] * N[q, dp]
     return (F @ Vinv)[:, :G]
 
 
@@ -303,27 +305,81 @@ def derive(spec, nz, pv):
     Ntime = Ntime * (nz["strain"][:, None] == nz["strain"][None, :])[None, :, :]
     if nz["nstrain"] > 1:
         return _derive_strains(spec, nz, pd, rate, sus, inf, af, Ntime, S, A, G)
-    # The ratio is recomputed where the matrix changes by more than TINY
-    # (src/model.cc:1335-1340, Model::equal 1303-1314).
-    ratio = np.zeros(S)
-    for s in range(S):
-        if s == 0 or np.any(np.abs(Ntime[s] - Ntime[s - 1]) > 1e-8):
-            ratio[s] = largest_eigenvalue(ngm(spec, nz, rate, sus, inf, Ntime[s]))
-        else:
-            ratio[s] = ratio[s - 1]
+    # Each R spline's ratio, over the dp distribution of its own areas
+    # (Model::calculate_beta_from_R, src/model.cc:1318-1347, with
+    # SplineInfo::democatpos_dist, src/model.cc:409-424), recomputed where the
+    # matrix changes by more than TINY (src/model.cc:1335-1340, Model::equal
+    # 1303-1314); beta of an area is its spline's.
     Rfac = pd[spec["Rfactor"]] if spec.get("Rfactor", -1) >= 0 else 1.0
-    beta = Rfac * disc_spline(spec, spec["R_spline"], pd) / ratio          # [S]
-    beta_SA = np.repeat(beta[:, None], A, axis=1)
-    if spec.get("efoi_spline", -1) >= 0:
-        efoi = disc_spline(spec, spec["efoi_spline"], pd) / nz["popsize"]
-    else:
-        efoi = np.zeros(S)
-    agedist = np.asarray(spec.get("efoi_agedist", [1.0 / G] * G), dtype=np.float64)
+    beta_SA = np.zeros((S, A))
+    ratios = []
+    for spl, areas, dist in r_groups(spec, nz):
+        ratio = np.zeros(S)
+        for s in range(S):
+            if s == 0 or np.any(np.abs(Ntime[s] - Ntime[s - 1]) > 1e-8):
+                ratio[s] = largest_eigenvalue(ngm(spec, nz, rate, sus, inf, Ntime[s], dist))
+            else:
+                ratio[s] = ratio[s - 1]
+        beta_SA[:, areas] = (Rfac * disc_spline(spec, spl, pd) / ratio)[:, None]
+        ratios.append(ratio)
+    ratio = ratios[0]
+    efoi, agedist, efoi_cell = efoi_tables(spec, nz, pd)
     geo = disc_spline(spec, spec["geo_spline"], pd) if spec.get("geo_spline", -1) >= 0 else np.ones(S)
     bm, em, tm = modification_tables(spec)
-    return dict(rate=rate, sus=sus, inf=inf, af=af, beta=beta_SA, efoi=efoi, agedist=agedist, geo=geo,
-                Ntime=Ntime, ratio=ratio[0] if not spec.get("age_mod") else ratio, beta_mult=bm, efoi_mult=em,
-                transmean_mult=tm)
+    return dict(rate=rate, sus=sus, inf=inf, af=af, beta=beta_SA, efoi=efoi, agedist=agedist, efoi_cell=efoi_cell,
+                geo=geo, Ntime=Ntime, ratio=ratio[0] if not spec.get("age_mod") else ratio, beta_mult=bm,
+                efoi_mult=em, transmean_mult=tm)
+
+
+def r_groups(spec, nz):
+    """The R splines as (spline, areas, dp distribution of their areas'
+    initial populations): spec["R_splines"], else R_spline for every area."""
+    A, G = spec["A"], spec["G"]
+    rs = spec.get("R_splines")
+    if not rs:
+        return [(spec["R_spline"], list(range(A)), nz["Rdist"])]
+    if nz["nstrain"] > 1 and len(rs) > 1:
+        raise ValueError("R splines per area are not supported with strains")
+    pop = np.asarray(spec["pop"], dtype=np.float64)                      # [A][C][G]
+    out = []
+    for e in rs:
+        areas = list(e["areas"])
+        d = pop[areas].sum(axis=(0, 1))
+        out.append((e["spline"], areas, d / d.sum()))
+    return out
+
+
+def efoi_tables(spec, nz, pd):
+    """The external force (State::set_transmean, src/state.cc:410-424): efoi
+    [S] and agedist [G] of the single efoi_spline, and the force of every
+    cell, efoi_cell [S][A][G] = efoi of its (strain, area)'s spline / popsize
+    times that spline's age weight, from spec["efoi_splines"] when given
+    (Model::update_efoi_ref, src/model.cc:432-480)."""
+    A, G = spec["A"], spec["G"]
+    S = spec["period"] * spec["spd"]
+    es = spec.get("efoi_splines")
+    if not es:
+        if spec.get("efoi_spline", -1) >= 0:
+            efoi = disc_spline(spec, spec["efoi_spline"], pd) / nz["popsize"]
+        else:
+            efoi = np.zeros(S)
+        agedist = np.asarray(spec.get("efoi_agedist", [1.0 / G] * G), dtype=np.float64)
+        return efoi, agedist, np.repeat((efoi[:, None] * agedist[None, :])[:, None, :], A, axis=1)
+    ns = nz["nstrain"]
+    strain = np.asarray(nz["strain"])
+    cell = np.zeros((S, A, G))
+    seen = np.zeros((ns, A), dtype=int)
+    for e in es:
+        v = disc_spline(spec, e["spline"], pd) / nz["popsize"]
+        ad = np.asarray(e["agedist"], dtype=np.float64)
+        for st in e.get("strains", range(ns)):
+            for c in e["areas"]:
+                seen[st, c] += 1
+                g = strain == st
+                cell[:, c, g] = v[:, None] * ad[None, g]
+    if (seen != 1).any():
+        raise ValueError("every (strain, area) needs exactly one external-force spline")
+    return np.zeros(S), np.zeros(G), cell
 
 
 def _derive_strains(spec, nz, pd, rate, sus, inf, af, Ntime, S, A, G):
@@ -368,17 +424,14 @@ def _derive_strains(spec, nz, pd, rate, sus, inf, af, Ntime, S, A, G):
     Rfac = np.array([pd[p] if p >= 0 else 1.0 for p in spec["strain_Rfactor"]])
     fold = (Rfac[None, :] / ratio)[:, nz["strain"]]                    # [S][G]
     Ntime = Ntime * fold[:, :, None]
-    beta = disc_spline(spec, spec["R_spline"], pd)
+    ((spl, _, _),) = r_groups(spec, nz)
+    beta = disc_spline(spec, spl, pd)
     beta_SA = np.repeat(beta[:, None], A, axis=1)
-    if spec.get("efoi_spline", -1) >= 0:
-        efoi = disc_spline(spec, spec["efoi_spline"], pd) / nz["popsize"]
-    else:
-        efoi = np.zeros(S)
-    agedist = np.asarray(spec.get("efoi_agedist", [1.0 / G] * G), dtype=np.float64)
+    efoi, agedist, efoi_cell = efoi_tables(spec, nz, pd)
     geo = disc_spline(spec, spec["geo_spline"], pd) if spec.get("geo_spline", -1) >= 0 else np.ones(S)
     bm, em, tm = modification_tables(spec)
-    return dict(rate=rate, sus=sus, inf=inf, af=af, beta=beta_SA, efoi=efoi, agedist=agedist, geo=geo,
-                Ntime=Ntime, ratio=ratio, beta_mult=bm, efoi_mult=em, transmean_mult=tm)
+    return dict(rate=rate, sus=sus, inf=inf, af=af, beta=beta_SA, efoi=efoi, agedist=agedist, efoi_cell=efoi_cell,
+                geo=geo, Ntime=Ntime, ratio=ratio, beta_mult=bm, efoi_mult=em, transmean_mult=tm)
 
 
 def prior(spec, pv):

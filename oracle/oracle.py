@@ -71,7 +71,8 @@ class _OrModel(C.Structure):
 
 class _OrTables(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in ("inf", "rate", "sus", "beta", "af", "efoi", "agedist", "geo",
-                                                     "Ntime", "beta_mult", "efoi_mult", "transmean_mult")]
+                                                     "Ntime", "beta_mult", "efoi_mult", "transmean_mult",
+                                                     "efoi_cell")]
 
 
 class _OrState(C.Structure):
@@ -171,6 +172,9 @@ class Oracle:
         # Prediction-mode modifications: null pointers when the spec has none.
         mods = [None if d.get(k) is None else np.ascontiguousarray(d[k], dtype=np.float64)
                 for k in ("beta_mult", "efoi_mult", "transmean_mult")]
+        # External-force splines per strain or area: each cell's force (null
+        # without them, where efoi * agedist is used).
+        mods.append(np.ascontiguousarray(d["efoi_cell"]) if self.spec.get("efoi_splines") else None)
         t = _OrTables(*[_d(a) for a in arrs], *[None if a is None else _d(a) for a in mods])
         t._keep = arrs + [a for a in mods if a is not None]
         t.derived = d

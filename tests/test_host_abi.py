@@ -38,7 +38,8 @@ def test_host_tables_match_oracle(gpu_lib, mk):
         assert np.allclose(t["rate"], d["rate"], rtol=1e-14)
         assert np.allclose(t["sus"], d["sus"]) and np.allclose(t["inf"], d["inf"])
         assert np.allclose(t["N"], d["Ntime"] if spec.get("age_mod") else d["Ntime"][0], rtol=1e-12)
-        assert np.allclose(t["efoi"], d["efoi"][:, None] * d["agedist"][None, :])
+        assert np.allclose(t["efoi"], d["efoi_cell"], rtol=1e-14, atol=0)
+        assert np.allclose(t["efoi"], (d["efoi"][:, None] * d["agedist"][None, :])[:, None, :], rtol=1e-14, atol=0)
         assert np.allclose(t["geo"], d["geo"])
         assert t["Pr"] == pytest.approx(derive.prior(spec, pv), rel=1e-14)
         assert t["inbounds"] == derive.inbounds(spec, pv)

@@ -749,3 +749,66 @@ def test_data_table_filters(tmp_path):
     p.write_text(bad)
     with pytest.raises(T.TomlError, match="does not generate any data columns"):
         T.load(str(p), datadir=str(data))
+
+
+def test_R_and_efoi_splines_per_area_and_strain(tmp_path):
+    """R_spline and efoi_spline entries naming an area ('geo_filt', an area
+    code) or a strain (Model::update_R_ref / update_efoi_ref,
+    src/model.cc:384-480, find_spline's parameter names src/inputs.cc:
+    1392-1403): one spline per area, parameters named '<root> <area>', the
+    groups in spec["R_splines"] / spec["efoi_splines"], spline names with the
+    area for modifications; every area (and strain) covered exactly once."""
+    text = open(os.path.join(FIX, "seir_sex_ages.toml")).read()
+    text = text.replace('R_spline = [{ value="2.1", prior="Uniform(0.5,4)"}]',
+                        'R_spline = [{ geo_filt="North", value="2.1", prior="Uniform(0.5,4)"},\n'
+                        '            { geo_filt="South", value="1.6", prior="Uniform(0.5,4)"}]')
+    text = text.replace('efoi_spline = [{ value="3", age_dist="1|3"}]',
+                        'efoi_spline = [{ geo_filt="North", value="3", age_dist="1|3"}, { geo_filt="South", value="5"}]')
+    p = tmp_path / "g.toml"
+    p.write_text(text + '\nmodification = [{type="spline_fac", name="R_t South", start="10", factor="0.5"}]\n')
+    spec, values, _ = T.load(str(p), datadir=os.path.join(FIX, "data_seir_sex_ages"), prediction=True)
+    names = [q["name"] for q in spec["params"]]
+    rs, es = spec["R_splines"], spec["efoi_splines"]
+    assert [g["areas"] for g in rs] == [[0], [1]] and [g["areas"] for g in es] == [[0], [1]]
+    assert [names[spec["splines"][g["spline"]]["param"][0]] for g in rs] == ["R-spline North", "R-spline South"]
+    assert [values[spec["splines"][g["spline"]]["param"][0]] for g in rs] == [2.1, 1.6]
+    assert [names[spec["splines"][g["spline"]]["param"][0]] for g in es] == ["efoi-spline North", "efoi-spline South"]
+    assert es[1]["agedist"] == [1.0] * 4 and es[0]["agedist"] != [1.0] * 4
+    assert spec["modifications"][0]["spline"] == rs[1]["spline"]
+    # The host model takes the groups: South's beta over North's is 1.6 / 2.1
+    # times the ratio of the two groups' NGM ratios.
+    from beep_amd import _lib
+    t = _lib.spec_tables(spec, values)
+    assert t["beta_af"].shape == (56, 2) and not np.allclose(t["beta_af"][:, 0], t["beta_af"][:, 1])
+    # Every area exactly once.
+    bad = text.replace('{ geo_filt="South", value="1.6", prior="Uniform(0.5,4)"}', '{ value="1.6"}')
+    p.write_text(bad)
+    with pytest.raises(T.TomlError, match="already has a spline"):
+        T.load(str(p), datadir=os.path.join(FIX, "data_seir_sex_ages"))
+    bad = text.replace(',\n            { geo_filt="South", value="1.6", prior="Uniform(0.5,4)"}', '')
+    p.write_text(bad)
+    with pytest.raises(T.TomlError, match="must be set for all areas"):
+        T.load(str(p), datadir=os.path.join(FIX, "data_seir_sex_ages"))
+    bad = text.replace('geo_filt="North", value="2.1"', 'geo_filt="East", value="2.1"')
+    p.write_text(bad)
+    with pytest.raises(T.TomlError, match="cannot find 'area=East'"):
+        T.load(str(p), datadir=os.path.join(FIX, "data_seir_sex_ages"))
+
+
+def test_efoi_spline_per_strain(tmp_path):
+    text = open(os.path.join(FIX, "sird_strains.toml")).read()
+    text = text.replace('efoi_spline = [{ value="2"}]', 'efoi_spline = [{ strain="s1", value="2"}, { strain="s2", value="0.5"}]')
+    p = tmp_path / "s.toml"
+    p.write_text(text)
+    spec, values, _ = T.load(str(p), datadir=os.path.join(FIX, "data_sird_strains"))
+    es = spec["efoi_splines"]
+    names = [q["name"] for q in spec["params"]]
+    assert [g["strains"] for g in es] == [[0], [1]]
+    assert [names[spec["splines"][g["spline"]]["param"][0]] for g in es] == ["efoi-spline s1", "efoi-spline s2"]
+    from beep_amd import _lib
+    t = _lib.spec_tables(spec, values)
+    dpm = spec["G"] // 2
+    assert np.allclose(t["efoi"][:, :, dpm:] / t["efoi"][:, :, :dpm], 0.25, rtol=1e-14)
+    p.write_text(text.replace('strain="s2"', 'strain="s3"'))
+    with pytest.raises(T.TomlError, match="cannot find 'strain=s3'"):
+        T.load(str(p), datadir=os.path.join(FIX, "data_sird_strains"))

@@ -384,8 +384,8 @@ extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_
     beep_fused_spec_sim(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
   beepdev::fused_spec_body<beepdev::MODE_SIM>(jobs, seed, stream);
 }
-// The coupled proposal needs more registers than a simulation: at most six
-// resident wavefronts per SIMD (80 registers) keep it from spilling.
+// The coupled proposal needs more registers than a simulation: at most five
+// resident wavefronts per SIMD (96 registers) keep its spills small.
 extern "C" __global__ void __launch_bounds__(BEEP_SPEC_MAX_BLOCK, beepspec::MIN_WAVES < 5 ? beepspec::MIN_WAVES : 5)
     beep_fused_spec_mbp(const beepdev::Job* jobs, uint64_t seed, uint32_t stream) {
   beepdev::fused_spec_body<beepdev::MODE_MBP>(jobs, seed, stream);

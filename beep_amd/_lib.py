@@ -451,6 +451,37 @@ WORK_SIMULATE, WORK_MBP = 0, 1
 BATCH_AUTOTUNE = 1
 
 
+# beep_work records as a numpy structured type (same offsets and size), for
+# callers that keep their work lists as column arrays.
+WORK_DTYPE = np.dtype({"names": [f[0] for f in Work._fields_],
+                       "formats": [np.uint64, np.int32, np.int32, np.int32, np.int32, np.uint32, np.int32, np.int32],
+                       "offsets": [getattr(Work, f[0]).offset for f in Work._fields_],
+                       "itemsize": C.sizeof(Work)})
+
+
+def work_records(ctx, kind, inst, init, slot, key, t0, t1):
+    """A beep_work array from columns (arrays or scalars, broadcast to one
+    length); ctx holds context handles (handles()).  Slices keep the record
+    layout; np.concatenate does not (it packs the records to 36 bytes), so
+    a batch made of parts is assembled by assigning into one array."""
+    cols = np.broadcast_arrays(*(np.asarray(v) for v in (ctx, kind, inst, init, slot, key, t0, t1)))
+    rec = np.zeros(cols[0].shape[0] if cols[0].ndim else 1, dtype=WORK_DTYPE)
+    for name, v in zip(WORK_DTYPE.names, cols):
+        rec[name] = v
+    return rec
+
+
+def _work_ptr(items):
+    """(n, pointer to beep_work records, first context handle, the object
+    holding the memory) for a list of items or a work_records array."""
+    if isinstance(items, np.ndarray):
+        assert items.dtype == WORK_DTYPE and items.flags.c_contiguous
+        n = len(items)
+        return n, items.ctypes.data, int(items["ctx"][0]) if n else 0, items
+    arr = _work_array(items)
+    return len(items), arr, (items[0][0]._h.value if items else 0), arr
+
+
 def _work_array(items):
     """The beep_work array of (ctx, kind, inst, init, slot, key, t0, t1)
     items, filled column by column."""
@@ -477,16 +508,15 @@ def run_batch(items, seed=1, stream=0, autotune=False, loglik=False):
     """Runs (ctx, kind, inst, init, slot, key, t0, t1) work items across
     contexts in one batch; returns (wall ms, fused launches, step-wise
     contexts, cell-steps), and with loglik=True also each item's observation
-    log-likelihood."""
-    n = len(items)
-    arr = _work_array(items)
+    log-likelihood.  items may also be a work_records array."""
+    n, arr, h0, _keep = _work_ptr(items)
     stats = np.zeros(4)
     L = np.zeros(max(1, n))
     flags = (BATCH_AUTOTUNE if autotune else 0) | (BATCH_LOGLIK if loglik else 0)
     rc = lib().beep_run_batch(n, arr, seed, stream, flags, _p(L, C.c_double) if loglik else None,
                               _p(stats, C.c_double))
     if rc != BEEP_OK:
-        raise BeepError(rc, lib().beep_last_error(C.c_void_p(items[0][0]._h.value)).decode())
+        raise BeepError(rc, lib().beep_last_error(C.c_void_p(h0)).decode())
     return (stats, L[:n]) if loglik else stats
 
 
@@ -517,16 +547,15 @@ def batch_set_param(ctxs, slots, paramvals):
 
 def batch_accept(items, invT, dlogq, mode=1, EFcut=0.0, seed=1, stream=0):
     """Metropolis-Hastings acceptance of MBP items across contexts on the
-    device: (al, accepted)."""
-    n = len(items)
-    arr = _work_array(items)
+    device: (al, accepted).  items may also be a work_records array."""
+    n, arr, h0, _keep = _work_ptr(items)
     invT, dlogq = _f64(np.broadcast_to(invT, (n,))), _f64(np.broadcast_to(dlogq, (n,)))
     al = np.zeros(n)
     acc = np.zeros(n, dtype=np.uint8)
     rc = lib().beep_batch_accept(n, arr, mode, _p(invT, C.c_double), float(EFcut), _p(dlogq, C.c_double), seed, stream,
                                  _p(al, C.c_double), _p(acc, C.c_uint8))
     if rc != BEEP_OK:
-        raise BeepError(rc, lib().beep_last_error(C.c_void_p(items[0][0]._h.value)).decode())
+        raise BeepError(rc, lib().beep_last_error(C.c_void_p(h0)).decode())
     return al, acc.astype(bool)
 
 
@@ -536,18 +565,26 @@ def batch_mc3_swap(ladders, seed=1, stream=0):
     (order, ntr, nac)."""
     ptr = np.cumsum([0] + [len(l) for l in ladders]).astype(np.int32)
     chains = [ch for l in ladders for ch in l]
-    n = len(chains)
-    ctxs = (C.c_void_p * n)(*[ch[0]._h.value for ch in chains])
-    inst = _i32([ch[1] for ch in chains])
-    run = _i32([ch[2] for ch in chains])
-    invT = _f64([ch[3] for ch in chains])
-    order, ntr, nac = (np.zeros(n, dtype=np.int32) for _ in range(3))
-    rc = lib().beep_batch_mc3_swap(len(ladders), _p(ptr, C.c_int32), ctxs, _p(inst, C.c_int32), _p(run, C.c_int32),
-                                   _p(invT, C.c_double), seed, stream, _p(order, C.c_int32), _p(ntr, C.c_int32),
-                                   _p(nac, C.c_int32))
-    if rc != BEEP_OK:
-        raise BeepError(rc, lib().beep_last_error(C.c_void_p(chains[0][0]._h.value)).decode())
+    order, ntr, nac = batch_mc3_swap_arrays(ptr, handles([ch[0] for ch in chains]), [ch[1] for ch in chains],
+                                            [ch[2] for ch in chains], [ch[3] for ch in chains], seed, stream)
     return [(order[a:b], ntr[a:b], nac[a:b]) for a, b in zip(ptr[:-1], ptr[1:])]
+
+
+def batch_mc3_swap_arrays(ladder_ptr, h, inst, run, invT, seed=1, stream=0):
+    """batch_mc3_swap on column arrays: ladder l holds chains ladder_ptr[l] ..
+    ladder_ptr[l+1]-1; chain i is instance inst[i] of the context with handle
+    h[i] (handles()).  Returns (order, ntr, nac) over all chains."""
+    ptr = _i32(ladder_ptr)
+    h = np.ascontiguousarray(h, dtype=np.uint64)
+    n = len(h)
+    inst, run, invT = _i32(inst), _i32(np.broadcast_to(run, (n,))), _f64(np.broadcast_to(invT, (n,)))
+    order, ntr, nac = (np.zeros(n, dtype=np.int32) for _ in range(3))
+    rc = lib().beep_batch_mc3_swap(len(ptr) - 1, _p(ptr, C.c_int32), h.ctypes.data,
+                                   _p(inst, C.c_int32), _p(run, C.c_int32), _p(invT, C.c_double), seed, stream,
+                                   _p(order, C.c_int32), _p(ntr, C.c_int32), _p(nac, C.c_int32))
+    if rc != BEEP_OK:
+        raise BeepError(rc, lib().beep_last_error(C.c_void_p(int(h[0]) if n else None)).decode())
+    return order, ntr, nac
 
 
 class Context:

@@ -109,6 +109,20 @@ class UnevenBatch:
             np.zeros(0, np.int32)
         self.h_mc3 = np.repeat(_lib.handles(self.mc3_ctx), L) if nl else np.zeros(0, np.uint64)
         self.mc3_S = np.array([c.S for c in self.mc3_ctx], dtype=np.int64)
+        # The sweep's work as one beep_work record array: the particles'
+        # records are fixed; the chains' take their instances from cur / alt
+        # at each sweep (sweep_records).
+        na = len(self.abc_items)
+        self.rec = np.zeros(na + nl * L, dtype=_lib.WORK_DTYPE)
+        if na:
+            self.rec[:na] = _lib.work_records(_lib.handles([it[0] for it in self.abc_items]), _lib.WORK_SIMULATE,
+                                              [it[2] for it in self.abc_items], 0, [it[4] for it in self.abc_items],
+                                              [it[5] for it in self.abc_items], 0, [it[7] for it in self.abc_items])
+        if nl:
+            self.rec[na:] = _lib.work_records(self.h_mc3, _lib.WORK_MBP, 0, 0, 0, self.mc3_keys.reshape(-1), 0,
+                                              np.repeat(self.mc3_S, L))
+        self.ladder_ptr = np.arange(0, nl * L + 1, L, dtype=np.int32)
+        self.invT_chains = np.tile(self.invT, nl)
         # Chains' current parameters and prior densities, and their first states.
         self.pv = specmod.sample_prior(self.spec0, rng, nl * L).reshape(nl, L, self.nparam) if nl else \
             np.zeros((0, L, self.nparam))
@@ -132,6 +146,16 @@ class UnevenBatch:
                 int(self.mc3_keys[l, k]), 0, c.S) for l, c in enumerate(self.mc3_ctx) for k in range(self.L)]
         return self.abc_items, mbp
 
+    def sweep_records(self):
+        """sweep_items as one beep_work record array (the particles, then the
+        chains) and the number of particle records."""
+        na = len(self.abc_items)
+        mbp = self.rec[na:]
+        mbp["inst"] = self.alt.reshape(-1)
+        mbp["init"] = self.cur.reshape(-1)
+        mbp["slot"] = self.alt.reshape(-1)
+        return self.rec, na
+
     def particle_steps(self):
         return sum(it[7] - it[6] for it in self.abc_items) + int(self.mc3_S.sum()) * self.L
 
@@ -151,14 +175,15 @@ class UnevenBatch:
         Pr_prop = Pr[len(self.abc_slots):].reshape(nl, L)
         t1 = time.perf_counter()
         # 2. simulations, proposals, log-likelihoods
-        abc, mbp = self.sweep_items()
-        stats, Lik = _lib.run_batch(abc + mbp, seed=seed, stream=stream, autotune=autotune, loglik=True)
+        rec, na = self.sweep_records()
+        stats, Lik = _lib.run_batch(rec, seed=seed, stream=stream, autotune=autotune, loglik=True)
+        mbp = rec[na:]
         t2 = time.perf_counter()
         # 3. acceptance at each chain's temperature (symmetric proposal: the
         # log ratio is the prior's)
         nacc = 0
         if nl:
-            al, acc = _lib.batch_accept(mbp, np.tile(self.invT, nl), (Pr_prop - self.Pr).reshape(-1), mode=1,
+            al, acc = _lib.batch_accept(mbp, self.invT_chains, (Pr_prop - self.Pr).reshape(-1), mode=1,
                                         seed=seed, stream=stream)
             acc = acc.reshape(nl, L)
             nacc = int(acc.sum())
@@ -171,11 +196,10 @@ class UnevenBatch:
         # 4. swaps along the ladders
         nswap = 0
         if nl:
-            ladders = [[(c, int(self.cur[l, k]), 0, float(self.invT[k])) for k in range(L)]
-                       for l, c in enumerate(self.mc3_ctx)]
-            res = _lib.batch_mc3_swap(ladders, seed=seed, stream=stream)
-            order = np.stack([r[0] for r in res])
-            nswap = int(sum(r[2].sum() for r in res))
+            order, _ntr, nac = _lib.batch_mc3_swap_arrays(self.ladder_ptr, self.h_mc3, self.cur.reshape(-1), 0,
+                                                          self.invT_chains, seed=seed, stream=stream)
+            order = order.reshape(nl, L)
+            nswap = int(nac.sum())
             rows = np.arange(nl)[:, None]
             self.cur, self.alt = self.cur[rows, order], self.alt[rows, order]
             self.pv, self.Pr = self.pv[rows, order], self.Pr[rows, order]

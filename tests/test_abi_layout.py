@@ -64,3 +64,29 @@ def test_struct_layout_matches_header(c_layout, cls):
     last = cls._fields_[-1][0]
     assert getattr(cls, last).offset + getattr(cls, last).size <= lay["size"] < \
         getattr(cls, last).offset + getattr(cls, last).size + 8
+
+
+def test_work_records_match_work_array():
+    """The numpy structured form of beep_work (work_records, for work kept as
+    column arrays) has the ctypes struct's size and offsets, and gives the
+    same bytes as the item-list form."""
+    import numpy as np
+    assert _lib.WORK_DTYPE.itemsize == C.sizeof(_lib.Work)
+    for name, _t in _lib.Work._fields_:
+        assert _lib.WORK_DTYPE.fields[name][1] == getattr(_lib.Work, name).offset
+
+    class _Ctx:
+        def __init__(self, h):
+            self._h = C.c_void_p(h)
+
+    rng = np.random.default_rng(1)
+    n = 37
+    hs = rng.integers(1, 2 ** 47, n)
+    cols = [rng.integers(0, 2, n), rng.integers(0, 1000, n), rng.integers(0, 1000, n), rng.integers(0, 1000, n),
+            rng.integers(0, 2 ** 32, n), rng.integers(0, 50, n), rng.integers(50, 400, n)]
+    items = [(_Ctx(int(hs[i])),) + tuple(int(c[i]) for c in cols) for i in range(n)]
+    rec = _lib.work_records(hs.astype(np.uint64), *cols)
+    assert bytes(_lib._work_array(items)) == rec.tobytes()
+    # scalars broadcast
+    rec1 = _lib.work_records(hs.astype(np.uint64), 1, cols[1], 0, cols[3], cols[4], 0, cols[6])
+    assert (rec1["kind"] == 1).all() and (rec1["init"] == 0).all() and (rec1["t0"] == 0).all()

@@ -104,6 +104,12 @@ def test_batch_accept_and_swaps_match_oracle(gpu_lib):
         dlogq += list(rng.normal(0, 1, L))
     _stats, Lik = _lib.run_batch(mbp, seed=2, stream=3, loglik=True)
     al, acc = _lib.batch_accept(mbp, np.array(invT), np.array(dlogq), seed=8, stream=5)
+    # The same work as column records: the same acceptance.
+    rec = _lib.work_records(_lib.handles([it[0] for it in mbp]), _lib.WORK_MBP, [it[2] for it in mbp],
+                            [it[3] for it in mbp], [it[4] for it in mbp], [it[5] for it in mbp], 0,
+                            [it[7] for it in mbp])
+    al2, acc2 = _lib.batch_accept(rec, np.array(invT), np.array(dlogq), seed=8, stream=5)
+    assert np.array_equal(al, al2) and np.array_equal(acc, acc2)
     for i, (ctx, _k, prop, init, _s, key, _a, _b) in enumerate(mbp):
         EFp = -2 * ctx.obs_loglik([prop])[0]
         EFi = -2 * ctx.obs_loglik([init])[0]
@@ -126,6 +132,12 @@ def test_batch_accept_and_swaps_match_oracle(gpu_lib):
         u = (_lib.debug_philox(ctr, seed, 0)[:, 0].astype(np.float64) + 0.5) * 2.0 ** -32
         o_order, o_ntr, o_nac, _ = O.mc3_swap([0] * L, list(temps), EF, list(u))
         assert list(order) == o_order and list(ntr) == o_ntr and list(nac) == o_nac
+    # The column-array form over the same ladders.
+    order2, _ntr2, nac2 = _lib.batch_mc3_swap_arrays(np.arange(0, 3 * L + 1, L), np.repeat(_lib.handles(ctxs), L),
+                                                     np.tile(np.arange(L), 3), 0, np.tile(temps, 3), seed=seed,
+                                                     stream=stream)
+    assert np.array_equal(order2, np.concatenate([r[0] for r in res]))
+    assert np.array_equal(nac2, np.concatenate([r[2] for r in res]))
     # Ladder 0 draws what the single-context swap draws.
     o1, n1, a1 = ctxs[0].mc3_swap(np.arange(L), np.zeros(L), temps, seed=seed, stream=stream)
     assert list(o1) == list(res[0][0]) and list(a1) == list(res[0][2])
@@ -140,6 +152,14 @@ def test_uneven_sweep_small(gpu_lib):
     w = configs.uneven_batch(96, seed=9)
     ub = UnevenBatch(w, seed=9)
     for k in range(3):
+        # The sweep's column records are its work items (field by field: the
+        # records' padding is not initialised).
+        abc, mbp = ub.sweep_items()
+        want = np.frombuffer(bytes(_lib._work_array(abc + mbp)), dtype=_lib.WORK_DTYPE)
+        got = ub.sweep_records()[0]
+        assert got.dtype == _lib.WORK_DTYPE and got.dtype.itemsize == 40
+        for name in _lib.WORK_DTYPE.names:
+            assert np.array_equal(want[name], got[name]), name
         r = ub.sweep(k)
         assert r["loglik_finite"] and r["cell_steps"] > 0
     L = ub.L

@@ -2203,8 +2203,8 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
     for (auto& it : fused)
       groups[{{pow2_ceil(it.AG), it.ctx->m.C, it.ctx->m.T, it.spec ? 1 : 0, it.job.mode}, it.spec}].push_back(it);
     std::vector<Job> all;
-    // key, offset, count, lds, maxAG, kernel, every job starts from the initial state
-    std::vector<std::tuple<TuneKey, size_t, size_t, size_t, int, SpecKernels*, bool>> launches;
+    // key, offset, count, lds, maxAG, kernel, every job starts from the initial state, the longest job's cost
+    std::vector<std::tuple<TuneKey, size_t, size_t, size_t, int, SpecKernels*, bool, double>> launches;
     for (auto& kv : groups) {
       auto& v = kv.second;
       std::stable_sort(v.begin(), v.end(), [](const Item& a, const Item& b) { return a.cost > b.cost; });
@@ -2216,9 +2216,16 @@ int beep_run_batch(int n, const beep_work* w, uint64_t seed, uint32_t stream, in
         maxAG = std::max(maxAG, it.AG);
         rerunnable = rerunnable && it.job.from_init;
       }
-      launches.emplace_back(kv.first.first, all.size(), v.size(), lds, maxAG, kv.first.second, rerunnable);
+      // A coupled proposal costs about 1.7 times a simulation per cell-step.
+      const double lead = v.front().cost * (kv.first.first.mode == beepdev::MODE_MBP ? 1.7 : 1.0);
+      launches.emplace_back(kv.first.first, all.size(), v.size(), lds, maxAG, kv.first.second, rerunnable, lead);
       for (auto& it : v) all.push_back(it.job);
     }
+    // Launches in order of their longest job, longest first, dealt round-robin
+    // over the batch streams: the classes whose workgroups run longest start
+    // first, and the short ones fill the device as those drain.
+    std::stable_sort(launches.begin(), launches.end(),
+                     [](const auto& a, const auto& b) { return std::get<7>(a) > std::get<7>(b); });
     if (all.size() > c0->bjobs_cap) {
       if (c0->d_bjobs) {
         HIPCHK(c0, hipStreamSynchronize(c0->stream));
